@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Throughput of one invert (ms2dirty) + one predict (dirty2ms) at the production shape.
+
+Workload (default "c1"): 512 stations (70 % Gaussian core sigma 500 m, 30 % log-uniform
+1-40 km), 128 hour angles over 4 h at dec -30 deg, 64 channels 100-164 MHz, 4
+polarisations -> 16,744,448 rows x 64 x 4 = 4.29e9 complex64 visibilities, image
+8192^2 at cell 1.1e-5 rad, epsilon 1e-5, w-stacking. Synthetic data (seeded random
+visibilities, unit weights, random model image) generated directly in HBM; inputs are
+resident before the timed region. One step = hipng ms2dirty of all 4 polarisations +
+hipng dirty2ms of a 4-polarisation model image.
+
+Prints one JSON line with value = Mvis/s (nvis / seconds per step), the roofline of the
+dominant kernel (algorithmic bytes / average launch time from HIP events recorded by the
+library on its launch stream during the timed steps) and the CPU baseline (the oracle's
+CPU port of the same algorithm on a bounded sample of the same workload).
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Mvis/s invert+predict, 8192² w-stacked, ε≤1e-5 vs FP64 DFT; FP32 vs FP64 mode"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c1", choices=["c1", "c0"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32", "fp64"])
+    ap.add_argument("--nchan", type=int, default=None, help="override channel count (not the bench line)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--fp64-steps", type=int, default=0,
+                    help="also time the FP64 plain mode for this many steps")
+    return ap.parse_args()
+
+
+def workload(cfg, nchan_override):
+    from rascil_amd import simulation
+
+    if cfg == "c1":
+        uvw, freq, _ = simulation.config1_geometry(nchan=nchan_override or 64)
+        return dict(name="c1", uvw=uvw, freq=freq, npol=4, npix=8192, cell=1.1e-5, eps=1e-5,
+                    desc=("512 stations (70% Gaussian core sigma 500 m + 30% log-uniform 1-40 km), "
+                          "128 HA over 4 h, dec -30, %d ch 100-164 MHz, 4 pol; 8192^2, cell 1.1e-5 rad, "
+                          "eps 1e-5, w-stacking" % len(freq)))
+    bvis = simulation.config0()
+    return dict(name="c0", uvw=bvis.uvw.reshape(-1, 3), freq=bvis.frequency, npol=1, npix=512,
+                cell=5e-4, eps=1e-7,
+                desc="100 stations 1 km disc, 30 HA, dec -45, 1 ch 100 MHz, Stokes I; 512^2, cell 5e-4, eps 1e-7")
+
+
+def algorithmic_bytes(name, plan, nitems, nb, csz):
+    """HBM bytes one launch of kernel class `name` must move at minimum."""
+    G2 = plan["gx"] * plan["gy"]
+    N2 = plan["nx"] * plan["ny"]
+    P = plan["nplanes"]
+    ww = plan["W"] if plan["wstacking"] else 1
+    rsz = 4 if csz == 8 else 8
+    items_pp = nitems * ww / P
+    if name == "grid":  # records of the plane's visibilities in, plane out
+        return items_pp * (4 + 3 * rsz + nb * csz) + nb * G2 * csz
+    if name == "degrid":  # plane in, records in, sorted accumulators read+write
+        return nb * G2 * csz + items_pp * (4 + 3 * rsz + 2 * nb * csz)
+    if name.startswith("fft"):  # one in-place read + write of the plane
+        return 2 * nb * G2 * csz
+    if name == "screen_acc":  # plane pixels in, FP64 accumulator read + write
+        return nb * N2 * (csz + 16)
+    if name == "screen_fill":  # corrected image in (FP64), plane out
+        return nb * N2 * 8 + nb * G2 * csz
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
+                                   "(the sharding design is described in DESIGN.md)" % args.gpus}))
+        sys.exit(2)
+    import numpy
+    import torch
+
+    from rascil_amd import _lib, wgridder
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    wl = workload(args.config, args.nchan)
+    prec = args.precision if args.config == "c1" else "fp64"
+    nrow, nchan, npol, N = wl["uvw"].shape[0], len(wl["freq"]), wl["npol"], wl["npix"]
+    nvis = nrow * nchan * npol
+    print(f"[bench] {wl['name']}: {nrow} rows x {nchan} ch x {npol} pol = {nvis:.3e} vis, "
+          f"{N}^2, precision {prec}", file=sys.stderr, flush=True)
+    cdt = torch.complex128 if prec == "fp64" else torch.complex64
+    rdt = torch.float64 if prec == "fp64" else torch.float32
+    g = torch.Generator(device=dev)
+    g.manual_seed(1234)
+    uvw_d = torch.from_numpy(numpy.ascontiguousarray(wl["uvw"])).to(dev)
+    vis_d = torch.randn((nrow, nchan, npol), dtype=cdt, device=dev, generator=g)
+    wgt_d = torch.ones((nrow, nchan, npol), dtype=rdt, device=dev)
+    model_d = torch.randn((npol, N, N), dtype=rdt, device=dev, generator=g)
+    torch.cuda.synchronize()
+    plan_i, plan_p = {}, {}
+
+    def step(p):
+        wgridder.ms2dirty(uvw_d, wl["freq"], vis_d, wgt_d, N, N, wl["cell"], wl["cell"], 0, 0,
+                          wl["eps"], True, precision=p, plan_out=plan_i)
+        wgridder.dirty2ms(uvw_d, wl["freq"], model_d, None, wl["cell"], wl["cell"], 0, 0,
+                          wl["eps"], True, precision=p, plan_out=plan_p)
+
+    for i in range(args.warmup):
+        t = time.time()
+        step(prec)
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {i} {time.time() - t:.2f} s", file=sys.stderr, flush=True)
+    _lib.profile_enable(True)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.time()
+    e0.record()
+    for i in range(args.steps):
+        step(prec)
+        print(f"[bench] step {i} done", file=sys.stderr, flush=True)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.time() - t0
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    ms_per_step = max(e0.elapsed_time(e1), wall * 1e3) / args.steps
+    value = nvis / (ms_per_step * 1e-3) / 1e6
+
+    csz = 16 if prec == "fp64" else 8
+    kern = []
+    for name, (tot, cnt) in prof.items():
+        plan = plan_p if name in ("degrid", "screen_fill", "fft_c2c_forward") else plan_i
+        b = algorithmic_bytes(name, plan, nrow * nchan, npol, csz)
+        kern.append((tot, name, cnt, b))
+    kern.sort(reverse=True)
+    roof = None
+    for tot, name, cnt, b in kern:
+        if b is not None:
+            avg_s = tot / cnt * 1e-3
+            ach = b / avg_s / 1e9
+            roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "avg_launch_ms": round(tot / cnt, 4), "launches": cnt,
+                    "algorithmic_bytes_per_launch": int(b)}
+            break
+    breakdown = {name: {"total_ms": round(tot, 2), "launches": cnt} for tot, name, cnt, _ in kern}
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mvis/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "mixed": "f32", "fp32": "f32"}[prec], "data": "synthetic",
+        "config": {"workload": wl["desc"], "nvis": nvis, "precision": prec,
+                   "W": plan_i.get("W"), "sigma": plan_i.get("sigma"), "grid": plan_i.get("gx"),
+                   "nplanes": plan_i.get("nplanes"), "tile": plan_i.get("tile")},
+        "roofline": roof,
+        "kernel_ms_total": breakdown,
+    }
+
+    if args.fp64_steps > 0 and prec != "fp64":
+        del vis_d, wgt_d, model_d
+        torch.cuda.empty_cache()
+        _lib.load().hipng_release()
+        vis64 = torch.randn((nrow, nchan, npol), dtype=torch.complex128, device=dev, generator=g)
+        wgt64 = torch.ones((nrow, nchan, npol), dtype=torch.float64, device=dev)
+        mod64 = torch.randn((npol, N, N), dtype=torch.float64, device=dev, generator=g)
+        vis_d, wgt_d, model_d = vis64, wgt64, mod64
+        step("fp64")
+        torch.cuda.synchronize()
+        t = time.time()
+        for _ in range(args.fp64_steps):
+            step("fp64")
+        torch.cuda.synchronize()
+        ms64 = (time.time() - t) * 1e3 / args.fp64_steps
+        out["fp64_plain"] = {"ms_per_step": round(ms64, 1), "value": round(nvis / (ms64 * 1e-3) / 1e6, 2),
+                             "speedup_of_fp32_mode": round(ms64 / ms_per_step, 2),
+                             "W": plan_i.get("W"), "grid": plan_i.get("gx"), "nplanes": plan_i.get("nplanes")}
+
+    if not args.no_cpu:
+        try:
+            from oracle import wstack_cpu
+
+            out["cpu_baseline"] = wstack_cpu.cpu_baseline(wl, plan_i, plan_p, args.cpu_seconds)
+        except Exception as e:  # the baseline leg must not hide the GPU result
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

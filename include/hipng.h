@@ -1,0 +1,143 @@
+/*
+ * hipng — MI355X (gfx950) w-stacking imaging core behind RASCIL's invert_ng / predict_ng.
+ *
+ * C-ABI boundary. Plain pointers and sizes only. Large arrays are DEVICE pointers
+ * (HBM-resident, e.g. torch tensors' data_ptr()); small per-channel arrays are host
+ * pointers. Every entry point returns 0 on success and a negative code on failure;
+ * hipng_last_error() then describes the failure (thread-local).
+ *
+ * The two compute entry points replace the ducc0.wgridder calls RASCIL makes:
+ *   hipng_ms2dirty  <- ducc0.wgridder.ms2dirty  (rascil/processing_components/imaging/ng.py:232-248, 259-275)
+ *   hipng_dirty2ms  <- ducc0.wgridder.dirty2ms  (rascil/processing_components/imaging/ng.py:91-103, 108-120)
+ * with the ducc0 0.24 argument meaning: uvw [nrow][3] in metres, freq [nchan] in Hz,
+ * pixel sizes in radians, image axes [npix_x][npix_y] (x = RA-like first axis),
+ * x = (ix - npix_x/2) * pixsize_x, y = (iy - npix_y/2) * pixsize_y and
+ *   dirty(x,y) = sum_{row,ch} wgt * Re[ ms * exp(+2 pi i f/c (u x + v y - w (n-1))) ] / n
+ *   ms(row,ch) = wgt * sum_{x,y} dirty(x,y)/n * exp(-2 pi i f/c (u x + v y - w (n-1)))
+ * where n = sqrt(1 - x^2 - y^2) with w-stacking and n = 1 (no w term) without it.
+ * Extension over ducc0: a batch of `nbatch` independent images (e.g. polarisations)
+ * that share uvw/freq is processed in one call; ms and wgt are then laid out
+ * [nrow][nchan][nbatch] (nbatch = 1 is exactly the ducc0 layout) and dirty
+ * [nbatch][npix_x][npix_y].
+ *
+ * Precision modes (`precision`):
+ *   HIPNG_FP64  : complex128 ms, float64 wgt/dirty; everything in FP64.
+ *   HIPNG_MIXED : complex64 ms, float32 wgt/dirty; uvw->grid scaling, the w-plane
+ *                 phase (n-1)w and the grid-correction divide in FP64; kernel
+ *                 evaluation, tile accumulation and the FFT in FP32.
+ *   HIPNG_FP32  : complex64 ms, float32 wgt/dirty; everything in FP32.
+ */
+#ifndef HIPNG_H
+#define HIPNG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { HIPNG_FP64 = 0, HIPNG_MIXED = 1, HIPNG_FP32 = 2 };
+
+enum {
+  HIPNG_OK = 0,
+  HIPNG_EINVAL = -1,  /* invalid argument (maps to ValueError) */
+  HIPNG_ERUNTIME = -2 /* HIP / rocFFT / allocation failure (maps to RuntimeError) */
+};
+
+/* Parameters of one gridding job: chosen by hipng_plan_params, reported back by the
+ * compute calls. Units: u/v/w extents in wavelengths, pixel sizes in radians. */
+typedef struct hipng_plan {
+  int32_t precision;  /* HIPNG_FP64 / HIPNG_MIXED / HIPNG_FP32 */
+  int32_t wstacking;  /* 1: improved w-stacking with 1/n, 0: plain 2-D */
+  int32_t W;          /* kernel support in cells (u, v and w) */
+  int32_t tile;       /* uv tile side in cells */
+  int32_t nx, ny;     /* image size */
+  int32_t gx, gy;     /* padded grid size (multiples of tile) */
+  int32_t ntx, nty;   /* tiles per grid side */
+  int32_t nplanes;    /* number of w planes (1 without w-stacking) */
+  int32_t keybits;    /* significant bits of the (plane, tile) bucket key */
+  double beta;        /* ES kernel exp(beta (sqrt(1-z^2) - 1)) */
+  double sigma;       /* nominal oversampling factor */
+  double pixsize_x, pixsize_y;
+  double w0, dw;      /* plane p sits at w = w0 + p dw (wavelengths) */
+  double nshift;      /* (n-1) centre: phase exp(-2 pi i w nshift) is applied per visibility */
+  double umax, vmax, wmin, wmax; /* |u|,|v| maxima and |w| range, wavelengths */
+  double epsilon;     /* requested accuracy */
+  double est_error;   /* modelled kernel error */
+} hipng_plan;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int32_t hipng_version(void);
+
+/* Description of the last error on this thread ("" if none). */
+const char* hipng_last_error(void);
+
+/* Host-only: choose W, beta, oversampling, grid, tile and w planes for the given uvw
+ * extents (wavelengths, after the w >= 0 hermitian flip) and image. No GPU use. */
+int32_t hipng_plan_params(double umax, double vmax, double wmin, double wmax,
+                          int32_t npix_x, int32_t npix_y, double pixsize_x, double pixsize_y,
+                          double epsilon, int32_t do_wstacking, int32_t precision,
+                          int32_t nbatch, hipng_plan* out);
+
+/* Host-only: per-channel scale factors the coordinate kernel uses (bit-exact contract
+ * with the CPU reference sort): sx[c] = f_c/c0 * pixsize_x * gx, sy[c] likewise,
+ * sw[c] = f_c/c0 / dw (0 without w-stacking). Arrays of nchan doubles. */
+int32_t hipng_channel_scales(const hipng_plan* plan, const double* freq, int32_t nchan,
+                             double* sx, double* sy, double* sw);
+
+/* Device: extent of the uvw set in wavelengths: out[0]=max|u|, out[1]=max|v|,
+ * out[2]=min|w|, out[3]=max|w| over all rows and channels. out is a host array. */
+int32_t hipng_uvw_extent(const double* uvw, int64_t nrow, const double* freq, int32_t nchan,
+                         double* out, void* stream);
+
+/* Device: bucket sort of the nrow*nchan visibilities by (w plane, uv tile) for `plan`.
+ * keys_out[j] is the bucket key of the j-th visibility in sorted order, perm_out[j] its
+ * original index row*nchan + chan. Stable: equal keys keep increasing original index. */
+int32_t hipng_bucket_sort(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                          const hipng_plan* plan, uint32_t* keys_out, uint32_t* perm_out,
+                          void* stream);
+
+/* Device: adjoint NUFFT with w-stacking (ducc0.wgridder.ms2dirty semantics, see above).
+ * uvw [nrow][3] f64 (device); freq [nchan] f64 (host); ms [nrow][nchan][nbatch] complex
+ * (c16 for HIPNG_FP64, c8 otherwise); wgt same layout, real, or NULL; dirty out
+ * [nbatch][npix_x][npix_y] real. plan_out (host, nullable) receives the parameters used. */
+int32_t hipng_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                       const void* ms, const void* wgt, int32_t nbatch,
+                       int32_t npix_x, int32_t npix_y, double pixsize_x, double pixsize_y,
+                       double epsilon, int32_t do_wstacking, int32_t precision,
+                       void* dirty, void* stream, hipng_plan* plan_out);
+
+/* Device: forward NUFFT with w-stacking (ducc0.wgridder.dirty2ms semantics).
+ * dirty [nbatch][npix_x][npix_y] real in; ms out [nrow][nchan][nbatch] complex; wgt as for
+ * hipng_ms2dirty (multiplies the output) or NULL. */
+int32_t hipng_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                       const void* dirty, const void* wgt, int32_t nbatch,
+                       int32_t npix_x, int32_t npix_y, double pixsize_x, double pixsize_y,
+                       double epsilon, int32_t do_wstacking, int32_t precision,
+                       void* ms, void* stream, hipng_plan* plan_out);
+
+/* Device: FP64 point-source DFT, replacing dft_kernel / dft_cpu_looped
+ * (rascil/processing_components/imaging/dft.py:123-178, 261-281; CUDA precedent
+ * dft.py:181-258): vis[t][b][c][p] = sum_k flux[k][c][p] exp(-2 pi i (u l_k + v m_k + w (n_k-1)))
+ * with uvw_lambda [t][b][c][3], direction_cosines [k][3] = (l, m, n-1), flux [k][c][p]
+ * complex128, vis complex128 out. All device pointers. */
+int32_t hipng_dft(const double* direction_cosines, const void* flux, const double* uvw_lambda,
+                  int32_t ncomp, int64_t ntimes_baselines, int32_t nchan, int32_t npol,
+                  void* vis, void* stream);
+
+/* Diagnostics (no counterpart in the reference): per-kernel-class timing with HIP
+ * events recorded on the launch stream around every kernel (and rocFFT execution).
+ * hipng_profile_enable(1) resets and starts, (0) stops. hipng_profile_query returns
+ * entry `idx` (name, total milliseconds, number of launches) or HIPNG_EINVAL past the end. */
+int32_t hipng_profile_enable(int32_t on);
+int32_t hipng_profile_query(int32_t idx, char* name, int32_t name_len, double* total_ms,
+                            int64_t* launches);
+
+/* Release cached device workspace and FFT plans (optional; also released at exit). */
+int32_t hipng_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPNG_H */

@@ -1,0 +1,21 @@
+"""rascil_amd: MI355X-native (gfx950) imaging core behind RASCIL's invert_ng / predict_ng.
+
+Public surface (mirrors the reference's names):
+  invert_ng, predict_ng                      (rascil/processing_components/imaging/ng.py)
+  invert_blockvisibility, predict_blockvisibility   (imaging/imaging.py)
+  wgridder.ms2dirty, wgridder.dirty2ms       (the ducc0.wgridder surface ng.py calls)
+  dft_skycomponent_visibility                (imaging/dft.py)
+  BlockVisibility, Image, PolarisationFrame, Skycomponent, Direction
+The compute runs in the hipng HIP library (rascil_amd/_hipng.so); nothing falls back to
+the CPU.
+"""
+
+from .data_models import (BlockVisibility, Direction, Image, PolarisationFrame, Skycomponent,
+                          copy_visibility, create_image, get_parameter, image_is_canonical)
+from .dft import dft_skycomponent_visibility
+from .imaging import invert_blockvisibility, predict_blockvisibility
+from .imaging_base import normalise_sumwt, shift_vis_to_image
+from .ng import invert_ng, predict_ng
+from .polarisation import convert_pol_frame
+
+__version__ = "0.1.0"

@@ -1,0 +1,155 @@
+"""Loader for the hipng C-ABI shared library (``rascil_amd/_hipng.so``).
+
+The library is the only compute path of this package: when it is missing or fails to
+load, every entry point raises instead of falling back to anything else.
+"""
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "_hipng.so")
+
+HIPNG_FP64, HIPNG_MIXED, HIPNG_FP32 = 0, 1, 2
+HIPNG_OK, HIPNG_EINVAL, HIPNG_ERUNTIME = 0, -1, -2
+
+PRECISIONS = {"fp64": HIPNG_FP64, "mixed": HIPNG_MIXED, "fp32": HIPNG_FP32}
+
+# Every function declared in include/hipng.h
+EXPORTS = (
+    "hipng_version",
+    "hipng_last_error",
+    "hipng_plan_params",
+    "hipng_channel_scales",
+    "hipng_uvw_extent",
+    "hipng_bucket_sort",
+    "hipng_ms2dirty",
+    "hipng_dirty2ms",
+    "hipng_dft",
+    "hipng_profile_enable",
+    "hipng_profile_query",
+    "hipng_release",
+)
+
+
+class HipngPlan(ctypes.Structure):
+    """Mirror of ``hipng_plan`` in include/hipng.h."""
+
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("wstacking", ctypes.c_int32),
+        ("W", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("gx", ctypes.c_int32),
+        ("gy", ctypes.c_int32),
+        ("ntx", ctypes.c_int32),
+        ("nty", ctypes.c_int32),
+        ("nplanes", ctypes.c_int32),
+        ("keybits", ctypes.c_int32),
+        ("beta", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+        ("pixsize_x", ctypes.c_double),
+        ("pixsize_y", ctypes.c_double),
+        ("w0", ctypes.c_double),
+        ("dw", ctypes.c_double),
+        ("nshift", ctypes.c_double),
+        ("umax", ctypes.c_double),
+        ("vmax", ctypes.c_double),
+        ("wmin", ctypes.c_double),
+        ("wmax", ctypes.c_double),
+        ("epsilon", ctypes.c_double),
+        ("est_error", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+class HipngLibraryMissing(RuntimeError):
+    """The compiled hipng library is not available."""
+
+
+def load():
+    """Load (once) and return the ctypes handle of the hipng library."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIBPATH):
+            raise HipngLibraryMissing(
+                f"hipng library not built: {LIBPATH} is missing "
+                "(run rascil_amd.build.build() or __graft_entry__.build())"
+            )
+        lib = ctypes.CDLL(LIBPATH)
+        c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+        pd = ctypes.POINTER(ctypes.c_double)
+        pplan = ctypes.POINTER(HipngPlan)
+        lib.hipng_version.restype = c_i32
+        lib.hipng_last_error.restype = ctypes.c_char_p
+        lib.hipng_plan_params.argtypes = [c_dbl, c_dbl, c_dbl, c_dbl, c_i32, c_i32, c_dbl, c_dbl,
+                                          c_dbl, c_i32, c_i32, c_i32, pplan]
+        lib.hipng_plan_params.restype = c_i32
+        lib.hipng_channel_scales.argtypes = [pplan, pd, c_i32, pd, pd, pd]
+        lib.hipng_channel_scales.restype = c_i32
+        lib.hipng_uvw_extent.argtypes = [c_vp, c_i64, pd, c_i32, pd, c_vp]
+        lib.hipng_uvw_extent.restype = c_i32
+        lib.hipng_bucket_sort.argtypes = [c_vp, pd, c_i64, c_i32, pplan, c_vp, c_vp, c_vp]
+        lib.hipng_bucket_sort.restype = c_i32
+        for name in ("hipng_ms2dirty", "hipng_dirty2ms"):
+            fn = getattr(lib, name)
+            fn.argtypes = [c_vp, pd, c_i64, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_dbl,
+                           c_dbl, c_i32, c_i32, c_vp, c_vp, pplan]
+            fn.restype = c_i32
+        lib.hipng_dft.argtypes = [c_vp, c_vp, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp, c_vp]
+        lib.hipng_dft.restype = c_i32
+        lib.hipng_release.restype = c_i32
+        lib.hipng_profile_enable.argtypes = [c_i32]
+        lib.hipng_profile_enable.restype = c_i32
+        lib.hipng_profile_query.argtypes = [c_i32, ctypes.c_char_p, c_i32, pd,
+                                            ctypes.POINTER(ctypes.c_int64)]
+        lib.hipng_profile_query.restype = c_i32
+        _lib = lib
+        return lib
+
+
+def check(code):
+    """Turn a hipng return code into the exception ducc0 would raise."""
+    if code == HIPNG_OK:
+        return
+    msg = load().hipng_last_error().decode(errors="replace")
+    if code == HIPNG_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def dptr(x):
+    """Address of a numpy array's buffer as ctypes double pointer."""
+    return x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def profile_enable(on=True):
+    """Start (reset) or stop per-kernel HIP-event timing inside the library."""
+    check(load().hipng_profile_enable(1 if on else 0))
+
+
+def profile_read():
+    """{kernel class: (total_ms, launches)} accumulated since profile_enable(True)."""
+    lib = load()
+    out = {}
+    i = 0
+    name = ctypes.create_string_buffer(64)
+    tot = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    while lib.hipng_profile_query(i, name, 64, ctypes.byref(tot), ctypes.byref(cnt)) == HIPNG_OK:
+        out[name.value.decode()] = (tot.value, cnt.value)
+        i += 1
+    return out

@@ -1,0 +1,1651 @@
+// hipng: w-stacking NUFFT imaging core for MI355X (gfx950).
+//
+// Replaces the arithmetic RASCIL hands to ducc0.wgridder (ms2dirty / dirty2ms) in
+// rascil/processing_components/imaging/ng.py:91-120 and :232-275. The algorithm is
+// this library's own design (exponential-of-semicircle kernel, improved w-stacking,
+// bucket sort by (w plane, uv tile), per-plane FFT), laid out for CDNA4:
+//
+//   k_extent       uvw extent reduction (one host sync to choose the plan)
+//   k_keys         FP64 uvw -> grid coordinates, (plane, tile) bucket key per visibility
+//   k_rs_*         stable LSD radix sort (8-bit digits, wave-ballot ranking) of the keys
+//   k_bounds       bucket [start, end) table from the sorted keys
+//   k_records      sorted, compact per-visibility records: packed first cell, FP32/FP64
+//                  kernel offsets and (for invert) the weighted, phase-shifted values
+//   k_grid         output-stationary tile gridder: one workgroup owns a T x T tile of
+//                  one w plane, accumulates the footprints of the buckets that reach it
+//                  in LDS and writes the tile once (no HBM atomics, no plane memset)
+//   rocFFT         in-place batched 2-D c2c transform of the plane
+//   k_screen_acc   fused w-screen exp(-2 pi i w_p (n-1)) * checkerboard * plane -> real
+//                  image accumulator (FP64 except in FP32 mode)
+//   k_finalize     grid correction 1/(Phi_x Phi_y Phi_w n), FP64
+//   k_prescale / k_screen_fill / k_degrid / k_unsort   the adjoint chain for predict;
+//                  k_degrid is a gather (one lane per visibility, LDS-resident tile
+//                  with halo), no atomics.
+//   k_dft          FP64 point-source DFT (dft.py:181-258 / :261-281 semantics)
+
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/hipng.h"
+
+namespace {
+
+constexpr double kSpeedOfLight = 299792458.0;  // rascil/phyconst.py:6
+constexpr double kPi = 3.14159265358979323846;
+constexpr int kBlock = 256;
+constexpr int kMaxBatch = 4;
+
+thread_local std::string g_err;
+
+struct HipngError {
+  int code;
+  std::string msg;
+};
+
+#define HIPNG_THROW(code, ...)                           \
+  do {                                                   \
+    char _b[512];                                        \
+    snprintf(_b, sizeof(_b), __VA_ARGS__);               \
+    throw HipngError{code, std::string(_b)};             \
+  } while (0)
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      HIPNG_THROW(HIPNG_ERUNTIME, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                  \
+  } while (0)
+
+#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
+
+#define FFT_TRY(expr)                                                                    \
+  do {                                                                                   \
+    rocfft_status _s = (expr);                                                           \
+    if (_s != rocfft_status_success)                                                     \
+      HIPNG_THROW(HIPNG_ERUNTIME, "%s failed: rocfft status %d", #expr, (int)_s);       \
+  } while (0)
+
+// ---------------------------------------------------------------------------------
+// Exponential-of-semicircle kernel table: psi(z) = exp(beta (sqrt(1 - z^2) - 1)), |z| < 1,
+// support W cells, grid oversampling sigma. beta/W and the measured 1-D L2 error of a
+// type-1 NUFFT (random points, exact sum as reference) per (sigma, W); generated by
+// rascil_amd/csrc/gen_es_table.py.
+// ---------------------------------------------------------------------------------
+constexpr int kNSig = 5;
+constexpr int kWMin = 4, kWMax = 16;
+const double kSigmas[kNSig] = {1.25, 1.5, 1.75, 2.0, 2.5};
+const double kBetaW[kNSig][13] = {
+    {1.750, 1.775, 1.800, 1.825, 1.825, 1.850, 1.800, 1.800, 1.800, 1.825, 1.825, 1.850, 1.875},
+    {1.950, 2.025, 2.025, 2.025, 2.050, 1.975, 2.050, 2.075, 2.025, 2.025, 2.075, 2.050, 2.050},
+    {2.125, 2.150, 2.175, 2.175, 2.200, 2.125, 2.150, 2.175, 2.200, 2.225, 2.200, 2.200, 2.200},
+    {2.225, 2.250, 2.300, 2.300, 2.325, 2.250, 2.250, 2.275, 2.325, 2.300, 2.325, 2.300, 2.325},
+    {2.400, 2.425, 2.450, 2.475, 2.475, 2.425, 2.450, 2.500, 2.500, 2.450, 2.475, 2.375, 2.500}};
+const double kErr1D[kNSig][13] = {
+    {5.4e-3, 1.2e-3, 3.1e-4, 8.3e-5, 2.1e-5, 5.5e-6, 1.5e-6, 4.1e-7, 1.1e-7, 3.3e-8, 7.3e-9, 1.5e-9, 3.2e-10},
+    {1.4e-3, 2.9e-4, 4.8e-5, 8.4e-6, 1.4e-6, 2.6e-7, 4.5e-8, 5.9e-9, 1.3e-9, 2.1e-10, 3.1e-11, 5.0e-12, 9.5e-13},
+    {7.9e-4, 1.0e-4, 1.5e-5, 2.0e-6, 2.3e-7, 4.0e-8, 5.9e-9, 5.9e-10, 7.7e-11, 9.4e-12, 1.2e-12, 1.7e-13, 4.5e-14},
+    {5.1e-4, 6.1e-5, 6.5e-6, 7.5e-7, 7.8e-8, 1.0e-8, 1.2e-9, 1.3e-10, 1.3e-11, 1.5e-12, 2.1e-13, 2.6e-14, 1.7e-14},
+    {2.6e-4, 2.7e-5, 2.5e-6, 2.4e-7, 2.1e-8, 2.1e-9, 2.0e-10, 1.7e-11, 1.5e-12, 1.4e-13, 2.3e-14, 2.2e-14, 2.2e-14}};
+
+// Gauss-Legendre nodes/weights on [-1, 1] (Newton on P_n), positive half returned.
+void gauss_legendre_half(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.clear();
+  w.clear();
+  for (int i = 1; i <= n / 2; ++i) {
+    double z = std::cos(kPi * (i - 0.25) / (n + 0.5));
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      double dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; ++k) {
+      double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    double dp = n * (z * p1 - p0) / (z * z - 1.0);
+    x.push_back(z);
+    w.push_back(2.0 / ((1.0 - z * z) * dp * dp));
+  }
+}
+
+constexpr int kQuadHalf = 64;  // 128-node Gauss-Legendre rule for the kernel's Fourier transform
+
+// Phi(xi) = int phi(s) exp(2 pi i s xi) ds, phi(s) = psi(2 s / W), as sum_q c_q cos(pi W z_q xi).
+struct KernelFT {
+  std::vector<double> z, c;
+  void init(int W, double beta) {
+    std::vector<double> x, w;
+    gauss_legendre_half(2 * kQuadHalf, x, w);
+    z = x;
+    c.resize(x.size());
+    for (size_t q = 0; q < x.size(); ++q)
+      c[q] = W * w[q] * std::exp(beta * (std::sqrt(1.0 - x[q] * x[q]) - 1.0));
+  }
+  double operator()(double xi, int W) const {
+    double s = 0.0;
+    for (size_t q = 0; q < z.size(); ++q) s += c[q] * std::cos(kPi * W * z[q] * xi);
+    return s;
+  }
+};
+
+bool good_factor(int64_t m) {
+  for (int f : {2, 3, 5}) while (m % f == 0) m /= f;
+  return m == 1;
+}
+
+// Smallest multiple of `tile` >= n whose quotient has only factors 2, 3, 5.
+int good_grid(double n, int tile) {
+  int64_t m = (int64_t)std::ceil(n / tile);
+  if (m < 2) m = 2;
+  while (!good_factor(m)) ++m;
+  return (int)(m * tile);
+}
+
+int bits_for(uint64_t v) {
+  int b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
+void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int ny, double px,
+               double py, double eps, int wstack, int prec, int nbatch, int64_t nitems,
+               hipng_plan* P) {
+  if (nx <= 0 || ny <= 0 || (nx & 1) || (ny & 1))
+    HIPNG_THROW(HIPNG_EINVAL, "npix_x and npix_y must be positive and even (got %d, %d)", nx, ny);
+  if (!(px > 0) || !(py > 0)) HIPNG_THROW(HIPNG_EINVAL, "pixel sizes must be positive");
+  if (prec < 0 || prec > 2) HIPNG_THROW(HIPNG_EINVAL, "unknown precision mode %d", prec);
+  if (nbatch < 1 || nbatch > kMaxBatch)
+    HIPNG_THROW(HIPNG_EINVAL, "nbatch must be in [1, %d] (got %d)", kMaxBatch, nbatch);
+  const double epsmin = prec == HIPNG_FP64 ? 1e-13 : 1e-7;
+  if (!(eps >= epsmin) || !(eps < 1.0))
+    HIPNG_THROW(HIPNG_EINVAL, "epsilon %g outside [%g, 1) for this precision", eps, epsmin);
+  std::memset(P, 0, sizeof(*P));
+  P->precision = prec;
+  P->wstacking = wstack ? 1 : 0;
+  P->nx = nx;
+  P->ny = ny;
+  P->pixsize_x = px;
+  P->pixsize_y = py;
+  P->umax = umax;
+  P->vmax = vmax;
+  P->wmin = wmin;
+  P->wmax = wmax;
+  P->epsilon = eps;
+  // corner of the image: largest direction cosine radius
+  const double xc = 0.5 * nx * px, yc = 0.5 * ny * py;
+  const double r2 = xc * xc + yc * yc;
+  if (wstack && !(r2 < 1.0))
+    HIPNG_THROW(HIPNG_EINVAL, "image extends beyond the celestial sphere (l^2+m^2 = %g)", r2);
+  const double nm1min = wstack ? -r2 / (std::sqrt(1.0 - r2) + 1.0) : 0.0;
+  const double nm1half = -0.5 * nm1min;  // max |(n-1) - nshift|
+  const int ndim = wstack ? 3 : 2;
+  const double safety = 2.0 * std::sqrt((double)ndim);
+  const int lds_budget = 64 * 1024;
+  const size_t csz = prec == HIPNG_FP64 ? 16 : 8;
+  double best_cost = 1e300;
+  for (int is = 0; is < kNSig; ++is) {
+    int W = -1;
+    for (int w = kWMin; w <= kWMax; ++w)
+      if (kErr1D[is][w - kWMin] * safety <= eps) {
+        W = w;
+        break;
+      }
+    if (W < 0) continue;
+    const double sigma = kSigmas[is];
+    // tile: largest of {32, 16} whose degridder LDS tile (T+W-1)^2 * batch fits
+    int T = 32;
+    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) T = 16;
+    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) continue;
+    if (T < W) continue;
+    const int gx = good_grid(std::max(sigma * nx, 2.0 * T), T);
+    const int gy = good_grid(std::max(sigma * ny, 2.0 * T), T);
+    int nplanes = 1;
+    double dw = 1.0, w0 = 0.0;
+    if (wstack) {
+      dw = 0.5 / (sigma * nm1half);
+      w0 = wmin - (0.5 * W + 1.0) * dw;
+      double span = std::floor((wmax - wmin) / dw);
+      if (!(span < 1e6)) continue;
+      nplanes = (int)span + W + 3;
+    }
+    const double ntiles = (double)(gx / T) * (gy / T);
+    if ((double)nplanes * ntiles >= 4294967295.0) continue;
+    const double ww = wstack ? W : 1;
+    const double ov = (1.0 + (double)(W - 1) / T) * (1.0 + (double)(W - 1) / T);
+    const double g2 = (double)gx * gy;
+    const double pf = prec == HIPNG_FP64 ? 2.0 : 1.0;
+    const double cost = pf * (5e-12 * (double)nitems * nbatch * W * W * ww * ov +
+                              5.3e-13 * nplanes * nbatch * g2 * std::log2(g2) +
+                              2e-12 * nplanes * nbatch * (g2 + 2.0 * nx * ny));
+    if (cost < best_cost) {
+      best_cost = cost;
+      P->W = W;
+      P->tile = T;
+      P->gx = gx;
+      P->gy = gy;
+      P->ntx = gx / T;
+      P->nty = gy / T;
+      P->nplanes = nplanes;
+      P->beta = kBetaW[is][W - kWMin] * W;
+      P->sigma = sigma;
+      P->dw = dw;
+      P->w0 = w0;
+      P->nshift = wstack ? 0.5 * nm1min : 0.0;
+      P->est_error = kErr1D[is][W - kWMin] * std::sqrt((double)ndim);
+    }
+  }
+  if (P->W == 0)
+    HIPNG_THROW(HIPNG_EINVAL, "no kernel/grid reaches epsilon %g for this problem", eps);
+  const uint64_t nkeys = (uint64_t)P->nplanes * P->ntx * P->nty;
+  P->keybits = bits_for(nkeys - 1);
+  if (P->keybits == 0) P->keybits = 1;
+}
+
+// ---------------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------------
+template <typename R>
+struct cplx {
+  R re, im;
+};
+
+template <typename R>
+struct rec3 {
+  R x, y, w;
+};
+
+__device__ __forceinline__ float esk(float z, float beta) {
+  const float t = 1.0f - z * z;
+  return t > 0.0f ? expf(beta * (sqrtf(t) - 1.0f)) : 0.0f;
+}
+__device__ __forceinline__ double esk(double z, double beta) {
+  const double t = 1.0 - z * z;
+  return t > 0.0 ? exp(beta * (sqrt(t) - 1.0)) : 0.0;
+}
+
+// Plan constants the kernels need, passed by value.
+struct KP {
+  int gx, gy, hgx, hgy;  // grid and half grid
+  int T, ntx, nty;
+  uint32_t ntiles;
+  int W, Ww, nplanes, wstack;
+  int nx, ny, nb, nchan;
+  double hgxd, hgyd, hw, mw0;  // G/2, W/2, -w0/dw
+  double beta, px, py, nshift, dw, w0;
+};
+
+struct ItemPos {
+  long long ax, ay;
+  int s;
+  double fx, fy, fw;
+  bool flip;
+};
+
+// The coordinate contract shared with the CPU reference sort: with F32C false every
+// operation is an IEEE FP64 fma/sub/floor; with F32C true the same in FP32.
+template <bool F32C>
+__device__ __forceinline__ ItemPos item_pos(double u, double v, double w, double sxc, double syc,
+                                            double swc, const KP& k) {
+  ItemPos r;
+  r.flip = w < 0.0;
+  if (r.flip) {
+    u = -u;
+    v = -v;
+    w = -w;
+  }
+  if (!F32C) {
+    const double ux = fma(u, sxc, k.hgxd);
+    const double a0 = floor(ux - k.hw) + 1.0;
+    r.fx = a0 - ux;
+    r.ax = (long long)a0;
+    const double vy = fma(v, syc, k.hgyd);
+    const double b0 = floor(vy - k.hw) + 1.0;
+    r.fy = b0 - vy;
+    r.ay = (long long)b0;
+    if (k.wstack) {
+      const double wh = fma(w, swc, k.mw0);
+      const double s0 = floor(wh - k.hw) + 1.0;
+      r.fw = s0 - wh;
+      r.s = (int)s0;
+    } else {
+      r.fw = 0.0;
+      r.s = 0;
+    }
+  } else {
+    const float hw = (float)k.hw;
+    const float ux = fmaf((float)u, (float)sxc, (float)k.hgxd);
+    const float a0 = floorf(ux - hw) + 1.0f;
+    r.fx = a0 - ux;
+    r.ax = (long long)a0;
+    const float vy = fmaf((float)v, (float)syc, (float)k.hgyd);
+    const float b0 = floorf(vy - hw) + 1.0f;
+    r.fy = b0 - vy;
+    r.ay = (long long)b0;
+    if (k.wstack) {
+      const float wh = fmaf((float)w, (float)swc, (float)k.mw0);
+      const float s0 = floorf(wh - hw) + 1.0f;
+      r.fw = s0 - wh;
+      r.s = (int)s0;
+    } else {
+      r.fw = 0.0;
+      r.s = 0;
+    }
+  }
+  r.ax %= k.gx;
+  if (r.ax < 0) r.ax += k.gx;
+  r.ay %= k.gy;
+  if (r.ay < 0) r.ay += k.gy;
+  if (r.s < 0) r.s = 0;
+  if (r.s > k.nplanes - k.Ww) r.s = k.nplanes - k.Ww;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t item_key(const ItemPos& p, const KP& k) {
+  return (uint32_t)p.s * k.ntiles + (uint32_t)(p.ax / k.T) * (uint32_t)k.nty +
+         (uint32_t)(p.ay / k.T);
+}
+
+// ---------------------------------------------------------------------------------
+// Kernels: extent, keys
+// ---------------------------------------------------------------------------------
+__global__ void k_extent(const double* __restrict__ uvw, int64_t nrow,
+                         unsigned long long* __restrict__ out) {
+  __shared__ double s[4][kBlock];
+  double mu = 0.0, mv = 0.0, wlo = INFINITY, whi = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow;
+       r += (int64_t)gridDim.x * kBlock) {
+    const double u = fabs(uvw[3 * r]), v = fabs(uvw[3 * r + 1]), w = fabs(uvw[3 * r + 2]);
+    mu = fmax(mu, u);
+    mv = fmax(mv, v);
+    wlo = fmin(wlo, w);
+    whi = fmax(whi, w);
+  }
+  s[0][threadIdx.x] = mu;
+  s[1][threadIdx.x] = mv;
+  s[2][threadIdx.x] = wlo;
+  s[3][threadIdx.x] = whi;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s[0][threadIdx.x] = fmax(s[0][threadIdx.x], s[0][threadIdx.x + o]);
+      s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][threadIdx.x + o]);
+      s[2][threadIdx.x] = fmin(s[2][threadIdx.x], s[2][threadIdx.x + o]);
+      s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // non-negative doubles order like their bit patterns
+    atomicMax(&out[0], (unsigned long long)__double_as_longlong(s[0][0]));
+    atomicMax(&out[1], (unsigned long long)__double_as_longlong(s[1][0]));
+    atomicMin(&out[2], (unsigned long long)__double_as_longlong(s[2][0]));
+    atomicMax(&out[3], (unsigned long long)__double_as_longlong(s[3][0]));
+  }
+}
+
+template <bool F32C>
+__global__ void k_keys(const double* __restrict__ uvw, int64_t nitems, const double* __restrict__ sx,
+                       const double* __restrict__ sy, const double* __restrict__ sw, KP k,
+                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nitems) return;
+  const int64_t row = i / k.nchan;
+  const int ch = (int)(i - row * k.nchan);
+  const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch],
+                                   sy[ch], sw[ch], k);
+  keys[i] = item_key(p, k);
+  vals[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits. A block owns kRsItems consecutive items and
+// walks them in rounds of 256 (thread order = item order); ranks inside a round come
+// from wave ballots (lanes with equal digit) plus per-wave digit counts in LDS.
+// ---------------------------------------------------------------------------------
+constexpr int kRsIpt = 16;
+constexpr int kRsItems = kBlock * kRsIpt;
+
+__global__ void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                          uint32_t* __restrict__ hist, int64_t nblocks) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kRsItems;
+  for (int r = 0; r < kRsIpt; ++r) {
+    const int64_t i = base + r * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan helpers (uint32), 4096 elements per block
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// returns the exclusive prefix of v over the block and the block total
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int w = 0; w < kBlock / 64; ++w) {
+    if (w < wv) off += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+__global__ void k_scan_partial(const uint32_t* __restrict__ a, int64_t m, uint32_t* __restrict__ part) {
+  const int64_t base = (int64_t)blockIdx.x * kRsItems + (int64_t)threadIdx.x * kRsIpt;
+  uint32_t s = 0;
+  for (int k = 0; k < kRsIpt; ++k)
+    if (base + k < m) s += a[base + k];
+  uint32_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_top(uint32_t* __restrict__ part, int64_t np) {
+  uint32_t carry = 0;
+  for (int64_t c = 0; c < np; c += kBlock) {
+    const int64_t i = c + threadIdx.x;
+    const uint32_t v = i < np ? part[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, &tot);
+    if (i < np) part[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ void k_scan_apply(uint32_t* __restrict__ a, int64_t m, const uint32_t* __restrict__ part) {
+  const int64_t base = (int64_t)blockIdx.x * kRsItems + (int64_t)threadIdx.x * kRsIpt;
+  uint32_t loc[kRsIpt];
+  uint32_t s = 0;
+  for (int k = 0; k < kRsIpt; ++k) {
+    loc[k] = base + k < m ? a[base + k] : 0u;
+    s += loc[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan(s, &tot) + part[blockIdx.x];
+  for (int k = 0; k < kRsIpt; ++k) {
+    if (base + k < m) a[base + k] = run;
+    run += loc[k];
+  }
+}
+
+__global__ void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                             uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
+                             int shift, const uint32_t* __restrict__ hist, int64_t nblocks) {
+  __shared__ uint32_t off[256];
+  __shared__ uint32_t wc[kBlock / 64][256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  off[threadIdx.x] = hist[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t base = (int64_t)blockIdx.x * kRsItems;
+  for (int r = 0; r < kRsIpt; ++r) {
+    for (int w = 0; w < kBlock / 64; ++w) wc[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = base + r * kBlock + threadIdx.x;
+    const bool ok = i < n;
+    uint32_t key = 0, val = 0, d = 0;
+    if (ok) {
+      key = kin[i];
+      val = vin[i];
+      d = (key >> shift) & 255u;
+    }
+    unsigned long long peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const unsigned long long m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (ok && (peers & lt) == 0ull) wc[wv][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      const int dd = threadIdx.x;
+      uint32_t run = off[dd];
+      for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t t = wc[w][dd];
+        wc[w][dd] = run;
+        run += t;
+      }
+      off[dd] = run;
+    }
+    __syncthreads();
+    if (ok) {
+      const uint32_t pos = wc[wv][d] + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    a[i] = v;
+}
+
+template <typename T>
+__global__ void k_zero(T* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    a[i] = T{};
+}
+
+__global__ void k_bounds(const uint32_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ bstart,
+                         uint32_t* __restrict__ bend) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t k = keys[j];
+  if (j == 0 || keys[j - 1] != k) bstart[k] = (uint32_t)j;
+  if (j == n - 1 || keys[j + 1] != k) bend[k] = (uint32_t)(j + 1);
+}
+
+// ---------------------------------------------------------------------------------
+// Sorted records
+// ---------------------------------------------------------------------------------
+template <typename R>
+struct VisT;
+template <>
+struct VisT<double> {
+  using C = cplx<double>;
+  using W = double;
+};
+template <>
+struct VisT<float> {
+  using C = cplx<float>;
+  using W = float;
+};
+
+// turns -> (cos, sin)(2 pi turns); FP64 range reduction unless F32C
+template <typename R, bool F32C>
+__device__ __forceinline__ void phase_cs(double turns, R* c, R* s);
+template <>
+__device__ __forceinline__ void phase_cs<double, false>(double turns, double* c, double* s) {
+  const double t = turns - rint(turns);
+  sincospi(2.0 * t, s, c);
+}
+template <>
+__device__ __forceinline__ void phase_cs<float, false>(double turns, float* c, float* s) {
+  const double t = turns - rint(turns);
+  sincospif((float)(2.0 * t), s, c);
+}
+template <>
+__device__ __forceinline__ void phase_cs<float, true>(double turns, float* c, float* s) {
+  float t = (float)turns;
+  t = t - rintf(t);
+  sincospif(2.0f * t, s, c);
+}
+
+template <typename R, bool F32C, bool VAL>
+__global__ void k_records(const double* __restrict__ uvw, const uint32_t* __restrict__ perm,
+                          int64_t n, const double* __restrict__ sx, const double* __restrict__ sy,
+                          const double* __restrict__ sw, const double* __restrict__ pn, KP k,
+                          const cplx<R>* __restrict__ ms, const R* __restrict__ wgt,
+                          uint32_t* __restrict__ rec_a, rec3<R>* __restrict__ rec_f,
+                          cplx<R>* __restrict__ rec_v) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  const int64_t row = i / k.nchan;
+  const int ch = (int)(i - row * k.nchan);
+  const double w = uvw[3 * row + 2];
+  const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], w, sx[ch], sy[ch], sw[ch], k);
+  rec_a[j] = (uint32_t)p.ax | ((uint32_t)p.ay << 16);
+  rec3<R> f;
+  f.x = (R)p.fx;
+  f.y = (R)p.fy;
+  f.w = (R)p.fw;
+  rec_f[j] = f;
+  if (VAL) {
+    R c = 1, s = 0;
+    if (k.wstack) phase_cs<R, F32C>(-fabs(w) * pn[ch], &c, &s);  // exp(-2 pi i w nshift)
+    for (int b = 0; b < k.nb; ++b) {
+      cplx<R> v = ms[(int64_t)i * k.nb + b];
+      if (p.flip) v.im = -v.im;
+      if (wgt) {
+        const R g = wgt[(int64_t)i * k.nb + b];
+        v.re *= g;
+        v.im *= g;
+      }
+      cplx<R> o;
+      o.re = v.re * c - v.im * s;
+      o.im = v.re * s + v.im * c;
+      rec_v[j * k.nb + b] = o;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Gridder: output-stationary, one workgroup per (plane p, tile)
+// ---------------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __restrict__ bstart,
+                                                 const uint32_t* __restrict__ bend,
+                                                 const uint32_t* __restrict__ rec_a,
+                                                 const rec3<R>* __restrict__ rec_f,
+                                                 const cplx<R>* __restrict__ rec_v,
+                                                 cplx<R>* __restrict__ plane) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R* acc = reinterpret_cast<R*>(smem_raw);  // [nb][T][T][2]
+  const int T = k.T, TT = k.T * k.T;
+  const int tile = blockIdx.x;
+  const int tx = tile / k.nty, ty = tile - tx * k.nty;
+  for (int e = threadIdx.x; e < 2 * k.nb * TT; e += kBlock) acc[e] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int W = k.W;
+  const R scale = (R)2 / (R)W;
+  const R beta = (R)k.beta;
+  const int slo = max(0, p - (k.Ww - 1)), shi = min(p, k.nplanes - k.Ww);
+  for (int s = slo; s <= shi; ++s) {
+    const R iw = (R)(p - s);
+    for (int q = 0; q < 4; ++q) {
+      int stx = tx - (q >> 1), sty = ty - (q & 1);
+      if (stx < 0) stx += k.ntx;
+      if (sty < 0) sty += k.nty;
+      const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)stx * k.nty + (uint32_t)sty;
+      const uint32_t st = bstart[key], en = bend[key];
+      for (uint32_t base = st + wv * 64; base < en; base += kBlock) {
+        const uint32_t j = base + lane;
+        bool ok = false;
+        int dx = 0, dy = 0;
+        if (j < en) {
+          const uint32_t A = rec_a[j];
+          dx = (int)(A & 0xffffu) - tx * T;
+          dy = (int)(A >> 16) - ty * T;
+          if (dx >= k.hgx) dx -= k.gx;
+          if (dx < -k.hgx) dx += k.gx;
+          if (dy >= k.hgy) dy -= k.gy;
+          if (dy < -k.hgy) dy += k.gy;
+          ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
+        }
+        unsigned long long m = __ballot(ok);
+        while (m) {
+          const int src = __ffsll((long long)m) - 1;
+          m &= m - 1;
+          const int dxk = __shfl(dx, src, 64), dyk = __shfl(dy, src, 64);
+          const uint32_t jk = base + src;
+          const rec3<R> f = rec_f[jk];
+          const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
+          for (int c = lane; c < W * W; c += 64) {
+            const int ix = c / W, iy = c - ix * W;
+            const int rx = dxk + ix, ry = dyk + iy;
+            if (rx >= 0 && rx < T && ry >= 0 && ry < T) {
+              const R wt = esk((f.x + (R)ix) * scale, beta) * esk((f.y + (R)iy) * scale, beta) * kw;
+              for (int b = 0; b < k.nb; ++b) {
+                const cplx<R> v = rec_v[(int64_t)jk * k.nb + b];
+                R* a = acc + 2 * (b * TT + rx * T + ry);
+                atomicAdd(a, wt * v.re);
+                atomicAdd(a + 1, wt * v.im);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < k.nb * TT; e += kBlock) {
+    const int b = e / TT, rem = e - b * TT, r = rem / T, c = rem - r * T;
+    cplx<R> o;
+    o.re = acc[2 * e];
+    o.im = acc[2 * e + 1];
+    plane[((int64_t)b * k.gx + tx * T + r) * k.gy + ty * T + c] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Image-domain kernels
+// ---------------------------------------------------------------------------------
+// (n-1) - nshift at pixel (tx, ty) relative to the image centre, FP64 or FP32
+template <bool F32C>
+__device__ __forceinline__ double nm1s_at(int tx, int ty, const KP& k, double* n_out) {
+  if (!F32C) {
+    const double x = tx * k.px, y = ty * k.py;
+    const double r2 = x * x + y * y;
+    const double rt = sqrt(1.0 - r2);
+    const double nm1 = -r2 / (rt + 1.0);
+    if (n_out) *n_out = rt;
+    return nm1 - k.nshift;
+  } else {
+    const float x = tx * (float)k.px, y = ty * (float)k.py;
+    const float r2 = x * x + y * y;
+    const float rt = sqrtf(1.0f - r2);
+    const float nm1 = -r2 / (rt + 1.0f);
+    if (n_out) *n_out = rt;
+    return (double)(nm1 - (float)k.nshift);
+  }
+}
+
+// invert: acc[b][ix][iy] += Re(exp(-2 pi i w_p nm1s) (-1)^(tx+ty) plane[b][tx mod G][ty mod G])
+template <typename R, typename A, bool F32C>
+__global__ void k_screen_acc(KP k, double wp, const cplx<R>* __restrict__ plane, A* __restrict__ acc) {
+  const int iy = blockIdx.x * kBlock + threadIdx.x;
+  const int ix = blockIdx.y;
+  if (iy >= k.ny) return;
+  const int tx = ix - k.nx / 2, ty = iy - k.ny / 2;
+  const int gxi = tx < 0 ? tx + k.gx : tx, gyi = ty < 0 ? ty + k.gy : ty;
+  R c = 1, s = 0;
+  if (k.wstack) phase_cs<R, F32C>(-wp * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &s);
+  const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
+  c *= sign;
+  s *= sign;
+  for (int b = 0; b < k.nb; ++b) {
+    const cplx<R> F = plane[((int64_t)b * k.gx + gxi) * k.gy + gyi];
+    acc[((int64_t)b * k.nx + ix) * k.ny + iy] += (A)(F.re * c - F.im * s);
+  }
+}
+
+// 1 / (Phi_x Phi_y [Phi_w(dw nm1s) n]) at pixel (ix, iy)
+template <typename A>
+__device__ __forceinline__ A corr_at(int ix, int iy, const KP& k, const double* __restrict__ phix,
+                                     const double* __restrict__ phiy, const double* __restrict__ qz,
+                                     const double* __restrict__ qc, int nq) {
+  double den = phix[ix] * phiy[iy];
+  if (k.wstack) {
+    double n;
+    const double xi = k.dw * nm1s_at<false>(ix - k.nx / 2, iy - k.ny / 2, k, &n);
+    double ph = 0.0;
+    for (int q = 0; q < nq; ++q) ph += qc[q] * cospi((double)k.W * qz[q] * xi);
+    den *= ph * n;
+  }
+  return (A)(1.0 / den);
+}
+
+template <typename A, typename O>
+__global__ void k_finalize(KP k, const A* __restrict__ acc, O* __restrict__ dirty,
+                           const double* __restrict__ phix, const double* __restrict__ phiy,
+                           const double* __restrict__ qz, const double* __restrict__ qc, int nq) {
+  const int iy = blockIdx.x * kBlock + threadIdx.x;
+  const int ix = blockIdx.y;
+  if (iy >= k.ny) return;
+  const A f = corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq);
+  for (int b = 0; b < k.nb; ++b) {
+    const int64_t e = ((int64_t)b * k.nx + ix) * k.ny + iy;
+    dirty[e] = (O)(acc[e] * f);
+  }
+}
+
+// predict: dc = dirty * correction
+template <typename I, typename A>
+__global__ void k_prescale(KP k, const I* __restrict__ dirty, A* __restrict__ dc,
+                           const double* __restrict__ phix, const double* __restrict__ phiy,
+                           const double* __restrict__ qz, const double* __restrict__ qc, int nq) {
+  const int iy = blockIdx.x * kBlock + threadIdx.x;
+  const int ix = blockIdx.y;
+  if (iy >= k.ny) return;
+  const A f = corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq);
+  for (int b = 0; b < k.nb; ++b) {
+    const int64_t e = ((int64_t)b * k.nx + ix) * k.ny + iy;
+    dc[e] = (A)dirty[e] * f;
+  }
+}
+
+// predict: plane[b][g] = (-1)^(tx+ty) dc[b][t] exp(+2 pi i w_p nm1s(t)) for image pixels, 0 elsewhere
+template <typename R, typename A, bool F32C>
+__global__ void k_screen_fill(KP k, double wp, const A* __restrict__ dc, cplx<R>* __restrict__ plane) {
+  const int gyi = blockIdx.x * kBlock + threadIdx.x;
+  const int gxi = blockIdx.y;
+  if (gyi >= k.gy) return;
+  const int hx = k.nx / 2, hy = k.ny / 2;
+  int tx, ty;
+  bool in = true;
+  if (gxi < hx) tx = gxi;
+  else if (gxi >= k.gx - hx) tx = gxi - k.gx;
+  else { tx = 0; in = false; }
+  if (gyi < hy) ty = gyi;
+  else if (gyi >= k.gy - hy) ty = gyi - k.gy;
+  else { ty = 0; in = false; }
+  if (!in) {
+    for (int b = 0; b < k.nb; ++b) plane[((int64_t)b * k.gx + gxi) * k.gy + gyi] = cplx<R>{0, 0};
+    return;
+  }
+  R c = 1, s = 0;
+  if (k.wstack) phase_cs<R, F32C>(wp * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &s);
+  const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
+  const int ix = tx + hx, iy = ty + hy;
+  for (int b = 0; b < k.nb; ++b) {
+    const R d = (R)dc[((int64_t)b * k.nx + ix) * k.ny + iy] * sign;
+    plane[((int64_t)b * k.gx + gxi) * k.gy + gyi] = cplx<R>{d * c, d * s};
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Degridder: one workgroup per (plane p, tile); plane tile + (W-1) halo resident in LDS;
+// one lane per visibility; separable kernel contraction in registers; no atomics.
+// ---------------------------------------------------------------------------------
+template <typename R, int W>
+__global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p, const uint32_t* __restrict__ bstart,
+                                                   const uint32_t* __restrict__ bend,
+                                                   const uint32_t* __restrict__ rec_a,
+                                                   const rec3<R>* __restrict__ rec_f,
+                                                   const cplx<R>* __restrict__ plane,
+                                                   cplx<R>* __restrict__ accs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);  // [nb][L][L]
+  const int T = k.T, L = k.T + W - 1, LL = L * L;
+  const int tile = blockIdx.x;
+  const int tx = tile / k.nty, ty = tile - tx * k.nty;
+  const int slo = max(0, p - (k.Ww - 1)), shi = min(p, k.nplanes - k.Ww);
+  // skip tiles without work for this plane
+  uint32_t total = 0;
+  for (int s = slo; s <= shi; ++s) {
+    const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)tile;
+    total += bend[key] - bstart[key];
+  }
+  if (total == 0) return;
+  for (int e = threadIdx.x; e < k.nb * LL; e += kBlock) {
+    const int b = e / LL, rem = e - b * LL, r = rem / L, c = rem - r * L;
+    int gxi = tx * T + r, gyi = ty * T + c;
+    if (gxi >= k.gx) gxi -= k.gx;
+    if (gyi >= k.gy) gyi -= k.gy;
+    g[e] = plane[((int64_t)b * k.gx + gxi) * k.gy + gyi];
+  }
+  __syncthreads();
+  const R scale = (R)2 / (R)W;
+  const R beta = (R)k.beta;
+  for (int s = slo; s <= shi; ++s) {
+    const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)tile;
+    const uint32_t st = bstart[key], en = bend[key];
+    const R iw = (R)(p - s);
+    for (uint32_t j = st + threadIdx.x; j < en; j += kBlock) {
+      const uint32_t A = rec_a[j];
+      const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
+      const rec3<R> f = rec_f[j];
+      R kx[W], ky[W];
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        kx[i] = esk((f.x + (R)i) * scale, beta);
+        ky[i] = esk((f.y + (R)i) * scale, beta);
+      }
+      const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
+      R sr[kMaxBatch], si[kMaxBatch];
+#pragma unroll
+      for (int b = 0; b < kMaxBatch; ++b) sr[b] = si[b] = 0;
+#pragma unroll
+      for (int b = 0; b < kMaxBatch; ++b) {
+        if (b < k.nb) {
+          const cplx<R>* gb = g + b * LL + rx0 * L + ry0;
+#pragma unroll
+          for (int ix = 0; ix < W; ++ix) {
+            R rr = 0, ri = 0;
+#pragma unroll
+            for (int iy = 0; iy < W; ++iy) {
+              const cplx<R> v = gb[ix * L + iy];
+              rr += ky[iy] * v.re;
+              ri += ky[iy] * v.im;
+            }
+            sr[b] += kx[ix] * rr;
+            si[b] += kx[ix] * ri;
+          }
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < kMaxBatch; ++b) {
+        if (b < k.nb) {
+          cplx<R> o = accs[(int64_t)j * k.nb + b];
+          o.re += kw * sr[b];
+          o.im += kw * si[b];
+          accs[(int64_t)j * k.nb + b] = o;
+        }
+      }
+    }
+  }
+}
+
+// predict epilogue: ms[i] = wgt * conj?(acc_sorted[j] exp(+2 pi i |w| nshift))
+template <typename R, bool F32C>
+__global__ void k_unsort(const double* __restrict__ uvw, const uint32_t* __restrict__ perm, int64_t n,
+                         const double* __restrict__ pn, KP k, const cplx<R>* __restrict__ accs,
+                         const R* __restrict__ wgt, cplx<R>* __restrict__ ms) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  const int64_t row = i / k.nchan;
+  const int ch = (int)(i - row * k.nchan);
+  const double w = uvw[3 * row + 2];
+  const bool flip = w < 0.0;
+  R c = 1, s = 0;
+  if (k.wstack) phase_cs<R, F32C>(fabs(w) * pn[ch], &c, &s);
+  for (int b = 0; b < k.nb; ++b) {
+    const cplx<R> a = accs[j * k.nb + b];
+    cplx<R> o;
+    o.re = a.re * c - a.im * s;
+    o.im = a.re * s + a.im * c;
+    if (flip) o.im = -o.im;
+    if (wgt) {
+      const R g = wgt[(int64_t)i * k.nb + b];
+      o.re *= g;
+      o.im *= g;
+    }
+    ms[(int64_t)i * k.nb + b] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 point-source DFT: one thread per (time*baseline, channel); components looped,
+// polarisations in registers.
+// ---------------------------------------------------------------------------------
+__global__ void k_dft(const double* __restrict__ dc, const cplx<double>* __restrict__ flux,
+                      const double* __restrict__ uvwl, int ncomp, int64_t ntb, int nchan, int npol,
+                      cplx<double>* __restrict__ vis) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ntb * nchan) return;
+  const int ch = (int)(t % nchan);
+  const double u = uvwl[3 * t], v = uvwl[3 * t + 1], w = uvwl[3 * t + 2];
+  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+  for (int c = 0; c < ncomp; ++c) {
+    const double ph = u * dc[3 * c] + v * dc[3 * c + 1] + w * dc[3 * c + 2];
+    double s, co;
+    sincospi(-2.0 * ph, &s, &co);
+    const cplx<double>* fl = flux + ((int64_t)c * nchan + ch) * npol;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (p < npol) {
+        const cplx<double> f = fl[p];
+        ar[p] += f.re * co - f.im * s;
+        ai[p] += f.re * s + f.im * co;
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    if (p < npol) vis[t * npol + p] = cplx<double>{ar[p], ai[p]};
+}
+
+// ---------------------------------------------------------------------------------
+// Optional per-kernel timing with HIP events on the launch stream
+// ---------------------------------------------------------------------------------
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+bool g_prof = false;
+std::vector<ProfRec> g_pending;
+std::vector<hipEvent_t> g_evpool;
+std::vector<std::pair<std::string, std::pair<double, int64_t>>> g_profacc;
+
+hipEvent_t ev_get() {
+  if (!g_evpool.empty()) {
+    hipEvent_t e = g_evpool.back();
+    g_evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIP_TRY(hipEventCreate(&e));
+  return e;
+}
+
+struct ProfScope {
+  const char* name;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  ProfScope(const char* n, hipStream_t s) : name(n), st(s) {
+    if (g_prof) {
+      a = ev_get();
+      HIP_TRY(hipEventRecord(a, st));
+    }
+  }
+  ~ProfScope() {
+    if (a) {
+      hipEvent_t b = ev_get();
+      if (hipEventRecord(b, st) == hipSuccess) g_pending.push_back({name, a, b});
+    }
+  }
+};
+
+void prof_flush() {
+  for (auto& r : g_pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventSynchronize(r.b));
+    HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+    bool found = false;
+    for (auto& e : g_profacc)
+      if (e.first == r.name) {
+        e.second.first += ms;
+        e.second.second += 1;
+        found = true;
+      }
+    if (!found) g_profacc.push_back({r.name, {(double)ms, 1}});
+    g_evpool.push_back(r.a);
+    g_evpool.push_back(r.b);
+  }
+  g_pending.clear();
+}
+
+#define PROF(name) ProfScope _prof_scope_##__LINE__(name, st)
+
+// ---------------------------------------------------------------------------------
+// Host side: workspace cache, FFT plans, orchestration
+// ---------------------------------------------------------------------------------
+enum Slot {
+  S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
+  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_NSLOTS
+};
+
+struct FftKey {
+  int gx, gy, nb, prec, fwd;
+  bool operator<(const FftKey& o) const {
+    return std::tie(gx, gy, nb, prec, fwd) < std::tie(o.gx, o.gy, o.nb, o.prec, o.fwd);
+  }
+};
+
+struct DevCtx {
+  void* buf[S_NSLOTS] = {};
+  size_t cap[S_NSLOTS] = {};
+  std::map<FftKey, rocfft_plan> plans;
+};
+
+std::mutex g_mu;
+std::map<int, DevCtx> g_ctx;
+bool g_fft_ready = false;
+
+DevCtx& ctx() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  return g_ctx[dev];
+}
+
+void* ws(DevCtx& c, Slot s, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (c.cap[s] < bytes) {
+    if (c.buf[s]) HIP_TRY(hipFree(c.buf[s]));
+    c.buf[s] = nullptr;
+    c.cap[s] = 0;
+    size_t nb = bytes + bytes / 8;  // headroom against repeated regrowth
+    if (hipMalloc(&c.buf[s], nb) != hipSuccess) {
+      (void)hipGetLastError();
+      HIPNG_THROW(HIPNG_ERUNTIME, "device allocation of %zu bytes failed", nb);
+    }
+    c.cap[s] = nb;
+  }
+  return c.buf[s];
+}
+
+rocfft_plan fft_plan(DevCtx& c, int gx, int gy, int nb, int prec, int fwd) {
+  if (!g_fft_ready) {
+    FFT_TRY(rocfft_setup());
+    g_fft_ready = true;
+  }
+  FftKey key{gx, gy, nb, prec == HIPNG_FP64 ? 0 : 1, fwd};
+  auto it = c.plans.find(key);
+  if (it != c.plans.end()) return it->second;
+  rocfft_plan plan = nullptr;
+  const size_t lengths[2] = {(size_t)gy, (size_t)gx};  // fastest dimension first
+  FFT_TRY(rocfft_plan_create(&plan, rocfft_placement_inplace,
+                             fwd ? rocfft_transform_type_complex_forward
+                                 : rocfft_transform_type_complex_inverse,
+                             prec == HIPNG_FP64 ? rocfft_precision_double : rocfft_precision_single,
+                             2, lengths, (size_t)nb, nullptr));
+  c.plans[key] = plan;
+  return plan;
+}
+
+void fft_exec(DevCtx& c, rocfft_plan plan, void* data, hipStream_t st) {
+  size_t wsz = 0;
+  FFT_TRY(rocfft_plan_get_work_buffer_size(plan, &wsz));
+  rocfft_execution_info info = nullptr;
+  FFT_TRY(rocfft_execution_info_create(&info));
+  FFT_TRY(rocfft_execution_info_set_stream(info, st));
+  if (wsz) FFT_TRY(rocfft_execution_info_set_work_buffer(info, ws(c, S_FFTWORK, wsz), wsz));
+  void* bufs[1] = {data};
+  rocfft_status e = rocfft_execute(plan, bufs, nullptr, info);
+  rocfft_execution_info_destroy(info);
+  if (e != rocfft_status_success) HIPNG_THROW(HIPNG_ERUNTIME, "rocfft_execute failed: %d", (int)e);
+}
+
+unsigned grid1(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+unsigned gridcap(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 20); }
+
+void extent(DevCtx& c, const double* uvw, int64_t nrow, const double* freq, int nchan, double out[4],
+            hipStream_t st) {
+  double fmin = INFINITY, fmax = 0.0;
+  for (int i = 0; i < nchan; ++i) {
+    if (!(freq[i] > 0)) HIPNG_THROW(HIPNG_EINVAL, "frequencies must be positive");
+    fmin = std::min(fmin, freq[i]);
+    fmax = std::max(fmax, freq[i]);
+  }
+  if (nrow == 0) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    return;
+  }
+  unsigned long long h[4] = {0ull, 0ull, (unsigned long long)0x7ff0000000000000ull, 0ull};
+  auto* d = (unsigned long long*)ws(c, S_EXT, sizeof(h));
+  HIP_TRY(hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, st));
+  k_extent<<<std::min<int64_t>(grid1(nrow), 2048), kBlock, 0, st>>>(uvw, nrow, d);
+  LAUNCH_CHECK();
+  HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  double e[4];
+  std::memcpy(e, h, sizeof(e));
+  if (!std::isfinite(e[0]) || !std::isfinite(e[1]) || !std::isfinite(e[3]))
+    HIPNG_THROW(HIPNG_EINVAL, "uvw contains non-finite values");
+  out[0] = e[0] * fmax / kSpeedOfLight;
+  out[1] = e[1] * fmax / kSpeedOfLight;
+  out[2] = e[2] * fmin / kSpeedOfLight;
+  out[3] = e[3] * fmax / kSpeedOfLight;
+}
+
+void channel_scales(const hipng_plan& P, const double* freq, int nchan, double* sx, double* sy,
+                    double* sw, double* pn) {
+  for (int c = 0; c < nchan; ++c) {
+    const double fc = freq[c] / kSpeedOfLight;
+    sx[c] = fc * P.pixsize_x * (double)P.gx;
+    sy[c] = fc * P.pixsize_y * (double)P.gy;
+    sw[c] = P.wstacking ? fc / P.dw : 0.0;
+    if (pn) pn[c] = fc * P.nshift;
+  }
+}
+
+KP make_kp(const hipng_plan& P, int nb, int nchan) {
+  KP k;
+  k.gx = P.gx;
+  k.gy = P.gy;
+  k.hgx = P.gx / 2;
+  k.hgy = P.gy / 2;
+  k.T = P.tile;
+  k.ntx = P.ntx;
+  k.nty = P.nty;
+  k.ntiles = (uint32_t)P.ntx * (uint32_t)P.nty;
+  k.W = P.W;
+  k.Ww = P.wstacking ? P.W : 1;
+  k.nplanes = P.nplanes;
+  k.wstack = P.wstacking;
+  k.nx = P.nx;
+  k.ny = P.ny;
+  k.nb = nb;
+  k.nchan = nchan;
+  k.hgxd = 0.5 * P.gx;
+  k.hgyd = 0.5 * P.gy;
+  k.hw = 0.5 * P.W;
+  k.mw0 = P.wstacking ? -(P.w0 / P.dw) : 0.0;
+  k.beta = P.beta;
+  k.px = P.pixsize_x;
+  k.py = P.pixsize_y;
+  k.nshift = P.nshift;
+  k.dw = P.dw;
+  k.w0 = P.w0;
+  return k;
+}
+
+// Small per-call device tables: sx, sy, sw, pn [nchan]; phix [nx], phiy [ny]; quad z, c.
+struct Tables {
+  double *sx, *sy, *sw, *pn, *phix, *phiy, *qz, *qc;
+  int nq;
+};
+
+Tables upload_tables(DevCtx& c, const hipng_plan& P, const double* freq, int nchan, hipStream_t st,
+                     std::vector<double>& host) {
+  KernelFT ft;
+  ft.init(P.W, P.beta);
+  const int nq = (int)ft.z.size();
+  const size_t n = 4 * (size_t)nchan + P.nx + P.ny + 2 * (size_t)nq;
+  host.assign(n, 0.0);
+  double* h = host.data();
+  channel_scales(P, freq, nchan, h, h + nchan, h + 2 * nchan, h + 3 * nchan);
+  double* hx = h + 4 * nchan;
+  double* hy = hx + P.nx;
+  for (int i = 0; i < P.nx; ++i) hx[i] = ft((double)(i - P.nx / 2) / P.gx, P.W);
+  for (int i = 0; i < P.ny; ++i) hy[i] = ft((double)(i - P.ny / 2) / P.gy, P.W);
+  double* hz = hy + P.ny;
+  double* hc = hz + nq;
+  for (int q = 0; q < nq; ++q) {
+    hz[q] = ft.z[q];
+    hc[q] = ft.c[q];
+  }
+  auto* d = (double*)ws(c, S_SMALL, n * sizeof(double));
+  HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st));
+  // host buffer must stay alive until the copy has completed
+  HIP_TRY(hipStreamSynchronize(st));
+  Tables t;
+  t.sx = d;
+  t.sy = d + nchan;
+  t.sw = d + 2 * nchan;
+  t.pn = d + 3 * nchan;
+  t.phix = d + 4 * nchan;
+  t.phiy = t.phix + P.nx;
+  t.qz = t.phiy + P.ny;
+  t.qc = t.qz + nq;
+  t.nq = nq;
+  return t;
+}
+
+// keys + stable radix sort; returns device pointers to sorted keys and permutation
+void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const KP& k,
+                const hipng_plan& P, hipStream_t st, uint32_t** keys_out, uint32_t** perm_out) {
+  auto* k0 = (uint32_t*)ws(c, S_KEYS0, n * 4);
+  auto* v0 = (uint32_t*)ws(c, S_VALS0, n * 4);
+  auto* k1 = (uint32_t*)ws(c, S_KEYS1, n * 4);
+  auto* v1 = (uint32_t*)ws(c, S_VALS1, n * 4);
+  {
+  PROF("keys");
+  if (P.precision == HIPNG_FP32)
+    k_keys<true><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
+  else
+    k_keys<false><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
+  LAUNCH_CHECK();
+  }
+  const int64_t nblocks = (n + kRsItems - 1) / kRsItems;
+  const int64_t m = 256 * nblocks;
+  const int64_t npart = (m + kRsItems - 1) / kRsItems;
+  auto* hist = (uint32_t*)ws(c, S_HIST, m * 4);
+  auto* part = (uint32_t*)ws(c, S_PART, npart * 4);
+  const int passes = (P.keybits + 7) / 8;
+  PROF("radix_sort");
+  for (int ps = 0; ps < passes; ++ps) {
+    const int shift = 8 * ps;
+    k_rs_hist<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, n, shift, hist, nblocks);
+    LAUNCH_CHECK();
+    k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
+    LAUNCH_CHECK();
+    k_scan_top<<<1, kBlock, 0, st>>>(part, npart);
+    LAUNCH_CHECK();
+    k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
+    LAUNCH_CHECK();
+    k_rs_scatter<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks);
+    LAUNCH_CHECK();
+    std::swap(k0, k1);
+    std::swap(v0, v1);
+  }
+  *keys_out = k0;
+  *perm_out = v0;
+}
+
+struct Prepared {
+  hipng_plan P;
+  KP k;
+  Tables tb;
+  uint32_t *keys, *perm, *bstart, *bend, *rec_a;
+  void* rec_f;
+  int64_t n;
+};
+
+void prepare(DevCtx& c, const double* uvw, const double* freq, int64_t nrow, int nchan, int nb,
+             int nx, int ny, double px, double py, double eps, int wstack, int prec, hipStream_t st,
+             std::vector<double>& host, Prepared& pr) {
+  if (!uvw || !freq) HIPNG_THROW(HIPNG_EINVAL, "uvw and freq must not be NULL");
+  if (nrow < 0 || nchan <= 0) HIPNG_THROW(HIPNG_EINVAL, "bad nrow/nchan (%lld, %d)", (long long)nrow, nchan);
+  const int64_t n = nrow * (int64_t)nchan;
+  if (n >= 4294967295LL) HIPNG_THROW(HIPNG_EINVAL, "too many visibilities for 32-bit indices (%lld)", (long long)n);
+  double e[4];
+  extent(c, uvw, nrow, freq, nchan, e, st);
+  make_plan(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wstack, prec, nb, n, &pr.P);
+  pr.k = make_kp(pr.P, nb, nchan);
+  pr.tb = upload_tables(c, pr.P, freq, nchan, st, host);
+  pr.n = n;
+  if (n == 0) return;
+  sort_items(c, uvw, n, pr.tb, pr.k, pr.P, st, &pr.keys, &pr.perm);
+  const int64_t nkeys = (int64_t)pr.P.nplanes * pr.P.ntx * pr.P.nty;
+  pr.bstart = (uint32_t*)ws(c, S_BSTART, nkeys * 4);
+  pr.bend = (uint32_t*)ws(c, S_BEND, nkeys * 4);
+  k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(pr.bstart, nkeys, 0u);
+  k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(pr.bend, nkeys, 0u);
+  LAUNCH_CHECK();
+  {
+  PROF("bounds");
+  k_bounds<<<grid1(n), kBlock, 0, st>>>(pr.keys, n, pr.bstart, pr.bend);
+  }
+  LAUNCH_CHECK();
+  pr.rec_a = (uint32_t*)ws(c, S_RECA, n * 4);
+  pr.rec_f = ws(c, S_RECF, n * (prec == HIPNG_FP64 ? sizeof(rec3<double>) : sizeof(rec3<float>)));
+}
+
+template <typename R>
+void launch_degrid(const KP& k, int p, const Prepared& pr, const cplx<R>* plane, cplx<R>* accs,
+                   hipStream_t st) {
+  const int L = k.T + k.W - 1;
+  const size_t sm = (size_t)k.nb * L * L * sizeof(cplx<R>);
+  const unsigned nt = k.ntiles;
+  const auto* rf = (const rec3<R>*)pr.rec_f;
+#define HIPNG_DG(WW)                                                                     \
+  case WW:                                                                               \
+    k_degrid<R, WW><<<nt, kBlock, sm, st>>>(k, p, pr.bstart, pr.bend, pr.rec_a, rf, plane, \
+                                            accs);                                       \
+    break;
+  switch (k.W) {
+    HIPNG_DG(4) HIPNG_DG(5) HIPNG_DG(6) HIPNG_DG(7) HIPNG_DG(8) HIPNG_DG(9) HIPNG_DG(10)
+    HIPNG_DG(11) HIPNG_DG(12) HIPNG_DG(13) HIPNG_DG(14) HIPNG_DG(15) HIPNG_DG(16)
+    default: HIPNG_THROW(HIPNG_EINVAL, "unsupported kernel support %d", k.W);
+  }
+#undef HIPNG_DG
+  LAUNCH_CHECK();
+}
+
+void set_lds_limits() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  // dynamic LDS up to 64 KiB for the tile kernels (plan keeps tiles within it)
+  (void)hipFuncSetAttribute((const void*)k_grid<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipFuncSetAttribute((const void*)k_grid<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipGetLastError();
+}
+
+// R: kernel/grid real type; A: image accumulator type; F32C: coordinates/phase in FP32
+template <typename R, typename A, bool F32C>
+void run_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int nchan, const void* ms,
+                  const void* wgt, int nb, int nx, int ny, double px, double py, double eps,
+                  int wstack, int prec, void* dirty, hipStream_t st, hipng_plan* plan_out) {
+  DevCtx& c = ctx();
+  set_lds_limits();
+  std::vector<double> host;
+  Prepared pr;
+  prepare(c, uvw, freq, nrow, nchan, nb, nx, ny, px, py, eps, wstack, prec, st, host, pr);
+  if (plan_out) *plan_out = pr.P;
+  const KP& k = pr.k;
+  const hipng_plan& P = pr.P;
+  using O = typename std::conditional<std::is_same<R, double>::value, double, float>::type;
+  const int64_t nimg = (int64_t)nb * nx * ny;
+  if (pr.n == 0) {
+    k_zero<O><<<gridcap(nimg), kBlock, 0, st>>>((O*)dirty, nimg);
+    LAUNCH_CHECK();
+    return;
+  }
+  if (!ms) HIPNG_THROW(HIPNG_EINVAL, "ms must not be NULL");
+  auto* recv = (cplx<R>*)ws(c, S_RECV, pr.n * nb * sizeof(cplx<R>));
+  {
+  PROF("records");
+  k_records<R, F32C, true><<<grid1(pr.n), kBlock, 0, st>>>(
+      uvw, pr.perm, pr.n, pr.tb.sx, pr.tb.sy, pr.tb.sw, pr.tb.pn, k, (const cplx<R>*)ms,
+      (const R*)wgt, pr.rec_a, (rec3<R>*)pr.rec_f, recv);
+  LAUNCH_CHECK();
+  }
+  auto* acc = (A*)ws(c, S_ACCIMG, nimg * sizeof(A));
+  k_zero<A><<<gridcap(nimg), kBlock, 0, st>>>(acc, nimg);
+  LAUNCH_CHECK();
+  auto* plane = (cplx<R>*)ws(c, S_PLANE, (size_t)nb * P.gx * P.gy * sizeof(cplx<R>));
+  rocfft_plan fp = fft_plan(c, P.gx, P.gy, nb, prec, 0);
+  const size_t sm = (size_t)nb * P.tile * P.tile * sizeof(cplx<R>);
+  const dim3 ig(grid1(ny), nx);
+  for (int p = 0; p < P.nplanes; ++p) {
+    {
+    PROF("grid");
+    k_grid<R><<<k.ntiles, kBlock, sm, st>>>(k, p, pr.bstart, pr.bend, pr.rec_a,
+                                            (const rec3<R>*)pr.rec_f, recv, plane);
+    LAUNCH_CHECK();
+    }
+    {
+    PROF("fft_c2c_inverse");
+    fft_exec(c, fp, plane, st);
+    }
+    const double wp = P.w0 + p * P.dw;
+    PROF("screen_acc");
+    k_screen_acc<R, A, F32C><<<ig, kBlock, 0, st>>>(k, wp, plane, acc);
+    LAUNCH_CHECK();
+  }
+  PROF("finalize");
+  k_finalize<A, O><<<ig, kBlock, 0, st>>>(k, acc, (O*)dirty, pr.tb.phix, pr.tb.phiy, pr.tb.qz,
+                                           pr.tb.qc, pr.tb.nq);
+  LAUNCH_CHECK();
+}
+
+template <typename R, typename A, bool F32C>
+void run_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int nchan, const void* dirty,
+                  const void* wgt, int nb, int nx, int ny, double px, double py, double eps,
+                  int wstack, int prec, void* ms, hipStream_t st, hipng_plan* plan_out) {
+  DevCtx& c = ctx();
+  set_lds_limits();
+  std::vector<double> host;
+  Prepared pr;
+  prepare(c, uvw, freq, nrow, nchan, nb, nx, ny, px, py, eps, wstack, prec, st, host, pr);
+  if (plan_out) *plan_out = pr.P;
+  if (pr.n == 0) return;
+  if (!dirty || !ms) HIPNG_THROW(HIPNG_EINVAL, "dirty and ms must not be NULL");
+  const KP& k = pr.k;
+  const hipng_plan& P = pr.P;
+  using I = typename std::conditional<std::is_same<R, double>::value, double, float>::type;
+  {
+  PROF("records");
+  k_records<R, F32C, false><<<grid1(pr.n), kBlock, 0, st>>>(
+      uvw, pr.perm, pr.n, pr.tb.sx, pr.tb.sy, pr.tb.sw, pr.tb.pn, k, nullptr, nullptr, pr.rec_a,
+      (rec3<R>*)pr.rec_f, nullptr);
+  LAUNCH_CHECK();
+  }
+  const int64_t nimg = (int64_t)nb * nx * ny;
+  auto* dc = (A*)ws(c, S_ACCIMG, nimg * sizeof(A));
+  const dim3 ig(grid1(ny), nx);
+  {
+  PROF("prescale");
+  k_prescale<I, A><<<ig, kBlock, 0, st>>>(k, (const I*)dirty, dc, pr.tb.phix, pr.tb.phiy, pr.tb.qz,
+                                          pr.tb.qc, pr.tb.nq);
+  LAUNCH_CHECK();
+  }
+  auto* accs = (cplx<R>*)ws(c, S_RECV, pr.n * nb * sizeof(cplx<R>));
+  k_zero<cplx<R>><<<gridcap(pr.n * nb), kBlock, 0, st>>>(accs, pr.n * nb);
+  LAUNCH_CHECK();
+  auto* plane = (cplx<R>*)ws(c, S_PLANE, (size_t)nb * P.gx * P.gy * sizeof(cplx<R>));
+  rocfft_plan fp = fft_plan(c, P.gx, P.gy, nb, prec, 1);
+  const dim3 gg(grid1(P.gy), P.gx);
+  for (int p = 0; p < P.nplanes; ++p) {
+    const double wp = P.w0 + p * P.dw;
+    {
+    PROF("screen_fill");
+    k_screen_fill<R, A, F32C><<<gg, kBlock, 0, st>>>(k, wp, dc, plane);
+    LAUNCH_CHECK();
+    }
+    {
+    PROF("fft_c2c_forward");
+    fft_exec(c, fp, plane, st);
+    }
+    PROF("degrid");
+    launch_degrid<R>(k, p, pr, plane, accs, st);
+  }
+  PROF("unsort");
+  k_unsort<R, F32C><<<grid1(pr.n), kBlock, 0, st>>>(uvw, pr.perm, pr.n, pr.tb.pn, k, accs,
+                                                    (const R*)wgt, (cplx<R>*)ms);
+  LAUNCH_CHECK();
+}
+
+template <typename F>
+int32_t guarded(F&& f) {
+  try {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_err.clear();
+    f();
+    if (g_prof) prof_flush();
+    return HIPNG_OK;
+  } catch (const HipngError& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = std::string("internal error: ") + e.what();
+    return HIPNG_ERUNTIME;
+  }
+}
+
+}  // namespace
+
+// =================================================================================
+// C-ABI
+// =================================================================================
+extern "C" {
+
+int32_t hipng_version(void) { return 100; }
+
+const char* hipng_last_error(void) { return g_err.c_str(); }
+
+int32_t hipng_plan_params(double umax, double vmax, double wmin, double wmax, int32_t npix_x,
+                          int32_t npix_y, double pixsize_x, double pixsize_y, double epsilon,
+                          int32_t do_wstacking, int32_t precision, int32_t nbatch, hipng_plan* out) {
+  return guarded([&] {
+    if (!out) HIPNG_THROW(HIPNG_EINVAL, "out must not be NULL");
+    make_plan(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
+              precision, nbatch, 0, out);
+  });
+}
+
+int32_t hipng_channel_scales(const hipng_plan* plan, const double* freq, int32_t nchan, double* sx,
+                             double* sy, double* sw) {
+  return guarded([&] {
+    if (!plan || !freq || !sx || !sy || !sw) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    channel_scales(*plan, freq, nchan, sx, sy, sw, nullptr);
+  });
+}
+
+int32_t hipng_uvw_extent(const double* uvw, int64_t nrow, const double* freq, int32_t nchan,
+                         double* out, void* stream) {
+  return guarded([&] {
+    if (!uvw || !freq || !out || nchan <= 0 || nrow < 0) HIPNG_THROW(HIPNG_EINVAL, "bad argument");
+    extent(ctx(), uvw, nrow, freq, nchan, out, (hipStream_t)stream);
+  });
+}
+
+int32_t hipng_bucket_sort(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                          const hipng_plan* plan, uint32_t* keys_out, uint32_t* perm_out,
+                          void* stream) {
+  return guarded([&] {
+    if (!uvw || !freq || !plan || !keys_out || !perm_out || nchan <= 0 || nrow < 0)
+      HIPNG_THROW(HIPNG_EINVAL, "bad argument");
+    const int64_t n = nrow * (int64_t)nchan;
+    if (n == 0) return;
+    if (n >= 4294967295LL) HIPNG_THROW(HIPNG_EINVAL, "too many visibilities");
+    hipStream_t st = (hipStream_t)stream;
+    DevCtx& c = ctx();
+    std::vector<double> host;
+    const KP k = make_kp(*plan, 1, nchan);
+    Tables tb = upload_tables(c, *plan, freq, nchan, st, host);
+    uint32_t *keys, *perm;
+    sort_items(c, uvw, n, tb, k, *plan, st, &keys, &perm);
+    HIP_TRY(hipMemcpyAsync(keys_out, keys, n * 4, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(perm_out, perm, n * 4, hipMemcpyDeviceToDevice, st));
+  });
+}
+
+int32_t hipng_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                       const void* ms, const void* wgt, int32_t nbatch, int32_t npix_x,
+                       int32_t npix_y, double pixsize_x, double pixsize_y, double epsilon,
+                       int32_t do_wstacking, int32_t precision, void* dirty, void* stream,
+                       hipng_plan* plan_out) {
+  return guarded([&] {
+    if (!dirty) HIPNG_THROW(HIPNG_EINVAL, "dirty must not be NULL");
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == HIPNG_FP64)
+      run_ms2dirty<double, double, false>(uvw, freq, nrow, nchan, ms, wgt, nbatch, npix_x, npix_y,
+                                          pixsize_x, pixsize_y, epsilon, do_wstacking, precision,
+                                          dirty, st, plan_out);
+    else if (precision == HIPNG_MIXED)
+      run_ms2dirty<float, double, false>(uvw, freq, nrow, nchan, ms, wgt, nbatch, npix_x, npix_y,
+                                         pixsize_x, pixsize_y, epsilon, do_wstacking, precision,
+                                         dirty, st, plan_out);
+    else if (precision == HIPNG_FP32)
+      run_ms2dirty<float, float, true>(uvw, freq, nrow, nchan, ms, wgt, nbatch, npix_x, npix_y,
+                                       pixsize_x, pixsize_y, epsilon, do_wstacking, precision,
+                                       dirty, st, plan_out);
+    else
+      HIPNG_THROW(HIPNG_EINVAL, "unknown precision mode %d", precision);
+  });
+}
+
+int32_t hipng_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                       const void* dirty, const void* wgt, int32_t nbatch, int32_t npix_x,
+                       int32_t npix_y, double pixsize_x, double pixsize_y, double epsilon,
+                       int32_t do_wstacking, int32_t precision, void* ms, void* stream,
+                       hipng_plan* plan_out) {
+  return guarded([&] {
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == HIPNG_FP64)
+      run_dirty2ms<double, double, false>(uvw, freq, nrow, nchan, dirty, wgt, nbatch, npix_x,
+                                          npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
+                                          precision, ms, st, plan_out);
+    else if (precision == HIPNG_MIXED)
+      run_dirty2ms<float, double, false>(uvw, freq, nrow, nchan, dirty, wgt, nbatch, npix_x,
+                                         npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
+                                         precision, ms, st, plan_out);
+    else if (precision == HIPNG_FP32)
+      run_dirty2ms<float, float, true>(uvw, freq, nrow, nchan, dirty, wgt, nbatch, npix_x, npix_y,
+                                       pixsize_x, pixsize_y, epsilon, do_wstacking, precision, ms,
+                                       st, plan_out);
+    else
+      HIPNG_THROW(HIPNG_EINVAL, "unknown precision mode %d", precision);
+  });
+}
+
+int32_t hipng_dft(const double* direction_cosines, const void* flux, const double* uvw_lambda,
+                  int32_t ncomp, int64_t ntimes_baselines, int32_t nchan, int32_t npol, void* vis,
+                  void* stream) {
+  return guarded([&] {
+    if (!direction_cosines || !flux || !uvw_lambda || !vis) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    if (npol < 1 || npol > 4 || nchan < 1 || ncomp < 0 || ntimes_baselines < 0)
+      HIPNG_THROW(HIPNG_EINVAL, "bad dft sizes");
+    const int64_t nt = ntimes_baselines * nchan;
+    if (nt == 0) return;
+    hipStream_t st = (hipStream_t)stream;
+    k_dft<<<grid1(nt), kBlock, 0, st>>>(direction_cosines, (const cplx<double>*)flux, uvw_lambda,
+                                        ncomp, ntimes_baselines, nchan, npol, (cplx<double>*)vis);
+    LAUNCH_CHECK();
+  });
+}
+
+int32_t hipng_profile_enable(int32_t on) {
+  return guarded([&] {
+    prof_flush();
+    g_profacc.clear();
+    g_prof = on != 0;
+  });
+}
+
+int32_t hipng_profile_query(int32_t idx, char* name, int32_t name_len, double* total_ms,
+                            int64_t* launches) {
+  return guarded([&] {
+    if (idx < 0 || idx >= (int32_t)g_profacc.size()) HIPNG_THROW(HIPNG_EINVAL, "no profile entry %d", idx);
+    const auto& e = g_profacc[idx];
+    if (name && name_len > 0) {
+      std::strncpy(name, e.first.c_str(), name_len - 1);
+      name[name_len - 1] = 0;
+    }
+    if (total_ms) *total_ms = e.second.first;
+    if (launches) *launches = e.second.second;
+  });
+}
+
+int32_t hipng_release(void) {
+  return guarded([&] {
+    for (auto& kv : g_ctx) {
+      HIP_TRY(hipSetDevice(kv.first));
+      for (int s = 0; s < S_NSLOTS; ++s)
+        if (kv.second.buf[s]) (void)hipFree(kv.second.buf[s]);
+      for (auto& pl : kv.second.plans) rocfft_plan_destroy(pl.second);
+    }
+    g_ctx.clear();
+  });
+}
+
+}  // extern "C"

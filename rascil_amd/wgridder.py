@@ -1,0 +1,240 @@
+"""ducc0.wgridder-shaped backend (ms2dirty / dirty2ms) running on the hipng HIP library.
+
+Same calling surface as ducc0 0.24 ``wgridder.ms2dirty`` / ``wgridder.dirty2ms`` as RASCIL
+uses them (rascil/processing_components/imaging/ng.py:91-103, 232-248): positional
+uvw, freq, ms/dirty, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu, nv, epsilon,
+do_wstacking, nthreads; keyword verbosity, mask, double_precision_accumulation.
+
+Inputs may be numpy arrays (copied to the current GPU, result returned as numpy, like
+ducc0) or torch tensors already on the GPU (result returned as a torch tensor on the
+same device, computed on the current stream).
+
+Extensions: ms of shape [nrow, nchan, nbatch] (and dirty [nbatch, npix_x, npix_y])
+processes up to nbatch independent images sharing uvw/freq in one pass; `precision`
+selects "fp64" (default for complex128 input), "mixed" (default for complex64 input) or
+"fp32"; `plan_out` (a dict) receives the kernel/grid/plane parameters that were used.
+"""
+
+import ctypes
+
+import numpy
+
+from . import _lib
+
+MAX_BATCH = 4
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("hipng needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+def _precision(precision, dtype_is_double):
+    if precision is None:
+        precision = "fp64" if dtype_is_double else "mixed"
+    if precision not in _lib.PRECISIONS:
+        raise ValueError(f"unknown precision {precision!r}; use one of {list(_lib.PRECISIONS)}")
+    return precision
+
+
+def _cdtype(torch, precision):
+    return torch.complex128 if precision == "fp64" else torch.complex64
+
+
+def _rdtype(torch, precision):
+    return torch.float64 if precision == "fp64" else torch.float32
+
+
+def _freq_host(freq):
+    if hasattr(freq, "detach"):
+        freq = freq.detach().cpu().numpy()
+    return numpy.ascontiguousarray(numpy.atleast_1d(freq), dtype=numpy.float64)
+
+
+def _dev(torch, x, dtype, device):
+    if isinstance(x, numpy.ndarray) or not hasattr(x, "detach"):
+        x = torch.from_numpy(numpy.ascontiguousarray(x))
+    return x.to(device=device, dtype=dtype).contiguous()
+
+
+def _stream(torch, device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=0,
+             epsilon=1e-7, do_wstacking=False, nthreads=1, verbosity=0, mask=None,
+             double_precision_accumulation=False, precision=None, plan_out=None):
+    """Weighted visibilities -> dirty image(s); see module docstring.
+
+    Returns [npix_x, npix_y] (or [nbatch, npix_x, npix_y] for batched ms); float64 for
+    "fp64", float32 otherwise.
+    """
+    torch = _torch()
+    lib = _lib.load()
+    as_numpy = isinstance(ms, numpy.ndarray)
+    is_double = (ms.dtype == numpy.complex128) if as_numpy else (ms.dtype == torch.complex128)
+    precision = _precision(precision, is_double)
+    device = torch.device("cuda", torch.cuda.current_device()) if as_numpy or not ms.is_cuda else ms.device
+    freq_h = _freq_host(freq)
+    nchan = freq_h.shape[0]
+    uvw_d = _dev(torch, uvw, torch.float64, device)
+    nrow = uvw_d.shape[0]
+    if tuple(uvw_d.shape) != (nrow, 3):
+        raise ValueError("uvw must have shape [nrow, 3]")
+    ms_d = _dev(torch, ms, _cdtype(torch, precision), device)
+    batched = ms_d.dim() == 3
+    if not batched:
+        ms_d = ms_d.reshape(nrow, nchan, 1)
+    if tuple(ms_d.shape[:2]) != (nrow, nchan):
+        raise ValueError(f"ms shape {tuple(ms_d.shape)} does not match [nrow={nrow}, nchan={nchan}]")
+    nb_total = ms_d.shape[2]
+    wgt_d = None
+    if wgt is not None:
+        wgt_d = _dev(torch, wgt, _rdtype(torch, precision), device).reshape(nrow, nchan, -1)
+    if mask is not None:
+        m = _dev(torch, mask, _rdtype(torch, precision), device).reshape(nrow, nchan, -1)
+        wgt_d = m if wgt_d is None else wgt_d * m
+    if wgt_d is not None and wgt_d.shape[2] != nb_total:
+        wgt_d = wgt_d.expand(nrow, nchan, nb_total).contiguous()
+    out = torch.empty((nb_total, npix_x, npix_y), dtype=_rdtype(torch, precision), device=device)
+    plan = _lib.HipngPlan()
+    for b0 in range(0, nb_total, MAX_BATCH):
+        b1 = min(nb_total, b0 + MAX_BATCH)
+        msb = ms_d[:, :, b0:b1].contiguous() if nb_total > MAX_BATCH else ms_d
+        wb = None
+        if wgt_d is not None:
+            wb = wgt_d[:, :, b0:b1].contiguous() if nb_total > MAX_BATCH else wgt_d
+        with torch.cuda.device(device):
+            code = lib.hipng_ms2dirty(
+                ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(freq_h), nrow, nchan,
+                ctypes.c_void_p(msb.data_ptr()),
+                ctypes.c_void_p(wb.data_ptr()) if wb is not None else None,
+                b1 - b0, int(npix_x), int(npix_y), float(pixsize_x), float(pixsize_y),
+                float(epsilon), int(bool(do_wstacking)), _lib.PRECISIONS[precision],
+                ctypes.c_void_p(out[b0:b1].data_ptr()), _stream(torch, device), ctypes.byref(plan))
+        _lib.check(code)
+    if plan_out is not None:
+        plan_out.update(plan.as_dict())
+    if verbosity:
+        print("hipng ms2dirty plan:", plan.as_dict())
+    if not batched:
+        out = out[0]
+    if as_numpy:
+        return out.cpu().numpy()
+    return out
+
+
+def dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e-7,
+             do_wstacking=False, nthreads=1, verbosity=0, mask=None, precision=None,
+             plan_out=None):
+    """Dirty image(s) -> visibilities [nrow, nchan] (or [nrow, nchan, nbatch])."""
+    torch = _torch()
+    lib = _lib.load()
+    as_numpy = isinstance(dirty, numpy.ndarray)
+    is_double = (dirty.dtype == numpy.float64) if as_numpy else (dirty.dtype == torch.float64)
+    precision = _precision(precision, is_double)
+    device = (torch.device("cuda", torch.cuda.current_device())
+              if as_numpy or not dirty.is_cuda else dirty.device)
+    freq_h = _freq_host(freq)
+    nchan = freq_h.shape[0]
+    uvw_d = _dev(torch, uvw, torch.float64, device)
+    nrow = uvw_d.shape[0]
+    if tuple(uvw_d.shape) != (nrow, 3):
+        raise ValueError("uvw must have shape [nrow, 3]")
+    d = _dev(torch, dirty, _rdtype(torch, precision), device)
+    batched = d.dim() == 3
+    if not batched:
+        d = d.reshape(1, *d.shape)
+    nb_total, npix_x, npix_y = d.shape
+    wgt_d = None
+    if wgt is not None:
+        wgt_d = _dev(torch, wgt, _rdtype(torch, precision), device).reshape(nrow, nchan, -1)
+    if mask is not None:
+        m = _dev(torch, mask, _rdtype(torch, precision), device).reshape(nrow, nchan, -1)
+        wgt_d = m if wgt_d is None else wgt_d * m
+    if wgt_d is not None and wgt_d.shape[2] != nb_total:
+        wgt_d = wgt_d.expand(nrow, nchan, nb_total).contiguous()
+    out = torch.empty((nrow, nchan, nb_total), dtype=_cdtype(torch, precision), device=device)
+    plan = _lib.HipngPlan()
+    for b0 in range(0, nb_total, MAX_BATCH):
+        b1 = min(nb_total, b0 + MAX_BATCH)
+        chunked = nb_total > MAX_BATCH
+        ob = torch.empty((nrow, nchan, b1 - b0), dtype=out.dtype, device=device) if chunked else out
+        wb = None
+        if wgt_d is not None:
+            wb = wgt_d[:, :, b0:b1].contiguous() if chunked else wgt_d
+        db = d[b0:b1].contiguous()
+        with torch.cuda.device(device):
+            code = lib.hipng_dirty2ms(
+                ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(freq_h), nrow, nchan,
+                ctypes.c_void_p(db.data_ptr()),
+                ctypes.c_void_p(wb.data_ptr()) if wb is not None else None,
+                b1 - b0, int(npix_x), int(npix_y), float(pixsize_x), float(pixsize_y),
+                float(epsilon), int(bool(do_wstacking)), _lib.PRECISIONS[precision],
+                ctypes.c_void_p(ob.data_ptr()), _stream(torch, device), ctypes.byref(plan))
+        _lib.check(code)
+        if chunked:
+            out[:, :, b0:b1] = ob
+    if plan_out is not None:
+        plan_out.update(plan.as_dict())
+    if verbosity:
+        print("hipng dirty2ms plan:", plan.as_dict())
+    if not batched:
+        out = out[:, :, 0]
+    if as_numpy:
+        return out.cpu().numpy()
+    return out
+
+
+def plan_params(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y, epsilon,
+                do_wstacking, precision="fp64", nbatch=1):
+    """Host-only plan choice (no GPU needed); returns a dict of hipng_plan fields."""
+    lib = _lib.load()
+    plan = _lib.HipngPlan()
+    _lib.check(lib.hipng_plan_params(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y,
+                                     epsilon, int(bool(do_wstacking)), _lib.PRECISIONS[precision],
+                                     nbatch, ctypes.byref(plan)))
+    return plan
+
+
+def channel_scales(plan, freq):
+    """Per-channel coordinate scales of the bucket-sort contract (sx, sy, sw)."""
+    lib = _lib.load()
+    f = _freq_host(freq)
+    sx, sy, sw = (numpy.zeros_like(f) for _ in range(3))
+    _lib.check(lib.hipng_channel_scales(ctypes.byref(plan), _lib.dptr(f), f.shape[0],
+                                        _lib.dptr(sx), _lib.dptr(sy), _lib.dptr(sw)))
+    return sx, sy, sw
+
+
+def bucket_sort(uvw, freq, plan):
+    """GPU bucket sort by (w plane, uv tile): returns (keys, perm) as numpy uint32."""
+    torch = _torch()
+    lib = _lib.load()
+    f = _freq_host(freq)
+    device = torch.device("cuda", torch.cuda.current_device())
+    uvw_d = _dev(torch, uvw, torch.float64, device)
+    n = uvw_d.shape[0] * f.shape[0]
+    keys = torch.empty(n, dtype=torch.int32, device=device)
+    perm = torch.empty(n, dtype=torch.int32, device=device)
+    _lib.check(lib.hipng_bucket_sort(ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(f), uvw_d.shape[0],
+                                     f.shape[0], ctypes.byref(plan), ctypes.c_void_p(keys.data_ptr()),
+                                     ctypes.c_void_p(perm.data_ptr()), _stream(torch, device)))
+    torch.cuda.synchronize(device)
+    return keys.cpu().numpy().view(numpy.uint32), perm.cpu().numpy().view(numpy.uint32)
+
+
+def uvw_extent(uvw, freq):
+    """(max|u|, max|v|, min|w|, max|w|) in wavelengths, computed on the GPU."""
+    torch = _torch()
+    lib = _lib.load()
+    f = _freq_host(freq)
+    device = torch.device("cuda", torch.cuda.current_device())
+    uvw_d = _dev(torch, uvw, torch.float64, device)
+    out = numpy.zeros(4)
+    _lib.check(lib.hipng_uvw_extent(ctypes.c_void_p(uvw_d.data_ptr()), uvw_d.shape[0], _lib.dptr(f),
+                                    f.shape[0], _lib.dptr(out), _stream(torch, device)))
+    return out

@@ -1,0 +1,163 @@
+"""GPU parity of the hipng kernels through the C-ABI: bit-exact bucket sort against the
+CPU oracle sort, and ms2dirty / dirty2ms against the FP64 brute-force sums (oracle)."""
+
+import numpy
+import pytest
+
+from conftest import rel_l2, rel_max
+from oracle import explicit as ox
+from oracle import sortref
+from rascil_amd import simulation, wgridder
+
+pytestmark = pytest.mark.gpu
+
+C = 299792458.0
+FREQ2 = numpy.array([1.0e8, 1.31e8])
+
+
+def _case(uvw, nchan_freq, seed=0, nb=None):
+    rng = numpy.random.default_rng(seed)
+    nrow = uvw.shape[0]
+    nch = len(nchan_freq)
+    shape = (nrow, nch) if nb is None else (nrow, nch, nb)
+    ms = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    wgt = rng.uniform(0.2, 1.5, shape)
+    return ms, wgt
+
+
+def _extent_plan(uvw, freq, nx, ny, px, py, eps, wst, prec):
+    e = wgridder.uvw_extent(uvw, freq)
+    return wgridder.plan_params(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wst, prec)
+
+
+@pytest.mark.parametrize("wst", [False, True])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("nchan", [1, 16])
+def test_bucket_sort_bit_exact(small_obs, wst, prec, nchan):
+    uvw, bvis = small_obs
+    uvw = bvis.uvw.reshape(-1, 3)  # full C0 geometry: 148,500 rows
+    freq = numpy.linspace(1e8, 1.3e8, nchan)
+    plan = _extent_plan(uvw, freq, 512, 512, 5e-4, 5e-4, 1e-5 if prec == "fp64" else 1e-4, wst, prec)
+    keys, perm = wgridder.bucket_sort(uvw, freq, plan)
+    sx, sy, sw = wgridder.channel_scales(plan, freq)
+    rk, rp = sortref.bucket_sort(uvw, sx, sy, sw, plan.as_dict())
+    assert numpy.array_equal(keys, rk)
+    assert numpy.array_equal(perm, rp)
+
+
+def test_bucket_sort_bit_exact_production_subsample():
+    """1 % of the 512-station / 64-channel production geometry, 8192^2 image plan."""
+    uvw, freq, _ = simulation.config1_geometry(ntimes=128, nchan=64)
+    rng = numpy.random.default_rng(0)
+    rows = numpy.sort(rng.choice(uvw.shape[0], uvw.shape[0] // 100, replace=False))
+    uvw = uvw[rows]
+    plan = _extent_plan(uvw, freq, 8192, 8192, 1.1e-5, 1.1e-5, 1e-5, True, "mixed")
+    keys, perm = wgridder.bucket_sort(uvw, freq, plan)
+    sx, sy, sw = wgridder.channel_scales(plan, freq)
+    rk, rp = sortref.bucket_sort(uvw, sx, sy, sw, plan.as_dict())
+    assert numpy.array_equal(keys, rk)
+    assert numpy.array_equal(perm, rp)
+
+
+CASES = [  # precision, epsilon, tolerance on the relative L2 error
+    ("fp64", 1e-12, 1e-11),
+    ("fp64", 1e-7, 1e-7),
+    ("fp64", 1e-4, 1e-4),
+    ("mixed", 1e-5, 1e-5),
+    ("mixed", 1e-3, 1e-3),
+    ("fp32", 1e-3, 1e-3),
+]
+
+
+@pytest.mark.parametrize("prec,eps,tol", CASES)
+@pytest.mark.parametrize("wst", [False, True])
+def test_ms2dirty_vs_explicit(small_obs, prec, eps, tol, wst):
+    uvw, _ = small_obs
+    nx, ny, px, py = 96, 80, 6e-4, 7e-4
+    ms, wgt = _case(uvw, FREQ2)
+    if prec != "fp64":
+        ms, wgt = ms.astype("c8"), wgt.astype("f4")
+    plan = {}
+    d = wgridder.ms2dirty(uvw, FREQ2, ms, wgt, nx, ny, px, py, 0, 0, eps, wst,
+                          precision=prec, plan_out=plan)
+    ref = ox.explicit_ms2dirty(uvw, FREQ2, ms.astype("c16"), wgt.astype("f8"), nx, ny, px, py, wst)
+    err = rel_l2(d, ref)
+    print(f"ms2dirty {prec} eps={eps} wst={wst}: rel_l2={err:.3e} rel_max={rel_max(d, ref):.3e} "
+          f"W={plan['W']} sigma={plan['sigma']} planes={plan['nplanes']}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("prec,eps,tol", CASES)
+@pytest.mark.parametrize("wst", [False, True])
+def test_dirty2ms_vs_explicit(small_obs, prec, eps, tol, wst):
+    uvw, _ = small_obs
+    nx, ny, px, py = 96, 80, 6e-4, 7e-4
+    rng = numpy.random.default_rng(7)
+    dirty = rng.normal(size=(nx, ny))
+    wgt = rng.uniform(0.2, 1.5, (uvw.shape[0], 2))
+    if prec != "fp64":
+        dirty, wgt = dirty.astype("f4"), wgt.astype("f4")
+    plan = {}
+    ms = wgridder.dirty2ms(uvw, FREQ2, dirty, wgt, px, py, 0, 0, eps, wst, precision=prec,
+                           plan_out=plan)
+    ref = ox.explicit_dirty2ms(uvw, FREQ2, dirty.astype("f8"), wgt.astype("f8"), px, py, wst)
+    err = rel_l2(ms, ref)
+    print(f"dirty2ms {prec} eps={eps} wst={wst}: rel_l2={err:.3e} rel_max={rel_max(ms, ref):.3e} "
+          f"W={plan['W']} sigma={plan['sigma']} planes={plan['nplanes']}")
+    assert err <= tol
+
+
+def test_batch_equals_single(small_obs):
+    uvw, _ = small_obs
+    ms, wgt = _case(uvw, FREQ2, nb=3)
+    d3 = wgridder.ms2dirty(uvw, FREQ2, ms, wgt, 64, 64, 6e-4, 6e-4, 0, 0, 1e-10, True)
+    for b in range(3):
+        d1 = wgridder.ms2dirty(uvw, FREQ2, numpy.ascontiguousarray(ms[..., b]),
+                               numpy.ascontiguousarray(wgt[..., b]), 64, 64, 6e-4, 6e-4, 0, 0, 1e-10, True)
+        assert rel_l2(d3[b], d1) < 1e-13
+    img = numpy.random.default_rng(3).normal(size=(3, 64, 64))
+    v3 = wgridder.dirty2ms(uvw, FREQ2, img, None, 6e-4, 6e-4, 0, 0, 1e-10, True)
+    for b in range(3):
+        v1 = wgridder.dirty2ms(uvw, FREQ2, img[b], None, 6e-4, 6e-4, 0, 0, 1e-10, True)
+        assert rel_l2(v3[..., b], v1) < 1e-13
+
+
+@pytest.mark.parametrize("prec,eps", [("fp64", 1e-12), ("mixed", 1e-5)])
+def test_adjoint_identity_full_c0(small_obs, prec, eps):
+    """<dirty2ms(I), V>_R = <I, ms2dirty(V)> on the full 148,500-visibility C0 set at 512^2."""
+    _, bvis = small_obs
+    uvw = bvis.uvw.reshape(-1, 3)
+    rng = numpy.random.default_rng(11)
+    V = rng.normal(size=(uvw.shape[0], 1)) + 1j * rng.normal(size=(uvw.shape[0], 1))
+    I = rng.normal(size=(512, 512))
+    cd, rd = ("c16", "f8") if prec == "fp64" else ("c8", "f4")
+    a = wgridder.dirty2ms(uvw, [1e8], I.astype(rd), None, 5e-4, 5e-4, 0, 0, eps, True, precision=prec)
+    b = wgridder.ms2dirty(uvw, [1e8], V.astype(cd), None, 512, 512, 5e-4, 5e-4, 0, 0, eps, True,
+                          precision=prec)
+    lhs = numpy.vdot(a.astype("c16"), V).real
+    rhs = numpy.sum(I * b.astype("f8"))
+    scale = numpy.linalg.norm(a) * numpy.linalg.norm(V)
+    assert abs(lhs - rhs) / scale < 10 * eps
+
+
+def test_hermitian_flip_gpu(small_obs):
+    uvw, _ = small_obs
+    ms, _ = _case(uvw, FREQ2)
+    a = wgridder.ms2dirty(uvw, FREQ2, ms, None, 64, 64, 6e-4, 6e-4, 0, 0, 1e-12, True)
+    b = wgridder.ms2dirty(-uvw, FREQ2, ms.conj(), None, 64, 64, 6e-4, 6e-4, 0, 0, 1e-12, True)
+    assert rel_l2(a, b) < 1e-12
+
+
+def test_no_rows_gives_zero_image():
+    d = wgridder.ms2dirty(numpy.zeros((0, 3)), [1e8], numpy.zeros((0, 1), "c16"), None, 32, 32,
+                          1e-3, 1e-3, 0, 0, 1e-7, True)
+    assert d.shape == (32, 32) and not d.any()
+
+
+def test_invalid_arguments_raise():
+    uvw = numpy.random.default_rng(0).normal(0, 100, (10, 3))
+    with pytest.raises(ValueError):
+        wgridder.ms2dirty(uvw, [1e8], numpy.ones((10, 1), "c16"), None, 33, 32, 1e-3, 1e-3, 0, 0, 1e-7, True)
+    with pytest.raises(ValueError):
+        wgridder.dirty2ms(uvw, [1e8], numpy.ones((32, 32), "f4"), None, 1e-3, 1e-3, 0, 0, 1e-9, True,
+                          precision="mixed")

@@ -1,0 +1,156 @@
+"""CPU-only tests: C-ABI library loads and exports the header's symbols, host-side plan
+logic, data-model accessors, pol conversion vs the oracle matrices, dispatch errors."""
+
+import ctypes
+import os
+import re
+
+import numpy
+import pytest
+
+import rascil_amd
+from rascil_amd import _lib, wgridder
+from rascil_amd.data_models import (BlockVisibility, Direction, PolarisationFrame, create_image,
+                                    direction_to_lmn, get_parameter, image_is_canonical,
+                                    lmn_to_direction)
+from rascil_amd.imaging_base import normalise_sumwt, shift_vis_to_image
+from rascil_amd.polarisation import convert_pol_frame
+from oracle import explicit as ox
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_header_symbol():
+    header = open(os.path.join(ROOT, "include", "hipng.h")).read()
+    declared = set(re.findall(r"\b(hipng_[a-z0-9_]+)\s*\(", header))
+    assert declared == set(_lib.EXPORTS)
+    lib = ctypes.CDLL(_lib.LIBPATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert _lib.load().hipng_version() >= 100
+
+
+@pytest.mark.parametrize("eps,prec", [(1e-4, "mixed"), (1e-5, "mixed"), (1e-7, "fp64"),
+                                      (1e-12, "fp64"), (1e-3, "fp32")])
+@pytest.mark.parametrize("wst", [False, True])
+def test_plan_params(eps, prec, wst):
+    p = wgridder.plan_params(600.0, 550.0, 1.0, 400.0, 512, 512, 5e-4, 5e-4, eps, wst, prec)
+    assert p.gx % p.tile == 0 and p.gy % p.tile == 0
+    assert p.gx >= 1.2 * 512 and p.ntx == p.gx // p.tile and p.ntx >= 2
+    assert 4 <= p.W <= 16 and p.tile >= p.W
+    assert p.est_error <= eps
+    if wst:
+        # plane spacing keeps |dw ((n-1) - nshift)| <= 1/(2 sigma) over the image
+        r2 = 2 * (256 * 5e-4) ** 2
+        nm1 = -r2 / (numpy.sqrt(1 - r2) + 1)
+        assert p.nshift == pytest.approx(nm1 / 2)
+        assert p.dw * abs(nm1 / 2) <= 0.5 / p.sigma * (1 + 1e-12)
+        # every |w| in [wmin, wmax] gets W planes inside [0, nplanes)
+        for w in (1.0, 400.0):
+            wh = (w - p.w0) / p.dw
+            s = numpy.floor(wh - p.W / 2) + 1
+            assert 0 <= s and s + p.W <= p.nplanes
+    else:
+        assert p.nplanes == 1
+    assert p.nplanes * p.ntx * p.nty < 2 ** p.keybits + 1
+
+
+def test_plan_errors():
+    with pytest.raises(ValueError):
+        wgridder.plan_params(1, 1, 0, 1, 511, 512, 1e-3, 1e-3, 1e-5, True)  # odd size
+    with pytest.raises(ValueError):
+        wgridder.plan_params(1, 1, 0, 1, 512, 512, 1e-3, 1e-3, 1e-9, True, "mixed")  # eps too small
+    with pytest.raises(ValueError):
+        wgridder.plan_params(1, 1, 0, 1, 2048, 2048, 1e-3, 1e-3, 1e-5, True)  # beyond the sphere
+
+
+def test_pol_conversion_matches_oracle():
+    rng = numpy.random.default_rng(0)
+    v = rng.normal(size=(7, 3, 4)) + 1j * rng.normal(size=(7, 3, 4))
+    for a, b in [("linear", "stokesIQUV"), ("stokesIQUV", "linear"), ("circular", "stokesIQUV"),
+                 ("stokesIQUV", "circular"), ("linear", "stokesI"), ("stokesIQUV", "stokesI")]:
+        got = convert_pol_frame(v, PolarisationFrame(a), PolarisationFrame(b), polaxis=2)
+        exp = ox.convert_pol(v, a, b, polaxis=2)
+        numpy.testing.assert_allclose(got, exp, atol=1e-15)
+    with pytest.raises(ValueError):
+        convert_pol_frame(v, PolarisationFrame("linear"), PolarisationFrame("circular"), polaxis=2)
+
+
+def test_direction_roundtrip_and_lmn():
+    pc = Direction(0.3, -0.7)
+    d = lmn_to_direction(0.01, -0.02, pc)
+    l, m, nm1 = direction_to_lmn(d, pc)
+    assert (l, m) == pytest.approx((0.01, -0.02), abs=1e-14)
+    assert nm1 == pytest.approx(numpy.sqrt(1 - 0.01 ** 2 - 0.02 ** 2) - 1, abs=1e-14)
+
+
+def test_blockvisibility_accessors_and_image():
+    pf = PolarisationFrame("linear")
+    vis = numpy.ones((2, 3, 4, 4), "c16")
+    flags = numpy.zeros(vis.shape, int)
+    flags[0, 1] = 1
+    b = BlockVisibility(vis, numpy.ones((2, 3, 3)), [1e8, 1.1e8, 1.2e8, 1.3e8], pf,
+                        Direction(0, -0.5), flags=flags)
+    assert b.blockvisibility_acc.flagged_vis[0, 1].sum() == 0
+    assert b.blockvisibility_acc.flagged_imaging_weight[0, 0].sum() == 16
+    assert b["vis"].data is b.vis
+    uvl = b.uvw_lambda
+    assert uvl.shape == (2, 3, 4, 3)
+    assert uvl[0, 0, 1, 0] == pytest.approx(1.1e8 / 299792458.0)
+    im = create_image(64, 1e-3, Direction(0, -0.5), [1e8, 1.1e8], PolarisationFrame("stokesIQUV"))
+    assert image_is_canonical(im) and im.pixels.shape == (2, 4, 64, 64)
+    assert list(numpy.round(im.image_acc.wcs.freq_to_pix([1e8, 1.1e8])).astype(int)) == [0, 1]
+    assert im.wcs.crpix[0] == 33 and im.wcs.cdelt[0] < 0
+
+
+def test_normalise_and_shift_noop():
+    im = create_image(16, 1e-3, Direction(0, -0.5), [1e8], PolarisationFrame("stokesIQ"))
+    im.pixels[...] = 2.0
+    out = normalise_sumwt(im, numpy.array([[4.0, 0.0]]))
+    assert numpy.all(out.pixels[0, 0] == 0.5) and numpy.all(out.pixels[0, 1] == 0.0)
+    b = BlockVisibility(numpy.ones((1, 2, 1, 1), "c16"), numpy.ones((1, 2, 3)), [1e8],
+                        PolarisationFrame("stokesI"), Direction(0, -0.5))
+    assert shift_vis_to_image(b, im) is b
+
+
+def test_shift_vis_to_image_rotates_phase():
+    pc_vis = Direction(0.0, -0.5)
+    im = create_image(16, 1e-3, Direction(0.01, -0.49), [1e8], PolarisationFrame("stokesI"))
+    b = BlockVisibility(numpy.ones((1, 2, 1, 1), "c16"), numpy.array([[[10., 20., 5.], [-3., 4., 1.]]]),
+                        [1e8], PolarisationFrame("stokesI"), pc_vis)
+    s = shift_vis_to_image(b, im, inverse=False)
+    l, m, nm1 = direction_to_lmn(im.phasecentre, pc_vis)
+    ph = ox.phasor(b.uvw_lambda, l, m, nm1)
+    numpy.testing.assert_allclose(s.vis[..., 0], numpy.conj(ph), atol=1e-14)
+    assert s.phasecentre == im.phasecentre
+
+
+def test_dispatch_unknown_context():
+    from rascil_amd.imaging import invert_blockvisibility, predict_blockvisibility
+
+    with pytest.raises(ValueError):
+        predict_blockvisibility(None, None, context="nope")
+    with pytest.raises(ValueError):
+        invert_blockvisibility(None, None, context="nope")
+
+
+def test_get_parameter():
+    assert get_parameter({"epsilon": 1e-5}, "epsilon", 1e-12) == 1e-5
+    assert get_parameter({}, "epsilon", 1e-12) == 1e-12
+    assert get_parameter(None, "x", 3) == 3
+
+
+def test_product_has_no_cpu_fallback(monkeypatch):
+    """Without the library the product raises instead of computing anything."""
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIBPATH", "/nonexistent/_hipng.so")
+    with pytest.raises(_lib.HipngLibraryMissing):
+        _lib.load()
+
+
+def test_package_does_not_import_oracle():
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "rascil_amd")):
+        for f in files:
+            if f.endswith(".py"):
+                src = open(os.path.join(dirpath, f)).read()
+                assert not re.search(r"^\s*(from|import)\s+oracle", src, re.M), f
