@@ -73,11 +73,13 @@ int32_t hipng_version(void);
 const char* hipng_last_error(void);
 
 /* Host-only: choose W, beta, oversampling, grid, tile and w planes for the given uvw
- * extents (wavelengths, after the w >= 0 hermitian flip) and image. No GPU use. */
+ * extents (wavelengths, after the w >= 0 hermitian flip), image and number of
+ * visibilities nvis = nrow * nchan (the cost model weighs gridding against FFTs).
+ * The compute calls make the same choice internally. No GPU use. */
 int32_t hipng_plan_params(double umax, double vmax, double wmin, double wmax,
                           int32_t npix_x, int32_t npix_y, double pixsize_x, double pixsize_y,
                           double epsilon, int32_t do_wstacking, int32_t precision,
-                          int32_t nbatch, hipng_plan* out);
+                          int32_t nbatch, int64_t nvis, hipng_plan* out);
 
 /* Host-only: per-channel scale factors the coordinate kernel uses (bit-exact contract
  * with the CPU reference sort): sx[c] = f_c/c0 * pixsize_x * gx, sy[c] likewise,

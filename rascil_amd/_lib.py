@@ -96,7 +96,7 @@ def load():
         lib.hipng_version.restype = c_i32
         lib.hipng_last_error.restype = ctypes.c_char_p
         lib.hipng_plan_params.argtypes = [c_dbl, c_dbl, c_dbl, c_dbl, c_i32, c_i32, c_dbl, c_dbl,
-                                          c_dbl, c_i32, c_i32, c_i32, pplan]
+                                          c_dbl, c_i32, c_i32, c_i32, c_i64, pplan]
         lib.hipng_plan_params.restype = c_i32
         lib.hipng_channel_scales.argtypes = [pplan, pd, c_i32, pd, pd, pd]
         lib.hipng_channel_scales.restype = c_i32

@@ -1504,11 +1504,12 @@ const char* hipng_last_error(void) { return g_err.c_str(); }
 
 int32_t hipng_plan_params(double umax, double vmax, double wmin, double wmax, int32_t npix_x,
                           int32_t npix_y, double pixsize_x, double pixsize_y, double epsilon,
-                          int32_t do_wstacking, int32_t precision, int32_t nbatch, hipng_plan* out) {
+                          int32_t do_wstacking, int32_t precision, int32_t nbatch, int64_t nvis,
+                          hipng_plan* out) {
   return guarded([&] {
     if (!out) HIPNG_THROW(HIPNG_EINVAL, "out must not be NULL");
     make_plan(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
-              precision, nbatch, 0, out);
+              precision, nbatch, nvis, out);
   });
 }
 

@@ -190,13 +190,13 @@ def dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e
 
 
 def plan_params(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y, epsilon,
-                do_wstacking, precision="fp64", nbatch=1):
+                do_wstacking, precision="fp64", nbatch=1, nvis=0):
     """Host-only plan choice (no GPU needed); returns a dict of hipng_plan fields."""
     lib = _lib.load()
     plan = _lib.HipngPlan()
     _lib.check(lib.hipng_plan_params(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y,
                                      epsilon, int(bool(do_wstacking)), _lib.PRECISIONS[precision],
-                                     nbatch, ctypes.byref(plan)))
+                                     nbatch, int(nvis), ctypes.byref(plan)))
     return plan
 
 
