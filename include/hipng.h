@@ -118,6 +118,26 @@ int32_t hipng_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int3
                        double epsilon, int32_t do_wstacking, int32_t precision,
                        void* ms, void* stream, hipng_plan* plan_out);
 
+/* Reusable geometry ("plan once, execute many"): everything that depends only on uvw, freq,
+ * the image, epsilon, precision and nbatch (plan, bucket sort, coordinate records, FFT plans,
+ * all device buffers) is built once by hipng_geometry_create (which allocates and
+ * synchronises); hipng_geometry_ms2dirty / _dirty2ms then only launch kernels on `stream`
+ * (no allocation, no host synchronisation, so they can be captured in a hipGraph) and give the
+ * same results as hipng_ms2dirty / hipng_dirty2ms with the same arguments. The geometry keeps
+ * its own copy of uvw. No counterpart in ducc0 0.24; it serves callers that repeat
+ * invert/predict on one visibility set (major cycles, gradient descent, graph replay). */
+typedef struct hipng_geometry_s* hipng_geometry;
+int32_t hipng_geometry_create(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                              int32_t nbatch, int32_t npix_x, int32_t npix_y, double pixsize_x,
+                              double pixsize_y, double epsilon, int32_t do_wstacking,
+                              int32_t precision, void* stream, hipng_geometry* out,
+                              hipng_plan* plan_out);
+int32_t hipng_geometry_ms2dirty(hipng_geometry geom, const void* ms, const void* wgt, void* dirty,
+                                void* stream);
+int32_t hipng_geometry_dirty2ms(hipng_geometry geom, const void* dirty, const void* wgt, void* ms,
+                                void* stream);
+int32_t hipng_geometry_destroy(hipng_geometry geom);
+
 /* Device: FP64 point-source DFT, replacing dft_kernel / dft_cpu_looped
  * (rascil/processing_components/imaging/dft.py:123-178, 261-281; CUDA precedent
  * dft.py:181-258): vis[t][b][c][p] = sum_k flux[k][c][p] exp(-2 pi i (u l_k + v m_k + w (n_k-1)))

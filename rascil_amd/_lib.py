@@ -26,6 +26,10 @@ EXPORTS = (
     "hipng_bucket_sort",
     "hipng_ms2dirty",
     "hipng_dirty2ms",
+    "hipng_geometry_create",
+    "hipng_geometry_ms2dirty",
+    "hipng_geometry_dirty2ms",
+    "hipng_geometry_destroy",
     "hipng_dft",
     "hipng_profile_enable",
     "hipng_profile_query",
@@ -109,6 +113,16 @@ def load():
             fn.argtypes = [c_vp, pd, c_i64, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_dbl,
                            c_dbl, c_i32, c_i32, c_vp, c_vp, pplan]
             fn.restype = c_i32
+        lib.hipng_geometry_create.argtypes = [c_vp, pd, c_i64, c_i32, c_i32, c_i32, c_i32, c_dbl,
+                                              c_dbl, c_dbl, c_i32, c_i32, c_vp,
+                                              ctypes.POINTER(c_vp), pplan]
+        lib.hipng_geometry_create.restype = c_i32
+        lib.hipng_geometry_ms2dirty.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp]
+        lib.hipng_geometry_ms2dirty.restype = c_i32
+        lib.hipng_geometry_dirty2ms.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp]
+        lib.hipng_geometry_dirty2ms.restype = c_i32
+        lib.hipng_geometry_destroy.argtypes = [c_vp]
+        lib.hipng_geometry_destroy.restype = c_i32
         lib.hipng_dft.argtypes = [c_vp, c_vp, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp, c_vp]
         lib.hipng_dft.restype = c_i32
         lib.hipng_release.restype = c_i32

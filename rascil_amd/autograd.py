@@ -13,63 +13,51 @@ import torch
 from . import wgridder
 
 
-class _Geometry:
-    def __init__(self, uvw, freq, npix_x, npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
-                 precision):
-        self.uvw = uvw
-        self.freq = freq
-        self.npix_x, self.npix_y = npix_x, npix_y
-        self.pixsize_x, self.pixsize_y = pixsize_x, pixsize_y
-        self.epsilon = epsilon
-        self.do_wstacking = do_wstacking
-        self.precision = precision
-
-    def degrid(self, image):
-        return wgridder.dirty2ms(self.uvw, self.freq, image.contiguous(), None, self.pixsize_x,
-                                 self.pixsize_y, 0, 0, self.epsilon, self.do_wstacking,
-                                 precision=self.precision)
-
-    def grid(self, vis):
-        return wgridder.ms2dirty(self.uvw, self.freq, vis.contiguous(), None, self.npix_x,
-                                 self.npix_y, self.pixsize_x, self.pixsize_y, 0, 0, self.epsilon,
-                                 self.do_wstacking, precision=self.precision)
-
-
 class _Degrid(torch.autograd.Function):
     @staticmethod
     def forward(ctx, image, geom):
         ctx.geom = geom
-        return geom.degrid(image)
+        ctx.image_shape = image.shape
+        vis = geom.dirty2ms(image.contiguous())
+        return vis.reshape(geom.nrow, geom.nchan) if image.dim() == 2 else vis
 
     @staticmethod
     def backward(ctx, grad_vis):
-        return _Grid.apply(grad_vis, ctx.geom), None
+        return _Grid.apply(grad_vis, ctx.geom, ctx.image_shape), None
 
 
 class _Grid(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, vis, geom):
+    def forward(ctx, vis, geom, image_shape):
         ctx.geom = geom
-        return geom.grid(vis)
+        ctx.vis_shape = vis.shape
+        return geom.ms2dirty(vis.contiguous()).reshape(image_shape)
 
     @staticmethod
     def backward(ctx, grad_image):
-        # adjoint of Re(A^H v) as a map C -> R: the real image gradient goes back through A
-        return _Degrid.apply(grad_image.to(grad_image.dtype), ctx.geom), None
+        # Re(A^H v) maps C -> R; its adjoint sends the real image gradient back through A
+        return _Degrid.apply(grad_image, ctx.geom).reshape(ctx.vis_shape), None, None
 
 
-def predict(image, uvw, freq, pixsize_x, pixsize_y=None, epsilon=1e-7, do_wstacking=True,
-            precision=None):
-    """Differentiable dirty2ms: image [nx, ny] (or [nb, nx, ny]) real GPU tensor -> vis
-    [nrow, nchan] (or [nrow, nchan, nb]) complex GPU tensor. uvw is a [nrow, 3] FP64 GPU
-    tensor in metres with ducc0's sign convention; freq a host array in Hz."""
-    if pixsize_y is None:
-        pixsize_y = pixsize_x
-    if precision is None:
-        precision = "fp64" if image.dtype == torch.float64 else "mixed"
-    nx, ny = image.shape[-2], image.shape[-1]
-    geom = _Geometry(uvw, freq, nx, ny, pixsize_x, pixsize_y, epsilon, do_wstacking, precision)
-    return _Degrid.apply(image, geom)
+def predict(image, uvw=None, freq=None, pixsize_x=None, pixsize_y=None, epsilon=1e-7,
+            do_wstacking=True, precision=None, geometry=None):
+    """Differentiable dirty2ms.
+
+    image: real GPU tensor [nx, ny] (or [nb, nx, ny]); returns complex vis [nrow, nchan] (or
+    [nrow, nchan, nb]). Pass a prebuilt wgridder.Geometry to reuse its bucket sort across
+    calls (forward, backward and double backward all use it); otherwise one is built from
+    uvw [nrow, 3] (metres, ducc0 sign convention), freq (Hz) and the pixel sizes.
+    """
+    if geometry is None:
+        if pixsize_y is None:
+            pixsize_y = pixsize_x
+        if precision is None:
+            precision = "fp64" if image.dtype == torch.float64 else "mixed"
+        nb = 1 if image.dim() == 2 else image.shape[0]
+        geometry = wgridder.Geometry(uvw, freq, image.shape[-2], image.shape[-1], pixsize_x,
+                                     pixsize_y, epsilon, do_wstacking, precision, nb,
+                                     device=image.device)
+    return _Degrid.apply(image, geometry)
 
 
 def apply_gains(vis, gains, ant1, ant2):

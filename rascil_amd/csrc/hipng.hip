@@ -614,47 +614,111 @@ __device__ __forceinline__ void phase_cs<float, true>(double turns, float* c, fl
   sincospif(2.0f * t, s, c);
 }
 
-template <typename R, bool F32C, bool VAL>
+template <typename R, bool F32C>
 __global__ void k_records(const double* __restrict__ uvw, const uint32_t* __restrict__ perm,
                           int64_t n, const double* __restrict__ sx, const double* __restrict__ sy,
-                          const double* __restrict__ sw, const double* __restrict__ pn, KP k,
-                          const cplx<R>* __restrict__ ms, const R* __restrict__ wgt,
-                          uint32_t* __restrict__ rec_a, rec3<R>* __restrict__ rec_f,
-                          cplx<R>* __restrict__ rec_v) {
+                          const double* __restrict__ sw, KP k, uint32_t* __restrict__ rec_a,
+                          rec3<R>* __restrict__ rec_f) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
   const int64_t row = i / k.nchan;
   const int ch = (int)(i - row * k.nchan);
-  const double w = uvw[3 * row + 2];
-  const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], w, sx[ch], sy[ch], sw[ch], k);
+  const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch], sy[ch],
+                                   sw[ch], k);
   rec_a[j] = (uint32_t)p.ax | ((uint32_t)p.ay << 16);
   rec3<R> f;
   f.x = (R)p.fx;
   f.y = (R)p.fy;
   f.w = (R)p.fw;
   rec_f[j] = f;
-  if (VAL) {
-    R c = 1, s = 0;
-    if (k.wstack) phase_cs<R, F32C>(-fabs(w) * pn[ch], &c, &s);  // exp(-2 pi i w nshift)
-    for (int b = 0; b < k.nb; ++b) {
-      cplx<R> v = ms[(int64_t)i * k.nb + b];
-      if (p.flip) v.im = -v.im;
-      if (wgt) {
-        const R g = wgt[(int64_t)i * k.nb + b];
-        v.re *= g;
-        v.im *= g;
-      }
-      cplx<R> o;
-      o.re = v.re * c - v.im * s;
-      o.im = v.re * s + v.im * c;
-      rec_v[j * k.nb + b] = o;
+}
+
+// invert prologue: sorted values = wgt * conj?(ms) * exp(-2 pi i |w| f/c nshift)
+template <typename R, bool F32C>
+__global__ void k_values(const double* __restrict__ uvw, const uint32_t* __restrict__ perm, int64_t n,
+                         const double* __restrict__ pn, KP k, const cplx<R>* __restrict__ ms,
+                         const R* __restrict__ wgt, cplx<R>* __restrict__ rec_v) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  const int64_t row = i / k.nchan;
+  const int ch = (int)(i - row * k.nchan);
+  const double w = uvw[3 * row + 2];
+  const bool flip = w < 0.0;
+  R c = 1, s = 0;
+  if (k.wstack) phase_cs<R, F32C>(-fabs(w) * pn[ch], &c, &s);
+  for (int b = 0; b < k.nb; ++b) {
+    cplx<R> v = ms[(int64_t)i * k.nb + b];
+    if (flip) v.im = -v.im;
+    if (wgt) {
+      const R g = wgt[(int64_t)i * k.nb + b];
+      v.re *= g;
+      v.im *= g;
     }
+    cplx<R> o;
+    o.re = v.re * c - v.im * s;
+    o.im = v.re * s + v.im * c;
+    rec_v[j * k.nb + b] = o;
   }
 }
 
 // ---------------------------------------------------------------------------------
-// Gridder: output-stationary, one workgroup per (plane p, tile)
+// Work queue for the tile kernels. For plane p the candidate visibilities of a tile are
+// the concatenation of its buckets (start plane s in [p - Ww + 1, p]; with the gridder
+// also the three buckets of the left/lower neighbour tiles whose footprints reach in). The
+// list is cut into chunks of `chunk` candidates; every chunk is one task. Persistent
+// workgroups pop tasks from a device counter until the plane's tasks are exhausted.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void plane_range(const KP& k, int p, int* slo, int* shi) {
+  *slo = max(0, p - (k.Ww - 1));
+  *shi = min(p, k.nplanes - k.Ww);
+}
+
+__device__ __forceinline__ uint32_t seg_key(const KP& k, int s, int tx, int ty, int q) {
+  int stx = tx - (q >> 1), sty = ty - (q & 1);
+  if (stx < 0) stx += k.ntx;
+  if (sty < 0) sty += k.nty;
+  return (uint32_t)s * k.ntiles + (uint32_t)stx * k.nty + (uint32_t)sty;
+}
+
+// ntask[t] = ceil(candidates / chunk); counters[1] += ntask[t]
+__global__ void k_tasks(KP k, int p, int nq, const uint32_t* __restrict__ bstart,
+                        const uint32_t* __restrict__ bend, uint32_t chunk, uint32_t* __restrict__ ntask,
+                        uint32_t* __restrict__ counters) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)k.ntiles) return;
+  const int tx = (int)(t / k.nty), ty = (int)(t - (int64_t)tx * k.nty);
+  int slo, shi;
+  plane_range(k, p, &slo, &shi);
+  uint32_t cnt = 0;
+  for (int s = slo; s <= shi; ++s)
+    for (int q = 0; q < nq; ++q) {
+      const uint32_t key = seg_key(k, s, tx, ty, q);
+      cnt += bend[key] - bstart[key];
+    }
+  const uint32_t nt = (cnt + chunk - 1) / chunk;
+  ntask[t] = nt;
+  if (nt) atomicAdd(&counters[1], nt);
+}
+
+// last tile whose first task index is <= task (tiles without tasks share the next offset)
+__device__ __forceinline__ int task_tile(const uint32_t* __restrict__ task_off, int ntiles, uint32_t task) {
+  int lo = 0, hi = ntiles;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (task_off[mid] <= task) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------------------------
+// Gridder: persistent workgroups; a task accumulates one chunk of a tile's candidates into
+// a T x T LDS tile of plane p (LDS float atomics across waves, one wave per visibility
+// after a 64-wide ballot filter, kernel taps evaluated once per visibility by 2W lanes
+// and shared by shuffles) and adds the non-zero cells to the zero-filled plane with
+// global atomics.
 // ---------------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __restrict__ bstart,
@@ -662,73 +726,98 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
                                                  const uint32_t* __restrict__ rec_a,
                                                  const rec3<R>* __restrict__ rec_f,
                                                  const cplx<R>* __restrict__ rec_v,
+                                                 const uint32_t* __restrict__ task_off, uint32_t chunk,
+                                                 uint32_t* __restrict__ counters,
                                                  cplx<R>* __restrict__ plane) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   R* acc = reinterpret_cast<R*>(smem_raw);  // [nb][T][T][2]
+  __shared__ uint32_t s_task;
   const int T = k.T, TT = k.T * k.T;
-  const int tile = blockIdx.x;
-  const int tx = tile / k.nty, ty = tile - tx * k.nty;
-  for (int e = threadIdx.x; e < 2 * k.nb * TT; e += kBlock) acc[e] = 0;
-  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int W = k.W;
   const R scale = (R)2 / (R)W;
   const R beta = (R)k.beta;
-  const int slo = max(0, p - (k.Ww - 1)), shi = min(p, k.nplanes - k.Ww);
-  for (int s = slo; s <= shi; ++s) {
-    const R iw = (R)(p - s);
-    for (int q = 0; q < 4; ++q) {
-      int stx = tx - (q >> 1), sty = ty - (q & 1);
-      if (stx < 0) stx += k.ntx;
-      if (sty < 0) sty += k.nty;
-      const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)stx * k.nty + (uint32_t)sty;
-      const uint32_t st = bstart[key], en = bend[key];
-      for (uint32_t base = st + wv * 64; base < en; base += kBlock) {
-        const uint32_t j = base + lane;
-        bool ok = false;
-        int dx = 0, dy = 0;
-        if (j < en) {
-          const uint32_t A = rec_a[j];
-          dx = (int)(A & 0xffffu) - tx * T;
-          dy = (int)(A >> 16) - ty * T;
-          if (dx >= k.hgx) dx -= k.gx;
-          if (dx < -k.hgx) dx += k.gx;
-          if (dy >= k.hgy) dy -= k.gy;
-          if (dy < -k.hgy) dy += k.gy;
-          ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
-        }
-        unsigned long long m = __ballot(ok);
-        while (m) {
-          const int src = __ffsll((long long)m) - 1;
-          m &= m - 1;
-          const int dxk = __shfl(dx, src, 64), dyk = __shfl(dy, src, 64);
-          const uint32_t jk = base + src;
-          const rec3<R> f = rec_f[jk];
-          const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
-          for (int c = lane; c < W * W; c += 64) {
-            const int ix = c / W, iy = c - ix * W;
-            const int rx = dxk + ix, ry = dyk + iy;
-            if (rx >= 0 && rx < T && ry >= 0 && ry < T) {
-              const R wt = esk((f.x + (R)ix) * scale, beta) * esk((f.y + (R)iy) * scale, beta) * kw;
-              for (int b = 0; b < k.nb; ++b) {
-                const cplx<R> v = rec_v[(int64_t)jk * k.nb + b];
-                R* a = acc + 2 * (b * TT + rx * T + ry);
-                atomicAdd(a, wt * v.re);
-                atomicAdd(a + 1, wt * v.im);
+  const uint32_t total = counters[1];
+  int slo, shi;
+  plane_range(k, p, &slo, &shi);
+  const int nloop = (W * W + 63) / 64;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = atomicAdd(&counters[0], 1u);
+    for (int e = threadIdx.x; e < 2 * k.nb * TT; e += kBlock) acc[e] = 0;
+    __syncthreads();
+    const uint32_t task = s_task;
+    if (task >= total) break;
+    const int tile = task_tile(task_off, (int)k.ntiles, task);
+    const uint32_t vlo = (task - task_off[tile]) * chunk, vhi = vlo + chunk;
+    const int tx = tile / k.nty, ty = tile - tx * k.nty;
+    uint32_t r = 0;
+    for (int s = slo; s <= shi && r < vhi; ++s) {
+      const R iw = (R)(p - s);
+      for (int q = 0; q < 4 && r < vhi; ++q) {
+        const uint32_t key = seg_key(k, s, tx, ty, q);
+        const uint32_t st = bstart[key], len = bend[key] - st;
+        const uint32_t a = max(vlo, r), b = min(vhi, r + len);
+        if (a < b) {
+          const uint32_t j0 = st + (a - r), j1 = st + (b - r);
+          for (uint32_t base = j0 + wv * 64; base < j1; base += kBlock) {
+            const uint32_t j = base + lane;
+            bool ok = false;
+            int dx = 0, dy = 0;
+            if (j < j1) {
+              const uint32_t A = rec_a[j];
+              dx = (int)(A & 0xffffu) - tx * T;
+              dy = (int)(A >> 16) - ty * T;
+              if (dx >= k.hgx) dx -= k.gx;
+              if (dx < -k.hgx) dx += k.gx;
+              if (dy >= k.hgy) dy -= k.gy;
+              if (dy < -k.hgy) dy += k.gy;
+              ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
+            }
+            unsigned long long m = __ballot(ok);
+            while (m) {
+              const int src = __ffsll((long long)m) - 1;
+              m &= m - 1;
+              const int dxk = __shfl(dx, src, 64), dyk = __shfl(dy, src, 64);
+              const uint32_t jk = base + src;
+              const rec3<R> f = rec_f[jk];
+              const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
+              // lanes 0..31 hold x taps, 32..63 y taps
+              const R z = lane < 32 ? f.x + (R)lane : f.y + (R)(lane - 32);
+              const R kv = esk(z * scale, beta);
+              for (int it = 0; it < nloop; ++it) {
+                const int c = lane + 64 * it;
+                const int ix = c < W * W ? c / W : 0;
+                const int iy = c < W * W ? c - ix * W : 0;
+                const R kx = __shfl(kv, ix, 64);
+                const R ky = __shfl(kv, 32 + iy, 64);
+                const int rx = dxk + ix, ry = dyk + iy;
+                if (c < W * W && rx >= 0 && rx < T && ry >= 0 && ry < T) {
+                  const R wt = kx * ky * kw;
+                  for (int b2 = 0; b2 < k.nb; ++b2) {
+                    const cplx<R> v = rec_v[(int64_t)jk * k.nb + b2];
+                    R* ap = acc + 2 * (b2 * TT + rx * T + ry);
+                    atomicAdd(ap, wt * v.re);
+                    atomicAdd(ap + 1, wt * v.im);
+                  }
+                }
               }
             }
           }
         }
+        r += len;
       }
     }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < k.nb * TT; e += kBlock) {
-    const int b = e / TT, rem = e - b * TT, r = rem / T, c = rem - r * T;
-    cplx<R> o;
-    o.re = acc[2 * e];
-    o.im = acc[2 * e + 1];
-    plane[((int64_t)b * k.gx + tx * T + r) * k.gy + ty * T + c] = o;
+    __syncthreads();
+    for (int e = threadIdx.x; e < k.nb * TT; e += kBlock) {
+      const R re = acc[2 * e], im = acc[2 * e + 1];
+      if (re != (R)0 || im != (R)0) {
+        const int b2 = e / TT, rem = e - b2 * TT, rr = rem / T, cc = rem - rr * T;
+        cplx<R>* o = plane + ((int64_t)b2 * k.gx + tx * T + rr) * k.gy + ty * T + cc;
+        atomicAdd(&o->re, re);
+        atomicAdd(&o->im, im);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -849,8 +938,9 @@ __global__ void k_screen_fill(KP k, double wp, const A* __restrict__ dc, cplx<R>
 }
 
 // ---------------------------------------------------------------------------------
-// Degridder: one workgroup per (plane p, tile); plane tile + (W-1) halo resident in LDS;
-// one lane per visibility; separable kernel contraction in registers; no atomics.
+// Degridder: persistent workgroups; a task loads its tile of plane p with a (W-1) halo into
+// LDS and degrids one chunk of the tile's visibilities, one lane per visibility, separable
+// contraction in registers, read-modify-write of the sorted accumulator; no atomics.
 // ---------------------------------------------------------------------------------
 template <typename R, int W>
 __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p, const uint32_t* __restrict__ bstart,
@@ -858,76 +948,89 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p, const uint32_t* 
                                                    const uint32_t* __restrict__ rec_a,
                                                    const rec3<R>* __restrict__ rec_f,
                                                    const cplx<R>* __restrict__ plane,
-                                                   cplx<R>* __restrict__ accs) {
+                                                   cplx<R>* __restrict__ accs,
+                                                   const uint32_t* __restrict__ task_off, uint32_t chunk,
+                                                   uint32_t* __restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);  // [nb][L][L]
+  __shared__ uint32_t s_task;
   const int T = k.T, L = k.T + W - 1, LL = L * L;
-  const int tile = blockIdx.x;
-  const int tx = tile / k.nty, ty = tile - tx * k.nty;
-  const int slo = max(0, p - (k.Ww - 1)), shi = min(p, k.nplanes - k.Ww);
-  // skip tiles without work for this plane
-  uint32_t total = 0;
-  for (int s = slo; s <= shi; ++s) {
-    const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)tile;
-    total += bend[key] - bstart[key];
-  }
-  if (total == 0) return;
-  for (int e = threadIdx.x; e < k.nb * LL; e += kBlock) {
-    const int b = e / LL, rem = e - b * LL, r = rem / L, c = rem - r * L;
-    int gxi = tx * T + r, gyi = ty * T + c;
-    if (gxi >= k.gx) gxi -= k.gx;
-    if (gyi >= k.gy) gyi -= k.gy;
-    g[e] = plane[((int64_t)b * k.gx + gxi) * k.gy + gyi];
-  }
-  __syncthreads();
   const R scale = (R)2 / (R)W;
   const R beta = (R)k.beta;
-  for (int s = slo; s <= shi; ++s) {
-    const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)tile;
-    const uint32_t st = bstart[key], en = bend[key];
-    const R iw = (R)(p - s);
-    for (uint32_t j = st + threadIdx.x; j < en; j += kBlock) {
-      const uint32_t A = rec_a[j];
-      const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
-      const rec3<R> f = rec_f[j];
-      R kx[W], ky[W];
-#pragma unroll
-      for (int i = 0; i < W; ++i) {
-        kx[i] = esk((f.x + (R)i) * scale, beta);
-        ky[i] = esk((f.y + (R)i) * scale, beta);
+  const uint32_t total = counters[1];
+  int slo, shi;
+  plane_range(k, p, &slo, &shi);
+  int loaded = -1;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = atomicAdd(&counters[0], 1u);
+    __syncthreads();
+    const uint32_t task = s_task;
+    if (task >= total) break;
+    const int tile = task_tile(task_off, (int)k.ntiles, task);
+    const uint32_t vlo = (task - task_off[tile]) * chunk, vhi = vlo + chunk;
+    const int tx = tile / k.nty, ty = tile - tx * k.nty;
+    if (tile != loaded) {
+      for (int e = threadIdx.x; e < k.nb * LL; e += kBlock) {
+        const int b = e / LL, rem = e - b * LL, rr = rem / L, cc = rem - rr * L;
+        int gxi = tx * T + rr, gyi = ty * T + cc;
+        if (gxi >= k.gx) gxi -= k.gx;
+        if (gyi >= k.gy) gyi -= k.gy;
+        g[e] = plane[((int64_t)b * k.gx + gxi) * k.gy + gyi];
       }
-      const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
-      R sr[kMaxBatch], si[kMaxBatch];
+      loaded = tile;
+      __syncthreads();
+    }
+    uint32_t r = 0;
+    for (int s = slo; s <= shi && r < vhi; ++s) {
+      const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)tile;
+      const uint32_t st = bstart[key], len = bend[key] - st;
+      const uint32_t a = max(vlo, r), bnd = min(vhi, r + len);
+      const R iw = (R)(p - s);
+      for (uint32_t j = st + (a - r) + threadIdx.x; a < bnd && j < st + (bnd - r); j += kBlock) {
+        const uint32_t A = rec_a[j];
+        const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
+        const rec3<R> f = rec_f[j];
+        R kx[W], ky[W];
 #pragma unroll
-      for (int b = 0; b < kMaxBatch; ++b) sr[b] = si[b] = 0;
+        for (int i = 0; i < W; ++i) {
+          kx[i] = esk((f.x + (R)i) * scale, beta);
+          ky[i] = esk((f.y + (R)i) * scale, beta);
+        }
+        const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
+        R sr[kMaxBatch], si[kMaxBatch];
 #pragma unroll
-      for (int b = 0; b < kMaxBatch; ++b) {
-        if (b < k.nb) {
-          const cplx<R>* gb = g + b * LL + rx0 * L + ry0;
+        for (int b = 0; b < kMaxBatch; ++b) sr[b] = si[b] = 0;
 #pragma unroll
-          for (int ix = 0; ix < W; ++ix) {
-            R rr = 0, ri = 0;
+        for (int b = 0; b < kMaxBatch; ++b) {
+          if (b < k.nb) {
+            const cplx<R>* gb = g + b * LL + rx0 * L + ry0;
 #pragma unroll
-            for (int iy = 0; iy < W; ++iy) {
-              const cplx<R> v = gb[ix * L + iy];
-              rr += ky[iy] * v.re;
-              ri += ky[iy] * v.im;
+            for (int ix = 0; ix < W; ++ix) {
+              R rr = 0, ri = 0;
+#pragma unroll
+              for (int iy = 0; iy < W; ++iy) {
+                const cplx<R> v = gb[ix * L + iy];
+                rr += ky[iy] * v.re;
+                ri += ky[iy] * v.im;
+              }
+              sr[b] += kx[ix] * rr;
+              si[b] += kx[ix] * ri;
             }
-            sr[b] += kx[ix] * rr;
-            si[b] += kx[ix] * ri;
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < kMaxBatch; ++b) {
+          if (b < k.nb) {
+            cplx<R> o = accs[(int64_t)j * k.nb + b];
+            o.re += kw * sr[b];
+            o.im += kw * si[b];
+            accs[(int64_t)j * k.nb + b] = o;
           }
         }
       }
-#pragma unroll
-      for (int b = 0; b < kMaxBatch; ++b) {
-        if (b < k.nb) {
-          cplx<R> o = accs[(int64_t)j * k.nb + b];
-          o.re += kw * sr[b];
-          o.im += kw * si[b];
-          accs[(int64_t)j * k.nb + b] = o;
-        }
-      }
+      r += len;
     }
+    __syncthreads();
   }
 }
 
@@ -1058,7 +1161,7 @@ void prof_flush() {
 // ---------------------------------------------------------------------------------
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
-  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_NSLOTS
+  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_NSLOTS
 };
 
 struct FftKey {
@@ -1206,19 +1309,40 @@ KP make_kp(const hipng_plan& P, int nb, int nchan) {
   return k;
 }
 
-// Small per-call device tables: sx, sy, sw, pn [nchan]; phix [nx], phiy [ny]; quad z, c.
+// Buffer provider: one-shot calls borrow the per-device workspace; a persistent geometry owns
+// its buffers (allocated once at creation, so its executions allocate nothing).
+struct Buffers {
+  DevCtx* borrow = nullptr;
+  std::vector<void*> owned;
+  void* get(Slot s, size_t bytes) {
+    if (borrow) return ws(*borrow, s, bytes);
+    if (bytes == 0) bytes = 256;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      HIPNG_THROW(HIPNG_ERUNTIME, "device allocation of %zu bytes failed", bytes);
+    }
+    owned.push_back(p);
+    return p;
+  }
+  void release() {
+    for (void* p : owned) (void)hipFree(p);
+    owned.clear();
+  }
+};
+
+// Small device tables: sx, sy, sw, pn [nchan]; phix [nx], phiy [ny]; quadrature z, c.
 struct Tables {
   double *sx, *sy, *sw, *pn, *phix, *phiy, *qz, *qc;
   int nq;
 };
 
-Tables upload_tables(DevCtx& c, const hipng_plan& P, const double* freq, int nchan, hipStream_t st,
-                     std::vector<double>& host) {
+Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nchan, hipStream_t st) {
   KernelFT ft;
   ft.init(P.W, P.beta);
   const int nq = (int)ft.z.size();
   const size_t n = 4 * (size_t)nchan + P.nx + P.ny + 2 * (size_t)nq;
-  host.assign(n, 0.0);
+  std::vector<double> host(n, 0.0);
   double* h = host.data();
   channel_scales(P, freq, nchan, h, h + nchan, h + 2 * nchan, h + 3 * nchan);
   double* hx = h + 4 * nchan;
@@ -1231,10 +1355,9 @@ Tables upload_tables(DevCtx& c, const hipng_plan& P, const double* freq, int nch
     hz[q] = ft.z[q];
     hc[q] = ft.c[q];
   }
-  auto* d = (double*)ws(c, S_SMALL, n * sizeof(double));
+  auto* d = (double*)B.get(S_SMALL, n * sizeof(double));
   HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st));
-  // host buffer must stay alive until the copy has completed
-  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipStreamSynchronize(st));  // host buffer is released on return
   Tables t;
   t.sx = d;
   t.sy = d + nchan;
@@ -1248,7 +1371,7 @@ Tables upload_tables(DevCtx& c, const hipng_plan& P, const double* freq, int nch
   return t;
 }
 
-// keys + stable radix sort; returns device pointers to sorted keys and permutation
+// keys + stable radix sort in the shared workspace; returns sorted keys and permutation
 void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const KP& k,
                 const hipng_plan& P, hipStream_t st, uint32_t** keys_out, uint32_t** perm_out) {
   auto* k0 = (uint32_t*)ws(c, S_KEYS0, n * 4);
@@ -1256,12 +1379,12 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
   auto* k1 = (uint32_t*)ws(c, S_KEYS1, n * 4);
   auto* v1 = (uint32_t*)ws(c, S_VALS1, n * 4);
   {
-  PROF("keys");
-  if (P.precision == HIPNG_FP32)
-    k_keys<true><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
-  else
-    k_keys<false><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
-  LAUNCH_CHECK();
+    PROF("keys");
+    if (P.precision == HIPNG_FP32)
+      k_keys<true><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
+    else
+      k_keys<false><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
+    LAUNCH_CHECK();
   }
   const int64_t nblocks = (n + kRsItems - 1) / kRsItems;
   const int64_t m = 256 * nblocks;
@@ -1289,56 +1412,180 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
   *perm_out = v0;
 }
 
-struct Prepared {
+void set_lds_limits() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  // dynamic LDS up to 64 KiB for the tile kernels (the plan keeps tiles within it)
+  (void)hipFuncSetAttribute((const void*)k_grid<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipFuncSetAttribute((const void*)k_grid<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipGetLastError();
+}
+
+// Everything that depends only on (uvw, freq, image geometry, epsilon, precision, batch):
+// plan, device tables, sorted permutation, bucket table, sorted coordinate records and the
+// execution buffers. Built once; executed any number of times.
+struct Geom {
   hipng_plan P;
   KP k;
   Tables tb;
-  uint32_t *keys, *perm, *bstart, *bend, *rec_a;
-  void* rec_f;
-  int64_t n;
+  int64_t nrow = 0, n = 0;
+  int nchan = 0, nb = 0, prec = 0, device = 0;
+  const double* uvw = nullptr;
+  uint32_t *perm = nullptr, *bstart = nullptr, *bend = nullptr, *rec_a = nullptr;
+  void *rec_f = nullptr, *plane = nullptr, *accimg = nullptr, *recv = nullptr, *fftwork = nullptr;
+  size_t fftwork_size = 0;
+  rocfft_plan fft_inv = nullptr, fft_fwd = nullptr;
+  uint32_t *task = nullptr, *taskpart = nullptr, *counters = nullptr;
+  uint32_t chunk_grid = 1024, chunk_degrid = 1024;
+  int npersist = 1024;
+  Buffers buf;
 };
 
-void prepare(DevCtx& c, const double* uvw, const double* freq, int64_t nrow, int nchan, int nb,
-             int nx, int ny, double px, double py, double eps, int wstack, int prec, hipStream_t st,
-             std::vector<double>& host, Prepared& pr) {
-  if (!uvw || !freq) HIPNG_THROW(HIPNG_EINVAL, "uvw and freq must not be NULL");
+int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus <= 0) cus = 256;
+  }
+  return cus;
+}
+
+uint32_t pick_chunk(double visits_per_plane, int npersist) {
+  double c = visits_per_plane / (2.0 * npersist);
+  c = std::min(16384.0, std::max(256.0, c));
+  return (uint32_t)((uint64_t)(c + 255) / 256 * 256);
+}
+
+void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64_t nrow, int nchan,
+                int nb, int nx, int ny, double px, double py, double eps, int wstack, int prec,
+                hipStream_t st, bool own) {
+  if (!freq) HIPNG_THROW(HIPNG_EINVAL, "freq must not be NULL");
   if (nrow < 0 || nchan <= 0) HIPNG_THROW(HIPNG_EINVAL, "bad nrow/nchan (%lld, %d)", (long long)nrow, nchan);
+  if (nrow > 0 && !uvw) HIPNG_THROW(HIPNG_EINVAL, "uvw must not be NULL");
   const int64_t n = nrow * (int64_t)nchan;
   if (n >= 4294967295LL) HIPNG_THROW(HIPNG_EINVAL, "too many visibilities for 32-bit indices (%lld)", (long long)n);
+  set_lds_limits();
+  HIP_TRY(hipGetDevice(&g.device));
+  g.buf.borrow = own ? nullptr : &c;
   double e[4];
   extent(c, uvw, nrow, freq, nchan, e, st);
-  make_plan(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wstack, prec, nb, n, &pr.P);
-  pr.k = make_kp(pr.P, nb, nchan);
-  pr.tb = upload_tables(c, pr.P, freq, nchan, st, host);
-  pr.n = n;
+  make_plan(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wstack, prec, nb, n, &g.P);
+  g.k = make_kp(g.P, nb, nchan);
+  g.tb = upload_tables(g.buf, g.P, freq, nchan, st);
+  g.nrow = nrow;
+  g.n = n;
+  g.nchan = nchan;
+  g.nb = nb;
+  g.prec = prec;
+  const hipng_plan& P = g.P;
+  const size_t csz = prec == HIPNG_FP64 ? 16 : 8;
+  const size_t asz = prec == HIPNG_FP32 ? 4 : 8;
+  g.accimg = g.buf.get(S_ACCIMG, (size_t)nb * nx * ny * asz);
   if (n == 0) return;
-  sort_items(c, uvw, n, pr.tb, pr.k, pr.P, st, &pr.keys, &pr.perm);
-  const int64_t nkeys = (int64_t)pr.P.nplanes * pr.P.ntx * pr.P.nty;
-  pr.bstart = (uint32_t*)ws(c, S_BSTART, nkeys * 4);
-  pr.bend = (uint32_t*)ws(c, S_BEND, nkeys * 4);
-  k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(pr.bstart, nkeys, 0u);
-  k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(pr.bend, nkeys, 0u);
+  if (own) {
+    auto* u = (double*)g.buf.get(S_EXT, (size_t)nrow * 3 * sizeof(double));
+    HIP_TRY(hipMemcpyAsync(u, uvw, (size_t)nrow * 3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    g.uvw = u;
+  } else {
+    g.uvw = uvw;
+  }
+  uint32_t *keys, *perm;
+  sort_items(c, g.uvw, n, g.tb, g.k, P, st, &keys, &perm);
+  if (own) {
+    g.perm = (uint32_t*)g.buf.get(S_VALS0, n * 4);
+    HIP_TRY(hipMemcpyAsync(g.perm, perm, n * 4, hipMemcpyDeviceToDevice, st));
+  } else {
+    g.perm = perm;
+  }
+  const int64_t nkeys = (int64_t)P.nplanes * P.ntx * P.nty;
+  g.bstart = (uint32_t*)g.buf.get(S_BSTART, nkeys * 4);
+  g.bend = (uint32_t*)g.buf.get(S_BEND, nkeys * 4);
+  k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(g.bstart, nkeys, 0u);
+  k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(g.bend, nkeys, 0u);
   LAUNCH_CHECK();
   {
-  PROF("bounds");
-  k_bounds<<<grid1(n), kBlock, 0, st>>>(pr.keys, n, pr.bstart, pr.bend);
+    PROF("bounds");
+    k_bounds<<<grid1(n), kBlock, 0, st>>>(keys, n, g.bstart, g.bend);
+    LAUNCH_CHECK();
   }
+  g.rec_a = (uint32_t*)g.buf.get(S_RECA, n * 4);
+  g.rec_f = g.buf.get(S_RECF, n * (prec == HIPNG_FP64 ? sizeof(rec3<double>) : sizeof(rec3<float>)));
+  {
+    PROF("records");
+    if (prec == HIPNG_FP64)
+      k_records<double, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.k,
+                                                            g.rec_a, (rec3<double>*)g.rec_f);
+    else if (prec == HIPNG_MIXED)
+      k_records<float, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.k,
+                                                           g.rec_a, (rec3<float>*)g.rec_f);
+    else
+      k_records<float, true><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.k,
+                                                          g.rec_a, (rec3<float>*)g.rec_f);
+    LAUNCH_CHECK();
+  }
+  g.recv = g.buf.get(S_RECV, (size_t)n * nb * csz);
+  g.plane = g.buf.get(S_PLANE, (size_t)nb * P.gx * P.gy * csz);
+  g.fft_inv = fft_plan(c, P.gx, P.gy, nb, prec, 0);
+  g.fft_fwd = fft_plan(c, P.gx, P.gy, nb, prec, 1);
+  size_t w1 = 0, w2 = 0;
+  FFT_TRY(rocfft_plan_get_work_buffer_size(g.fft_inv, &w1));
+  FFT_TRY(rocfft_plan_get_work_buffer_size(g.fft_fwd, &w2));
+  g.fftwork_size = std::max(w1, w2);
+  if (g.fftwork_size) g.fftwork = g.buf.get(S_FFTWORK, g.fftwork_size);
+  const int64_t ntiles = (int64_t)P.ntx * P.nty;
+  g.task = (uint32_t*)g.buf.get(S_TASK, ntiles * 4);
+  g.taskpart = (uint32_t*)g.buf.get(S_TASKPART, ((ntiles + kRsItems - 1) / kRsItems) * 4);
+  g.counters = (uint32_t*)g.buf.get(S_CNT, 16);
+  g.npersist = device_cus() * 4;
+  const double ww = P.wstacking ? P.W : 1;
+  g.chunk_grid = pick_chunk((double)n * ww / P.nplanes * 4.0, g.npersist);
+  g.chunk_degrid = pick_chunk((double)n * ww / P.nplanes, g.npersist);
+  if (own) HIP_TRY(hipStreamSynchronize(st));
+}
+
+// per-plane task list: counters reset, per-tile task counts, exclusive scan -> task offsets
+void build_tasks(const Geom& g, int p, int nq, uint32_t chunk, hipStream_t st) {
+  const int64_t ntiles = g.k.ntiles;
+  k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, 2, 0u);
   LAUNCH_CHECK();
-  pr.rec_a = (uint32_t*)ws(c, S_RECA, n * 4);
-  pr.rec_f = ws(c, S_RECF, n * (prec == HIPNG_FP64 ? sizeof(rec3<double>) : sizeof(rec3<float>)));
+  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, p, nq, g.bstart, g.bend, chunk, g.task, g.counters);
+  LAUNCH_CHECK();
+  const int64_t npart = (ntiles + kRsItems - 1) / kRsItems;
+  k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
+  LAUNCH_CHECK();
+  k_scan_top<<<1, kBlock, 0, st>>>(g.taskpart, npart);
+  LAUNCH_CHECK();
+  k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
+  LAUNCH_CHECK();
+}
+
+void geom_fft(const Geom& g, rocfft_plan plan, hipStream_t st) {
+  rocfft_execution_info info = nullptr;
+  FFT_TRY(rocfft_execution_info_create(&info));
+  rocfft_status e = rocfft_execution_info_set_stream(info, st);
+  if (e == rocfft_status_success && g.fftwork_size)
+    e = rocfft_execution_info_set_work_buffer(info, g.fftwork, g.fftwork_size);
+  void* bufs[1] = {g.plane};
+  if (e == rocfft_status_success) e = rocfft_execute(plan, bufs, nullptr, info);
+  rocfft_execution_info_destroy(info);
+  if (e != rocfft_status_success) HIPNG_THROW(HIPNG_ERUNTIME, "rocfft failed: %d", (int)e);
 }
 
 template <typename R>
-void launch_degrid(const KP& k, int p, const Prepared& pr, const cplx<R>* plane, cplx<R>* accs,
-                   hipStream_t st) {
+void launch_degrid(const Geom& g, int p, cplx<R>* accs, hipStream_t st) {
+  const KP& k = g.k;
   const int L = k.T + k.W - 1;
   const size_t sm = (size_t)k.nb * L * L * sizeof(cplx<R>);
-  const unsigned nt = k.ntiles;
-  const auto* rf = (const rec3<R>*)pr.rec_f;
-#define HIPNG_DG(WW)                                                                     \
-  case WW:                                                                               \
-    k_degrid<R, WW><<<nt, kBlock, sm, st>>>(k, p, pr.bstart, pr.bend, pr.rec_a, rf, plane, \
-                                            accs);                                       \
+  const auto* rf = (const rec3<R>*)g.rec_f;
+  const auto* plane = (const cplx<R>*)g.plane;
+  build_tasks(g, p, 1, g.chunk_degrid, st);
+#define HIPNG_DG(WW)                                                                        \
+  case WW:                                                                                  \
+    k_degrid<R, WW><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, rf, plane, \
+                                                    accs, g.task, g.chunk_degrid, g.counters); \
     break;
   switch (k.W) {
     HIPNG_DG(4) HIPNG_DG(5) HIPNG_DG(6) HIPNG_DG(7) HIPNG_DG(8) HIPNG_DG(9) HIPNG_DG(10)
@@ -1349,62 +1596,46 @@ void launch_degrid(const KP& k, int p, const Prepared& pr, const cplx<R>* plane,
   LAUNCH_CHECK();
 }
 
-void set_lds_limits() {
-  static bool done = false;
-  if (done) return;
-  done = true;
-  // dynamic LDS up to 64 KiB for the tile kernels (plan keeps tiles within it)
-  (void)hipFuncSetAttribute((const void*)k_grid<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-  (void)hipFuncSetAttribute((const void*)k_grid<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-  (void)hipGetLastError();
-}
-
-// R: kernel/grid real type; A: image accumulator type; F32C: coordinates/phase in FP32
+// R: kernel/grid real type; A: image accumulator type; F32C: coordinates/phase in FP32.
+// Kernel launches and rocFFT only: no allocation, no host synchronisation.
 template <typename R, typename A, bool F32C>
-void run_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int nchan, const void* ms,
-                  const void* wgt, int nb, int nx, int ny, double px, double py, double eps,
-                  int wstack, int prec, void* dirty, hipStream_t st, hipng_plan* plan_out) {
-  DevCtx& c = ctx();
-  set_lds_limits();
-  std::vector<double> host;
-  Prepared pr;
-  prepare(c, uvw, freq, nrow, nchan, nb, nx, ny, px, py, eps, wstack, prec, st, host, pr);
-  if (plan_out) *plan_out = pr.P;
-  const KP& k = pr.k;
-  const hipng_plan& P = pr.P;
+void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, hipStream_t st) {
   using O = typename std::conditional<std::is_same<R, double>::value, double, float>::type;
-  const int64_t nimg = (int64_t)nb * nx * ny;
-  if (pr.n == 0) {
+  const KP& k = g.k;
+  const hipng_plan& P = g.P;
+  const int64_t nimg = (int64_t)g.nb * P.nx * P.ny;
+  if (g.n == 0) {
     k_zero<O><<<gridcap(nimg), kBlock, 0, st>>>((O*)dirty, nimg);
     LAUNCH_CHECK();
     return;
   }
   if (!ms) HIPNG_THROW(HIPNG_EINVAL, "ms must not be NULL");
-  auto* recv = (cplx<R>*)ws(c, S_RECV, pr.n * nb * sizeof(cplx<R>));
+  auto* recv = (cplx<R>*)g.recv;
   {
-  PROF("records");
-  k_records<R, F32C, true><<<grid1(pr.n), kBlock, 0, st>>>(
-      uvw, pr.perm, pr.n, pr.tb.sx, pr.tb.sy, pr.tb.sw, pr.tb.pn, k, (const cplx<R>*)ms,
-      (const R*)wgt, pr.rec_a, (rec3<R>*)pr.rec_f, recv);
-  LAUNCH_CHECK();
+    PROF("values");
+    k_values<R, F32C><<<grid1(g.n), kBlock, 0, st>>>(g.uvw, g.perm, g.n, g.tb.pn, k, (const cplx<R>*)ms,
+                                                       (const R*)wgt, recv);
+    LAUNCH_CHECK();
   }
-  auto* acc = (A*)ws(c, S_ACCIMG, nimg * sizeof(A));
+  auto* acc = (A*)g.accimg;
   k_zero<A><<<gridcap(nimg), kBlock, 0, st>>>(acc, nimg);
   LAUNCH_CHECK();
-  auto* plane = (cplx<R>*)ws(c, S_PLANE, (size_t)nb * P.gx * P.gy * sizeof(cplx<R>));
-  rocfft_plan fp = fft_plan(c, P.gx, P.gy, nb, prec, 0);
-  const size_t sm = (size_t)nb * P.tile * P.tile * sizeof(cplx<R>);
-  const dim3 ig(grid1(ny), nx);
+  auto* plane = (cplx<R>*)g.plane;
+  const size_t sm = (size_t)g.nb * P.tile * P.tile * sizeof(cplx<R>);
+  const dim3 ig(grid1(P.ny), P.nx);
   for (int p = 0; p < P.nplanes; ++p) {
     {
-    PROF("grid");
-    k_grid<R><<<k.ntiles, kBlock, sm, st>>>(k, p, pr.bstart, pr.bend, pr.rec_a,
-                                            (const rec3<R>*)pr.rec_f, recv, plane);
-    LAUNCH_CHECK();
+      PROF("grid");
+      build_tasks(g, p, 4, g.chunk_grid, st);
+      const int64_t np = (int64_t)g.nb * P.gx * P.gy;
+      k_zero<cplx<R>><<<gridcap(np), kBlock, 0, st>>>(plane, np);
+      k_grid<R><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, (const rec3<R>*)g.rec_f,
+                                                recv, g.task, g.chunk_grid, g.counters, plane);
+      LAUNCH_CHECK();
     }
     {
-    PROF("fft_c2c_inverse");
-    fft_exec(c, fp, plane, st);
+      PROF("fft_c2c_inverse");
+      geom_fft(g, g.fft_inv, st);
     }
     const double wp = P.w0 + p * P.dw;
     PROF("screen_acc");
@@ -1412,66 +1643,65 @@ void run_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int nchan
     LAUNCH_CHECK();
   }
   PROF("finalize");
-  k_finalize<A, O><<<ig, kBlock, 0, st>>>(k, acc, (O*)dirty, pr.tb.phix, pr.tb.phiy, pr.tb.qz,
-                                           pr.tb.qc, pr.tb.nq);
+  k_finalize<A, O><<<ig, kBlock, 0, st>>>(k, acc, (O*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc,
+                                           g.tb.nq);
   LAUNCH_CHECK();
 }
 
 template <typename R, typename A, bool F32C>
-void run_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int nchan, const void* dirty,
-                  const void* wgt, int nb, int nx, int ny, double px, double py, double eps,
-                  int wstack, int prec, void* ms, hipStream_t st, hipng_plan* plan_out) {
-  DevCtx& c = ctx();
-  set_lds_limits();
-  std::vector<double> host;
-  Prepared pr;
-  prepare(c, uvw, freq, nrow, nchan, nb, nx, ny, px, py, eps, wstack, prec, st, host, pr);
-  if (plan_out) *plan_out = pr.P;
-  if (pr.n == 0) return;
-  if (!dirty || !ms) HIPNG_THROW(HIPNG_EINVAL, "dirty and ms must not be NULL");
-  const KP& k = pr.k;
-  const hipng_plan& P = pr.P;
+void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, hipStream_t st) {
   using I = typename std::conditional<std::is_same<R, double>::value, double, float>::type;
+  if (g.n == 0) return;
+  if (!dirty || !ms) HIPNG_THROW(HIPNG_EINVAL, "dirty and ms must not be NULL");
+  const KP& k = g.k;
+  const hipng_plan& P = g.P;
+  auto* dc = (A*)g.accimg;
+  const dim3 ig(grid1(P.ny), P.nx);
   {
-  PROF("records");
-  k_records<R, F32C, false><<<grid1(pr.n), kBlock, 0, st>>>(
-      uvw, pr.perm, pr.n, pr.tb.sx, pr.tb.sy, pr.tb.sw, pr.tb.pn, k, nullptr, nullptr, pr.rec_a,
-      (rec3<R>*)pr.rec_f, nullptr);
-  LAUNCH_CHECK();
+    PROF("prescale");
+    k_prescale<I, A><<<ig, kBlock, 0, st>>>(k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz,
+                                            g.tb.qc, g.tb.nq);
+    LAUNCH_CHECK();
   }
-  const int64_t nimg = (int64_t)nb * nx * ny;
-  auto* dc = (A*)ws(c, S_ACCIMG, nimg * sizeof(A));
-  const dim3 ig(grid1(ny), nx);
-  {
-  PROF("prescale");
-  k_prescale<I, A><<<ig, kBlock, 0, st>>>(k, (const I*)dirty, dc, pr.tb.phix, pr.tb.phiy, pr.tb.qz,
-                                          pr.tb.qc, pr.tb.nq);
+  auto* accs = (cplx<R>*)g.recv;
+  k_zero<cplx<R>><<<gridcap(g.n * g.nb), kBlock, 0, st>>>(accs, g.n * g.nb);
   LAUNCH_CHECK();
-  }
-  auto* accs = (cplx<R>*)ws(c, S_RECV, pr.n * nb * sizeof(cplx<R>));
-  k_zero<cplx<R>><<<gridcap(pr.n * nb), kBlock, 0, st>>>(accs, pr.n * nb);
-  LAUNCH_CHECK();
-  auto* plane = (cplx<R>*)ws(c, S_PLANE, (size_t)nb * P.gx * P.gy * sizeof(cplx<R>));
-  rocfft_plan fp = fft_plan(c, P.gx, P.gy, nb, prec, 1);
+  auto* plane = (cplx<R>*)g.plane;
   const dim3 gg(grid1(P.gy), P.gx);
   for (int p = 0; p < P.nplanes; ++p) {
     const double wp = P.w0 + p * P.dw;
     {
-    PROF("screen_fill");
-    k_screen_fill<R, A, F32C><<<gg, kBlock, 0, st>>>(k, wp, dc, plane);
-    LAUNCH_CHECK();
+      PROF("screen_fill");
+      k_screen_fill<R, A, F32C><<<gg, kBlock, 0, st>>>(k, wp, dc, plane);
+      LAUNCH_CHECK();
     }
     {
-    PROF("fft_c2c_forward");
-    fft_exec(c, fp, plane, st);
+      PROF("fft_c2c_forward");
+      geom_fft(g, g.fft_fwd, st);
     }
     PROF("degrid");
-    launch_degrid<R>(k, p, pr, plane, accs, st);
+    launch_degrid<R>(g, p, accs, st);
   }
   PROF("unsort");
-  k_unsort<R, F32C><<<grid1(pr.n), kBlock, 0, st>>>(uvw, pr.perm, pr.n, pr.tb.pn, k, accs,
-                                                    (const R*)wgt, (cplx<R>*)ms);
+  k_unsort<R, F32C><<<grid1(g.n), kBlock, 0, st>>>(g.uvw, g.perm, g.n, g.tb.pn, k, accs, (const R*)wgt,
+                                                   (cplx<R>*)ms);
   LAUNCH_CHECK();
+}
+
+void exec_dispatch(const Geom& g, bool invert, const void* in, const void* wgt, void* out, hipStream_t st) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != g.device) HIPNG_THROW(HIPNG_EINVAL, "geometry was built on device %d, current device is %d", g.device, dev);
+  if (g.prec == HIPNG_FP64) {
+    if (invert) exec_ms2dirty<double, double, false>(g, in, wgt, out, st);
+    else exec_dirty2ms<double, double, false>(g, in, wgt, out, st);
+  } else if (g.prec == HIPNG_MIXED) {
+    if (invert) exec_ms2dirty<float, double, false>(g, in, wgt, out, st);
+    else exec_dirty2ms<float, double, false>(g, in, wgt, out, st);
+  } else {
+    if (invert) exec_ms2dirty<float, float, true>(g, in, wgt, out, st);
+    else exec_dirty2ms<float, float, true>(g, in, wgt, out, st);
+  }
 }
 
 template <typename F>
@@ -1533,16 +1763,17 @@ int32_t hipng_bucket_sort(const double* uvw, const double* freq, int64_t nrow, i
                           const hipng_plan* plan, uint32_t* keys_out, uint32_t* perm_out,
                           void* stream) {
   return guarded([&] {
-    if (!uvw || !freq || !plan || !keys_out || !perm_out || nchan <= 0 || nrow < 0)
-      HIPNG_THROW(HIPNG_EINVAL, "bad argument");
+    if (!freq || !plan || nchan <= 0 || nrow < 0) HIPNG_THROW(HIPNG_EINVAL, "bad argument");
     const int64_t n = nrow * (int64_t)nchan;
     if (n == 0) return;
+    if (!uvw || !keys_out || !perm_out) HIPNG_THROW(HIPNG_EINVAL, "NULL device pointer");
     if (n >= 4294967295LL) HIPNG_THROW(HIPNG_EINVAL, "too many visibilities");
     hipStream_t st = (hipStream_t)stream;
     DevCtx& c = ctx();
-    std::vector<double> host;
+    Buffers B;
+    B.borrow = &c;
     const KP k = make_kp(*plan, 1, nchan);
-    Tables tb = upload_tables(c, *plan, freq, nchan, st, host);
+    Tables tb = upload_tables(B, *plan, freq, nchan, st);
     uint32_t *keys, *perm;
     sort_items(c, uvw, n, tb, k, *plan, st, &keys, &perm);
     HIP_TRY(hipMemcpyAsync(keys_out, keys, n * 4, hipMemcpyDeviceToDevice, st));
@@ -1558,20 +1789,11 @@ int32_t hipng_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int3
   return guarded([&] {
     if (!dirty) HIPNG_THROW(HIPNG_EINVAL, "dirty must not be NULL");
     hipStream_t st = (hipStream_t)stream;
-    if (precision == HIPNG_FP64)
-      run_ms2dirty<double, double, false>(uvw, freq, nrow, nchan, ms, wgt, nbatch, npix_x, npix_y,
-                                          pixsize_x, pixsize_y, epsilon, do_wstacking, precision,
-                                          dirty, st, plan_out);
-    else if (precision == HIPNG_MIXED)
-      run_ms2dirty<float, double, false>(uvw, freq, nrow, nchan, ms, wgt, nbatch, npix_x, npix_y,
-                                         pixsize_x, pixsize_y, epsilon, do_wstacking, precision,
-                                         dirty, st, plan_out);
-    else if (precision == HIPNG_FP32)
-      run_ms2dirty<float, float, true>(uvw, freq, nrow, nchan, ms, wgt, nbatch, npix_x, npix_y,
-                                       pixsize_x, pixsize_y, epsilon, do_wstacking, precision,
-                                       dirty, st, plan_out);
-    else
-      HIPNG_THROW(HIPNG_EINVAL, "unknown precision mode %d", precision);
+    Geom g;
+    build_geom(g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
+               epsilon, do_wstacking, precision, st, false);
+    if (plan_out) *plan_out = g.P;
+    exec_dispatch(g, true, ms, wgt, dirty, st);
   });
 }
 
@@ -1582,20 +1804,62 @@ int32_t hipng_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int3
                        hipng_plan* plan_out) {
   return guarded([&] {
     hipStream_t st = (hipStream_t)stream;
-    if (precision == HIPNG_FP64)
-      run_dirty2ms<double, double, false>(uvw, freq, nrow, nchan, dirty, wgt, nbatch, npix_x,
-                                          npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
-                                          precision, ms, st, plan_out);
-    else if (precision == HIPNG_MIXED)
-      run_dirty2ms<float, double, false>(uvw, freq, nrow, nchan, dirty, wgt, nbatch, npix_x,
-                                         npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
-                                         precision, ms, st, plan_out);
-    else if (precision == HIPNG_FP32)
-      run_dirty2ms<float, float, true>(uvw, freq, nrow, nchan, dirty, wgt, nbatch, npix_x, npix_y,
-                                       pixsize_x, pixsize_y, epsilon, do_wstacking, precision, ms,
-                                       st, plan_out);
-    else
-      HIPNG_THROW(HIPNG_EINVAL, "unknown precision mode %d", precision);
+    Geom g;
+    build_geom(g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
+               epsilon, do_wstacking, precision, st, false);
+    if (plan_out) *plan_out = g.P;
+    exec_dispatch(g, false, dirty, wgt, ms, st);
+  });
+}
+
+struct hipng_geometry_s {
+  Geom g;
+};
+
+int32_t hipng_geometry_create(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                              int32_t nbatch, int32_t npix_x, int32_t npix_y, double pixsize_x,
+                              double pixsize_y, double epsilon, int32_t do_wstacking,
+                              int32_t precision, void* stream, hipng_geometry* out,
+                              hipng_plan* plan_out) {
+  return guarded([&] {
+    if (!out) HIPNG_THROW(HIPNG_EINVAL, "out must not be NULL");
+    *out = nullptr;
+    auto* h = new hipng_geometry_s();
+    try {
+      build_geom(h->g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
+                 epsilon, do_wstacking, precision, (hipStream_t)stream, true);
+    } catch (...) {
+      h->g.buf.release();
+      delete h;
+      throw;
+    }
+    if (plan_out) *plan_out = h->g.P;
+    *out = h;
+  });
+}
+
+int32_t hipng_geometry_ms2dirty(hipng_geometry geom, const void* ms, const void* wgt, void* dirty,
+                                void* stream) {
+  return guarded([&] {
+    if (!geom || !dirty) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    exec_dispatch(geom->g, true, ms, wgt, dirty, (hipStream_t)stream);
+  });
+}
+
+int32_t hipng_geometry_dirty2ms(hipng_geometry geom, const void* dirty, const void* wgt, void* ms,
+                                void* stream) {
+  return guarded([&] {
+    if (!geom) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    exec_dispatch(geom->g, false, dirty, wgt, ms, (hipStream_t)stream);
+  });
+}
+
+int32_t hipng_geometry_destroy(hipng_geometry geom) {
+  return guarded([&] {
+    if (!geom) return;
+    HIP_TRY(hipDeviceSynchronize());
+    geom->g.buf.release();
+    delete geom;
   });
 }
 

@@ -238,3 +238,86 @@ def uvw_extent(uvw, freq):
     _lib.check(lib.hipng_uvw_extent(ctypes.c_void_p(uvw_d.data_ptr()), uvw_d.shape[0], _lib.dptr(f),
                                     f.shape[0], _lib.dptr(out), _stream(torch, device)))
     return out
+
+
+class Geometry:
+    """Plan once, execute many: a fixed visibility geometry (uvw, freq), image and accuracy.
+
+    Construction runs the extent reduction, plan choice, bucket sort and coordinate records
+    once and allocates every device buffer. ms2dirty / dirty2ms then only launch kernels on
+    the current stream (no allocation, no host synchronisation), so they can be captured in
+    a torch.cuda.CUDAGraph (hipGraph) and replayed. Results equal the one-shot functions.
+    """
+
+    def __init__(self, uvw, freq, npix_x, npix_y, pixsize_x, pixsize_y, epsilon,
+                 do_wstacking=True, precision="mixed", nbatch=1, device=None):
+        torch = _torch()
+        self._lib = _lib.load()
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"unknown precision {precision!r}")
+        if not 1 <= nbatch <= MAX_BATCH:
+            raise ValueError(f"nbatch must be in [1, {MAX_BATCH}]")
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self.freq = _freq_host(freq)
+        self.nchan = self.freq.shape[0]
+        uvw_d = _dev(torch, uvw, torch.float64, self.device)
+        self.nrow = uvw_d.shape[0]
+        self.npix_x, self.npix_y, self.nbatch, self.precision = int(npix_x), int(npix_y), nbatch, precision
+        self.cdtype, self.rdtype = _cdtype(torch, precision), _rdtype(torch, precision)
+        self._handle = ctypes.c_void_p()
+        plan = _lib.HipngPlan()
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.hipng_geometry_create(
+                ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(self.freq), self.nrow, self.nchan, nbatch,
+                self.npix_x, self.npix_y, float(pixsize_x), float(pixsize_y), float(epsilon),
+                int(bool(do_wstacking)), _lib.PRECISIONS[precision], _stream(torch, self.device),
+                ctypes.byref(self._handle), ctypes.byref(plan)))
+        self.plan = plan.as_dict()
+
+    def _shape_ms(self):
+        return (self.nrow, self.nchan, self.nbatch)
+
+    def ms2dirty(self, ms, wgt=None, out=None):
+        """ms [nrow, nchan, nbatch] (or [nrow, nchan] when nbatch == 1) GPU tensor -> dirty."""
+        torch = _torch()
+        ms = ms.reshape(self._shape_ms())
+        if ms.dtype != self.cdtype or not ms.is_contiguous() or ms.device != self.device:
+            raise ValueError("ms must be a contiguous %s tensor on %s" % (self.cdtype, self.device))
+        if wgt is not None:
+            wgt = wgt.reshape(self._shape_ms())
+            if wgt.dtype != self.rdtype or not wgt.is_contiguous():
+                raise ValueError("wgt must be a contiguous %s tensor" % self.rdtype)
+        if out is None:
+            out = torch.empty((self.nbatch, self.npix_x, self.npix_y), dtype=self.rdtype, device=self.device)
+        _lib.check(self._lib.hipng_geometry_ms2dirty(
+            self._handle, ctypes.c_void_p(ms.data_ptr()),
+            ctypes.c_void_p(wgt.data_ptr()) if wgt is not None else None,
+            ctypes.c_void_p(out.data_ptr()), _stream(torch, self.device)))
+        return out
+
+    def dirty2ms(self, dirty, wgt=None, out=None):
+        """dirty [nbatch, npix_x, npix_y] (or [npix_x, npix_y]) GPU tensor -> ms [nrow, nchan, nbatch]."""
+        torch = _torch()
+        dirty = dirty.reshape(self.nbatch, self.npix_x, self.npix_y)
+        if dirty.dtype != self.rdtype or not dirty.is_contiguous() or dirty.device != self.device:
+            raise ValueError("dirty must be a contiguous %s tensor on %s" % (self.rdtype, self.device))
+        if wgt is not None:
+            wgt = wgt.reshape(self._shape_ms())
+        if out is None:
+            out = torch.empty(self._shape_ms(), dtype=self.cdtype, device=self.device)
+        _lib.check(self._lib.hipng_geometry_dirty2ms(
+            self._handle, ctypes.c_void_p(dirty.data_ptr()),
+            ctypes.c_void_p(wgt.data_ptr()) if wgt is not None else None,
+            ctypes.c_void_p(out.data_ptr()), _stream(torch, self.device)))
+        return out
+
+    def close(self):
+        if getattr(self, "_handle", None) and self._handle.value:
+            self._lib.hipng_geometry_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
