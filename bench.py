@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--nchan", type=int, default=None, help="override channel count (not the bench line)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy probe")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="also time the FP64 plain mode for this many steps")
     return ap.parse_args()
@@ -52,8 +53,8 @@ def workload(cfg, nchan_override):
         uvw, freq, _ = simulation.config1_geometry(nchan=nchan_override or 64)
         return dict(name="c1", uvw=uvw, freq=freq, npol=4, npix=8192, cell=1.1e-5, eps=1e-5,
                     desc=("512 stations (70% Gaussian core sigma 500 m + 30% log-uniform 1-40 km), "
-                          "128 HA over 4 h, dec -30, %d ch 100-164 MHz, 4 pol; 8192^2, cell 1.1e-5 rad, "
-                          "eps 1e-5, w-stacking" % len(freq)))
+                          f"128 HA over 4 h, dec -30, {len(freq)} ch 100-164 MHz, 4 pol; 8192^2, "
+                          "cell 1.1e-5 rad, eps 1e-5, w-stacking"))
     bvis = simulation.config0()
     return dict(name="c0", uvw=bvis.uvw.reshape(-1, 3), freq=bvis.frequency, npol=1, npix=512,
                 cell=5e-4, eps=1e-7,
@@ -81,6 +82,61 @@ def algorithmic_bytes(name, plan, nitems, nb, csz):
     return None
 
 
+def accuracy_probe(wl, prec, nsrc=20, frac=0.01, seed=7):
+    """Max / RMS relative error of predict and invert at the workload's plan, on a row
+    subsample, against FP64 direct sums: predict vs the FP64 HIP DFT of pixel-centred
+    sources (flux / n, the w-stacking convention); invert at the source pixels vs the FP64
+    adjoint sum evaluated with torch on the GPU."""
+    import numpy
+    import torch
+
+    from rascil_amd import wgridder
+    from rascil_amd.dft import dft_kernel
+
+    rng = numpy.random.default_rng(seed)
+    uvw = wl["uvw"]
+    rows = numpy.sort(rng.choice(uvw.shape[0], max(1, int(uvw.shape[0] * frac)), replace=False))
+    su = numpy.ascontiguousarray(uvw[rows])
+    freq, N, cell, eps = wl["freq"], wl["npix"], wl["cell"], wl["eps"]
+    rdt = numpy.float64 if prec == "fp64" else numpy.float32
+    model = numpy.zeros((N, N), rdt)
+    ix = rng.integers(N // 8, N - N // 8, nsrc)
+    iy = rng.integers(N // 8, N - N // 8, nsrc)
+    flux = rng.uniform(0.1, 1.0, nsrc)
+    model[ix, iy] = flux
+    # ducc0 convention (the caller negates u and w): x = (ix - N/2) cell -> l = -x, m = y
+    x = (ix - N // 2) * cell
+    y = (iy - N // 2) * cell
+    n = numpy.sqrt(1 - x * x - y * y)
+    ruvw = su * numpy.array([-1.0, 1.0, -1.0])
+    uvl = (ruvw[:, None, :] * (numpy.asarray(freq) / 299792458.0)[None, :, None])[None]
+    dc = numpy.stack([-x, y, n - 1], axis=1)
+    ref = dft_kernel(dc, (flux / n)[:, None, None] * numpy.ones((1, len(freq), 1)), uvl)[0, :, :, 0]
+    t = time.time()
+    vis = wgridder.dirty2ms(su, freq, model, None, cell, cell, 0, 0, eps, True, precision=prec)
+    err = numpy.abs(vis - ref)
+    out = {"rows": len(rows), "nvis": int(ref.size), "sources": nsrc,
+           "predict": {"max_rel": float(err.max() / numpy.abs(ref).max()),
+                       "rms_rel": float(numpy.sqrt((err ** 2).mean() / (numpy.abs(ref) ** 2).mean()))}}
+    # invert the exact visibilities; compare the source pixels with the FP64 adjoint sum
+    dirty = wgridder.ms2dirty(su, freq, ref.astype(vis.dtype), None, N, N, cell, cell, 0, 0, eps, True,
+                              precision=prec)
+    dev = torch.device("cuda", 0)
+    U = torch.tensor(uvl[0].reshape(-1, 3), device=dev)
+    V = torch.tensor(ref.reshape(-1), device=dev)
+    exact = []
+    for k in range(nsrc):
+        ph = U[:, 0] * (-x[k]) + U[:, 1] * y[k] + U[:, 2] * (n[k] - 1)  # +phase of the DFT above
+        exact.append(float((V * torch.exp(2j * numpy.pi * ph.to(torch.complex128))).real.sum()) / n[k])
+    exact = numpy.array(exact)
+    got = dirty[ix, iy].astype(numpy.float64)
+    e2 = numpy.abs(got - exact)
+    out["invert"] = {"max_rel": float(e2.max() / numpy.abs(exact).max()),
+                     "rms_rel": float(numpy.sqrt((e2 ** 2).mean() / (exact ** 2).mean()))}
+    out["seconds"] = round(time.time() - t, 1)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -106,16 +162,16 @@ def main():
     g.manual_seed(1234)
     uvw_d = torch.from_numpy(numpy.ascontiguousarray(wl["uvw"])).to(dev)
     vis_d = torch.randn((nrow, nchan, npol), dtype=cdt, device=dev, generator=g)
-    wgt_d = torch.ones((nrow, nchan, npol), dtype=rdt, device=dev)
     model_d = torch.randn((npol, N, N), dtype=rdt, device=dev, generator=g)
     torch.cuda.synchronize()
     plan_i, plan_p = {}, {}
 
-    def step(p):
-        wgridder.ms2dirty(uvw_d, wl["freq"], vis_d, wgt_d, N, N, wl["cell"], wl["cell"], 0, 0,
-                          wl["eps"], True, precision=p, plan_out=plan_i)
+    def step(p, bc=None):
+        # unit weights: wgt=None is the same sum without reading a weight array
+        wgridder.ms2dirty(uvw_d, wl["freq"], vis_d, None, N, N, wl["cell"], wl["cell"], 0, 0,
+                          wl["eps"], True, precision=p, plan_out=plan_i, batch_chunk=bc)
         wgridder.dirty2ms(uvw_d, wl["freq"], model_d, None, wl["cell"], wl["cell"], 0, 0,
-                          wl["eps"], True, precision=p, plan_out=plan_p)
+                          wl["eps"], True, precision=p, plan_out=plan_p, batch_chunk=bc)
 
     for i in range(args.warmup):
         t = time.time()
@@ -170,19 +226,26 @@ def main():
         "kernel_ms_total": breakdown,
     }
 
+    if not args.no_accuracy:
+        try:
+            out["accuracy"] = accuracy_probe(wl, prec)
+            print(f"[bench] accuracy {out['accuracy']}", file=sys.stderr, flush=True)
+        except Exception as e:
+            out["accuracy"] = {"error": repr(e)}
+
     if args.fp64_steps > 0 and prec != "fp64":
-        del vis_d, wgt_d, model_d
+        del vis_d, model_d
         torch.cuda.empty_cache()
         _lib.load().hipng_release()
         vis64 = torch.randn((nrow, nchan, npol), dtype=torch.complex128, device=dev, generator=g)
-        wgt64 = torch.ones((nrow, nchan, npol), dtype=torch.float64, device=dev)
         mod64 = torch.randn((npol, N, N), dtype=torch.float64, device=dev, generator=g)
-        vis_d, wgt_d, model_d = vis64, wgt64, mod64
-        step("fp64")
+        vis_d, model_d = vis64, mod64
+        bc = 2 if nvis > 1e9 else None  # two polarisations per pass keeps FP64 buffers in HBM
+        step("fp64", bc)
         torch.cuda.synchronize()
         t = time.time()
         for _ in range(args.fp64_steps):
-            step("fp64")
+            step("fp64", bc)
         torch.cuda.synchronize()
         ms64 = (time.time() - t) * 1e3 / args.fp64_steps
         out["fp64_plain"] = {"ms_per_step": round(ms64, 1), "value": round(nvis / (ms64 * 1e-3) / 1e6, 2),

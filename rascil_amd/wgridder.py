@@ -66,7 +66,7 @@ def _stream(torch, device):
 
 def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=0,
              epsilon=1e-7, do_wstacking=False, nthreads=1, verbosity=0, mask=None,
-             double_precision_accumulation=False, precision=None, plan_out=None):
+             double_precision_accumulation=False, precision=None, plan_out=None, batch_chunk=None):
     """Weighted visibilities -> dirty image(s); see module docstring.
 
     Returns [npix_x, npix_y] (or [nbatch, npix_x, npix_y] for batched ms); float64 for
@@ -101,12 +101,13 @@ def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=
         wgt_d = wgt_d.expand(nrow, nchan, nb_total).contiguous()
     out = torch.empty((nb_total, npix_x, npix_y), dtype=_rdtype(torch, precision), device=device)
     plan = _lib.HipngPlan()
-    for b0 in range(0, nb_total, MAX_BATCH):
-        b1 = min(nb_total, b0 + MAX_BATCH)
-        msb = ms_d[:, :, b0:b1].contiguous() if nb_total > MAX_BATCH else ms_d
+    bc = min(MAX_BATCH, batch_chunk or MAX_BATCH)
+    for b0 in range(0, nb_total, bc):
+        b1 = min(nb_total, b0 + bc)
+        msb = ms_d[:, :, b0:b1].contiguous() if nb_total > bc else ms_d
         wb = None
         if wgt_d is not None:
-            wb = wgt_d[:, :, b0:b1].contiguous() if nb_total > MAX_BATCH else wgt_d
+            wb = wgt_d[:, :, b0:b1].contiguous() if nb_total > bc else wgt_d
         with torch.cuda.device(device):
             code = lib.hipng_ms2dirty(
                 ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(freq_h), nrow, nchan,
@@ -129,7 +130,7 @@ def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=
 
 def dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e-7,
              do_wstacking=False, nthreads=1, verbosity=0, mask=None, precision=None,
-             plan_out=None):
+             plan_out=None, batch_chunk=None):
     """Dirty image(s) -> visibilities [nrow, nchan] (or [nrow, nchan, nbatch])."""
     torch = _torch()
     lib = _lib.load()
@@ -159,9 +160,10 @@ def dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e
         wgt_d = wgt_d.expand(nrow, nchan, nb_total).contiguous()
     out = torch.empty((nrow, nchan, nb_total), dtype=_cdtype(torch, precision), device=device)
     plan = _lib.HipngPlan()
-    for b0 in range(0, nb_total, MAX_BATCH):
-        b1 = min(nb_total, b0 + MAX_BATCH)
-        chunked = nb_total > MAX_BATCH
+    bc = min(MAX_BATCH, batch_chunk or MAX_BATCH)
+    for b0 in range(0, nb_total, bc):
+        b1 = min(nb_total, b0 + bc)
+        chunked = nb_total > bc
         ob = torch.empty((nrow, nchan, b1 - b0), dtype=out.dtype, device=device) if chunked else out
         wb = None
         if wgt_d is not None:
