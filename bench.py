@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy probe")
+    ap.add_argument("--study", default=None, choices=["precision"],
+                    help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="also time the FP64 plain mode for this many steps")
     return ap.parse_args()
@@ -137,8 +139,84 @@ def accuracy_probe(wl, prec, nsrc=20, frac=0.01, seed=7):
     return out
 
 
+def precision_study(seed=3):
+    """Three precision modes x epsilon on the production geometry cut to 8 channels and one
+    polarisation (134M visibilities), 200 power-law point sources up to 5 deg off-axis at
+    pixel centres of an 8192^2, 2.2e-5 rad image. Errors against FP64 direct sums: predicted
+    visibilities (all of them) vs the FP64 HIP DFT of flux / n; the dirty image at the
+    source pixels vs the FP64 adjoint sum."""
+    import numpy
+    import torch
+
+    from rascil_amd import simulation, wgridder
+    from rascil_amd.dft import dft_kernel
+
+    dev = torch.device("cuda", 0)
+    rng = numpy.random.default_rng(seed)
+    uvw, freq, _ = simulation.config1_geometry(nchan=8)
+    N, cell, nsrc = 8192, 2.2e-5, 200
+    r = numpy.radians(5.0) * numpy.sqrt(rng.uniform(0, 1, nsrc))
+    phi = rng.uniform(0, 2 * numpy.pi, nsrc)
+    ix = numpy.clip(N // 2 + numpy.round(r * numpy.cos(phi) / cell).astype(int), 0, N - 1)
+    iy = numpy.clip(N // 2 + numpy.round(r * numpy.sin(phi) / cell).astype(int), 0, N - 1)
+    flux = simulation.powerlaw_fluxes(nsrc, -1.6, 1e-3, 1.0, rng)
+    x, y = (ix - N // 2) * cell, (iy - N // 2) * cell
+    n = numpy.sqrt(1 - x * x - y * y)
+    model = numpy.zeros((N, N))
+    numpy.add.at(model, (ix, iy), flux)
+    uvl = ((uvw * numpy.array([-1.0, 1.0, -1.0]))[:, None, :] * (freq / 299792458.0)[None, :, None])[None]
+    dc = numpy.stack([-x, y, n - 1], axis=1)
+    t = time.time()
+    vtrue = dft_kernel(dc, (flux / n)[:, None, None] * numpy.ones((1, len(freq), 1)), uvl)[0, :, :, 0]
+    print(f"[study] FP64 DFT of {nsrc} sources over {vtrue.size} vis: {time.time() - t:.1f} s",
+          file=sys.stderr, flush=True)
+    U = torch.tensor(uvl[0].reshape(-1, 3), device=dev)
+    V = torch.tensor(vtrue.reshape(-1), device=dev)
+    probe = list(range(0, nsrc, 4))
+    exact = []
+    for k in probe:
+        ph = U[:, 0] * (-x[k]) + U[:, 1] * y[k] + U[:, 2] * (n[k] - 1)
+        exact.append(float((V * torch.exp(2j * numpy.pi * ph.to(torch.complex128))).real.sum()) / n[k])
+    exact = numpy.array(exact)
+    del U
+    uvw_d = torch.tensor(uvw, device=dev)
+    rows = []
+    for mode in ("fp64", "mixed", "fp32"):
+        for eps in (1e-3, 1e-4, 1e-5, 1e-6):
+            cd, rd = (torch.complex128, torch.float64) if mode == "fp64" else (torch.complex64, torch.float32)
+            ms = V.to(cd).reshape(len(uvw), len(freq))
+            img = torch.tensor(model, dtype=rd, device=dev)
+            plan = {}
+            torch.cuda.synchronize()
+            t0 = time.time()
+            dirty = wgridder.ms2dirty(uvw_d, freq, ms, None, N, N, cell, cell, 0, 0, eps, True,
+                                      precision=mode, plan_out=plan)
+            torch.cuda.synchronize()
+            t1 = time.time()
+            vis = wgridder.dirty2ms(uvw_d, freq, img, None, cell, cell, 0, 0, eps, True, precision=mode)
+            torch.cuda.synchronize()
+            t2 = time.time()
+            e = (vis.to(torch.complex128).reshape(-1) - V).abs()
+            got = dirty[ix[probe], iy[probe]].double().cpu().numpy()
+            e2 = numpy.abs(got - exact)
+            row = {"mode": mode, "eps": eps, "W": plan["W"], "sigma": plan["sigma"], "grid": plan["gx"],
+                   "nplanes": plan["nplanes"], "invert_s": round(t1 - t0, 3), "predict_s": round(t2 - t1, 3),
+                   "vis_max_rel": float(e.max() / V.abs().max()),
+                   "vis_rms_rel": float(torch.sqrt((e ** 2).mean() / (V.abs() ** 2).mean())),
+                   "dirty_max_rel": float(e2.max() / numpy.abs(exact).max()),
+                   "dirty_rms_rel": float(numpy.sqrt((e2 ** 2).mean() / (exact ** 2).mean()))}
+            print("[study]", row, file=sys.stderr, flush=True)
+            rows.append(row)
+            del dirty, vis, ms
+    return {"study": "precision", "nvis": int(vtrue.size), "image": N, "cell": cell, "sources": nsrc,
+            "rows": rows}
+
+
 def main():
     args = parse()
+    if args.study == "precision":
+        print(json.dumps(precision_study()), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
                                    "(the sharding design is described in DESIGN.md)" % args.gpus}))

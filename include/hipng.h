@@ -147,6 +147,22 @@ int32_t hipng_dft(const double* direction_cosines, const void* flux, const doubl
                   int32_t ncomp, int64_t ntimes_baselines, int32_t nchan, int32_t npol,
                   void* vis, void* stream);
 
+/* Device: imaging weights, replacing weight_visibility
+ * (rascil/processing_components/imaging/weighting.py:28-52) ->
+ * grid_blockvisibility_weight_to_griddata + griddata_blockvisibility_reweight
+ * (griddata/gridding.py:260-308, 336-424). Every (row, chan) falls on the nearest cell
+ * pu = rint(u/cell_u + npix_x/2), pv = rint(v/cell_v + npix_y/2) of an image-sized grid
+ * (cell_u = 1/(npix_x * cdelt_x), negative for RA); grid[imchan][pol][pv][pu] sums
+ * imaging_weight [nrow][nchan][npol] (flagged entries already zero). weighting 1
+ * (uniform): w <- w*w/g; 2 (robust): f2 = (5*10^-R)^2 sum(w)/sum(g^2),
+ * w <- w*(1+f2 w)/(1+f2 g); g <= 0 -> 0. imaging_weight is updated in place, grid
+ * [nimchan][npol][npix_y][npix_x] f64 is workspace/output. HIPNG_EINVAL if any visibility
+ * falls outside the grid (the reference asserts). */
+int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                                int32_t npol, const int32_t* vis_to_im, int32_t nimchan, int32_t npix_x,
+                                int32_t npix_y, double cell_u, double cell_v, int32_t weighting,
+                                double robustness, double* imaging_weight, double* grid, void* stream);
+
 /* Diagnostics (no counterpart in the reference): per-kernel-class timing with HIP
  * events recorded on the launch stream around every kernel (and rocFFT execution).
  * hipng_profile_enable(1) resets and starts, (0) stops. hipng_profile_query returns

@@ -5,6 +5,9 @@ Public surface (mirrors the reference's names):
   invert_blockvisibility, predict_blockvisibility   (imaging/imaging.py)
   wgridder.ms2dirty, wgridder.dirty2ms       (the ducc0.wgridder surface ng.py calls)
   dft_skycomponent_visibility                (imaging/dft.py)
+  weight_visibility                          (imaging/weighting.py)
+  wgridder.Geometry                          plan once / execute many (hipGraph capturable)
+  autograd.predict                           differentiable predict (gridder = adjoint)
   BlockVisibility, Image, PolarisationFrame, Skycomponent, Direction
 The compute runs in the hipng HIP library (rascil_amd/_hipng.so); nothing falls back to
 the CPU.
@@ -17,5 +20,6 @@ from .imaging import invert_blockvisibility, predict_blockvisibility
 from .imaging_base import normalise_sumwt, shift_vis_to_image
 from .ng import invert_ng, predict_ng
 from .polarisation import convert_pol_frame
+from .weighting import weight_visibility
 
 __version__ = "0.1.0"

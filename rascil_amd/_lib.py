@@ -31,6 +31,7 @@ EXPORTS = (
     "hipng_geometry_dirty2ms",
     "hipng_geometry_destroy",
     "hipng_dft",
+    "hipng_weight_visibility",
     "hipng_profile_enable",
     "hipng_profile_query",
     "hipng_release",
@@ -126,6 +127,10 @@ def load():
         lib.hipng_dft.argtypes = [c_vp, c_vp, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp, c_vp]
         lib.hipng_dft.restype = c_i32
         lib.hipng_release.restype = c_i32
+        lib.hipng_weight_visibility.argtypes = [c_vp, pd, c_i64, c_i32, c_i32,
+                                                ctypes.POINTER(ctypes.c_int32), c_i32, c_i32, c_i32,
+                                                c_dbl, c_dbl, c_i32, c_dbl, c_vp, c_vp, c_vp]
+        lib.hipng_weight_visibility.restype = c_i32
         lib.hipng_profile_enable.argtypes = [c_i32]
         lib.hipng_profile_enable.restype = c_i32
         lib.hipng_profile_query.argtypes = [c_i32, ctypes.c_char_p, c_i32, pd,

@@ -1064,6 +1064,90 @@ __global__ void k_unsort(const double* __restrict__ uvw, const uint32_t* __restr
 }
 
 // ---------------------------------------------------------------------------------
+// Imaging weights (uniform / robust): nearest-cell weight density on the image-sized uv
+// grid and the per-visibility reweighting of RASCIL's weight_visibility.
+// ---------------------------------------------------------------------------------
+struct WgtArgs {
+  int64_t nrow;
+  int nchan, npol, nimchan, nx, ny;
+  double du, dv;  // grid cell sizes in wavelengths (du negative: RA increases to the left)
+};
+
+__device__ __forceinline__ bool weight_cell(const double* __restrict__ uvw, int64_t row, double fc,
+                                            const WgtArgs& a, int* pu, int* pv) {
+  const double u = uvw[3 * row] * fc, v = uvw[3 * row + 1] * fc;
+  *pu = (int)rint(u / a.du + (double)(a.nx / 2 + 1) - 1.0);
+  *pv = (int)rint(v / a.dv + (double)(a.ny / 2 + 1) - 1.0);
+  return *pu >= 0 && *pu < a.nx && *pv >= 0 && *pv < a.ny;
+}
+
+// grid[imchan][pol][pv][pu] += imaging_weight; out-of-grid visibilities raise flag[0]
+__global__ void k_weight_grid(const double* __restrict__ uvw, const double* __restrict__ fc,
+                              const int* __restrict__ vis_to_im, WgtArgs a,
+                              const double* __restrict__ wt, double* __restrict__ grid,
+                              unsigned* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nrow * a.nchan) return;
+  const int64_t row = i / a.nchan;
+  const int ch = (int)(i - row * a.nchan);
+  int pu, pv;
+  if (!weight_cell(uvw, row, fc[ch], a, &pu, &pv)) {
+    atomicOr(flag, 1u);
+    return;
+  }
+  const int ic = vis_to_im[ch];
+  for (int p = 0; p < a.npol; ++p) {
+    const double w = wt[i * a.npol + p];
+    if (w != 0.0) atomicAdd(&grid[(((int64_t)ic * a.npol + p) * a.ny + pv) * a.nx + pu], w);
+  }
+}
+
+// mode 1: w <- w * w / g ; mode 2: w <- w * (1 + f2 w) / (1 + f2 g) ; g <= 0 -> 0
+__global__ void k_weight_apply(const double* __restrict__ uvw, const double* __restrict__ fc,
+                               const int* __restrict__ vis_to_im, WgtArgs a,
+                               const double* __restrict__ grid, int mode, double f2,
+                               double* __restrict__ wt) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nrow * a.nchan) return;
+  const int64_t row = i / a.nchan;
+  const int ch = (int)(i - row * a.nchan);
+  int pu, pv;
+  if (!weight_cell(uvw, row, fc[ch], a, &pu, &pv)) return;
+  const int ic = vis_to_im[ch];
+  for (int p = 0; p < a.npol; ++p) {
+    const double g = grid[(((int64_t)ic * a.npol + p) * a.ny + pv) * a.nx + pu];
+    double& w = wt[i * a.npol + p];
+    if (g > 0.0) w = mode == 1 ? w * (w / g) : w * ((1.0 + f2 * w) / (1.0 + f2 * g));
+    else w = 0.0;
+  }
+}
+
+// out[0] += sum(x^2), out[1] += sum(y)
+__global__ void k_sum2(const double* __restrict__ x, int64_t nx, const double* __restrict__ y,
+                       int64_t ny, double* __restrict__ out) {
+  __shared__ double s[2][kBlock];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nx; i += (int64_t)gridDim.x * kBlock)
+    a += x[i] * x[i];
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < ny; i += (int64_t)gridDim.x * kBlock)
+    b += y[i];
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + o];
+      s[1][threadIdx.x] += s[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], s[0][0]);
+    atomicAdd(&out[1], s[1][0]);
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // FP64 point-source DFT: one thread per (time*baseline, channel); components looped,
 // polarisations in registers.
 // ---------------------------------------------------------------------------------
@@ -1898,6 +1982,58 @@ int32_t hipng_profile_query(int32_t idx, char* name, int32_t name_len, double* t
     }
     if (total_ms) *total_ms = e.second.first;
     if (launches) *launches = e.second.second;
+  });
+}
+
+int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                                int32_t npol, const int32_t* vis_to_im, int32_t nimchan, int32_t npix_x,
+                                int32_t npix_y, double cell_u, double cell_v, int32_t weighting,
+                                double robustness, double* imaging_weight, double* grid, void* stream) {
+  return guarded([&] {
+    if (!freq || !vis_to_im || !imaging_weight || !grid) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    if (nrow < 0 || nchan <= 0 || npol <= 0 || nimchan <= 0 || npix_x <= 0 || npix_y <= 0)
+      HIPNG_THROW(HIPNG_EINVAL, "bad sizes");
+    if (weighting != 1 && weighting != 2) HIPNG_THROW(HIPNG_EINVAL, "weighting must be 1 (uniform) or 2 (robust)");
+    hipStream_t st = (hipStream_t)stream;
+    DevCtx& c = ctx();
+    const int64_t n = nrow * (int64_t)nchan;
+    const int64_t ng = (int64_t)nimchan * npol * npix_x * npix_y;
+    std::vector<double> hfc(nchan);
+    std::vector<int> hmap(nchan);
+    for (int i = 0; i < nchan; ++i) {
+      hfc[i] = freq[i] / kSpeedOfLight;
+      hmap[i] = vis_to_im[i];
+      if (hmap[i] < 0 || hmap[i] >= nimchan) HIPNG_THROW(HIPNG_EINVAL, "vis_to_im out of range");
+    }
+    auto* small = (char*)ws(c, S_SMALL, nchan * (sizeof(double) + sizeof(int)) + 64);
+    auto* dfc = (double*)small;
+    auto* dmap = (int*)(small + nchan * sizeof(double));
+    auto* red = (double*)ws(c, S_EXT, 4 * sizeof(double));
+    HIP_TRY(hipMemcpyAsync(dfc, hfc.data(), nchan * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dmap, hmap.data(), nchan * sizeof(int), hipMemcpyHostToDevice, st));
+    k_zero<double><<<gridcap(ng), kBlock, 0, st>>>(grid, ng);
+    k_zero<double><<<1, kBlock, 0, st>>>(red, 4);
+    LAUNCH_CHECK();
+    WgtArgs a{nrow, nchan, npol, nimchan, npix_x, npix_y, cell_u, cell_v};
+    if (n) {
+      k_weight_grid<<<grid1(n), kBlock, 0, st>>>(uvw, dfc, dmap, a, imaging_weight, grid, (unsigned*)(red + 3));
+      LAUNCH_CHECK();
+    }
+    k_sum2<<<std::min<int64_t>(1024, gridcap(std::max(ng, n * npol))), kBlock, 0, st>>>(grid, ng, imaging_weight,
+                                                                                      n * npol, red);
+    LAUNCH_CHECK();
+    double h[4];
+    HIP_TRY(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    unsigned fl;
+    std::memcpy(&fl, &h[3], sizeof(fl));
+    if (fl) HIPNG_THROW(HIPNG_EINVAL, "visibilities fall outside the uv grid: cellsize is too large");
+    // Briggs' robust parameter with RASCIL's normalisation (equation 3.15 of the thesis)
+    const double f2 = std::pow(5.0 * std::pow(10.0, -robustness), 2.0) * h[1] / h[0];
+    if (n) {
+      k_weight_apply<<<grid1(n), kBlock, 0, st>>>(uvw, dfc, dmap, a, grid, weighting, f2, imaging_weight);
+      LAUNCH_CHECK();
+    }
   });
 }
 
