@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy probe")
+    ap.add_argument("--sigma", type=float, default=None,
+                    help="force the grid oversampling factor (cost-model calibration, not the bench line)")
     ap.add_argument("--study", default=None, choices=["precision"],
                     help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
@@ -214,6 +216,8 @@ def precision_study(seed=3):
 
 def main():
     args = parse()
+    if args.sigma:
+        os.environ["HIPNG_SIGMA"] = str(args.sigma)
     if args.study == "precision":
         print(json.dumps(precision_study()), flush=True)
         return

@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -207,7 +208,11 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
   const int lds_budget = 64 * 1024;
   const size_t csz = prec == HIPNG_FP64 ? 16 : 8;
   double best_cost = 1e300;
+  // HIPNG_SIGMA restricts the search to one oversampling factor (cost-model calibration)
+  const char* force = std::getenv("HIPNG_SIGMA");
+  const double forced = force ? std::atof(force) : 0.0;
   for (int is = 0; is < kNSig; ++is) {
+    if (forced > 0.0 && std::fabs(kSigmas[is] - forced) > 1e-9) continue;
     int W = -1;
     for (int w = kWMin; w <= kWMax; ++w)
       if (kErr1D[is][w - kWMin] * safety <= eps) {
@@ -714,110 +719,161 @@ __device__ __forceinline__ int task_tile(const uint32_t* __restrict__ task_off, 
 }
 
 // ---------------------------------------------------------------------------------
-// Gridder: persistent workgroups; a task accumulates one chunk of a tile's candidates into
-// a T x T LDS tile of plane p (LDS float atomics across waves, one wave per visibility
-// after a 64-wide ballot filter, kernel taps evaluated once per visibility by 2W lanes
-// and shared by shuffles) and adds the non-zero cells to the zero-filled plane with
-// global atomics.
+// Gridder: persistent workgroups; a task grids one chunk of a tile's candidates into the
+// T x T tile of plane p. Per round, up to S candidates are staged in LDS by one thread each
+// (coalesced record loads, tile-overlap test, wave-ballot compaction, the 2W+1 kernel taps
+// evaluated once per visibility). Every thread then owns CPT = T*T/256 cells of the tile
+// and accumulates, in registers, the staged visibilities whose footprint covers them
+// (gather: no LDS atomics, deterministic order). The tile is added to the zero-filled
+// plane with one global atomic per non-zero cell (tiles can be split over tasks).
 // ---------------------------------------------------------------------------------
-template <typename R>
+constexpr int kMaxSeg = 4 * 16;
+
+template <typename R, int T>
 __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __restrict__ bstart,
                                                  const uint32_t* __restrict__ bend,
                                                  const uint32_t* __restrict__ rec_a,
                                                  const rec3<R>* __restrict__ rec_f,
                                                  const cplx<R>* __restrict__ rec_v,
                                                  const uint32_t* __restrict__ task_off, uint32_t chunk,
-                                                 uint32_t* __restrict__ counters,
+                                                 uint32_t* __restrict__ counters, int S,
                                                  cplx<R>* __restrict__ plane) {
+  constexpr int CPT = T * T / kBlock;  // cells per thread
+  constexpr int RUN = T / CPT;         // threads per tile column
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  R* acc = reinterpret_cast<R*>(smem_raw);  // [nb][T][T][2]
-  __shared__ uint32_t s_task;
-  const int T = k.T, TT = k.T * k.T;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int W = k.W;
+  // staged data: taps x [S][W], taps y [S][W], kw [S], values [S][4], packed offsets [S]
+  R* s_kx = reinterpret_cast<R*>(smem_raw);
+  R* s_ky = s_kx + S * W;
+  R* s_kw = s_ky + S * W;
+  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + S);
+  int* s_dxy = reinterpret_cast<int*>(s_v + 4 * S);
+  __shared__ uint32_t s_task;
+  __shared__ uint32_t s_segv[kMaxSeg + 1];
+  __shared__ uint32_t s_segj[kMaxSeg];
+  __shared__ float s_segw[kMaxSeg];
+  __shared__ int s_wcnt[kBlock / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int cx = t / RUN, cy0 = (t - cx * RUN) * CPT;
   const R scale = (R)2 / (R)W;
   const R beta = (R)k.beta;
   const uint32_t total = counters[1];
   int slo, shi;
   plane_range(k, p, &slo, &shi);
-  const int nloop = (W * W + 63) / 64;
+  const int nseg = (shi - slo + 1) * 4;
   for (;;) {
-    if (threadIdx.x == 0) s_task = atomicAdd(&counters[0], 1u);
-    for (int e = threadIdx.x; e < 2 * k.nb * TT; e += kBlock) acc[e] = 0;
+    if (t == 0) s_task = atomicAdd(&counters[0], 1u);
     __syncthreads();
     const uint32_t task = s_task;
     if (task >= total) break;
     const int tile = task_tile(task_off, (int)k.ntiles, task);
-    const uint32_t vlo = (task - task_off[tile]) * chunk, vhi = vlo + chunk;
+    const uint32_t vlo = (task - task_off[tile]) * chunk;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
-    uint32_t r = 0;
-    for (int s = slo; s <= shi && r < vhi; ++s) {
-      const R iw = (R)(p - s);
-      for (int q = 0; q < 4 && r < vhi; ++q) {
-        const uint32_t key = seg_key(k, s, tx, ty, q);
-        const uint32_t st = bstart[key], len = bend[key] - st;
-        const uint32_t a = max(vlo, r), b = min(vhi, r + len);
-        if (a < b) {
-          const uint32_t j0 = st + (a - r), j1 = st + (b - r);
-          for (uint32_t base = j0 + wv * 64; base < j1; base += kBlock) {
-            const uint32_t j = base + lane;
-            bool ok = false;
-            int dx = 0, dy = 0;
-            if (j < j1) {
-              const uint32_t A = rec_a[j];
-              dx = (int)(A & 0xffffu) - tx * T;
-              dy = (int)(A >> 16) - ty * T;
-              if (dx >= k.hgx) dx -= k.gx;
-              if (dx < -k.hgx) dx += k.gx;
-              if (dy >= k.hgy) dy -= k.gy;
-              if (dy < -k.hgy) dy += k.gy;
-              ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
-            }
-            unsigned long long m = __ballot(ok);
-            while (m) {
-              const int src = __ffsll((long long)m) - 1;
-              m &= m - 1;
-              const int dxk = __shfl(dx, src, 64), dyk = __shfl(dy, src, 64);
-              const uint32_t jk = base + src;
-              const rec3<R> f = rec_f[jk];
-              const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
-              // lanes 0..31 hold x taps, 32..63 y taps
-              const R z = lane < 32 ? f.x + (R)lane : f.y + (R)(lane - 32);
-              const R kv = esk(z * scale, beta);
-              for (int it = 0; it < nloop; ++it) {
-                const int c = lane + 64 * it;
-                const int ix = c < W * W ? c / W : 0;
-                const int iy = c < W * W ? c - ix * W : 0;
-                const R kx = __shfl(kv, ix, 64);
-                const R ky = __shfl(kv, 32 + iy, 64);
-                const int rx = dxk + ix, ry = dyk + iy;
-                if (c < W * W && rx >= 0 && rx < T && ry >= 0 && ry < T) {
-                  const R wt = kx * ky * kw;
-                  for (int b2 = 0; b2 < k.nb; ++b2) {
-                    const cplx<R> v = rec_v[(int64_t)jk * k.nb + b2];
-                    R* ap = acc + 2 * (b2 * TT + rx * T + ry);
-                    atomicAdd(ap, wt * v.re);
-                    atomicAdd(ap + 1, wt * v.im);
-                  }
+    if (t < nseg) {
+      const int s = slo + t / 4;
+      const uint32_t key = seg_key(k, s, tx, ty, t & 3);
+      s_segj[t] = bstart[key];
+      s_segv[t + 1] = bend[key] - bstart[key];
+      s_segw[t] = (float)(p - s);
+    }
+    __syncthreads();
+    if (t == 0) {
+      s_segv[0] = 0;
+      for (int q = 1; q <= nseg; ++q) s_segv[q] += s_segv[q - 1];
+    }
+    __syncthreads();
+    const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
+    R ar[CPT][kMaxBatch], ai[CPT][kMaxBatch];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+      for (int b = 0; b < kMaxBatch; ++b) ar[c][b] = ai[c][b] = 0;
+    for (uint32_t vb = vlo; vb < vhi; vb += (uint32_t)S) {
+      // ---- stage ----
+      const uint32_t v = vb + (uint32_t)t;
+      bool ok = false;
+      int dx = 0, dy = 0;
+      uint32_t j = 0;
+      R iw = 0;
+      if (t < S && v < vhi) {
+        int lo = 0, hi = nseg;  // last segment with start <= v
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_segv[mid] <= v) lo = mid;
+          else hi = mid;
+        }
+        j = s_segj[lo] + (v - s_segv[lo]);
+        iw = (R)s_segw[lo];
+        const uint32_t A = rec_a[j];
+        dx = (int)(A & 0xffffu) - tx * T;
+        dy = (int)(A >> 16) - ty * T;
+        if (dx >= k.hgx) dx -= k.gx;
+        if (dx < -k.hgx) dx += k.gx;
+        if (dy >= k.hgy) dy -= k.gy;
+        if (dy < -k.hgy) dy += k.gy;
+        ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
+      }
+      const unsigned long long m = __ballot(ok);
+      if (lane == 0) s_wcnt[wv] = (int)__popcll(m);
+      __syncthreads();
+      int base = 0, M = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wv) base += s_wcnt[w];
+        M += s_wcnt[w];
+      }
+      if (ok) {
+        const int pos = base + (int)__popcll(m & lt);
+        const rec3<R> f = rec_f[j];
+        s_kw[pos] = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
+        for (int i = 0; i < W; ++i) {
+          s_kx[pos * W + i] = esk((f.x + (R)i) * scale, beta);
+          s_ky[pos * W + i] = esk((f.y + (R)i) * scale, beta);
+        }
+        for (int b = 0; b < k.nb; ++b) s_v[pos * 4 + b] = rec_v[(int64_t)j * k.nb + b];
+        s_dxy[pos] = ((dx + 32768) & 0xffff) | ((dy + 32768) << 16);
+      }
+      __syncthreads();
+      // ---- gather ----
+      for (int i = 0; i < M; ++i) {
+        const int d = s_dxy[i];
+        const int rx = cx - ((d & 0xffff) - 32768);
+        if ((unsigned)rx < (unsigned)W) {
+          const R kxw = s_kx[i * W + rx] * s_kw[i];
+          const int dyi = (d >> 16) - 32768;
+          cplx<R> vv[kMaxBatch];
+#pragma unroll
+          for (int b = 0; b < kMaxBatch; ++b)
+            if (b < k.nb) vv[b] = s_v[i * 4 + b];
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const int ry = cy0 + c - dyi;
+            if ((unsigned)ry < (unsigned)W) {
+              const R wgt = kxw * s_ky[i * W + ry];
+#pragma unroll
+              for (int b = 0; b < kMaxBatch; ++b) {
+                if (b < k.nb) {
+                  ar[c][b] += wgt * vv[b].re;
+                  ai[c][b] += wgt * vv[b].im;
                 }
               }
             }
           }
         }
-        r += len;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+      for (int b = 0; b < kMaxBatch; ++b) {
+        if (b < k.nb && (ar[c][b] != (R)0 || ai[c][b] != (R)0)) {
+          cplx<R>* o = plane + ((int64_t)b * k.gx + tx * T + cx) * k.gy + ty * T + cy0 + c;
+          atomicAdd(&o->re, ar[c][b]);
+          atomicAdd(&o->im, ai[c][b]);
+        }
       }
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < k.nb * TT; e += kBlock) {
-      const R re = acc[2 * e], im = acc[2 * e + 1];
-      if (re != (R)0 || im != (R)0) {
-        const int b2 = e / TT, rem = e - b2 * TT, rr = rem / T, cc = rem - rr * T;
-        cplx<R>* o = plane + ((int64_t)b2 * k.gx + tx * T + rr) * k.gy + ty * T + cc;
-        atomicAdd(&o->re, re);
-        atomicAdd(&o->im, im);
-      }
-    }
-    __syncthreads();
   }
 }
 
@@ -1501,8 +1557,10 @@ void set_lds_limits() {
   if (done) return;
   done = true;
   // dynamic LDS up to 64 KiB for the tile kernels (the plan keeps tiles within it)
-  (void)hipFuncSetAttribute((const void*)k_grid<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-  (void)hipFuncSetAttribute((const void*)k_grid<double>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipFuncSetAttribute((const void*)k_grid<float, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipFuncSetAttribute((const void*)k_grid<float, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipFuncSetAttribute((const void*)k_grid<double, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+  (void)hipFuncSetAttribute((const void*)k_grid<double, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
   (void)hipGetLastError();
 }
 
@@ -1705,7 +1763,10 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   k_zero<A><<<gridcap(nimg), kBlock, 0, st>>>(acc, nimg);
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
-  const size_t sm = (size_t)g.nb * P.tile * P.tile * sizeof(cplx<R>);
+  // staged candidates per round: 256 unless the staging arrays would pass 48 KiB of LDS
+  const size_t per = 2 * (size_t)P.W * sizeof(R) + sizeof(R) + 4 * sizeof(cplx<R>) + sizeof(int);
+  const int S = (int)std::min<size_t>(kBlock, (48 * 1024 / per) / 64 * 64);
+  const size_t sm = per * S;
   const dim3 ig(grid1(P.ny), P.nx);
   for (int p = 0; p < P.nplanes; ++p) {
     {
@@ -1713,8 +1774,12 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       build_tasks(g, p, 4, g.chunk_grid, st);
       const int64_t np = (int64_t)g.nb * P.gx * P.gy;
       k_zero<cplx<R>><<<gridcap(np), kBlock, 0, st>>>(plane, np);
-      k_grid<R><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, (const rec3<R>*)g.rec_f,
-                                                recv, g.task, g.chunk_grid, g.counters, plane);
+      if (P.tile == 32)
+        k_grid<R, 32><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, (const rec3<R>*)g.rec_f,
+                                                      recv, g.task, g.chunk_grid, g.counters, S, plane);
+      else
+        k_grid<R, 16><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, (const rec3<R>*)g.rec_f,
+                                                      recv, g.task, g.chunk_grid, g.counters, S, plane);
       LAUNCH_CHECK();
     }
     {
