@@ -86,6 +86,29 @@ def algorithmic_bytes(name, plan, nitems, nb, csz):
     return None
 
 
+def cpu_plans(plan_i, npol, nitems):
+    """Other oversampling choices for the CPU leg (it reports the fastest plan it tried)."""
+    from rascil_amd import wgridder
+
+    out = []
+    old = os.environ.get("HIPNG_SIGMA")
+    try:
+        for sg in (1.25, 2.0):
+            if abs(sg - plan_i["sigma"]) < 1e-9:
+                continue
+            os.environ["HIPNG_SIGMA"] = str(sg)
+            p = wgridder.plan_params(plan_i["umax"], plan_i["vmax"], plan_i["wmin"], plan_i["wmax"],
+                                     plan_i["nx"], plan_i["ny"], plan_i["pixsize_x"], plan_i["pixsize_y"],
+                                     plan_i["epsilon"], True, "fp64", npol, nitems).as_dict()
+            out.append((p, p))
+    finally:
+        if old is None:
+            os.environ.pop("HIPNG_SIGMA", None)
+        else:
+            os.environ["HIPNG_SIGMA"] = old
+    return out
+
+
 def accuracy_probe(wl, prec, nsrc=20, frac=0.01, seed=7):
     """Max / RMS relative error of predict and invert at the workload's plan, on a row
     subsample, against FP64 direct sums: predict vs the FP64 HIP DFT of pixel-centred
@@ -338,7 +361,8 @@ def main():
         try:
             from oracle import wstack_cpu
 
-            out["cpu_baseline"] = wstack_cpu.cpu_baseline(wl, plan_i, plan_p, args.cpu_seconds)
+            out["cpu_baseline"] = wstack_cpu.cpu_baseline(wl, plan_i, plan_p, args.cpu_seconds,
+                                                          plans=cpu_plans(plan_i, npol, nrow * nchan))
         except Exception as e:  # the baseline leg must not hide the GPU result
             out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)

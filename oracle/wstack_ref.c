@@ -168,11 +168,13 @@ void oracle_records(const double* uvw, int32_t nchan, const uint32_t* perm, int6
 }
 
 /* plane [nb][gx][gy] complex (re, im) += contributions of items with start plane in
- * [p - Ww + 1, p]. Tiles are processed in 3x3 colour classes so that concurrently
- * processed tiles never write the same cell (a footprint reaches one tile further). */
+ * [p - Ww + 1, p] of the tiles listed in `tiles` (those with items in that plane range).
+ * Tiles are processed in 3x3 colour classes so that concurrently processed tiles never
+ * write the same cell (a footprint reaches one tile further). */
 void oracle_grid_plane(int32_t p, const oracle_plan* P, double beta, const uint32_t* bstart,
                        const uint32_t* bend, const int32_t* ax, const int32_t* ay, const double* f3,
-                       const double* vals, int32_t nb, double* plane) {
+                       const double* vals, int32_t nb, const uint32_t* tiles, int64_t ntl,
+                       double* plane) {
   const int W = P->W, ww = P->wstack ? P->W : 1;
   const int ntx = P->ntx, nty = P->nty;
   const uint32_t ntiles = (uint32_t)ntx * (uint32_t)nty;
@@ -182,14 +184,15 @@ void oracle_grid_plane(int32_t p, const oracle_plan* P, double beta, const uint3
   for (int cx = 0; cx < 3; ++cx)
     for (int cy = 0; cy < 3; ++cy) {
 #pragma omp parallel for schedule(dynamic, 4)
-      for (int64_t t = 0; t < (int64_t)ntiles; ++t) {
+      for (int64_t it = 0; it < ntl; ++it) {
+        const uint32_t t = tiles[it];
         const int tx = (int)(t / nty), ty = (int)(t % nty);
         const int ccx = (ntx % 2 == 1 && tx == ntx - 1) ? 2 : tx % 2;
         const int ccy = (nty % 2 == 1 && ty == nty - 1) ? 2 : ty % 2;
         if (ccx != cx || ccy != cy) continue;
         double kx[16], ky[16];
         for (int s = slo; s <= shi; ++s) {
-          const uint32_t key = (uint32_t)s * ntiles + (uint32_t)t;
+          const uint32_t key = (uint32_t)s * ntiles + t;
           const double iw = (double)(p - s);
           for (uint32_t j = bstart[key]; j < bend[key]; ++j) {
             const double* f = f3 + 3 * (int64_t)j;

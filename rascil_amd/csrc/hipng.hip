@@ -740,19 +740,23 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
                                                  cplx<R>* __restrict__ plane) {
   constexpr int CPT = T * T / kBlock;  // cells per thread
   constexpr int RUN = T / CPT;         // threads per tile column
+  constexpr int NW = kBlock / 64;      // waves; wave w owns columns [w*CW, (w+1)*CW)
+  constexpr int CW = T / NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int W = k.W;
-  // staged data: taps x [S][W], taps y [S][W], kw [S], values [S][4], packed offsets [S]
+  // staged data: taps x [S][W], taps y [S][W], kw [S], values [S][4], packed offsets [S],
+  // per-wave lists of staged entries [NW][S]
   R* s_kx = reinterpret_cast<R*>(smem_raw);
   R* s_ky = s_kx + S * W;
   R* s_kw = s_ky + S * W;
   cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + S);
-  int* s_dxy = reinterpret_cast<int*>(s_v + 4 * S);
+  uint32_t* s_dxy = reinterpret_cast<uint32_t*>(s_v + 4 * S);
+  uint16_t* s_list = reinterpret_cast<uint16_t*>(s_dxy + S);
   __shared__ uint32_t s_task;
   __shared__ uint32_t s_segv[kMaxSeg + 1];
   __shared__ uint32_t s_segj[kMaxSeg];
   __shared__ float s_segw[kMaxSeg];
-  __shared__ int s_wcnt[kBlock / 64];
+  __shared__ int s_cnt[NW][NW + 1];  // [source wave][target list, all]
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int cx = t / RUN, cy0 = (t - cx * RUN) * CPT;
@@ -814,14 +818,24 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
         if (dy < -k.hgy) dy += k.gy;
         ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
       }
+      // staged position (all lists) and per-wave list positions, deterministic (ballots)
       const unsigned long long m = __ballot(ok);
-      if (lane == 0) s_wcnt[wv] = (int)__popcll(m);
+      unsigned long long ml[NW];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) ml[w] = __ballot(ok && dx < (w + 1) * CW && dx + W > w * CW);
+      if (lane == 0) {
+        s_cnt[wv][NW] = (int)__popcll(m);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s_cnt[wv][w] = (int)__popcll(ml[w]);
+      }
       __syncthreads();
       int base = 0, M = 0;
-      for (int w = 0; w < kBlock / 64; ++w) {
-        if (w < wv) base += s_wcnt[w];
-        M += s_wcnt[w];
+      for (int w = 0; w < NW; ++w) {
+        if (w < wv) base += s_cnt[w][NW];
+        M += s_cnt[w][NW];
       }
+      int Mw = 0;  // length of this wave's list
+      for (int w = 0; w < NW; ++w) Mw += s_cnt[w][wv];
       if (ok) {
         const int pos = base + (int)__popcll(m & lt);
         const rec3<R> f = rec_f[j];
@@ -831,25 +845,35 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
           s_ky[pos * W + i] = esk((f.y + (R)i) * scale, beta);
         }
         for (int b = 0; b < k.nb; ++b) s_v[pos * 4 + b] = rec_v[(int64_t)j * k.nb + b];
-        s_dxy[pos] = ((dx + 32768) & 0xffff) | ((dy + 32768) << 16);
+        s_dxy[pos] = (uint32_t)(dx + 32768) | ((uint32_t)(dy + 32768) << 16);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          if ((ml[w] >> lane) & 1ull) {
+            int lb = 0;
+            for (int src = 0; src < wv; ++src) lb += s_cnt[src][w];
+            s_list[w * S + lb + (int)__popcll(ml[w] & lt)] = (uint16_t)pos;
+          }
+        }
       }
       __syncthreads();
-      // ---- gather ----
-      for (int i = 0; i < M; ++i) {
-        const int d = s_dxy[i];
-        const int rx = cx - ((d & 0xffff) - 32768);
+      // ---- gather: this wave's list only ----
+      const uint16_t* mylist = s_list + wv * S;
+      for (int i = 0; i < Mw; ++i) {
+        const int e = mylist[i];
+        const uint32_t d = s_dxy[e];
+        const int rx = cx - ((int)(d & 0xffffu) - 32768);
         if ((unsigned)rx < (unsigned)W) {
-          const R kxw = s_kx[i * W + rx] * s_kw[i];
-          const int dyi = (d >> 16) - 32768;
+          const R kxw = s_kx[e * W + rx] * s_kw[e];
+          const int dyi = (int)(d >> 16) - 32768;
           cplx<R> vv[kMaxBatch];
 #pragma unroll
           for (int b = 0; b < kMaxBatch; ++b)
-            if (b < k.nb) vv[b] = s_v[i * 4 + b];
+            if (b < k.nb) vv[b] = s_v[e * 4 + b];
 #pragma unroll
           for (int c = 0; c < CPT; ++c) {
             const int ry = cy0 + c - dyi;
             if ((unsigned)ry < (unsigned)W) {
-              const R wgt = kxw * s_ky[i * W + ry];
+              const R wgt = kxw * s_ky[e * W + ry];
 #pragma unroll
               for (int b = 0; b < kMaxBatch; ++b) {
                 if (b < k.nb) {
@@ -1764,7 +1788,8 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
   // staged candidates per round: 256 unless the staging arrays would pass 48 KiB of LDS
-  const size_t per = 2 * (size_t)P.W * sizeof(R) + sizeof(R) + 4 * sizeof(cplx<R>) + sizeof(int);
+  const size_t per = 2 * (size_t)P.W * sizeof(R) + sizeof(R) + 4 * sizeof(cplx<R>) + sizeof(uint32_t) +
+                     (kBlock / 64) * sizeof(uint16_t);
   const int S = (int)std::min<size_t>(kBlock, (48 * 1024 / per) / 64 * 64);
   const size_t sm = per * S;
   const dim3 ig(grid1(P.ny), P.nx);
