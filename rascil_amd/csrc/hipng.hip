@@ -687,15 +687,13 @@ __device__ __forceinline__ uint32_t seg_key(const KP& k, int s, int tx, int ty, 
   return (uint32_t)s * k.ntiles + (uint32_t)stx * k.nty + (uint32_t)sty;
 }
 
-// ntask[t] = ceil(candidates / chunk); counters[1] += ntask[t]
-__global__ void k_tasks(KP k, int p, int nq, const uint32_t* __restrict__ bstart,
+// ntask[t] = ceil(candidates / chunk) for start planes [slo, shi]; counters[1] += ntask[t]
+__global__ void k_tasks(KP k, int slo, int shi, int nq, const uint32_t* __restrict__ bstart,
                         const uint32_t* __restrict__ bend, uint32_t chunk, uint32_t* __restrict__ ntask,
                         uint32_t* __restrict__ counters) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= (int64_t)k.ntiles) return;
   const int tx = (int)(t / k.nty), ty = (int)(t - (int64_t)tx * k.nty);
-  int slo, shi;
-  plane_range(k, p, &slo, &shi);
   uint32_t cnt = 0;
   for (int s = slo; s <= shi; ++s)
     for (int q = 0; q < nq; ++q) {
@@ -727,10 +725,18 @@ __device__ __forceinline__ int task_tile(const uint32_t* __restrict__ task_off, 
 // (gather: no LDS atomics, deterministic order). The tile is added to the zero-filled
 // plane with one global atomic per non-zero cell (tiles can be split over tasks).
 // ---------------------------------------------------------------------------------
-constexpr int kMaxSeg = 4 * 16;
+constexpr int kMaxPC = 4;                    // w planes gridded per pass
+constexpr int kMaxSeg = 4 * (16 + kMaxPC - 1);  // 4 neighbour buckets x start planes
 
-template <typename R, int T>
-__global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __restrict__ bstart,
+// start planes whose W_w-plane window meets planes [p0, p0 + npl)
+__device__ __host__ __forceinline__ void chunk_range(int Ww, int nplanes, int p0, int npl, int* slo,
+                                                     int* shi) {
+  *slo = max(0, p0 - (Ww - 1));
+  *shi = min(p0 + npl - 1, nplanes - Ww);
+}
+
+template <typename R, int T, int PC>
+__global__ __launch_bounds__(kBlock) void k_grid(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                  const uint32_t* __restrict__ bend,
                                                  const uint32_t* __restrict__ rec_a,
                                                  const rec3<R>* __restrict__ rec_f,
@@ -744,18 +750,18 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
   constexpr int CW = T / NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int W = k.W;
-  // staged data: taps x [S][W], taps y [S][W], kw [S], values [S][4], packed offsets [S],
+  // staged data: taps x [S][W], taps y [S][W], w taps [S][PC], values [S][4], packed offsets [S],
   // per-wave lists of staged entries [NW][S]
   R* s_kx = reinterpret_cast<R*>(smem_raw);
   R* s_ky = s_kx + S * W;
   R* s_kw = s_ky + S * W;
-  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + S);
+  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + S * PC);
   uint32_t* s_dxy = reinterpret_cast<uint32_t*>(s_v + 4 * S);
   uint16_t* s_list = reinterpret_cast<uint16_t*>(s_dxy + S);
   __shared__ uint32_t s_task;
   __shared__ uint32_t s_segv[kMaxSeg + 1];
   __shared__ uint32_t s_segj[kMaxSeg];
-  __shared__ float s_segw[kMaxSeg];
+  __shared__ int s_segs[kMaxSeg];
   __shared__ int s_cnt[NW][NW + 1];  // [source wave][target list, all]
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -764,7 +770,7 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
   const R beta = (R)k.beta;
   const uint32_t total = counters[1];
   int slo, shi;
-  plane_range(k, p, &slo, &shi);
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const int nseg = (shi - slo + 1) * 4;
   for (;;) {
     if (t == 0) s_task = atomicAdd(&counters[0], 1u);
@@ -779,7 +785,7 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
       const uint32_t key = seg_key(k, s, tx, ty, t & 3);
       s_segj[t] = bstart[key];
       s_segv[t + 1] = bend[key] - bstart[key];
-      s_segw[t] = (float)(p - s);
+      s_segs[t] = s;
     }
     __syncthreads();
     if (t == 0) {
@@ -788,18 +794,19 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
     }
     __syncthreads();
     const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
-    R ar[CPT][kMaxBatch], ai[CPT][kMaxBatch];
+    R ar[CPT][PC][kMaxBatch], ai[CPT][PC][kMaxBatch];
 #pragma unroll
     for (int c = 0; c < CPT; ++c)
 #pragma unroll
-      for (int b = 0; b < kMaxBatch; ++b) ar[c][b] = ai[c][b] = 0;
+      for (int q = 0; q < PC; ++q)
+#pragma unroll
+        for (int b = 0; b < kMaxBatch; ++b) ar[c][q][b] = ai[c][q][b] = 0;
     for (uint32_t vb = vlo; vb < vhi; vb += (uint32_t)S) {
       // ---- stage ----
       const uint32_t v = vb + (uint32_t)t;
       bool ok = false;
-      int dx = 0, dy = 0;
+      int dx = 0, dy = 0, sp = 0;
       uint32_t j = 0;
-      R iw = 0;
       if (t < S && v < vhi) {
         int lo = 0, hi = nseg;  // last segment with start <= v
         while (hi - lo > 1) {
@@ -808,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
           else hi = mid;
         }
         j = s_segj[lo] + (v - s_segv[lo]);
-        iw = (R)s_segw[lo];
+        sp = s_segs[lo];
         const uint32_t A = rec_a[j];
         dx = (int)(A & 0xffffu) - tx * T;
         dy = (int)(A >> 16) - ty * T;
@@ -829,17 +836,20 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
         for (int w = 0; w < NW; ++w) s_cnt[wv][w] = (int)__popcll(ml[w]);
       }
       __syncthreads();
-      int base = 0, M = 0;
-      for (int w = 0; w < NW; ++w) {
-        if (w < wv) base += s_cnt[w][NW];
-        M += s_cnt[w][NW];
-      }
+      int base = 0;
+      for (int w = 0; w < wv; ++w) base += s_cnt[w][NW];
       int Mw = 0;  // length of this wave's list
       for (int w = 0; w < NW; ++w) Mw += s_cnt[w][wv];
       if (ok) {
         const int pos = base + (int)__popcll(m & lt);
         const rec3<R> f = rec_f[j];
-        s_kw[pos] = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
+#pragma unroll
+        for (int q = 0; q < PC; ++q) {
+          const int iw = p0 + q - sp;  // plane offset inside this visibility's window
+          s_kw[pos * PC + q] = (q < npl && iw >= 0 && iw < k.Ww)
+                                   ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
+                                   : (R)0;
+        }
         for (int i = 0; i < W; ++i) {
           s_kx[pos * W + i] = esk((f.x + (R)i) * scale, beta);
           s_ky[pos * W + i] = esk((f.y + (R)i) * scale, beta);
@@ -863,8 +873,11 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
         const uint32_t d = s_dxy[e];
         const int rx = cx - ((int)(d & 0xffffu) - 32768);
         if ((unsigned)rx < (unsigned)W) {
-          const R kxw = s_kx[e * W + rx] * s_kw[e];
+          const R kx = s_kx[e * W + rx];
           const int dyi = (int)(d >> 16) - 32768;
+          R kw[PC];
+#pragma unroll
+          for (int q = 0; q < PC; ++q) kw[q] = kx * s_kw[e * PC + q];
           cplx<R> vv[kMaxBatch];
 #pragma unroll
           for (int b = 0; b < kMaxBatch; ++b)
@@ -873,12 +886,16 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
           for (int c = 0; c < CPT; ++c) {
             const int ry = cy0 + c - dyi;
             if ((unsigned)ry < (unsigned)W) {
-              const R wgt = kxw * s_ky[e * W + ry];
+              const R ky = s_ky[e * W + ry];
 #pragma unroll
-              for (int b = 0; b < kMaxBatch; ++b) {
-                if (b < k.nb) {
-                  ar[c][b] += wgt * vv[b].re;
-                  ai[c][b] += wgt * vv[b].im;
+              for (int q = 0; q < PC; ++q) {
+                const R wgt = kw[q] * ky;
+#pragma unroll
+                for (int b = 0; b < kMaxBatch; ++b) {
+                  if (b < k.nb) {
+                    ar[c][q][b] += wgt * vv[b].re;
+                    ai[c][q][b] += wgt * vv[b].im;
+                  }
                 }
               }
             }
@@ -890,11 +907,14 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
 #pragma unroll
-      for (int b = 0; b < kMaxBatch; ++b) {
-        if (b < k.nb && (ar[c][b] != (R)0 || ai[c][b] != (R)0)) {
-          cplx<R>* o = plane + ((int64_t)b * k.gx + tx * T + cx) * k.gy + ty * T + cy0 + c;
-          atomicAdd(&o->re, ar[c][b]);
-          atomicAdd(&o->im, ai[c][b]);
+      for (int q = 0; q < PC; ++q) {
+#pragma unroll
+        for (int b = 0; b < kMaxBatch; ++b) {
+          if (b < k.nb && (ar[c][q][b] != (R)0 || ai[c][q][b] != (R)0)) {
+            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + cx) * k.gy + ty * T + cy0 + c;
+            atomicAdd(&o->re, ar[c][q][b]);
+            atomicAdd(&o->im, ai[c][q][b]);
+          }
         }
       }
     }
@@ -926,21 +946,27 @@ __device__ __forceinline__ double nm1s_at(int tx, int ty, const KP& k, double* n
 
 // invert: acc[b][ix][iy] += Re(exp(-2 pi i w_p nm1s) (-1)^(tx+ty) plane[b][tx mod G][ty mod G])
 template <typename R, typename A, bool F32C>
-__global__ void k_screen_acc(KP k, double wp, const cplx<R>* __restrict__ plane, A* __restrict__ acc) {
+__global__ void k_screen_acc(KP k, int p0, int npl, const cplx<R>* __restrict__ plane, A* __restrict__ acc) {
   const int iy = blockIdx.x * kBlock + threadIdx.x;
   const int ix = blockIdx.y;
   if (iy >= k.ny) return;
   const int tx = ix - k.nx / 2, ty = iy - k.ny / 2;
   const int gxi = tx < 0 ? tx + k.gx : tx, gyi = ty < 0 ? ty + k.gy : ty;
-  R c = 1, s = 0;
-  if (k.wstack) phase_cs<R, F32C>(-wp * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &s);
+  const double nm1s = k.wstack ? nm1s_at<F32C>(tx, ty, k, nullptr) : 0.0;
   const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
-  c *= sign;
-  s *= sign;
-  for (int b = 0; b < k.nb; ++b) {
-    const cplx<R> F = plane[((int64_t)b * k.gx + gxi) * k.gy + gyi];
-    acc[((int64_t)b * k.nx + ix) * k.ny + iy] += (A)(F.re * c - F.im * s);
+  A sum[kMaxBatch] = {0, 0, 0, 0};
+  for (int q = 0; q < npl; ++q) {
+    R c = 1, s = 0;
+    if (k.wstack) phase_cs<R, F32C>(-(k.w0 + (p0 + q) * k.dw) * nm1s, &c, &s);
+#pragma unroll
+    for (int b = 0; b < kMaxBatch; ++b) {
+      if (b < k.nb) {
+        const cplx<R> F = plane[((int64_t)(q * k.nb + b) * k.gx + gxi) * k.gy + gyi];
+        sum[b] += (A)(F.re * c - F.im * s);
+      }
+    }
   }
+  for (int b = 0; b < k.nb; ++b) acc[((int64_t)b * k.nx + ix) * k.ny + iy] += (A)sign * sum[b];
 }
 
 // 1 / (Phi_x Phi_y [Phi_w(dw nm1s) n]) at pixel (ix, iy)
@@ -990,7 +1016,7 @@ __global__ void k_prescale(KP k, const I* __restrict__ dirty, A* __restrict__ dc
 
 // predict: plane[b][g] = (-1)^(tx+ty) dc[b][t] exp(+2 pi i w_p nm1s(t)) for image pixels, 0 elsewhere
 template <typename R, typename A, bool F32C>
-__global__ void k_screen_fill(KP k, double wp, const A* __restrict__ dc, cplx<R>* __restrict__ plane) {
+__global__ void k_screen_fill(KP k, int p0, int npl, const A* __restrict__ dc, cplx<R>* __restrict__ plane) {
   const int gyi = blockIdx.x * kBlock + threadIdx.x;
   const int gxi = blockIdx.y;
   if (gyi >= k.gy) return;
@@ -1004,16 +1030,21 @@ __global__ void k_screen_fill(KP k, double wp, const A* __restrict__ dc, cplx<R>
   else if (gyi >= k.gy - hy) ty = gyi - k.gy;
   else { ty = 0; in = false; }
   if (!in) {
-    for (int b = 0; b < k.nb; ++b) plane[((int64_t)b * k.gx + gxi) * k.gy + gyi] = cplx<R>{0, 0};
+    for (int q = 0; q < npl * k.nb; ++q) plane[((int64_t)q * k.gx + gxi) * k.gy + gyi] = cplx<R>{0, 0};
     return;
   }
-  R c = 1, s = 0;
-  if (k.wstack) phase_cs<R, F32C>(wp * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &s);
+  const double nm1s = k.wstack ? nm1s_at<F32C>(tx, ty, k, nullptr) : 0.0;
   const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
   const int ix = tx + hx, iy = ty + hy;
-  for (int b = 0; b < k.nb; ++b) {
-    const R d = (R)dc[((int64_t)b * k.nx + ix) * k.ny + iy] * sign;
-    plane[((int64_t)b * k.gx + gxi) * k.gy + gyi] = cplx<R>{d * c, d * s};
+  R d[kMaxBatch];
+#pragma unroll
+  for (int b = 0; b < kMaxBatch; ++b)
+    d[b] = b < k.nb ? (R)dc[((int64_t)b * k.nx + ix) * k.ny + iy] * sign : (R)0;
+  for (int q = 0; q < npl; ++q) {
+    R c = 1, s = 0;
+    if (k.wstack) phase_cs<R, F32C>((k.w0 + (p0 + q) * k.dw) * nm1s, &c, &s);
+    for (int b = 0; b < k.nb; ++b)
+      plane[((int64_t)(q * k.nb + b) * k.gx + gxi) * k.gy + gyi] = cplx<R>{d[b] * c, d[b] * s};
   }
 }
 
@@ -1581,10 +1612,13 @@ void set_lds_limits() {
   if (done) return;
   done = true;
   // dynamic LDS up to 64 KiB for the tile kernels (the plan keeps tiles within it)
-  (void)hipFuncSetAttribute((const void*)k_grid<float, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-  (void)hipFuncSetAttribute((const void*)k_grid<float, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-  (void)hipFuncSetAttribute((const void*)k_grid<double, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-  (void)hipFuncSetAttribute((const void*)k_grid<double, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+#define HIPNG_LDS(RR, TT, QQ) \
+  (void)hipFuncSetAttribute((const void*)k_grid<RR, TT, QQ>, hipFuncAttributeMaxDynamicSharedMemorySize, 65536)
+  HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 32, 4);
+  HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2); HIPNG_LDS(float, 16, 4);
+  HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 32, 4);
+  HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2); HIPNG_LDS(double, 16, 4);
+#undef HIPNG_LDS
   (void)hipGetLastError();
 }
 
@@ -1605,6 +1639,7 @@ struct Geom {
   uint32_t *task = nullptr, *taskpart = nullptr, *counters = nullptr;
   uint32_t chunk_grid = 1024, chunk_degrid = 1024;
   int npersist = 1024;
+  int pc = 1;  // w planes per gridding pass / FFT batch
   Buffers buf;
 };
 
@@ -1693,9 +1728,21 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     LAUNCH_CHECK();
   }
   g.recv = g.buf.get(S_RECV, (size_t)n * nb * csz);
-  g.plane = g.buf.get(S_PLANE, (size_t)nb * P.gx * P.gy * csz);
-  g.fft_inv = fft_plan(c, P.gx, P.gy, nb, prec, 0);
-  g.fft_fwd = fft_plan(c, P.gx, P.gy, nb, prec, 1);
+  // planes per pass: up to kMaxPC while the plane batch stays within 40 GB and the
+  // gridder's per-thread accumulators (cells x planes x batch x complex) within 128 VGPRs
+  const size_t plane_bytes = (size_t)nb * P.gx * P.gy * csz;
+  const int cpt = P.tile * P.tile / kBlock;
+  g.pc = 1;
+  for (int q = kMaxPC; q > 1; q /= 2) {
+    const int regs = cpt * q * nb * 2 * (prec == HIPNG_FP64 ? 2 : 1);
+    if ((double)q * plane_bytes <= 40e9 && regs <= 128 && q <= P.nplanes) {
+      g.pc = q;
+      break;
+    }
+  }
+  g.plane = g.buf.get(S_PLANE, g.pc * plane_bytes);
+  g.fft_inv = fft_plan(c, P.gx, P.gy, nb * g.pc, prec, 0);
+  g.fft_fwd = fft_plan(c, P.gx, P.gy, nb * g.pc, prec, 1);
   size_t w1 = 0, w2 = 0;
   FFT_TRY(rocfft_plan_get_work_buffer_size(g.fft_inv, &w1));
   FFT_TRY(rocfft_plan_get_work_buffer_size(g.fft_fwd, &w2));
@@ -1713,11 +1760,11 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
 }
 
 // per-plane task list: counters reset, per-tile task counts, exclusive scan -> task offsets
-void build_tasks(const Geom& g, int p, int nq, uint32_t chunk, hipStream_t st) {
+void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st) {
   const int64_t ntiles = g.k.ntiles;
   k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, 2, 0u);
   LAUNCH_CHECK();
-  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, p, nq, g.bstart, g.bend, chunk, g.task, g.counters);
+  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, g.bstart, g.bend, chunk, g.task, g.counters);
   LAUNCH_CHECK();
   const int64_t npart = (ntiles + kRsItems - 1) / kRsItems;
   k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
@@ -1741,13 +1788,14 @@ void geom_fft(const Geom& g, rocfft_plan plan, hipStream_t st) {
 }
 
 template <typename R>
-void launch_degrid(const Geom& g, int p, cplx<R>* accs, hipStream_t st) {
+void launch_degrid(const Geom& g, int p, const cplx<R>* plane, cplx<R>* accs, hipStream_t st) {
   const KP& k = g.k;
   const int L = k.T + k.W - 1;
   const size_t sm = (size_t)k.nb * L * L * sizeof(cplx<R>);
   const auto* rf = (const rec3<R>*)g.rec_f;
-  const auto* plane = (const cplx<R>*)g.plane;
-  build_tasks(g, p, 1, g.chunk_degrid, st);
+  int slo, shi;
+  chunk_range(k.Ww, k.nplanes, p, 1, &slo, &shi);
+  build_tasks(g, slo, shi, 1, g.chunk_degrid, st);
 #define HIPNG_DG(WW)                                                                        \
   case WW:                                                                                  \
     k_degrid<R, WW><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, rf, plane, \
@@ -1787,33 +1835,40 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   k_zero<A><<<gridcap(nimg), kBlock, 0, st>>>(acc, nimg);
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
+  const int PCn = g.pc;
   // staged candidates per round: 256 unless the staging arrays would pass 48 KiB of LDS
-  const size_t per = 2 * (size_t)P.W * sizeof(R) + sizeof(R) + 4 * sizeof(cplx<R>) + sizeof(uint32_t) +
-                     (kBlock / 64) * sizeof(uint16_t);
+  const size_t per = 2 * (size_t)P.W * sizeof(R) + PCn * sizeof(R) + 4 * sizeof(cplx<R>) +
+                     sizeof(uint32_t) + (kBlock / 64) * sizeof(uint16_t);
   const int S = (int)std::min<size_t>(kBlock, (48 * 1024 / per) / 64 * 64);
   const size_t sm = per * S;
   const dim3 ig(grid1(P.ny), P.nx);
-  for (int p = 0; p < P.nplanes; ++p) {
+  const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
+  for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
+    const int npl = std::min(PCn, P.nplanes - p0);
     {
       PROF("grid");
-      build_tasks(g, p, 4, g.chunk_grid, st);
-      const int64_t np = (int64_t)g.nb * P.gx * P.gy;
-      k_zero<cplx<R>><<<gridcap(np), kBlock, 0, st>>>(plane, np);
-      if (P.tile == 32)
-        k_grid<R, 32><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, (const rec3<R>*)g.rec_f,
-                                                      recv, g.task, g.chunk_grid, g.counters, S, plane);
-      else
-        k_grid<R, 16><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, (const rec3<R>*)g.rec_f,
-                                                      recv, g.task, g.chunk_grid, g.counters, S, plane);
+      int slo, shi;
+      chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
+      build_tasks(g, slo, shi, 4, g.chunk_grid, st);
+      k_zero<cplx<R>><<<gridcap(npl * np1), kBlock, 0, st>>>(plane, npl * np1);
+#define HIPNG_GR(TT, QQ)                                                                          \
+  k_grid<R, TT, QQ><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,         \
+                                                    (const rec3<R>*)g.rec_f, recv, g.task,        \
+                                                    g.chunk_grid, g.counters, S, plane)
+      if (P.tile == 32) {
+        if (PCn == 4) HIPNG_GR(32, 4); else if (PCn == 2) HIPNG_GR(32, 2); else HIPNG_GR(32, 1);
+      } else {
+        if (PCn == 4) HIPNG_GR(16, 4); else if (PCn == 2) HIPNG_GR(16, 2); else HIPNG_GR(16, 1);
+      }
+#undef HIPNG_GR
       LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_inverse");
       geom_fft(g, g.fft_inv, st);
     }
-    const double wp = P.w0 + p * P.dw;
     PROF("screen_acc");
-    k_screen_acc<R, A, F32C><<<ig, kBlock, 0, st>>>(k, wp, plane, acc);
+    k_screen_acc<R, A, F32C><<<ig, kBlock, 0, st>>>(k, p0, npl, plane, acc);
     LAUNCH_CHECK();
   }
   PROF("finalize");
@@ -1842,11 +1897,12 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
   const dim3 gg(grid1(P.gy), P.gx);
-  for (int p = 0; p < P.nplanes; ++p) {
-    const double wp = P.w0 + p * P.dw;
+  const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
+  for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
+    const int npl = std::min(g.pc, P.nplanes - p0);
     {
       PROF("screen_fill");
-      k_screen_fill<R, A, F32C><<<gg, kBlock, 0, st>>>(k, wp, dc, plane);
+      k_screen_fill<R, A, F32C><<<gg, kBlock, 0, st>>>(k, p0, npl, dc, plane);
       LAUNCH_CHECK();
     }
     {
@@ -1854,7 +1910,7 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
       geom_fft(g, g.fft_fwd, st);
     }
     PROF("degrid");
-    launch_degrid<R>(g, p, accs, st);
+    for (int q = 0; q < npl; ++q) launch_degrid<R>(g, p0 + q, plane + q * np1, accs, st);
   }
   PROF("unsort");
   k_unsort<R, F32C><<<grid1(g.n), kBlock, 0, st>>>(g.uvw, g.perm, g.n, g.tb.pn, k, accs, (const R*)wgt,
