@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy probe")
     ap.add_argument("--sigma", type=float, default=None,
                     help="force the grid oversampling factor (cost-model calibration, not the bench line)")
-    ap.add_argument("--study", default=None, choices=["precision"],
+    ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched"],
                     help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="also time the FP64 plain mode for this many steps")
@@ -237,12 +237,247 @@ def precision_study(seed=3):
             "rows": rows}
 
 
+def autograd_study(seed=11, nsrc=500, descent_steps=50):
+    """Differentiable predict at 2048^2 (cell 2e-5 rad): ~1M rows x 4 channels from 256
+    stations, 256 per-antenna complex gains (|g| = 1 +- 0.1, phase +-30 deg), V_obs = FP64
+    DFT of power-law sources + noise 0.01. Loss = sum |V_obs - g_i conj(g_j) predict(I)|^2.
+    Image and gain gradients are checked by central finite differences (exact for the
+    image: the loss is quadratic in it) and the image gradient against the FP64 adjoint
+    sum; then gradient descent on the gains."""
+    import numpy
+    import torch
+
+    from rascil_amd import simulation, wgridder
+    from rascil_amd.autograd import apply_gains, predict
+    from rascil_amd.dft import dft_kernel
+
+    dev = torch.device("cuda", 0)
+    rng = numpy.random.default_rng(seed)
+    nant = 256
+    ants = simulation.core_plus_arms_stations(nant, 0.7, 500.0, 1000.0, 40000.0, rng)
+    a1, a2 = simulation.baseline_indices(nant, autocorrelations=False)
+    ntimes = int(numpy.ceil(1_000_000 / len(a1)))
+    dec = numpy.radians(-30.0)
+    uvw = simulation.baseline_uvw(ants, numpy.linspace(-2, 2, ntimes) * numpy.pi / 12, dec,
+                                  autocorrelations=False).reshape(-1, 3)
+    ant1, ant2 = numpy.tile(a1, ntimes), numpy.tile(a2, ntimes)
+    freq = numpy.array([100e6, 110e6, 120e6, 130e6])
+    N, cell = 2048, 2e-5
+    # sky at pixel centres; V_obs = DFT of flux / n (the gridder's w-stacking convention)
+    ix = rng.integers(N // 8, N - N // 8, nsrc)
+    iy = rng.integers(N // 8, N - N // 8, nsrc)
+    flux = simulation.powerlaw_fluxes(nsrc, -1.6, 1e-3, 1.0, rng)
+    x, y = (ix - N // 2) * cell, (iy - N // 2) * cell
+    n = numpy.sqrt(1 - x * x - y * y)
+    uvl = ((uvw * numpy.array([-1.0, 1.0, -1.0]))[:, None, :] * (freq / 299792458.0)[None, :, None])[None]
+    vtrue = dft_kernel(numpy.stack([-x, y, n - 1], 1), (flux / n)[:, None, None] * numpy.ones((1, 4, 1)), uvl)
+    vtrue = vtrue[0, :, :, 0]
+    gtrue = rng.uniform(0.9, 1.1, nant) * numpy.exp(1j * numpy.radians(rng.uniform(-30, 30, nant)))
+    vobs = gtrue[ant1, None] * numpy.conj(gtrue[ant2, None]) * vtrue
+    vobs = vobs + 0.01 * (rng.normal(size=vobs.shape) + 1j * rng.normal(size=vobs.shape))
+    out = {"study": "autograd", "rows": int(uvw.shape[0]), "channels": 4, "image": N, "stations": nant,
+           "sources": nsrc}
+    uvw_d = torch.tensor(uvw, device=dev)
+    i1 = torch.tensor(ant1, device=dev)
+    i2 = torch.tensor(ant2, device=dev)
+    model = numpy.zeros((N, N))
+    numpy.add.at(model, (ix, iy), flux)
+    for prec, eps in (("fp64", 1e-10), ("mixed", 1e-5)):
+        rd = torch.float64 if prec == "fp64" else torch.float32
+        cd = torch.complex64 if prec == "mixed" else torch.complex128
+        t = time.time()
+        geom = wgridder.Geometry(uvw_d, freq, N, N, cell, cell, eps, True, prec)
+        t_geom = time.time() - t
+        vo = torch.tensor(vobs, dtype=torch.complex128, device=dev)
+        img0 = torch.tensor(model * 0.9, dtype=rd, device=dev)
+        g0 = torch.tensor(gtrue * (1 + 0.05 * rng.normal(size=nant)), dtype=cd, device=dev)
+
+        def loss(img, g):
+            v = predict(img, geometry=geom).to(torch.complex128)
+            r = vo - apply_gains(v, g.to(torch.complex128), i1, i2)
+            return (r.abs() ** 2).sum()
+
+        img = img0.clone().requires_grad_(True)
+        g = g0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t = time.time()
+        L = loss(img, g)
+        L.backward()
+        torch.cuda.synchronize()
+        t_fb = time.time() - t
+        gi, gg = img.grad.detach(), g.grad.detach()
+        errs_i, errs_g = [], []
+        h, hg = 1.0, (1e-5 if prec == "fp64" else 1e-2)
+        with torch.no_grad():
+            for k in range(4):
+                px, py = int(ix[k]), int(iy[k])
+                ip, im_ = img0.clone(), img0.clone()
+                ip[px, py] += h
+                im_[px, py] -= h
+                fd = (loss(ip, g0) - loss(im_, g0)).item() / (2 * h)
+                errs_i.append(abs(fd - gi[px, py].item()) / max(1.0, abs(fd)))
+            for k in (0, 7):
+                for dirn in (1.0, 1j):
+                    gp, gm = g0.clone(), g0.clone()
+                    gp[k] += hg * dirn
+                    gm[k] -= hg * dirn
+                    fd = (loss(img0, gp) - loss(img0, gm)).item() / (2 * hg)
+                    an = gg[k].real.item() if dirn == 1.0 else gg[k].imag.item()
+                    errs_g.append(abs(fd - an) / max(1.0, abs(fd)))
+            # image gradient vs the FP64 adjoint sum at the source pixels
+            v = predict(img0, geometry=geom).to(torch.complex128)
+            gprod = g0.to(torch.complex128)[i1] * torch.conj(g0.to(torch.complex128))[i2]
+            back = (-2.0 * torch.conj(gprod)[:, None] * (vo - gprod[:, None] * v)).reshape(-1)
+            U = torch.tensor(uvl[0].reshape(-1, 3), device=dev)
+            ad = []
+            for k in range(8):
+                ph = U[:, 0] * (-x[k]) + U[:, 1] * y[k] + U[:, 2] * (n[k] - 1)
+                ad.append(float((back * torch.exp(2j * numpy.pi * ph.to(torch.complex128))).real.sum()) / n[k])
+            ad = numpy.array(ad)
+            got = numpy.array([gi[int(ix[k]), int(iy[k])].item() for k in range(8)])
+            adj_err = float(numpy.max(numpy.abs(got - ad)) / numpy.max(numpy.abs(ad)))
+        # gradient descent on the gains (image fixed at the true model)
+        imgt = torch.tensor(model, dtype=rd, device=dev)
+        gd = torch.ones(nant, dtype=cd, device=dev).requires_grad_(True)
+        opt = torch.optim.Adam([gd], lr=0.02)
+        hist = []
+        torch.cuda.synchronize()
+        t = time.time()
+        for it in range(descent_steps):
+            opt.zero_grad()
+            Lg = loss(imgt, gd)
+            Lg.backward()
+            opt.step()
+            hist.append(Lg.item())
+        torch.cuda.synchronize()
+        t_desc = time.time() - t
+        out[prec] = {"epsilon": eps, "plan": {k: geom.plan[k] for k in ("W", "sigma", "gx", "nplanes")},
+                     "geometry_s": round(t_geom, 3), "forward_backward_s": round(t_fb, 3),
+                     "fd_image_max_rel": max(errs_i), "fd_gain_max_rel": max(errs_g),
+                     "image_grad_vs_fp64_adjoint_max_rel": adj_err,
+                     "descent": {"steps": descent_steps, "loss_first": hist[0], "loss_last": hist[-1],
+                                 "seconds_per_step": round(t_desc / descent_steps, 4)}}
+        print("[study]", prec, out[prec], file=sys.stderr, flush=True)
+        geom.close()
+    return out
+
+
+def batched_study(njobs=300, seed=5):
+    """Many independent snapshot jobs of uneven size: rows log-uniform 1k-2M (capped to
+    ~60M visibilities in total), images from {256..4096}^2 with the cell matched to each
+    job's longest baseline, 1-16 channels, 2-20 min windows, dec in [-80, +10] deg,
+    epsilon 1e-4 (mixed precision). Serial loop of one-shot invert+predict calls vs
+    prebuilt geometries whose invert+predict are captured in hipGraphs and replayed."""
+    import numpy
+    import torch
+
+    from rascil_amd import simulation, wgridder
+
+    dev = torch.device("cuda", 0)
+    rng = numpy.random.default_rng(seed)
+    ants = simulation.core_plus_arms_stations(512, 0.7, 500.0, 1000.0, 40000.0, rng)
+    a1, a2 = simulation.baseline_indices(512, autocorrelations=False)
+    nbl = len(a1)
+    jobs = []
+    total = 0
+    for jb in range(njobs):
+        rows = int(numpy.exp(rng.uniform(numpy.log(1e3), numpy.log(2e6))))
+        nch = int(rng.integers(1, 17))
+        N = int(rng.choice([256, 512, 1024, 2048, 4096]))
+        dec = numpy.radians(rng.uniform(-80, 10))
+        win = rng.uniform(2, 20) / 60.0 * numpy.pi / 12
+        nt = max(1, rows // nbl + 1)
+        ha = numpy.linspace(-win / 2, win / 2, nt)
+        sel = rng.choice(nbl, min(rows, nbl), replace=False)
+        uvw = simulation.baseline_uvw(ants, ha, dec, autocorrelations=False)[:, sel].reshape(-1, 3)[:rows]
+        freq = numpy.linspace(100e6, 164e6, nch)
+        umax = numpy.abs(uvw[:, :2]).max() * freq.max() / 299792458.0
+        cell = 1.0 / (2.2 * umax)  # 2.2 pixels across the finest fringe
+        jobs.append(dict(uvw=uvw, freq=freq, N=N, cell=cell, nvis=rows * nch))
+        total += rows * nch
+        if total > 60e6:
+            break
+    eps = 1e-4
+    out = {"study": "batched", "jobs": len(jobs), "nvis": int(total)}
+    # serial loop of one-shot calls
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for j in jobs:
+        u = torch.tensor(j["uvw"], device=dev)
+        ms = torch.ones((len(j["uvw"]), len(j["freq"])), dtype=torch.complex64, device=dev)
+        img = torch.zeros((j["N"], j["N"]), dtype=torch.float32, device=dev)
+        t = time.time()
+        wgridder.ms2dirty(u, j["freq"], ms, None, j["N"], j["N"], j["cell"], j["cell"], 0, 0, eps, True,
+                          precision="mixed")
+        wgridder.dirty2ms(u, j["freq"], img, None, j["cell"], j["cell"], 0, 0, eps, True, precision="mixed")
+        torch.cuda.synchronize()
+        lat.append(time.time() - t)
+    t_serial = time.time() - t0
+    out["serial"] = {"seconds": round(t_serial, 3), "mvis_per_s": round(total / t_serial / 1e6, 2),
+                     "p50_ms": round(1e3 * float(numpy.percentile(lat, 50)), 2),
+                     "p99_ms": round(1e3 * float(numpy.percentile(lat, 99)), 2)}
+    print("[study] serial", out["serial"], file=sys.stderr, flush=True)
+    # geometries built once (in groups), invert+predict captured in one hipGraph per group
+    lat = []
+    t_build = 0.0
+    t_run = 0.0
+    group = 24
+    for g0 in range(0, len(jobs), group):
+        grp = jobs[g0:g0 + group]
+        t = time.time()
+        items = []
+        for j in grp:
+            geom = wgridder.Geometry(j["uvw"], j["freq"], j["N"], j["N"], j["cell"], j["cell"], eps, True, "mixed")
+            ms = torch.ones((len(j["uvw"]), len(j["freq"]), 1), dtype=torch.complex64, device=dev)
+            img = torch.zeros((1, j["N"], j["N"]), dtype=torch.float32, device=dev)
+            d = torch.empty((1, j["N"], j["N"]), dtype=torch.float32, device=dev)
+            v = torch.empty_like(ms)
+            items.append((geom, ms, img, d, v))
+        torch.cuda.synchronize()
+        t_build += time.time() - t
+        st = torch.cuda.Stream()
+        st.wait_stream(torch.cuda.current_stream())
+        graphs = []
+        with torch.cuda.stream(st):
+            for geom, ms, img, d, v in items:  # warm-up outside capture
+                geom.ms2dirty(ms, out=d)
+                geom.dirty2ms(img, out=v)
+        torch.cuda.current_stream().wait_stream(st)
+        torch.cuda.synchronize()
+        for geom, ms, img, d, v in items:
+            gph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gph):
+                geom.ms2dirty(ms, out=d)
+                geom.dirty2ms(img, out=v)
+            graphs.append(gph)
+        torch.cuda.synchronize()
+        t = time.time()
+        for gph in graphs:
+            t1 = time.time()
+            gph.replay()
+            torch.cuda.synchronize()
+            lat.append(time.time() - t1)
+        t_run += time.time() - t
+        for geom, *_ in items:
+            geom.close()
+        del items, graphs
+        torch.cuda.empty_cache()
+    out["graph_replay"] = {"seconds": round(t_run, 3), "mvis_per_s": round(total / t_run / 1e6, 2),
+                           "p50_ms": round(1e3 * float(numpy.percentile(lat, 50)), 2),
+                           "p99_ms": round(1e3 * float(numpy.percentile(lat, 99)), 2),
+                           "geometry_build_seconds": round(t_build, 3)}
+    print("[study] graph", out["graph_replay"], file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse()
     if args.sigma:
         os.environ["HIPNG_SIGMA"] = str(args.sigma)
-    if args.study == "precision":
-        print(json.dumps(precision_study()), flush=True)
+    if args.study:
+        fn = {"precision": precision_study, "autograd": autograd_study, "batched": batched_study}[args.study]
+        print(json.dumps(fn()), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
