@@ -64,6 +64,18 @@ def _stream(torch, device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+def _split_batch(fn, kwargs):
+    """Retry with half the batch when the plan reports that the batch does not fit in LDS."""
+    bc = kwargs.get("batch_chunk") or MAX_BATCH
+    while True:
+        try:
+            return fn(**dict(kwargs, batch_chunk=bc))
+        except ValueError as e:
+            if "split the batch" not in str(e) or bc == 1:
+                raise
+            bc //= 2
+
+
 def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=0,
              epsilon=1e-7, do_wstacking=False, nthreads=1, verbosity=0, mask=None,
              double_precision_accumulation=False, precision=None, plan_out=None, batch_chunk=None):
@@ -72,6 +84,25 @@ def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=
     Returns [npix_x, npix_y] (or [nbatch, npix_x, npix_y] for batched ms); float64 for
     "fp64", float32 otherwise.
     """
+    return _split_batch(_ms2dirty, dict(
+        uvw=uvw, freq=freq, ms=ms, wgt=wgt, npix_x=npix_x, npix_y=npix_y, pixsize_x=pixsize_x,
+        pixsize_y=pixsize_y, epsilon=epsilon, do_wstacking=do_wstacking, verbosity=verbosity,
+        mask=mask, precision=precision, plan_out=plan_out, batch_chunk=batch_chunk))
+
+
+def dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e-7,
+             do_wstacking=False, nthreads=1, verbosity=0, mask=None, precision=None,
+             plan_out=None, batch_chunk=None):
+    """Dirty image(s) -> visibilities [nrow, nchan] (or [nrow, nchan, nbatch])."""
+    return _split_batch(_dirty2ms, dict(
+        uvw=uvw, freq=freq, dirty=dirty, wgt=wgt, pixsize_x=pixsize_x, pixsize_y=pixsize_y,
+        epsilon=epsilon, do_wstacking=do_wstacking, verbosity=verbosity, mask=mask,
+        precision=precision, plan_out=plan_out, batch_chunk=batch_chunk))
+
+
+def _ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=0,
+              epsilon=1e-7, do_wstacking=False, nthreads=1, verbosity=0, mask=None,
+              double_precision_accumulation=False, precision=None, plan_out=None, batch_chunk=None):
     torch = _torch()
     lib = _lib.load()
     as_numpy = isinstance(ms, numpy.ndarray)
@@ -128,10 +159,9 @@ def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=
     return out
 
 
-def dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e-7,
-             do_wstacking=False, nthreads=1, verbosity=0, mask=None, precision=None,
-             plan_out=None, batch_chunk=None):
-    """Dirty image(s) -> visibilities [nrow, nchan] (or [nrow, nchan, nbatch])."""
+def _dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e-7,
+              do_wstacking=False, nthreads=1, verbosity=0, mask=None, precision=None,
+              plan_out=None, batch_chunk=None):
     torch = _torch()
     lib = _lib.load()
     as_numpy = isinstance(dirty, numpy.ndarray)
