@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy probe")
     ap.add_argument("--sigma", type=float, default=None,
                     help="force the grid oversampling factor (cost-model calibration, not the bench line)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a plain 1 GiB device copy 4 times (counter calibration for rocprof --pmc)")
     ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched"],
                     help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
@@ -471,8 +473,29 @@ def batched_study(njobs=300, seed=5):
     return out
 
 
+def copy_probe(nbytes=1 << 30, reps=4):
+    """A plain device copy of known size: the rocprof FETCH_SIZE / WRITE_SIZE counters read on
+    it give the scale factor between counter units and bytes moved."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t = time.time()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.time() - t) / reps
+    return {"probe": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
+            "GBps": round(2 * nbytes / dt / 1e9, 1)}
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        print(json.dumps(copy_probe()), flush=True)
+        return
     if args.sigma:
         os.environ["HIPNG_SIGMA"] = str(args.sigma)
     if args.study:

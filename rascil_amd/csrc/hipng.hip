@@ -223,10 +223,8 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
       }
     if (W < 0) continue;
     const double sigma = kSigmas[is];
-    // tile: largest of {32, 16} whose LDS tile + halo (T+W-1)^2 * batch plus 64 staged
-    // visibilities of the gridder fit the LDS budget
-    const size_t rsz = prec == HIPNG_FP64 ? 8 : 4;
-    const size_t stage = 64 * (2 * W * rsz + rsz + 4 * csz + 4) + 64;
+    // tile: largest of {32, 16} whose degridder LDS tile with halo (T+W-1)^2 * batch fits
+    const size_t stage = 0;
     int T = 32;
     if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz + stage > (size_t)lds_budget) T = 16;
     if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz + stage > (size_t)lds_budget) {
@@ -247,13 +245,17 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
     }
     const double ntiles = (double)(gx / T) * (gy / T);
     if ((double)nplanes * ntiles >= 4294967295.0) continue;
+    // cost model, constants measured on MI355X at the production shape (mixed precision,
+    // 4 polarisations): gather gridder per visibility-plane-pol x halo overlap, degridder per
+    // cell-pol, plane FFT / fill / screen per grid cell-pol (both directions)
     const double ww = wstack ? W : 1;
     const double ov = (1.0 + (double)(W - 1) / T) * (1.0 + (double)(W - 1) / T);
     const double g2 = (double)gx * gy;
     const double pf = prec == HIPNG_FP64 ? 2.0 : 1.0;
-    const double cost = pf * (5e-12 * (double)nitems * nbatch * W * W * ww * ov +
-                              5.3e-13 * nplanes * nbatch * g2 * std::log2(g2) +
-                              2e-12 * nplanes * nbatch * (g2 + 2.0 * nx * ny));
+    const double cost = pf * (0.7e-10 * (double)nitems * nbatch * ww * ov +
+                              2.6e-13 * (double)nitems * nbatch * ww * W * W +
+                              2.0 * 1.4e-11 * nplanes * nbatch * g2 +
+                              2.0 * 1.5e-12 * nplanes * nbatch * (g2 + (double)nx * ny));
     if (cost < best_cost) {
       best_cost = cost;
       P->W = W;
@@ -735,7 +737,7 @@ __device__ __forceinline__ int task_tile(const uint32_t* __restrict__ task_off, 
 // (gather: no LDS atomics, deterministic order). The tile is added to the zero-filled
 // plane with one global atomic per non-zero cell (tiles can be split over tasks).
 // ---------------------------------------------------------------------------------
-constexpr int kMaxPC = 4;                    // w planes gridded per pass
+constexpr int kMaxPC = 2;                    // w planes gridded per pass
 constexpr int kMaxSeg = 4 * (16 + kMaxPC - 1);  // 4 neighbour buckets x start planes
 
 // start planes whose W_w-plane window meets planes [p0, p0 + npl)
@@ -745,14 +747,8 @@ __device__ __host__ __forceinline__ void chunk_range(int Ww, int nplanes, int p0
   *shi = min(p0 + npl - 1, nplanes - Ww);
 }
 
-// Gridder (staged scatter): persistent workgroups over (tile, chunk) tasks of plane p; the
-// task's own-tile visibilities (their footprints lie inside the tile plus a W-1 halo) are
-// staged S at a time in LDS by one thread each (coalesced record loads, the 2W+1 kernel taps
-// evaluated once); then each wave takes one staged visibility at a time with its lanes on the
-// W x W footprint cells and adds into the LDS tile+halo accumulator with LDS float atomics;
-// the accumulator is added to the zero-filled plane with one global atomic per non-zero cell.
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __restrict__ bstart,
+template <typename R, int T, int PC>
+__global__ __launch_bounds__(kBlock) void k_grid(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                  const uint32_t* __restrict__ bend,
                                                  const uint32_t* __restrict__ rec_a,
                                                  const rec3<R>* __restrict__ rec_f,
@@ -760,28 +756,36 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
                                                  const uint32_t* __restrict__ task_off, uint32_t chunk,
                                                  uint32_t* __restrict__ counters, int S,
                                                  cplx<R>* __restrict__ plane) {
+  constexpr int CPT = T * T / kBlock;  // cells per thread
+  constexpr int RUN = T / CPT;         // threads per tile column
+  constexpr int NW = kBlock / 64;      // waves; wave w owns columns [w*CW, (w+1)*CW)
+  constexpr int CW = T / NW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int W = k.W, T = k.T, L = k.T + k.W - 1, LL = L * L, nb = k.nb;
-  R* acc = reinterpret_cast<R*>(smem_raw);  // [nb][L][L][2]
-  R* s_kx = acc + 2 * nb * LL;              // [S][W]
-  R* s_ky = s_kx + S * W;                   // [S][W]
-  R* s_kw = s_ky + S * W;                   // [S]
-  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + S + (S & 1));  // [S][4]
-  uint32_t* s_off = reinterpret_cast<uint32_t*>(s_v + 4 * S);     // packed (rx0, ry0)
+  const int W = k.W;
+  // staged data: taps x [S][W], taps y [S][W], w taps [S][PC], values [S][4], packed offsets [S],
+  // per-wave lists of staged entries [NW][S]
+  R* s_kx = reinterpret_cast<R*>(smem_raw);
+  R* s_ky = s_kx + S * W;
+  R* s_kw = s_ky + S * W;
+  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + S * PC);
+  uint32_t* s_dxy = reinterpret_cast<uint32_t*>(s_v + 4 * S);
+  uint16_t* s_list = reinterpret_cast<uint16_t*>(s_dxy + S);
   __shared__ uint32_t s_task;
-  __shared__ uint32_t s_segv[17];
-  __shared__ uint32_t s_segj[16];
+  __shared__ uint32_t s_segv[kMaxSeg + 1];
+  __shared__ uint32_t s_segj[kMaxSeg];
+  __shared__ int s_segs[kMaxSeg];
+  __shared__ int s_cnt[NW][NW + 1];  // [source wave][target list, all]
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int cx = t / RUN, cy0 = (t - cx * RUN) * CPT;
   const R scale = (R)2 / (R)W;
   const R beta = (R)k.beta;
   const uint32_t total = counters[1];
   int slo, shi;
-  plane_range(k, p, &slo, &shi);
-  const int nseg = shi - slo + 1;
-  const int ncell = W * W;
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
+  const int nseg = (shi - slo + 1) * 4;
   for (;;) {
     if (t == 0) s_task = atomicAdd(&counters[0], 1u);
-    for (int e = t; e < 2 * nb * LL; e += kBlock) acc[e] = 0;
     __syncthreads();
     const uint32_t task = s_task;
     if (task >= total) break;
@@ -789,9 +793,11 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
     const uint32_t vlo = (task - task_off[tile]) * chunk;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
     if (t < nseg) {
-      const uint32_t key = (uint32_t)(slo + t) * k.ntiles + (uint32_t)tile;
+      const int s = slo + t / 4;
+      const uint32_t key = seg_key(k, s, tx, ty, t & 3);
       s_segj[t] = bstart[key];
       s_segv[t + 1] = bend[key] - bstart[key];
+      s_segs[t] = s;
     }
     __syncthreads();
     if (t == 0) {
@@ -800,64 +806,146 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p, const uint32_t* __
     }
     __syncthreads();
     const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
+    // a tile owned by a single task is stored; tiles split over tasks are added atomically
+    const uint32_t tnext = tile + 1 < (int)k.ntiles ? task_off[tile + 1] : total;
+    const bool solo = tnext - task_off[tile] == 1;
+    R ar[CPT][PC][kMaxBatch], ai[CPT][PC][kMaxBatch];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+      for (int q = 0; q < PC; ++q)
+#pragma unroll
+        for (int b = 0; b < kMaxBatch; ++b) ar[c][q][b] = ai[c][q][b] = 0;
     for (uint32_t vb = vlo; vb < vhi; vb += (uint32_t)S) {
-      const int nst = (int)min((uint32_t)S, vhi - vb);
       // ---- stage ----
-      if (t < nst) {
-        const uint32_t v = vb + (uint32_t)t;
-        int sg = 0;
-        while (sg + 1 < nseg && s_segv[sg + 1] <= v) ++sg;
-        const uint32_t j = s_segj[sg] + (v - s_segv[sg]);
-        const uint32_t A = rec_a[j];
-        const uint32_t rx0 = (A & 0xffffu) - (uint32_t)(tx * T), ry0 = (A >> 16) - (uint32_t)(ty * T);
-        s_off[t] = rx0 | (ry0 << 16);
-        const rec3<R> f = rec_f[j];
-        s_kw[t] = k.wstack ? esk((f.w + (R)(p - (slo + sg))) * scale, beta) : (R)1;
-        for (int i = 0; i < W; ++i) {
-          s_kx[t * W + i] = esk((f.x + (R)i) * scale, beta);
-          s_ky[t * W + i] = esk((f.y + (R)i) * scale, beta);
+      const uint32_t v = vb + (uint32_t)t;
+      bool ok = false;
+      int dx = 0, dy = 0, sp = 0;
+      uint32_t j = 0;
+      if (t < S && v < vhi) {
+        int lo = 0, hi = nseg;  // last segment with start <= v
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_segv[mid] <= v) lo = mid;
+          else hi = mid;
         }
-        for (int b = 0; b < nb; ++b) s_v[t * 4 + b] = rec_v[(int64_t)j * nb + b];
+        j = s_segj[lo] + (v - s_segv[lo]);
+        sp = s_segs[lo];
+        const uint32_t A = rec_a[j];
+        dx = (int)(A & 0xffffu) - tx * T;
+        dy = (int)(A >> 16) - ty * T;
+        if (dx >= k.hgx) dx -= k.gx;
+        if (dx < -k.hgx) dx += k.gx;
+        if (dy >= k.hgy) dy -= k.gy;
+        if (dy < -k.hgy) dy += k.gy;
+        ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
+      }
+      // staged position (all lists) and per-wave list positions, deterministic (ballots)
+      const unsigned long long m = __ballot(ok);
+      unsigned long long ml[NW];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) ml[w] = __ballot(ok && dx < (w + 1) * CW && dx + W > w * CW);
+      if (lane == 0) {
+        s_cnt[wv][NW] = (int)__popcll(m);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s_cnt[wv][w] = (int)__popcll(ml[w]);
       }
       __syncthreads();
-      // ---- scatter: one wave per staged visibility, lanes on footprint cells ----
-      for (int i = wv; i < nst; i += kBlock / 64) {
-        const uint32_t o = s_off[i];
-        const int rx0 = (int)(o & 0xffffu), ry0 = (int)(o >> 16);
-        const R kw = s_kw[i];
-        cplx<R> vv[kMaxBatch];
+      int base = 0;
+      for (int w = 0; w < wv; ++w) base += s_cnt[w][NW];
+      int Mw = 0;  // length of this wave's list
+      for (int w = 0; w < NW; ++w) Mw += s_cnt[w][wv];
+      if (ok) {
+        const int pos = base + (int)__popcll(m & lt);
+        const rec3<R> f = rec_f[j];
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b)
-          if (b < nb) vv[b] = s_v[i * 4 + b];
-        for (int c = lane; c < ncell; c += 64) {
-          const int ix = c / W, iy = c - ix * W;
-          const R wgt = s_kx[i * W + ix] * s_ky[i * W + iy] * kw;
-          R* a = acc + 2 * ((rx0 + ix) * L + ry0 + iy);
+        for (int q = 0; q < PC; ++q) {
+          const int iw = p0 + q - sp;  // plane offset inside this visibility's window
+          s_kw[pos * PC + q] = (q < npl && iw >= 0 && iw < k.Ww)
+                                   ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
+                                   : (R)0;
+        }
+        for (int i = 0; i < W; ++i) {
+          s_kx[pos * W + i] = esk((f.x + (R)i) * scale, beta);
+          s_ky[pos * W + i] = esk((f.y + (R)i) * scale, beta);
+        }
+        for (int b = 0; b < k.nb; ++b) s_v[pos * 4 + b] = rec_v[(int64_t)j * k.nb + b];
+        s_dxy[pos] = (uint32_t)(dx + 32768) | ((uint32_t)(dy + 32768) << 16);
 #pragma unroll
-          for (int b = 0; b < kMaxBatch; ++b) {
-            if (b < nb) {
-              atomicAdd(a + 2 * b * LL, wgt * vv[b].re);
-              atomicAdd(a + 2 * b * LL + 1, wgt * vv[b].im);
-            }
+        for (int w = 0; w < NW; ++w) {
+          if ((ml[w] >> lane) & 1ull) {
+            int lb = 0;
+            for (int src = 0; src < wv; ++src) lb += s_cnt[src][w];
+            s_list[w * S + lb + (int)__popcll(ml[w] & lt)] = (uint16_t)pos;
           }
         }
       }
       __syncthreads();
+      // ---- gather: this wave's list only, two entries per iteration ----
+      const uint16_t* mylist = s_list + wv * S;
+      auto gather_one = [&](int e, uint32_t d) {
+        const int rx = cx - ((int)(d & 0xffffu) - 32768);
+        if ((unsigned)rx < (unsigned)W) {
+          const R kx = s_kx[e * W + rx];
+          const int dyi = (int)(d >> 16) - 32768;
+          R kw[PC];
+#pragma unroll
+          for (int q = 0; q < PC; ++q) kw[q] = kx * s_kw[e * PC + q];
+          cplx<R> vv[kMaxBatch];
+#pragma unroll
+          for (int b = 0; b < kMaxBatch; ++b)
+            if (b < k.nb) vv[b] = s_v[e * 4 + b];
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const int ry = cy0 + c - dyi;
+            if ((unsigned)ry < (unsigned)W) {
+              const R ky = s_ky[e * W + ry];
+#pragma unroll
+              for (int q = 0; q < PC; ++q) {
+                const R wgt = kw[q] * ky;
+#pragma unroll
+                for (int b = 0; b < kMaxBatch; ++b) {
+                  if (b < k.nb) {
+                    ar[c][q][b] += wgt * vv[b].re;
+                    ai[c][q][b] += wgt * vv[b].im;
+                  }
+                }
+              }
+            }
+          }
+        }
+      };
+      int i = 0;
+      for (; i + 1 < Mw; i += 2) {
+        const int e0 = mylist[i], e1 = mylist[i + 1];
+        const uint32_t d0 = s_dxy[e0], d1 = s_dxy[e1];
+        gather_one(e0, d0);
+        gather_one(e1, d1);
+      }
+      if (i < Mw) {
+        const int e0 = mylist[i];
+        gather_one(e0, s_dxy[e0]);
+      }
+      __syncthreads();
     }
-    // ---- flush tile + halo ----
-    for (int e = t; e < nb * LL; e += kBlock) {
-      const R re = acc[2 * e], im = acc[2 * e + 1];
-      if (re != (R)0 || im != (R)0) {
-        const int b = e / LL, rem = e - b * LL, rr = rem / L, cc = rem - rr * L;
-        int gxi = tx * T + rr, gyi = ty * T + cc;
-        if (gxi >= k.gx) gxi -= k.gx;
-        if (gyi >= k.gy) gyi -= k.gy;
-        cplx<R>* out = plane + ((int64_t)b * k.gx + gxi) * k.gy + gyi;
-        atomicAdd(&out->re, re);
-        atomicAdd(&out->im, im);
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+#pragma unroll
+        for (int b = 0; b < kMaxBatch; ++b) {
+          if (b < k.nb && (ar[c][q][b] != (R)0 || ai[c][q][b] != (R)0)) {
+            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + cx) * k.gy + ty * T + cy0 + c;
+            if (solo) {
+              *o = cplx<R>{ar[c][q][b], ai[c][q][b]};
+            } else {
+              atomicAdd(&o->re, ar[c][q][b]);
+              atomicAdd(&o->im, ai[c][q][b]);
+            }
+          }
+        }
       }
     }
-    __syncthreads();
   }
 }
 
@@ -1552,8 +1640,11 @@ void set_lds_limits() {
   if (done) return;
   done = true;
   // dynamic LDS up to 64 KiB for the tile kernels (the plan keeps tiles within it)
-  (void)hipFuncSetAttribute((const void*)k_grid<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
-  (void)hipFuncSetAttribute((const void*)k_grid<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+#define HIPNG_LDS(RR, TT, QQ) \
+  (void)hipFuncSetAttribute((const void*)k_grid<RR, TT, QQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget)
+  HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
+  HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
+#undef HIPNG_LDS
   (void)hipGetLastError();
 }
 
@@ -1771,28 +1862,32 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
   const int PCn = g.pc;
-  const int L = P.tile + P.W - 1;
-  const size_t tile_bytes = (size_t)g.nb * L * L * sizeof(cplx<R>);
-  const size_t per = 2 * (size_t)P.W * sizeof(R) + sizeof(R) + 4 * sizeof(cplx<R>) + sizeof(uint32_t);
-  const int S = (int)std::min<size_t>(kBlock, ((kLdsBudget - tile_bytes - 64) / per) / 64 * 64);
-  if (S < 64) HIPNG_THROW(HIPNG_EINVAL, "gridder tile does not fit in LDS (reduce nbatch)");
-  const size_t sm = tile_bytes + per * S + 64;
+  // staged candidates per round: 256 unless the staging arrays would pass 48 KiB of LDS
+  const size_t per = 2 * (size_t)P.W * sizeof(R) + PCn * sizeof(R) + 4 * sizeof(cplx<R>) +
+                     sizeof(uint32_t) + (kBlock / 64) * sizeof(uint16_t);
+  const int S = (int)std::min<size_t>(kBlock, (48 * 1024 / per) / 64 * 64);
+  const size_t sm = per * S;
   const dim3 ig(grid1(P.ny), P.nx);
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
     {
       PROF("grid");
+      int slo, shi;
+      chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
+      build_tasks(g, slo, shi, 4, g.chunk_grid, st);
       k_zero<cplx<R>><<<gridcap(npl * np1), kBlock, 0, st>>>(plane, npl * np1);
-      for (int q = 0; q < npl; ++q) {
-        int slo, shi;
-        chunk_range(k.Ww, P.nplanes, p0 + q, 1, &slo, &shi);
-        build_tasks(g, slo, shi, 1, g.chunk_degrid, st);
-        k_grid<R><<<g.npersist, kBlock, sm, st>>>(k, p0 + q, g.bstart, g.bend, g.rec_a,
-                                                  (const rec3<R>*)g.rec_f, recv, g.task, g.chunk_degrid,
-                                                  g.counters, S, plane + q * np1);
-        LAUNCH_CHECK();
+#define HIPNG_GR(TT, QQ)                                                                          \
+  k_grid<R, TT, QQ><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,         \
+                                                    (const rec3<R>*)g.rec_f, recv, g.task,        \
+                                                    g.chunk_grid, g.counters, S, plane)
+      if (P.tile == 32) {
+        if (PCn == 2) HIPNG_GR(32, 2); else HIPNG_GR(32, 1);
+      } else {
+        if (PCn == 2) HIPNG_GR(16, 2); else HIPNG_GR(16, 1);
       }
+#undef HIPNG_GR
+      LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_inverse");
