@@ -156,11 +156,20 @@ bool good_factor(int64_t m) {
   return m == 1;
 }
 
-// Smallest multiple of `tile` >= n whose quotient has only factors 2, 3, 5.
+// 2^a, 3 * 2^a or 5 * 2^a: the large transform lengths rocFFT runs at full speed on MI355X
+// (measured at 4 x G^2 complex64: 12288, 16384, 20480 at 19-22 ps per point; 11520, 12800,
+// 13824, 14400, 15360, 17280, 18432 at 42-49 ps per point)
+bool fast_large(int64_t g) {
+  while (g % 2 == 0) g /= 2;
+  return g == 1 || g == 3 || g == 5;
+}
+
+// Smallest multiple of `tile` >= n whose quotient has only factors 2, 3, 5; from 4096 on
+// also a fast rocFFT length.
 int good_grid(double n, int tile) {
   int64_t m = (int64_t)std::ceil(n / tile);
   if (m < 2) m = 2;
-  while (!good_factor(m)) ++m;
+  while (!good_factor(m) || (m * tile >= 4096 && !fast_large(m * tile))) ++m;
   return (int)(m * tile);
 }
 
@@ -702,7 +711,7 @@ __device__ __forceinline__ uint32_t seg_key(const KP& k, int s, int tx, int ty, 
 // ntask[t] = ceil(candidates / chunk) for start planes [slo, shi]; counters[1] += ntask[t]
 __global__ void k_tasks(KP k, int slo, int shi, int nq, const uint32_t* __restrict__ bstart,
                         const uint32_t* __restrict__ bend, uint32_t chunk, uint32_t* __restrict__ ntask,
-                        uint32_t* __restrict__ counters) {
+                        uint32_t* __restrict__ counters, uint32_t mult) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= (int64_t)k.ntiles) return;
   const int tx = (int)(t / k.nty), ty = (int)(t - (int64_t)tx * k.nty);
@@ -712,7 +721,7 @@ __global__ void k_tasks(KP k, int slo, int shi, int nq, const uint32_t* __restri
       const uint32_t key = seg_key(k, s, tx, ty, q);
       cnt += bend[key] - bstart[key];
     }
-  const uint32_t nt = (cnt + chunk - 1) / chunk;
+  const uint32_t nt = (cnt + chunk - 1) / chunk * mult;
   ntask[t] = nt;
   if (nt) atomicAdd(&counters[1], nt);
 }
@@ -927,6 +936,204 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p0, int npl, const ui
         gather_one(e0, s_dxy[e0]);
       }
       __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+#pragma unroll
+        for (int b = 0; b < kMaxBatch; ++b) {
+          if (b < k.nb && (ar[c][q][b] != (R)0 || ai[c][q][b] != (R)0)) {
+            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + cx) * k.gy + ty * T + cy0 + c;
+            if (solo) {
+              *o = cplx<R>{ar[c][q][b], ai[c][q][b]};
+            } else {
+              atomicAdd(&o->re, ar[c][q][b]);
+              atomicAdd(&o->im, ai[c][q][b]);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Gridder, wave-independent variant: a task is (tile, chunk, column strip) and belongs to one
+// wave. The wave scans the chunk's candidates 64 at a time (one per lane: record load, test
+// against its T/4-column strip of the tile), compacts the overlapping ones into its private
+// LDS slots with the kernel taps evaluated once, then every lane gathers the staged
+// visibilities into its CPT cells x PC planes x nb in registers. Only wave-level ordering
+// is needed, so the four waves of a workgroup never wait for each other.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <typename R, int T, int PC>
+__global__ __launch_bounds__(kBlock) void k_gridw(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
+                                                  const uint32_t* __restrict__ bend,
+                                                  const uint32_t* __restrict__ rec_a,
+                                                  const rec3<R>* __restrict__ rec_f,
+                                                  const cplx<R>* __restrict__ rec_v,
+                                                  const uint32_t* __restrict__ task_off, uint32_t chunk,
+                                                  uint32_t* __restrict__ counters,
+                                                  cplx<R>* __restrict__ plane) {
+  constexpr int NS = kBlock / 64;       // strips per tile = waves per workgroup
+  constexpr int CW = T / NS;            // columns per strip
+  constexpr int CPT = T * T / kBlock;   // cells per lane
+  constexpr int RUN = T / CPT;          // lanes per column
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int W = k.W;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // per-wave region: taps x [64][W], taps y [64][W], w taps [64][PC], values [64][4],
+  // packed offsets [64], segment table
+  const size_t region = (size_t)64 * (2 * W + PC) * sizeof(R) + 64 * 4 * sizeof(cplx<R>) + 64 * 4 +
+                        (2 * kMaxSeg + 1) * 4 + 16;
+  unsigned char* base = smem_raw + wv * ((region + 15) / 16 * 16);
+  R* s_kx = reinterpret_cast<R*>(base);
+  R* s_ky = s_kx + 64 * W;
+  R* s_kw = s_ky + 64 * W;
+  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + 64 * PC);
+  uint32_t* s_dxy = reinterpret_cast<uint32_t*>(s_v + 64 * 4);
+  uint32_t* s_segv = s_dxy + 64;          // [kMaxSeg + 1]
+  uint32_t* s_segj = s_segv + kMaxSeg + 1;  // [kMaxSeg]
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const R scale = (R)2 / (R)W;
+  const R beta = (R)k.beta;
+  const uint32_t total = counters[1];
+  int slo, shi;
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
+  const int nseg = (shi - slo + 1) * 4;
+  for (;;) {
+    uint32_t task = 0;
+    if (lane == 0) task = atomicAdd(&counters[0], 1u);
+    task = __shfl(task, 0, 64);
+    if (task >= total) break;
+    const int tile = task_tile(task_off, (int)k.ntiles, task);
+    const uint32_t local = task - task_off[tile];
+    const uint32_t ci = local / NS;
+    const int strip = (int)(local - ci * NS);
+    const uint32_t tnext = tile + 1 < (int)k.ntiles ? task_off[tile + 1] : total;
+    const bool solo = tnext - task_off[tile] == (uint32_t)NS;
+    const int tx = tile / k.nty, ty = tile - tx * k.nty;
+    // segment table: lanes < nseg load one bucket each, inclusive scan by shuffles
+    uint32_t len = 0, st = 0;
+    for (int sg0 = 0; sg0 < nseg; sg0 += 64) {
+      const int sg = sg0 + lane;
+      len = 0;
+      if (sg < nseg) {
+        const uint32_t key = seg_key(k, slo + sg / 4, tx, ty, sg & 3);
+        st = bstart[key];
+        len = bend[key] - st;
+        s_segj[sg] = st;
+      }
+      uint32_t inc = len;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t tt = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += tt;
+      }
+      const uint32_t carry = sg0 ? s_segv[sg0] : 0u;
+      if (sg < nseg) s_segv[sg + 1] = carry + inc;
+      if (sg0 == 0 && lane == 0) s_segv[0] = 0;
+      wave_sync();
+    }
+    const uint32_t vlo = ci * chunk;
+    const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
+    const int slo_x = strip * CW, shi_x = slo_x + CW;
+    const int cx = slo_x + lane / RUN, cy0 = (lane % RUN) * CPT;
+    R ar[CPT][PC][kMaxBatch], ai[CPT][PC][kMaxBatch];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c)
+#pragma unroll
+      for (int q = 0; q < PC; ++q)
+#pragma unroll
+        for (int b = 0; b < kMaxBatch; ++b) ar[c][q][b] = ai[c][q][b] = 0;
+    for (uint32_t vb = vlo; vb < vhi; vb += 64) {
+      const uint32_t v = vb + (uint32_t)lane;
+      bool ok = false;
+      int dx = 0, dy = 0, sp = 0;
+      uint32_t j = 0;
+      if (v < vhi) {
+        int lo = 0, hi = nseg;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_segv[mid] <= v) lo = mid;
+          else hi = mid;
+        }
+        j = s_segj[lo] + (v - s_segv[lo]);
+        sp = slo + lo / 4;
+        const uint32_t A = rec_a[j];
+        dx = (int)(A & 0xffffu) - tx * T;
+        dy = (int)(A >> 16) - ty * T;
+        if (dx >= k.hgx) dx -= k.gx;
+        if (dx < -k.hgx) dx += k.gx;
+        if (dy >= k.hgy) dy -= k.gy;
+        if (dy < -k.hgy) dy += k.gy;
+        ok = (dx + W > slo_x) && (dx < shi_x) && (dy + W > 0) && (dy < T);
+      }
+      const unsigned long long m = __ballot(ok);
+      const int nok = (int)__popcll(m);
+      if (ok) {
+        const int pos = (int)__popcll(m & lt);
+        const rec3<R> f = rec_f[j];
+#pragma unroll
+        for (int q = 0; q < PC; ++q) {
+          const int iw = p0 + q - sp;
+          s_kw[pos * PC + q] = (q < npl && iw >= 0 && iw < k.Ww)
+                                   ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
+                                   : (R)0;
+        }
+        for (int i = 0; i < W; ++i) {
+          s_kx[pos * W + i] = esk((f.x + (R)i) * scale, beta);
+          s_ky[pos * W + i] = esk((f.y + (R)i) * scale, beta);
+        }
+        for (int b = 0; b < k.nb; ++b) s_v[pos * 4 + b] = rec_v[(int64_t)j * k.nb + b];
+        s_dxy[pos] = (uint32_t)(dx + 32768) | ((uint32_t)(dy + 32768) << 16);
+      }
+      wave_sync();
+      auto gather_one = [&](int e, uint32_t d) {
+        const int rx = cx - ((int)(d & 0xffffu) - 32768);
+        if ((unsigned)rx < (unsigned)W) {
+          const R kx = s_kx[e * W + rx];
+          const int dyi = (int)(d >> 16) - 32768;
+          R kw[PC];
+#pragma unroll
+          for (int q = 0; q < PC; ++q) kw[q] = kx * s_kw[e * PC + q];
+          cplx<R> vv[kMaxBatch];
+#pragma unroll
+          for (int b = 0; b < kMaxBatch; ++b)
+            if (b < k.nb) vv[b] = s_v[e * 4 + b];
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const int ry = cy0 + c - dyi;
+            if ((unsigned)ry < (unsigned)W) {
+              const R ky = s_ky[e * W + ry];
+#pragma unroll
+              for (int q = 0; q < PC; ++q) {
+                const R wgt = kw[q] * ky;
+#pragma unroll
+                for (int b = 0; b < kMaxBatch; ++b) {
+                  if (b < k.nb) {
+                    ar[c][q][b] += wgt * vv[b].re;
+                    ai[c][q][b] += wgt * vv[b].im;
+                  }
+                }
+              }
+            }
+          }
+        }
+      };
+      int i = 0;
+      for (; i + 1 < nok; i += 2) {
+        const uint32_t d0 = s_dxy[i], d1 = s_dxy[i + 1];
+        gather_one(i, d0);
+        gather_one(i + 1, d1);
+      }
+      if (i < nok) gather_one(i, s_dxy[i]);
+      wave_sync();
     }
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
@@ -1645,6 +1852,11 @@ void set_lds_limits() {
   HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
   HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
 #undef HIPNG_LDS
+#define HIPNG_LDS(RR, TT, QQ) \
+  (void)hipFuncSetAttribute((const void*)k_gridw<RR, TT, QQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget)
+  HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
+  HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
+#undef HIPNG_LDS
   (void)hipGetLastError();
 }
 
@@ -1668,6 +1880,16 @@ struct Geom {
   int pc = 1;  // w planes per gridding pass / FFT batch
   Buffers buf;
 };
+
+// HIPNG_GRIDDER=block selects the workgroup-staged gridder instead of the wave-independent one
+bool grid_block_variant() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("HIPNG_GRIDDER");
+    v = (e && std::strcmp(e, "block") == 0) ? 1 : 0;
+  }
+  return v == 1;
+}
 
 int device_cus() {
   static int cus = 0;
@@ -1786,11 +2008,12 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
 }
 
 // per-plane task list: counters reset, per-tile task counts, exclusive scan -> task offsets
-void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st) {
+void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st, uint32_t mult = 1) {
   const int64_t ntiles = g.k.ntiles;
   k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, 2, 0u);
   LAUNCH_CHECK();
-  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, g.bstart, g.bend, chunk, g.task, g.counters);
+  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, g.bstart, g.bend, chunk, g.task, g.counters,
+                                            mult);
   LAUNCH_CHECK();
   const int64_t npart = (ntiles + kRsItems - 1) / kRsItems;
   k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
@@ -1875,8 +2098,26 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       PROF("grid");
       int slo, shi;
       chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
-      build_tasks(g, slo, shi, 4, g.chunk_grid, st);
+      // per-wave LDS: taps, values, offsets and segment table of 64 staged visibilities
+      const size_t region = (size_t)64 * (2 * P.W + PCn) * sizeof(R) + 64 * 4 * sizeof(cplx<R>) + 64 * 4 +
+                            (2 * kMaxSeg + 1) * 4 + 16;
+      const size_t smw = (kBlock / 64) * ((region + 15) / 16 * 16);
+      const bool wave = !grid_block_variant() && smw <= kLdsBudget;
+      build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? kBlock / 64 : 1);
       k_zero<cplx<R>><<<gridcap(npl * np1), kBlock, 0, st>>>(plane, npl * np1);
+      if (wave) {
+#define HIPNG_GW(TT, QQ)                                                                          \
+  k_gridw<R, TT, QQ><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,       \
+                                                      (const rec3<R>*)g.rec_f, recv, g.task,      \
+                                                      g.chunk_grid, g.counters, plane)
+        if (P.tile == 32) {
+          if (PCn == 2) HIPNG_GW(32, 2); else HIPNG_GW(32, 1);
+        } else {
+          if (PCn == 2) HIPNG_GW(16, 2); else HIPNG_GW(16, 1);
+        }
+#undef HIPNG_GW
+        LAUNCH_CHECK();
+      } else {
 #define HIPNG_GR(TT, QQ)                                                                          \
   k_grid<R, TT, QQ><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,         \
                                                     (const rec3<R>*)g.rec_f, recv, g.task,        \
@@ -1888,6 +2129,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       }
 #undef HIPNG_GR
       LAUNCH_CHECK();
+      }
     }
     {
       PROF("fft_c2c_inverse");
