@@ -959,20 +959,28 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p0, int npl, const ui
 }
 
 // ---------------------------------------------------------------------------------
-// Gridder, wave-independent variant: a task is (tile, chunk, column strip) and belongs to one
-// wave. The wave scans the chunk's candidates 64 at a time (one per lane: record load, test
-// against its T/4-column strip of the tile), compacts the overlapping ones into its private
-// LDS slots with the kernel taps evaluated once, then every lane gathers the staged
-// visibilities into its CPT cells x PC planes x nb in registers. Only wave-level ordering
-// is needed, so the four waves of a workgroup never wait for each other.
+// Gridder, sub-tile variant. A task is (tile, chunk, 8-column strip of the tile) and belongs
+// to one wave; the wave grids the strip as T/8 sub-tiles of 8 x 8 cells, one cell per lane,
+// all PC planes and nb polarisations of that cell in registers. Per sub-tile the wave
+// filters the chunk's candidates 64 at a time and queues the overlapping ones in LDS; each
+// full batch of 64 is staged with one visibility per lane (kernel taps for the sub-tile's
+// 8 columns and 8 rows, zero outside the support; w taps; values), then every lane
+// accumulates all staged visibilities into its cell without branches. Only wave-level
+// ordering is used, so the waves of a workgroup never wait for each other.
 // ---------------------------------------------------------------------------------
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
+// per-wave LDS bytes of k_grid8: values [64][4], taps [64][16 + PC], queue [3][128], segments
+__host__ __device__ constexpr size_t grid8_region(size_t rsz, int pc) {
+  return ((size_t)64 * 4 * 2 * rsz + (size_t)64 * (16 + pc) * rsz + 3 * 128 * 4 + (2 * kMaxSeg + 1) * 4 + 15) /
+         16 * 16;
+}
+
 template <typename R, int T, int PC>
-__global__ __launch_bounds__(kBlock) void k_gridw(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
+__global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                   const uint32_t* __restrict__ bend,
                                                   const uint32_t* __restrict__ rec_a,
                                                   const rec3<R>* __restrict__ rec_f,
@@ -980,29 +988,24 @@ __global__ __launch_bounds__(kBlock) void k_gridw(KP k, int p0, int npl, const u
                                                   const uint32_t* __restrict__ task_off, uint32_t chunk,
                                                   uint32_t* __restrict__ counters,
                                                   cplx<R>* __restrict__ plane) {
-  constexpr int NS = kBlock / 64;       // strips per tile = waves per workgroup
-  constexpr int CW = T / NS;            // columns per strip
-  constexpr int CPT = T * T / kBlock;   // cells per lane
-  constexpr int RUN = T / CPT;          // lanes per column
+  constexpr int NS = T / 8;   // strips per tile = sub-tiles per strip
+  constexpr int NT = 16 + PC;  // taps per staged visibility
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int W = k.W;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // per-wave region: taps x [64][W], taps y [64][W], w taps [64][PC], values [64][4],
-  // packed offsets [64], segment table
-  const size_t region = (size_t)64 * (2 * W + PC) * sizeof(R) + 64 * 4 * sizeof(cplx<R>) + 64 * 4 +
-                        (2 * kMaxSeg + 1) * 4 + 16;
-  unsigned char* base = smem_raw + wv * ((region + 15) / 16 * 16);
-  R* s_kx = reinterpret_cast<R*>(base);
-  R* s_ky = s_kx + 64 * W;
-  R* s_kw = s_ky + 64 * W;
-  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + 64 * PC);
-  uint32_t* s_dxy = reinterpret_cast<uint32_t*>(s_v + 64 * 4);
-  uint32_t* s_segv = s_dxy + 64;          // [kMaxSeg + 1]
+  unsigned char* base = smem_raw + wv * grid8_region(sizeof(R), PC);
+  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(base);  // [64][4]
+  R* s_t = reinterpret_cast<R*>(s_v + 64 * 4);        // [64][NT]
+  uint32_t* q_j = reinterpret_cast<uint32_t*>(s_t + 64 * NT);
+  uint32_t* q_d = q_j + 128;
+  uint32_t* q_s = q_d + 128;
+  uint32_t* s_segv = q_s + 128;             // [kMaxSeg + 1]
   uint32_t* s_segj = s_segv + kMaxSeg + 1;  // [kMaxSeg]
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const R scale = (R)2 / (R)W;
   const R beta = (R)k.beta;
   const uint32_t total = counters[1];
+  const int cx = lane >> 3, cy = lane & 7;
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const int nseg = (shi - slo + 1) * 4;
@@ -1014,18 +1017,17 @@ __global__ __launch_bounds__(kBlock) void k_gridw(KP k, int p0, int npl, const u
     const int tile = task_tile(task_off, (int)k.ntiles, task);
     const uint32_t local = task - task_off[tile];
     const uint32_t ci = local / NS;
-    const int strip = (int)(local - ci * NS);
+    const int sx0 = (int)(local - ci * NS) * 8;
     const uint32_t tnext = tile + 1 < (int)k.ntiles ? task_off[tile + 1] : total;
     const bool solo = tnext - task_off[tile] == (uint32_t)NS;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
-    // segment table: lanes < nseg load one bucket each, inclusive scan by shuffles
-    uint32_t len = 0, st = 0;
+    // segment table: one bucket per lane, inclusive scan by shuffles
     for (int sg0 = 0; sg0 < nseg; sg0 += 64) {
       const int sg = sg0 + lane;
-      len = 0;
+      uint32_t len = 0;
       if (sg < nseg) {
         const uint32_t key = seg_key(k, slo + sg / 4, tx, ty, sg & 3);
-        st = bstart[key];
+        const uint32_t st = bstart[key];
         len = bend[key] - st;
         s_segj[sg] = st;
       }
@@ -1042,112 +1044,125 @@ __global__ __launch_bounds__(kBlock) void k_gridw(KP k, int p0, int npl, const u
     }
     const uint32_t vlo = ci * chunk;
     const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
-    const int slo_x = strip * CW, shi_x = slo_x + CW;
-    const int cx = slo_x + lane / RUN, cy0 = (lane % RUN) * CPT;
-    R ar[CPT][PC][kMaxBatch], ai[CPT][PC][kMaxBatch];
-#pragma unroll
-    for (int c = 0; c < CPT; ++c)
+    for (int sy0 = 0; sy0 < T; sy0 += 8) {
+      R ar[PC][kMaxBatch], ai[PC][kMaxBatch];
 #pragma unroll
       for (int q = 0; q < PC; ++q)
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) ar[c][q][b] = ai[c][q][b] = 0;
-    for (uint32_t vb = vlo; vb < vhi; vb += 64) {
-      const uint32_t v = vb + (uint32_t)lane;
-      bool ok = false;
-      int dx = 0, dy = 0, sp = 0;
-      uint32_t j = 0;
-      if (v < vhi) {
-        int lo = 0, hi = nseg;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (s_segv[mid] <= v) lo = mid;
-          else hi = mid;
+        for (int b = 0; b < kMaxBatch; ++b) ar[q][b] = ai[q][b] = 0;
+      // stage queue entries [0, cnt) into slots [0, cnt) and accumulate them
+      auto stage_gather = [&](int cnt) {
+        if (lane < cnt) {
+          const uint32_t j = q_j[lane], d = q_d[lane];
+          const int sp = (int)q_s[lane];
+          const int dx = (int)(d & 0xffffu) - 32768, dy = (int)(d >> 16) - 32768;
+          const rec3<R> f = rec_f[j];
+          R* t = s_t + lane * NT;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int rx = sx0 + i - dx, ry = sy0 + i - dy;
+            t[i] = (unsigned)rx < (unsigned)W ? esk((f.x + (R)rx) * scale, beta) : (R)0;
+            t[8 + i] = (unsigned)ry < (unsigned)W ? esk((f.y + (R)ry) * scale, beta) : (R)0;
+          }
+#pragma unroll
+          for (int q = 0; q < PC; ++q) {
+            const int iw = p0 + q - sp;
+            t[16 + q] = (q < npl && iw >= 0 && iw < k.Ww)
+                            ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
+                            : (R)0;
+          }
+          for (int b = 0; b < k.nb; ++b) s_v[lane * 4 + b] = rec_v[(int64_t)j * k.nb + b];
         }
-        j = s_segj[lo] + (v - s_segv[lo]);
-        sp = slo + lo / 4;
-        const uint32_t A = rec_a[j];
-        dx = (int)(A & 0xffffu) - tx * T;
-        dy = (int)(A >> 16) - ty * T;
-        if (dx >= k.hgx) dx -= k.gx;
-        if (dx < -k.hgx) dx += k.gx;
-        if (dy >= k.hgy) dy -= k.gy;
-        if (dy < -k.hgy) dy += k.gy;
-        ok = (dx + W > slo_x) && (dx < shi_x) && (dy + W > 0) && (dy < T);
-      }
-      const unsigned long long m = __ballot(ok);
-      const int nok = (int)__popcll(m);
-      if (ok) {
-        const int pos = (int)__popcll(m & lt);
-        const rec3<R> f = rec_f[j];
+        wave_sync();
+        auto gather_one = [&](int e) {
+          const R* t = s_t + e * NT;
+          const R kxy = t[cx] * t[8 + cy];
+          R wq[PC];
 #pragma unroll
-        for (int q = 0; q < PC; ++q) {
-          const int iw = p0 + q - sp;
-          s_kw[pos * PC + q] = (q < npl && iw >= 0 && iw < k.Ww)
-                                   ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
-                                   : (R)0;
-        }
-        for (int i = 0; i < W; ++i) {
-          s_kx[pos * W + i] = esk((f.x + (R)i) * scale, beta);
-          s_ky[pos * W + i] = esk((f.y + (R)i) * scale, beta);
-        }
-        for (int b = 0; b < k.nb; ++b) s_v[pos * 4 + b] = rec_v[(int64_t)j * k.nb + b];
-        s_dxy[pos] = (uint32_t)(dx + 32768) | ((uint32_t)(dy + 32768) << 16);
-      }
-      wave_sync();
-      auto gather_one = [&](int e, uint32_t d) {
-        const int rx = cx - ((int)(d & 0xffffu) - 32768);
-        if ((unsigned)rx < (unsigned)W) {
-          const R kx = s_kx[e * W + rx];
-          const int dyi = (int)(d >> 16) - 32768;
-          R kw[PC];
+          for (int q = 0; q < PC; ++q) wq[q] = kxy * t[16 + q];
 #pragma unroll
-          for (int q = 0; q < PC; ++q) kw[q] = kx * s_kw[e * PC + q];
-          cplx<R> vv[kMaxBatch];
-#pragma unroll
-          for (int b = 0; b < kMaxBatch; ++b)
-            if (b < k.nb) vv[b] = s_v[e * 4 + b];
-#pragma unroll
-          for (int c = 0; c < CPT; ++c) {
-            const int ry = cy0 + c - dyi;
-            if ((unsigned)ry < (unsigned)W) {
-              const R ky = s_ky[e * W + ry];
+          for (int b = 0; b < kMaxBatch; ++b) {
+            if (b < k.nb) {
+              const cplx<R> vv = s_v[e * 4 + b];
 #pragma unroll
               for (int q = 0; q < PC; ++q) {
-                const R wgt = kw[q] * ky;
-#pragma unroll
-                for (int b = 0; b < kMaxBatch; ++b) {
-                  if (b < k.nb) {
-                    ar[c][q][b] += wgt * vv[b].re;
-                    ai[c][q][b] += wgt * vv[b].im;
-                  }
-                }
+                ar[q][b] += wq[q] * vv.re;
+                ai[q][b] += wq[q] * vv.im;
               }
             }
           }
+        };
+        int e = 0;
+        for (; e + 1 < cnt; e += 2) {
+          gather_one(e);
+          gather_one(e + 1);
         }
+        if (e < cnt) gather_one(e);
+        wave_sync();
       };
-      int i = 0;
-      for (; i + 1 < nok; i += 2) {
-        const uint32_t d0 = s_dxy[i], d1 = s_dxy[i + 1];
-        gather_one(i, d0);
-        gather_one(i + 1, d1);
+      int qn = 0;
+      for (uint32_t vb = vlo; vb < vhi; vb += 64) {
+        const uint32_t v = vb + (uint32_t)lane;
+        bool ok = false;
+        int dx = 0, dy = 0, sp = 0;
+        uint32_t j = 0;
+        if (v < vhi) {
+          int lo = 0, hi = nseg;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_segv[mid] <= v) lo = mid;
+            else hi = mid;
+          }
+          j = s_segj[lo] + (v - s_segv[lo]);
+          sp = slo + lo / 4;
+          const uint32_t A = rec_a[j];
+          dx = (int)(A & 0xffffu) - tx * T;
+          dy = (int)(A >> 16) - ty * T;
+          if (dx >= k.hgx) dx -= k.gx;
+          if (dx < -k.hgx) dx += k.gx;
+          if (dy >= k.hgy) dy -= k.gy;
+          if (dy < -k.hgy) dy += k.gy;
+          ok = (dx + W > sx0) && (dx < sx0 + 8) && (dy + W > sy0) && (dy < sy0 + 8);
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) {
+          const int pos = qn + (int)__popcll(m & lt);
+          q_j[pos] = j;
+          q_d[pos] = (uint32_t)(dx + 32768) | ((uint32_t)(dy + 32768) << 16);
+          q_s[pos] = (uint32_t)sp;
+        }
+        qn += (int)__popcll(m);
+        wave_sync();
+        if (qn >= 64) {
+          stage_gather(64);
+          qn -= 64;
+          uint32_t a0 = 0, a1 = 0, a2 = 0;
+          if (lane < qn) {
+            a0 = q_j[64 + lane];
+            a1 = q_d[64 + lane];
+            a2 = q_s[64 + lane];
+          }
+          wave_sync();
+          if (lane < qn) {
+            q_j[lane] = a0;
+            q_d[lane] = a1;
+            q_s[lane] = a2;
+          }
+          wave_sync();
+        }
       }
-      if (i < nok) gather_one(i, s_dxy[i]);
-      wave_sync();
-    }
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) {
+      if (qn > 0) stage_gather(qn);
 #pragma unroll
       for (int q = 0; q < PC; ++q) {
 #pragma unroll
         for (int b = 0; b < kMaxBatch; ++b) {
-          if (b < k.nb && (ar[c][q][b] != (R)0 || ai[c][q][b] != (R)0)) {
-            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + cx) * k.gy + ty * T + cy0 + c;
+          if (b < k.nb && (ar[q][b] != (R)0 || ai[q][b] != (R)0)) {
+            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + sx0 + cx) * k.gy + ty * T + sy0 + cy;
             if (solo) {
-              *o = cplx<R>{ar[c][q][b], ai[c][q][b]};
+              *o = cplx<R>{ar[q][b], ai[q][b]};
             } else {
-              atomicAdd(&o->re, ar[c][q][b]);
-              atomicAdd(&o->im, ai[c][q][b]);
+              atomicAdd(&o->re, ar[q][b]);
+              atomicAdd(&o->im, ai[q][b]);
             }
           }
         }
@@ -1853,7 +1868,7 @@ void set_lds_limits() {
   HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
 #undef HIPNG_LDS
 #define HIPNG_LDS(RR, TT, QQ) \
-  (void)hipFuncSetAttribute((const void*)k_gridw<RR, TT, QQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget)
+  (void)hipFuncSetAttribute((const void*)k_grid8<RR, TT, QQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget)
   HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
   HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
 #undef HIPNG_LDS
@@ -1881,7 +1896,7 @@ struct Geom {
   Buffers buf;
 };
 
-// HIPNG_GRIDDER=block selects the workgroup-staged gridder instead of the wave-independent one
+// HIPNG_GRIDDER=block selects the workgroup-staged gridder instead of the sub-tile one
 bool grid_block_variant() {
   static int v = -1;
   if (v < 0) {
@@ -2098,16 +2113,13 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       PROF("grid");
       int slo, shi;
       chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
-      // per-wave LDS: taps, values, offsets and segment table of 64 staged visibilities
-      const size_t region = (size_t)64 * (2 * P.W + PCn) * sizeof(R) + 64 * 4 * sizeof(cplx<R>) + 64 * 4 +
-                            (2 * kMaxSeg + 1) * 4 + 16;
-      const size_t smw = (kBlock / 64) * ((region + 15) / 16 * 16);
+      const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn);
       const bool wave = !grid_block_variant() && smw <= kLdsBudget;
-      build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? kBlock / 64 : 1);
+      build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? P.tile / 8 : 1);
       k_zero<cplx<R>><<<gridcap(npl * np1), kBlock, 0, st>>>(plane, npl * np1);
       if (wave) {
 #define HIPNG_GW(TT, QQ)                                                                          \
-  k_gridw<R, TT, QQ><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,       \
+  k_grid8<R, TT, QQ><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,       \
                                                       (const rec3<R>*)g.rec_f, recv, g.task,      \
                                                       g.chunk_grid, g.counters, plane)
         if (P.tile == 32) {
