@@ -11,16 +11,21 @@
 //   k_bounds       bucket [start, end) table from the sorted keys
 //   k_records      sorted, compact per-visibility records: packed first cell, FP32/FP64
 //                  kernel offsets and (for invert) the weighted, phase-shifted values
-//   k_grid         output-stationary tile gridder: one workgroup owns a T x T tile of
-//                  one w plane, accumulates the footprints of the buckets that reach it
-//                  in LDS and writes the tile once (no HBM atomics, no plane memset)
-//   rocFFT         in-place batched 2-D c2c transform of the plane
-//   k_screen_acc   fused w-screen exp(-2 pi i w_p (n-1)) * checkerboard * plane -> real
-//                  image accumulator (FP64 except in FP32 mode)
-//   k_finalize     grid correction 1/(Phi_x Phi_y Phi_w n), FP64
-//   k_prescale / k_screen_fill / k_degrid / k_unsort   the adjoint chain for predict;
-//                  k_degrid is a gather (one lane per visibility, LDS-resident tile
-//                  with halo), no atomics.
+//   k_tasks        per-tile task counts (chunks of a tile's candidate visibilities)
+//   k_grid8        sub-tile gridder: persistent waves pop (tile, chunk, 8-column strip)
+//                  tasks; each 8 x 8 sub-tile is one cell per lane, all planes of the
+//                  pass and all polarisations in registers; candidates queued in LDS and
+//                  staged 64 at a time; one store (or atomic, for split tiles) per cell
+//   k_grid         workgroup-staged gather gridder (kept as an alternative, HIPNG_GRIDDER=block)
+//   rocFFT         pruned plane transform: batched 1-D c2c along gy over the band of grid
+//                  rows footprints can touch, k_tr_in transpose of the image's ny columns,
+//                  batched 1-D c2c along gx (forward: the same in reverse order)
+//   k_screen_acc_t fused w-screen exp(-2 pi i w_p (n-1)) * checkerboard * plane -> real
+//                  image accumulator (FP64 except in FP32 mode), transposed image layout
+//   k_finalize_t   grid correction 1/(Phi_x Phi_y Phi_w n) in FP64 with the transpose back
+//   k_prescale_t / k_screen_fill_t / k_tr_out / k_degrid / k_unsort   the adjoint chain for
+//                  predict; k_degrid is a gather (one lane per visibility, LDS-resident
+//                  tile with halo), no atomics.
 //   k_dft          FP64 point-source DFT (dft.py:181-258 / :261-281 semantics)
 
 #include <hip/hip_runtime.h>
@@ -233,10 +238,9 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
     if (W < 0) continue;
     const double sigma = kSigmas[is];
     // tile: largest of {32, 16} whose degridder LDS tile with halo (T+W-1)^2 * batch fits
-    const size_t stage = 0;
     int T = 32;
-    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz + stage > (size_t)lds_budget) T = 16;
-    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz + stage > (size_t)lds_budget) {
+    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) T = 16;
+    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) {
       lds_limited = true;
       continue;
     }
@@ -255,13 +259,13 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
     const double ntiles = (double)(gx / T) * (gy / T);
     if ((double)nplanes * ntiles >= 4294967295.0) continue;
     // cost model, constants measured on MI355X at the production shape (mixed precision,
-    // 4 polarisations): gather gridder per visibility-plane-pol x halo overlap, degridder per
-    // cell-pol, plane FFT / fill / screen per grid cell-pol (both directions)
+    // 4 polarisations): sub-tile gridder per visibility-plane-pol x 8x8 sub-tiles touched,
+    // degridder per cell-pol, plane FFT / fill / screen per grid cell-pol (both directions)
     const double ww = wstack ? W : 1;
-    const double ov = (1.0 + (double)(W - 1) / T) * (1.0 + (double)(W - 1) / T);
+    const double sv = (1.0 + (double)(W - 1) / 8) * (1.0 + (double)(W - 1) / 8);
     const double g2 = (double)gx * gy;
     const double pf = prec == HIPNG_FP64 ? 2.0 : 1.0;
-    const double cost = pf * (0.7e-10 * (double)nitems * nbatch * ww * ov +
+    const double cost = pf * (1.85e-11 * (double)nitems * nbatch * ww * sv +
                               2.6e-13 * (double)nitems * nbatch * ww * W * W +
                               2.0 * 1.4e-11 * nplanes * nbatch * g2 +
                               2.0 * 1.5e-12 * nplanes * nbatch * (g2 + (double)nx * ny));
@@ -979,7 +983,7 @@ __host__ __device__ constexpr size_t grid8_region(size_t rsz, int pc) {
          16 * 16;
 }
 
-template <typename R, int T, int PC>
+template <typename R, int T, int PC, int NB>
 __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                   const uint32_t* __restrict__ bend,
                                                   const uint32_t* __restrict__ rec_a,
@@ -1045,11 +1049,11 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
     const uint32_t vlo = ci * chunk;
     const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
     for (int sy0 = 0; sy0 < T; sy0 += 8) {
-      R ar[PC][kMaxBatch], ai[PC][kMaxBatch];
+      R ar[PC][NB], ai[PC][NB];
 #pragma unroll
       for (int q = 0; q < PC; ++q)
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) ar[q][b] = ai[q][b] = 0;
+        for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
       // stage queue entries [0, cnt) into slots [0, cnt) and accumulate them
       auto stage_gather = [&](int cnt) {
         if (lane < cnt) {
@@ -1071,33 +1075,49 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
                             ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
                             : (R)0;
           }
-          for (int b = 0; b < k.nb; ++b) s_v[lane * 4 + b] = rec_v[(int64_t)j * k.nb + b];
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+            s_v[lane * 4 + b] = b < k.nb ? rec_v[(int64_t)j * k.nb + b] : cplx<R>{0, 0};
         }
         wave_sync();
-        auto gather_one = [&](int e) {
+        // all operands of an entry are read before any is used, two entries per round, no
+        // branches inside, so the LDS latency of one entry overlaps the arithmetic of the other
+        auto load = [&](int e, R& kxy, R (&wq)[PC], cplx<R> (&vv)[NB]) {
           const R* t = s_t + e * NT;
-          const R kxy = t[cx] * t[8 + cy];
-          R wq[PC];
+          const R kx = t[cx], ky = t[8 + cy];
+          R kw[PC];
 #pragma unroll
-          for (int q = 0; q < PC; ++q) wq[q] = kxy * t[16 + q];
+          for (int q = 0; q < PC; ++q) kw[q] = t[16 + q];
 #pragma unroll
-          for (int b = 0; b < kMaxBatch; ++b) {
-            if (b < k.nb) {
-              const cplx<R> vv = s_v[e * 4 + b];
+          for (int b = 0; b < NB; ++b) vv[b] = s_v[e * 4 + b];
+          kxy = kx * ky;
 #pragma unroll
-              for (int q = 0; q < PC; ++q) {
-                ar[q][b] += wq[q] * vv.re;
-                ai[q][b] += wq[q] * vv.im;
-              }
+          for (int q = 0; q < PC; ++q) wq[q] = kxy * kw[q];
+        };
+        auto accum = [&](const R (&wq)[PC], const cplx<R> (&vv)[NB]) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int q = 0; q < PC; ++q) {
+              ar[q][b] += wq[q] * vv[b].re;
+              ai[q][b] += wq[q] * vv[b].im;
             }
-          }
         };
         int e = 0;
         for (; e + 1 < cnt; e += 2) {
-          gather_one(e);
-          gather_one(e + 1);
+          R k0, k1, w0[PC], w1[PC];
+          cplx<R> v0[NB], v1[NB];
+          load(e, k0, w0, v0);
+          load(e + 1, k1, w1, v1);
+          accum(w0, v0);
+          accum(w1, v1);
         }
-        if (e < cnt) gather_one(e);
+        if (e < cnt) {
+          R k0, w0[PC];
+          cplx<R> v0[NB];
+          load(e, k0, w0, v0);
+          accum(w0, v0);
+        }
         wave_sync();
       };
       int qn = 0;
@@ -1155,7 +1175,7 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
 #pragma unroll
       for (int q = 0; q < PC; ++q) {
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) {
+        for (int b = 0; b < NB; ++b) {
           if (b < k.nb && (ar[q][b] != (R)0 || ai[q][b] != (R)0)) {
             cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + sx0 + cx) * k.gy + ty * T + sy0 + cy;
             if (solo) {
@@ -1195,30 +1215,6 @@ __device__ __forceinline__ double nm1s_at(int tx, int ty, const KP& k, double* n
 }
 
 // invert: acc[b][ix][iy] += Re(exp(-2 pi i w_p nm1s) (-1)^(tx+ty) plane[b][tx mod G][ty mod G])
-template <typename R, typename A, bool F32C>
-__global__ void k_screen_acc(KP k, int p0, int npl, const cplx<R>* __restrict__ plane, A* __restrict__ acc) {
-  const int iy = blockIdx.x * kBlock + threadIdx.x;
-  const int ix = blockIdx.y;
-  if (iy >= k.ny) return;
-  const int tx = ix - k.nx / 2, ty = iy - k.ny / 2;
-  const int gxi = tx < 0 ? tx + k.gx : tx, gyi = ty < 0 ? ty + k.gy : ty;
-  const double nm1s = k.wstack ? nm1s_at<F32C>(tx, ty, k, nullptr) : 0.0;
-  const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
-  A sum[kMaxBatch] = {0, 0, 0, 0};
-  for (int q = 0; q < npl; ++q) {
-    R c = 1, s = 0;
-    if (k.wstack) phase_cs<R, F32C>(-(k.w0 + (p0 + q) * k.dw) * nm1s, &c, &s);
-#pragma unroll
-    for (int b = 0; b < kMaxBatch; ++b) {
-      if (b < k.nb) {
-        const cplx<R> F = plane[((int64_t)(q * k.nb + b) * k.gx + gxi) * k.gy + gyi];
-        sum[b] += (A)(F.re * c - F.im * s);
-      }
-    }
-  }
-  for (int b = 0; b < k.nb; ++b) acc[((int64_t)b * k.nx + ix) * k.ny + iy] += (A)sign * sum[b];
-}
-
 // 1 / (Phi_x Phi_y [Phi_w(dw nm1s) n]) at pixel (ix, iy)
 template <typename A>
 __device__ __forceinline__ A corr_at(int ix, int iy, const KP& k, const double* __restrict__ phix,
@@ -1235,66 +1231,189 @@ __device__ __forceinline__ A corr_at(int ix, int iy, const KP& k, const double* 
   return (A)(1.0 / den);
 }
 
-template <typename A, typename O>
-__global__ void k_finalize(KP k, const A* __restrict__ acc, O* __restrict__ dirty,
-                           const double* __restrict__ phix, const double* __restrict__ phiy,
-                           const double* __restrict__ qz, const double* __restrict__ qc, int nq) {
-  const int iy = blockIdx.x * kBlock + threadIdx.x;
-  const int ix = blockIdx.y;
-  if (iy >= k.ny) return;
-  const A f = corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq);
-  for (int b = 0; b < k.nb; ++b) {
-    const int64_t e = ((int64_t)b * k.nx + ix) * k.ny + iy;
-    dirty[e] = (O)(acc[e] * f);
+// ---------------------------------------------------------------------------------
+// Pruned plane transforms. A plane's 2-D FFT runs as two batched 1-D passes with a
+// transpose between them, skipping what is known to be zero or unused: grid rows outside
+// the band [xlo, xhi) that visibility footprints can reach are zero on the gridding side and
+// unused on the degridding side, and only the image's ny columns of the other axis are kept.
+// The image-side buffer T holds, per plane, ny rows of gx complex values: T[i][iy][gxi].
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int img_col_to_grid(int iy, int ny, int gy) {  // image column -> grid column
+  const int h = ny / 2;
+  return iy < h ? gy - h + iy : iy - h;
+}
+__device__ __forceinline__ int grid_col_to_img(int c, int ny, int gy) {  // -1 outside the image
+  const int h = ny / 2;
+  if (c < h) return c + h;
+  if (c >= gy - h) return c - gy + h;
+  return -1;
+}
+
+// T[i][iy][x] = plane[i][x][col(iy)] for x inside the band, 0 outside; 32 x 32 tiles
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_tr_in(KP k, int xlo, int xhi, const cplx<R>* __restrict__ plane,
+                                                  cplx<R>* __restrict__ tb) {
+  __shared__ cplx<R> t[32][33];
+  const int i = blockIdx.z;
+  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  const bool any = x0 + 32 > xlo && x0 < xhi;
+  if (any) {
+    for (int r = ly; r < 32; r += 8) {
+      const int x = x0 + r, iy = y0 + lx;
+      cplx<R> v{0, 0};
+      if (x >= xlo && x < xhi && iy < k.ny) v = plane[((int64_t)i * k.gx + x) * k.gy + img_col_to_grid(iy, k.ny, k.gy)];
+      t[r][lx] = v;
+    }
+  }
+  __syncthreads();
+  for (int r = ly; r < 32; r += 8) {
+    const int iy = y0 + r, x = x0 + lx;
+    if (iy < k.ny && x < k.gx) tb[((int64_t)i * k.ny + iy) * k.gx + x] = any ? t[lx][r] : cplx<R>{0, 0};
   }
 }
 
-// predict: dc = dirty * correction
-template <typename I, typename A>
-__global__ void k_prescale(KP k, const I* __restrict__ dirty, A* __restrict__ dc,
-                           const double* __restrict__ phix, const double* __restrict__ phiy,
-                           const double* __restrict__ qz, const double* __restrict__ qc, int nq) {
-  const int iy = blockIdx.x * kBlock + threadIdx.x;
-  const int ix = blockIdx.y;
-  if (iy >= k.ny) return;
-  const A f = corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq);
-  for (int b = 0; b < k.nb; ++b) {
-    const int64_t e = ((int64_t)b * k.nx + ix) * k.ny + iy;
-    dc[e] = (A)dirty[e] * f;
+// band rows: plane[i][x][c] = T[i][img(c)][x] for image columns c, 0 for the others
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_tr_out(KP k, int xlo, int xhi, const cplx<R>* __restrict__ tb,
+                                                   cplx<R>* __restrict__ plane) {
+  __shared__ cplx<R> t[32][33];
+  const int i = blockIdx.z;
+  const int x0 = xlo + blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  for (int r = ly; r < 32; r += 8) {
+    const int c = c0 + r, x = x0 + lx;
+    cplx<R> v{0, 0};
+    if (c < k.gy && x < xhi) {
+      const int iy = grid_col_to_img(c, k.ny, k.gy);
+      if (iy >= 0) v = tb[((int64_t)i * k.ny + iy) * k.gx + x];
+    }
+    t[r][lx] = v;
+  }
+  __syncthreads();
+  for (int r = ly; r < 32; r += 8) {
+    const int x = x0 + r, c = c0 + lx;
+    if (x < xhi && c < k.gy) plane[((int64_t)i * k.gx + x) * k.gy + c] = t[lx][r];
   }
 }
 
-// predict: plane[b][g] = (-1)^(tx+ty) dc[b][t] exp(+2 pi i w_p nm1s(t)) for image pixels, 0 elsewhere
+// invert: accT[b][iy][ix] += (-1)^(tx+ty) sum_q Re(T_q[b][iy][gxi] exp(-2 pi i w_q nm1s))
 template <typename R, typename A, bool F32C>
-__global__ void k_screen_fill(KP k, int p0, int npl, const A* __restrict__ dc, cplx<R>* __restrict__ plane) {
-  const int gyi = blockIdx.x * kBlock + threadIdx.x;
-  const int gxi = blockIdx.y;
-  if (gyi >= k.gy) return;
-  const int hx = k.nx / 2, hy = k.ny / 2;
-  int tx, ty;
+__global__ void k_screen_acc_t(KP k, int p0, int npl, const cplx<R>* __restrict__ tb, A* __restrict__ acc) {
+  const int ix = blockIdx.x * kBlock + threadIdx.x;
+  const int iy = blockIdx.y;
+  if (ix >= k.nx) return;
+  const int tx = ix - k.nx / 2, ty = iy - k.ny / 2;
+  const int gxi = tx < 0 ? tx + k.gx : tx;
+  const double nm1s = k.wstack ? nm1s_at<F32C>(tx, ty, k, nullptr) : 0.0;
+  const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
+  A sum[kMaxBatch] = {0, 0, 0, 0};
+  for (int q = 0; q < npl; ++q) {
+    R c = 1, s = 0;
+    if (k.wstack) phase_cs<R, F32C>(-(k.w0 + (p0 + q) * k.dw) * nm1s, &c, &s);
+#pragma unroll
+    for (int b = 0; b < kMaxBatch; ++b) {
+      if (b < k.nb) {
+        const cplx<R> F = tb[((int64_t)(q * k.nb + b) * k.ny + iy) * k.gx + gxi];
+        sum[b] += (A)(F.re * c - F.im * s);
+      }
+    }
+  }
+  for (int b = 0; b < k.nb; ++b) acc[((int64_t)b * k.ny + iy) * k.nx + ix] += (A)sign * sum[b];
+}
+
+// predict: T_q[b][iy][gxi] = (-1)^(tx+ty) dcT[b][iy][ix] exp(+2 pi i w_q nm1s) on image rows, 0 elsewhere
+template <typename R, typename A, bool F32C>
+__global__ void k_screen_fill_t(KP k, int p0, int npl, const A* __restrict__ dc, cplx<R>* __restrict__ tb) {
+  const int gxi = blockIdx.x * kBlock + threadIdx.x;
+  const int iy = blockIdx.y;
+  if (gxi >= k.gx) return;
+  const int hx = k.nx / 2;
+  int tx = 0;
   bool in = true;
   if (gxi < hx) tx = gxi;
   else if (gxi >= k.gx - hx) tx = gxi - k.gx;
-  else { tx = 0; in = false; }
-  if (gyi < hy) ty = gyi;
-  else if (gyi >= k.gy - hy) ty = gyi - k.gy;
-  else { ty = 0; in = false; }
+  else in = false;
   if (!in) {
-    for (int q = 0; q < npl * k.nb; ++q) plane[((int64_t)q * k.gx + gxi) * k.gy + gyi] = cplx<R>{0, 0};
+    for (int q = 0; q < npl * k.nb; ++q) tb[((int64_t)q * k.ny + iy) * k.gx + gxi] = cplx<R>{0, 0};
     return;
   }
+  const int ty = iy - k.ny / 2;
   const double nm1s = k.wstack ? nm1s_at<F32C>(tx, ty, k, nullptr) : 0.0;
   const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
-  const int ix = tx + hx, iy = ty + hy;
+  const int ix = tx + hx;
   R d[kMaxBatch];
 #pragma unroll
-  for (int b = 0; b < kMaxBatch; ++b)
-    d[b] = b < k.nb ? (R)dc[((int64_t)b * k.nx + ix) * k.ny + iy] * sign : (R)0;
+  for (int b = 0; b < kMaxBatch; ++b) d[b] = b < k.nb ? (R)dc[((int64_t)b * k.ny + iy) * k.nx + ix] * sign : (R)0;
   for (int q = 0; q < npl; ++q) {
     R c = 1, s = 0;
     if (k.wstack) phase_cs<R, F32C>((k.w0 + (p0 + q) * k.dw) * nm1s, &c, &s);
     for (int b = 0; b < k.nb; ++b)
-      plane[((int64_t)(q * k.nb + b) * k.gx + gxi) * k.gy + gyi] = cplx<R>{d[b] * c, d[b] * s};
+      tb[((int64_t)(q * k.nb + b) * k.ny + iy) * k.gx + gxi] = cplx<R>{d[b] * c, d[b] * s};
+  }
+}
+
+// dirty[b][ix][iy] = accT[b][iy][ix] * correction(ix, iy); 32 x 32 tiles through LDS
+template <typename A, typename O>
+__global__ __launch_bounds__(kBlock) void k_finalize_t(KP k, const A* __restrict__ acc, O* __restrict__ dirty,
+                                                       const double* __restrict__ phix,
+                                                       const double* __restrict__ phiy,
+                                                       const double* __restrict__ qz,
+                                                       const double* __restrict__ qc, int nq) {
+  __shared__ A t[32][33];
+  const int ix0 = blockIdx.x * 32, iy0 = blockIdx.y * 32;
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  A f[4];
+#pragma unroll
+  for (int r4 = 0; r4 < 4; ++r4) {
+    const int ix = ix0 + ly + 8 * r4, iy = iy0 + lx;
+    f[r4] = (ix < k.nx && iy < k.ny) ? corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq) : (A)0;
+  }
+  for (int b = 0; b < k.nb; ++b) {
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) {
+      const int iy = iy0 + ly + 8 * r4, ix = ix0 + lx;
+      t[ly + 8 * r4][lx] = (ix < k.nx && iy < k.ny) ? acc[((int64_t)b * k.ny + iy) * k.nx + ix] : (A)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) {
+      const int ix = ix0 + ly + 8 * r4, iy = iy0 + lx;
+      if (ix < k.nx && iy < k.ny) dirty[((int64_t)b * k.nx + ix) * k.ny + iy] = (O)(t[lx][ly + 8 * r4] * f[r4]);
+    }
+    __syncthreads();
+  }
+}
+
+// dcT[b][iy][ix] = dirty[b][ix][iy] * correction(ix, iy)
+template <typename I, typename A>
+__global__ __launch_bounds__(kBlock) void k_prescale_t(KP k, const I* __restrict__ dirty, A* __restrict__ dc,
+                                                       const double* __restrict__ phix,
+                                                       const double* __restrict__ phiy,
+                                                       const double* __restrict__ qz,
+                                                       const double* __restrict__ qc, int nq) {
+  __shared__ A t[32][33];
+  const int ix0 = blockIdx.x * 32, iy0 = blockIdx.y * 32;
+  const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+  A f[4];
+#pragma unroll
+  for (int r4 = 0; r4 < 4; ++r4) {
+    const int ix = ix0 + ly + 8 * r4, iy = iy0 + lx;
+    f[r4] = (ix < k.nx && iy < k.ny) ? corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq) : (A)0;
+  }
+  for (int b = 0; b < k.nb; ++b) {
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) {
+      const int ix = ix0 + ly + 8 * r4, iy = iy0 + lx;
+      t[ly + 8 * r4][lx] = (ix < k.nx && iy < k.ny) ? (A)dirty[((int64_t)b * k.nx + ix) * k.ny + iy] * f[r4] : (A)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) {
+      const int iy = iy0 + ly + 8 * r4, ix = ix0 + lx;
+      if (ix < k.nx && iy < k.ny) dc[((int64_t)b * k.ny + iy) * k.nx + ix] = t[lx][ly + 8 * r4];
+    }
+    __syncthreads();
   }
 }
 
@@ -1606,7 +1725,7 @@ void prof_flush() {
 // ---------------------------------------------------------------------------------
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
-  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_NSLOTS
+  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_NSLOTS
 };
 
 struct FftKey {
@@ -1648,36 +1767,24 @@ void* ws(DevCtx& c, Slot s, size_t bytes) {
   return c.buf[s];
 }
 
-rocfft_plan fft_plan(DevCtx& c, int gx, int gy, int nb, int prec, int fwd) {
+// batched in-place 1-D transforms of `len` contiguous points, batches `len` apart
+rocfft_plan fft_plan1(DevCtx& c, int len, int batch, int prec, int fwd) {
   if (!g_fft_ready) {
     FFT_TRY(rocfft_setup());
     g_fft_ready = true;
   }
-  FftKey key{gx, gy, nb, prec == HIPNG_FP64 ? 0 : 1, fwd};
+  FftKey key{len, batch, 1, prec == HIPNG_FP64 ? 0 : 1, fwd};
   auto it = c.plans.find(key);
   if (it != c.plans.end()) return it->second;
   rocfft_plan plan = nullptr;
-  const size_t lengths[2] = {(size_t)gy, (size_t)gx};  // fastest dimension first
+  const size_t lengths[1] = {(size_t)len};
   FFT_TRY(rocfft_plan_create(&plan, rocfft_placement_inplace,
                              fwd ? rocfft_transform_type_complex_forward
                                  : rocfft_transform_type_complex_inverse,
                              prec == HIPNG_FP64 ? rocfft_precision_double : rocfft_precision_single,
-                             2, lengths, (size_t)nb, nullptr));
+                             1, lengths, (size_t)batch, nullptr));
   c.plans[key] = plan;
   return plan;
-}
-
-void fft_exec(DevCtx& c, rocfft_plan plan, void* data, hipStream_t st) {
-  size_t wsz = 0;
-  FFT_TRY(rocfft_plan_get_work_buffer_size(plan, &wsz));
-  rocfft_execution_info info = nullptr;
-  FFT_TRY(rocfft_execution_info_create(&info));
-  FFT_TRY(rocfft_execution_info_set_stream(info, st));
-  if (wsz) FFT_TRY(rocfft_execution_info_set_work_buffer(info, ws(c, S_FFTWORK, wsz), wsz));
-  void* bufs[1] = {data};
-  rocfft_status e = rocfft_execute(plan, bufs, nullptr, info);
-  rocfft_execution_info_destroy(info);
-  if (e != rocfft_status_success) HIPNG_THROW(HIPNG_ERUNTIME, "rocfft_execute failed: %d", (int)e);
 }
 
 unsigned grid1(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1867,11 +1974,14 @@ void set_lds_limits() {
   HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
   HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
 #undef HIPNG_LDS
-#define HIPNG_LDS(RR, TT, QQ) \
-  (void)hipFuncSetAttribute((const void*)k_grid8<RR, TT, QQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget)
+#define HIPNG_LDS1(RR, TT, QQ, BB)                                                                  \
+  (void)hipFuncSetAttribute((const void*)k_grid8<RR, TT, QQ, BB>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                            (int)kLdsBudget)
+#define HIPNG_LDS(RR, TT, QQ) HIPNG_LDS1(RR, TT, QQ, 1); HIPNG_LDS1(RR, TT, QQ, 2); HIPNG_LDS1(RR, TT, QQ, 4)
   HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
   HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
 #undef HIPNG_LDS
+#undef HIPNG_LDS1
   (void)hipGetLastError();
 }
 
@@ -1888,7 +1998,10 @@ struct Geom {
   uint32_t *perm = nullptr, *bstart = nullptr, *bend = nullptr, *rec_a = nullptr;
   void *rec_f = nullptr, *plane = nullptr, *accimg = nullptr, *recv = nullptr, *fftwork = nullptr;
   size_t fftwork_size = 0;
-  rocfft_plan fft_inv = nullptr, fft_fwd = nullptr;
+  // pruned plane transforms: grid rows [xlo, xhi) (band, along gy), image side T (along gx)
+  rocfft_plan fft_b_inv = nullptr, fft_b_fwd = nullptr, fft_t_inv = nullptr, fft_t_fwd = nullptr;
+  int xlo = 0, xhi = 0;
+  void* tbuf = nullptr;
   uint32_t *task = nullptr, *taskpart = nullptr, *counters = nullptr;
   uint32_t chunk_grid = 1024, chunk_degrid = 1024;
   int npersist = 1024;
@@ -2004,12 +2117,30 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     }
   }
   g.plane = g.buf.get(S_PLANE, g.pc * plane_bytes);
-  g.fft_inv = fft_plan(c, P.gx, P.gy, nb * g.pc, prec, 0);
-  g.fft_fwd = fft_plan(c, P.gx, P.gy, nb * g.pc, prec, 1);
-  size_t w1 = 0, w2 = 0;
-  FFT_TRY(rocfft_plan_get_work_buffer_size(g.fft_inv, &w1));
-  FFT_TRY(rocfft_plan_get_work_buffer_size(g.fft_fwd, &w2));
-  g.fftwork_size = std::max(w1, w2);
+  // band of grid rows visibility footprints can touch: |u| sx <= umax px gx around gx/2, plus
+  // the kernel half-width and two rows of rounding margin; the whole grid when it would wrap
+  {
+    const double U = P.umax * P.pixsize_x * P.gx;
+    const double lo = std::floor(0.5 * P.gx - U - 0.5 * P.W - 2.0);
+    const double hi = std::ceil(0.5 * P.gx + U + 0.5 * P.W + 2.0) + 1.0;
+    g.xlo = lo < 0.0 ? 0 : (int)lo;
+    g.xhi = hi > P.gx ? P.gx : (int)hi;
+    if (lo < 0.0 || hi > P.gx) {
+      g.xlo = 0;
+      g.xhi = P.gx;
+    }
+  }
+  g.tbuf = g.buf.get(S_TBUF, (size_t)g.pc * nb * P.ny * P.gx * csz);
+  g.fft_b_inv = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 0);
+  g.fft_b_fwd = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1);
+  g.fft_t_inv = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 0);
+  g.fft_t_fwd = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 1);
+  g.fftwork_size = 0;
+  for (rocfft_plan pl : {g.fft_b_inv, g.fft_b_fwd, g.fft_t_inv, g.fft_t_fwd}) {
+    size_t w = 0;
+    FFT_TRY(rocfft_plan_get_work_buffer_size(pl, &w));
+    g.fftwork_size = std::max(g.fftwork_size, w);
+  }
   if (g.fftwork_size) g.fftwork = g.buf.get(S_FFTWORK, g.fftwork_size);
   const int64_t ntiles = (int64_t)P.ntx * P.nty;
   g.task = (uint32_t*)g.buf.get(S_TASK, ntiles * 4);
@@ -2039,13 +2170,13 @@ void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStr
   LAUNCH_CHECK();
 }
 
-void geom_fft(const Geom& g, rocfft_plan plan, hipStream_t st) {
+void geom_fft(const Geom& g, rocfft_plan plan, void* data, hipStream_t st) {
   rocfft_execution_info info = nullptr;
   FFT_TRY(rocfft_execution_info_create(&info));
   rocfft_status e = rocfft_execution_info_set_stream(info, st);
   if (e == rocfft_status_success && g.fftwork_size)
     e = rocfft_execution_info_set_work_buffer(info, g.fftwork, g.fftwork_size);
-  void* bufs[1] = {g.plane};
+  void* bufs[1] = {data};
   if (e == rocfft_status_success) e = rocfft_execute(plan, bufs, nullptr, info);
   rocfft_execution_info_destroy(info);
   if (e != rocfft_status_success) HIPNG_THROW(HIPNG_ERUNTIME, "rocfft failed: %d", (int)e);
@@ -2099,13 +2230,13 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   k_zero<A><<<gridcap(nimg), kBlock, 0, st>>>(acc, nimg);
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
+  auto* tbuf = (cplx<R>*)g.tbuf;
   const int PCn = g.pc;
   // staged candidates per round: 256 unless the staging arrays would pass 48 KiB of LDS
   const size_t per = 2 * (size_t)P.W * sizeof(R) + PCn * sizeof(R) + 4 * sizeof(cplx<R>) +
                      sizeof(uint32_t) + (kBlock / 64) * sizeof(uint16_t);
   const int S = (int)std::min<size_t>(kBlock, (48 * 1024 / per) / 64 * 64);
   const size_t sm = per * S;
-  const dim3 ig(grid1(P.ny), P.nx);
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
@@ -2116,17 +2247,25 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn);
       const bool wave = !grid_block_variant() && smw <= kLdsBudget;
       build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? P.tile / 8 : 1);
-      k_zero<cplx<R>><<<gridcap(npl * np1), kBlock, 0, st>>>(plane, npl * np1);
+      // footprints stay inside the band, so only its rows need clearing
+      const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
+      for (int i = 0; i < npl * g.nb; ++i)
+        k_zero<cplx<R>><<<gridcap(nband), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nband);
       if (wave) {
-#define HIPNG_GW(TT, QQ)                                                                          \
-  k_grid8<R, TT, QQ><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,       \
-                                                      (const rec3<R>*)g.rec_f, recv, g.task,      \
-                                                      g.chunk_grid, g.counters, plane)
+#define HIPNG_GW(TT, QQ, BB)                                                                      \
+  k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,   \
+                                                          (const rec3<R>*)g.rec_f, recv, g.task,  \
+                                                          g.chunk_grid, g.counters, plane)
+#define HIPNG_GWB(TT, QQ)                 \
+  if (g.nb == 1) HIPNG_GW(TT, QQ, 1);     \
+  else if (g.nb == 2) HIPNG_GW(TT, QQ, 2); \
+  else HIPNG_GW(TT, QQ, 4)
         if (P.tile == 32) {
-          if (PCn == 2) HIPNG_GW(32, 2); else HIPNG_GW(32, 1);
+          if (PCn == 2) { HIPNG_GWB(32, 2); } else { HIPNG_GWB(32, 1); }
         } else {
-          if (PCn == 2) HIPNG_GW(16, 2); else HIPNG_GW(16, 1);
+          if (PCn == 2) { HIPNG_GWB(16, 2); } else { HIPNG_GWB(16, 1); }
         }
+#undef HIPNG_GWB
 #undef HIPNG_GW
         LAUNCH_CHECK();
       } else {
@@ -2145,15 +2284,26 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     }
     {
       PROF("fft_c2c_inverse");
-      geom_fft(g, g.fft_inv, st);
+      for (int i = 0; i < npl * g.nb; ++i)
+        geom_fft(g, g.fft_b_inv, plane + ((int64_t)i * P.gx + g.xlo) * P.gy, st);
+    }
+    {
+      PROF("transpose");
+      k_tr_in<R><<<dim3((P.gx + 31) / 32, (P.ny + 31) / 32, npl * g.nb), kBlock, 0, st>>>(k, g.xlo, g.xhi, plane,
+                                                                                       tbuf);
+      LAUNCH_CHECK();
+    }
+    {
+      PROF("fft_c2c_inverse");
+      geom_fft(g, g.fft_t_inv, tbuf, st);
     }
     PROF("screen_acc");
-    k_screen_acc<R, A, F32C><<<ig, kBlock, 0, st>>>(k, p0, npl, plane, acc);
+    k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, p0, npl, tbuf, acc);
     LAUNCH_CHECK();
   }
   PROF("finalize");
-  k_finalize<A, O><<<ig, kBlock, 0, st>>>(k, acc, (O*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc,
-                                           g.tb.nq);
+  k_finalize_t<A, O><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
+      k, acc, (O*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
   LAUNCH_CHECK();
 }
 
@@ -2165,29 +2315,39 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   const KP& k = g.k;
   const hipng_plan& P = g.P;
   auto* dc = (A*)g.accimg;
-  const dim3 ig(grid1(P.ny), P.nx);
   {
     PROF("prescale");
-    k_prescale<I, A><<<ig, kBlock, 0, st>>>(k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz,
-                                            g.tb.qc, g.tb.nq);
+    k_prescale_t<I, A><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
+        k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
     LAUNCH_CHECK();
   }
   auto* accs = (cplx<R>*)g.recv;
   k_zero<cplx<R>><<<gridcap(g.n * g.nb), kBlock, 0, st>>>(accs, g.n * g.nb);
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
-  const dim3 gg(grid1(P.gy), P.gx);
+  auto* tbuf = (cplx<R>*)g.tbuf;
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
     const int npl = std::min(g.pc, P.nplanes - p0);
     {
       PROF("screen_fill");
-      k_screen_fill<R, A, F32C><<<gg, kBlock, 0, st>>>(k, p0, npl, dc, plane);
+      k_screen_fill_t<R, A, F32C><<<dim3(grid1(P.gx), P.ny), kBlock, 0, st>>>(k, p0, npl, dc, tbuf);
       LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_forward");
-      geom_fft(g, g.fft_fwd, st);
+      geom_fft(g, g.fft_t_fwd, tbuf, st);
+    }
+    {
+      PROF("transpose");
+      k_tr_out<R><<<dim3((g.xhi - g.xlo + 31) / 32, (P.gy + 31) / 32, npl * g.nb), kBlock, 0, st>>>(
+          k, g.xlo, g.xhi, tbuf, plane);
+      LAUNCH_CHECK();
+    }
+    {
+      PROF("fft_c2c_forward");
+      for (int i = 0; i < npl * g.nb; ++i)
+        geom_fft(g, g.fft_b_fwd, plane + ((int64_t)i * P.gx + g.xlo) * P.gy, st);
     }
     PROF("degrid");
     for (int q = 0; q < npl; ++q) launch_degrid<R>(g, p0 + q, plane + q * np1, accs, st);
