@@ -28,6 +28,8 @@ if ROOT not in sys.path:
 
 METRIC = "Mvis/s invert+predict, 8192² w-stacked, ε≤1e-5 vs FP64 DFT; FP32 vs FP64 mode"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
@@ -67,25 +69,55 @@ def workload(cfg, nchan_override):
                 desc="100 stations 1 km disc, 30 HA, dec -45, 1 ch 100 MHz, Stokes I; 512^2, cell 5e-4, eps 1e-7")
 
 
-def algorithmic_bytes(name, plan, nitems, nb, csz):
-    """HBM bytes one launch of kernel class `name` must move at minimum."""
+def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc):
+    """Minimum work of one timed step for kernel class `name`: (FLOP, HBM bytes).
+
+    FLOP follow the per-visibility counts of the gridding work model: 4 W^2 W_w (one complex x
+    real FMA pair per cell and w plane) + 3 W c_ker (kernel taps, c_ker = 10) per visibility and
+    direction; FFT 5 G^2 log2(G^2) per plane and polarisation. Bytes: each record is read once
+    per pass of pc planes its w window meets, plus the planes written (gridder) or read
+    (degridder, with the sorted accumulators read and written)."""
     G2 = plan["gx"] * plan["gy"]
     N2 = plan["nx"] * plan["ny"]
     P = plan["nplanes"]
-    ww = plan["W"] if plan["wstacking"] else 1
+    W = plan["W"]
+    ww = W if plan["wstacking"] else 1
     rsz = 4 if csz == 8 else 8
-    items_pp = nitems * ww / P
-    if name == "grid":  # records of the plane's visibilities in, plane out
-        return items_pp * (4 + 3 * rsz + nb * csz) + nb * G2 * csz
-    if name == "degrid":  # plane in, records in, sorted accumulators read+write
-        return nb * G2 * csz + items_pp * (4 + 3 * rsz + 2 * nb * csz)
-    if name.startswith("fft"):  # one in-place read + write of the plane
-        return 2 * nb * G2 * csz
-    if name == "screen_acc":  # plane pixels in, FP64 accumulator read + write
-        return nb * N2 * (csz + 16)
-    if name == "screen_fill":  # corrected image in (FP64), plane out
-        return nb * N2 * 8 + nb * G2 * csz
-    return None
+    passes = (ww + pc - 1) / pc if plan["wstacking"] else 1  # passes an item takes part in
+    if name == "grid":
+        return (nvis * (4.0 * W * W * ww + 30.0 * W),
+                nitems * passes * (4 + 3 * rsz + nb * csz) + P * nb * G2 * csz)
+    if name == "degrid":
+        return (nvis * (4.0 * W * W * ww + 30.0 * W),
+                P * nb * G2 * csz + nitems * ww * (4 + 3 * rsz + 2 * nb * csz))
+    if name.startswith("fft"):
+        return (nb * P * 5.0 * G2 * math.log2(G2), 2 * 2 * nb * P * G2 * csz)
+    if name == "screen_acc":
+        return (None, P * nb * N2 * (csz + 16))
+    if name == "screen_fill":
+        return (None, P * nb * (N2 * 8 + G2 * csz))
+    return (None, None)
+
+
+def measured_traffic(kernel, plan, prec, nb):
+    """HBM bytes per launch of kernel class `kernel` from the committed rocprofv3 --pmc
+    summaries (profiles/*_traffic.json): FETCH_SIZE and WRITE_SIZE per dispatch, each scaled
+    by what the counter reads on a plain device copy of known size. Only a summary measured
+    on the same plan (precision, batch, kernel support, grid, planes) is used."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json"))):
+        try:
+            with open(path) as f:
+                rec = json.load(f)
+        except (OSError, ValueError):
+            continue
+        pl = rec.get("plan", {})
+        if (rec.get("kernel_class") == kernel and pl.get("precision") == prec and pl.get("nb") == nb
+                and pl.get("W") == plan.get("W") and pl.get("grid") == plan.get("gx")
+                and pl.get("nplanes") == plan.get("nplanes")):
+            return rec["traffic_bytes_per_launch"], os.path.basename(path)
+    return None, None
 
 
 def cpu_plans(plan_i, npol, nitems):
@@ -559,23 +591,44 @@ def main():
     value = nvis / (ms_per_step * 1e-3) / 1e6
 
     csz = 16 if prec == "fp64" else 8
-    kern = []
-    for name, (tot, cnt) in prof.items():
-        plan = plan_p if name in ("degrid", "screen_fill", "fft_c2c_forward") else plan_i
-        b = algorithmic_bytes(name, plan, nrow * nchan, npol, csz)
-        kern.append((tot, name, cnt, b))
-    kern.sort(reverse=True)
+    fp_peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
+
+    def plan_of(name):
+        return plan_p if name in ("degrid", "screen_fill", "fft_c2c_forward") else plan_i
+
+    kern = sorted(((tot, name, cnt) for name, (tot, cnt) in prof.items()), reverse=True)
     roof = None
-    for tot, name, cnt, b in kern:
-        if b is not None:
-            avg_s = tot / cnt * 1e-3
-            ach = b / avg_s / 1e9
-            roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+    for tot, name, cnt in kern:
+        plan = plan_of(name)
+        per_step = cnt / args.steps
+        pc = max(1, round(plan["nplanes"] / per_step)) if name == "grid" else 1  # planes per pass
+        flop, nbytes = algorithmic_work(name, plan, nvis, nrow * nchan, npol, csz, pc)
+        if flop is None and nbytes is None:
+            continue
+        avg_s = tot / cnt * 1e-3
+        traffic, src = measured_traffic(name, plan, prec, npol)
+        hbm_view = {"achieved": round(nbytes / per_step / avg_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "algorithmic_bytes_per_launch": int(nbytes / per_step)}
+        hbm_view["frac"] = round(hbm_view["achieved"] / HBM_PEAK_GBS, 4)
+        if flop is not None:
+            # the gridders are FP32 (FP64) FMA-bound on the vector ALUs; MI355X's dense FP32
+            # peak is the same for vector and matrix units
+            ach = flop / per_step / avg_s / 1e12
+            roof = {"kernel": name, "bound": "mfma", "achieved": round(ach, 2), "peak": fp_peak,
+                    "unit": "TFLOP/s", "frac": round(ach / fp_peak, 4), "traffic": traffic,
                     "avg_launch_ms": round(tot / cnt, 4), "launches": cnt,
-                    "algorithmic_bytes_per_launch": int(b)}
-            break
-    breakdown = {name: {"total_ms": round(tot, 2), "launches": cnt} for tot, name, cnt, _ in kern}
+                    "algorithmic_flop_per_launch": int(flop / per_step),
+                    "compute": "vector FMA (no MFMA instructions); peak = dense FP%d" % (64 if prec == "fp64" else 32),
+                    "hbm_view": hbm_view}
+        else:
+            roof = {"kernel": name, "bound": "hbm", "achieved": hbm_view["achieved"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": hbm_view["frac"], "traffic": traffic,
+                    "avg_launch_ms": round(tot / cnt, 4), "launches": cnt,
+                    "algorithmic_bytes_per_launch": hbm_view["algorithmic_bytes_per_launch"]}
+        if src:
+            roof["traffic_source"] = "profiles/" + src
+        break
+    breakdown = {name: {"total_ms": round(tot, 2), "launches": cnt} for tot, name, cnt in kern}
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mvis/s", "n_gpus": 1,

@@ -41,6 +41,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hipng.h"
@@ -227,6 +228,11 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
   // HIPNG_SIGMA restricts the search to one oversampling factor (cost-model calibration)
   const char* force = std::getenv("HIPNG_SIGMA");
   const double forced = force ? std::atof(force) : 0.0;
+  // HIPNG_TILE=32 selects 32 x 32 sort tiles where they fit (tile-size calibration); 16 x 16
+  // tiles are the default: the sub-tile gridder scans a tile's candidates once per 8 x 8
+  // sub-tile, and a smaller tile has fewer candidates per sub-tile
+  const char* ftile = std::getenv("HIPNG_TILE");
+  const int forced_tile = ftile ? std::atoi(ftile) : 0;
   for (int is = 0; is < kNSig; ++is) {
     if (forced > 0.0 && std::fabs(kSigmas[is] - forced) > 1e-9) continue;
     int W = -1;
@@ -237,8 +243,8 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
       }
     if (W < 0) continue;
     const double sigma = kSigmas[is];
-    // tile: largest of {32, 16} whose degridder LDS tile with halo (T+W-1)^2 * batch fits
-    int T = 32;
+    // tile: 16, or 32 on request when the degridder's LDS tile with halo (T+W-1)^2 * batch fits
+    int T = forced_tile == 32 ? 32 : 16;
     if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) T = 16;
     if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) {
       lds_limited = true;
@@ -259,15 +265,16 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
     const double ntiles = (double)(gx / T) * (gy / T);
     if ((double)nplanes * ntiles >= 4294967295.0) continue;
     // cost model, constants measured on MI355X at the production shape (mixed precision,
-    // 4 polarisations): sub-tile gridder per visibility-plane-pol x 8x8 sub-tiles touched,
-    // degridder per cell-pol, plane FFT / fill / screen per grid cell-pol (both directions)
+    // 4 polarisations, 16 x 16 tiles): sub-tile gridder per visibility-plane-pol x 8x8
+    // sub-tiles touched, degridder per cell-pol, pruned plane FFT with its transposes and
+    // fill / screen per grid cell-pol (both directions)
     const double ww = wstack ? W : 1;
     const double sv = (1.0 + (double)(W - 1) / 8) * (1.0 + (double)(W - 1) / 8);
     const double g2 = (double)gx * gy;
     const double pf = prec == HIPNG_FP64 ? 2.0 : 1.0;
-    const double cost = pf * (1.85e-11 * (double)nitems * nbatch * ww * sv +
-                              2.6e-13 * (double)nitems * nbatch * ww * W * W +
-                              2.0 * 1.4e-11 * nplanes * nbatch * g2 +
+    const double cost = pf * (6.2e-12 * (double)nitems * nbatch * ww * sv +
+                              2.8e-13 * (double)nitems * nbatch * ww * W * W +
+                              2.0 * 7.4e-12 * nplanes * nbatch * g2 +
                               2.0 * 1.5e-12 * nplanes * nbatch * (g2 + (double)nx * ny));
     if (cost < best_cost) {
       best_cost = cost;
@@ -751,6 +758,8 @@ __device__ __forceinline__ int task_tile(const uint32_t* __restrict__ task_off, 
 // plane with one global atomic per non-zero cell (tiles can be split over tasks).
 // ---------------------------------------------------------------------------------
 constexpr int kMaxPC = 2;                    // w planes gridded per pass
+constexpr int kXcd = 8;                      // XCDs per device: work queues of the sub-tile gridder
+constexpr int kQueueBase = 8;                // counters[kQueueBase + x]: pops from queue x
 constexpr int kMaxSeg = 4 * (16 + kMaxPC - 1);  // 4 neighbour buckets x start planes
 
 // start planes whose W_w-plane window meets planes [p0, p0 + npl)
@@ -977,11 +986,21 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// per-wave LDS bytes of k_grid8: values [64][4], taps [64][16 + PC], queue [3][128], segments
-__host__ __device__ constexpr size_t grid8_region(size_t rsz, int pc) {
-  return ((size_t)64 * 4 * 2 * rsz + (size_t)64 * (16 + pc) * rsz + 3 * 128 * 4 + (2 * kMaxSeg + 1) * 4 + 15) /
-         16 * 16;
+// per-wave LDS bytes of k_grid8: values [NB][64], taps [64][(16 + PC) | 1], queue [3][128],
+// segments. The tap row stride is odd so that the 64 staging lanes write to distinct banks.
+__host__ __device__ constexpr int grid8_taps(int pc) { return (16 + pc) | 1; }
+__host__ __device__ constexpr size_t grid8_region(size_t rsz, int pc, int nb) {
+  return ((size_t)nb * 64 * 2 * rsz + (size_t)64 * grid8_taps(pc) * rsz + 3 * 128 * 4 + (2 * kMaxSeg + 1) * 4 +
+          15) / 16 * 16;
 }
+
+// exp(beta (sqrt(1 - z^2) - 1)) from the hardware base-2 exponential and square root
+// (FP32 kernel modes; beta2 = beta log2(e))
+__device__ __forceinline__ float esk_fast(float z, float beta2) {
+  const float t = fmaf(-z, z, 1.0f);
+  return t > 0.0f ? __builtin_amdgcn_exp2f(beta2 * (__builtin_amdgcn_sqrtf(t) - 1.0f)) : 0.0f;
+}
+__device__ __forceinline__ double esk_fast(double z, double beta) { return esk(z, beta); }
 
 template <typename R, int T, int PC, int NB>
 __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
@@ -992,14 +1011,14 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
                                                   const uint32_t* __restrict__ task_off, uint32_t chunk,
                                                   uint32_t* __restrict__ counters,
                                                   cplx<R>* __restrict__ plane) {
-  constexpr int NS = T / 8;   // strips per tile = sub-tiles per strip
-  constexpr int NT = 16 + PC;  // taps per staged visibility
+  constexpr int NS = T / 8;               // strips per tile = sub-tiles per strip
+  constexpr int NT = grid8_taps(PC);      // per staged visibility: 8 x taps, 8 y taps, PC w taps
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int W = k.W;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  unsigned char* base = smem_raw + wv * grid8_region(sizeof(R), PC);
-  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(base);  // [64][4]
-  R* s_t = reinterpret_cast<R*>(s_v + 64 * 4);        // [64][NT]
+  unsigned char* base = smem_raw + wv * grid8_region(sizeof(R), PC, NB);
+  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(base);  // [NB][64]
+  R* s_t = reinterpret_cast<R*>(s_v + NB * 64);       // [64][NT]
   uint32_t* q_j = reinterpret_cast<uint32_t*>(s_t + 64 * NT);
   uint32_t* q_d = q_j + 128;
   uint32_t* q_s = q_d + 128;
@@ -1007,17 +1026,34 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
   uint32_t* s_segj = s_segv + kMaxSeg + 1;  // [kMaxSeg]
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const R scale = (R)2 / (R)W;
-  const R beta = (R)k.beta;
+  // the FP32 tap evaluation works in base 2
+  const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
   const uint32_t total = counters[1];
   const int cx = lane >> 3, cy = lane & 7;
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const int nseg = (shi - slo + 1) * 4;
+  // XCD-local work queues: the NS strip tasks of a (tile, chunk) read the same records, so
+  // groups are dealt round-robin to kXcd queues and a wave pops from the queue of its own
+  // XCD (workgroups are placed on XCDs round-robin by index) before helping the others
+  const uint32_t ngroups = total / NS;
+  const int xcd0 = (int)(blockIdx.x % kXcd);
+  int qi = 0;
   for (;;) {
-    uint32_t task = 0;
-    if (lane == 0) task = atomicAdd(&counters[0], 1u);
-    task = __shfl(task, 0, 64);
-    if (task >= total) break;
+    uint32_t task = 0xffffffffu;
+    while (qi < kXcd) {
+      const int x = (xcd0 + qi) % kXcd;
+      const uint32_t qsize = ngroups > (uint32_t)x ? (ngroups - x + kXcd - 1) / kXcd : 0u;
+      uint32_t l = 0;
+      if (lane == 0) l = atomicAdd(&counters[kQueueBase + x], 1u);
+      l = __shfl(l, 0, 64);
+      if (l < qsize * NS) {
+        task = ((uint32_t)x + kXcd * (l / NS)) * NS + l % NS;
+        break;
+      }
+      ++qi;
+    }
+    if (task == 0xffffffffu) break;
     const int tile = task_tile(task_off, (int)k.ntiles, task);
     const uint32_t local = task - task_off[tile];
     const uint32_t ci = local / NS;
@@ -1056,43 +1092,40 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
         for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
       // stage queue entries [0, cnt) into slots [0, cnt) and accumulate them
       auto stage_gather = [&](int cnt) {
+        // staging: one queued visibility per lane; its values, the taps of the sub-tile's 8
+        // columns and 8 rows (zero outside the support) and its w taps for the pass's planes
         if (lane < cnt) {
           const uint32_t j = q_j[lane], d = q_d[lane];
           const int sp = (int)q_s[lane];
           const int dx = (int)(d & 0xffffu) - 32768, dy = (int)(d >> 16) - 32768;
           const rec3<R> f = rec_f[j];
-          R* t = s_t + lane * NT;
 #pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            const int rx = sx0 + i - dx, ry = sy0 + i - dy;
-            t[i] = (unsigned)rx < (unsigned)W ? esk((f.x + (R)rx) * scale, beta) : (R)0;
-            t[8 + i] = (unsigned)ry < (unsigned)W ? esk((f.y + (R)ry) * scale, beta) : (R)0;
-          }
+          for (int b = 0; b < NB; ++b) s_v[b * 64 + lane] = rec_v[(int64_t)j * NB + b];
+          R* t = s_t + lane * NT;
 #pragma unroll
           for (int q = 0; q < PC; ++q) {
             const int iw = p0 + q - sp;
             t[16 + q] = (q < npl && iw >= 0 && iw < k.Ww)
-                            ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
+                            ? (k.wstack ? esk_fast((f.w + (R)iw) * scale, beta) : (R)1)
                             : (R)0;
           }
 #pragma unroll
-          for (int b = 0; b < NB; ++b)
-            s_v[lane * 4 + b] = b < k.nb ? rec_v[(int64_t)j * k.nb + b] : cplx<R>{0, 0};
+          for (int i = 0; i < 8; ++i) {
+            const int rx = sx0 + i - dx, ry = sy0 + i - dy;
+            t[i] = (unsigned)rx < (unsigned)W ? esk_fast((f.x + (R)rx) * scale, beta) : (R)0;
+            t[8 + i] = (unsigned)ry < (unsigned)W ? esk_fast((f.y + (R)ry) * scale, beta) : (R)0;
+          }
         }
         wave_sync();
-        // all operands of an entry are read before any is used, two entries per round, no
-        // branches inside, so the LDS latency of one entry overlaps the arithmetic of the other
-        auto load = [&](int e, R& kxy, R (&wq)[PC], cplx<R> (&vv)[NB]) {
+        // gather: every operand of an entry is read before any is used, two entries per round,
+        // no branches, so the LDS latency of one entry overlaps the arithmetic of the other
+        auto load = [&](int e, R (&wq)[PC], cplx<R> (&vv)[NB]) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) vv[b] = s_v[b * 64 + e];
           const R* t = s_t + e * NT;
-          const R kx = t[cx], ky = t[8 + cy];
-          R kw[PC];
+          const R kxy = t[cx] * t[8 + cy];
 #pragma unroll
-          for (int q = 0; q < PC; ++q) kw[q] = t[16 + q];
-#pragma unroll
-          for (int b = 0; b < NB; ++b) vv[b] = s_v[e * 4 + b];
-          kxy = kx * ky;
-#pragma unroll
-          for (int q = 0; q < PC; ++q) wq[q] = kxy * kw[q];
+          for (int q = 0; q < PC; ++q) wq[q] = kxy * t[16 + q];
         };
         auto accum = [&](const R (&wq)[PC], const cplx<R> (&vv)[NB]) {
 #pragma unroll
@@ -1105,34 +1138,41 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
         };
         int e = 0;
         for (; e + 1 < cnt; e += 2) {
-          R k0, k1, w0[PC], w1[PC];
+          R w0[PC], w1[PC];
           cplx<R> v0[NB], v1[NB];
-          load(e, k0, w0, v0);
-          load(e + 1, k1, w1, v1);
+          load(e, w0, v0);
+          load(e + 1, w1, v1);
           accum(w0, v0);
           accum(w1, v1);
         }
         if (e < cnt) {
-          R k0, w0[PC];
+          R w0[PC];
           cplx<R> v0[NB];
-          load(e, k0, w0, v0);
+          load(e, w0, v0);
           accum(w0, v0);
         }
         wave_sync();
       };
       int qn = 0;
+      // segment of the chunk's first candidate by bisection, then walked forward block by block
+      // (a block of 64 consecutive candidates rarely crosses more than one segment boundary)
+      int cur = 0;
+      {
+        int hi = nseg;
+        while (hi - cur > 1) {
+          const int mid = (cur + hi) >> 1;
+          if (s_segv[mid] <= vlo) cur = mid;
+          else hi = mid;
+        }
+      }
       for (uint32_t vb = vlo; vb < vhi; vb += 64) {
         const uint32_t v = vb + (uint32_t)lane;
         bool ok = false;
         int dx = 0, dy = 0, sp = 0;
         uint32_t j = 0;
+        int lo = cur;  // last segment starting at or before v
         if (v < vhi) {
-          int lo = 0, hi = nseg;
-          while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_segv[mid] <= v) lo = mid;
-            else hi = mid;
-          }
+          while (lo + 1 < nseg && s_segv[lo + 1] <= v) ++lo;
           j = s_segj[lo] + (v - s_segv[lo]);
           sp = slo + lo / 4;
           const uint32_t A = rec_a[j];
@@ -1144,6 +1184,7 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
           if (dy < -k.hgy) dy += k.gy;
           ok = (dx + W > sx0) && (dx < sx0 + 8) && (dy + W > sy0) && (dy < sy0 + 8);
         }
+        cur = __shfl(lo, 63, 64);
         const unsigned long long m = __ballot(ok);
         if (ok) {
           const int pos = qn + (int)__popcll(m & lt);
@@ -1977,7 +2018,8 @@ void set_lds_limits() {
 #define HIPNG_LDS1(RR, TT, QQ, BB)                                                                  \
   (void)hipFuncSetAttribute((const void*)k_grid8<RR, TT, QQ, BB>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             (int)kLdsBudget)
-#define HIPNG_LDS(RR, TT, QQ) HIPNG_LDS1(RR, TT, QQ, 1); HIPNG_LDS1(RR, TT, QQ, 2); HIPNG_LDS1(RR, TT, QQ, 4)
+#define HIPNG_LDS(RR, TT, QQ) \
+  HIPNG_LDS1(RR, TT, QQ, 1); HIPNG_LDS1(RR, TT, QQ, 2); HIPNG_LDS1(RR, TT, QQ, 3); HIPNG_LDS1(RR, TT, QQ, 4)
   HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
   HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
 #undef HIPNG_LDS
@@ -2145,7 +2187,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   const int64_t ntiles = (int64_t)P.ntx * P.nty;
   g.task = (uint32_t*)g.buf.get(S_TASK, ntiles * 4);
   g.taskpart = (uint32_t*)g.buf.get(S_TASKPART, ((ntiles + kRsItems - 1) / kRsItems) * 4);
-  g.counters = (uint32_t*)g.buf.get(S_CNT, 16);
+  g.counters = (uint32_t*)g.buf.get(S_CNT, (kQueueBase + kXcd) * sizeof(uint32_t));
   g.npersist = device_cus() * 4;
   const double ww = P.wstacking ? P.W : 1;
   g.chunk_grid = pick_chunk((double)n * ww / P.nplanes * 4.0, g.npersist);
@@ -2156,7 +2198,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
 // per-plane task list: counters reset, per-tile task counts, exclusive scan -> task offsets
 void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st, uint32_t mult = 1) {
   const int64_t ntiles = g.k.ntiles;
-  k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, 2, 0u);
+  k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, kQueueBase + kXcd, 0u);
   LAUNCH_CHECK();
   k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, g.bstart, g.bend, chunk, g.task, g.counters,
                                             mult);
@@ -2188,9 +2230,6 @@ void launch_degrid(const Geom& g, int p, const cplx<R>* plane, cplx<R>* accs, hi
   const int L = k.T + k.W - 1;
   const size_t sm = (size_t)k.nb * L * L * sizeof(cplx<R>);
   const auto* rf = (const rec3<R>*)g.rec_f;
-  int slo, shi;
-  chunk_range(k.Ww, k.nplanes, p, 1, &slo, &shi);
-  build_tasks(g, slo, shi, 1, g.chunk_degrid, st);
 #define HIPNG_DG(WW)                                                                        \
   case WW:                                                                                  \
     k_degrid<R, WW><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, rf, plane, \
@@ -2240,17 +2279,20 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
+    const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn, g.nb);
+    const bool wave = !grid_block_variant() && smw <= kLdsBudget;
     {
-      PROF("grid");
+      PROF("grid_setup");
       int slo, shi;
       chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
-      const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn);
-      const bool wave = !grid_block_variant() && smw <= kLdsBudget;
       build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? P.tile / 8 : 1);
       // footprints stay inside the band, so only its rows need clearing
       const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
       for (int i = 0; i < npl * g.nb; ++i)
         k_zero<cplx<R>><<<gridcap(nband), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nband);
+    }
+    {
+      PROF("grid");
       if (wave) {
 #define HIPNG_GW(TT, QQ, BB)                                                                      \
   k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,   \
@@ -2259,6 +2301,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
 #define HIPNG_GWB(TT, QQ)                 \
   if (g.nb == 1) HIPNG_GW(TT, QQ, 1);     \
   else if (g.nb == 2) HIPNG_GW(TT, QQ, 2); \
+  else if (g.nb == 3) HIPNG_GW(TT, QQ, 3); \
   else HIPNG_GW(TT, QQ, 4)
         if (P.tile == 32) {
           if (PCn == 2) { HIPNG_GWB(32, 2); } else { HIPNG_GWB(32, 1); }
@@ -2349,8 +2392,16 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
       for (int i = 0; i < npl * g.nb; ++i)
         geom_fft(g, g.fft_b_fwd, plane + ((int64_t)i * P.gx + g.xlo) * P.gy, st);
     }
-    PROF("degrid");
-    for (int q = 0; q < npl; ++q) launch_degrid<R>(g, p0 + q, plane + q * np1, accs, st);
+    for (int q = 0; q < npl; ++q) {
+      {
+        PROF("degrid_setup");
+        int slo, shi;
+        chunk_range(k.Ww, P.nplanes, p0 + q, 1, &slo, &shi);
+        build_tasks(g, slo, shi, 1, g.chunk_degrid, st);
+      }
+      PROF("degrid");
+      launch_degrid<R>(g, p0 + q, plane + q * np1, accs, st);
+    }
   }
   PROF("unsort");
   k_unsort<R, F32C><<<grid1(g.n), kBlock, 0, st>>>(g.uvw, g.perm, g.n, g.tb.pn, k, accs, (const R*)wgt,
