@@ -53,6 +53,7 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr int kBlock = 256;
 constexpr int kMaxBatch = 4;
 constexpr size_t kLdsBudget = 65536;  // dynamic LDS per workgroup the tile kernels may use
+constexpr size_t kDegridLds = 40960;  // LDS of a multi-plane degridder workgroup (4 per CU)
 
 thread_local std::string g_err;
 
@@ -1465,28 +1466,33 @@ __global__ __launch_bounds__(kBlock) void k_prescale_t(KP k, const I* __restrict
 }
 
 // ---------------------------------------------------------------------------------
-// Degridder: persistent workgroups; a task loads its tile of plane p with a (W-1) halo into
-// LDS and degrids one chunk of the tile's visibilities, one lane per visibility, separable
-// contraction in registers, read-modify-write of the sorted accumulator; no atomics.
+// Degridder: persistent workgroups over (tile, chunk) tasks of one pass of npl w planes.
+// A task loads the tile of all npl planes with its (W-1) halo into LDS, cell-major with the
+// NB polarisations of a cell contiguous, and degrids one chunk of the tile's visibilities:
+// one lane per visibility, u/v taps evaluated once per pass, separable contraction in
+// registers for every plane of the pass the visibility's w window meets, one
+// read-modify-write of the sorted accumulator per pass; no atomics. The candidates of a
+// start-plane segment share their plane range, so the plane loop does not diverge.
 // ---------------------------------------------------------------------------------
-template <typename R, int W>
-__global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p, const uint32_t* __restrict__ bstart,
+template <typename R, int W, int NB>
+__global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                    const uint32_t* __restrict__ bend,
                                                    const uint32_t* __restrict__ rec_a,
                                                    const rec3<R>* __restrict__ rec_f,
-                                                   const cplx<R>* __restrict__ plane,
+                                                   const cplx<R>* __restrict__ plane, int64_t plane_stride,
                                                    cplx<R>* __restrict__ accs,
                                                    const uint32_t* __restrict__ task_off, uint32_t chunk,
                                                    uint32_t* __restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);  // [nb][L][L]
+  cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);  // [npl][L][L][NB]
   __shared__ uint32_t s_task;
   const int T = k.T, L = k.T + W - 1, LL = L * L;
   const R scale = (R)2 / (R)W;
-  const R beta = (R)k.beta;
+  // the FP32 tap evaluation works in base 2
+  const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
   const uint32_t total = counters[1];
   int slo, shi;
-  plane_range(k, p, &slo, &shi);
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   int loaded = -1;
   for (;;) {
     if (threadIdx.x == 0) s_task = atomicAdd(&counters[0], 1u);
@@ -1497,12 +1503,15 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p, const uint32_t* 
     const uint32_t vlo = (task - task_off[tile]) * chunk, vhi = vlo + chunk;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
     if (tile != loaded) {
-      for (int e = threadIdx.x; e < k.nb * LL; e += kBlock) {
-        const int b = e / LL, rem = e - b * LL, rr = rem / L, cc = rem - rr * L;
+      // coalesced along grid columns; LDS written cell-major (polarisation fastest)
+      const int nrows = npl * NB * L;
+      for (int e = threadIdx.x; e < nrows * L; e += kBlock) {
+        const int row = e / L, cc = e - row * L;
+        const int rr = row % L, qb = row / L, b = qb % NB, q = qb / NB;
         int gxi = tx * T + rr, gyi = ty * T + cc;
         if (gxi >= k.gx) gxi -= k.gx;
         if (gyi >= k.gy) gyi -= k.gy;
-        g[e] = plane[((int64_t)b * k.gx + gxi) * k.gy + gyi];
+        g[((q * L + rr) * L + cc) * NB + b] = plane[q * plane_stride + ((int64_t)b * k.gx + gxi) * k.gy + gyi];
       }
       loaded = tile;
       __syncthreads();
@@ -1512,7 +1521,8 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p, const uint32_t* 
       const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)tile;
       const uint32_t st = bstart[key], len = bend[key] - st;
       const uint32_t a = max(vlo, r), bnd = min(vhi, r + len);
-      const R iw = (R)(p - s);
+      // planes of the pass inside the W_w-plane window of start plane s
+      const int qlo = max(0, s - p0), qhi = min(npl, s + k.Ww - p0);
       for (uint32_t j = st + (a - r) + threadIdx.x; a < bnd && j < st + (bnd - r); j += kBlock) {
         const uint32_t A = rec_a[j];
         const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
@@ -1520,38 +1530,51 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p, const uint32_t* 
         R kx[W], ky[W];
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-          kx[i] = esk((f.x + (R)i) * scale, beta);
-          ky[i] = esk((f.y + (R)i) * scale, beta);
+          kx[i] = esk_fast((f.x + (R)i) * scale, beta);
+          ky[i] = esk_fast((f.y + (R)i) * scale, beta);
         }
-        const R kw = k.wstack ? esk((f.w + iw) * scale, beta) : (R)1;
-        R sr[kMaxBatch], si[kMaxBatch];
+        R sr[NB], si[NB];
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) sr[b] = si[b] = 0;
+        for (int b = 0; b < NB; ++b) sr[b] = si[b] = 0;
+        for (int q = qlo; q < qhi; ++q) {
+          const R kw = k.wstack ? esk_fast((f.w + (R)(p0 + q - s)) * scale, beta) : (R)1;
+          const cplx<R>* gq = g + ((q * L + rx0) * L + ry0) * NB;
+          R tr[NB], ti[NB];
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) {
-          if (b < k.nb) {
-            const cplx<R>* gb = g + b * LL + rx0 * L + ry0;
+          for (int b = 0; b < NB; ++b) tr[b] = ti[b] = 0;
 #pragma unroll
-            for (int ix = 0; ix < W; ++ix) {
-              R rr = 0, ri = 0;
+          for (int ix = 0; ix < W; ++ix) {
+            R rr[NB], ri[NB];
 #pragma unroll
-              for (int iy = 0; iy < W; ++iy) {
-                const cplx<R> v = gb[ix * L + iy];
-                rr += ky[iy] * v.re;
-                ri += ky[iy] * v.im;
+            for (int b = 0; b < NB; ++b) rr[b] = ri[b] = 0;
+#pragma unroll
+            for (int iy = 0; iy < W; ++iy) {
+#pragma unroll
+              for (int b = 0; b < NB; ++b) {
+                const cplx<R> v = gq[(ix * L + iy) * NB + b];
+                rr[b] += ky[iy] * v.re;
+                ri[b] += ky[iy] * v.im;
               }
-              sr[b] += kx[ix] * rr;
-              si[b] += kx[ix] * ri;
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              tr[b] += kx[ix] * rr[b];
+              ti[b] += kx[ix] * ri[b];
             }
           }
-        }
 #pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) {
-          if (b < k.nb) {
-            cplx<R> o = accs[(int64_t)j * k.nb + b];
-            o.re += kw * sr[b];
-            o.im += kw * si[b];
-            accs[(int64_t)j * k.nb + b] = o;
+          for (int b = 0; b < NB; ++b) {
+            sr[b] += kw * tr[b];
+            si[b] += kw * ti[b];
+          }
+        }
+        if (qlo < qhi) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            cplx<R> o = accs[(int64_t)j * NB + b];
+            o.re += sr[b];
+            o.im += si[b];
+            accs[(int64_t)j * NB + b] = o;
           }
         }
       }
@@ -2230,16 +2253,32 @@ void geom_fft(const Geom& g, rocfft_plan plan, void* data, hipStream_t st) {
   if (e != rocfft_status_success) HIPNG_THROW(HIPNG_ERUNTIME, "rocfft failed: %d", (int)e);
 }
 
+// planes whose tiles the degridder holds in LDS at once: as many of the pass's planes as fit
+// in kDegridLds (four workgroups per CU), at least one
+int degrid_planes(const Geom& g, int npl) {
+  const size_t L = (size_t)g.k.T + g.k.W - 1;
+  const size_t per = L * L * g.nb * (g.prec == HIPNG_FP64 ? 16 : 8);
+  int q = npl;
+  while (q > 1 && q * per > kDegridLds) --q;
+  return q;
+}
+
 template <typename R>
-void launch_degrid(const Geom& g, int p, const cplx<R>* plane, cplx<R>* accs, hipStream_t st) {
+void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>* accs, hipStream_t st) {
   const KP& k = g.k;
   const int L = k.T + k.W - 1;
-  const size_t sm = (size_t)k.nb * L * L * sizeof(cplx<R>);
+  const size_t sm = (size_t)npl * g.nb * L * L * sizeof(cplx<R>);
+  const int64_t pstride = (int64_t)g.nb * k.gx * k.gy;
   const auto* rf = (const rec3<R>*)g.rec_f;
-#define HIPNG_DG(WW)                                                                        \
-  case WW:                                                                                  \
-    k_degrid<R, WW><<<g.npersist, kBlock, sm, st>>>(k, p, g.bstart, g.bend, g.rec_a, rf, plane, \
-                                                    accs, g.task, g.chunk_degrid, g.counters); \
+#define HIPNG_DG1(WW, BB)                                                                          \
+  k_degrid<R, WW, BB><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a, rf, plane, \
+                                                      pstride, accs, g.task, g.chunk_degrid, g.counters)
+#define HIPNG_DG(WW)                        \
+  case WW:                                  \
+    if (g.nb == 1) HIPNG_DG1(WW, 1);        \
+    else if (g.nb == 2) HIPNG_DG1(WW, 2);   \
+    else if (g.nb == 3) HIPNG_DG1(WW, 3);   \
+    else HIPNG_DG1(WW, 4);                  \
     break;
   switch (k.W) {
     HIPNG_DG(4) HIPNG_DG(5) HIPNG_DG(6) HIPNG_DG(7) HIPNG_DG(8) HIPNG_DG(9) HIPNG_DG(10)
@@ -2247,6 +2286,7 @@ void launch_degrid(const Geom& g, int p, const cplx<R>* plane, cplx<R>* accs, hi
     default: HIPNG_THROW(HIPNG_EINVAL, "unsupported kernel support %d", k.W);
   }
 #undef HIPNG_DG
+#undef HIPNG_DG1
   LAUNCH_CHECK();
 }
 
@@ -2398,15 +2438,17 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
       for (int i = 0; i < npl * g.nb; ++i)
         geom_fft(g, g.fft_b_fwd, plane + ((int64_t)i * P.gx + g.xlo) * P.gy, st);
     }
-    for (int q = 0; q < npl; ++q) {
+    const int dq = degrid_planes(g, npl);
+    for (int q = 0; q < npl; q += dq) {
+      const int nq = std::min(dq, npl - q);
       {
         PROF("degrid_setup");
         int slo, shi;
-        chunk_range(k.Ww, P.nplanes, p0 + q, 1, &slo, &shi);
+        chunk_range(k.Ww, P.nplanes, p0 + q, nq, &slo, &shi);
         build_tasks(g, slo, shi, 1, g.chunk_degrid, st);
       }
       PROF("degrid");
-      launch_degrid<R>(g, p0 + q, plane + q * np1, accs, st);
+      launch_degrid<R>(g, p0 + q, nq, plane + q * np1, accs, st);
     }
   }
   PROF("unsort");
