@@ -92,9 +92,11 @@ int32_t hipng_channel_scales(const hipng_plan* plan, const double* freq, int32_t
 int32_t hipng_uvw_extent(const double* uvw, int64_t nrow, const double* freq, int32_t nchan,
                          double* out, void* stream);
 
-/* Device: bucket sort of the nrow*nchan visibilities by (w plane, uv tile) for `plan`.
- * keys_out[j] is the bucket key of the j-th visibility in sorted order, perm_out[j] its
- * original index row*nchan + chan. Stable: equal keys keep increasing original index. */
+/* Device: bucket sort of the nrow*nchan visibilities by (w plane, uv tile) for `plan`,
+ * and inside a bucket by the in-tile origin digit (first cell mod tile along u) * tile +
+ * (first cell mod tile along v). keys_out[j] is the bucket key of the j-th visibility in
+ * sorted order, perm_out[j] its original index row*nchan + chan. Stable: equal
+ * (key, digit) pairs keep increasing original index. */
 int32_t hipng_bucket_sort(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
                           const hipng_plan* plan, uint32_t* keys_out, uint32_t* perm_out,
                           void* stream);
@@ -137,6 +139,12 @@ int32_t hipng_geometry_ms2dirty(hipng_geometry geom, const void* ms, const void*
 int32_t hipng_geometry_dirty2ms(hipng_geometry geom, const void* dirty, const void* wgt, void* ms,
                                 void* stream);
 int32_t hipng_geometry_destroy(hipng_geometry geom);
+
+/* Diagnostics (no counterpart in the reference): out[0] = visibilities (rows x channels),
+ * out[1] = dense-run tasks, out[2] = visibilities gridded by the dense-run path, out[3] = 1
+ * if that path is enabled for this geometry (W <= 8, grid <= 32768 per axis). Writes
+ * min(nout, 4) entries; synchronises the device. */
+int32_t hipng_geometry_stats(hipng_geometry geom, int64_t* out, int32_t nout);
 
 /* Device: FP64 point-source DFT, replacing dft_kernel / dft_cpu_looped
  * (rascil/processing_components/imaging/dft.py:123-178, 261-281; CUDA precedent

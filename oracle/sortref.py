@@ -32,7 +32,8 @@ def _p(a):
 
 
 def bucket_sort(uvw, sx, sy, sw, plan):
-    """CPU keys + stable counting sort. `plan` holds the hipng_plan fields (dict-like)."""
+    """CPU keys + stable sort by (bucket key, in-tile origin digit). `plan` holds the
+    hipng_plan fields (dict-like). Returns the sorted bucket keys and the permutation."""
     uvw = numpy.ascontiguousarray(uvw, "f8")
     sx, sy, sw = (numpy.ascontiguousarray(a, "f8") for a in (sx, sy, sw))
     nrow, nchan = uvw.shape[0], sx.shape[0]
@@ -41,13 +42,15 @@ def bucket_sort(uvw, sx, sy, sw, plan):
                     plan["w0"], plan["dw"])
     n = nrow * nchan
     keys = numpy.empty(n, numpy.uint32)
+    digits = numpy.empty(n, numpy.uint32)
     L = lib()
     L.oracle_keys(_p(uvw), ctypes.c_int64(nrow), ctypes.c_int32(nchan), _p(sx), _p(sy), _p(sw),
-                  ctypes.byref(op), _p(keys))
+                  ctypes.byref(op), _p(keys), _p(digits))
     nkeys = plan["nplanes"] * plan["ntx"] * plan["nty"]
     ks = numpy.empty(n, numpy.uint32)
     perm = numpy.empty(n, numpy.uint32)
-    rc = L.oracle_counting_sort(_p(keys), ctypes.c_int64(n), ctypes.c_uint64(nkeys), _p(ks), _p(perm))
+    rc = L.oracle_bucket_sort(_p(keys), _p(digits), ctypes.c_int64(n), ctypes.c_uint64(nkeys),
+                              ctypes.c_uint64(plan["tile"] * plan["tile"]), _p(ks), _p(perm))
     if rc != 0:
         raise MemoryError("oracle counting sort allocation failed")
     return ks, perm

@@ -1,8 +1,9 @@
 /* ORACLE — test infrastructure only (never linked into rascil_amd).
  *
  * CPU reference of the w-stacking gridder's visibility ordering: per-visibility
- * (w plane, uv tile) bucket keys and a stable counting sort. This is the checker the
- * GPU bucket sort must match bit for bit. The coordinate contract (one IEEE fma, one
+ * (w plane, uv tile) bucket keys, in-tile origin digits ((first cell mod tile) along u,
+ * then v) and a stable sort by (bucket key, origin digit). This is the checker the GPU
+ * bucket sort must match bit for bit. The coordinate contract (one IEEE fma, one
  * subtraction and one floor per axis, FP64 or FP32) is the one documented in
  * include/hipng.h for hipng_bucket_sort; the hermitian flip w < 0 -> (-u, -v, -w)
  * restates the ducc0 wgridder's published treatment of negative w (ducc0 0.24, not
@@ -54,9 +55,10 @@ static void item_pos(double u, double v, double w, double sxc, double syc, doubl
   *s = sp;
 }
 
-/* keys[i] for i = row * nchan + chan */
+/* keys[i] for i = row * nchan + chan; digits[i] (if not NULL) = (ax mod tile) * tile + ay mod tile */
 void oracle_keys(const double* uvw, int64_t nrow, int32_t nchan, const double* sx,
-                 const double* sy, const double* sw, const oracle_plan* P, uint32_t* keys) {
+                 const double* sy, const double* sw, const oracle_plan* P, uint32_t* keys,
+                 uint32_t* digits) {
   const uint32_t ntiles = (uint32_t)P->ntx * (uint32_t)P->nty;
 #pragma omp parallel for schedule(static)
   for (int64_t r = 0; r < nrow; ++r) {
@@ -66,6 +68,7 @@ void oracle_keys(const double* uvw, int64_t nrow, int32_t nchan, const double* s
       item_pos(uvw[3 * r], uvw[3 * r + 1], uvw[3 * r + 2], sx[c], sy[c], sw[c], P, &ax, &ay, &s);
       keys[r * nchan + c] = (uint32_t)s * ntiles + (uint32_t)(ax / P->tile) * (uint32_t)P->nty +
                             (uint32_t)(ay / P->tile);
+      if (digits) digits[r * nchan + c] = (uint32_t)((ax % P->tile) * P->tile + ay % P->tile);
     }
   }
 }
@@ -84,6 +87,32 @@ int oracle_counting_sort(const uint32_t* keys, int64_t n, uint64_t nkeys, uint32
   }
   free(start);
   return 0;
+}
+
+/* Stable sort by (keys, digits): a counting sort by digit (ndigits bins), then a stable
+ * counting sort of that order by key. perm holds original indices, keys_sorted the keys. */
+int oracle_bucket_sort(const uint32_t* keys, const uint32_t* digits, int64_t n, uint64_t nkeys,
+                       uint64_t ndigits, uint32_t* keys_sorted, uint32_t* perm) {
+  uint32_t* dk = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));
+  uint32_t* p1 = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));
+  uint32_t* p2 = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));
+  if (!dk || !p1 || !p2) {
+    free(dk);
+    free(p1);
+    free(p2);
+    return -1;
+  }
+  int rc = oracle_counting_sort(digits, n, ndigits, dk, p1);
+  if (rc == 0) {
+    for (int64_t i = 0; i < n; ++i) dk[i] = keys[p1[i]];
+    rc = oracle_counting_sort(dk, n, nkeys, keys_sorted, p2);
+  }
+  if (rc == 0)
+    for (int64_t i = 0; i < n; ++i) perm[i] = p1[p2[i]];
+  free(dk);
+  free(p1);
+  free(p2);
+  return rc;
 }
 
 /* ---------------------------------------------------------------------------------

@@ -30,6 +30,7 @@ EXPORTS = (
     "hipng_geometry_ms2dirty",
     "hipng_geometry_dirty2ms",
     "hipng_geometry_destroy",
+    "hipng_geometry_stats",
     "hipng_dft",
     "hipng_weight_visibility",
     "hipng_profile_enable",
@@ -124,6 +125,8 @@ def load():
         lib.hipng_geometry_dirty2ms.restype = c_i32
         lib.hipng_geometry_destroy.argtypes = [c_vp]
         lib.hipng_geometry_destroy.restype = c_i32
+        lib.hipng_geometry_stats.argtypes = [c_vp, ctypes.POINTER(c_i64), c_i32]
+        lib.hipng_geometry_stats.restype = c_i32
         lib.hipng_dft.argtypes = [c_vp, c_vp, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp, c_vp]
         lib.hipng_dft.restype = c_i32
         lib.hipng_release.restype = c_i32

@@ -6,8 +6,10 @@
 // bucket sort by (w plane, uv tile), per-plane FFT), laid out for CDNA4:
 //
 //   k_extent       uvw extent reduction (one host sync to choose the plan)
-//   k_keys         FP64 uvw -> grid coordinates, (plane, tile) bucket key per visibility
-//   k_rs_*         stable LSD radix sort (8-bit digits, wave-ballot ranking) of the keys
+//   k_digits / k_rekey  FP64 uvw -> grid coordinates: in-tile origin digit, then the
+//                  (plane, tile) bucket key of each visibility in digit order
+//   k_rs_*         stable LSD radix sort (8-bit digits, wave-ballot ranking): origin digit
+//                  passes, then bucket key passes
 //   k_bounds       bucket [start, end) table from the sorted keys
 //   k_records      sorted, compact per-visibility records: packed first cell, FP32/FP64
 //                  kernel offsets and (for invert) the weighted, phase-shifted values
@@ -338,6 +340,9 @@ struct KP {
   int nx, ny, nb, nchan;
   double hgxd, hgyd, hw, mw0;  // G/2, W/2, -w0/dw
   double beta, px, py, nshift, dw, w0;
+  // packed first cell ax | ay << 16 of a record: amask clears the dense-run flag dflag
+  // (bit 31, set on records the dense-run gridder owns; both 0 / all ones when unused)
+  uint32_t amask, dflag;
 };
 
 struct ItemPos {
@@ -449,18 +454,38 @@ __global__ void k_extent(const double* __restrict__ uvw, int64_t nrow,
   }
 }
 
+// in-tile origin digit (first cell mod tile along u, then v): the secondary sort key
+__device__ __forceinline__ uint32_t item_digit(const ItemPos& p, const KP& k) {
+  return (uint32_t)(p.ax % k.T) * (uint32_t)k.T + (uint32_t)(p.ay % k.T);
+}
+
 template <bool F32C>
-__global__ void k_keys(const double* __restrict__ uvw, int64_t nitems, const double* __restrict__ sx,
-                       const double* __restrict__ sy, const double* __restrict__ sw, KP k,
-                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+__global__ void k_digits(const double* __restrict__ uvw, int64_t nitems, const double* __restrict__ sx,
+                         const double* __restrict__ sy, const double* __restrict__ sw, KP k,
+                         uint32_t* __restrict__ digits, uint32_t* __restrict__ vals) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= nitems) return;
   const int64_t row = i / k.nchan;
   const int ch = (int)(i - row * k.nchan);
   const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch],
                                    sy[ch], sw[ch], k);
-  keys[i] = item_key(p, k);
+  digits[i] = item_digit(p, k);
   vals[i] = (uint32_t)i;
+}
+
+// bucket keys of the items in the order perm gives them
+template <bool F32C>
+__global__ void k_rekey(const double* __restrict__ uvw, const uint32_t* __restrict__ perm, int64_t nitems,
+                        const double* __restrict__ sx, const double* __restrict__ sy,
+                        const double* __restrict__ sw, KP k, uint32_t* __restrict__ keys) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= nitems) return;
+  const uint32_t i = perm[j];
+  const int64_t row = i / k.nchan;
+  const int ch = (int)(i - row * k.nchan);
+  const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch],
+                                   sy[ch], sw[ch], k);
+  keys[j] = item_key(p, k);
 }
 
 // ---------------------------------------------------------------------------------
@@ -766,6 +791,16 @@ constexpr int kMaxPC = 2;                    // w planes gridded per pass
 constexpr int kXcd = 8;                      // XCDs per device: work queues of the sub-tile gridder
 constexpr int kQueueBase = 8;                // counters[kQueueBase + x]: pops from queue x
 constexpr int kMaxSeg = 4 * (16 + kMaxPC - 1);  // 4 neighbour buckets x start planes
+constexpr int kDenseCnt = kQueueBase + kXcd;     // counters[kDenseCnt]: dense-run chunk pops
+constexpr int kDenseTotal = kDenseCnt + 1;       // counters[kDenseTotal]: dense-run chunks
+constexpr int kRuns = kDenseCnt + 2;             // counters[kRuns]: runs (geometry build)
+constexpr int kNCounters = kDenseCnt + 3;
+// Shortest run the dense path takes: each dense task ends in 64 x PC x NB x 2 global atomic
+// adds, which cost more than the sub-tile gatherer spends on runs shorter than ~100
+// (measured at the production shape: gridding 593 / 577 / 593 / 614 ms per invert for
+// minimum runs of 64 / 128 / 256 / 1024, 698 ms with the dense path off).
+constexpr uint32_t kDenseMin = 128;
+constexpr uint32_t kDenseChunk = 512;            // visibilities per dense-run task
 
 // start planes whose W_w-plane window meets planes [p0, p0 + npl)
 __device__ __host__ __forceinline__ void chunk_range(int Ww, int nplanes, int p0, int npl, int* slo,
@@ -858,14 +893,14 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p0, int npl, const ui
         }
         j = s_segj[lo] + (v - s_segv[lo]);
         sp = s_segs[lo];
-        const uint32_t A = rec_a[j];
+        const uint32_t Ar = rec_a[j], A = Ar & k.amask;
         dx = (int)(A & 0xffffu) - tx * T;
         dy = (int)(A >> 16) - ty * T;
         if (dx >= k.hgx) dx -= k.gx;
         if (dx < -k.hgx) dx += k.gx;
         if (dy >= k.hgy) dy -= k.gy;
         if (dy < -k.hgy) dy += k.gy;
-        ok = (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
+        ok = (Ar & k.dflag) == 0u && (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
       }
       // staged position (all lists) and per-wave list positions, deterministic (ballots)
       const unsigned long long m = __ballot(ok);
@@ -1182,14 +1217,14 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
           while (lo + 1 < nseg && s_segv[lo + 1] <= v) ++lo;
           j = s_segj[lo] + (v - s_segv[lo]);
           sp = slo + lo / 4;
-          const uint32_t A = rec_a[j];
+          const uint32_t Ar = rec_a[j], A = Ar & k.amask;
           dx = (int)(A & 0xffffu) - tx * T;
           dy = (int)(A >> 16) - ty * T;
           if (dx >= k.hgx) dx -= k.gx;
           if (dx < -k.hgx) dx += k.gx;
           if (dy >= k.hgy) dy -= k.gy;
           if (dy < -k.hgy) dy += k.gy;
-          ok = (dx + W > sx0) && (dx < sx0 + 8) && (dy + W > sy0) && (dy < sy0 + 8);
+          ok = (Ar & k.dflag) == 0u && (dx + W > sx0) && (dx < sx0 + 8) && (dy + W > sy0) && (dy < sy0 + 8);
         }
         cur = __shfl(lo, 63, 64);
         const unsigned long long m = __ballot(ok);
@@ -1236,6 +1271,255 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
         }
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Dense runs. The sort orders a bucket by in-tile origin, so visibilities with the same
+// start plane and the same first cell are consecutive: a run. Runs of at least kDenseMin
+// visibilities (the dense uv core: short baselines at many channels and times) are
+// gridded by k_grid_dense with one lane per footprint cell, which accumulates the whole
+// run in registers; the sub-tile gatherer skips their records (flag bit in rec_a).
+// ---------------------------------------------------------------------------------
+// head[j] = 1 where a run starts (bucket key or first cell differ from the previous record)
+__global__ void k_run_heads(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a, int64_t n,
+                            uint32_t* __restrict__ head) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  head[j] = (j == 0 || keys[j] != keys[j - 1] || rec_a[j] != rec_a[j - 1]) ? 1u : 0u;
+}
+
+// with A the exclusive scan of the heads: rstart[run] = first record, rstart[nruns] = n
+__global__ void k_run_starts(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a, int64_t n,
+                             const uint32_t* __restrict__ A, uint32_t* __restrict__ rstart,
+                             uint32_t* __restrict__ counters) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const bool head = j == 0 || keys[j] != keys[j - 1] || rec_a[j] != rec_a[j - 1];
+  if (head) rstart[A[j]] = (uint32_t)j;
+  if (j == n - 1) {
+    const uint32_t nruns = A[j] + (head ? 1u : 0u);
+    rstart[nruns] = (uint32_t)n;
+    counters[kRuns] = nruns;
+  }
+}
+
+// flag the records of runs with at least minlen visibilities
+__global__ void k_run_mark(uint32_t* __restrict__ rec_a, int64_t n, const uint32_t* __restrict__ A,
+                           const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ counters,
+                           uint32_t minlen, uint32_t flag) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t nruns = counters[kRuns];
+  const uint32_t a = A[j];
+  const uint32_t r = (a < nruns && rstart[a] == (uint32_t)j) ? a : a - 1u;
+  if (rstart[r + 1] - rstart[r] >= minlen) rec_a[j] |= flag;
+}
+
+// cnt[r] = tasks of run r (0 for short runs and for r >= nruns), r in [0, n)
+__global__ void k_run_count(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ counters, int64_t n,
+                            uint32_t minlen, uint32_t chunk, uint32_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  uint32_t c = 0;
+  if (r < (int64_t)counters[kRuns]) {
+    const uint32_t len = rstart[r + 1] - rstart[r];
+    if (len >= minlen) c = (len + chunk - 1) / chunk;
+  }
+  cnt[r] = c;
+}
+
+// with coff the exclusive scan of the task counts: the task list (first record, count,
+// packed first cell, start plane) and each task's start plane alone; tasks are in record
+// order, so their start planes ascend
+__global__ void k_run_tasks(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a,
+                            const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ coff,
+                            uint32_t* __restrict__ counters, int64_t n, uint32_t minlen, uint32_t chunk,
+                            uint32_t ntiles, uint32_t amask, uint4* __restrict__ dtask,
+                            uint32_t* __restrict__ dplane) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t nruns = counters[kRuns];
+  uint32_t nt = 0;
+  if (r < (int64_t)nruns) {
+    const uint32_t st = rstart[r], len = rstart[r + 1] - st;
+    if (len >= minlen) {
+      nt = (len + chunk - 1) / chunk;
+      const uint32_t sp = keys[st] / ntiles, A = rec_a[st] & amask;
+      for (uint32_t c = 0; c < nt; ++c) {
+        dtask[coff[r] + c] = make_uint4(st + c * chunk, min(chunk, len - c * chunk), A, sp);
+        dplane[coff[r] + c] = sp;
+      }
+    }
+  }
+  if (r == n - 1) counters[kDenseTotal] = coff[r] + nt;
+}
+
+// diagnostics: out[0] += dense-run tasks, out[1] += visibilities they hold
+__global__ void k_dense_stats(const uint4* __restrict__ dtask, const uint32_t* __restrict__ counters,
+                              unsigned long long* __restrict__ out) {
+  const uint32_t total = counters[kDenseTotal];
+  unsigned long long nv = 0, nt = 0;
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+    nv += dtask[t].y;
+    ++nt;
+  }
+  if (nt) {
+    atomicAdd(&out[0], nt);
+    atomicAdd(&out[1], nv);
+  }
+}
+
+// first index in [0, n) with a[i] >= v (a ascending)
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Dense-run gridder (W <= 8): a wave takes a task (up to kDenseChunk visibilities of one
+// run: same start plane, same first cell); lane (ci, cj) = footprint cell (ax + ci, ay + cj)
+// and accumulates all PC planes and NB polarisations of the run in registers. Visibilities
+// are staged 64 at a time, one per lane (8 u taps, 8 v taps, the pass's w taps, values);
+// the next 64 records (of this task or the next) are loaded while the current ones are
+// accumulated, and the next task is popped one task ahead. Each task ends with one global
+// atomic add per cell, plane and polarisation. Runs after the sub-tile gatherer of the
+// same pass (whose solo tiles are written with plain stores).
+template <typename R, int NB>
+__global__ __launch_bounds__(kBlock) void k_grid_dense(KP k, int p0, int npl, const uint4* __restrict__ dtask,
+                                                       const uint32_t* __restrict__ dplane,
+                                                       const rec3<R>* __restrict__ rec_f,
+                                                       const cplx<R>* __restrict__ rec_v,
+                                                       uint32_t* __restrict__ counters, cplx<R>* __restrict__ plane) {
+  constexpr int PC = kMaxPC;
+  constexpr int NW = kBlock / 64;
+  __shared__ R s_tx[NW][64][9];
+  __shared__ R s_ty[NW][64][9];
+  __shared__ R s_tw[NW][64][PC];
+  __shared__ cplx<R> s_v[NW][NB][64];
+  const int W = k.W;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ci = lane >> 3, cj = lane & 7;
+  const R scale = (R)2 / (R)W;
+  const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
+  int slo, shi;
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
+  const uint32_t total = counters[kDenseTotal];
+  const uint32_t lo = lower_bound_u32(dplane, total, (uint32_t)slo);
+  const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
+  auto pop = [&]() {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
+    return lo + __shfl(t, 0, 64);
+  };
+  uint32_t t = pop();
+  if (t >= hi) return;
+  uint4 cur = dtask[t];  // first record, count, packed first cell, start plane
+  uint32_t tn = pop();
+  // registers of the record this lane stages next
+  rec3<R> f{};
+  cplx<R> v[NB];
+  auto load = [&](const uint4& tk, uint32_t b0) {
+    if (b0 + (uint32_t)lane < tk.y) {
+      const uint32_t j = tk.x + b0 + (uint32_t)lane;
+      f = rec_f[j];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) v[b] = rec_v[(int64_t)j * NB + b];
+    }
+  };
+  load(cur, 0);
+  R ar[PC][NB], ai[PC][NB];
+#pragma unroll
+  for (int q = 0; q < PC; ++q)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
+  uint32_t b0 = 0;
+  for (;;) {
+    const int cnt = (int)min(64u, cur.y - b0);
+    const int sp = (int)cur.w;
+    if (lane < cnt) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        s_tx[wv][lane][i] = i < W ? esk_fast((f.x + (R)i) * scale, beta) : (R)0;
+        s_ty[wv][lane][i] = i < W ? esk_fast((f.y + (R)i) * scale, beta) : (R)0;
+      }
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        const int iw = p0 + q - sp;
+        s_tw[wv][lane][q] = (q < npl && iw >= 0 && iw < k.Ww)
+                                ? (k.wstack ? esk_fast((f.w + (R)iw) * scale, beta) : (R)1)
+                                : (R)0;
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) s_v[wv][b][lane] = v[b];
+    }
+    wave_sync();
+    // next batch: the rest of this task, else the first records of the next task
+    const bool last = b0 + 64u >= cur.y;
+    uint4 nxt = cur;
+    uint32_t nb0 = b0 + 64u;
+    if (last) {
+      nb0 = 0;
+      nxt = tn < hi ? dtask[tn] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    load(nxt, nb0);
+    auto accum = [&](int e) {
+      const R kxy = s_tx[wv][e][ci] * s_ty[wv][e][cj];
+      R wq[PC];
+#pragma unroll
+      for (int q = 0; q < PC; ++q) wq[q] = kxy * s_tw[wv][e][q];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const cplx<R> vv = s_v[wv][b][e];
+#pragma unroll
+        for (int q = 0; q < PC; ++q) {
+          ar[q][b] += wq[q] * vv.re;
+          ai[q][b] += wq[q] * vv.im;
+        }
+      }
+    };
+    int e = 0;
+    for (; e + 1 < cnt; e += 2) {
+      accum(e);
+      accum(e + 1);
+    }
+    if (e < cnt) accum(e);
+    wave_sync();
+    if (!last) {
+      b0 = nb0;
+      continue;
+    }
+    if (ci < W && cj < W) {
+      const uint32_t A = cur.z;
+      int gxi = (int)(A & 0xffffu) + ci, gyi = (int)(A >> 16) + cj;
+      if (gxi >= k.gx) gxi -= k.gx;
+      if (gyi >= k.gy) gyi -= k.gy;
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        if (q < npl) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            if (ar[q][b] != (R)0 || ai[q][b] != (R)0) {
+              cplx<R>* o = plane + ((int64_t)(q * NB + b) * k.gx + gxi) * k.gy + gyi;
+              atomicAdd(&o->re, ar[q][b]);
+              atomicAdd(&o->im, ai[q][b]);
+            }
+          }
+        }
+      }
+    }
+    if (tn >= hi) break;
+#pragma unroll
+    for (int q = 0; q < PC; ++q)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
+    cur = nxt;
+    b0 = 0;
+    tn = pop();
   }
 }
 
@@ -1524,7 +1808,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
       // planes of the pass inside the W_w-plane window of start plane s
       const int qlo = max(0, s - p0), qhi = min(npl, s + k.Ww - p0);
       for (uint32_t j = st + (a - r) + threadIdx.x; a < bnd && j < st + (bnd - r); j += kBlock) {
-        const uint32_t A = rec_a[j];
+        const uint32_t A = rec_a[j] & k.amask;
         const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
         const rec3<R> f = rec_f[j];
         R kx[W], ky[W];
@@ -1795,7 +2079,8 @@ void prof_flush() {
 // ---------------------------------------------------------------------------------
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
-  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_NSLOTS
+  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_DTASK,
+  S_DPLANE, S_NSLOTS
 };
 
 struct FftKey {
@@ -1928,6 +2213,8 @@ KP make_kp(const hipng_plan& P, int nb, int nchan) {
   k.nshift = P.nshift;
   k.dw = P.dw;
   k.w0 = P.w0;
+  k.amask = 0xffffffffu;
+  k.dflag = 0u;
   return k;
 }
 
@@ -1995,43 +2282,60 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
 
 // keys + stable radix sort in the shared workspace; returns sorted keys and permutation
 void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const KP& k,
-                const hipng_plan& P, hipStream_t st, uint32_t** keys_out, uint32_t** perm_out) {
-  auto* k0 = (uint32_t*)ws(c, S_KEYS0, n * 4);
-  auto* v0 = (uint32_t*)ws(c, S_VALS0, n * 4);
-  auto* k1 = (uint32_t*)ws(c, S_KEYS1, n * 4);
-  auto* v1 = (uint32_t*)ws(c, S_VALS1, n * 4);
-  {
-    PROF("keys");
-    if (P.precision == HIPNG_FP32)
-      k_keys<true><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
-    else
-      k_keys<false><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
-    LAUNCH_CHECK();
-  }
+                const hipng_plan& P, hipStream_t st, uint32_t** keys_out, uint32_t** perm_out,
+                uint32_t** free0 = nullptr, uint32_t** free1 = nullptr) {
+  // one spare entry each: the ping-pong pair left free after the sort is reused as scratch
+  // of n + 1 entries (run starts of the dense-run detection)
+  auto* k0 = (uint32_t*)ws(c, S_KEYS0, (n + 1) * 4);
+  auto* v0 = (uint32_t*)ws(c, S_VALS0, (n + 1) * 4);
+  auto* k1 = (uint32_t*)ws(c, S_KEYS1, (n + 1) * 4);
+  auto* v1 = (uint32_t*)ws(c, S_VALS1, (n + 1) * 4);
   const int64_t nblocks = (n + kRsItems - 1) / kRsItems;
   const int64_t m = 256 * nblocks;
   const int64_t npart = (m + kRsItems - 1) / kRsItems;
   auto* hist = (uint32_t*)ws(c, S_HIST, m * 4);
   auto* part = (uint32_t*)ws(c, S_PART, npart * 4);
-  const int passes = (P.keybits + 7) / 8;
-  PROF("radix_sort");
-  for (int ps = 0; ps < passes; ++ps) {
-    const int shift = 8 * ps;
-    k_rs_hist<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, n, shift, hist, nblocks);
+  auto passes = [&](int bits) {
+    PROF("radix_sort");
+    for (int ps = 0; ps < (bits + 7) / 8; ++ps) {
+      const int shift = 8 * ps;
+      k_rs_hist<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, n, shift, hist, nblocks);
+      LAUNCH_CHECK();
+      k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
+      LAUNCH_CHECK();
+      k_scan_top<<<1, kBlock, 0, st>>>(part, npart);
+      LAUNCH_CHECK();
+      k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
+      LAUNCH_CHECK();
+      k_rs_scatter<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks);
+      LAUNCH_CHECK();
+      std::swap(k0, k1);
+      std::swap(v0, v1);
+    }
+  };
+  // LSD: the in-tile origin digit first, then the bucket key of each item in that order
+  {
+    PROF("keys");
+    if (P.precision == HIPNG_FP32)
+      k_digits<true><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
+    else
+      k_digits<false><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
     LAUNCH_CHECK();
-    k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
-    LAUNCH_CHECK();
-    k_scan_top<<<1, kBlock, 0, st>>>(part, npart);
-    LAUNCH_CHECK();
-    k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
-    LAUNCH_CHECK();
-    k_rs_scatter<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks);
-    LAUNCH_CHECK();
-    std::swap(k0, k1);
-    std::swap(v0, v1);
   }
+  passes(2 * bits_for((uint64_t)P.tile - 1));
+  {
+    PROF("keys");
+    if (P.precision == HIPNG_FP32)
+      k_rekey<true><<<grid1(n), kBlock, 0, st>>>(uvw, v0, n, tb.sx, tb.sy, tb.sw, k, k0);
+    else
+      k_rekey<false><<<grid1(n), kBlock, 0, st>>>(uvw, v0, n, tb.sx, tb.sy, tb.sw, k, k0);
+    LAUNCH_CHECK();
+  }
+  passes(P.keybits);
   *keys_out = k0;
   *perm_out = v0;
+  if (free0) *free0 = k1;
+  if (free1) *free1 = v1;
 }
 
 void set_lds_limits() {
@@ -2077,8 +2381,22 @@ struct Geom {
   uint32_t chunk_grid = 1024, chunk_degrid = 1024;
   int npersist = 1024;
   int pc = 1;  // w planes per gridding pass / FFT batch
+  // dense runs: tasks (first record, count) and their start planes, ascending
+  bool dense = false;
+  uint4* dtask = nullptr;
+  uint32_t* dplane = nullptr;
   Buffers buf;
 };
+
+// HIPNG_DENSE=0 turns the dense-run gridder off (every record through the sub-tile gatherer)
+bool dense_enabled() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = std::getenv("HIPNG_DENSE");
+    v = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
+  }
+  return v == 1;
+}
 
 // HIPNG_GRIDDER=block selects the workgroup-staged gridder instead of the sub-tile one
 bool grid_block_variant() {
@@ -2088,6 +2406,17 @@ bool grid_block_variant() {
     v = (e && std::strcmp(e, "block") == 0) ? 1 : 0;
   }
   return v == 1;
+}
+
+// shortest run the dense-run gridder takes (HIPNG_DENSE_MIN overrides, for calibration)
+uint32_t dense_min_run() {
+  static uint32_t v = 0;
+  if (v == 0) {
+    const char* e = std::getenv("HIPNG_DENSE_MIN");
+    const long x = e ? std::atol(e) : 0;
+    v = x >= 2 ? (uint32_t)x : kDenseMin;
+  }
+  return v;
 }
 
 int device_cus() {
@@ -2105,6 +2434,17 @@ uint32_t pick_chunk(double visits_per_plane, int npersist) {
   double c = visits_per_plane / (2.0 * npersist);
   c = std::min(16384.0, std::max(256.0, c));
   return (uint32_t)((uint64_t)(c + 255) / 256 * 256);
+}
+
+// in-place exclusive scan of a[0, m); part holds ceil(m / kRsItems) entries
+void excl_scan(uint32_t* a, int64_t m, uint32_t* part, hipStream_t st) {
+  const int64_t np = (m + kRsItems - 1) / kRsItems;
+  k_scan_partial<<<(unsigned)np, kBlock, 0, st>>>(a, m, part);
+  LAUNCH_CHECK();
+  k_scan_top<<<1, kBlock, 0, st>>>(part, np);
+  LAUNCH_CHECK();
+  k_scan_apply<<<(unsigned)np, kBlock, 0, st>>>(a, m, part);
+  LAUNCH_CHECK();
 }
 
 void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64_t nrow, int nchan,
@@ -2140,8 +2480,15 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   } else {
     g.uvw = uvw;
   }
-  uint32_t *keys, *perm;
-  sort_items(c, g.uvw, n, g.tb, g.k, P, st, &keys, &perm);
+  uint32_t *keys, *perm, *scr0, *scr1;
+  sort_items(c, g.uvw, n, g.tb, g.k, P, st, &keys, &perm, &scr0, &scr1);
+  // dense-run gridding needs W <= 8 (one lane per footprint cell) and a free flag bit
+  // in the packed first cell (grids up to 32768 cells per axis)
+  g.dense = dense_enabled() && P.W <= 8 && P.gx <= 32768 && P.gy <= 32768;
+  if (g.dense) {
+    g.k.amask = 0x7fff7fffu;
+    g.k.dflag = 0x80000000u;
+  }
   if (own) {
     g.perm = (uint32_t*)g.buf.get(S_VALS0, n * 4);
     HIP_TRY(hipMemcpyAsync(g.perm, perm, n * 4, hipMemcpyDeviceToDevice, st));
@@ -2216,18 +2563,40 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   const int64_t ntiles = (int64_t)P.ntx * P.nty;
   g.task = (uint32_t*)g.buf.get(S_TASK, ntiles * 4);
   g.taskpart = (uint32_t*)g.buf.get(S_TASKPART, ((ntiles + kRsItems - 1) / kRsItems) * 4);
-  g.counters = (uint32_t*)g.buf.get(S_CNT, (kQueueBase + kXcd) * sizeof(uint32_t));
+  g.counters = (uint32_t*)g.buf.get(S_CNT, kNCounters * sizeof(uint32_t));
   g.npersist = device_cus() * 4;
   const double ww = P.wstacking ? P.W : 1;
   g.chunk_grid = pick_chunk((double)n * ww / P.nplanes * 4.0, g.npersist);
   g.chunk_degrid = pick_chunk((double)n * ww / P.nplanes, g.npersist);
+  if (g.dense) {
+    PROF("runs");
+    // scratch: scr0 = heads / run ids / task offsets, scr1 = run starts (n + 1 entries)
+    uint32_t* part = (uint32_t*)ws(c, S_HIST, (size_t)((n + kRsItems - 1) / kRsItems) * 4);
+    const uint32_t dmin = dense_min_run();
+    const int64_t maxt = n / dmin + n / kDenseChunk + 1;
+    g.dtask = (uint4*)g.buf.get(S_DTASK, (size_t)maxt * sizeof(uint4));
+    g.dplane = (uint32_t*)g.buf.get(S_DPLANE, (size_t)maxt * 4);
+    k_run_heads<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, n, scr0);
+    LAUNCH_CHECK();
+    excl_scan(scr0, n, part, st);
+    k_run_starts<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, n, scr0, scr1, g.counters);
+    LAUNCH_CHECK();
+    k_run_mark<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, scr0, scr1, g.counters, dmin, g.k.dflag);
+    LAUNCH_CHECK();
+    k_run_count<<<grid1(n), kBlock, 0, st>>>(scr1, g.counters, n, dmin, kDenseChunk, scr0);
+    LAUNCH_CHECK();
+    excl_scan(scr0, n, part, st);
+    k_run_tasks<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, scr1, scr0, g.counters, n, dmin, kDenseChunk,
+                                             g.k.ntiles, g.k.amask, g.dtask, g.dplane);
+    LAUNCH_CHECK();
+  }
   if (own) HIP_TRY(hipStreamSynchronize(st));
 }
 
 // per-plane task list: counters reset, per-tile task counts, exclusive scan -> task offsets
 void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st, uint32_t mult = 1) {
   const int64_t ntiles = g.k.ntiles;
-  k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, kQueueBase + kXcd, 0u);
+  k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, kDenseCnt + 1, 0u);
   LAUNCH_CHECK();
   k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, g.bstart, g.bend, chunk, g.task, g.counters,
                                             mult);
@@ -2370,6 +2739,24 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
 #undef HIPNG_GR
       LAUNCH_CHECK();
       }
+    }
+    if (g.dense) {
+      PROF("grid_dense");
+      const auto* rf = (const rec3<R>*)g.rec_f;
+      const int nblk = device_cus() * 5;  // 5 workgroups per CU fit its LDS (29 KiB each, FP32)
+      if (g.nb == 1)
+        k_grid_dense<R, 1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
+                                                          plane);
+      else if (g.nb == 2)
+        k_grid_dense<R, 2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
+                                                          plane);
+      else if (g.nb == 3)
+        k_grid_dense<R, 3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
+                                                          plane);
+      else
+        k_grid_dense<R, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
+                                                          plane);
+      LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_inverse");
@@ -2620,6 +3007,31 @@ int32_t hipng_geometry_dirty2ms(hipng_geometry geom, const void* dirty, const vo
   return guarded([&] {
     if (!geom) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
     exec_dispatch(geom->g, false, dirty, wgt, ms, (hipStream_t)stream);
+  });
+}
+
+int32_t hipng_geometry_stats(hipng_geometry geom, int64_t* out, int32_t nout) {
+  return guarded([&] {
+    if (!geom || !out || nout < 1) HIPNG_THROW(HIPNG_EINVAL, "geometry and output array required");
+    const Geom& g = geom->g;
+    int64_t v[4] = {g.n, 0, 0, g.dense ? 1 : 0};
+    if (g.dense && g.n > 0) {
+      int cur = 0;
+      HIP_TRY(hipGetDevice(&cur));
+      HIP_TRY(hipSetDevice(g.device));
+      unsigned long long* d = nullptr;
+      HIP_TRY(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+      HIP_TRY(hipMemset(d, 0, 2 * sizeof(unsigned long long)));
+      k_dense_stats<<<256, kBlock>>>(g.dtask, g.counters, d);
+      LAUNCH_CHECK();
+      unsigned long long h[2] = {0ull, 0ull};
+      HIP_TRY(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
+      HIP_TRY(hipFree(d));
+      HIP_TRY(hipSetDevice(cur));
+      v[1] = (int64_t)h[0];
+      v[2] = (int64_t)h[1];
+    }
+    for (int i = 0; i < nout && i < 4; ++i) out[i] = v[i];
   });
 }
 

@@ -343,6 +343,13 @@ class Geometry:
             ctypes.c_void_p(out.data_ptr()), _stream(torch, self.device)))
         return out
 
+    def stats(self):
+        """Diagnostics: visibilities, dense-run tasks, visibilities on the dense-run path and
+        whether that path is enabled for this geometry."""
+        out = (ctypes.c_int64 * 4)()
+        _lib.check(self._lib.hipng_geometry_stats(self._handle, out, 4))
+        return {"nvis": out[0], "dense_tasks": out[1], "dense_vis": out[2], "dense_enabled": bool(out[3])}
+
     def close(self):
         if getattr(self, "_handle", None) and self._handle.value:
             self._lib.hipng_geometry_destroy(self._handle)

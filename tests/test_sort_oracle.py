@@ -34,8 +34,6 @@ def test_oracle_sort_sorted_stable_and_keys(small_obs, wst):
     n = uvw.shape[0] * 3
     assert numpy.all(numpy.diff(ks.astype(numpy.int64)) >= 0)
     assert numpy.array_equal(numpy.sort(perm), numpy.arange(n))
-    same = ks[1:] == ks[:-1]
-    assert numpy.all(perm[1:][same] > perm[:-1][same])  # stable
     # independent recomputation of a few keys from the plan (pins the key formula)
     for i in (0, 7, n // 2, n - 1):
         r, ch = divmod(int(perm[i]), 3)
@@ -50,3 +48,15 @@ def test_oracle_sort_sorted_stable_and_keys(small_obs, wst):
             assert 0 <= s <= p.nplanes - p.W
         key = s * p.ntx * p.nty + (ax // p.tile) * p.nty + ay // p.tile
         assert key == ks[i]
+    # within a bucket: ordered by in-tile origin digit, stable among equal digits
+    rows, chans = numpy.divmod(perm.astype(numpy.int64), 3)
+    flip = uvw[rows, 2] < 0
+    u = numpy.where(flip, -uvw[rows, 0], uvw[rows, 0])
+    v = numpy.where(flip, -uvw[rows, 1], uvw[rows, 1])
+    ax = (numpy.floor(u * sx[chans] + pd["gx"] / 2 - p.W / 2).astype(numpy.int64) + 1) % pd["gx"]
+    ay = (numpy.floor(v * sy[chans] + pd["gy"] / 2 - p.W / 2).astype(numpy.int64) + 1) % pd["gy"]
+    d = (ax % p.tile) * p.tile + ay % p.tile  # unfused multiply-add: exact away from cell edges
+    same_key = ks[1:] == ks[:-1]
+    assert numpy.all(d[1:][same_key] >= d[:-1][same_key])
+    same = same_key & (d[1:] == d[:-1])
+    assert numpy.all(perm[1:][same] > perm[:-1][same])  # stable
