@@ -76,7 +76,9 @@ def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc):
     real FMA pair per cell and w plane) + 3 W c_ker (kernel taps, c_ker = 10) per visibility and
     direction; FFT 5 G^2 log2(G^2) per plane and polarisation. Bytes: each record is read once
     per pass of pc planes its w window meets, plus the planes written (gridder) or read
-    (degridder, with the sorted accumulators read and written)."""
+    (degridder, with the sorted accumulators read and written). The gridding of an invert is
+    split between the sub-tile gatherer ("grid": planes written, records outside dense runs)
+    and the dense-run gridder ("grid_dense": the plan's dense_items records)."""
     G2 = plan["gx"] * plan["gy"]
     N2 = plan["nx"] * plan["ny"]
     P = plan["nplanes"]
@@ -84,9 +86,14 @@ def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc):
     ww = W if plan["wstacking"] else 1
     rsz = 4 if csz == 8 else 8
     passes = (ww + pc - 1) / pc if plan["wstacking"] else 1  # passes an item takes part in
+    dense = int(plan.get("dense_items", 0))
+    per_vis = 4.0 * W * W * ww + 30.0 * W
     if name == "grid":
-        return (nvis * (4.0 * W * W * ww + 30.0 * W),
-                nitems * passes * (4 + 3 * rsz + nb * csz) + P * nb * G2 * csz)
+        items = nitems - dense
+        return (items * nb * per_vis,
+                items * passes * (4 + 3 * rsz + nb * csz) + P * nb * G2 * csz)
+    if name == "grid_dense":
+        return (dense * nb * per_vis, dense * passes * (3 * rsz + nb * csz))
     if name == "degrid":
         return (nvis * (4.0 * W * W * ww + 30.0 * W),
                 P * nb * G2 * csz + nitems * ww * (4 + 3 * rsz + 2 * nb * csz))
@@ -103,7 +110,8 @@ def measured_traffic(kernel, plan, prec, nb):
     """HBM bytes per launch of kernel class `kernel` from the committed rocprofv3 --pmc
     summaries (profiles/*_traffic.json): FETCH_SIZE and WRITE_SIZE per dispatch, each scaled
     by what the counter reads on a plain device copy of known size. Only a summary measured
-    on the same plan (precision, batch, kernel support, grid, planes) is used."""
+    on the same plan (precision, batch, kernel support, grid, planes, dense-run threshold) is
+    used."""
     import glob
 
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.json"))):
@@ -115,7 +123,8 @@ def measured_traffic(kernel, plan, prec, nb):
         pl = rec.get("plan", {})
         if (rec.get("kernel_class") == kernel and pl.get("precision") == prec and pl.get("nb") == nb
                 and pl.get("W") == plan.get("W") and pl.get("grid") == plan.get("gx")
-                and pl.get("nplanes") == plan.get("nplanes")):
+                and pl.get("nplanes") == plan.get("nplanes")
+                and pl.get("dense_min", 0) == plan.get("dense_min", 0)):
             return rec["traffic_bytes_per_launch"], os.path.basename(path)
     return None, None
 
@@ -599,7 +608,7 @@ def main():
     for tot, name, cnt in kern:
         plan = plan_of(name)
         per_step = cnt / args.steps
-        pc = max(1, round(plan["nplanes"] / per_step)) if name == "grid" else 1  # planes per pass
+        pc = max(1, round(plan["nplanes"] / per_step)) if name in ("grid", "grid_dense") else 1  # planes per pass
         flop, nbytes = algorithmic_work(name, plan, nvis, nrow * nchan, npol, csz, pc)
         if flop is None and nbytes is None:
             continue
@@ -635,7 +644,9 @@ def main():
         "dtype": {"fp64": "f64", "mixed": "f32", "fp32": "f32"}[prec], "data": "synthetic",
         "config": {"workload": wl["desc"], "nvis": nvis, "precision": prec,
                    "W": plan_i.get("W"), "sigma": plan_i.get("sigma"), "grid": plan_i.get("gx"),
-                   "nplanes": plan_i.get("nplanes"), "tile": plan_i.get("tile")},
+                   "nplanes": plan_i.get("nplanes"), "tile": plan_i.get("tile"),
+                   "dense_min": plan_i.get("dense_min"),
+                   "dense_fraction": round(plan_i.get("dense_items", 0) / max(1, nrow * nchan), 4)},
         "roofline": roof,
         "kernel_ms_total": breakdown,
     }

@@ -64,6 +64,10 @@ typedef struct hipng_plan {
   double umax, vmax, wmin, wmax; /* |u|,|v| maxima and |w| range, wavelengths */
   double epsilon;     /* requested accuracy */
   double est_error;   /* modelled kernel error */
+  int32_t dense_min;  /* shortest run for the dense-run gridder, 0 if that path is off */
+  int32_t reserved;
+  int64_t dense_items; /* visibilities (rows x channels) on the dense-run path; filled in
+                          plan_out by the compute calls (one extra device synchronisation) */
 } hipng_plan;
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -68,6 +68,9 @@ class HipngPlan(ctypes.Structure):
         ("wmax", ctypes.c_double),
         ("epsilon", ctypes.c_double),
         ("est_error", ctypes.c_double),
+        ("dense_min", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("dense_items", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -510,6 +510,12 @@ __global__ void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shif
   hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// orders the LDS accesses of one wave (no workgroup barrier)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // exclusive scan helpers (uint32), 4096 elements per block
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   const int lane = threadIdx.x & 63;
@@ -576,52 +582,91 @@ __global__ void k_scan_apply(uint32_t* __restrict__ a, int64_t m, const uint32_t
   }
 }
 
-__global__ void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                             uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
-                             int shift, const uint32_t* __restrict__ hist, int64_t nblocks) {
-  __shared__ uint32_t off[256];
-  __shared__ uint32_t wc[kBlock / 64][256];
+// Scatter of one pass. Wave w of block b ranks the contiguous items
+// [b kRsItems + w kRsItems/4, ...) 64 at a time (ballots find lanes with equal digit), so
+// wave-major order is item order and the result is stable. The block then places its items
+// in LDS in digit order and writes each digit's run to consecutive output positions
+// (coalesced) starting at the block's offset from the scanned histogram.
+__global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                       int64_t n, int shift, const uint32_t* __restrict__ hist,
+                                                       int64_t nblocks) {
+  constexpr int NW = kBlock / 64;
+  constexpr int R = kRsItems / kBlock;  // rounds of 64 items per wave
+  __shared__ uint32_t s_k[kRsItems];
+  __shared__ uint32_t s_v[kRsItems];
+  __shared__ uint32_t wcnt[NW][256];
+  __shared__ uint32_t lbase[256];
+  __shared__ uint32_t gbase[256];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  off[threadIdx.x] = hist[(int64_t)threadIdx.x * nblocks + blockIdx.x];
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t base = (int64_t)blockIdx.x * kRsItems;
-  for (int r = 0; r < kRsIpt; ++r) {
-    for (int w = 0; w < kBlock / 64; ++w) wc[w][threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t i = base + r * kBlock + threadIdx.x;
+  const int64_t wbase = base + (int64_t)wv * (kRsItems / NW);
+  for (int w = 0; w < NW; ++w) wcnt[w][threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t key[R], val[R], rk[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = wbase + r * 64 + lane;
     const bool ok = i < n;
-    uint32_t key = 0, val = 0, d = 0;
-    if (ok) {
-      key = kin[i];
-      val = vin[i];
-      d = (key >> shift) & 255u;
-    }
+    key[r] = ok ? kin[i] : 0u;
+    val[r] = ok ? vin[i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool ok = wbase + r * 64 + lane < n;
+    const uint32_t d = (key[r] >> shift) & 255u;
     unsigned long long peers = __ballot(ok);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const unsigned long long m = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
-    const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    if (ok && (peers & lt) == 0ull) wc[wv][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      const int dd = threadIdx.x;
-      uint32_t run = off[dd];
-      for (int w = 0; w < kBlock / 64; ++w) {
-        const uint32_t t = wc[w][dd];
-        wc[w][dd] = run;
-        run += t;
-      }
-      off[dd] = run;
+    const int leader = ok ? __ffsll((long long)peers) - 1 : 0;
+    uint32_t before = 0;
+    if (ok && lane == leader) {
+      before = wcnt[wv][d];
+      wcnt[wv][d] = before + (uint32_t)__popcll(peers);
     }
-    __syncthreads();
-    if (ok) {
-      const uint32_t pos = wc[wv][d] + rank;
-      kout[pos] = key;
-      vout[pos] = val;
+    before = __shfl(before, leader, 64);
+    rk[r] = before + (uint32_t)__popcll(peers & lt);
+    wave_sync();
+  }
+  __syncthreads();
+  {
+    // digit d = thread: block total, exclusive scan over digits, per-wave offsets
+    const int d = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < NW; ++w) tot += wcnt[w][d];
+    uint32_t all;
+    const uint32_t ex = block_excl_scan(tot, &all);
+    lbase[d] = ex;
+    gbase[d] = hist[(int64_t)d * nblocks + blockIdx.x];
+    uint32_t run = ex;
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t c = wcnt[w][d];
+      wcnt[w][d] = run;
+      run += c;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (wbase + r * 64 + lane < n) {
+      const uint32_t pos = wcnt[wv][(key[r] >> shift) & 255u] + rk[r];
+      s_k[pos] = key[r];
+      s_v[pos] = val[r];
+    }
+  }
+  __syncthreads();
+  const int cnt = (int)min((int64_t)kRsItems, n - base);
+  for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
+    const uint32_t kk = s_k[idx];
+    const uint32_t d = (kk >> shift) & 255u;
+    const uint32_t g = gbase[d] + (uint32_t)idx - lbase[d];
+    kout[g] = kk;
+    vout[g] = s_v[idx];
   }
 }
 
@@ -702,19 +747,44 @@ __global__ void k_records(const double* __restrict__ uvw, const uint32_t* __rest
 }
 
 // invert prologue: sorted values = wgt * conj?(ms) * exp(-2 pi i |w| f/c nshift)
+// Per sorted item, from the geometry alone: the phase exp(+2 pi i |w| nshift f/c) of the
+// predict epilogue (the invert prologue applies its conjugate) and the hermitian flip bit
+// (w < 0), packed 32 items per word. Computed once per geometry so that the per-call
+// prologue / epilogue read them in sorted order instead of gathering uvw rows.
 template <typename R, bool F32C>
-__global__ void k_values(const double* __restrict__ uvw, const uint32_t* __restrict__ perm, int64_t n,
-                         const double* __restrict__ pn, KP k, const cplx<R>* __restrict__ ms,
+__global__ void k_phase(const double* __restrict__ uvw, const uint32_t* __restrict__ perm, int64_t n,
+                        const double* __restrict__ pn, KP k, cplx<R>* __restrict__ rec_p,
+                        uint32_t* __restrict__ flipb) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool flip = false;
+  if (j < n) {
+    const uint32_t i = perm[j];
+    const int64_t row = i / k.nchan;
+    const int ch = (int)(i - row * k.nchan);
+    const double w = uvw[3 * row + 2];
+    flip = w < 0.0;
+    cplx<R> ph{(R)1, (R)0};
+    if (k.wstack) phase_cs<R, F32C>(fabs(w) * pn[ch], &ph.re, &ph.im);
+    rec_p[j] = ph;
+  }
+  const unsigned long long m = __ballot(flip);
+  const int64_t w0 = (j - lane) >> 5;  // waves start at multiples of 64 items
+  if (lane == 0 && j < n) flipb[w0] = (uint32_t)m;
+  if (lane == 32 && j < n) flipb[w0 + 1] = (uint32_t)(m >> 32);
+}
+
+// invert prologue: rec_v[j] = conj?(wgt ms[perm[j]]) exp(-2 pi i |w| nshift f/c)
+template <typename R>
+__global__ void k_values(const uint32_t* __restrict__ perm, int64_t n, const cplx<R>* __restrict__ rec_p,
+                         const uint32_t* __restrict__ flipb, KP k, const cplx<R>* __restrict__ ms,
                          const R* __restrict__ wgt, cplx<R>* __restrict__ rec_v) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
-  const int64_t row = i / k.nchan;
-  const int ch = (int)(i - row * k.nchan);
-  const double w = uvw[3 * row + 2];
-  const bool flip = w < 0.0;
-  R c = 1, s = 0;
-  if (k.wstack) phase_cs<R, F32C>(-fabs(w) * pn[ch], &c, &s);
+  const bool flip = (flipb[j >> 5] >> (j & 31)) & 1u;
+  const cplx<R> ph = rec_p[j];
+  const R c = ph.re, s = -ph.im;
   for (int b = 0; b < k.nb; ++b) {
     cplx<R> v = ms[(int64_t)i * k.nb + b];
     if (flip) v.im = -v.im;
@@ -1021,10 +1091,6 @@ __global__ __launch_bounds__(kBlock) void k_grid(KP k, int p0, int npl, const ui
 // accumulates all staged visibilities into its cell without branches. Only wave-level
 // ordering is used, so the waves of a workgroup never wait for each other.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
 
 // per-wave LDS bytes of k_grid8: values [NB][64], taps [64][(16 + PC) | 1], queue [3][128],
 // segments. The tap row stride is odd so that the 64 staging lanes write to distinct banks.
@@ -1869,19 +1935,16 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
 }
 
 // predict epilogue: ms[i] = wgt * conj?(acc_sorted[j] exp(+2 pi i |w| nshift))
-template <typename R, bool F32C>
-__global__ void k_unsort(const double* __restrict__ uvw, const uint32_t* __restrict__ perm, int64_t n,
-                         const double* __restrict__ pn, KP k, const cplx<R>* __restrict__ accs,
+template <typename R>
+__global__ void k_unsort(const uint32_t* __restrict__ perm, int64_t n, const cplx<R>* __restrict__ rec_p,
+                         const uint32_t* __restrict__ flipb, KP k, const cplx<R>* __restrict__ accs,
                          const R* __restrict__ wgt, cplx<R>* __restrict__ ms) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
-  const int64_t row = i / k.nchan;
-  const int ch = (int)(i - row * k.nchan);
-  const double w = uvw[3 * row + 2];
-  const bool flip = w < 0.0;
-  R c = 1, s = 0;
-  if (k.wstack) phase_cs<R, F32C>(fabs(w) * pn[ch], &c, &s);
+  const bool flip = (flipb[j >> 5] >> (j & 31)) & 1u;
+  const cplx<R> ph = rec_p[j];
+  const R c = ph.re, s = ph.im;
   for (int b = 0; b < k.nb; ++b) {
     const cplx<R> a = accs[j * k.nb + b];
     cplx<R> o;
@@ -2080,7 +2143,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_DTASK,
-  S_DPLANE, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_NSLOTS
 };
 
 struct FftKey {
@@ -2370,7 +2433,8 @@ struct Geom {
   int64_t nrow = 0, n = 0;
   int nchan = 0, nb = 0, prec = 0, device = 0;
   const double* uvw = nullptr;
-  uint32_t *perm = nullptr, *bstart = nullptr, *bend = nullptr, *rec_a = nullptr;
+  uint32_t *perm = nullptr, *bstart = nullptr, *bend = nullptr, *rec_a = nullptr, *flipb = nullptr;
+  void* rec_p = nullptr;  // per sorted item phase (cplx<R>) of the unsort epilogue
   void *rec_f = nullptr, *plane = nullptr, *accimg = nullptr, *recv = nullptr, *fftwork = nullptr;
   size_t fftwork_size = 0;
   // pruned plane transforms: grid rows [xlo, xhi) (band, along gy), image side T (along gx)
@@ -2488,6 +2552,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   if (g.dense) {
     g.k.amask = 0x7fff7fffu;
     g.k.dflag = 0x80000000u;
+    g.P.dense_min = (int32_t)dense_min_run();
   }
   if (own) {
     g.perm = (uint32_t*)g.buf.get(S_VALS0, n * 4);
@@ -2519,6 +2584,21 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     else
       k_records<float, true><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.k,
                                                           g.rec_a, (rec3<float>*)g.rec_f);
+    LAUNCH_CHECK();
+  }
+  g.rec_p = g.buf.get(S_RECP, (size_t)n * csz);
+  g.flipb = (uint32_t*)g.buf.get(S_FLIPB, (size_t)((n + 31) / 32 + 2) * 4);
+  {
+    PROF("records");
+    if (prec == HIPNG_FP64)
+      k_phase<double, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.pn, g.k,
+                                                          (cplx<double>*)g.rec_p, g.flipb);
+    else if (prec == HIPNG_MIXED)
+      k_phase<float, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.pn, g.k, (cplx<float>*)g.rec_p,
+                                                         g.flipb);
+    else
+      k_phase<float, true><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.pn, g.k, (cplx<float>*)g.rec_p,
+                                                        g.flipb);
     LAUNCH_CHECK();
   }
   g.recv = g.buf.get(S_RECV, (size_t)n * nb * csz);
@@ -2676,8 +2756,8 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   auto* recv = (cplx<R>*)g.recv;
   {
     PROF("values");
-    k_values<R, F32C><<<grid1(g.n), kBlock, 0, st>>>(g.uvw, g.perm, g.n, g.tb.pn, k, (const cplx<R>*)ms,
-                                                       (const R*)wgt, recv);
+    k_values<R><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, (const cplx<R>*)g.rec_p, g.flipb, k,
+                                                (const cplx<R>*)ms, (const R*)wgt, recv);
     LAUNCH_CHECK();
   }
   auto* acc = (A*)g.accimg;
@@ -2839,8 +2919,8 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     }
   }
   PROF("unsort");
-  k_unsort<R, F32C><<<grid1(g.n), kBlock, 0, st>>>(g.uvw, g.perm, g.n, g.tb.pn, k, accs, (const R*)wgt,
-                                                   (cplx<R>*)ms);
+  k_unsort<R><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, (const cplx<R>*)g.rec_p, g.flipb, k, accs,
+                                              (const R*)wgt, (cplx<R>*)ms);
   LAUNCH_CHECK();
 }
 
@@ -2858,6 +2938,30 @@ void exec_dispatch(const Geom& g, bool invert, const void* in, const void* wgt, 
     if (invert) exec_ms2dirty<float, float, true>(g, in, wgt, out, st);
     else exec_dirty2ms<float, float, true>(g, in, wgt, out, st);
   }
+}
+
+// dense-run tasks and the visibilities they hold (device reduction, one synchronisation)
+void dense_counts(const Geom& g, hipStream_t st, int64_t* ntask, int64_t* nvis) {
+  *ntask = 0;
+  *nvis = 0;
+  if (!g.dense || g.n == 0) return;
+  auto* d = (unsigned long long*)ws(ctx(), S_EXT, 2 * sizeof(unsigned long long));
+  k_zero<unsigned long long><<<1, kBlock, 0, st>>>(d, 2);
+  LAUNCH_CHECK();
+  k_dense_stats<<<256, kBlock, 0, st>>>(g.dtask, g.counters, d);
+  LAUNCH_CHECK();
+  unsigned long long h[2] = {0ull, 0ull};
+  HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  *ntask = (int64_t)h[0];
+  *nvis = (int64_t)h[1];
+}
+
+void fill_plan_out(const Geom& g, hipStream_t st, hipng_plan* out) {
+  if (!out) return;
+  *out = g.P;
+  int64_t nt = 0;
+  dense_counts(g, st, &nt, &out->dense_items);
 }
 
 template <typename F>
@@ -2948,7 +3052,7 @@ int32_t hipng_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int3
     Geom g;
     build_geom(g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
                epsilon, do_wstacking, precision, st, false);
-    if (plan_out) *plan_out = g.P;
+    fill_plan_out(g, st, plan_out);
     exec_dispatch(g, true, ms, wgt, dirty, st);
   });
 }
@@ -2963,7 +3067,7 @@ int32_t hipng_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int3
     Geom g;
     build_geom(g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
                epsilon, do_wstacking, precision, st, false);
-    if (plan_out) *plan_out = g.P;
+    fill_plan_out(g, st, plan_out);
     exec_dispatch(g, false, dirty, wgt, ms, st);
   });
 }
@@ -2989,7 +3093,7 @@ int32_t hipng_geometry_create(const double* uvw, const double* freq, int64_t nro
       delete h;
       throw;
     }
-    if (plan_out) *plan_out = h->g.P;
+    fill_plan_out(h->g, (hipStream_t)stream, plan_out);
     *out = h;
   });
 }
@@ -3014,23 +3118,11 @@ int32_t hipng_geometry_stats(hipng_geometry geom, int64_t* out, int32_t nout) {
   return guarded([&] {
     if (!geom || !out || nout < 1) HIPNG_THROW(HIPNG_EINVAL, "geometry and output array required");
     const Geom& g = geom->g;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != g.device) HIPNG_THROW(HIPNG_EINVAL, "geometry was built on device %d, current device is %d", g.device, dev);
     int64_t v[4] = {g.n, 0, 0, g.dense ? 1 : 0};
-    if (g.dense && g.n > 0) {
-      int cur = 0;
-      HIP_TRY(hipGetDevice(&cur));
-      HIP_TRY(hipSetDevice(g.device));
-      unsigned long long* d = nullptr;
-      HIP_TRY(hipMalloc(&d, 2 * sizeof(unsigned long long)));
-      HIP_TRY(hipMemset(d, 0, 2 * sizeof(unsigned long long)));
-      k_dense_stats<<<256, kBlock>>>(g.dtask, g.counters, d);
-      LAUNCH_CHECK();
-      unsigned long long h[2] = {0ull, 0ull};
-      HIP_TRY(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
-      HIP_TRY(hipFree(d));
-      HIP_TRY(hipSetDevice(cur));
-      v[1] = (int64_t)h[0];
-      v[2] = (int64_t)h[1];
-    }
+    dense_counts(g, nullptr, &v[1], &v[2]);
     for (int i = 0; i < nout && i < 4; ++i) out[i] = v[i];
   });
 }
