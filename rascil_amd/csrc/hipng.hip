@@ -1113,6 +1113,7 @@ __device__ __forceinline__ double esk_fast(double z, double beta) { return esk(z
 template <typename R, int T, int PC, int NB>
 __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                   const uint32_t* __restrict__ bend,
+                                                  const uint32_t* __restrict__ sidx,
                                                   const uint32_t* __restrict__ rec_a,
                                                   const rec3<R>* __restrict__ rec_f,
                                                   const cplx<R>* __restrict__ rec_v,
@@ -1282,6 +1283,7 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
         if (v < vhi) {
           while (lo + 1 < nseg && s_segv[lo + 1] <= v) ++lo;
           j = s_segj[lo] + (v - s_segv[lo]);
+          if (sidx) j = sidx[j];
           sp = slo + lo / 4;
           const uint32_t Ar = rec_a[j], A = Ar & k.amask;
           dx = (int)(A & 0xffffu) - tx * T;
@@ -1419,6 +1421,26 @@ __global__ void k_run_tasks(const uint32_t* __restrict__ keys, const uint32_t* _
     }
   }
   if (r == n - 1) counters[kDenseTotal] = coff[r] + nt;
+}
+
+// keep[j] = 1 for records outside dense runs
+__global__ void k_sparse_flags(const uint32_t* __restrict__ rec_a, int64_t n, uint32_t flag,
+                               uint32_t* __restrict__ keep) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  keep[j] = (rec_a[j] & flag) ? 0u : 1u;
+}
+
+// with pos the exclusive scan of keep: sidx[pos] = j, skey[pos] = key of j
+__global__ void k_sparse_compact(const uint32_t* __restrict__ rec_a, const uint32_t* __restrict__ keys, int64_t n,
+                                 uint32_t flag, const uint32_t* __restrict__ pos, uint32_t* __restrict__ sidx,
+                                 uint32_t* __restrict__ skey) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  if (!(rec_a[j] & flag)) {
+    sidx[pos[j]] = (uint32_t)j;
+    skey[pos[j]] = keys[j];
+  }
 }
 
 // diagnostics: out[0] += dense-run tasks, out[1] += visibilities they hold
@@ -2143,7 +2165,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SIDX, S_SBSTART, S_SBEND, S_NSLOTS
 };
 
 struct FftKey {
@@ -2449,6 +2471,8 @@ struct Geom {
   bool dense = false;
   uint4* dtask = nullptr;
   uint32_t* dplane = nullptr;
+  // records outside dense runs: sorted record indices and their bucket table
+  uint32_t *sidx = nullptr, *sbstart = nullptr, *sbend = nullptr;
   Buffers buf;
 };
 
@@ -2669,17 +2693,41 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     k_run_tasks<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, scr1, scr0, g.counters, n, dmin, kDenseChunk,
                                              g.k.ntiles, g.k.amask, g.dtask, g.dplane);
     LAUNCH_CHECK();
+    // compacted records outside dense runs for the sub-tile gatherer: scr0 = keep flags and
+    // their scan, scr1 = compacted keys (n + 1 entries each); one synchronisation to size
+    // the index list
+    k_sparse_flags<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, g.k.dflag, scr0);
+    LAUNCH_CHECK();
+    uint32_t last_keep = 0, last_pos = 0;
+    HIP_TRY(hipMemcpyAsync(&last_keep, scr0 + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    excl_scan(scr0, n, part, st);
+    HIP_TRY(hipMemcpyAsync(&last_pos, scr0 + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const int64_t ns = (int64_t)last_pos + last_keep;
+    g.sidx = (uint32_t*)g.buf.get(S_SIDX, (size_t)std::max<int64_t>(ns, 1) * 4);
+    g.sbstart = (uint32_t*)g.buf.get(S_SBSTART, nkeys * 4);
+    g.sbend = (uint32_t*)g.buf.get(S_SBEND, nkeys * 4);
+    k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(g.sbstart, nkeys, 0u);
+    k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(g.sbend, nkeys, 0u);
+    LAUNCH_CHECK();
+    if (ns > 0) {
+      k_sparse_compact<<<grid1(n), kBlock, 0, st>>>(g.rec_a, keys, n, g.k.dflag, scr0, g.sidx, scr1);
+      LAUNCH_CHECK();
+      k_bounds<<<grid1(ns), kBlock, 0, st>>>(scr1, ns, g.sbstart, g.sbend);
+      LAUNCH_CHECK();
+    }
   }
   if (own) HIP_TRY(hipStreamSynchronize(st));
 }
 
 // per-plane task list: counters reset, per-tile task counts, exclusive scan -> task offsets
-void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st, uint32_t mult = 1) {
+void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st, uint32_t mult = 1,
+                 const uint32_t* bs = nullptr, const uint32_t* be = nullptr) {
   const int64_t ntiles = g.k.ntiles;
   k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, kDenseCnt + 1, 0u);
   LAUNCH_CHECK();
-  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, g.bstart, g.bend, chunk, g.task, g.counters,
-                                            mult);
+  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, bs ? bs : g.bstart, be ? be : g.bend, chunk, g.task,
+                                            g.counters, mult);
   LAUNCH_CHECK();
   const int64_t npart = (ntiles + kRsItems - 1) / kRsItems;
   k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
@@ -2772,6 +2820,8 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   const int S = (int)std::min<size_t>(kBlock, (48 * 1024 / per) / 64 * 64);
   const size_t sm = per * S;
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
+  const uint32_t* gb0 = g.sidx ? g.sbstart : g.bstart;
+  const uint32_t* gb1 = g.sidx ? g.sbend : g.bend;
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
     const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn, g.nb);
@@ -2780,7 +2830,9 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       PROF("grid_setup");
       int slo, shi;
       chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
-      build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? P.tile / 8 : 1);
+      // the sub-tile gatherer walks the compacted table of records outside dense runs
+      build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? P.tile / 8 : 1, wave ? gb0 : nullptr,
+                  wave ? gb1 : nullptr);
       // footprints stay inside the band, so only its rows need clearing
       const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
       for (int i = 0; i < npl * g.nb; ++i)
@@ -2790,7 +2842,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       PROF("grid");
       if (wave) {
 #define HIPNG_GW(TT, QQ, BB)                                                                      \
-  k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,   \
+  k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.sidx, g.rec_a,   \
                                                           (const rec3<R>*)g.rec_f, recv, g.task,  \
                                                           g.chunk_grid, g.counters, plane)
 #define HIPNG_GWB(TT, QQ)                 \
