@@ -1858,6 +1858,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);  // [npl][L][L][NB]
   __shared__ uint32_t s_task;
+  __shared__ uint32_t s_segj[64], s_segv[65];  // per start plane: first record, exclusive prefix
   const int T = k.T, L = k.T + W - 1, LL = L * L;
   const R scale = (R)2 / (R)W;
   // the FP32 tap evaluation works in base 2
@@ -1865,6 +1866,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
   const uint32_t total = counters[1];
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
+  const int nseg = shi - slo + 1;  // <= W_w + npl - 1 <= 64
   int loaded = -1;
   for (;;) {
     if (threadIdx.x == 0) s_task = atomicAdd(&counters[0], 1u);
@@ -1888,14 +1890,43 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
       loaded = tile;
       __syncthreads();
     }
-    uint32_t r = 0;
-    for (int s = slo; s <= shi && r < vhi; ++s) {
-      const uint32_t key = (uint32_t)s * k.ntiles + (uint32_t)tile;
-      const uint32_t st = bstart[key], len = bend[key] - st;
-      const uint32_t a = max(vlo, r), bnd = min(vhi, r + len);
+    // the task's candidates as one list over the buckets of start planes [slo, shi]
+    // (segment table in LDS), so that every lane takes an item whatever the bucket sizes
+    if (threadIdx.x < 64) {
+      const int sg = threadIdx.x;
+      uint32_t len = 0;
+      if (sg < nseg) {
+        const uint32_t key = (uint32_t)(slo + sg) * k.ntiles + (uint32_t)tile;
+        const uint32_t st0 = bstart[key];
+        len = bend[key] - st0;
+        s_segj[sg] = st0;
+      }
+      uint32_t inc = len;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t tt = __shfl_up(inc, o, 64);
+        if (sg >= o) inc += tt;
+      }
+      if (sg < nseg) s_segv[sg + 1] = inc;
+      if (sg == 0) s_segv[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t vend = min(vhi, s_segv[nseg]);
+    for (uint32_t vi = vlo + threadIdx.x; vi < vend; vi += kBlock) {
+      int sg = 0;
+      {
+        int hi = nseg;  // last segment starting at or before vi
+        while (hi - sg > 1) {
+          const int mid = (sg + hi) >> 1;
+          if (s_segv[mid] <= vi) sg = mid;
+          else hi = mid;
+        }
+      }
+      const uint32_t j = s_segj[sg] + (vi - s_segv[sg]);
+      const int s = slo + sg;
       // planes of the pass inside the W_w-plane window of start plane s
       const int qlo = max(0, s - p0), qhi = min(npl, s + k.Ww - p0);
-      for (uint32_t j = st + (a - r) + threadIdx.x; a < bnd && j < st + (bnd - r); j += kBlock) {
+      {
         const uint32_t A = rec_a[j] & k.amask;
         const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
         const rec3<R> f = rec_f[j];
@@ -1950,7 +1981,6 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
           }
         }
       }
-      r += len;
     }
     __syncthreads();
   }
