@@ -726,35 +726,17 @@ __device__ __forceinline__ void phase_cs<float, true>(double turns, float* c, fl
   sincospif(2.0f * t, s, c);
 }
 
+// Per sorted item, from the geometry alone: packed first cell, kernel offsets, the phase
+// exp(+2 pi i |w| nshift f/c) of the predict epilogue (the invert prologue applies its
+// conjugate) and the hermitian flip bit (w < 0), packed 32 items per word. The phase and flip
+// are kept so that the per-call prologue / epilogue read them in sorted order instead of
+// gathering uvw rows again.
 template <typename R, bool F32C>
 __global__ void k_records(const double* __restrict__ uvw, const uint32_t* __restrict__ perm,
                           int64_t n, const double* __restrict__ sx, const double* __restrict__ sy,
-                          const double* __restrict__ sw, KP k, uint32_t* __restrict__ rec_a,
-                          rec3<R>* __restrict__ rec_f) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t i = perm[j];
-  const int64_t row = i / k.nchan;
-  const int ch = (int)(i - row * k.nchan);
-  const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch], sy[ch],
-                                   sw[ch], k);
-  rec_a[j] = (uint32_t)p.ax | ((uint32_t)p.ay << 16);
-  rec3<R> f;
-  f.x = (R)p.fx;
-  f.y = (R)p.fy;
-  f.w = (R)p.fw;
-  rec_f[j] = f;
-}
-
-// invert prologue: sorted values = wgt * conj?(ms) * exp(-2 pi i |w| f/c nshift)
-// Per sorted item, from the geometry alone: the phase exp(+2 pi i |w| nshift f/c) of the
-// predict epilogue (the invert prologue applies its conjugate) and the hermitian flip bit
-// (w < 0), packed 32 items per word. Computed once per geometry so that the per-call
-// prologue / epilogue read them in sorted order instead of gathering uvw rows.
-template <typename R, bool F32C>
-__global__ void k_phase(const double* __restrict__ uvw, const uint32_t* __restrict__ perm, int64_t n,
-                        const double* __restrict__ pn, KP k, cplx<R>* __restrict__ rec_p,
-                        uint32_t* __restrict__ flipb) {
+                          const double* __restrict__ sw, const double* __restrict__ pn, KP k,
+                          uint32_t* __restrict__ rec_a, rec3<R>* __restrict__ rec_f,
+                          cplx<R>* __restrict__ rec_p, uint32_t* __restrict__ flipb) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool flip = false;
@@ -763,7 +745,14 @@ __global__ void k_phase(const double* __restrict__ uvw, const uint32_t* __restri
     const int64_t row = i / k.nchan;
     const int ch = (int)(i - row * k.nchan);
     const double w = uvw[3 * row + 2];
-    flip = w < 0.0;
+    const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], w, sx[ch], sy[ch], sw[ch], k);
+    rec_a[j] = (uint32_t)p.ax | ((uint32_t)p.ay << 16);
+    rec3<R> f;
+    f.x = (R)p.fx;
+    f.y = (R)p.fy;
+    f.w = (R)p.fw;
+    rec_f[j] = f;
+    flip = p.flip;
     cplx<R> ph{(R)1, (R)0};
     if (k.wstack) phase_cs<R, F32C>(fabs(w) * pn[ch], &ph.re, &ph.im);
     rec_p[j] = ph;
@@ -1372,75 +1361,75 @@ __global__ void k_run_starts(const uint32_t* __restrict__ keys, const uint32_t* 
   }
 }
 
-// flag the records of runs with at least minlen visibilities
+// per run r < nruns: tcnt[r] = tasks of a dense run (len >= minlen), dlen[r] = its length
+// (0 for short runs); entry nruns is 0 so that the exclusive scans end with the totals
+__global__ void k_run_info(const uint32_t* __restrict__ rstart, int64_t nruns, uint32_t minlen, uint32_t chunk,
+                           uint32_t* __restrict__ tcnt, uint32_t* __restrict__ dlen) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r > nruns) return;
+  uint32_t t = 0, d = 0;
+  if (r < nruns) {
+    const uint32_t len = rstart[r + 1] - rstart[r];
+    if (len >= minlen) {
+      t = (len + chunk - 1) / chunk;
+      d = len;
+    }
+  }
+  tcnt[r] = t;
+  dlen[r] = d;
+}
+
+// with toff the exclusive scan of the task counts: the task list (first record, count,
+// packed first cell, start plane) and each task's start plane alone; tasks are in record
+// order, so their start planes ascend
+__global__ void k_run_tasks(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a,
+                            const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ toff,
+                            int64_t nruns, uint32_t minlen, uint32_t chunk, uint32_t ntiles, uint32_t amask,
+                            uint4* __restrict__ dtask, uint32_t* __restrict__ dplane,
+                            uint32_t* __restrict__ counters) {
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r == 0) counters[kDenseTotal] = toff[nruns];
+  if (r >= nruns) return;
+  const uint32_t st = rstart[r], len = rstart[r + 1] - st;
+  if (len < minlen) return;
+  const uint32_t nt = (len + chunk - 1) / chunk;
+  const uint32_t sp = keys[st] / ntiles, A = rec_a[st] & amask;
+  for (uint32_t c = 0; c < nt; ++c) {
+    dtask[toff[r] + c] = make_uint4(st + c * chunk, min(chunk, len - c * chunk), A, sp);
+    dplane[toff[r] + c] = sp;
+  }
+}
+
+// per record: its run (A = exclusive scan of the run heads), dense runs flagged in rec_a,
+// the other records listed in sidx at their rank among non-dense records (dpre = exclusive
+// scan of the dense run lengths)
 __global__ void k_run_mark(uint32_t* __restrict__ rec_a, int64_t n, const uint32_t* __restrict__ A,
-                           const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ counters,
-                           uint32_t minlen, uint32_t flag) {
+                           const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ dpre,
+                           const uint32_t* __restrict__ counters, uint32_t minlen, uint32_t flag,
+                           uint32_t* __restrict__ sidx) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t nruns = counters[kRuns];
   const uint32_t a = A[j];
   const uint32_t r = (a < nruns && rstart[a] == (uint32_t)j) ? a : a - 1u;
   if (rstart[r + 1] - rstart[r] >= minlen) rec_a[j] |= flag;
+  else sidx[(uint32_t)j - dpre[r]] = (uint32_t)j;
 }
 
-// cnt[r] = tasks of run r (0 for short runs and for r >= nruns), r in [0, n)
-__global__ void k_run_count(const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ counters, int64_t n,
-                            uint32_t minlen, uint32_t chunk, uint32_t* __restrict__ cnt) {
-  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= n) return;
-  uint32_t c = 0;
-  if (r < (int64_t)counters[kRuns]) {
-    const uint32_t len = rstart[r + 1] - rstart[r];
-    if (len >= minlen) c = (len + chunk - 1) / chunk;
-  }
-  cnt[r] = c;
-}
-
-// with coff the exclusive scan of the task counts: the task list (first record, count,
-// packed first cell, start plane) and each task's start plane alone; tasks are in record
-// order, so their start planes ascend
-__global__ void k_run_tasks(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a,
-                            const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ coff,
-                            uint32_t* __restrict__ counters, int64_t n, uint32_t minlen, uint32_t chunk,
-                            uint32_t ntiles, uint32_t amask, uint4* __restrict__ dtask,
-                            uint32_t* __restrict__ dplane) {
-  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (r >= n) return;
+// bucket table of the non-dense records: a bucket's first record and end are run starts
+// (or n), so the non-dense records before them are the position minus the dense ones
+__global__ void k_sparse_bounds(const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
+                                int64_t nkeys, int64_t n, const uint32_t* __restrict__ A,
+                                const uint32_t* __restrict__ dpre, const uint32_t* __restrict__ counters,
+                                uint32_t* __restrict__ sbstart, uint32_t* __restrict__ sbend) {
+  const int64_t key = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (key >= nkeys) return;
   const uint32_t nruns = counters[kRuns];
-  uint32_t nt = 0;
-  if (r < (int64_t)nruns) {
-    const uint32_t st = rstart[r], len = rstart[r + 1] - st;
-    if (len >= minlen) {
-      nt = (len + chunk - 1) / chunk;
-      const uint32_t sp = keys[st] / ntiles, A = rec_a[st] & amask;
-      for (uint32_t c = 0; c < nt; ++c) {
-        dtask[coff[r] + c] = make_uint4(st + c * chunk, min(chunk, len - c * chunk), A, sp);
-        dplane[coff[r] + c] = sp;
-      }
-    }
-  }
-  if (r == n - 1) counters[kDenseTotal] = coff[r] + nt;
-}
-
-// keep[j] = 1 for records outside dense runs
-__global__ void k_sparse_flags(const uint32_t* __restrict__ rec_a, int64_t n, uint32_t flag,
-                               uint32_t* __restrict__ keep) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  keep[j] = (rec_a[j] & flag) ? 0u : 1u;
-}
-
-// with pos the exclusive scan of keep: sidx[pos] = j, skey[pos] = key of j
-__global__ void k_sparse_compact(const uint32_t* __restrict__ rec_a, const uint32_t* __restrict__ keys, int64_t n,
-                                 uint32_t flag, const uint32_t* __restrict__ pos, uint32_t* __restrict__ sidx,
-                                 uint32_t* __restrict__ skey) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  if (!(rec_a[j] & flag)) {
-    sidx[pos[j]] = (uint32_t)j;
-    skey[pos[j]] = keys[j];
-  }
+  const uint32_t x0 = bstart[key], x1 = bend[key];
+  const uint32_t d0 = (int64_t)x0 < n ? dpre[A[x0]] : dpre[nruns];
+  const uint32_t d1 = (int64_t)x1 < n ? dpre[A[x1]] : dpre[nruns];
+  sbstart[key] = x0 - d0;
+  sbend[key] = x1 - d1;
 }
 
 // diagnostics: out[0] += dense-run tasks, out[1] += visibilities they hold
@@ -2195,7 +2184,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_SIDX, S_SBSTART, S_SBEND, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SIDX, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_NSLOTS
 };
 
 struct FftKey {
@@ -2396,6 +2385,9 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
 }
 
 // keys + stable radix sort in the shared workspace; returns sorted keys and permutation
+// The bucket keys are recomputed from uvw in digit order (k_rekey): uvw rows are shared by all
+// channels of a row and stay cache resident, which makes this cheaper than keeping the keys
+// of every item and gathering them (measured at the production shape).
 void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const KP& k,
                 const hipng_plan& P, hipStream_t st, uint32_t** keys_out, uint32_t** perm_out,
                 uint32_t** free0 = nullptr, uint32_t** free1 = nullptr) {
@@ -2599,6 +2591,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     g.uvw = uvw;
   }
   uint32_t *keys, *perm, *scr0, *scr1;
+  g.rec_a = (uint32_t*)g.buf.get(S_RECA, n * 4);
   sort_items(c, g.uvw, n, g.tb, g.k, P, st, &keys, &perm, &scr0, &scr1);
   // dense-run gridding needs W <= 8 (one lane per footprint cell) and a free flag bit
   // in the packed first cell (grids up to 32768 cells per axis)
@@ -2625,34 +2618,23 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     k_bounds<<<grid1(n), kBlock, 0, st>>>(keys, n, g.bstart, g.bend);
     LAUNCH_CHECK();
   }
-  g.rec_a = (uint32_t*)g.buf.get(S_RECA, n * 4);
   g.rec_f = g.buf.get(S_RECF, n * (prec == HIPNG_FP64 ? sizeof(rec3<double>) : sizeof(rec3<float>)));
-  {
-    PROF("records");
-    if (prec == HIPNG_FP64)
-      k_records<double, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.k,
-                                                            g.rec_a, (rec3<double>*)g.rec_f);
-    else if (prec == HIPNG_MIXED)
-      k_records<float, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.k,
-                                                           g.rec_a, (rec3<float>*)g.rec_f);
-    else
-      k_records<float, true><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.k,
-                                                          g.rec_a, (rec3<float>*)g.rec_f);
-    LAUNCH_CHECK();
-  }
   g.rec_p = g.buf.get(S_RECP, (size_t)n * csz);
   g.flipb = (uint32_t*)g.buf.get(S_FLIPB, (size_t)((n + 31) / 32 + 2) * 4);
   {
     PROF("records");
     if (prec == HIPNG_FP64)
-      k_phase<double, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.pn, g.k,
-                                                          (cplx<double>*)g.rec_p, g.flipb);
+      k_records<double, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.tb.pn,
+                                                            g.k, g.rec_a, (rec3<double>*)g.rec_f,
+                                                            (cplx<double>*)g.rec_p, g.flipb);
     else if (prec == HIPNG_MIXED)
-      k_phase<float, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.pn, g.k, (cplx<float>*)g.rec_p,
-                                                         g.flipb);
+      k_records<float, false><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.tb.pn,
+                                                           g.k, g.rec_a, (rec3<float>*)g.rec_f,
+                                                           (cplx<float>*)g.rec_p, g.flipb);
     else
-      k_phase<float, true><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.pn, g.k, (cplx<float>*)g.rec_p,
-                                                        g.flipb);
+      k_records<float, true><<<grid1(n), kBlock, 0, st>>>(g.uvw, g.perm, n, g.tb.sx, g.tb.sy, g.tb.sw, g.tb.pn,
+                                                          g.k, g.rec_a, (rec3<float>*)g.rec_f,
+                                                          (cplx<float>*)g.rec_p, g.flipb);
     LAUNCH_CHECK();
   }
   g.recv = g.buf.get(S_RECV, (size_t)n * nb * csz);
@@ -2704,48 +2686,44 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   g.chunk_degrid = pick_chunk((double)n * ww / P.nplanes, g.npersist);
   if (g.dense) {
     PROF("runs");
-    // scratch: scr0 = heads / run ids / task offsets, scr1 = run starts (n + 1 entries)
-    uint32_t* part = (uint32_t*)ws(c, S_HIST, (size_t)((n + kRsItems - 1) / kRsItems) * 4);
+    // scr0 = run heads, then their exclusive scan (run of each record); scr1 = run starts
+    // (n + 1 entries); per-run task counts and dense lengths in workspace, scanned over the
+    // runs only. Two synchronisations: the run count and the dense total size the lists.
+    uint32_t* part = (uint32_t*)ws(c, S_HIST, (size_t)((n + kRsItems) / kRsItems + 1) * 4);
     const uint32_t dmin = dense_min_run();
-    const int64_t maxt = n / dmin + n / kDenseChunk + 1;
-    g.dtask = (uint4*)g.buf.get(S_DTASK, (size_t)maxt * sizeof(uint4));
-    g.dplane = (uint32_t*)g.buf.get(S_DPLANE, (size_t)maxt * 4);
     k_run_heads<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, n, scr0);
     LAUNCH_CHECK();
     excl_scan(scr0, n, part, st);
     k_run_starts<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, n, scr0, scr1, g.counters);
     LAUNCH_CHECK();
-    k_run_mark<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, scr0, scr1, g.counters, dmin, g.k.dflag);
-    LAUNCH_CHECK();
-    k_run_count<<<grid1(n), kBlock, 0, st>>>(scr1, g.counters, n, dmin, kDenseChunk, scr0);
-    LAUNCH_CHECK();
-    excl_scan(scr0, n, part, st);
-    k_run_tasks<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, scr1, scr0, g.counters, n, dmin, kDenseChunk,
-                                             g.k.ntiles, g.k.amask, g.dtask, g.dplane);
-    LAUNCH_CHECK();
-    // compacted records outside dense runs for the sub-tile gatherer: scr0 = keep flags and
-    // their scan, scr1 = compacted keys (n + 1 entries each); one synchronisation to size
-    // the index list
-    k_sparse_flags<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, g.k.dflag, scr0);
-    LAUNCH_CHECK();
-    uint32_t last_keep = 0, last_pos = 0;
-    HIP_TRY(hipMemcpyAsync(&last_keep, scr0 + (n - 1), 4, hipMemcpyDeviceToHost, st));
-    excl_scan(scr0, n, part, st);
-    HIP_TRY(hipMemcpyAsync(&last_pos, scr0 + (n - 1), 4, hipMemcpyDeviceToHost, st));
+    uint32_t nruns = 0;
+    HIP_TRY(hipMemcpyAsync(&nruns, g.counters + kRuns, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    const int64_t ns = (int64_t)last_pos + last_keep;
+    auto* tcnt = (uint32_t*)ws(c, S_RUNT, ((size_t)nruns + 1) * 4);
+    auto* dpre = (uint32_t*)ws(c, S_RUND, ((size_t)nruns + 1) * 4);
+    k_run_info<<<grid1((int64_t)nruns + 1), kBlock, 0, st>>>(scr1, nruns, dmin, kDenseChunk, tcnt, dpre);
+    LAUNCH_CHECK();
+    excl_scan(tcnt, (int64_t)nruns + 1, part, st);
+    excl_scan(dpre, (int64_t)nruns + 1, part, st);
+    uint32_t ntask = 0, ndense = 0;
+    HIP_TRY(hipMemcpyAsync(&ntask, tcnt + nruns, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&ndense, dpre + nruns, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    g.dtask = (uint4*)g.buf.get(S_DTASK, (size_t)std::max<uint32_t>(ntask, 1) * sizeof(uint4));
+    g.dplane = (uint32_t*)g.buf.get(S_DPLANE, (size_t)std::max<uint32_t>(ntask, 1) * 4);
+    k_run_tasks<<<grid1(std::max<int64_t>(nruns, 1)), kBlock, 0, st>>>(keys, g.rec_a, scr1, tcnt, nruns, dmin,
+                                                                       kDenseChunk, g.k.ntiles, g.k.amask,
+                                                                       g.dtask, g.dplane, g.counters);
+    LAUNCH_CHECK();
+    const int64_t ns = n - (int64_t)ndense;
     g.sidx = (uint32_t*)g.buf.get(S_SIDX, (size_t)std::max<int64_t>(ns, 1) * 4);
+    k_run_mark<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, scr0, scr1, dpre, g.counters, dmin, g.k.dflag, g.sidx);
+    LAUNCH_CHECK();
     g.sbstart = (uint32_t*)g.buf.get(S_SBSTART, nkeys * 4);
     g.sbend = (uint32_t*)g.buf.get(S_SBEND, nkeys * 4);
-    k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(g.sbstart, nkeys, 0u);
-    k_fill_u32<<<gridcap(nkeys), kBlock, 0, st>>>(g.sbend, nkeys, 0u);
+    k_sparse_bounds<<<grid1(nkeys), kBlock, 0, st>>>(g.bstart, g.bend, nkeys, n, scr0, dpre, g.counters,
+                                                     g.sbstart, g.sbend);
     LAUNCH_CHECK();
-    if (ns > 0) {
-      k_sparse_compact<<<grid1(n), kBlock, 0, st>>>(g.rec_a, keys, n, g.k.dflag, scr0, g.sidx, scr1);
-      LAUNCH_CHECK();
-      k_bounds<<<grid1(ns), kBlock, 0, st>>>(scr1, ns, g.sbstart, g.sbend);
-      LAUNCH_CHECK();
-    }
   }
   if (own) HIP_TRY(hipStreamSynchronize(st));
 }
