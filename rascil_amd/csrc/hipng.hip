@@ -496,25 +496,50 @@ __global__ void k_rekey(const double* __restrict__ uvw, const uint32_t* __restri
 constexpr int kRsIpt = 16;
 constexpr int kRsItems = kBlock * kRsIpt;
 
-__global__ void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                          uint32_t* __restrict__ hist, int64_t nblocks) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kRsItems;
-  for (int r = 0; r < kRsIpt; ++r) {
-    const int64_t i = base + r * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-}
-
 // orders the LDS accesses of one wave (no workgroup barrier)
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+
+// Per-block digit histogram of one pass: each wave counts its contiguous quarter of the
+// block's items 64 at a time; lanes with equal digits are found by ballots and their leader
+// adds the group size to the wave's own LDS histogram (no atomics).
+__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                    uint32_t* __restrict__ hist, int64_t nblocks) {
+  constexpr int NW = kBlock / 64;
+  constexpr int R = kRsItems / kBlock;
+  __shared__ uint32_t h[NW][256];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int w = 0; w < NW; ++w) h[w][threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t wbase = (int64_t)blockIdx.x * kRsItems + (int64_t)wv * (kRsItems / NW);
+  uint32_t key[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t i = wbase + r * 64 + lane;
+    key[r] = i < n ? keys[i] : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool ok = wbase + r * 64 + lane < n;
+    const uint32_t d = (key[r] >> shift) & 255u;
+    unsigned long long peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const unsigned long long m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    if (ok && lane == __ffsll((long long)peers) - 1) h[wv][d] += (uint32_t)__popcll(peers);
+    wave_sync();
+  }
+  __syncthreads();
+  uint32_t tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) tot += h[w][threadIdx.x];
+  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = tot;
+}
+
 
 // exclusive scan helpers (uint32), 4096 elements per block
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -763,10 +788,39 @@ __global__ void k_records(const double* __restrict__ uvw, const uint32_t* __rest
   if (lane == 32 && j < n) flipb[w0 + 1] = (uint32_t)(m >> 32);
 }
 
+// NB complex values of one item: 16-byte accesses when VEC (the caller checked alignment;
+// FP32 with an even batch, or FP64)
+template <typename R, int NB, bool VEC>
+__device__ __forceinline__ void load_vals(const cplx<R>* __restrict__ p, cplx<R> (&v)[NB]) {
+  if constexpr (VEC && std::is_same<R, float>::value && NB % 2 == 0) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int h = 0; h < NB / 2; ++h) {
+      const float4 t = q[h];
+      v[2 * h] = cplx<R>{t.x, t.y};
+      v[2 * h + 1] = cplx<R>{t.z, t.w};
+    }
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) v[b] = p[b];
+  }
+}
+template <typename R, int NB, bool VEC>
+__device__ __forceinline__ void store_vals(cplx<R>* __restrict__ p, const cplx<R> (&v)[NB]) {
+  if constexpr (VEC && std::is_same<R, float>::value && NB % 2 == 0) {
+    float4* q = reinterpret_cast<float4*>(p);
+#pragma unroll
+    for (int h = 0; h < NB / 2; ++h) q[h] = make_float4(v[2 * h].re, v[2 * h].im, v[2 * h + 1].re, v[2 * h + 1].im);
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) p[b] = v[b];
+  }
+}
+
 // invert prologue: rec_v[j] = conj?(wgt ms[perm[j]]) exp(-2 pi i |w| nshift f/c)
-template <typename R>
+template <typename R, int NB, bool VEC>
 __global__ void k_values(const uint32_t* __restrict__ perm, int64_t n, const cplx<R>* __restrict__ rec_p,
-                         const uint32_t* __restrict__ flipb, KP k, const cplx<R>* __restrict__ ms,
+                         const uint32_t* __restrict__ flipb, const cplx<R>* __restrict__ ms,
                          const R* __restrict__ wgt, cplx<R>* __restrict__ rec_v) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
@@ -774,19 +828,20 @@ __global__ void k_values(const uint32_t* __restrict__ perm, int64_t n, const cpl
   const bool flip = (flipb[j >> 5] >> (j & 31)) & 1u;
   const cplx<R> ph = rec_p[j];
   const R c = ph.re, s = -ph.im;
-  for (int b = 0; b < k.nb; ++b) {
-    cplx<R> v = ms[(int64_t)i * k.nb + b];
-    if (flip) v.im = -v.im;
+  cplx<R> v[NB], o[NB];
+  load_vals<R, NB, VEC>(ms + (int64_t)i * NB, v);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (flip) v[b].im = -v[b].im;
     if (wgt) {
-      const R g = wgt[(int64_t)i * k.nb + b];
-      v.re *= g;
-      v.im *= g;
+      const R g = wgt[(int64_t)i * NB + b];
+      v[b].re *= g;
+      v[b].im *= g;
     }
-    cplx<R> o;
-    o.re = v.re * c - v.im * s;
-    o.im = v.re * s + v.im * c;
-    rec_v[j * k.nb + b] = o;
+    o[b].re = v[b].re * c - v[b].im * s;
+    o[b].im = v[b].re * s + v[b].im * c;
   }
+  store_vals<R, NB, VEC>(rec_v + j * NB, o);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1975,10 +2030,10 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
   }
 }
 
-// predict epilogue: ms[i] = wgt * conj?(acc_sorted[j] exp(+2 pi i |w| nshift))
-template <typename R>
+// predict epilogue: ms[perm[j]] = wgt conj?(acc_sorted[j] exp(+2 pi i |w| nshift f/c))
+template <typename R, int NB, bool VEC>
 __global__ void k_unsort(const uint32_t* __restrict__ perm, int64_t n, const cplx<R>* __restrict__ rec_p,
-                         const uint32_t* __restrict__ flipb, KP k, const cplx<R>* __restrict__ accs,
+                         const uint32_t* __restrict__ flipb, const cplx<R>* __restrict__ accs,
                          const R* __restrict__ wgt, cplx<R>* __restrict__ ms) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
@@ -1986,19 +2041,20 @@ __global__ void k_unsort(const uint32_t* __restrict__ perm, int64_t n, const cpl
   const bool flip = (flipb[j >> 5] >> (j & 31)) & 1u;
   const cplx<R> ph = rec_p[j];
   const R c = ph.re, s = ph.im;
-  for (int b = 0; b < k.nb; ++b) {
-    const cplx<R> a = accs[j * k.nb + b];
-    cplx<R> o;
-    o.re = a.re * c - a.im * s;
-    o.im = a.re * s + a.im * c;
-    if (flip) o.im = -o.im;
+  cplx<R> a[NB], o[NB];
+  load_vals<R, NB, true>(accs + j * NB, a);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    o[b].re = a[b].re * c - a[b].im * s;
+    o[b].im = a[b].re * s + a[b].im * c;
+    if (flip) o[b].im = -o[b].im;
     if (wgt) {
-      const R g = wgt[(int64_t)i * k.nb + b];
-      o.re *= g;
-      o.im *= g;
+      const R g = wgt[(int64_t)i * NB + b];
+      o[b].re *= g;
+      o[b].im *= g;
     }
-    ms[(int64_t)i * k.nb + b] = o;
   }
+  store_vals<R, NB, VEC>(ms + (int64_t)i * NB, o);
 }
 
 // ---------------------------------------------------------------------------------
@@ -2795,6 +2851,36 @@ void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>
   LAUNCH_CHECK();
 }
 
+// 16-byte value accesses need the caller's arrays 16-byte aligned (FP32 with an even batch)
+template <typename R>
+bool vec_ok(const Geom& g, const void* p) {
+  return std::is_same<R, float>::value && g.nb % 2 == 0 && ((uintptr_t)p & 15u) == 0;
+}
+
+template <typename R>
+void launch_values(const Geom& g, const cplx<R>* ms, const R* wgt, cplx<R>* recv, hipStream_t st) {
+  const auto* rp = (const cplx<R>*)g.rec_p;
+  const bool vec = vec_ok<R>(g, ms);
+#define HIPNG_KV(BB)                                                                                    \
+  if (vec) k_values<R, BB, true><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, ms, wgt, recv); \
+  else k_values<R, BB, false><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, ms, wgt, recv)
+  if (g.nb == 1) { HIPNG_KV(1); } else if (g.nb == 2) { HIPNG_KV(2); } else if (g.nb == 3) { HIPNG_KV(3); } else { HIPNG_KV(4); }
+#undef HIPNG_KV
+  LAUNCH_CHECK();
+}
+
+template <typename R>
+void launch_unsort(const Geom& g, const cplx<R>* accs, const R* wgt, cplx<R>* ms, hipStream_t st) {
+  const auto* rp = (const cplx<R>*)g.rec_p;
+  const bool vec = vec_ok<R>(g, ms);
+#define HIPNG_KU(BB)                                                                                    \
+  if (vec) k_unsort<R, BB, true><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, accs, wgt, ms); \
+  else k_unsort<R, BB, false><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, accs, wgt, ms)
+  if (g.nb == 1) { HIPNG_KU(1); } else if (g.nb == 2) { HIPNG_KU(2); } else if (g.nb == 3) { HIPNG_KU(3); } else { HIPNG_KU(4); }
+#undef HIPNG_KU
+  LAUNCH_CHECK();
+}
+
 // R: kernel/grid real type; A: image accumulator type; F32C: coordinates/phase in FP32.
 // Kernel launches and rocFFT only: no allocation, no host synchronisation.
 template <typename R, typename A, bool F32C>
@@ -2812,8 +2898,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   auto* recv = (cplx<R>*)g.recv;
   {
     PROF("values");
-    k_values<R><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, (const cplx<R>*)g.rec_p, g.flipb, k,
-                                                (const cplx<R>*)ms, (const R*)wgt, recv);
+    launch_values<R>(g, (const cplx<R>*)ms, (const R*)wgt, recv, st);
     LAUNCH_CHECK();
   }
   auto* acc = (A*)g.accimg;
@@ -2979,8 +3064,7 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     }
   }
   PROF("unsort");
-  k_unsort<R><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, (const cplx<R>*)g.rec_p, g.flipb, k, accs,
-                                              (const R*)wgt, (cplx<R>*)ms);
+  launch_unsort<R>(g, accs, (const R*)wgt, (cplx<R>*)ms, st);
   LAUNCH_CHECK();
 }
 
