@@ -700,6 +700,22 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
     a[i] = v;
 }
 
+// clears the band rows [xlo, xhi) of the nplb planes except the cells of tiles with exactly
+// `solo` tasks (those are stored in full by the sub-tile gatherer)
+template <typename R>
+__global__ void k_zero_shared(KP k, int xlo, int xhi, int nplb, const uint32_t* __restrict__ task_off,
+                              const uint32_t* __restrict__ counters, uint32_t solo, cplx<R>* __restrict__ plane) {
+  const int64_t nb = (int64_t)(xhi - xlo) * k.gy;
+  const uint32_t total = counters[1];
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nb; e += (int64_t)gridDim.x * kBlock) {
+    const int x = xlo + (int)(e / k.gy), y = (int)(e - (e / k.gy) * k.gy);
+    const uint32_t tile = (uint32_t)(x / k.T) * (uint32_t)k.nty + (uint32_t)(y / k.T);
+    const uint32_t nt = (tile + 1 < k.ntiles ? task_off[tile + 1] : total) - task_off[tile];
+    if (nt == solo) continue;
+    for (int i = 0; i < nplb; ++i) plane[((int64_t)i * k.gx + x) * k.gy + y] = cplx<R>{(R)0, (R)0};
+  }
+}
+
 template <typename T>
 __global__ void k_zero(T* __restrict__ a, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
@@ -1367,11 +1383,13 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
         }
       }
       if (qn > 0) stage_gather(qn);
+      // a tile with a single task is owned by it: every cell is stored (zeros included; the
+      // plane is not cleared there beforehand); shared tiles add their non-zero cells
 #pragma unroll
       for (int q = 0; q < PC; ++q) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b < k.nb && (ar[q][b] != (R)0 || ai[q][b] != (R)0)) {
+          if (q < npl && b < k.nb && (solo || ar[q][b] != (R)0 || ai[q][b] != (R)0)) {
             cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + sx0 + cx) * k.gy + ty * T + sy0 + cy;
             if (solo) {
               *o = cplx<R>{ar[q][b], ai[q][b]};
@@ -2926,10 +2944,17 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       // the sub-tile gatherer walks the compacted table of records outside dense runs
       build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? P.tile / 8 : 1, wave ? gb0 : nullptr,
                   wave ? gb1 : nullptr);
-      // footprints stay inside the band, so only its rows need clearing
+      // footprints stay inside the band, so only its rows need clearing; with the sub-tile
+      // gatherer, tiles it owns alone are stored in full and are skipped here
       const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
-      for (int i = 0; i < npl * g.nb; ++i)
-        k_zero<cplx<R>><<<gridcap(nband), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nband);
+      if (wave) {
+        k_zero_shared<R><<<gridcap(nband), kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
+                                                           (uint32_t)(P.tile / 8), plane);
+        LAUNCH_CHECK();
+      } else {
+        for (int i = 0; i < npl * g.nb; ++i)
+          k_zero<cplx<R>><<<gridcap(nband), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nband);
+      }
     }
     {
       PROF("grid");
