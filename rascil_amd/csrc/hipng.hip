@@ -18,7 +18,6 @@
 //                  tasks; each 8 x 8 sub-tile is one cell per lane, all planes of the
 //                  pass and all polarisations in registers; candidates queued in LDS and
 //                  staged 64 at a time; one store (or atomic, for split tiles) per cell
-//   k_grid         workgroup-staged gather gridder (kept as an alternative, HIPNG_GRIDDER=block)
 //   rocFFT         pruned plane transform: batched 1-D c2c along gy over the band of grid
 //                  rows footprints can touch, k_tr_in transpose of the image's ny columns,
 //                  batched 1-D c2c along gx (forward: the same in reverse order)
@@ -909,13 +908,7 @@ __device__ __forceinline__ int task_tile(const uint32_t* __restrict__ task_off, 
 }
 
 // ---------------------------------------------------------------------------------
-// Gridder: persistent workgroups; a task grids one chunk of a tile's candidates into the
-// T x T tile of plane p. Per round, up to S candidates are staged in LDS by one thread each
-// (coalesced record loads, tile-overlap test, wave-ballot compaction, the 2W+1 kernel taps
-// evaluated once per visibility). Every thread then owns CPT = T*T/256 cells of the tile
-// and accumulates, in registers, the staged visibilities whose footprint covers them
-// (gather: no LDS atomics, deterministic order). The tile is added to the zero-filled
-// plane with one global atomic per non-zero cell (tiles can be split over tasks).
+// Gridding constants and the plane window of a pass
 // ---------------------------------------------------------------------------------
 constexpr int kMaxPC = 2;                    // w planes gridded per pass
 constexpr int kXcd = 8;                      // XCDs per device: work queues of the sub-tile gridder
@@ -937,208 +930,6 @@ __device__ __host__ __forceinline__ void chunk_range(int Ww, int nplanes, int p0
                                                      int* shi) {
   *slo = max(0, p0 - (Ww - 1));
   *shi = min(p0 + npl - 1, nplanes - Ww);
-}
-
-template <typename R, int T, int PC>
-__global__ __launch_bounds__(kBlock) void k_grid(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
-                                                 const uint32_t* __restrict__ bend,
-                                                 const uint32_t* __restrict__ rec_a,
-                                                 const rec3<R>* __restrict__ rec_f,
-                                                 const cplx<R>* __restrict__ rec_v,
-                                                 const uint32_t* __restrict__ task_off, uint32_t chunk,
-                                                 uint32_t* __restrict__ counters, int S,
-                                                 cplx<R>* __restrict__ plane) {
-  constexpr int CPT = T * T / kBlock;  // cells per thread
-  constexpr int RUN = T / CPT;         // threads per tile column
-  constexpr int NW = kBlock / 64;      // waves; wave w owns columns [w*CW, (w+1)*CW)
-  constexpr int CW = T / NW;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int W = k.W;
-  // staged data: taps x [S][W], taps y [S][W], w taps [S][PC], values [S][4], packed offsets [S],
-  // per-wave lists of staged entries [NW][S]
-  R* s_kx = reinterpret_cast<R*>(smem_raw);
-  R* s_ky = s_kx + S * W;
-  R* s_kw = s_ky + S * W;
-  cplx<R>* s_v = reinterpret_cast<cplx<R>*>(s_kw + S * PC);
-  uint32_t* s_dxy = reinterpret_cast<uint32_t*>(s_v + 4 * S);
-  uint16_t* s_list = reinterpret_cast<uint16_t*>(s_dxy + S);
-  __shared__ uint32_t s_task;
-  __shared__ uint32_t s_segv[kMaxSeg + 1];
-  __shared__ uint32_t s_segj[kMaxSeg];
-  __shared__ int s_segs[kMaxSeg];
-  __shared__ int s_cnt[NW][NW + 1];  // [source wave][target list, all]
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int cx = t / RUN, cy0 = (t - cx * RUN) * CPT;
-  const R scale = (R)2 / (R)W;
-  const R beta = (R)k.beta;
-  const uint32_t total = counters[1];
-  int slo, shi;
-  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
-  const int nseg = (shi - slo + 1) * 4;
-  for (;;) {
-    if (t == 0) s_task = atomicAdd(&counters[0], 1u);
-    __syncthreads();
-    const uint32_t task = s_task;
-    if (task >= total) break;
-    const int tile = task_tile(task_off, (int)k.ntiles, task);
-    const uint32_t vlo = (task - task_off[tile]) * chunk;
-    const int tx = tile / k.nty, ty = tile - tx * k.nty;
-    if (t < nseg) {
-      const int s = slo + t / 4;
-      const uint32_t key = seg_key(k, s, tx, ty, t & 3);
-      s_segj[t] = bstart[key];
-      s_segv[t + 1] = bend[key] - bstart[key];
-      s_segs[t] = s;
-    }
-    __syncthreads();
-    if (t == 0) {
-      s_segv[0] = 0;
-      for (int q = 1; q <= nseg; ++q) s_segv[q] += s_segv[q - 1];
-    }
-    __syncthreads();
-    const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
-    // a tile owned by a single task is stored; tiles split over tasks are added atomically
-    const uint32_t tnext = tile + 1 < (int)k.ntiles ? task_off[tile + 1] : total;
-    const bool solo = tnext - task_off[tile] == 1;
-    R ar[CPT][PC][kMaxBatch], ai[CPT][PC][kMaxBatch];
-#pragma unroll
-    for (int c = 0; c < CPT; ++c)
-#pragma unroll
-      for (int q = 0; q < PC; ++q)
-#pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) ar[c][q][b] = ai[c][q][b] = 0;
-    for (uint32_t vb = vlo; vb < vhi; vb += (uint32_t)S) {
-      // ---- stage ----
-      const uint32_t v = vb + (uint32_t)t;
-      bool ok = false;
-      int dx = 0, dy = 0, sp = 0;
-      uint32_t j = 0;
-      if (t < S && v < vhi) {
-        int lo = 0, hi = nseg;  // last segment with start <= v
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (s_segv[mid] <= v) lo = mid;
-          else hi = mid;
-        }
-        j = s_segj[lo] + (v - s_segv[lo]);
-        sp = s_segs[lo];
-        const uint32_t Ar = rec_a[j], A = Ar & k.amask;
-        dx = (int)(A & 0xffffu) - tx * T;
-        dy = (int)(A >> 16) - ty * T;
-        if (dx >= k.hgx) dx -= k.gx;
-        if (dx < -k.hgx) dx += k.gx;
-        if (dy >= k.hgy) dy -= k.gy;
-        if (dy < -k.hgy) dy += k.gy;
-        ok = (Ar & k.dflag) == 0u && (dx + W > 0) && (dx < T) && (dy + W > 0) && (dy < T);
-      }
-      // staged position (all lists) and per-wave list positions, deterministic (ballots)
-      const unsigned long long m = __ballot(ok);
-      unsigned long long ml[NW];
-#pragma unroll
-      for (int w = 0; w < NW; ++w) ml[w] = __ballot(ok && dx < (w + 1) * CW && dx + W > w * CW);
-      if (lane == 0) {
-        s_cnt[wv][NW] = (int)__popcll(m);
-#pragma unroll
-        for (int w = 0; w < NW; ++w) s_cnt[wv][w] = (int)__popcll(ml[w]);
-      }
-      __syncthreads();
-      int base = 0;
-      for (int w = 0; w < wv; ++w) base += s_cnt[w][NW];
-      int Mw = 0;  // length of this wave's list
-      for (int w = 0; w < NW; ++w) Mw += s_cnt[w][wv];
-      if (ok) {
-        const int pos = base + (int)__popcll(m & lt);
-        const rec3<R> f = rec_f[j];
-#pragma unroll
-        for (int q = 0; q < PC; ++q) {
-          const int iw = p0 + q - sp;  // plane offset inside this visibility's window
-          s_kw[pos * PC + q] = (q < npl && iw >= 0 && iw < k.Ww)
-                                   ? (k.wstack ? esk((f.w + (R)iw) * scale, beta) : (R)1)
-                                   : (R)0;
-        }
-        for (int i = 0; i < W; ++i) {
-          s_kx[pos * W + i] = esk((f.x + (R)i) * scale, beta);
-          s_ky[pos * W + i] = esk((f.y + (R)i) * scale, beta);
-        }
-        for (int b = 0; b < k.nb; ++b) s_v[pos * 4 + b] = rec_v[(int64_t)j * k.nb + b];
-        s_dxy[pos] = (uint32_t)(dx + 32768) | ((uint32_t)(dy + 32768) << 16);
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-          if ((ml[w] >> lane) & 1ull) {
-            int lb = 0;
-            for (int src = 0; src < wv; ++src) lb += s_cnt[src][w];
-            s_list[w * S + lb + (int)__popcll(ml[w] & lt)] = (uint16_t)pos;
-          }
-        }
-      }
-      __syncthreads();
-      // ---- gather: this wave's list only, two entries per iteration ----
-      const uint16_t* mylist = s_list + wv * S;
-      auto gather_one = [&](int e, uint32_t d) {
-        const int rx = cx - ((int)(d & 0xffffu) - 32768);
-        if ((unsigned)rx < (unsigned)W) {
-          const R kx = s_kx[e * W + rx];
-          const int dyi = (int)(d >> 16) - 32768;
-          R kw[PC];
-#pragma unroll
-          for (int q = 0; q < PC; ++q) kw[q] = kx * s_kw[e * PC + q];
-          cplx<R> vv[kMaxBatch];
-#pragma unroll
-          for (int b = 0; b < kMaxBatch; ++b)
-            if (b < k.nb) vv[b] = s_v[e * 4 + b];
-#pragma unroll
-          for (int c = 0; c < CPT; ++c) {
-            const int ry = cy0 + c - dyi;
-            if ((unsigned)ry < (unsigned)W) {
-              const R ky = s_ky[e * W + ry];
-#pragma unroll
-              for (int q = 0; q < PC; ++q) {
-                const R wgt = kw[q] * ky;
-#pragma unroll
-                for (int b = 0; b < kMaxBatch; ++b) {
-                  if (b < k.nb) {
-                    ar[c][q][b] += wgt * vv[b].re;
-                    ai[c][q][b] += wgt * vv[b].im;
-                  }
-                }
-              }
-            }
-          }
-        }
-      };
-      int i = 0;
-      for (; i + 1 < Mw; i += 2) {
-        const int e0 = mylist[i], e1 = mylist[i + 1];
-        const uint32_t d0 = s_dxy[e0], d1 = s_dxy[e1];
-        gather_one(e0, d0);
-        gather_one(e1, d1);
-      }
-      if (i < Mw) {
-        const int e0 = mylist[i];
-        gather_one(e0, s_dxy[e0]);
-      }
-      __syncthreads();
-    }
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) {
-#pragma unroll
-      for (int q = 0; q < PC; ++q) {
-#pragma unroll
-        for (int b = 0; b < kMaxBatch; ++b) {
-          if (b < k.nb && (ar[c][q][b] != (R)0 || ai[c][q][b] != (R)0)) {
-            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + cx) * k.gy + ty * T + cy0 + c;
-            if (solo) {
-              *o = cplx<R>{ar[c][q][b], ai[c][q][b]};
-            } else {
-              atomicAdd(&o->re, ar[c][q][b]);
-              atomicAdd(&o->im, ai[c][q][b]);
-            }
-          }
-        }
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -2524,11 +2315,6 @@ void set_lds_limits() {
   if (done) return;
   done = true;
   // dynamic LDS up to 64 KiB for the tile kernels (the plan keeps tiles within it)
-#define HIPNG_LDS(RR, TT, QQ) \
-  (void)hipFuncSetAttribute((const void*)k_grid<RR, TT, QQ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget)
-  HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
-  HIPNG_LDS(double, 32, 1); HIPNG_LDS(double, 32, 2); HIPNG_LDS(double, 16, 1); HIPNG_LDS(double, 16, 2);
-#undef HIPNG_LDS
 #define HIPNG_LDS1(RR, TT, QQ, BB)                                                                  \
   (void)hipFuncSetAttribute((const void*)k_grid8<RR, TT, QQ, BB>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                             (int)kLdsBudget)
@@ -2582,15 +2368,6 @@ bool dense_enabled() {
   return v == 1;
 }
 
-// HIPNG_GRIDDER=block selects the workgroup-staged gridder instead of the sub-tile one
-bool grid_block_variant() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = std::getenv("HIPNG_GRIDDER");
-    v = (e && std::strcmp(e, "block") == 0) ? 1 : 0;
-  }
-  return v == 1;
-}
 
 // shortest run the dense-run gridder takes (HIPNG_DENSE_MIN overrides, for calibration)
 uint32_t dense_min_run() {
@@ -2712,14 +2489,12 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     LAUNCH_CHECK();
   }
   g.recv = g.buf.get(S_RECV, (size_t)n * nb * csz);
-  // planes per pass: up to kMaxPC while the plane batch stays within 40 GB and the
-  // gridder's per-thread accumulators (cells x planes x batch x complex) within 128 VGPRs
+  // planes per pass: up to kMaxPC while the plane batch stays within 40 GB (the gridders'
+  // per-lane accumulators, one cell x planes x batch x complex, stay small)
   const size_t plane_bytes = (size_t)nb * P.gx * P.gy * csz;
-  const int cpt = P.tile * P.tile / kBlock;
   g.pc = 1;
   for (int q = kMaxPC; q > 1; q /= 2) {
-    const int regs = cpt * q * nb * 2 * (prec == HIPNG_FP64 ? 2 : 1);
-    if ((double)q * plane_bytes <= 40e9 && regs <= 128 && q <= P.nplanes) {
+    if ((double)q * plane_bytes <= 40e9 && q <= P.nplanes) {
       g.pc = q;
       break;
     }
@@ -2925,40 +2700,30 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   auto* plane = (cplx<R>*)g.plane;
   auto* tbuf = (cplx<R>*)g.tbuf;
   const int PCn = g.pc;
-  // staged candidates per round: 256 unless the staging arrays would pass 48 KiB of LDS
-  const size_t per = 2 * (size_t)P.W * sizeof(R) + PCn * sizeof(R) + 4 * sizeof(cplx<R>) +
-                     sizeof(uint32_t) + (kBlock / 64) * sizeof(uint16_t);
-  const int S = (int)std::min<size_t>(kBlock, (48 * 1024 / per) / 64 * 64);
-  const size_t sm = per * S;
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   const uint32_t* gb0 = g.sidx ? g.sbstart : g.bstart;
   const uint32_t* gb1 = g.sidx ? g.sbend : g.bend;
+  // per-wave LDS of the sub-tile gatherer (at most 62.6 KiB: FP64, 4 polarisations, 2 planes)
+  const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn, g.nb);
+  if (smw > kLdsBudget) HIPNG_THROW(HIPNG_ERUNTIME, "gridder LDS %zu exceeds %zu", smw, kLdsBudget);
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
-    const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn, g.nb);
-    const bool wave = !grid_block_variant() && smw <= kLdsBudget;
     {
       PROF("grid_setup");
       int slo, shi;
       chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
       // the sub-tile gatherer walks the compacted table of records outside dense runs
-      build_tasks(g, slo, shi, 4, g.chunk_grid, st, wave ? P.tile / 8 : 1, wave ? gb0 : nullptr,
-                  wave ? gb1 : nullptr);
-      // footprints stay inside the band, so only its rows need clearing; with the sub-tile
-      // gatherer, tiles it owns alone are stored in full and are skipped here
+      build_tasks(g, slo, shi, 4, g.chunk_grid, st, P.tile / 8, gb0, gb1);
+      // footprints stay inside the band, so only its rows need clearing; tiles the gatherer
+      // owns alone are stored in full by it and are skipped here
       const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
-      if (wave) {
-        k_zero_shared<R><<<gridcap(nband), kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
-                                                           (uint32_t)(P.tile / 8), plane);
-        LAUNCH_CHECK();
-      } else {
-        for (int i = 0; i < npl * g.nb; ++i)
-          k_zero<cplx<R>><<<gridcap(nband), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nband);
-      }
+      k_zero_shared<R><<<gridcap(nband), kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
+                                                         (uint32_t)(P.tile / 8), plane);
+      LAUNCH_CHECK();
     }
     {
       PROF("grid");
-      if (wave) {
+      {
 #define HIPNG_GW(TT, QQ, BB)                                                                      \
   k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.sidx, g.rec_a,   \
                                                           (const rec3<R>*)g.rec_f, recv, g.task,  \
@@ -2976,18 +2741,6 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
 #undef HIPNG_GWB
 #undef HIPNG_GW
         LAUNCH_CHECK();
-      } else {
-#define HIPNG_GR(TT, QQ)                                                                          \
-  k_grid<R, TT, QQ><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a,         \
-                                                    (const rec3<R>*)g.rec_f, recv, g.task,        \
-                                                    g.chunk_grid, g.counters, S, plane)
-      if (P.tile == 32) {
-        if (PCn == 2) HIPNG_GR(32, 2); else HIPNG_GR(32, 1);
-      } else {
-        if (PCn == 2) HIPNG_GR(16, 2); else HIPNG_GR(16, 1);
-      }
-#undef HIPNG_GR
-      LAUNCH_CHECK();
       }
     }
     if (g.dense) {
