@@ -1464,6 +1464,141 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense(KP k, int p0, int npl, co
   }
 }
 
+// Dense-run gridder on the matrix cores (FP32 kernel modes, W <= 8, 2 planes per pass).
+// For polarisation b the wave accumulates the 16 x 16 block
+//   G_b[(a, q)][(c, h)] = sum_v kx_v[a] kw_v[q] * ky_v[c] (h ? Im : Re)(val_v[b])
+// over the task's visibilities v, for footprint cells (ax + a, ay + c), a, c < 8, and the
+// pass's planes q < 2: a sum of rank-1 terms that v_mfma_f32_16x16x4_f32 takes four
+// visibilities at a time. Lane l supplies row l % 16 of A and column l % 16 of B for
+// visibility l / 16 of the group; register r of lane l receives row 4 (l / 16) + r, column
+// l % 16 of the block. Tasks, staging and the atomic flush follow k_grid_dense; staging
+// lanes past the end of a batch write zeros, so a partial group of four adds nothing.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl, const uint4* __restrict__ dtask,
+                                                          const uint32_t* __restrict__ dplane,
+                                                          const rec3<float>* __restrict__ rec_f,
+                                                          const cplx<float>* __restrict__ rec_v,
+                                                          uint32_t* __restrict__ counters,
+                                                          cplx<float>* __restrict__ plane) {
+  static_assert(kMaxPC == 2, "the 16 rows of a block are 8 cells x 2 planes");
+  constexpr int PC = kMaxPC;
+  constexpr int NW = kBlock / 64;
+  __shared__ float s_tx[NW][64][9];
+  __shared__ float s_ty[NW][64][9];
+  __shared__ float s_tw[NW][64][PC];
+  __shared__ float s_v[NW][NB][64][2];
+  const int W = k.W;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gk = lane >> 4, gr = lane & 15;  // visibility of the group, row / column of the block
+  const int c8 = gr & 7, h2 = gr >> 3;       // cell, and plane (A side) or Re / Im (B side)
+  const float scale = 2.0f / (float)W;
+  const float beta = (float)(k.beta * 1.4426950408889634);
+  int slo, shi;
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
+  const uint32_t total = counters[kDenseTotal];
+  const uint32_t lo = lower_bound_u32(dplane, total, (uint32_t)slo);
+  const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
+  auto pop = [&]() {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
+    return lo + __shfl(t, 0, 64);
+  };
+  uint32_t t = pop();
+  if (t >= hi) return;
+  uint4 cur = dtask[t];  // first record, count, packed first cell, start plane
+  uint32_t tn = pop();
+  rec3<float> f{};
+  cplx<float> v[NB];
+  auto load = [&](const uint4& tk, uint32_t b0) {
+    if (b0 + (uint32_t)lane < tk.y) {
+      const uint32_t j = tk.x + b0 + (uint32_t)lane;
+      f = rec_f[j];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) v[b] = rec_v[(int64_t)j * NB + b];
+    }
+  };
+  load(cur, 0);
+  f32x4 acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint32_t b0 = 0;
+  for (;;) {
+    const int cnt = (int)min(64u, cur.y - b0);
+    const int sp = (int)cur.w;
+    {
+      const bool live = lane < cnt;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        s_tx[wv][lane][i] = live && i < W ? esk_fast((f.x + (float)i) * scale, beta) : 0.f;
+        s_ty[wv][lane][i] = live && i < W ? esk_fast((f.y + (float)i) * scale, beta) : 0.f;
+      }
+#pragma unroll
+      for (int q = 0; q < PC; ++q) {
+        const int iw = p0 + q - sp;
+        s_tw[wv][lane][q] = (live && q < npl && iw >= 0 && iw < k.Ww)
+                                ? (k.wstack ? esk_fast((f.w + (float)iw) * scale, beta) : 1.f)
+                                : 0.f;
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        s_v[wv][b][lane][0] = live ? v[b].re : 0.f;
+        s_v[wv][b][lane][1] = live ? v[b].im : 0.f;
+      }
+    }
+    wave_sync();
+    const bool last = b0 + 64u >= cur.y;
+    uint4 nxt = cur;
+    uint32_t nb0 = b0 + 64u;
+    if (last) {
+      nb0 = 0;
+      nxt = tn < hi ? dtask[tn] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    load(nxt, nb0);
+    for (int e0 = 0; e0 < cnt; e0 += 4) {
+      const int e = e0 + gk;
+      const float av = s_tx[wv][e][c8] * s_tw[wv][e][h2];
+      const float ty = s_ty[wv][e][c8];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * s_v[wv][b][e][h2], acc[b], 0, 0, 0);
+    }
+    wave_sync();
+    if (!last) {
+      b0 = nb0;
+      continue;
+    }
+    {
+      const uint32_t A = cur.z;
+      int gyi = (int)(A >> 16) + c8;
+      if (gyi >= k.gy) gyi -= k.gy;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * gk + r;
+        const int ca = row & 7, q = row >> 3;
+        if (q < npl && ca < W && c8 < W) {
+          int gxi = (int)(A & 0xffffu) + ca;
+          if (gxi >= k.gx) gxi -= k.gx;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const float val = acc[b][r];
+            if (val != 0.f)
+              atomicAdd(reinterpret_cast<float*>(plane + ((int64_t)(q * NB + b) * k.gx + gxi) * k.gy + gyi) + h2,
+                        val);
+          }
+        }
+      }
+    }
+    if (tn >= hi) break;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    cur = nxt;
+    b0 = 0;
+    tn = pop();
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // Image-domain kernels
 // ---------------------------------------------------------------------------------
@@ -2674,6 +2809,33 @@ void launch_unsort(const Geom& g, const cplx<R>* accs, const R* wgt, cplx<R>* ms
   LAUNCH_CHECK();
 }
 
+// dense-run gridder launch: the matrix-core kernel for FP32 records, the vector kernel for FP64
+constexpr int kDenseBlocksPerCu = 5;  // 5 workgroups per CU fit its LDS (29 KiB each, FP32)
+void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<float>* rf,
+                       const cplx<float>* recv, cplx<float>* plane, hipStream_t st) {
+  const int nblk = device_cus() * kDenseBlocksPerCu;
+  if (g.nb == 1)
+    k_grid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+  else if (g.nb == 2)
+    k_grid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+  else if (g.nb == 3)
+    k_grid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+  else
+    k_grid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+}
+void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<double>* rf,
+                       const cplx<double>* recv, cplx<double>* plane, hipStream_t st) {
+  const int nblk = device_cus() * kDenseBlocksPerCu;
+  if (g.nb == 1)
+    k_grid_dense<double, 1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+  else if (g.nb == 2)
+    k_grid_dense<double, 2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+  else if (g.nb == 3)
+    k_grid_dense<double, 3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+  else
+    k_grid_dense<double, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+}
+
 // R: kernel/grid real type; A: image accumulator type; F32C: coordinates/phase in FP32.
 // Kernel launches and rocFFT only: no allocation, no host synchronisation.
 template <typename R, typename A, bool F32C>
@@ -2745,20 +2907,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     }
     if (g.dense) {
       PROF("grid_dense");
-      const auto* rf = (const rec3<R>*)g.rec_f;
-      const int nblk = device_cus() * 5;  // 5 workgroups per CU fit its LDS (29 KiB each, FP32)
-      if (g.nb == 1)
-        k_grid_dense<R, 1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
-                                                          plane);
-      else if (g.nb == 2)
-        k_grid_dense<R, 2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
-                                                          plane);
-      else if (g.nb == 3)
-        k_grid_dense<R, 3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
-                                                          plane);
-      else
-        k_grid_dense<R, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters,
-                                                          plane);
+      launch_grid_dense(g, k, p0, npl, (const rec3<R>*)g.rec_f, recv, plane, st);
       LAUNCH_CHECK();
     }
     {
