@@ -961,8 +961,139 @@ __device__ __forceinline__ float esk_fast(float z, float beta2) {
 }
 __device__ __forceinline__ double esk_fast(double z, double beta) { return esk(z, beta); }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Per-wave accumulator of one 8 x 8 sub-tile: PC planes x NB polarisations of every cell.
+// Staged entries: taps s_t[e * NT + i] (i < 8: x taps of the sub-tile's columns, 8 + i: y
+// taps of its rows, 16 + q: w taps of the pass's planes), values s_v[b * 64 + e].
+// Vector form: lane (cx, cy) = cell (sx0 + cx, sy0 + cy), accumulated one entry at a time.
+template <typename R, int PC, int NB, int NT>
+struct Grid8Vec {
+  static constexpr bool kZeroPad = false;  // entries past the count are never read
+  R ar[PC][NB], ai[PC][NB];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int q = 0; q < PC; ++q)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
+  }
+  // every operand of an entry is read before any is used, two entries per round, no
+  // branches, so the LDS latency of one entry overlaps the arithmetic of the other
+  __device__ __forceinline__ void accumulate(const R* s_t, const cplx<R>* s_v, int cnt, int lane) {
+    const int cx = lane >> 3, cy = lane & 7;
+    auto load = [&](int e, R (&wq)[PC], cplx<R> (&vv)[NB]) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) vv[b] = s_v[b * 64 + e];
+      const R* t = s_t + e * NT;
+      const R kxy = t[cx] * t[8 + cy];
+#pragma unroll
+      for (int q = 0; q < PC; ++q) wq[q] = kxy * t[16 + q];
+    };
+    auto accum = [&](const R (&wq)[PC], const cplx<R> (&vv)[NB]) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int q = 0; q < PC; ++q) {
+          ar[q][b] += wq[q] * vv[b].re;
+          ai[q][b] += wq[q] * vv[b].im;
+        }
+    };
+    int e = 0;
+    for (; e + 1 < cnt; e += 2) {
+      R w0[PC], w1[PC];
+      cplx<R> v0[NB], v1[NB];
+      load(e, w0, v0);
+      load(e + 1, w1, v1);
+      accum(w0, v0);
+      accum(w1, v1);
+    }
+    if (e < cnt) {
+      R w0[PC];
+      cplx<R> v0[NB];
+      load(e, w0, v0);
+      accum(w0, v0);
+    }
+  }
+  // solo: the wave owns the tile, every cell is stored; otherwise non-zero cells are added
+  __device__ __forceinline__ void store(cplx<R>* plane, const KP& k, int npl, int x0, int y0, bool solo,
+                                        int lane) const {
+    const int cx = lane >> 3, cy = lane & 7;
+#pragma unroll
+    for (int q = 0; q < PC; ++q) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (q < npl && b < k.nb && (solo || ar[q][b] != (R)0 || ai[q][b] != (R)0)) {
+          cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + cx) * k.gy + y0 + cy;
+          if (solo) {
+            *o = cplx<R>{ar[q][b], ai[q][b]};
+          } else {
+            atomicAdd(&o->re, ar[q][b]);
+            atomicAdd(&o->im, ai[q][b]);
+          }
+        }
+      }
+    }
+  }
+};
+
+// Matrix-core form (FP32, PC <= 2): per polarisation b the 16 x 16 block
+//   G_b[(a, q)][(c, h)] = sum_e kx_e[a] kw_e[q] * ky_e[c] (h ? Im : Re)(val_e[b])
+// for cells (sx0 + a, sy0 + c), taken four entries at a time by v_mfma_f32_16x16x4_f32:
+// lane l supplies row l % 16 of A and column l % 16 of B for entry l / 16 of the group, and
+// register r of lane l receives row 4 (l / 16) + r, column l % 16. Entries past the count up
+// to the next multiple of four must be staged as zeros.
+template <int PC, int NB, int NT>
+struct Grid8Mx {
+  static_assert(PC <= 2, "the 16 rows of a block are 8 cells x 2 planes");
+  static constexpr bool kZeroPad = true;
+  f32x4 acc[NB];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  __device__ __forceinline__ void accumulate(const float* s_t, const cplx<float>* s_v, int cnt, int lane) {
+    const int gk = lane >> 4, c8 = lane & 7, h2 = (lane >> 3) & 1;
+    for (int e0 = 0; e0 < cnt; e0 += 4) {
+      const int e = e0 + gk;
+      const float* t = s_t + e * NT;
+      const float av = t[c8] * (h2 < PC ? t[16 + h2] : 0.f);
+      const float ty = t[8 + c8];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const float vb = reinterpret_cast<const float*>(s_v + b * 64 + e)[h2];
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * vb, acc[b], 0, 0, 0);
+      }
+    }
+  }
+  __device__ __forceinline__ void store(cplx<float>* plane, const KP& k, int npl, int x0, int y0, bool solo,
+                                        int lane) const {
+    const int gk = lane >> 4, c8 = lane & 7, h2 = (lane >> 3) & 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = 4 * gk + r;
+      const int a = row & 7, q = row >> 3;
+      if (q < PC && q < npl) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const float val = acc[b][r];
+          if (b < k.nb && (solo || val != 0.f)) {
+            float* o = reinterpret_cast<float*>(plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + a) * k.gy + y0 + c8) + h2;
+            if (solo) *o = val;
+            else atomicAdd(o, val);
+          }
+        }
+      }
+    }
+  }
+};
+
+template <typename R, int PC, int NB, int NT>
+using Grid8Acc = typename std::conditional<std::is_same<R, float>::value, Grid8Mx<PC, NB, NT>,
+                                           Grid8Vec<R, PC, NB, NT>>::type;
+
+// FP32: at most 128 VGPRs, so that 4 waves per SIMD stay resident (as many as the LDS allows)
 template <typename R, int T, int PC, int NB>
-__global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
+__global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                   const uint32_t* __restrict__ bend,
                                                   const uint32_t* __restrict__ sidx,
                                                   const uint32_t* __restrict__ rec_a,
@@ -973,6 +1104,7 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
                                                   cplx<R>* __restrict__ plane) {
   constexpr int NS = T / 8;               // strips per tile = sub-tiles per strip
   constexpr int NT = grid8_taps(PC);      // per staged visibility: 8 x taps, 8 y taps, PC w taps
+  using Acc = Grid8Acc<R, PC, NB, NT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int W = k.W;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -989,7 +1121,6 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
   // the FP32 tap evaluation works in base 2
   const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
   const uint32_t total = counters[1];
-  const int cx = lane >> 3, cy = lane & 7;
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const int nseg = (shi - slo + 1) * 4;
@@ -1045,15 +1176,19 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
     const uint32_t vlo = ci * chunk;
     const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
     for (int sy0 = 0; sy0 < T; sy0 += 8) {
-      R ar[PC][NB], ai[PC][NB];
-#pragma unroll
-      for (int q = 0; q < PC; ++q)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
+      Acc acc;
+      acc.zero();
       // stage queue entries [0, cnt) into slots [0, cnt) and accumulate them
       auto stage_gather = [&](int cnt) {
         // staging: one queued visibility per lane; its values, the taps of the sub-tile's 8
         // columns and 8 rows (zero outside the support) and its w taps for the pass's planes
+        if (Acc::kZeroPad && lane >= cnt) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) s_v[b * 64 + lane] = cplx<R>{0, 0};
+          R* t = s_t + lane * NT;
+#pragma unroll
+          for (int i = 0; i < NT; ++i) t[i] = (R)0;
+        }
         if (lane < cnt) {
           const uint32_t j = q_j[lane], d = q_d[lane];
           const int sp = (int)q_s[lane];
@@ -1077,40 +1212,7 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
           }
         }
         wave_sync();
-        // gather: every operand of an entry is read before any is used, two entries per round,
-        // no branches, so the LDS latency of one entry overlaps the arithmetic of the other
-        auto load = [&](int e, R (&wq)[PC], cplx<R> (&vv)[NB]) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) vv[b] = s_v[b * 64 + e];
-          const R* t = s_t + e * NT;
-          const R kxy = t[cx] * t[8 + cy];
-#pragma unroll
-          for (int q = 0; q < PC; ++q) wq[q] = kxy * t[16 + q];
-        };
-        auto accum = [&](const R (&wq)[PC], const cplx<R> (&vv)[NB]) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int q = 0; q < PC; ++q) {
-              ar[q][b] += wq[q] * vv[b].re;
-              ai[q][b] += wq[q] * vv[b].im;
-            }
-        };
-        int e = 0;
-        for (; e + 1 < cnt; e += 2) {
-          R w0[PC], w1[PC];
-          cplx<R> v0[NB], v1[NB];
-          load(e, w0, v0);
-          load(e + 1, w1, v1);
-          accum(w0, v0);
-          accum(w1, v1);
-        }
-        if (e < cnt) {
-          R w0[PC];
-          cplx<R> v0[NB];
-          load(e, w0, v0);
-          accum(w0, v0);
-        }
+        acc.accumulate(s_t, s_v, cnt, lane);
         wave_sync();
       };
       int qn = 0;
@@ -1176,21 +1278,7 @@ __global__ __launch_bounds__(kBlock) void k_grid8(KP k, int p0, int npl, const u
       if (qn > 0) stage_gather(qn);
       // a tile with a single task is owned by it: every cell is stored (zeros included; the
       // plane is not cleared there beforehand); shared tiles add their non-zero cells
-#pragma unroll
-      for (int q = 0; q < PC; ++q) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          if (q < npl && b < k.nb && (solo || ar[q][b] != (R)0 || ai[q][b] != (R)0)) {
-            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + tx * T + sx0 + cx) * k.gy + ty * T + sy0 + cy;
-            if (solo) {
-              *o = cplx<R>{ar[q][b], ai[q][b]};
-            } else {
-              atomicAdd(&o->re, ar[q][b]);
-              atomicAdd(&o->im, ai[q][b]);
-            }
-          }
-        }
-      }
+      acc.store(plane, k, npl, tx * T + sx0, ty * T + sy0, solo, lane);
     }
   }
 }
@@ -1473,7 +1561,6 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense(KP k, int p0, int npl, co
 // visibility l / 16 of the group; register r of lane l receives row 4 (l / 16) + r, column
 // l % 16 of the block. Tasks, staging and the atomic flush follow k_grid_dense; staging
 // lanes past the end of a batch write zeros, so a partial group of four adds nothing.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 template <int NB>
 __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl, const uint4* __restrict__ dtask,
