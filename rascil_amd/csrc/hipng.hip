@@ -1095,7 +1095,7 @@ using Grid8Acc = typename std::conditional<std::is_same<R, float>::value, Grid8M
 template <typename R, int T, int PC, int NB>
 __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                   const uint32_t* __restrict__ bend,
-                                                  const uint32_t* __restrict__ sidx,
+                                                  const uint2* __restrict__ srec,
                                                   const uint32_t* __restrict__ rec_a,
                                                   const rec3<R>* __restrict__ rec_f,
                                                   const cplx<R>* __restrict__ rec_v,
@@ -1227,18 +1227,38 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
           else hi = mid;
         }
       }
-      for (uint32_t vb = vlo; vb < vhi; vb += 64) {
-        const uint32_t v = vb + (uint32_t)lane;
-        bool ok = false;
-        int dx = 0, dy = 0, sp = 0;
-        uint32_t j = 0;
-        int lo = cur;  // last segment starting at or before v
+      // candidate v: its segment (walked forward from `from`), record index and origin word;
+      // the next block's are loaded while the current block is filtered
+      auto fetch = [&](uint32_t v, int from, uint32_t& j, uint32_t& Ar, int& lo) {
+        lo = from;
+        j = 0;
+        Ar = 0;
         if (v < vhi) {
           while (lo + 1 < nseg && s_segv[lo + 1] <= v) ++lo;
-          j = s_segj[lo] + (v - s_segv[lo]);
-          if (sidx) j = sidx[j];
+          const uint32_t sj = s_segj[lo] + (v - s_segv[lo]);
+          if (srec) {
+            const uint2 e = srec[sj];
+            j = e.x;
+            Ar = e.y;
+          } else {
+            j = sj;
+            Ar = rec_a[sj];
+          }
+        }
+      };
+      uint32_t j_n, A_n;
+      int lo_n;
+      fetch(vlo + (uint32_t)lane, cur, j_n, A_n, lo_n);
+      for (uint32_t vb = vlo; vb < vhi; vb += 64) {
+        const uint32_t v = vb + (uint32_t)lane;
+        const uint32_t j = j_n, Ar = A_n;
+        const int lo = lo_n;  // last segment starting at or before v
+        fetch(v + 64u, __shfl(lo, 63, 64), j_n, A_n, lo_n);
+        bool ok = false;
+        int dx = 0, dy = 0, sp = 0;
+        if (v < vhi) {
           sp = slo + lo / 4;
-          const uint32_t Ar = rec_a[j], A = Ar & k.amask;
+          const uint32_t A = Ar & k.amask;
           dx = (int)(A & 0xffffu) - tx * T;
           dy = (int)(A >> 16) - ty * T;
           if (dx >= k.hgx) dx -= k.gx;
@@ -1247,7 +1267,6 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
           if (dy < -k.hgy) dy += k.gy;
           ok = (Ar & k.dflag) == 0u && (dx + W > sx0) && (dx < sx0 + 8) && (dy + W > sy0) && (dy < sy0 + 8);
         }
-        cur = __shfl(lo, 63, 64);
         const unsigned long long m = __ballot(ok);
         if (ok) {
           const int pos = qn + (int)__popcll(m & lt);
@@ -1353,19 +1372,20 @@ __global__ void k_run_tasks(const uint32_t* __restrict__ keys, const uint32_t* _
 }
 
 // per record: its run (A = exclusive scan of the run heads), dense runs flagged in rec_a,
-// the other records listed in sidx at their rank among non-dense records (dpre = exclusive
+// the other records listed in srec (record index, origin word) at their rank among non-dense
+// records (dpre = exclusive
 // scan of the dense run lengths)
 __global__ void k_run_mark(uint32_t* __restrict__ rec_a, int64_t n, const uint32_t* __restrict__ A,
                            const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ dpre,
                            const uint32_t* __restrict__ counters, uint32_t minlen, uint32_t flag,
-                           uint32_t* __restrict__ sidx) {
+                           uint2* __restrict__ srec) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t nruns = counters[kRuns];
   const uint32_t a = A[j];
   const uint32_t r = (a < nruns && rstart[a] == (uint32_t)j) ? a : a - 1u;
   if (rstart[r + 1] - rstart[r] >= minlen) rec_a[j] |= flag;
-  else sidx[(uint32_t)j - dpre[r]] = (uint32_t)j;
+  else srec[(uint32_t)j - dpre[r]] = make_uint2((uint32_t)j, rec_a[j]);
 }
 
 // bucket table of the non-dense records: a bucket's first record and end are run starts
@@ -2271,7 +2291,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_SIDX, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_NSLOTS
 };
 
 struct FftKey {
@@ -2576,7 +2596,8 @@ struct Geom {
   uint4* dtask = nullptr;
   uint32_t* dplane = nullptr;
   // records outside dense runs: sorted record indices and their bucket table
-  uint32_t *sidx = nullptr, *sbstart = nullptr, *sbend = nullptr;
+  uint2* srec = nullptr;  // non-dense records: (record index, origin word)
+  uint32_t *sbstart = nullptr, *sbend = nullptr;
   Buffers buf;
 };
 
@@ -2787,8 +2808,8 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
                                                                        g.dtask, g.dplane, g.counters);
     LAUNCH_CHECK();
     const int64_t ns = n - (int64_t)ndense;
-    g.sidx = (uint32_t*)g.buf.get(S_SIDX, (size_t)std::max<int64_t>(ns, 1) * 4);
-    k_run_mark<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, scr0, scr1, dpre, g.counters, dmin, g.k.dflag, g.sidx);
+    g.srec = (uint2*)g.buf.get(S_SREC, (size_t)std::max<int64_t>(ns, 1) * sizeof(uint2));
+    k_run_mark<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, scr0, scr1, dpre, g.counters, dmin, g.k.dflag, g.srec);
     LAUNCH_CHECK();
     g.sbstart = (uint32_t*)g.buf.get(S_SBSTART, nkeys * 4);
     g.sbend = (uint32_t*)g.buf.get(S_SBEND, nkeys * 4);
@@ -2950,8 +2971,8 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   auto* tbuf = (cplx<R>*)g.tbuf;
   const int PCn = g.pc;
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
-  const uint32_t* gb0 = g.sidx ? g.sbstart : g.bstart;
-  const uint32_t* gb1 = g.sidx ? g.sbend : g.bend;
+  const uint32_t* gb0 = g.srec ? g.sbstart : g.bstart;
+  const uint32_t* gb1 = g.srec ? g.sbend : g.bend;
   // per-wave LDS of the sub-tile gatherer (at most 62.6 KiB: FP64, 4 polarisations, 2 planes)
   const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn, g.nb);
   if (smw > kLdsBudget) HIPNG_THROW(HIPNG_ERUNTIME, "gridder LDS %zu exceeds %zu", smw, kLdsBudget);
@@ -2974,7 +2995,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       PROF("grid");
       {
 #define HIPNG_GW(TT, QQ, BB)                                                                      \
-  k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.sidx, g.rec_a,   \
+  k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.srec, g.rec_a,   \
                                                           (const rec3<R>*)g.rec_f, recv, g.task,  \
                                                           g.chunk_grid, g.counters, plane)
 #define HIPNG_GWB(TT, QQ)                 \
