@@ -69,7 +69,7 @@ def workload(cfg, nchan_override):
                 desc="100 stations 1 km disc, 30 HA, dec -45, 1 ch 100 MHz, Stokes I; 512^2, cell 5e-4, eps 1e-7")
 
 
-def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc):
+def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc, dense_degrid=False):
     """Minimum work of one timed step for kernel class `name`: (FLOP, HBM bytes).
 
     FLOP follow the per-visibility counts of the gridding work model: 4 W^2 W_w (one complex x
@@ -78,7 +78,9 @@ def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc):
     per pass of pc planes its w window meets, plus the planes written (gridder) or read
     (degridder, with the sorted accumulators read and written). The gridding of an invert is
     split between the sub-tile gatherer ("grid": planes written, records outside dense runs)
-    and the dense-run gridder ("grid_dense": the plan's dense_items records)."""
+    and the dense-run gridder ("grid_dense": the plan's dense_items records); with
+    `dense_degrid` the degridding of a predict is split the same way ("degrid": planes read,
+    records outside dense runs; "degrid_dense": the dense_items records)."""
     G2 = plan["gx"] * plan["gy"]
     N2 = plan["nx"] * plan["ny"]
     P = plan["nplanes"]
@@ -95,8 +97,10 @@ def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc):
     if name == "grid_dense":
         return (dense * nb * per_vis, dense * passes * (3 * rsz + nb * csz))
     if name == "degrid":
-        return (nvis * (4.0 * W * W * ww + 30.0 * W),
-                P * nb * G2 * csz + nitems * ww * (4 + 3 * rsz + 2 * nb * csz))
+        items = nitems - dense if dense_degrid else nitems
+        return (items * nb * per_vis, P * nb * G2 * csz + items * ww * (4 + 3 * rsz + 2 * nb * csz))
+    if name == "degrid_dense":
+        return (dense * nb * per_vis, dense * ww * (3 * rsz + 2 * nb * csz))
     if name.startswith("fft"):
         return (nb * P * 5.0 * G2 * math.log2(G2), 2 * 2 * nb * P * G2 * csz)
     if name == "screen_acc":
@@ -601,7 +605,7 @@ def main():
     fp_peak = FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS
 
     def plan_of(name):
-        return plan_p if name in ("degrid", "screen_fill", "fft_c2c_forward") else plan_i
+        return plan_p if name in ("degrid", "degrid_dense", "screen_fill", "fft_c2c_forward") else plan_i
 
     kern = sorted(((tot, name, cnt) for name, (tot, cnt) in prof.items()), reverse=True)
     roof = None
@@ -609,7 +613,7 @@ def main():
         plan = plan_of(name)
         per_step = cnt / args.steps
         pc = max(1, round(plan["nplanes"] / per_step)) if name in ("grid", "grid_dense") else 1  # planes per pass
-        flop, nbytes = algorithmic_work(name, plan, nvis, nrow * nchan, npol, csz, pc)
+        flop, nbytes = algorithmic_work(name, plan, nvis, nrow * nchan, npol, csz, pc, "degrid_dense" in prof)
         if flop is None and nbytes is None:
             continue
         avg_s = tot / cnt * 1e-3

@@ -1944,6 +1944,7 @@ __global__ __launch_bounds__(kBlock) void k_prescale_t(KP k, const I* __restrict
 template <typename R, int W, int NB>
 __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                    const uint32_t* __restrict__ bend,
+                                                   const uint2* __restrict__ srec,
                                                    const uint32_t* __restrict__ rec_a,
                                                    const rec3<R>* __restrict__ rec_f,
                                                    const cplx<R>* __restrict__ plane, int64_t plane_stride,
@@ -2017,12 +2018,21 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
           else hi = mid;
         }
       }
-      const uint32_t j = s_segj[sg] + (vi - s_segv[sg]);
+      // candidate: a record, or (srec given) an entry of the non-dense record list
+      const uint32_t sj = s_segj[sg] + (vi - s_segv[sg]);
+      uint32_t j = sj, Ar;
+      if (srec) {
+        const uint2 e = srec[sj];
+        j = e.x;
+        Ar = e.y;
+      } else {
+        Ar = rec_a[sj];
+      }
       const int s = slo + sg;
       // planes of the pass inside the W_w-plane window of start plane s
       const int qlo = max(0, s - p0), qhi = min(npl, s + k.Ww - p0);
       {
-        const uint32_t A = rec_a[j] & k.amask;
+        const uint32_t A = Ar & k.amask;
         const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
         const rec3<R> f = rec_f[j];
         R kx[W], ky[W];
@@ -2078,6 +2088,124 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
       }
     }
     __syncthreads();
+  }
+}
+
+// Dense-run degridder on the matrix cores (FP32 kernel modes, W <= 8, up to 2 planes per
+// launch). The visibilities of a run share their footprint window, so per polarisation b
+//   U_b[(c, h)][v] = sum_(a, q) (h ? Im : Re) G_q[ax + a][ay + c][b] * kx_v[a] kw_v[q]
+// is the product of the window's 16 x 16 matrix and a 16-row column per visibility, taken
+// 16 visibilities x 4 terms at a time by v_mfma_f32_16x16x4_f32 (lane l supplies A row
+// l % 16, column 4 s + l / 16 and B row 4 s + l / 16, column l % 16 of step s; register r of
+// lane l receives row 4 (l / 16) + r, column l % 16). A lane so holds 4 of the 8 y cells of
+// one visibility and part h: the y contraction sum_c ky_v[c] U_b[(c, h)][v] is 4 products in
+// registers plus one exchange with lane l ^ 16. The results are staged in LDS and added to
+// the visibilities' accumulators one visibility per lane. Tasks are the dense-run gridder's;
+// the other records go through k_degrid.
+template <int NB>
+__global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int npl, const uint4* __restrict__ dtask,
+                                                            const uint32_t* __restrict__ dplane,
+                                                            const rec3<float>* __restrict__ rec_f,
+                                                            const cplx<float>* __restrict__ plane,
+                                                            int64_t plane_stride, cplx<float>* __restrict__ accs,
+                                                            uint32_t* __restrict__ counters) {
+  static_assert(kMaxPC == 2, "the 16 columns of A are 8 cells x 2 planes");
+  constexpr int NW = kBlock / 64;
+  __shared__ float s_tx[NW][64][9];
+  __shared__ float s_ty[NW][64][9];
+  __shared__ float s_tw[NW][64][2];
+  __shared__ float s_out[NW][64][NB][2];
+  const int W = k.W;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gk = lane >> 4, gm = lane & 15;
+  const int c8 = gm & 7, h2 = gm >> 3;
+  const float scale = 2.0f / (float)W;
+  const float beta = (float)(k.beta * 1.4426950408889634);
+  int slo, shi;
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
+  const uint32_t total = counters[kDenseTotal];
+  const uint32_t lo = lower_bound_u32(dplane, total, (uint32_t)slo);
+  const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
+    t = lo + __shfl(t, 0, 64);
+    if (t >= hi) break;
+    const uint4 cur = dtask[t];  // first record, count, packed first cell, start plane
+    const int sp = (int)cur.w;
+    // A: the run's window, row (c, h) = l % 16, column (a, q) = 4 s + l / 16 of step s
+    float bw[NB][4];
+    {
+      int gyi = (int)(cur.z >> 16) + c8;
+      if (gyi >= k.gy) gyi -= k.gy;
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const int kk = 4 * st + gk;
+        const int a = kk & 7, q = kk >> 3;
+        int gxi = (int)(cur.z & 0xffffu) + a;
+        if (gxi >= k.gx) gxi -= k.gx;
+        const bool in = q < npl && a < W && c8 < W;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          bw[b][st] = in ? reinterpret_cast<const float*>(
+                               plane + q * plane_stride + ((int64_t)b * k.gx + gxi) * k.gy + gyi)[h2]
+                         : 0.f;
+      }
+    }
+    for (uint32_t b0 = 0; b0 < cur.y; b0 += 64) {
+      const int cnt = (int)min(64u, cur.y - b0);
+      {
+        const bool live = lane < cnt;
+        rec3<float> f{};
+        if (live) f = rec_f[cur.x + b0 + (uint32_t)lane];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          s_tx[wv][lane][i] = live && i < W ? esk_fast((f.x + (float)i) * scale, beta) : 0.f;
+          s_ty[wv][lane][i] = live && i < W ? esk_fast((f.y + (float)i) * scale, beta) : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int iw = p0 + q - sp;
+          s_tw[wv][lane][q] = (live && q < npl && iw >= 0 && iw < k.Ww)
+                                  ? (k.wstack ? esk_fast((f.w + (float)iw) * scale, beta) : 1.f)
+                                  : 0.f;
+        }
+      }
+      wave_sync();
+      for (int g0 = 0; g0 < cnt; g0 += 16) {
+        const int m = g0 + gm;  // this lane's visibility (B column, D column)
+        float av[4], ky[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          const int kk = 4 * st + gk;
+          av[st] = s_tx[wv][m][kk & 7] * s_tw[wv][m][kk >> 3];
+        }
+        // D rows 4 gk + r = (c, h): c = 4 (gk & 1) + r, h = gk >> 1
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ky[r] = s_ty[wv][m][4 * (gk & 1) + r];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          f32x4 u = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int st = 0; st < 4; ++st) u = __builtin_amdgcn_mfma_f32_16x16x4f32(bw[b][st], av[st], u, 0, 0, 0);
+          float x = u[0] * ky[0] + u[1] * ky[1] + u[2] * ky[2] + u[3] * ky[3];
+          x += __shfl_xor(x, 16, 64);
+          if ((gk & 1) == 0) s_out[wv][m][b][gk >> 1] = x;
+        }
+      }
+      wave_sync();
+      if (lane < cnt) {
+        cplx<float>* o = accs + (int64_t)(cur.x + b0 + (uint32_t)lane) * NB;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          cplx<float> a = o[b];
+          a.re += s_out[wv][lane][b][0];
+          a.im += s_out[wv][lane][b][1];
+          o[b] = a;
+        }
+      }
+      wave_sync();
+    }
   }
 }
 
@@ -2860,6 +2988,36 @@ int degrid_planes(const Geom& g, int npl) {
   return q;
 }
 
+// dense-run kernels: 5 workgroups per CU fit their LDS (29 KiB each, FP32)
+constexpr int kDenseBlocksPerCu = 5;
+
+// FP32 with dense runs: the dense runs on the matrix cores, the other records in k_degrid;
+// otherwise every record in k_degrid
+template <typename R>
+bool degrid_split(const Geom& g) {
+  return std::is_same<R, float>::value && g.srec != nullptr;
+}
+
+void launch_degrid_dense(const Geom& g, int p0, int npl, const cplx<float>* plane, cplx<float>* accs,
+                         hipStream_t st) {
+  const KP& k = g.k;
+  const int64_t pstride = (int64_t)g.nb * k.gx * k.gy;
+  const auto* rf = (const rec3<float>*)g.rec_f;
+  const int nblk = device_cus() * kDenseBlocksPerCu;
+  if (g.nb == 1)
+    k_degrid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+  else if (g.nb == 2)
+    k_degrid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+  else if (g.nb == 3)
+    k_degrid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+  else
+    k_degrid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+  LAUNCH_CHECK();
+}
+void launch_degrid_dense(const Geom&, int, int, const cplx<double>*, cplx<double>*, hipStream_t) {
+  HIPNG_THROW(HIPNG_ERUNTIME, "dense-run degridder is FP32 only");
+}
+
 template <typename R>
 void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>* accs, hipStream_t st) {
   const KP& k = g.k;
@@ -2867,8 +3025,12 @@ void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>
   const size_t sm = (size_t)npl * g.nb * L * L * sizeof(cplx<R>);
   const int64_t pstride = (int64_t)g.nb * k.gx * k.gy;
   const auto* rf = (const rec3<R>*)g.rec_f;
+  const bool split = degrid_split<R>(g);
+  const uint32_t* b0 = split ? g.sbstart : g.bstart;
+  const uint32_t* b1 = split ? g.sbend : g.bend;
+  const uint2* sr = split ? g.srec : nullptr;
 #define HIPNG_DG1(WW, BB)                                                                          \
-  k_degrid<R, WW, BB><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, g.bstart, g.bend, g.rec_a, rf, plane, \
+  k_degrid<R, WW, BB><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, b0, b1, sr, g.rec_a, rf, plane, \
                                                       pstride, accs, g.task, g.chunk_degrid, g.counters)
 #define HIPNG_DG(WW)                        \
   case WW:                                  \
@@ -2918,7 +3080,6 @@ void launch_unsort(const Geom& g, const cplx<R>* accs, const R* wgt, cplx<R>* ms
 }
 
 // dense-run gridder launch: the matrix-core kernel for FP32 records, the vector kernel for FP64
-constexpr int kDenseBlocksPerCu = 5;  // 5 workgroups per CU fit its LDS (29 KiB each, FP32)
 void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<float>* rf,
                        const cplx<float>* recv, cplx<float>* plane, hipStream_t st) {
   const int nblk = device_cus() * kDenseBlocksPerCu;
@@ -3092,10 +3253,17 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
         PROF("degrid_setup");
         int slo, shi;
         chunk_range(k.Ww, P.nplanes, p0 + q, nq, &slo, &shi);
-        build_tasks(g, slo, shi, 1, g.chunk_degrid, st);
+        if (degrid_split<R>(g)) build_tasks(g, slo, shi, 1, g.chunk_degrid, st, 1, g.sbstart, g.sbend);
+        else build_tasks(g, slo, shi, 1, g.chunk_degrid, st);
       }
-      PROF("degrid");
-      launch_degrid<R>(g, p0 + q, nq, plane + q * np1, accs, st);
+      {
+        PROF("degrid");
+        launch_degrid<R>(g, p0 + q, nq, plane + q * np1, accs, st);
+      }
+      if (degrid_split<R>(g)) {
+        PROF("degrid_dense");
+        launch_degrid_dense(g, p0 + q, nq, plane + q * np1, accs, st);
+      }
     }
   }
   PROF("unsort");
