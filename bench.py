@@ -622,14 +622,16 @@ def main():
                     "unit": "GB/s", "algorithmic_bytes_per_launch": int(nbytes / per_step)}
         hbm_view["frac"] = round(hbm_view["achieved"] / HBM_PEAK_GBS, 4)
         if flop is not None:
-            # the gridders are FP32 (FP64) FMA-bound on the vector ALUs; MI355X's dense FP32
-            # peak is the same for vector and matrix units
+            # FP32 modes accumulate on the matrix cores (v_mfma_f32_16x16x4_f32, dense FP32
+            # matrix peak); FP64 gridding is vector FMA (dense FP64 peak)
             ach = flop / per_step / avg_s / 1e12
             roof = {"kernel": name, "bound": "mfma", "achieved": round(ach, 2), "peak": fp_peak,
                     "unit": "TFLOP/s", "frac": round(ach / fp_peak, 4), "traffic": traffic,
                     "avg_launch_ms": round(tot / cnt, 4), "launches": cnt,
                     "algorithmic_flop_per_launch": int(flop / per_step),
-                    "compute": "vector FMA (no MFMA instructions); peak = dense FP%d" % (64 if prec == "fp64" else 32),
+                    "compute": ("vector FP64 FMA; peak = dense FP64" if prec == "fp64" else
+                                "FP32 MFMA (v_mfma_f32_16x16x4_f32) plus vector tap evaluation; "
+                                "peak = dense FP32 matrix"),
                     "hbm_view": hbm_view}
         else:
             roof = {"kernel": name, "bound": "hbm", "achieved": hbm_view["achieved"], "peak": HBM_PEAK_GBS,
