@@ -1973,15 +1973,40 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
     const uint32_t vlo = (task - task_off[tile]) * chunk, vhi = vlo + chunk;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
     if (tile != loaded) {
-      // coalesced along grid columns; LDS written cell-major (polarisation fastest)
+      // coalesced along grid columns; LDS written cell-major (polarisation fastest). Thread t
+      // takes column t % CW (CW >= L) of every RS-th row, row = (q * NB + b) * L + rr advanced
+      // without divisions, and has kTileLoads loads in flight before it writes them to LDS
+      constexpr int kTileLoads = sizeof(R) == 4 ? 8 : 4;
       const int nrows = npl * NB * L;
-      for (int e = threadIdx.x; e < nrows * L; e += kBlock) {
-        const int row = e / L, cc = e - row * L;
-        const int rr = row % L, qb = row / L, b = qb % NB, q = qb / NB;
-        int gxi = tx * T + rr, gyi = ty * T + cc;
-        if (gxi >= k.gx) gxi -= k.gx;
+      const int CW = L > 32 ? 64 : 32, RS = kBlock / CW;  // RS < L: a row step wraps rr at most once
+      const int cc = threadIdx.x & (CW - 1);
+      if (cc < L) {
+        int gyi = ty * T + cc;
         if (gyi >= k.gy) gyi -= k.gy;
-        g[((q * L + rr) * L + cc) * NB + b] = plane[q * plane_stride + ((int64_t)b * k.gx + gxi) * k.gy + gyi];
+        int row = threadIdx.x / CW, rr = row, qb = 0;
+        while (row < nrows) {
+          cplx<R> v[kTileLoads];
+          int dst[kTileLoads];
+#pragma unroll
+          for (int u = 0; u < kTileLoads; ++u) {
+            const bool ok = row < nrows;
+            const int r1 = ok ? rr : 0, qb1 = ok ? qb : 0;  // past the end: reload row 0
+            const int b = qb1 % NB, q = qb1 / NB;
+            int gxi = tx * T + r1;
+            if (gxi >= k.gx) gxi -= k.gx;
+            v[u] = plane[q * plane_stride + ((int64_t)b * k.gx + gxi) * k.gy + gyi];
+            dst[u] = ok ? ((q * L + r1) * L + cc) * NB + b : -1;
+            row += RS;
+            rr += RS;
+            if (rr >= L) {
+              rr -= L;
+              ++qb;
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kTileLoads; ++u)
+            if (dst[u] >= 0) g[dst[u]] = v[u];
+        }
       }
       loaded = tile;
       __syncthreads();
