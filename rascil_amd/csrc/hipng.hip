@@ -568,11 +568,23 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total)
   return off + inc - v;
 }
 
+// A thread's kRsIpt consecutive elements are read (written) as 16-byte vectors when they are
+// all in range (arrays are 16-byte aligned), element by element at the end of the array.
+static_assert(kRsIpt % 4 == 0, "scan segments are whole 16-byte vectors");
 __global__ void k_scan_partial(const uint32_t* __restrict__ a, int64_t m, uint32_t* __restrict__ part) {
   const int64_t base = (int64_t)blockIdx.x * kRsItems + (int64_t)threadIdx.x * kRsIpt;
   uint32_t s = 0;
-  for (int k = 0; k < kRsIpt; ++k)
-    if (base + k < m) s += a[base + k];
+  if (base + kRsIpt <= m) {
+    const uint4* p = reinterpret_cast<const uint4*>(a + base);
+#pragma unroll
+    for (int k = 0; k < kRsIpt / 4; ++k) {
+      const uint4 v = p[k];
+      s += v.x + v.y + v.z + v.w;
+    }
+  } else {
+    for (int k = 0; k < kRsIpt; ++k)
+      if (base + k < m) s += a[base + k];
+  }
   uint32_t tot;
   block_excl_scan(s, &tot);
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
@@ -592,17 +604,40 @@ __global__ void k_scan_top(uint32_t* __restrict__ part, int64_t np) {
 
 __global__ void k_scan_apply(uint32_t* __restrict__ a, int64_t m, const uint32_t* __restrict__ part) {
   const int64_t base = (int64_t)blockIdx.x * kRsItems + (int64_t)threadIdx.x * kRsIpt;
+  const bool whole = base + kRsIpt <= m;
   uint32_t loc[kRsIpt];
-  uint32_t s = 0;
-  for (int k = 0; k < kRsIpt; ++k) {
-    loc[k] = base + k < m ? a[base + k] : 0u;
-    s += loc[k];
+  if (whole) {
+    const uint4* p = reinterpret_cast<const uint4*>(a + base);
+#pragma unroll
+    for (int k = 0; k < kRsIpt / 4; ++k) {
+      const uint4 v = p[k];
+      loc[4 * k] = v.x;
+      loc[4 * k + 1] = v.y;
+      loc[4 * k + 2] = v.z;
+      loc[4 * k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kRsIpt; ++k) loc[k] = base + k < m ? a[base + k] : 0u;
   }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kRsIpt; ++k) s += loc[k];
   uint32_t tot;
   uint32_t run = block_excl_scan(s, &tot) + part[blockIdx.x];
+#pragma unroll
   for (int k = 0; k < kRsIpt; ++k) {
-    if (base + k < m) a[base + k] = run;
-    run += loc[k];
+    const uint32_t x = loc[k];
+    loc[k] = run;
+    run += x;
+  }
+  if (whole) {
+    uint4* p = reinterpret_cast<uint4*>(a + base);
+#pragma unroll
+    for (int k = 0; k < kRsIpt / 4; ++k) p[k] = make_uint4(loc[4 * k], loc[4 * k + 1], loc[4 * k + 2], loc[4 * k + 3]);
+  } else {
+    for (int k = 0; k < kRsIpt; ++k)
+      if (base + k < m) a[base + k] = loc[k];
   }
 }
 
