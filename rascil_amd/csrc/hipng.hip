@@ -1076,7 +1076,7 @@ struct Grid8Vec {
 // for cells (sx0 + a, sy0 + c), taken four entries at a time by v_mfma_f32_16x16x4_f32:
 // lane l supplies row l % 16 of A and column l % 16 of B for entry l / 16 of the group, and
 // register r of lane l receives row 4 (l / 16) + r, column l % 16. Entries past the count up
-// to the next multiple of four must be staged as zeros.
+// to the end of the batch of 64 must be staged as zeros.
 template <int PC, int NB, int NT>
 struct Grid8Mx {
   static_assert(PC <= 2, "the 16 rows of a block are 8 cells x 2 planes");
@@ -1086,17 +1086,23 @@ struct Grid8Mx {
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
+  // rounds of 16 entries (4 MFMA groups, unrolled); the staging pads the batch of 64 with
+  // zeros, so a round may run past the count
   __device__ __forceinline__ void accumulate(const float* s_t, const cplx<float>* s_v, int cnt, int lane) {
     const int gk = lane >> 4, c8 = lane & 7, h2 = (lane >> 3) & 1;
-    for (int e0 = 0; e0 < cnt; e0 += 4) {
-      const int e = e0 + gk;
-      const float* t = s_t + e * NT;
-      const float av = t[c8] * (h2 < PC ? t[16 + h2] : 0.f);
-      const float ty = t[8 + c8];
+    const int n16 = __builtin_amdgcn_readfirstlane(cnt);
+    for (int e0 = 0; e0 < n16; e0 += 16) {
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const float vb = reinterpret_cast<const float*>(s_v + b * 64 + e)[h2];
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * vb, acc[b], 0, 0, 0);
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + 4 * u + gk;
+        const float* t = s_t + e * NT;
+        const float av = t[c8] * (h2 < PC ? t[16 + h2] : 0.f);
+        const float ty = t[8 + c8];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const float vb = reinterpret_cast<const float*>(s_v + b * 64 + e)[h2];
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * vb, acc[b], 0, 0, 0);
+        }
       }
     }
   }
@@ -1647,9 +1653,14 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
     if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
     return lo + __shfl(t, 0, 64);
   };
+  auto task_at = [&](uint32_t i) {  // wave-uniform task fields
+    const uint4 d = dtask[i];
+    return make_uint4(__builtin_amdgcn_readfirstlane(d.x), __builtin_amdgcn_readfirstlane(d.y),
+                      __builtin_amdgcn_readfirstlane(d.z), __builtin_amdgcn_readfirstlane(d.w));
+  };
   uint32_t t = pop();
   if (t >= hi) return;
-  uint4 cur = dtask[t];  // first record, count, packed first cell, start plane
+  uint4 cur = task_at(t);  // first record, count, packed first cell, start plane
   uint32_t tn = pop();
   rec3<float> f{};
   cplx<float> v[NB];
@@ -1695,16 +1706,20 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
     uint32_t nb0 = b0 + 64u;
     if (last) {
       nb0 = 0;
-      nxt = tn < hi ? dtask[tn] : make_uint4(0u, 0u, 0u, 0u);
+      nxt = tn < hi ? task_at(tn) : make_uint4(0u, 0u, 0u, 0u);
     }
     load(nxt, nb0);
-    for (int e0 = 0; e0 < cnt; e0 += 4) {
-      const int e = e0 + gk;
-      const float av = s_tx[wv][e][c8] * s_tw[wv][e][h2];
-      const float ty = s_ty[wv][e][c8];
+    // rounds of 16 visibilities (4 MFMA groups, unrolled); the batch is zero-padded to 64
+    for (int e0 = 0; e0 < cnt; e0 += 16) {
 #pragma unroll
-      for (int b = 0; b < NB; ++b)
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * s_v[wv][b][e][h2], acc[b], 0, 0, 0);
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + 4 * u + gk;
+        const float av = s_tx[wv][e][c8] * s_tw[wv][e][h2];
+        const float ty = s_ty[wv][e][c8];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * s_v[wv][b][e][h2], acc[b], 0, 0, 0);
+      }
     }
     wave_sync();
     if (!last) {
@@ -2191,7 +2206,9 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
     if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
     t = lo + __shfl(t, 0, 64);
     if (t >= hi) break;
-    const uint4 cur = dtask[t];  // first record, count, packed first cell, start plane
+    const uint4 d = dtask[t];  // first record, count, packed first cell, start plane
+    const uint4 cur = make_uint4(__builtin_amdgcn_readfirstlane(d.x), __builtin_amdgcn_readfirstlane(d.y),
+                                 __builtin_amdgcn_readfirstlane(d.z), __builtin_amdgcn_readfirstlane(d.w));
     const int sp = (int)cur.w;
     // A: the run's window, row (c, h) = l % 16, column (a, q) = 4 s + l / 16 of step s
     float bw[NB][4];
@@ -2964,6 +2981,10 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   const double ww = P.wstacking ? P.W : 1;
   g.chunk_grid = pick_chunk((double)n * ww / P.nplanes * 4.0, g.npersist);
   g.chunk_degrid = pick_chunk((double)n * ww / P.nplanes, g.npersist);
+  // HIPNG_CHUNK / HIPNG_CHUNK_DEGRID override the candidates per task (calibration)
+  if (const char* e = std::getenv("HIPNG_CHUNK")) g.chunk_grid = (uint32_t)std::max(256, std::atoi(e)) / 256 * 256;
+  if (const char* e = std::getenv("HIPNG_CHUNK_DEGRID"))
+    g.chunk_degrid = (uint32_t)std::max(256, std::atoi(e)) / 256 * 256;
   if (g.dense) {
     PROF("runs");
     // scr0 = run heads, then their exclusive scan (run of each record); scr1 = run starts
