@@ -241,8 +241,8 @@ def _dense_core(seed=31):
     return uvw, freq
 
 
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_dense_runs_match_explicit(prec, monkeypatch):
+@pytest.mark.parametrize("prec,eps", [("fp64", 1e-5), ("mixed", 1e-5), ("fp32", 1e-4)])
+def test_dense_runs_match_explicit(prec, eps, monkeypatch):
     """Runs of visibilities with a common first cell go through the dense-run gridder; with
     the scattered rest through the sub-tile gatherer, invert and predict still equal the
     brute-force sums to epsilon. Oversampling 2 gives support 7 (lanes outside the
@@ -253,7 +253,6 @@ def test_dense_runs_match_explicit(prec, monkeypatch):
     uvw, freq = _dense_core()
     rng = numpy.random.default_rng(32)
     nx, ny, px = 48, 48, 2e-3
-    eps = 1e-5
     ms = rng.normal(size=(uvw.shape[0], freq.size)) + 1j * rng.normal(size=(uvw.shape[0], freq.size))
     img = rng.normal(size=(nx, ny))
     dev = torch.device("cuda", 0)
@@ -271,15 +270,17 @@ def test_dense_runs_match_explicit(prec, monkeypatch):
     assert rel_l2(v.astype(numpy.complex128), ref_v) < eps
 
 
-def test_dense_runs_four_polarisations_match_explicit(monkeypatch):
-    """The 4-polarisation instances of both gridding paths and of the degridder (the
-    production shape's batch) against the brute-force sums, polarisation by polarisation."""
+@pytest.mark.parametrize("nb", [2, 3, 4])
+def test_dense_runs_polarisations_match_explicit(nb, monkeypatch):
+    """The batched instances of both gridding paths and of both degridders (4 polarisations
+    is the production shape's batch) against the brute-force sums, polarisation by
+    polarisation."""
     import torch
 
     monkeypatch.setenv("HIPNG_SIGMA", "2.0")
     uvw, freq = _dense_core(seed=33)
     rng = numpy.random.default_rng(34)
-    nx, ny, px, eps, nb = 48, 40, 2e-3, 1e-5, 4
+    nx, ny, px, eps = 48, 40, 2e-3, 1e-5
     ms = rng.normal(size=(uvw.shape[0], freq.size, nb)) + 1j * rng.normal(size=(uvw.shape[0], freq.size, nb))
     img = rng.normal(size=(nb, nx, ny))
     dev = torch.device("cuda", 0)
