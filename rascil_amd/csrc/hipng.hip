@@ -458,18 +458,37 @@ __device__ __forceinline__ uint32_t item_digit(const ItemPos& p, const KP& k) {
   return (uint32_t)(p.ax % k.T) * (uint32_t)k.T + (uint32_t)(p.ay % k.T);
 }
 
+// Sort words of item i. Unpacked (db = 0): digit and index. Packed (db > 0, the digit's bit
+// count): the bucket key rides along, its low 32 - db bits above the digit in the key word
+// and the rest above the vb index bits of the value word, so that no later pass has to
+// recompute it (see unpack_key).
 template <bool F32C>
 __global__ void k_digits(const double* __restrict__ uvw, int64_t nitems, const double* __restrict__ sx,
                          const double* __restrict__ sy, const double* __restrict__ sw, KP k,
-                         uint32_t* __restrict__ digits, uint32_t* __restrict__ vals) {
+                         uint32_t* __restrict__ digits, uint32_t* __restrict__ vals, int db, int vb) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= nitems) return;
   const int64_t row = i / k.nchan;
   const int ch = (int)(i - row * k.nchan);
   const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch],
                                    sy[ch], sw[ch], k);
-  digits[i] = item_digit(p, k);
-  vals[i] = (uint32_t)i;
+  const uint32_t dg = item_digit(p, k);
+  if (db == 0) {
+    digits[i] = dg;
+    vals[i] = (uint32_t)i;
+  } else {
+    const uint32_t key = item_key(p, k);
+    digits[i] = (key << db) | dg;
+    vals[i] = (uint32_t)i | ((key >> (32 - db)) << vb);
+  }
+}
+
+// bucket key and item index from a packed pair of sort words (k_digits with db > 0)
+__device__ __forceinline__ uint32_t unpack_key(uint32_t kw, uint32_t vw, int db, int vb) {
+  return (kw >> db) | ((vw >> vb) << (32 - db));
+}
+__device__ __forceinline__ uint32_t unpack_index(uint32_t vw, int vb) {
+  return vb >= 32 ? vw : vw & ((1u << vb) - 1u);
 }
 
 // bucket keys of the items in the order perm gives them
@@ -504,8 +523,10 @@ __device__ __forceinline__ void wave_sync() {
 // Per-block digit histogram of one pass: each wave counts its contiguous quarter of the
 // block's items 64 at a time; lanes with equal digits are found by ballots and their leader
 // adds the group size to the wave's own LDS histogram (no atomics).
-__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                    uint32_t* __restrict__ hist, int64_t nblocks) {
+// db > 0: the input is packed (k_digits) and the pass sorts by the unpacked bucket key
+__global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__ keys,
+                                                    const uint32_t* __restrict__ vals, int db, int vb, int64_t n,
+                                                    int shift, uint32_t* __restrict__ hist, int64_t nblocks) {
   constexpr int NW = kBlock / 64;
   constexpr int R = kRsItems / kBlock;
   __shared__ uint32_t h[NW][256];
@@ -518,6 +539,13 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__
   for (int r = 0; r < R; ++r) {
     const int64_t i = wbase + r * 64 + lane;
     key[r] = i < n ? keys[i] : 0u;
+  }
+  if (db > 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = wbase + r * 64 + lane;
+      key[r] = unpack_key(key[r], i < n ? vals[i] : 0u, db, vb);
+    }
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -646,11 +674,13 @@ __global__ void k_scan_apply(uint32_t* __restrict__ a, int64_t m, const uint32_t
 // wave-major order is item order and the result is stable. The block then places its items
 // in LDS in digit order and writes each digit's run to consecutive output positions
 // (coalesced) starting at the block's offset from the scanned histogram.
+// db > 0: the input is packed (k_digits); the pass sorts by the bucket key and writes the
+// unpacked key and index
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                        int64_t n, int shift, const uint32_t* __restrict__ hist,
-                                                       int64_t nblocks) {
+                                                       int64_t nblocks, int db, int vb) {
   constexpr int NW = kBlock / 64;
   constexpr int R = kRsItems / kBlock;  // rounds of 64 items per wave
   __shared__ uint32_t s_k[kRsItems];
@@ -671,6 +701,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
     const bool ok = i < n;
     key[r] = ok ? kin[i] : 0u;
     val[r] = ok ? vin[i] : 0u;
+    if (db > 0) {
+      key[r] = unpack_key(key[r], val[r], db, vb);
+      val[r] = unpack_index(val[r], vb);
+    }
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -2714,11 +2748,18 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
   const int64_t npart = (m + kRsItems - 1) / kRsItems;
   auto* hist = (uint32_t*)ws(c, S_HIST, m * 4);
   auto* part = (uint32_t*)ws(c, S_PART, npart * 4);
-  auto passes = [&](int bits) {
+  // the digit passes carry the bucket key packed into the sort words when key, digit and
+  // index bits fit in 64; the first key pass unpacks it (no second pass over uvw)
+  const int dbits = 2 * bits_for((uint64_t)P.tile - 1);
+  const int ibits = std::max(1, bits_for((uint64_t)std::max<int64_t>(n, 1) - 1));
+  const char* pe = std::getenv("HIPNG_SORT_PACK");  // 0: always recompute the keys (k_rekey)
+  const bool pack = !(pe && std::atoi(pe) == 0) && dbits < 32 && ibits < 32 && P.keybits + dbits + ibits <= 64;
+  auto passes = [&](int bits, int udb) {
     PROF("radix_sort");
     for (int ps = 0; ps < (bits + 7) / 8; ++ps) {
       const int shift = 8 * ps;
-      k_rs_hist<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, n, shift, hist, nblocks);
+      const int db = ps == 0 ? udb : 0;
+      k_rs_hist<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
       LAUNCH_CHECK();
       k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
       LAUNCH_CHECK();
@@ -2726,7 +2767,7 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
       LAUNCH_CHECK();
       k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
       LAUNCH_CHECK();
-      k_rs_scatter<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks);
+      k_rs_scatter<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
       LAUNCH_CHECK();
       std::swap(k0, k1);
       std::swap(v0, v1);
@@ -2735,14 +2776,15 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
   // LSD: the in-tile origin digit first, then the bucket key of each item in that order
   {
     PROF("keys");
+    const int db = pack ? dbits : 0;
     if (P.precision == HIPNG_FP32)
-      k_digits<true><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
+      k_digits<true><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0, db, ibits);
     else
-      k_digits<false><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0);
+      k_digits<false><<<grid1(n), kBlock, 0, st>>>(uvw, n, tb.sx, tb.sy, tb.sw, k, k0, v0, db, ibits);
     LAUNCH_CHECK();
   }
-  passes(2 * bits_for((uint64_t)P.tile - 1));
-  {
+  passes(dbits, 0);
+  if (!pack) {
     PROF("keys");
     if (P.precision == HIPNG_FP32)
       k_rekey<true><<<grid1(n), kBlock, 0, st>>>(uvw, v0, n, tb.sx, tb.sy, tb.sw, k, k0);
@@ -2750,7 +2792,7 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
       k_rekey<false><<<grid1(n), kBlock, 0, st>>>(uvw, v0, n, tb.sx, tb.sy, tb.sw, k, k0);
     LAUNCH_CHECK();
   }
-  passes(P.keybits);
+  passes(std::max(P.keybits, 1), pack ? dbits : 0);
   *keys_out = k0;
   *perm_out = v0;
   if (free0) *free0 = k1;

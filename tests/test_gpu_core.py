@@ -45,13 +45,20 @@ def test_bucket_sort_bit_exact(small_obs, wst, prec, nchan):
     assert numpy.array_equal(perm, rp)
 
 
-def test_bucket_sort_bit_exact_production_subsample():
-    """1 % of the 512-station / 64-channel production geometry, 8192^2 image plan."""
+@pytest.mark.parametrize("pack,tile", [("1", "16"), ("0", "16"), ("1", "32")])
+def test_bucket_sort_bit_exact_production_subsample(pack, tile, monkeypatch):
+    """1 % of the 512-station / 64-channel production geometry, 8192^2 image plan; with the
+    bucket key carried through the digit passes in the sort words (pack 1, the default when
+    the bits fit) or recomputed from uvw after them (pack 0), and with 32 x 32 tiles (two
+    digit passes, the second also sees low key bits)."""
+    monkeypatch.setenv("HIPNG_SORT_PACK", pack)
+    monkeypatch.setenv("HIPNG_TILE", tile)
     uvw, freq, _ = simulation.config1_geometry(ntimes=128, nchan=64)
     rng = numpy.random.default_rng(0)
     rows = numpy.sort(rng.choice(uvw.shape[0], uvw.shape[0] // 100, replace=False))
     uvw = uvw[rows]
     plan = _extent_plan(uvw, freq, 8192, 8192, 1.1e-5, 1.1e-5, 1e-5, True, "mixed")
+    assert plan.tile == int(tile)
     keys, perm = wgridder.bucket_sort(uvw, freq, plan)
     sx, sy, sw = wgridder.channel_scales(plan, freq)
     rk, rp = sortref.bucket_sort(uvw, sx, sy, sw, plan.as_dict())
