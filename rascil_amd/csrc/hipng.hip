@@ -1003,21 +1003,23 @@ __device__ __host__ __forceinline__ void chunk_range(int Ww, int nplanes, int p0
 
 // ---------------------------------------------------------------------------------
 // Gridder, sub-tile variant. A task is (tile, chunk, 8-column strip of the tile) and belongs
-// to one wave; the wave grids the strip as T/8 sub-tiles of 8 x 8 cells, one cell per lane,
-// all PC planes and nb polarisations of that cell in registers. Per sub-tile the wave
-// filters the chunk's candidates 64 at a time and queues the overlapping ones in LDS; each
-// full batch of 64 is staged with one visibility per lane (kernel taps for the sub-tile's
-// 8 columns and 8 rows, zero outside the support; w taps; values), then every lane
-// accumulates all staged visibilities into its cell without branches. Only wave-level
-// ordering is used, so the waves of a workgroup never wait for each other.
+// to one wave; the wave grids the strip in groups of RG rows (NQ = RG / 8 sub-tiles of 8 x 8
+// cells), all PC planes and nb polarisations of every cell in registers. Per row group the
+// wave filters the chunk's candidates 64 at a time once and queues each overlapping one in
+// LDS for every sub-tile of the group it meets (one queue per sub-tile); each full batch of
+// 64 of a queue is staged with one visibility per lane (kernel taps of the sub-tile's 8
+// columns and 8 rows, zero outside the support; w taps; values), then the wave accumulates
+// all staged visibilities into that sub-tile without branches. Only wave-level ordering is
+// used, so the waves of a workgroup never wait for each other.
 // ---------------------------------------------------------------------------------
 
-// per-wave LDS bytes of k_grid8: values [NB][64], taps [64][(16 + PC) | 1], queue [3][128],
-// segments. The tap row stride is odd so that the 64 staging lanes write to distinct banks.
+// per-wave LDS bytes of k_grid8: values [NB][64], taps [64][(16 + PC) | 1], nq queues of
+// [2][128] words (record; packed offsets and start plane), segments. The tap row stride is
+// odd so that the 64 staging lanes write to distinct banks.
 __host__ __device__ constexpr int grid8_taps(int pc) { return (16 + pc) | 1; }
-__host__ __device__ constexpr size_t grid8_region(size_t rsz, int pc, int nb) {
-  return ((size_t)nb * 64 * 2 * rsz + (size_t)64 * grid8_taps(pc) * rsz + 3 * 128 * 4 + (2 * kMaxSeg + 1) * 4 +
-          15) / 16 * 16;
+__host__ __device__ constexpr size_t grid8_region(size_t rsz, int pc, int nb, int nq) {
+  return ((size_t)nb * 64 * 2 * rsz + (size_t)64 * grid8_taps(pc) * rsz + (size_t)nq * 2 * 128 * 4 +
+          (2 * kMaxSeg + 1) * 4 + 15) / 16 * 16;
 }
 
 // exp(beta (sqrt(1 - z^2) - 1)) from the hardware base-2 exponential and square root
@@ -1032,22 +1034,27 @@ __device__ __forceinline__ double esk_fast(double z, double beta) { return esk(z
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Per-wave accumulator of one 8 x 8 sub-tile: PC planes x NB polarisations of every cell.
-// Staged entries: taps s_t[e * NT + i] (i < 8: x taps of the sub-tile's columns, 8 + i: y
-// taps of its rows, 16 + q: w taps of the pass's planes), values s_v[b * 64 + e].
-// Vector form: lane (cx, cy) = cell (sx0 + cx, sy0 + cy), accumulated one entry at a time.
-template <typename R, int PC, int NB, int NT>
+// Staged entry e: taps s_t[e * NT + i] (i < 8: x taps of the sub-tile's 8 columns; 8 + i:
+// y taps of its 8 rows; 16 + q: w taps of the pass's planes), values s_v[b * 64 + e].
+// Per-wave accumulator of one row group of a strip (sub-tiles h < NH of 8 x 8 cells): PC
+// planes x NB polarisations of every cell; accumulate(h, ...) adds a staged batch of
+// sub-tile h.
+// Vector form: lane (cx, cy) = cells (sx0 + cx, sy0 + 8 h + cy), one entry at a time.
+template <typename R, int PC, int NB, int NT, int NH>
 struct Grid8Vec {
   static constexpr bool kZeroPad = false;  // entries past the count are never read
-  R ar[PC][NB], ai[PC][NB];
+  R ar[NH][PC][NB], ai[NH][PC][NB];
   __device__ __forceinline__ void zero() {
 #pragma unroll
-    for (int q = 0; q < PC; ++q)
+    for (int h = 0; h < NH; ++h)
 #pragma unroll
-      for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
+      for (int q = 0; q < PC; ++q)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) ar[h][q][b] = ai[h][q][b] = 0;
   }
   // every operand of an entry is read before any is used, two entries per round, no
   // branches, so the LDS latency of one entry overlaps the arithmetic of the other
+  template <int H>
   __device__ __forceinline__ void accumulate(const R* s_t, const cplx<R>* s_v, int cnt, int lane) {
     const int cx = lane >> 3, cy = lane & 7;
     auto load = [&](int e, R (&wq)[PC], cplx<R> (&vv)[NB]) {
@@ -1063,8 +1070,8 @@ struct Grid8Vec {
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int q = 0; q < PC; ++q) {
-          ar[q][b] += wq[q] * vv[b].re;
-          ai[q][b] += wq[q] * vv[b].im;
+          ar[H][q][b] += wq[q] * vv[b].re;
+          ai[H][q][b] += wq[q] * vv[b].im;
         }
     };
     int e = 0;
@@ -1088,40 +1095,45 @@ struct Grid8Vec {
                                         int lane) const {
     const int cx = lane >> 3, cy = lane & 7;
 #pragma unroll
-    for (int q = 0; q < PC; ++q) {
+    for (int h = 0; h < NH; ++h)
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        if (q < npl && b < k.nb && (solo || ar[q][b] != (R)0 || ai[q][b] != (R)0)) {
-          cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + cx) * k.gy + y0 + cy;
-          if (solo) {
-            *o = cplx<R>{ar[q][b], ai[q][b]};
-          } else {
-            atomicAdd(&o->re, ar[q][b]);
-            atomicAdd(&o->im, ai[q][b]);
+      for (int q = 0; q < PC; ++q) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (q < npl && b < k.nb && (solo || ar[h][q][b] != (R)0 || ai[h][q][b] != (R)0)) {
+            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + cx) * k.gy + y0 + 8 * h + cy;
+            if (solo) {
+              *o = cplx<R>{ar[h][q][b], ai[h][q][b]};
+            } else {
+              atomicAdd(&o->re, ar[h][q][b]);
+              atomicAdd(&o->im, ai[h][q][b]);
+            }
           }
         }
       }
-    }
   }
 };
 
-// Matrix-core form (FP32, PC <= 2): per polarisation b the 16 x 16 block
-//   G_b[(a, q)][(c, h)] = sum_e kx_e[a] kw_e[q] * ky_e[c] (h ? Im : Re)(val_e[b])
-// for cells (sx0 + a, sy0 + c), taken four entries at a time by v_mfma_f32_16x16x4_f32:
-// lane l supplies row l % 16 of A and column l % 16 of B for entry l / 16 of the group, and
-// register r of lane l receives row 4 (l / 16) + r, column l % 16. Entries past the count up
-// to the end of the batch of 64 must be staged as zeros.
-template <int PC, int NB, int NT>
+// Matrix-core form (FP32, PC <= 2): per sub-tile h and polarisation b the 16 x 16 block
+//   G_hb[(a, q)][(c, p)] = sum_e kx_e[a] kw_e[q] * ky_e[c] (p ? Im : Re)(val_e[b])
+// for cells (sx0 + a, sy0 + 8 h + c), taken four entries at a time by
+// v_mfma_f32_16x16x4_f32: lane l supplies row l % 16 of A and column l % 16 of B for entry
+// l / 16 of the group, and register r of lane l receives row 4 (l / 16) + r, column l % 16.
+// Entries past the count up to the end of the batch of 64 must be staged as zeros.
+template <int PC, int NB, int NT, int NH>
 struct Grid8Mx {
   static_assert(PC <= 2, "the 16 rows of a block are 8 cells x 2 planes");
   static constexpr bool kZeroPad = true;
-  f32x4 acc[NB];
+  f32x4 acc[NH][NB];
   __device__ __forceinline__ void zero() {
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int h = 0; h < NH; ++h)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[h][b] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
   // rounds of 16 entries (4 MFMA groups, unrolled); the staging pads the batch of 64 with
   // zeros, so a round may run past the count
+  template <int H>
   __device__ __forceinline__ void accumulate(const float* s_t, const cplx<float>* s_v, int cnt, int lane) {
     const int gk = lane >> 4, c8 = lane & 7, h2 = (lane >> 3) & 1;
     const int n16 = __builtin_amdgcn_readfirstlane(cnt);
@@ -1135,7 +1147,7 @@ struct Grid8Mx {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           const float vb = reinterpret_cast<const float*>(s_v + b * 64 + e)[h2];
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * vb, acc[b], 0, 0, 0);
+          acc[H][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * vb, acc[H][b], 0, 0, 0);
         }
       }
     }
@@ -1144,30 +1156,39 @@ struct Grid8Mx {
                                         int lane) const {
     const int gk = lane >> 4, c8 = lane & 7, h2 = (lane >> 3) & 1;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = 4 * gk + r;
-      const int a = row & 7, q = row >> 3;
-      if (q < PC && q < npl) {
+    for (int h = 0; h < NH; ++h)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const float val = acc[b][r];
-          if (b < k.nb && (solo || val != 0.f)) {
-            float* o = reinterpret_cast<float*>(plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + a) * k.gy + y0 + c8) + h2;
-            if (solo) *o = val;
-            else atomicAdd(o, val);
+      for (int r = 0; r < 4; ++r) {
+        const int row = 4 * gk + r;
+        const int a = row & 7, q = row >> 3;
+        if (q < PC && q < npl) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const float val = acc[h][b][r];
+            if (b < k.nb && (solo || val != 0.f)) {
+              float* o = reinterpret_cast<float*>(plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + a) * k.gy + y0 +
+                                                  8 * h + c8) + h2;
+              if (solo) *o = val;
+              else atomicAdd(o, val);
+            }
           }
         }
       }
-    }
   }
 };
 
-template <typename R, int PC, int NB, int NT>
-using Grid8Acc = typename std::conditional<std::is_same<R, float>::value, Grid8Mx<PC, NB, NT>,
-                                           Grid8Vec<R, PC, NB, NT>>::type;
+template <typename R, int PC, int NB, int NT, int NH>
+using Grid8Acc = typename std::conditional<std::is_same<R, float>::value, Grid8Mx<PC, NB, NT, NH>,
+                                           Grid8Vec<R, PC, NB, NT, NH>>::type;
+
+// rows per candidate scan of the sub-tile gatherer (RG of k_grid8)
+template <typename R>
+__host__ __device__ constexpr int grid8_rows() {
+  return std::is_same<R, float>::value ? 16 : 8;
+}
 
 // FP32: at most 128 VGPRs, so that 4 waves per SIMD stay resident (as many as the LDS allows)
-template <typename R, int T, int PC, int NB>
+template <typename R, int T, int PC, int NB, int RG>
 __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                   const uint32_t* __restrict__ bend,
                                                   const uint2* __restrict__ srec,
@@ -1177,19 +1198,20 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
                                                   const uint32_t* __restrict__ task_off, uint32_t chunk,
                                                   uint32_t* __restrict__ counters,
                                                   cplx<R>* __restrict__ plane) {
-  constexpr int NS = T / 8;               // strips per tile = sub-tiles per strip
+  constexpr int NS = T / 8;               // strips per tile
+  static_assert((RG == 8 || RG == 16) && T % RG == 0, "row groups of 8 or 16 tile the strip");
   constexpr int NT = grid8_taps(PC);      // per staged visibility: 8 x taps, 8 y taps, PC w taps
-  using Acc = Grid8Acc<R, PC, NB, NT>;
+  constexpr int NQ = RG / 8;              // sub-tiles (queues) of a row group
+  using Acc = Grid8Acc<R, PC, NB, NT, NQ>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int W = k.W;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  unsigned char* base = smem_raw + wv * grid8_region(sizeof(R), PC, NB);
+  unsigned char* base = smem_raw + wv * grid8_region(sizeof(R), PC, NB, NQ);
   cplx<R>* s_v = reinterpret_cast<cplx<R>*>(base);  // [NB][64]
   R* s_t = reinterpret_cast<R*>(s_v + NB * 64);       // [64][NT]
-  uint32_t* q_j = reinterpret_cast<uint32_t*>(s_t + 64 * NT);
-  uint32_t* q_d = q_j + 128;
-  uint32_t* q_s = q_d + 128;
-  uint32_t* s_segv = q_s + 128;             // [kMaxSeg + 1]
+  uint32_t* q_j = reinterpret_cast<uint32_t*>(s_t + 64 * NT);  // [NQ][128] records
+  uint32_t* q_p = q_j + NQ * 128;  // [NQ][128] offsets dx + 128, dy + 128 (8 bits each), start plane
+  uint32_t* s_segv = q_p + NQ * 128;        // [kMaxSeg + 1]
   uint32_t* s_segj = s_segv + kMaxSeg + 1;  // [kMaxSeg]
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const R scale = (R)2 / (R)W;
@@ -1250,11 +1272,13 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
     }
     const uint32_t vlo = ci * chunk;
     const uint32_t vhi = min(vlo + chunk, s_segv[nseg]);
-    for (int sy0 = 0; sy0 < T; sy0 += 8) {
+    for (int sy0 = 0; sy0 < T; sy0 += RG) {
       Acc acc;
       acc.zero();
-      // stage queue entries [0, cnt) into slots [0, cnt) and accumulate them
-      auto stage_gather = [&](int cnt) {
+      // stage entries [0, cnt) of queue H (sub-tile rows sy0 + 8 H + [0, 8)) and accumulate them
+      auto stage_gather = [&](auto hq, int cnt) {
+        constexpr int H = decltype(hq)::value;
+        const int ry0 = sy0 + 8 * H;
         // staging: one queued visibility per lane; its values, the taps of the sub-tile's 8
         // columns and 8 rows (zero outside the support) and its w taps for the pass's planes
         if (Acc::kZeroPad && lane >= cnt) {
@@ -1265,9 +1289,9 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
           for (int i = 0; i < NT; ++i) t[i] = (R)0;
         }
         if (lane < cnt) {
-          const uint32_t j = q_j[lane], d = q_d[lane];
-          const int sp = (int)q_s[lane];
-          const int dx = (int)(d & 0xffffu) - 32768, dy = (int)(d >> 16) - 32768;
+          const uint32_t j = q_j[H * 128 + lane], pk = q_p[H * 128 + lane];
+          const int dx = (int)(pk & 255u) - 128, dy = (int)((pk >> 8) & 255u) - 128;
+          const int sp = (int)(pk >> 16);
           const rec3<R> f = rec_f[j];
 #pragma unroll
           for (int b = 0; b < NB; ++b) s_v[b * 64 + lane] = rec_v[(int64_t)j * NB + b];
@@ -1281,16 +1305,37 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
           }
 #pragma unroll
           for (int i = 0; i < 8; ++i) {
-            const int rx = sx0 + i - dx, ry = sy0 + i - dy;
+            const int rx = sx0 + i - dx, ry = ry0 + i - dy;
             t[i] = (unsigned)rx < (unsigned)W ? esk_fast((f.x + (R)rx) * scale, beta) : (R)0;
             t[8 + i] = (unsigned)ry < (unsigned)W ? esk_fast((f.y + (R)ry) * scale, beta) : (R)0;
           }
         }
         wave_sync();
-        acc.accumulate(s_t, s_v, cnt, lane);
+        acc.template accumulate<H>(s_t, s_v, cnt, lane);
         wave_sync();
       };
-      int qn = 0;
+      int qn[NQ];
+#pragma unroll
+      for (int h = 0; h < NQ; ++h) qn[h] = 0;
+      // queue H with 64 or more entries: stage and accumulate 64, move the rest to the front
+      auto drain = [&](auto hq) {
+        constexpr int H = decltype(hq)::value;
+        if (qn[H] >= 64) {
+          stage_gather(hq, 64);
+          qn[H] -= 64;
+          uint32_t a0 = 0, a1 = 0;
+          if (lane < qn[H]) {
+            a0 = q_j[H * 128 + 64 + lane];
+            a1 = q_p[H * 128 + 64 + lane];
+          }
+          wave_sync();
+          if (lane < qn[H]) {
+            q_j[H * 128 + lane] = a0;
+            q_p[H * 128 + lane] = a1;
+          }
+          wave_sync();
+        }
+      };
       // segment of the chunk's first candidate by bisection, then walked forward block by block
       // (a block of 64 consecutive candidates rarely crosses more than one segment boundary)
       int cur = 0;
@@ -1329,7 +1374,9 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
         const uint32_t j = j_n, Ar = A_n;
         const int lo = lo_n;  // last segment starting at or before v
         fetch(v + 64u, __shfl(lo, 63, 64), j_n, A_n, lo_n);
-        bool ok = false;
+        bool okq[NQ];
+#pragma unroll
+        for (int h = 0; h < NQ; ++h) okq[h] = false;
         int dx = 0, dy = 0, sp = 0;
         if (v < vhi) {
           sp = slo + lo / 4;
@@ -1340,36 +1387,28 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
           if (dx < -k.hgx) dx += k.gx;
           if (dy >= k.hgy) dy -= k.gy;
           if (dy < -k.hgy) dy += k.gy;
-          ok = (Ar & k.dflag) == 0u && (dx + W > sx0) && (dx < sx0 + 8) && (dy + W > sy0) && (dy < sy0 + 8);
+          const bool okx = (Ar & k.dflag) == 0u && (dx + W > sx0) && (dx < sx0 + 8);
+#pragma unroll
+          for (int h = 0; h < NQ; ++h) okq[h] = okx && (dy + W > sy0 + 8 * h) && (dy < sy0 + 8 * h + 8);
         }
-        const unsigned long long m = __ballot(ok);
-        if (ok) {
-          const int pos = qn + (int)__popcll(m & lt);
-          q_j[pos] = j;
-          q_d[pos] = (uint32_t)(dx + 32768) | ((uint32_t)(dy + 32768) << 16);
-          q_s[pos] = (uint32_t)sp;
+        // accepted offsets lie in (-16, 32): 8 bits each with a bias of 128
+        const uint32_t pk = (uint32_t)(dx + 128) | ((uint32_t)(dy + 128) << 8) | ((uint32_t)sp << 16);
+#pragma unroll
+        for (int h = 0; h < NQ; ++h) {
+          const unsigned long long m = __ballot(okq[h]);
+          if (okq[h]) {
+            const int pos = qn[h] + (int)__popcll(m & lt);
+            q_j[h * 128 + pos] = j;
+            q_p[h * 128 + pos] = pk;
+          }
+          qn[h] += (int)__popcll(m);
         }
-        qn += (int)__popcll(m);
         wave_sync();
-        if (qn >= 64) {
-          stage_gather(64);
-          qn -= 64;
-          uint32_t a0 = 0, a1 = 0, a2 = 0;
-          if (lane < qn) {
-            a0 = q_j[64 + lane];
-            a1 = q_d[64 + lane];
-            a2 = q_s[64 + lane];
-          }
-          wave_sync();
-          if (lane < qn) {
-            q_j[lane] = a0;
-            q_d[lane] = a1;
-            q_s[lane] = a2;
-          }
-          wave_sync();
-        }
+        drain(std::integral_constant<int, 0>{});
+        if (NQ > 1) drain(std::integral_constant<int, NQ - 1>{});
       }
-      if (qn > 0) stage_gather(qn);
+      if (qn[0] > 0) stage_gather(std::integral_constant<int, 0>{}, qn[0]);
+      if (NQ > 1 && qn[NQ - 1] > 0) stage_gather(std::integral_constant<int, NQ - 1>{}, qn[NQ - 1]);
       // a tile with a single task is owned by it: every cell is stored (zeros included; the
       // plane is not cleared there beforehand); shared tiles add their non-zero cells
       acc.store(plane, k, npl, tx * T + sx0, ty * T + sy0, solo, lane);
@@ -2804,9 +2843,9 @@ void set_lds_limits() {
   if (done) return;
   done = true;
   // dynamic LDS up to 64 KiB for the tile kernels (the plan keeps tiles within it)
-#define HIPNG_LDS1(RR, TT, QQ, BB)                                                                  \
-  (void)hipFuncSetAttribute((const void*)k_grid8<RR, TT, QQ, BB>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                            (int)kLdsBudget)
+#define HIPNG_LDS1(RR, TT, QQ, BB)                                                                          \
+  (void)hipFuncSetAttribute((const void*)k_grid8<RR, TT, QQ, BB, grid8_rows<RR>()>,                               \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget)
 #define HIPNG_LDS(RR, TT, QQ) \
   HIPNG_LDS1(RR, TT, QQ, 1); HIPNG_LDS1(RR, TT, QQ, 2); HIPNG_LDS1(RR, TT, QQ, 3); HIPNG_LDS1(RR, TT, QQ, 4)
   HIPNG_LDS(float, 32, 1); HIPNG_LDS(float, 32, 2); HIPNG_LDS(float, 16, 1); HIPNG_LDS(float, 16, 2);
@@ -3258,7 +3297,10 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   const uint32_t* gb0 = g.srec ? g.sbstart : g.bstart;
   const uint32_t* gb1 = g.srec ? g.sbend : g.bend;
   // per-wave LDS of the sub-tile gatherer (at most 62.6 KiB: FP64, 4 polarisations, 2 planes)
-  const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn, g.nb);
+  const size_t smw = (kBlock / 64) * grid8_region(sizeof(R), PCn, g.nb, grid8_rows<R>() / 8);
+  // sub-tile gatherer row groups: two sub-tiles per candidate scan in the FP32 modes, one in
+  // FP64 (whose vector accumulators would not leave room for a second sub-tile)
+  constexpr int RG = grid8_rows<R>();
   if (smw > kLdsBudget) HIPNG_THROW(HIPNG_ERUNTIME, "gridder LDS %zu exceeds %zu", smw, kLdsBudget);
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
@@ -3279,9 +3321,9 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       PROF("grid");
       {
 #define HIPNG_GW(TT, QQ, BB)                                                                      \
-  k_grid8<R, TT, QQ, BB><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.srec, g.rec_a,   \
-                                                          (const rec3<R>*)g.rec_f, recv, g.task,  \
-                                                          g.chunk_grid, g.counters, plane)
+  k_grid8<R, TT, QQ, BB, RG><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.srec, g.rec_a,  \
+                                                              (const rec3<R>*)g.rec_f, recv, g.task, \
+                                                              g.chunk_grid, g.counters, plane)
 #define HIPNG_GWB(TT, QQ)                 \
   if (g.nb == 1) HIPNG_GW(TT, QQ, 1);     \
   else if (g.nb == 2) HIPNG_GW(TT, QQ, 2); \
