@@ -1887,27 +1887,25 @@ __device__ __forceinline__ int grid_col_to_img(int c, int ny, int gy) {  // -1 o
   return -1;
 }
 
-// T[i][iy][x] = plane[i][x][col(iy)] for x inside the band, 0 outside; 32 x 32 tiles
+// T[i][iy][x] = plane[i][x][col(iy)] for x inside the band [xlo, xhi) (T stays zero outside
+// it); 32 x 32 tiles, the blocks covering the band from column xbase
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_tr_in(KP k, int xlo, int xhi, const cplx<R>* __restrict__ plane,
-                                                  cplx<R>* __restrict__ tb) {
+__global__ __launch_bounds__(kBlock) void k_tr_in(KP k, int xlo, int xhi, int xbase,
+                                                  const cplx<R>* __restrict__ plane, cplx<R>* __restrict__ tb) {
   __shared__ cplx<R> t[32][33];
   const int i = blockIdx.z;
-  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const int x0 = xbase + blockIdx.x * 32, y0 = blockIdx.y * 32;
   const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
-  const bool any = x0 + 32 > xlo && x0 < xhi;
-  if (any) {
-    for (int r = ly; r < 32; r += 8) {
-      const int x = x0 + r, iy = y0 + lx;
-      cplx<R> v{0, 0};
-      if (x >= xlo && x < xhi && iy < k.ny) v = plane[((int64_t)i * k.gx + x) * k.gy + img_col_to_grid(iy, k.ny, k.gy)];
-      t[r][lx] = v;
-    }
+  for (int r = ly; r < 32; r += 8) {
+    const int x = x0 + r, iy = y0 + lx;
+    cplx<R> v{0, 0};
+    if (x >= xlo && x < xhi && iy < k.ny) v = plane[((int64_t)i * k.gx + x) * k.gy + img_col_to_grid(iy, k.ny, k.gy)];
+    t[r][lx] = v;
   }
   __syncthreads();
   for (int r = ly; r < 32; r += 8) {
     const int iy = y0 + r, x = x0 + lx;
-    if (iy < k.ny && x < k.gx) tb[((int64_t)i * k.ny + iy) * k.gx + x] = any ? t[lx][r] : cplx<R>{0, 0};
+    if (iy < k.ny && x >= xlo && x < xhi) tb[((int64_t)i * k.ny + iy) * k.gx + x] = t[lx][r];
   }
 }
 
@@ -1960,7 +1958,8 @@ __global__ void k_screen_acc_t(KP k, int p0, int npl, const cplx<R>* __restrict_
   for (int b = 0; b < k.nb; ++b) acc[((int64_t)b * k.ny + iy) * k.nx + ix] += (A)sign * sum[b];
 }
 
-// predict: T_q[b][iy][gxi] = (-1)^(tx+ty) dcT[b][iy][ix] exp(+2 pi i w_q nm1s) on image rows, 0 elsewhere
+// predict: T_q[b][iy][gxi] = (-1)^(tx+ty) dcT[b][iy][ix] exp(+2 pi i w_q nm1s) on image rows (T stays
+// zero on the others)
 template <typename R, typename A, bool F32C>
 __global__ void k_screen_fill_t(KP k, int p0, int npl, const A* __restrict__ dc, cplx<R>* __restrict__ tb) {
   const int gxi = blockIdx.x * kBlock + threadIdx.x;
@@ -1972,10 +1971,7 @@ __global__ void k_screen_fill_t(KP k, int p0, int npl, const A* __restrict__ dc,
   if (gxi < hx) tx = gxi;
   else if (gxi >= k.gx - hx) tx = gxi - k.gx;
   else in = false;
-  if (!in) {
-    for (int q = 0; q < npl * k.nb; ++q) tb[((int64_t)q * k.ny + iy) * k.gx + gxi] = cplx<R>{0, 0};
-    return;
-  }
+  if (!in) return;
   const int ty = iy - k.ny / 2;
   const double nm1s = k.wstack ? nm1s_at<F32C>(tx, ty, k, nullptr) : 0.0;
   const R sign = ((tx + ty) & 1) ? (R)-1 : (R)1;
@@ -2568,7 +2564,7 @@ void prof_flush() {
 // ---------------------------------------------------------------------------------
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
-  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_DTASK,
+  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
   S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_NSLOTS
 };
 
@@ -2612,17 +2608,18 @@ void* ws(DevCtx& c, Slot s, size_t bytes) {
 }
 
 // batched in-place 1-D transforms of `len` contiguous points, batches `len` apart
-rocfft_plan fft_plan1(DevCtx& c, int len, int batch, int prec, int fwd) {
+// (out of place: from one contiguous batch to another)
+rocfft_plan fft_plan1(DevCtx& c, int len, int batch, int prec, int fwd, bool inplace = true) {
   if (!g_fft_ready) {
     FFT_TRY(rocfft_setup());
     g_fft_ready = true;
   }
-  FftKey key{len, batch, 1, prec == HIPNG_FP64 ? 0 : 1, fwd};
+  FftKey key{len, batch, inplace ? 1 : 2, prec == HIPNG_FP64 ? 0 : 1, fwd};
   auto it = c.plans.find(key);
   if (it != c.plans.end()) return it->second;
   rocfft_plan plan = nullptr;
   const size_t lengths[1] = {(size_t)len};
-  FFT_TRY(rocfft_plan_create(&plan, rocfft_placement_inplace,
+  FFT_TRY(rocfft_plan_create(&plan, inplace ? rocfft_placement_inplace : rocfft_placement_notinplace,
                              fwd ? rocfft_transform_type_complex_forward
                                  : rocfft_transform_type_complex_inverse,
                              prec == HIPNG_FP64 ? rocfft_precision_double : rocfft_precision_single,
@@ -2872,7 +2869,8 @@ struct Geom {
   // pruned plane transforms: grid rows [xlo, xhi) (band, along gy), image side T (along gx)
   rocfft_plan fft_b_inv = nullptr, fft_b_fwd = nullptr, fft_t_inv = nullptr, fft_t_fwd = nullptr;
   int xlo = 0, xhi = 0;
-  void* tbuf = nullptr;
+  void* tbuf = nullptr;   // transposed band / image rows, zero where no kernel writes (cleared per call)
+  void* tbuf2 = nullptr;  // its transforms along the grid rows
   uint32_t *task = nullptr, *taskpart = nullptr, *counters = nullptr;
   uint32_t chunk_grid = 1024, chunk_degrid = 1024;
   int npersist = 1024;
@@ -3043,10 +3041,11 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     }
   }
   g.tbuf = g.buf.get(S_TBUF, (size_t)g.pc * nb * P.ny * P.gx * csz);
+  g.tbuf2 = g.buf.get(S_TBUF2, (size_t)g.pc * nb * P.ny * P.gx * csz);
   g.fft_b_inv = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 0);
   g.fft_b_fwd = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1);
-  g.fft_t_inv = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 0);
-  g.fft_t_fwd = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 1);
+  g.fft_t_inv = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 0, false);
+  g.fft_t_fwd = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 1, false);
   g.fftwork_size = 0;
   for (rocfft_plan pl : {g.fft_b_inv, g.fft_b_fwd, g.fft_t_inv, g.fft_t_fwd}) {
     size_t w = 0;
@@ -3128,14 +3127,16 @@ void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStr
   LAUNCH_CHECK();
 }
 
-void geom_fft(const Geom& g, rocfft_plan plan, void* data, hipStream_t st) {
+// in place on `data`, or from `data` to `out` with an out-of-place plan
+void geom_fft(const Geom& g, rocfft_plan plan, void* data, hipStream_t st, void* out = nullptr) {
   rocfft_execution_info info = nullptr;
   FFT_TRY(rocfft_execution_info_create(&info));
   rocfft_status e = rocfft_execution_info_set_stream(info, st);
   if (e == rocfft_status_success && g.fftwork_size)
     e = rocfft_execution_info_set_work_buffer(info, g.fftwork, g.fftwork_size);
   void* bufs[1] = {data};
-  if (e == rocfft_status_success) e = rocfft_execute(plan, bufs, nullptr, info);
+  void* obufs[1] = {out};
+  if (e == rocfft_status_success) e = rocfft_execute(plan, bufs, out ? obufs : nullptr, info);
   rocfft_execution_info_destroy(info);
   if (e != rocfft_status_success) HIPNG_THROW(HIPNG_ERUNTIME, "rocfft failed: %d", (int)e);
 }
@@ -3292,6 +3293,14 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
   auto* tbuf = (cplx<R>*)g.tbuf;
+  auto* tbuf2 = (cplx<R>*)g.tbuf2;
+  {
+    // the per-pass kernels write only the band (invert) or the image (predict) part of T
+    PROF("transpose");
+    const int64_t nt = (int64_t)g.pc * g.nb * P.ny * P.gx;
+    k_zero<cplx<R>><<<gridcap(nt), kBlock, 0, st>>>(tbuf, nt);
+    LAUNCH_CHECK();
+  }
   const int PCn = g.pc;
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   const uint32_t* gb0 = g.srec ? g.sbstart : g.bstart;
@@ -3351,16 +3360,17 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     }
     {
       PROF("transpose");
-      k_tr_in<R><<<dim3((P.gx + 31) / 32, (P.ny + 31) / 32, npl * g.nb), kBlock, 0, st>>>(k, g.xlo, g.xhi, plane,
-                                                                                       tbuf);
+      const int xb = g.xlo & ~31;
+      k_tr_in<R><<<dim3((g.xhi - xb + 31) / 32, (P.ny + 31) / 32, npl * g.nb), kBlock, 0, st>>>(
+          k, g.xlo, g.xhi, xb, plane, tbuf);
       LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_inverse");
-      geom_fft(g, g.fft_t_inv, tbuf, st);
+      geom_fft(g, g.fft_t_inv, tbuf, st, tbuf2);
     }
     PROF("screen_acc");
-    k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, p0, npl, tbuf, acc);
+    k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, p0, npl, tbuf2, acc);
     LAUNCH_CHECK();
   }
   PROF("finalize");
@@ -3388,6 +3398,14 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
   auto* tbuf = (cplx<R>*)g.tbuf;
+  auto* tbuf2 = (cplx<R>*)g.tbuf2;
+  {
+    // the per-pass kernels write only the band (invert) or the image (predict) part of T
+    PROF("transpose");
+    const int64_t nt = (int64_t)g.pc * g.nb * P.ny * P.gx;
+    k_zero<cplx<R>><<<gridcap(nt), kBlock, 0, st>>>(tbuf, nt);
+    LAUNCH_CHECK();
+  }
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
     const int npl = std::min(g.pc, P.nplanes - p0);
@@ -3398,12 +3416,12 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     }
     {
       PROF("fft_c2c_forward");
-      geom_fft(g, g.fft_t_fwd, tbuf, st);
+      geom_fft(g, g.fft_t_fwd, tbuf, st, tbuf2);
     }
     {
       PROF("transpose");
       k_tr_out<R><<<dim3((g.xhi - g.xlo + 31) / 32, (P.gy + 31) / 32, npl * g.nb), kBlock, 0, st>>>(
-          k, g.xlo, g.xhi, tbuf, plane);
+          k, g.xlo, g.xhi, tbuf2, plane);
       LAUNCH_CHECK();
     }
     {
