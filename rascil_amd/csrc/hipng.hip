@@ -992,7 +992,11 @@ constexpr int kNCounters = kDenseCnt + 3;
 // (measured at the production shape: gridding 593 / 577 / 593 / 614 ms per invert for
 // minimum runs of 64 / 128 / 256 / 1024, 698 ms with the dense path off).
 constexpr uint32_t kDenseMin = 128;
-constexpr uint32_t kDenseChunk = 512;            // visibilities per dense-run task
+// visibilities per dense-run task: a task pays one flush (gridder) or window load (degridder)
+// per pass, so long tasks pay off until the longest runs unbalance the waves (dense-run
+// gridder + degridder per production-shape step: 205 / 175 / 175 / 191 ms for tasks of at
+// most 512 / 2048 / 8192 / 65536)
+constexpr uint32_t kDenseChunk = 2048;
 
 // start planes whose W_w-plane window meets planes [p0, p0 + npl)
 __device__ __host__ __forceinline__ void chunk_range(int Ww, int nplanes, int p0, int npl, int* slo,
