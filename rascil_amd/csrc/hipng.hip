@@ -965,15 +965,14 @@ __global__ void k_tasks(KP k, int slo, int shi, int nq, const uint32_t* __restri
   if (nt) atomicAdd(&counters[1], nt);
 }
 
-// last tile whose first task index is <= task (tiles without tasks share the next offset)
-__device__ __forceinline__ int task_tile(const uint32_t* __restrict__ task_off, int ntiles, uint32_t task) {
-  int lo = 0, hi = ntiles;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (task_off[mid] <= task) lo = mid;
-    else hi = mid;
-  }
-  return lo;
+// tab[i] = tile of task i, for every task of the pass (task_off = exclusive scan of the per-tile
+// task counts, counters[1] = their total): one load per task instead of a binary search
+__global__ void k_task_tiles(const uint32_t* __restrict__ task_off, int64_t ntiles,
+                             const uint32_t* __restrict__ counters, uint32_t* __restrict__ tab) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ntiles) return;
+  const uint32_t end = t + 1 < ntiles ? task_off[t + 1] : counters[1];
+  for (uint32_t i = task_off[t]; i < end; ++i) tab[i] = (uint32_t)t;
 }
 
 // ---------------------------------------------------------------------------------
@@ -1199,7 +1198,8 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
                                                   const uint32_t* __restrict__ rec_a,
                                                   const rec3<R>* __restrict__ rec_f,
                                                   const cplx<R>* __restrict__ rec_v,
-                                                  const uint32_t* __restrict__ task_off, uint32_t chunk,
+                                                  const uint32_t* __restrict__ task_off,
+                                                  const uint32_t* __restrict__ task_tab, uint32_t chunk,
                                                   uint32_t* __restrict__ counters,
                                                   cplx<R>* __restrict__ plane) {
   constexpr int NS = T / 8;               // strips per tile
@@ -1246,7 +1246,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
       ++qi;
     }
     if (task == 0xffffffffu) break;
-    const int tile = task_tile(task_off, (int)k.ntiles, task);
+    const int tile = (int)task_tab[task];
     const uint32_t local = task - task_off[tile];
     const uint32_t ci = local / NS;
     const int sx0 = (int)(local - ci * NS) * 8;
@@ -2072,7 +2072,8 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
                                                    const rec3<R>* __restrict__ rec_f,
                                                    const cplx<R>* __restrict__ plane, int64_t plane_stride,
                                                    cplx<R>* __restrict__ accs,
-                                                   const uint32_t* __restrict__ task_off, uint32_t chunk,
+                                                   const uint32_t* __restrict__ task_off,
+                                                   const uint32_t* __restrict__ task_tab, uint32_t chunk,
                                                    uint32_t* __restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);  // [npl][L][L][NB]
@@ -2092,7 +2093,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
     __syncthreads();
     const uint32_t task = s_task;
     if (task >= total) break;
-    const int tile = task_tile(task_off, (int)k.ntiles, task);
+    const int tile = (int)task_tab[task];
     const uint32_t vlo = (task - task_off[tile]) * chunk, vhi = vlo + chunk;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
     if (tile != loaded) {
@@ -2568,7 +2569,7 @@ void prof_flush() {
 // ---------------------------------------------------------------------------------
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
-  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
+  S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_TASKTAB, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
   S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_NSLOTS
 };
 
@@ -2876,6 +2877,7 @@ struct Geom {
   void* tbuf = nullptr;   // transposed band / image rows, zero where no kernel writes (cleared per call)
   void* tbuf2 = nullptr;  // its transforms along the grid rows
   uint32_t *task = nullptr, *taskpart = nullptr, *counters = nullptr;
+  uint32_t* tasktab = nullptr;  // tile of every task of a pass
   uint32_t chunk_grid = 1024, chunk_degrid = 1024;
   int npersist = 1024;
   int pc = 1;  // w planes per gridding pass / FFT batch
@@ -3069,6 +3071,13 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   if (const char* e = std::getenv("HIPNG_CHUNK")) g.chunk_grid = (uint32_t)std::max(256, std::atoi(e)) / 256 * 256;
   if (const char* e = std::getenv("HIPNG_CHUNK_DEGRID"))
     g.chunk_degrid = (uint32_t)std::max(256, std::atoi(e)) / 256 * 256;
+  {
+    // tasks of a pass: sum over tiles of ceil(candidates / chunk) x strips, where a record is
+    // a candidate of at most 4 tiles (gridder) or 1 (degridder)
+    const int64_t tg = (4 * n / g.chunk_grid + ntiles) * (P.tile / 8);
+    const int64_t td = n / g.chunk_degrid + ntiles;
+    g.tasktab = (uint32_t*)g.buf.get(S_TASKTAB, (size_t)(std::max(tg, td) + 1) * 4);
+  }
   if (g.dense) {
     PROF("runs");
     // scr0 = run heads, then their exclusive scan (run of each record); scr1 = run starts
@@ -3128,6 +3137,8 @@ void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStr
   k_scan_top<<<1, kBlock, 0, st>>>(g.taskpart, npart);
   LAUNCH_CHECK();
   k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
+  LAUNCH_CHECK();
+  k_task_tiles<<<grid1(ntiles), kBlock, 0, st>>>(g.task, ntiles, g.counters, g.tasktab);
   LAUNCH_CHECK();
 }
 
@@ -3198,7 +3209,8 @@ void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>
   const uint2* sr = split ? g.srec : nullptr;
 #define HIPNG_DG1(WW, BB)                                                                          \
   k_degrid<R, WW, BB><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, b0, b1, sr, g.rec_a, rf, plane, \
-                                                      pstride, accs, g.task, g.chunk_degrid, g.counters)
+                                                      pstride, accs, g.task, g.tasktab, g.chunk_degrid,  \
+                                                      g.counters)
 #define HIPNG_DG(WW)                        \
   case WW:                                  \
     if (g.nb == 1) HIPNG_DG1(WW, 1);        \
@@ -3336,7 +3348,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
 #define HIPNG_GW(TT, QQ, BB)                                                                      \
   k_grid8<R, TT, QQ, BB, RG><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.srec, g.rec_a,  \
                                                               (const rec3<R>*)g.rec_f, recv, g.task, \
-                                                              g.chunk_grid, g.counters, plane)
+                                                              g.tasktab, g.chunk_grid, g.counters, plane)
 #define HIPNG_GWB(TT, QQ)                 \
   if (g.nb == 1) HIPNG_GW(TT, QQ, 1);     \
   else if (g.nb == 2) HIPNG_GW(TT, QQ, 2); \
