@@ -986,11 +986,12 @@ constexpr int kDenseCnt = kQueueBase + kXcd;     // counters[kDenseCnt]: dense-r
 constexpr int kDenseTotal = kDenseCnt + 1;       // counters[kDenseTotal]: dense-run chunks
 constexpr int kRuns = kDenseCnt + 2;             // counters[kRuns]: runs (geometry build)
 constexpr int kNCounters = kDenseCnt + 3;
-// Shortest run the dense path takes: each dense task ends in 64 x PC x NB x 2 global atomic
-// adds, which cost more than the sub-tile gatherer spends on runs shorter than ~100
-// (measured at the production shape: gridding 593 / 577 / 593 / 614 ms per invert for
-// minimum runs of 64 / 128 / 256 / 1024, 698 ms with the dense path off).
-constexpr uint32_t kDenseMin = 128;
+// Shortest run the dense path takes: a dense task pays a flush of 64 x PC x NB x 2 global
+// atomic adds (gridder) or a window load (degridder) per pass, which for short runs costs
+// more than the sub-tile gatherer and the tile degridder spend on them (production shape,
+// invert + predict: 3130 / 3182 / 3200 / 3204 / 3224 / 3181 / 3152 Mvis/s for minimum runs
+// of 64 / 96 / 128 / 192 / 256 / 384 / 512).
+constexpr uint32_t kDenseMin = 256;
 // visibilities per dense-run task: a task pays one flush (gridder) or window load (degridder)
 // per pass, so long tasks pay off until the longest runs unbalance the waves (dense-run
 // gridder + degridder per production-shape step: 205 / 175 / 175 / 191 ms for tasks of at

@@ -253,7 +253,7 @@ def test_dense_runs_match_explicit(prec, eps, monkeypatch):
     """Runs of visibilities with a common first cell go through the dense-run gridder; with
     the scattered rest through the sub-tile gatherer, invert and predict still equal the
     brute-force sums to epsilon. Oversampling 2 gives support 7 (lanes outside the
-    footprint idle) and about 2/3 of the visibilities in runs of 128 or more."""
+    footprint idle) and more than half of the visibilities in dense runs."""
     import torch
 
     monkeypatch.setenv("HIPNG_SIGMA", "2.0")
