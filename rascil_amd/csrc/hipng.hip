@@ -1215,7 +1215,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
   cplx<R>* s_v = reinterpret_cast<cplx<R>*>(base);  // [NB][64]
   R* s_t = reinterpret_cast<R*>(s_v + NB * 64);       // [64][NT]
   uint32_t* q_j = reinterpret_cast<uint32_t*>(s_t + 64 * NT);  // [NQ][128] records
-  uint32_t* q_p = q_j + NQ * 128;  // [NQ][128] offsets dx + 128, dy + 128 (8 bits each), start plane
+  uint32_t* q_p = q_j + NQ * 128;  // [NQ][128] offsets dx + 128, dy + 128 (8 bits each), start plane - slo
   uint32_t* s_segv = q_p + NQ * 128;        // [kMaxSeg + 1]
   uint32_t* s_segj = s_segv + kMaxSeg + 1;  // [kMaxSeg]
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1296,7 +1296,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
         if (lane < cnt) {
           const uint32_t j = q_j[H * 128 + lane], pk = q_p[H * 128 + lane];
           const int dx = (int)(pk & 255u) - 128, dy = (int)((pk >> 8) & 255u) - 128;
-          const int sp = (int)(pk >> 16);
+          const int sp = slo + (int)(pk >> 16);
           const rec3<R> f = rec_f[j];
 #pragma unroll
           for (int b = 0; b < NB; ++b) s_v[b * 64 + lane] = rec_v[(int64_t)j * NB + b];
@@ -1396,8 +1396,9 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
 #pragma unroll
           for (int h = 0; h < NQ; ++h) okq[h] = okx && (dy + W > sy0 + 8 * h) && (dy < sy0 + 8 * h + 8);
         }
-        // accepted offsets lie in (-16, 32): 8 bits each with a bias of 128
-        const uint32_t pk = (uint32_t)(dx + 128) | ((uint32_t)(dy + 128) << 8) | ((uint32_t)sp << 16);
+        // accepted offsets lie in (-16, 32): 8 bits each with a bias of 128; the start plane
+        // relative to the pass's first candidate plane is at most W + PC - 2
+        const uint32_t pk = (uint32_t)(dx + 128) | ((uint32_t)(dy + 128) << 8) | ((uint32_t)(sp - slo) << 16);
 #pragma unroll
         for (int h = 0; h < NQ; ++h) {
           const unsigned long long m = __ballot(okq[h]);
