@@ -1430,25 +1430,87 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
 // run in registers; the sub-tile gatherer skips their records (flag bit in rec_a).
 // ---------------------------------------------------------------------------------
 // head[j] = 1 where a run starts (bucket key or first cell differ from the previous record)
-__global__ void k_run_heads(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a, int64_t n,
-                            uint32_t* __restrict__ head) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  head[j] = (j == 0 || keys[j] != keys[j - 1] || rec_a[j] != rec_a[j - 1]) ? 1u : 0u;
+// Run heads of the kRsIpt consecutive records from `base`: a run starts where the bucket key
+// or the first cell changes (h = 0 past the end). Vector loads when all are in range.
+__device__ __forceinline__ void run_heads(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a,
+                                          int64_t n, int64_t base, uint32_t (&h)[kRsIpt]) {
+  uint32_t kk[kRsIpt], aa[kRsIpt];
+  if (base + kRsIpt <= n) {
+    const uint4* pk = reinterpret_cast<const uint4*>(keys + base);
+    const uint4* pa = reinterpret_cast<const uint4*>(rec_a + base);
+#pragma unroll
+    for (int q = 0; q < kRsIpt / 4; ++q) {
+      const uint4 a = pk[q], b = pa[q];
+      kk[4 * q] = a.x, kk[4 * q + 1] = a.y, kk[4 * q + 2] = a.z, kk[4 * q + 3] = a.w;
+      aa[4 * q] = b.x, aa[4 * q + 1] = b.y, aa[4 * q + 2] = b.z, aa[4 * q + 3] = b.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kRsIpt; ++i) {
+      kk[i] = base + i < n ? keys[base + i] : 0u;
+      aa[i] = base + i < n ? rec_a[base + i] : 0u;
+    }
+  }
+  uint32_t kp = 0, ap = 0;
+  if (base > 0 && base <= n) {
+    kp = keys[base - 1];
+    ap = rec_a[base - 1];
+  }
+#pragma unroll
+  for (int i = 0; i < kRsIpt; ++i) {
+    const int64_t j = base + i;
+    h[i] = (j < n && (j == 0 || kk[i] != kp || aa[i] != ap)) ? 1u : 0u;
+    kp = kk[i];
+    ap = aa[i];
+  }
 }
 
-// with A the exclusive scan of the heads: rstart[run] = first record, rstart[nruns] = n
-__global__ void k_run_starts(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a, int64_t n,
-                             const uint32_t* __restrict__ A, uint32_t* __restrict__ rstart,
-                             uint32_t* __restrict__ counters) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= n) return;
-  const bool head = j == 0 || keys[j] != keys[j - 1] || rec_a[j] != rec_a[j - 1];
-  if (head) rstart[A[j]] = (uint32_t)j;
-  if (j == n - 1) {
-    const uint32_t nruns = A[j] + (head ? 1u : 0u);
-    rstart[nruns] = (uint32_t)n;
-    counters[kRuns] = nruns;
+// Exclusive scan of the run heads, computed on the fly (no heads array), in kRsItems blocks:
+// partial sums per block, then A[j] = number of run heads before record j, with each head's
+// position written to rstart[A[j]] and, at the last record, rstart[nruns] = n and
+// counters[kRuns] = nruns.
+__global__ void k_runscan_partial(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a, int64_t n,
+                                  uint32_t* __restrict__ part) {
+  const int64_t base = (int64_t)blockIdx.x * kRsItems + (int64_t)threadIdx.x * kRsIpt;
+  uint32_t h[kRsIpt];
+  run_heads(keys, rec_a, n, base, h);
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kRsIpt; ++i) s += h[i];
+  uint32_t tot;
+  block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void k_runscan_apply(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ rec_a, int64_t n,
+                                const uint32_t* __restrict__ part, uint32_t* __restrict__ A,
+                                uint32_t* __restrict__ rstart, uint32_t* __restrict__ counters) {
+  const int64_t base = (int64_t)blockIdx.x * kRsItems + (int64_t)threadIdx.x * kRsIpt;
+  uint32_t h[kRsIpt];
+  run_heads(keys, rec_a, n, base, h);
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kRsIpt; ++i) s += h[i];
+  uint32_t tot;
+  uint32_t run = block_excl_scan(s, &tot) + part[blockIdx.x];
+  uint32_t loc[kRsIpt];
+#pragma unroll
+  for (int i = 0; i < kRsIpt; ++i) {
+    loc[i] = run;
+    if (h[i]) rstart[run] = (uint32_t)(base + i);
+    run += h[i];
+  }
+  if (base + kRsIpt <= n) {
+    uint4* p = reinterpret_cast<uint4*>(A + base);
+#pragma unroll
+    for (int q = 0; q < kRsIpt / 4; ++q) p[q] = make_uint4(loc[4 * q], loc[4 * q + 1], loc[4 * q + 2], loc[4 * q + 3]);
+  } else {
+    for (int i = 0; i < kRsIpt; ++i)
+      if (base + i < n) A[base + i] = loc[i];
+  }
+  if (base < n && base + kRsIpt >= n) {  // the thread holding the last record
+    rstart[run] = (uint32_t)n;
+    counters[kRuns] = run;
   }
 }
 
@@ -3082,16 +3144,20 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   }
   if (g.dense) {
     PROF("runs");
-    // scr0 = run heads, then their exclusive scan (run of each record); scr1 = run starts
+    // scr0 = exclusive scan of the run heads (run of each record); scr1 = run starts
     // (n + 1 entries); per-run task counts and dense lengths in workspace, scanned over the
     // runs only. Two synchronisations: the run count and the dense total size the lists.
     uint32_t* part = (uint32_t*)ws(c, S_HIST, (size_t)((n + kRsItems) / kRsItems + 1) * 4);
     const uint32_t dmin = dense_min_run();
-    k_run_heads<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, n, scr0);
-    LAUNCH_CHECK();
-    excl_scan(scr0, n, part, st);
-    k_run_starts<<<grid1(n), kBlock, 0, st>>>(keys, g.rec_a, n, scr0, scr1, g.counters);
-    LAUNCH_CHECK();
+    {
+      const int64_t np = (n + kRsItems - 1) / kRsItems;
+      k_runscan_partial<<<(unsigned)np, kBlock, 0, st>>>(keys, g.rec_a, n, part);
+      LAUNCH_CHECK();
+      k_scan_top<<<1, kBlock, 0, st>>>(part, np);
+      LAUNCH_CHECK();
+      k_runscan_apply<<<(unsigned)np, kBlock, 0, st>>>(keys, g.rec_a, n, part, scr0, scr1, g.counters);
+      LAUNCH_CHECK();
+    }
     uint32_t nruns = 0;
     HIP_TRY(hipMemcpyAsync(&nruns, g.counters + kRuns, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
