@@ -523,15 +523,21 @@ __device__ __forceinline__ void wave_sync() {
 // Per-block digit histogram of one pass: each wave counts its contiguous quarter of the
 // block's items 64 at a time; lanes with equal digits are found by ballots and their leader
 // adds the group size to the wave's own LDS histogram (no atomics).
-// db > 0: the input is packed (k_digits) and the pass sorts by the unpacked bucket key
+// BITS-bit digits (8 or 9) from bit `shift`; db > 0: the input is packed (k_digits) and
+// the pass sorts by the unpacked bucket key
+template <int BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ vals, int db, int vb, int64_t n,
                                                     int shift, uint32_t* __restrict__ hist, int64_t nblocks) {
   constexpr int NW = kBlock / 64;
   constexpr int R = kRsItems / kBlock;
-  __shared__ uint32_t h[NW][256];
+  constexpr int ND = 1 << BITS, DPT = ND / kBlock;  // digits, digits per thread
+  static_assert(DPT >= 1 && ND % kBlock == 0, "digits per thread");
+  __shared__ uint32_t h[NW][ND];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int w = 0; w < NW; ++w) h[w][threadIdx.x] = 0;
+  for (int w = 0; w < NW; ++w)
+#pragma unroll
+    for (int e = 0; e < DPT; ++e) h[w][threadIdx.x + e * kBlock] = 0;
   __syncthreads();
   const int64_t wbase = (int64_t)blockIdx.x * kRsItems + (int64_t)wv * (kRsItems / NW);
   uint32_t key[R];
@@ -550,10 +556,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const bool ok = wbase + r * 64 + lane < n;
-    const uint32_t d = (key[r] >> shift) & 255u;
+    const uint32_t d = (key[r] >> shift) & (uint32_t)(ND - 1);
     unsigned long long peers = __ballot(ok);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
       const unsigned long long m = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
@@ -561,10 +567,14 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__
     wave_sync();
   }
   __syncthreads();
-  uint32_t tot = 0;
 #pragma unroll
-  for (int w = 0; w < NW; ++w) tot += h[w][threadIdx.x];
-  hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = tot;
+  for (int e = 0; e < DPT; ++e) {
+    const int dd = threadIdx.x + e * kBlock;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) tot += h[w][dd];
+    hist[(int64_t)dd * nblocks + blockIdx.x] = tot;
+  }
 }
 
 
@@ -674,8 +684,9 @@ __global__ void k_scan_apply(uint32_t* __restrict__ a, int64_t m, const uint32_t
 // wave-major order is item order and the result is stable. The block then places its items
 // in LDS in digit order and writes each digit's run to consecutive output positions
 // (coalesced) starting at the block's offset from the scanned histogram.
-// db > 0: the input is packed (k_digits); the pass sorts by the bucket key and writes the
-// unpacked key and index
+// BITS-bit digits (8 or 9). db > 0: the input is packed (k_digits); the pass sorts by the
+// bucket key and writes the unpacked key and index
+template <int BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -683,16 +694,20 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
                                                        int64_t nblocks, int db, int vb) {
   constexpr int NW = kBlock / 64;
   constexpr int R = kRsItems / kBlock;  // rounds of 64 items per wave
+  constexpr int ND = 1 << BITS, DPT = ND / kBlock;  // digits, digits per thread
+  static_assert(DPT >= 1 && ND % kBlock == 0, "digits per thread");
   __shared__ uint32_t s_k[kRsItems];
   __shared__ uint32_t s_v[kRsItems];
-  __shared__ uint32_t wcnt[NW][256];
-  __shared__ uint32_t lbase[256];
-  __shared__ uint32_t gbase[256];
+  __shared__ uint32_t wcnt[NW][ND];
+  __shared__ uint32_t lbase[ND];
+  __shared__ uint32_t gbase[ND];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t base = (int64_t)blockIdx.x * kRsItems;
   const int64_t wbase = base + (int64_t)wv * (kRsItems / NW);
-  for (int w = 0; w < NW; ++w) wcnt[w][threadIdx.x] = 0;
+  for (int w = 0; w < NW; ++w)
+#pragma unroll
+    for (int e = 0; e < DPT; ++e) wcnt[w][threadIdx.x + e * kBlock] = 0;
   __syncthreads();
   uint32_t key[R], val[R], rk[R];
 #pragma unroll
@@ -709,10 +724,10 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const bool ok = wbase + r * 64 + lane < n;
-    const uint32_t d = (key[r] >> shift) & 255u;
+    const uint32_t d = (key[r] >> shift) & (uint32_t)(ND - 1);
     unsigned long long peers = __ballot(ok);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < BITS; ++b) {
       const unsigned long long m = __ballot((d >> b) & 1u);
       peers &= ((d >> b) & 1u) ? m : ~m;
     }
@@ -728,26 +743,37 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
   }
   __syncthreads();
   {
-    // digit d = thread: block total, exclusive scan over digits, per-wave offsets
-    const int d = threadIdx.x;
-    uint32_t tot = 0;
-    for (int w = 0; w < NW; ++w) tot += wcnt[w][d];
+    // digits DPT t .. DPT t + DPT - 1 = thread t: block totals, exclusive scan over digits,
+    // per-wave offsets
+    uint32_t tot[DPT], sum = 0;
+#pragma unroll
+    for (int e = 0; e < DPT; ++e) {
+      const int d = DPT * threadIdx.x + e;
+      tot[e] = 0;
+      for (int w = 0; w < NW; ++w) tot[e] += wcnt[w][d];
+      sum += tot[e];
+    }
     uint32_t all;
-    const uint32_t ex = block_excl_scan(tot, &all);
-    lbase[d] = ex;
-    gbase[d] = hist[(int64_t)d * nblocks + blockIdx.x];
-    uint32_t run = ex;
-    for (int w = 0; w < NW; ++w) {
-      const uint32_t c = wcnt[w][d];
-      wcnt[w][d] = run;
-      run += c;
+    uint32_t ex = block_excl_scan(sum, &all);
+#pragma unroll
+    for (int e = 0; e < DPT; ++e) {
+      const int d = DPT * threadIdx.x + e;
+      lbase[d] = ex;
+      gbase[d] = hist[(int64_t)d * nblocks + blockIdx.x];
+      uint32_t run = ex;
+      for (int w = 0; w < NW; ++w) {
+        const uint32_t c = wcnt[w][d];
+        wcnt[w][d] = run;
+        run += c;
+      }
+      ex += tot[e];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     if (wbase + r * 64 + lane < n) {
-      const uint32_t pos = wcnt[wv][(key[r] >> shift) & 255u] + rk[r];
+      const uint32_t pos = wcnt[wv][(key[r] >> shift) & (uint32_t)(ND - 1)] + rk[r];
       s_k[pos] = key[r];
       s_v[pos] = val[r];
     }
@@ -756,7 +782,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
   const int cnt = (int)min((int64_t)kRsItems, n - base);
   for (int idx = threadIdx.x; idx < cnt; idx += kBlock) {
     const uint32_t kk = s_k[idx];
-    const uint32_t d = (kk >> shift) & 255u;
+    const uint32_t d = (kk >> shift) & (uint32_t)(ND - 1);
     const uint32_t g = gbase[d] + (uint32_t)idx - lbase[d];
     kout[g] = kk;
     vout[g] = s_v[idx];
@@ -2849,22 +2875,30 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
   auto* k1 = (uint32_t*)ws(c, S_KEYS1, (n + 1) * 4);
   auto* v1 = (uint32_t*)ws(c, S_VALS1, (n + 1) * 4);
   const int64_t nblocks = (n + kRsItems - 1) / kRsItems;
-  const int64_t m = 256 * nblocks;
-  const int64_t npart = (m + kRsItems - 1) / kRsItems;
-  auto* hist = (uint32_t*)ws(c, S_HIST, m * 4);
-  auto* part = (uint32_t*)ws(c, S_PART, npart * 4);
+  const int64_t mmax = 512 * nblocks;  // histogram entries of a 9-bit pass
+  auto* hist = (uint32_t*)ws(c, S_HIST, mmax * 4);
+  auto* part = (uint32_t*)ws(c, S_PART, ((mmax + kRsItems - 1) / kRsItems) * 4);
   // the digit passes carry the bucket key packed into the sort words when key, digit and
   // index bits fit in 64; the first key pass unpacks it (no second pass over uvw)
   const int dbits = 2 * bits_for((uint64_t)P.tile - 1);
   const int ibits = std::max(1, bits_for((uint64_t)std::max<int64_t>(n, 1) - 1));
   const char* pe = std::getenv("HIPNG_SORT_PACK");  // 0: always recompute the keys (k_rekey)
   const bool pack = !(pe && std::atoi(pe) == 0) && dbits < 32 && ibits < 32 && P.keybits + dbits + ibits <= 64;
+  // `bits` bits from bit 0 in 8-bit passes, or in 9-bit passes where that saves a pass (a
+  // 9-bit pass while the remaining bits need more than 8 per remaining pass)
   auto passes = [&](int bits, int udb) {
     PROF("radix_sort");
-    for (int ps = 0; ps < (bits + 7) / 8; ++ps) {
-      const int shift = 8 * ps;
+    const int np = std::min((bits + 7) / 8, (bits + 8) / 9);
+    int shift = 0;
+    for (int ps = 0; ps < np; ++ps) {
+      const int wbits = bits - shift > 8 * (np - ps) ? 9 : 8;
       const int db = ps == 0 ? udb : 0;
-      k_rs_hist<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
+      const int64_t m = ((int64_t)1 << wbits) * nblocks;
+      const int64_t npart = (m + kRsItems - 1) / kRsItems;
+      if (wbits == 9)
+        k_rs_hist<9><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
+      else
+        k_rs_hist<8><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
       LAUNCH_CHECK();
       k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
       LAUNCH_CHECK();
@@ -2872,10 +2906,14 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
       LAUNCH_CHECK();
       k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
       LAUNCH_CHECK();
-      k_rs_scatter<<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
+      if (wbits == 9)
+        k_rs_scatter<9><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
+      else
+        k_rs_scatter<8><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
       LAUNCH_CHECK();
       std::swap(k0, k1);
       std::swap(v0, v1);
+      shift += wbits;
     }
   };
   // LSD: the in-tile origin digit first, then the bucket key of each item in that order
