@@ -47,7 +47,7 @@ def parse():
                     help="force the grid oversampling factor (cost-model calibration, not the bench line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy 4 times (counter calibration for rocprof --pmc)")
-    ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched"],
+    ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting"],
                     help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="also time the FP64 plain mode for this many steps")
@@ -516,6 +516,99 @@ def batched_study(njobs=300, seed=5):
     return out
 
 
+def weighting_study(reps=5, seed=9, cpu_fracs=(0.0025, 0.005)):
+    """Imaging weights (uniform and robust) at the production shape: 16,744,448 rows x 64
+    channels x 4 polarisations of FP64 imaging weights (seeded uniform 0.5-1.5), density on the
+    8192^2 uv grid of the 1.1e-5 rad image (one image channel). Inputs resident in HBM; each
+    timed call starts from the same weights (restored outside the timed region). Throughput in
+    Mvis/s (weights per second); HBM view of the whole call against its algorithmic bytes:
+    weights read by the density pass and read + written by the reweighting pass (24 B per
+    visibility), uvw read by both passes (48 B per row), the density grid cleared, summed and
+    read (3 x 8 B per cell). CPU baseline: the oracle's numpy restatement on two row
+    subsamples through the full grid, extrapolated linearly to all rows."""
+    import ctypes
+
+    import numpy
+    import torch
+
+    from rascil_amd import _lib, simulation
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    lib = _lib.load()
+    ntimes = 128
+    uvw, freq, _ = simulation.config1_geometry(ntimes=ntimes)
+    nrow, nchan, npol, N, cell = uvw.shape[0], len(freq), 4, 8192, 1.1e-5
+    nvis = nrow * nchan * npol
+    cell_u, cell_v = -1.0 / (N * cell), 1.0 / (N * cell)  # RA axis: negative cdelt
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    uvw_d = torch.from_numpy(numpy.ascontiguousarray(uvw)).to(dev)
+    wt0 = torch.rand((nrow, nchan, npol), dtype=torch.float64, device=dev, generator=g) + 0.5
+    wt = torch.empty_like(wt0)
+    grid = torch.empty((1, N, N, npol), dtype=torch.float64, device=dev)
+    v2i = numpy.zeros(nchan, numpy.int32)
+    fq = numpy.ascontiguousarray(freq, "f8")
+    st = torch.cuda.current_stream(dev)
+    out = {"study": "weighting", "nvis": int(nvis), "rows": int(nrow), "channels": nchan, "pols": npol,
+           "grid": [1, N, N, npol], "cell_rad": cell, "data": "synthetic"}
+
+    def call(mode, rob):
+        _lib.check(lib.hipng_weight_visibility(
+            ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(fq), ntimes, nrow // ntimes, nchan, npol,
+            v2i.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 1, N, N, cell_u, cell_v, mode, rob,
+            ctypes.c_void_p(wt.data_ptr()), ctypes.c_void_p(grid.data_ptr()), ctypes.c_void_p(st.cuda_stream)))
+
+    algo = 24.0 * nvis + 48.0 * nrow + 3 * 8.0 * grid.numel()
+    for name, mode, rob in (("uniform", 1, 0.0), ("robust", 2, 0.5)):
+        wt.copy_(wt0)
+        call(mode, rob)  # warm-up
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        ms = []
+        for _ in range(reps):
+            wt.copy_(wt0)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call(mode, rob)
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        prof = _lib.profile_read()
+        _lib.profile_enable(False)
+        t = float(numpy.median(ms))
+        res = {"ms_per_call": round(t, 3), "value": round(nvis / (t * 1e-3) / 1e6, 1), "unit": "Mvis/s",
+               "kernel_ms": {k: round(tot / cnt, 3) for k, (tot, cnt) in prof.items()},
+               "roofline": {"bound": "hbm", "achieved": round(algo / (t * 1e-3) / 1e9, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(algo / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes_per_call": int(algo)},
+               "sum_weights": float(wt.sum()), "zero_weights": int((wt == 0).sum())}
+        out[name] = res
+        print("[study] weighting", name, res, file=sys.stderr, flush=True)
+    # CPU baseline: the oracle on row subsamples (time = fixed grid cost + per-row cost)
+    from oracle import weighting as ow
+
+    rng = numpy.random.default_rng(seed)
+    pts = []
+    for f in cpu_fracs:
+        rows = numpy.sort(rng.choice(nrow, int(nrow * f), replace=False))
+        w = rng.uniform(0.5, 1.5, (len(rows), nchan, npol))
+        t0 = time.time()
+        ow.weight(uvw[rows], freq, w, v2i.astype(int), 1, N, N, numpy.degrees(-cell), numpy.degrees(cell),
+                  "uniform")
+        pts.append((len(rows), time.time() - t0))
+    (r1, t1), (r2, t2) = pts
+    per_row = (t2 - t1) / (r2 - r1)
+    t_full = t1 + per_row * (nrow - r1)
+    out["cpu_baseline"] = {"value": round(nvis / t_full / 1e6, 3), "unit": "Mvis/s", "cores": 1, "kind": "port",
+                           "sample": (f"numpy oracle (uniform) on {r1} and {r2} of {nrow} rows through the full "
+                                      f"8192^2 grid ({t1:.1f} s, {t2:.1f} s), extrapolated linearly to all rows")}
+    out["speedup_vs_cpu_uniform"] = round(out["uniform"]["value"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def copy_probe(nbytes=1 << 30, reps=4):
     """A plain device copy of known size: the rocprof FETCH_SIZE / WRITE_SIZE counters read on
     it give the scale factor between counter units and bytes moved."""
@@ -542,7 +635,8 @@ def main():
     if args.sigma:
         os.environ["HIPNG_SIGMA"] = str(args.sigma)
     if args.study:
-        fn = {"precision": precision_study, "autograd": autograd_study, "batched": batched_study}[args.study]
+        fn = {"precision": precision_study, "autograd": autograd_study, "batched": batched_study,
+              "weighting": weighting_study}[args.study]
         print(json.dumps(fn()), flush=True)
         return
     if args.gpus != 1:

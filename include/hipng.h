@@ -162,18 +162,20 @@ int32_t hipng_dft(const double* direction_cosines, const void* flux, const doubl
 /* Device: imaging weights, replacing weight_visibility
  * (rascil/processing_components/imaging/weighting.py:28-52) ->
  * grid_blockvisibility_weight_to_griddata + griddata_blockvisibility_reweight
- * (griddata/gridding.py:260-308, 336-424). Every (row, chan) falls on the nearest cell
+ * (griddata/gridding.py:260-308, 336-424). Rows are [ntimes][nbaselines] as in a
+ * BlockVisibility. Every (row, chan) falls on the nearest cell
  * pu = rint(u/cell_u + npix_x/2), pv = rint(v/cell_v + npix_y/2) of an image-sized grid
- * (cell_u = 1/(npix_x * cdelt_x), negative for RA); grid[imchan][pol][pv][pu] sums
- * imaging_weight [nrow][nchan][npol] (flagged entries already zero). weighting 1
- * (uniform): w <- w*w/g; 2 (robust): f2 = (5*10^-R)^2 sum(w)/sum(g^2),
- * w <- w*(1+f2 w)/(1+f2 g); g <= 0 -> 0. imaging_weight is updated in place, grid
- * [nimchan][npol][npix_y][npix_x] f64 is workspace/output. HIPNG_EINVAL if any visibility
- * falls outside the grid (the reference asserts). */
-int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
-                                int32_t npol, const int32_t* vis_to_im, int32_t nimchan, int32_t npix_x,
-                                int32_t npix_y, double cell_u, double cell_v, int32_t weighting,
-                                double robustness, double* imaging_weight, double* grid, void* stream);
+ * (cell_u = 1/(npix_x * cdelt_x), negative for RA); the grid sums imaging_weight
+ * [nrow][nchan][npol] (flagged entries already zero), npol <= 4. weighting 1 (uniform):
+ * w <- w*w/g; 2 (robust): f2 = (5*10^-R)^2 sum(w)/sum(g^2), w <- w*(1+f2 w)/(1+f2 g);
+ * g <= 0 -> 0. imaging_weight is updated in place; grid [nimchan][npix_y][npix_x][npol]
+ * f64 (polarisation innermost) is workspace and holds the density on return.
+ * HIPNG_EINVAL if any visibility falls outside the grid (the reference asserts). */
+int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t ntimes, int64_t nbaselines,
+                                int32_t nchan, int32_t npol, const int32_t* vis_to_im, int32_t nimchan,
+                                int32_t npix_x, int32_t npix_y, double cell_u, double cell_v,
+                                int32_t weighting, double robustness, double* imaging_weight, double* grid,
+                                void* stream);
 
 /* Diagnostics (no counterpart in the reference): per-kernel-class timing with HIP
  * events recorded on the launch stream around every kernel (and rocFFT execution).

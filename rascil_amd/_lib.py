@@ -133,7 +133,7 @@ def load():
         lib.hipng_dft.argtypes = [c_vp, c_vp, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp, c_vp]
         lib.hipng_dft.restype = c_i32
         lib.hipng_release.restype = c_i32
-        lib.hipng_weight_visibility.argtypes = [c_vp, pd, c_i64, c_i32, c_i32,
+        lib.hipng_weight_visibility.argtypes = [c_vp, pd, c_i64, c_i64, c_i32, c_i32,
                                                 ctypes.POINTER(ctypes.c_int32), c_i32, c_i32, c_i32,
                                                 c_dbl, c_dbl, c_i32, c_dbl, c_vp, c_vp, c_vp]
         lib.hipng_weight_visibility.restype = c_i32

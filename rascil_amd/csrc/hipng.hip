@@ -2479,12 +2479,18 @@ __global__ void k_unsort(const uint32_t* __restrict__ perm, int64_t n, const cpl
 
 // ---------------------------------------------------------------------------------
 // Imaging weights (uniform / robust): nearest-cell weight density on the image-sized uv
-// grid and the per-visibility reweighting of RASCIL's weight_visibility.
+// grid and the per-visibility reweighting of RASCIL's weight_visibility. Grid layout
+// [nimchan][ny][nx][npol]: the polarisations of a cell share one 32-byte segment.
 // ---------------------------------------------------------------------------------
+constexpr int kWgtMaxPol = 4;
+
 struct WgtArgs {
-  int64_t nrow;
+  int64_t ntimes, nbl;  // rows are [time][baseline]
   int nchan, npol, nimchan, nx, ny;
-  double du, dv;  // grid cell sizes in wavelengths (du negative: RA increases to the left)
+  double du, dv;        // grid cell sizes in wavelengths (du negative: RA increases to the left)
+  int ch_lanes;         // channels per wave (min(nchan, 64))
+  int bl_per_wave;      // baselines per wave (64 / ch_lanes)
+  int tseg;             // times per work unit of the density pass
 };
 
 __device__ __forceinline__ bool weight_cell(const double* __restrict__ uvw, int64_t row, double fc,
@@ -2495,44 +2501,87 @@ __device__ __forceinline__ bool weight_cell(const double* __restrict__ uvw, int6
   return *pu >= 0 && *pu < a.nx && *pv >= 0 && *pv < a.ny;
 }
 
-// grid[imchan][pol][pv][pu] += imaging_weight; out-of-grid visibilities raise flag[0]
-__global__ void k_weight_grid(const double* __restrict__ uvw, const double* __restrict__ fc,
-                              const int* __restrict__ vis_to_im, WgtArgs a,
-                              const double* __restrict__ wt, double* __restrict__ grid,
-                              unsigned* __restrict__ flag) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.nrow * a.nchan) return;
-  const int64_t row = i / a.nchan;
-  const int ch = (int)(i - row * a.nchan);
-  int pu, pv;
-  if (!weight_cell(uvw, row, fc[ch], a, &pu, &pv)) {
-    atomicOr(flag, 1u);
-    return;
+// Density grid[imchan][pv][pu][pol] += imaging_weight, and red[1] += sum(imaging_weight).
+// A lane holds one polarisation of one (baseline, channel): the npol lanes of a visibility
+// are adjacent, so their adds to a cell are one 32-byte segment of one instruction. A work
+// unit is (segment of tseg times, bl_per_wave baselines, ch_lanes channels); each lane
+// follows its (baseline, channel, pol) along the segment and sums the weights of
+// consecutive times that fall on the same cell in registers (a short baseline moves a
+// fraction of a cell per time step), adding them to the grid when the cell changes. The hot
+// cells of the short baselines thus see a fraction of the atomics of one add per
+// visibility. Persistent waves stride over the units; out-of-grid visibilities raise flag[0].
+__global__ __launch_bounds__(kBlock) void k_weight_grid(const double* __restrict__ uvw,
+                                                        const double* __restrict__ fc,
+                                                        const int* __restrict__ vis_to_im, WgtArgs a,
+                                                        const double* __restrict__ wt,
+                                                        double* __restrict__ grid, double* __restrict__ red,
+                                                        unsigned* __restrict__ flag, int64_t nunits) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+  const int np = a.npol;
+  const int isub = lane / np, p = lane - isub * np;
+  const int bsub = isub / a.ch_lanes, csub = isub - bsub * a.ch_lanes;
+  const int ncb = (a.nchan + a.ch_lanes - 1) / a.ch_lanes;
+  const int64_t nbg = (a.nbl + a.bl_per_wave - 1) / a.bl_per_wave;
+  double tot = 0.0;
+  bool bad = false;
+  for (int64_t u = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; u < nunits; u += nwaves) {
+    const int cb = (int)(u % ncb);
+    const int64_t r = u / ncb;
+    const int64_t bg = r % nbg, ts = r / nbg;
+    const int64_t b = bg * a.bl_per_wave + bsub;
+    const int c = cb * a.ch_lanes + csub;
+    if (bsub >= a.bl_per_wave || b >= a.nbl || c >= a.nchan) continue;
+    const double f = fc[c];
+    const int64_t icell = (int64_t)vis_to_im[c] * a.ny;
+    const int64_t t0 = ts * a.tseg, t1 = min(a.ntimes, t0 + a.tseg);
+    int64_t cur = -1;
+    double sum = 0.0;
+    for (int64_t t = t0; t < t1; ++t) {
+      const int64_t row = t * a.nbl + b;
+      int pu, pv;
+      if (!weight_cell(uvw, row, f, a, &pu, &pv)) {
+        bad = true;
+        continue;
+      }
+      const int64_t cell = (icell + pv) * a.nx + pu;
+      const double w = wt[(row * a.nchan + c) * np + p];
+      if (cell != cur) {
+        if (cur >= 0 && sum != 0.0) atomicAdd(&grid[cur * np + p], sum);
+        cur = cell;
+        sum = 0.0;
+      }
+      sum += w;
+      tot += w;
+    }
+    if (cur >= 0 && sum != 0.0) atomicAdd(&grid[cur * np + p], sum);
   }
-  const int ic = vis_to_im[ch];
-  for (int p = 0; p < a.npol; ++p) {
-    const double w = wt[i * a.npol + p];
-    if (w != 0.0) atomicAdd(&grid[(((int64_t)ic * a.npol + p) * a.ny + pv) * a.nx + pu], w);
-  }
+  // one add of the weight total per wave
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+  if (lane == 0 && tot != 0.0) atomicAdd(&red[1], tot);
+  if (bad) atomicOr(flag, 1u);
 }
 
+// Threads stride over the weights (row, chan, pol) in memory order (a version that walks the
+// density pass's time tracks for cell locality measured slower at the production shape:
+// fewer independent loads in flight).
 // mode 1: w <- w * w / g ; mode 2: w <- w * (1 + f2 w) / (1 + f2 g) ; g <= 0 -> 0
 __global__ void k_weight_apply(const double* __restrict__ uvw, const double* __restrict__ fc,
                                const int* __restrict__ vis_to_im, WgtArgs a,
                                const double* __restrict__ grid, int mode, double f2,
                                double* __restrict__ wt) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.nrow * a.nchan) return;
-  const int64_t row = i / a.nchan;
-  const int ch = (int)(i - row * a.nchan);
-  int pu, pv;
-  if (!weight_cell(uvw, row, fc[ch], a, &pu, &pv)) return;
-  const int ic = vis_to_im[ch];
-  for (int p = 0; p < a.npol; ++p) {
-    const double g = grid[(((int64_t)ic * a.npol + p) * a.ny + pv) * a.nx + pu];
-    double& w = wt[i * a.npol + p];
-    if (g > 0.0) w = mode == 1 ? w * (w / g) : w * ((1.0 + f2 * w) / (1.0 + f2 * g));
-    else w = 0.0;
+  const int per_row = a.nchan * a.npol;
+  const int64_t total = a.ntimes * a.nbl * per_row;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < total; j += (int64_t)gridDim.x * kBlock) {
+    const int64_t row = j / per_row;
+    const int rem = (int)(j - row * per_row);
+    const int ch = rem / a.npol, p = rem - ch * a.npol;
+    int pu, pv;
+    if (!weight_cell(uvw, row, fc[ch], a, &pu, &pv)) continue;
+    const double g = grid[(((int64_t)vis_to_im[ch] * a.ny + pv) * a.nx + pu) * a.npol + p];
+    const double w = wt[j];
+    wt[j] = g > 0.0 ? (mode == 1 ? w * (w / g) : w * ((1.0 + f2 * w) / (1.0 + f2 * g))) : 0.0;
   }
 }
 
@@ -3826,19 +3875,21 @@ int32_t hipng_profile_query(int32_t idx, char* name, int32_t name_len, double* t
   });
 }
 
-int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
-                                int32_t npol, const int32_t* vis_to_im, int32_t nimchan, int32_t npix_x,
-                                int32_t npix_y, double cell_u, double cell_v, int32_t weighting,
+int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t ntimes, int64_t nbaselines,
+                                int32_t nchan, int32_t npol, const int32_t* vis_to_im, int32_t nimchan,
+                                int32_t npix_x, int32_t npix_y, double cell_u, double cell_v, int32_t weighting,
                                 double robustness, double* imaging_weight, double* grid, void* stream) {
   return guarded([&] {
     if (!freq || !vis_to_im || !imaging_weight || !grid) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
-    if (nrow < 0 || nchan <= 0 || npol <= 0 || nimchan <= 0 || npix_x <= 0 || npix_y <= 0)
+    if (ntimes < 0 || nbaselines < 0 || nchan <= 0 || npol <= 0 || nimchan <= 0 || npix_x <= 0 || npix_y <= 0)
       HIPNG_THROW(HIPNG_EINVAL, "bad sizes");
+    if (npol > kWgtMaxPol) HIPNG_THROW(HIPNG_EINVAL, "npol %d > %d", npol, kWgtMaxPol);
     if (weighting != 1 && weighting != 2) HIPNG_THROW(HIPNG_EINVAL, "weighting must be 1 (uniform) or 2 (robust)");
     hipStream_t st = (hipStream_t)stream;
     DevCtx& c = ctx();
-    const int64_t n = nrow * (int64_t)nchan;
+    const int64_t n = ntimes * nbaselines * (int64_t)nchan;
     const int64_t ng = (int64_t)nimchan * npol * npix_x * npix_y;
+    if (n && !uvw) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
     std::vector<double> hfc(nchan);
     std::vector<int> hmap(nchan);
     for (int i = 0; i < nchan; ++i) {
@@ -3855,14 +3906,28 @@ int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t n
     k_zero<double><<<gridcap(ng), kBlock, 0, st>>>(grid, ng);
     k_zero<double><<<1, kBlock, 0, st>>>(red, 4);
     LAUNCH_CHECK();
-    WgtArgs a{nrow, nchan, npol, nimchan, npix_x, npix_y, cell_u, cell_v};
+    WgtArgs a{ntimes, nbaselines, nchan, npol, nimchan, npix_x, npix_y, cell_u, cell_v, 0, 0, 0};
+    a.ch_lanes = std::min(nchan, 64 / npol);  // visibilities per wave: 64 / npol
+    a.bl_per_wave = 64 / npol / a.ch_lanes;
+    const int64_t ncb = (nchan + a.ch_lanes - 1) / a.ch_lanes;
+    const int64_t nbg = (nbaselines + a.bl_per_wave - 1) / a.bl_per_wave;
+    // whole time tracks per work unit unless that leaves fewer than 64k units
+    int64_t tseg = std::max<int64_t>(ntimes, 1);
+    while (tseg > 1 && ncb * nbg * ((ntimes + tseg - 1) / tseg) < 65536) tseg = (tseg + 1) / 2;
+    a.tseg = (int)std::min<int64_t>(tseg, 1 << 30);
+    const int64_t nunits = ncb * nbg * ((ntimes + a.tseg - 1) / a.tseg);
     if (n) {
-      k_weight_grid<<<grid1(n), kBlock, 0, st>>>(uvw, dfc, dmap, a, imaging_weight, grid, (unsigned*)(red + 3));
+      PROF("weight_grid");
+      const int64_t blocks = std::min<int64_t>((nunits + 3) / 4, 2048);  // 4 waves per block
+      k_weight_grid<<<(unsigned)blocks, kBlock, 0, st>>>(uvw, dfc, dmap, a, imaging_weight, grid, red,
+                                                         (unsigned*)(red + 3), nunits);
       LAUNCH_CHECK();
     }
-    k_sum2<<<std::min<int64_t>(1024, gridcap(std::max(ng, n * npol))), kBlock, 0, st>>>(grid, ng, imaging_weight,
-                                                                                      n * npol, red);
-    LAUNCH_CHECK();
+    if (weighting == 2) {
+      PROF("weight_sums");
+      k_sum2<<<std::min<int64_t>(1024, gridcap(ng)), kBlock, 0, st>>>(grid, ng, nullptr, 0, red);
+      LAUNCH_CHECK();
+    }
     double h[4];
     HIP_TRY(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -3870,9 +3935,10 @@ int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t n
     std::memcpy(&fl, &h[3], sizeof(fl));
     if (fl) HIPNG_THROW(HIPNG_EINVAL, "visibilities fall outside the uv grid: cellsize is too large");
     // Briggs' robust parameter with RASCIL's normalisation (equation 3.15 of the thesis)
-    const double f2 = std::pow(5.0 * std::pow(10.0, -robustness), 2.0) * h[1] / h[0];
+    const double f2 = weighting == 2 ? std::pow(5.0 * std::pow(10.0, -robustness), 2.0) * h[1] / h[0] : 0.0;
     if (n) {
-      k_weight_apply<<<grid1(n), kBlock, 0, st>>>(uvw, dfc, dmap, a, grid, weighting, f2, imaging_weight);
+      PROF("weight_apply");
+      k_weight_apply<<<gridcap(n * npol), kBlock, 0, st>>>(uvw, dfc, dmap, a, grid, weighting, f2, imaging_weight);
       LAUNCH_CHECK();
     }
   });

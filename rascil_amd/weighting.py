@@ -38,10 +38,10 @@ def weight_visibility(vis, model, weighting="uniform", robustness=0.0, **kwargs)
     uvw = torch.from_numpy(numpy.ascontiguousarray(vis.uvw.reshape(-1, 3), "f8")).to(dev)
     wt = torch.from_numpy(numpy.ascontiguousarray(
         (vis.imaging_weight * (1 - vis.flags)).reshape(-1, nchan, npol), "f8")).to(dev)
-    grid = torch.empty((nimchan, npol, ny, nx), dtype=torch.float64, device=dev)
+    grid = torch.empty((nimchan, ny, nx, npol), dtype=torch.float64, device=dev)
     freq = numpy.ascontiguousarray(vis.frequency, "f8")
     _lib.check(lib.hipng_weight_visibility(
-        ctypes.c_void_p(uvw.data_ptr()), _lib.dptr(freq), nt * nb, nchan, npol,
+        ctypes.c_void_p(uvw.data_ptr()), _lib.dptr(freq), nt, nb, nchan, npol,
         vis_to_im.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), nimchan, nx, ny, cell_u, cell_v,
         1 if weighting == "uniform" else 2, float(robustness), ctypes.c_void_p(wt.data_ptr()),
         ctypes.c_void_p(grid.data_ptr()), ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)))
