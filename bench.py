@@ -47,7 +47,7 @@ def parse():
                     help="force the grid oversampling factor (cost-model calibration, not the bench line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy 4 times (counter calibration for rocprof --pmc)")
-    ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting"],
+    ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting", "gains"],
                     help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="also time the FP64 plain mode for this many steps")
@@ -609,6 +609,100 @@ def weighting_study(reps=5, seed=9, cpu_fracs=(0.0025, 0.005)):
     return out
 
 
+def gains_study(reps=5, seed=13, cpu_times=(1, 2)):
+    """Gain application (apply_gaintable) at the production shape: 128 times x 130,816
+    baselines x 64 channels of linear-polarisation complex128 visibilities (4.29e9, 68.6 GB)
+    with f64 weights, a per-channel (B) gain table of 2x2 Jones matrices per time, antenna
+    (512) and channel; forward and inverse. Inputs resident in HBM; the host's time matching
+    is outside the timed call. HBM view against the algorithmic bytes of a call: every
+    visibility read and written (128 B), weights untouched (2x2 gains, no flags), the gain
+    table read and its applied form written and read. CPU baseline: the numpy oracle on the
+    visibilities of one and two times, extrapolated linearly to all times."""
+    import ctypes
+
+    import numpy
+    import torch
+
+    from rascil_amd import _lib, simulation
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    lib = _lib.load()
+    ntimes, nant, nchan, npol = 128, 512, 64, 4
+    a1, a2 = simulation.baseline_indices(nant, autocorrelations=False)
+    nbl = len(a1)
+    nvis = ntimes * nbl * nchan * npol
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    vis = torch.randn((ntimes, nbl, nchan, npol), dtype=torch.complex128, device=dev, generator=g)
+    wt = torch.ones((ntimes, nbl, nchan, npol), dtype=torch.float64, device=dev)
+    rng = numpy.random.default_rng(seed)
+    gshape = (ntimes, nant, nchan, 2, 2)
+    gain = numpy.zeros(gshape, complex)
+    for k in range(2):
+        gain[..., k, k] = numpy.exp(1j * 0.1 * rng.normal(size=gshape[:3]))
+    gain[..., 0, 1] = 0.01 * rng.normal(size=gshape[:3])
+    gain[..., 1, 0] = 0.01 * rng.normal(size=gshape[:3])
+    gw = numpy.ones(gshape)
+    dg = torch.from_numpy(gain).to(dev)
+    dgw = torch.from_numpy(gw).to(dev)
+    d1 = torch.from_numpy(a1.astype(numpy.int32)).to(dev)
+    d2 = torch.from_numpy(a2.astype(numpy.int32)).to(dev)
+    t2g = torch.arange(ntimes, dtype=torch.int32, device=dev)
+    st = torch.cuda.current_stream(dev)
+    out = {"study": "gains", "nvis": int(nvis), "times": ntimes, "baselines": nbl, "channels": nchan,
+           "pols": npol, "jones": "B (2x2 per time, antenna, channel)", "data": "synthetic"}
+
+    def call(inverse):
+        _lib.check(lib.hipng_apply_gaintable(
+            ctypes.c_void_p(vis.data_ptr()), ctypes.c_void_p(wt.data_ptr()), None, None, ntimes, nbl, nchan, npol,
+            ctypes.c_void_p(d1.data_ptr()), ctypes.c_void_p(d2.data_ptr()), ctypes.c_void_p(dg.data_ptr()),
+            ctypes.c_void_p(dgw.data_ptr()), ntimes, nant, nchan, 2, ctypes.c_void_p(t2g.data_ptr()),
+            1 if inverse else 0, ctypes.c_void_p(st.cuda_stream)))
+
+    algo = 128.0 * (nvis // npol) + 3 * gain.nbytes + 4 * ntimes * nant * nchan
+    for name, inverse in (("forward", False), ("inverse", True)):
+        call(inverse)
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        ms = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call(inverse)
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        prof = _lib.profile_read()
+        _lib.profile_enable(False)
+        t = float(numpy.median(ms))
+        res = {"ms_per_call": round(t, 3), "value": round(nvis / (t * 1e-3) / 1e6, 1), "unit": "Mvis/s",
+               "kernel_ms": {k: round(tot / cnt, 3) for k, (tot, cnt) in prof.items()},
+               "roofline": {"bound": "hbm", "achieved": round(algo / (t * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(algo / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes_per_call": int(algo)}}
+        out[name] = res
+        print("[study] gains", name, res, file=sys.stderr, flush=True)
+    # CPU baseline: the oracle on the visibilities of the first one and two times
+    from oracle import gains as og
+
+    pts = []
+    vh = vis[:max(cpu_times)].cpu().numpy()
+    for k in cpu_times:
+        t0 = time.time()
+        og.apply_gaintable(vh[:k], numpy.ones(vh[:k].shape), numpy.zeros(vh[:k].shape), numpy.arange(k, dtype=float),
+                           a1, a2, gain[:k], gw[:k], numpy.arange(k, dtype=float), numpy.ones(k))
+        pts.append((k, time.time() - t0))
+    (k1, s1), (k2, s2) = pts
+    t_full = s1 + (s2 - s1) / (k2 - k1) * (ntimes - k1)
+    out["cpu_baseline"] = {"value": round(nvis / t_full / 1e6, 3), "unit": "Mvis/s", "cores": 1, "kind": "port",
+                           "sample": (f"numpy oracle (forward) on the visibilities of {k1} and {k2} of {ntimes} "
+                                      f"times ({s1:.1f} s, {s2:.1f} s), extrapolated linearly to all times")}
+    out["speedup_vs_cpu_forward"] = round(out["forward"]["value"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def copy_probe(nbytes=1 << 30, reps=4):
     """A plain device copy of known size: the rocprof FETCH_SIZE / WRITE_SIZE counters read on
     it give the scale factor between counter units and bytes moved."""
@@ -636,7 +730,7 @@ def main():
         os.environ["HIPNG_SIGMA"] = str(args.sigma)
     if args.study:
         fn = {"precision": precision_study, "autograd": autograd_study, "batched": batched_study,
-              "weighting": weighting_study}[args.study]
+              "weighting": weighting_study, "gains": gains_study}[args.study]
         print(json.dumps(fn()), flush=True)
         return
     if args.gpus != 1:

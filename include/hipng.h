@@ -177,6 +177,28 @@ int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t n
                                 int32_t weighting, double robustness, double* imaging_weight, double* grid,
                                 void* stream);
 
+/* Device: gain application, replacing apply_gaintable
+ * (rascil/processing_components/calibration/operations.py:37-274). vis complex128 and
+ * weight f64 [ntimes][nbaselines][nchan][npol] are updated in place; ant1/ant2 [nbaselines]
+ * index the antennas (0 <= a < nant); gain complex128 and gain_weight f64
+ * [ngain_times][nant][gain_nchan][nrec][nrec]; time_to_gain [ntimes] names the gain-table row
+ * applied to each visibility time (-1: none; the caller matches times within half the row's
+ * interval and calls once per match when a time matches several rows, in row order).
+ * Channels >= gain_nchan are left unchanged, as in the reference (a T or G table has one
+ * channel). With M the gain (inverse = 0) or its inverse: npol 1: V *= sum M1 conj(M2) and
+ * the weight becomes gain_weight[a1] * gain_weight[a2] (both 0 where that sum is 0; 1/g is
+ * 0 for g = 0); npol 2: V = diag(M1 diag(V) conj(M2)); npol 4: V = M1 V conj(M2) (conj
+ * elementwise, no transpose, as the reference). A singular 2x2 gain in inverse mode zeroes
+ * the visibility and weight (npol 4) or their first polarisation (npol 2). time_flagged
+ * [ntimes] (or NULL) with flags [like vis] (or NULL): where set, V and the weight are first
+ * multiplied by (1 - flags) (the reference's use_flags). All arrays are device memory. */
+int32_t hipng_apply_gaintable(void* vis, double* weight, const double* flags, const int8_t* time_flagged,
+                              int64_t ntimes, int32_t nbaselines, int32_t nchan, int32_t npol,
+                              const int32_t* ant1, const int32_t* ant2, const void* gain,
+                              const double* gain_weight, int32_t ngain_times, int32_t nant,
+                              int32_t gain_nchan, int32_t nrec, const int32_t* time_to_gain,
+                              int32_t inverse, void* stream);
+
 /* Diagnostics (no counterpart in the reference): per-kernel-class timing with HIP
  * events recorded on the launch stream around every kernel (and rocFFT execution).
  * hipng_profile_enable(1) resets and starts, (0) stops. hipng_profile_query returns

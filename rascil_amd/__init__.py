@@ -6,6 +6,8 @@ Public surface (mirrors the reference's names):
   wgridder.ms2dirty, wgridder.dirty2ms       (the ducc0.wgridder surface ng.py calls)
   dft_skycomponent_visibility                (imaging/dft.py)
   weight_visibility                          (imaging/weighting.py)
+  apply_gaintable, create_gaintable_from_blockvisibility, GainTable
+                                             (calibration/operations.py)
   wgridder.Geometry                          plan once / execute many (hipGraph capturable)
   autograd.predict                           differentiable predict (gridder = adjoint)
   BlockVisibility, Image, PolarisationFrame, Skycomponent, Direction
@@ -13,6 +15,7 @@ The compute runs in the hipng HIP library (rascil_amd/_hipng.so); nothing falls 
 the CPU.
 """
 
+from .calibration import GainTable, apply_gaintable, create_gaintable_from_blockvisibility
 from .data_models import (BlockVisibility, Direction, Image, PolarisationFrame, Skycomponent,
                           copy_visibility, create_image, get_parameter, image_is_canonical)
 from .dft import dft_skycomponent_visibility

@@ -33,6 +33,7 @@ EXPORTS = (
     "hipng_geometry_stats",
     "hipng_dft",
     "hipng_weight_visibility",
+    "hipng_apply_gaintable",
     "hipng_profile_enable",
     "hipng_profile_query",
     "hipng_release",
@@ -137,6 +138,9 @@ def load():
                                                 ctypes.POINTER(ctypes.c_int32), c_i32, c_i32, c_i32,
                                                 c_dbl, c_dbl, c_i32, c_dbl, c_vp, c_vp, c_vp]
         lib.hipng_weight_visibility.restype = c_i32
+        lib.hipng_apply_gaintable.argtypes = [c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp,
+                                              c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_i32, c_vp]
+        lib.hipng_apply_gaintable.restype = c_i32
         lib.hipng_profile_enable.argtypes = [c_i32]
         lib.hipng_profile_enable.restype = c_i32
         lib.hipng_profile_query.argtypes = [c_i32, ctypes.c_char_p, c_i32, pd,

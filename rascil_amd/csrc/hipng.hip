@@ -2642,6 +2642,160 @@ __global__ void k_dft(const double* __restrict__ dc, const cplx<double>* __restr
 }
 
 // ---------------------------------------------------------------------------------
+// Gain application: V' = M1 V conj(M2) per (time, baseline, channel), M the gain of the
+// baseline's antennas (or its inverse) for the gain-table row of the visibility's time.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ cplx<double> cmul(cplx<double> a, cplx<double> b) {
+  return cplx<double>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx<double> cmulc(cplx<double> a, cplx<double> b) {  // a * conj(b)
+  return cplx<double>{a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+__device__ __forceinline__ cplx<double> cadd(cplx<double> a, cplx<double> b) {
+  return cplx<double>{a.re + b.re, a.im + b.im};
+}
+__device__ __forceinline__ cplx<double> cdiv(cplx<double> a, cplx<double> b) {
+  // scaled division (Smith's method): no overflow for large |b|
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return cplx<double>{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.im + b.re * r;
+  return cplx<double>{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+
+// Per (gain row, antenna, channel): the matrix applied (nrec x nrec, row-major) and whether
+// it exists. Forward: the gain. Inverse, nrec = 1: 1/g, or 0 where g = 0. Inverse, nrec = 2:
+// the 2x2 inverse, or "no inverse" where the determinant is exactly zero.
+__global__ void k_gain_prep(const cplx<double>* __restrict__ gain, int64_t nent, int nrec, int inverse,
+                            cplx<double>* __restrict__ m, int* __restrict__ valid) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= nent) return;
+  const int n2 = nrec * nrec;
+  const cplx<double>* g = gain + e * n2;
+  cplx<double>* o = m + e * n2;
+  int ok = 1;
+  if (!inverse) {
+    for (int k = 0; k < n2; ++k) o[k] = g[k];
+  } else if (nrec == 1) {
+    const cplx<double> x = g[0];
+    o[0] = (x.re != 0.0 || x.im != 0.0) ? cdiv(cplx<double>{1.0, 0.0}, x) : cplx<double>{0.0, 0.0};
+  } else {
+    const cplx<double> a = g[0], b = g[1], c = g[2], d = g[3];
+    const cplx<double> ad = cmul(a, d), bc = cmul(b, c);
+    const cplx<double> det{ad.re - bc.re, ad.im - bc.im};
+    if (det.re == 0.0 && det.im == 0.0) {
+      ok = 0;
+      for (int k = 0; k < 4; ++k) o[k] = cplx<double>{0.0, 0.0};
+    } else {
+      o[0] = cdiv(d, det);
+      o[1] = cdiv(cplx<double>{-b.re, -b.im}, det);
+      o[2] = cdiv(cplx<double>{-c.re, -c.im}, det);
+      o[3] = cdiv(a, det);
+    }
+  }
+  valid[e] = ok;
+}
+
+struct GainArgs {
+  int64_t ntimes;
+  int nbl, nchan, npol, gnchan, nant, nrec, ngain;
+};
+
+// One thread per (time, baseline, channel) with a gain-table row; channels at or beyond the
+// table's channel count get only the flag mask below. npol 1: v *= sum_k m1[k] conj(m2[k]) and the
+// weight becomes the product of the antennas' gain weights (both zero where that sum is
+// zero). npol 2: the diagonal of M1 diag(v) conj(M2). npol 4: M1 [[v0 v1] [v2 v3]] conj(M2).
+// Without an inverse (inverse mode, singular 2x2 gain): npol 4 zeroes the visibility and its
+// weights, npol 2 its first polarisation. time_flagged[t] != 0: the visibility and weight are
+// multiplied by (1 - flags) first.
+__global__ void k_apply_gains(cplx<double>* __restrict__ vis, double* __restrict__ wt,
+                              const double* __restrict__ flags, const int8_t* __restrict__ time_flagged,
+                              const int* __restrict__ ant1, const int* __restrict__ ant2,
+                              const cplx<double>* __restrict__ m, const int* __restrict__ valid,
+                              const double* __restrict__ gw, const int* __restrict__ time_to_gain,
+                              GainArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ntimes * a.nbl * a.nchan) return;
+  const int64_t tb = i / a.nchan;
+  const int ch = (int)(i - tb * a.nchan);
+  const int64_t t = tb / a.nbl;
+  const int b = (int)(tb - t * a.nbl);
+  const int r = time_to_gain[t];
+  const int a1 = ant1[b], a2 = ant2[b];
+  // the host validates these; out-of-range entries are left unchanged rather than read past
+  if (r < 0 || r >= a.ngain || a1 < 0 || a1 >= a.nant || a2 < 0 || a2 >= a.nant) return;
+  const bool fl = time_flagged && time_flagged[t];
+  if (ch >= a.gnchan && !fl) return;
+  const int np = a.npol, n2 = a.nrec * a.nrec;
+  const int64_t e1 = ((int64_t)r * a.nant + a1) * a.gnchan + ch;
+  const int64_t e2 = ((int64_t)r * a.nant + a2) * a.gnchan + ch;
+  const cplx<double>* m1 = m + e1 * n2;
+  const cplx<double>* m2 = m + e2 * n2;
+  // weights are read only where they can change (flag mask, one polarisation, no inverse)
+  cplx<double> v[4];
+  double w[4] = {0.0, 0.0, 0.0, 0.0};
+  bool wdirty = false;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    if (p >= np) break;
+    v[p] = vis[i * np + p];
+    if (fl) {
+      const double k = 1.0 - flags[i * np + p];
+      v[p] = cplx<double>{v[p].re * k, v[p].im * k};
+      w[p] = wt[i * np + p] * k;
+    }
+  }
+  wdirty = fl;
+  if (ch >= a.gnchan) {
+    // beyond the table's channels: only the flag mask
+  } else if (np == 1) {
+    cplx<double> s{0.0, 0.0};
+    for (int k = 0; k < n2; ++k) s = cadd(s, cmulc(m1[k], m2[k]));
+    if (s.re != 0.0 || s.im != 0.0) {
+      v[0] = cmul(v[0], s);
+      w[0] = gw[e1 * n2] * gw[e2 * n2];
+    } else {
+      v[0] = cplx<double>{0.0, 0.0};
+      w[0] = 0.0;
+    }
+    wdirty = true;
+  } else if (!(valid[e1] && valid[e2])) {
+    if (!fl) {
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        if (p < np) w[p] = wt[i * np + p];
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (p >= (np == 4 ? 4 : 1)) break;
+      v[p] = cplx<double>{0.0, 0.0};
+      w[p] = 0.0;
+    }
+    wdirty = true;
+  } else if (np == 2) {
+    const cplx<double> x0 = v[0], x1 = v[1];
+    v[0] = cadd(cmulc(cmul(m1[0], x0), m2[0]), cmulc(cmul(m1[1], x1), m2[2]));
+    v[1] = cadd(cmulc(cmul(m1[2], x0), m2[1]), cmulc(cmul(m1[3], x1), m2[3]));
+  } else {
+    // T = M1 V, then T conj(M2)
+    cplx<double> tm[4];
+    for (int ii = 0; ii < 2; ++ii)
+      for (int l = 0; l < 2; ++l)
+        tm[2 * ii + l] = cadd(cmul(m1[2 * ii], v[l]), cmul(m1[2 * ii + 1], v[2 + l]));
+    for (int ii = 0; ii < 2; ++ii)
+      for (int k = 0; k < 2; ++k)
+        v[2 * ii + k] = cadd(cmulc(tm[2 * ii], m2[k]), cmulc(tm[2 * ii + 1], m2[2 + k]));
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    if (p >= np) break;
+    vis[i * np + p] = v[p];
+    if (wdirty) wt[i * np + p] = w[p];
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // Optional per-kernel timing with HIP events on the launch stream
 // ---------------------------------------------------------------------------------
 struct ProfRec {
@@ -2709,7 +2863,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_TASKTAB, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_NSLOTS
 };
 
 struct FftKey {
@@ -3849,6 +4003,43 @@ int32_t hipng_dft(const double* direction_cosines, const void* flux, const doubl
     hipStream_t st = (hipStream_t)stream;
     k_dft<<<grid1(nt), kBlock, 0, st>>>(direction_cosines, (const cplx<double>*)flux, uvw_lambda,
                                         ncomp, ntimes_baselines, nchan, npol, (cplx<double>*)vis);
+    LAUNCH_CHECK();
+  });
+}
+
+int32_t hipng_apply_gaintable(void* vis, double* weight, const double* flags, const int8_t* time_flagged,
+                              int64_t ntimes, int32_t nbaselines, int32_t nchan, int32_t npol,
+                              const int32_t* ant1, const int32_t* ant2, const void* gain,
+                              const double* gain_weight, int32_t ngain_times, int32_t nant,
+                              int32_t gain_nchan, int32_t nrec, const int32_t* time_to_gain,
+                              int32_t inverse, void* stream) {
+  return guarded([&] {
+    if (ntimes < 0 || nbaselines < 0 || nchan <= 0 || ngain_times < 0 || nant <= 0 || gain_nchan <= 0)
+      HIPNG_THROW(HIPNG_EINVAL, "bad sizes");
+    if (npol != 1 && npol != 2 && npol != 4) HIPNG_THROW(HIPNG_EINVAL, "npol must be 1, 2 or 4");
+    if (nrec != 1 && nrec != 2) HIPNG_THROW(HIPNG_EINVAL, "nrec must be 1 or 2");
+    if (npol > 1 && nrec != 2) HIPNG_THROW(HIPNG_EINVAL, "npol %d needs 2x2 gains", npol);
+    const int64_t n = ntimes * nbaselines * (int64_t)nchan;
+    if (n == 0 || ngain_times == 0) return;
+    if (!vis || !weight || !ant1 || !ant2 || !gain || !gain_weight || !time_to_gain)
+      HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    if (time_flagged && !flags) HIPNG_THROW(HIPNG_EINVAL, "time_flagged needs flags");
+    hipStream_t st = (hipStream_t)stream;
+    DevCtx& c = ctx();
+    const int64_t nent = (int64_t)ngain_times * nant * gain_nchan;
+    const int n2 = nrec * nrec;
+    auto* buf = (char*)ws(c, S_GAIN, nent * (n2 * sizeof(cplx<double>) + sizeof(int)) + 256);
+    auto* m = (cplx<double>*)buf;
+    auto* valid = (int*)(buf + nent * n2 * sizeof(cplx<double>));
+    {
+      PROF("gain_prep");
+      k_gain_prep<<<grid1(nent), kBlock, 0, st>>>((const cplx<double>*)gain, nent, nrec, inverse ? 1 : 0, m, valid);
+      LAUNCH_CHECK();
+    }
+    GainArgs a{ntimes, nbaselines, nchan, npol, gain_nchan, nant, nrec, ngain_times};
+    PROF("apply_gains");
+    k_apply_gains<<<grid1(n), kBlock, 0, st>>>((cplx<double>*)vis, weight, flags, time_flagged, ant1, ant2, m,
+                                               valid, gain_weight, time_to_gain, a);
     LAUNCH_CHECK();
   });
 }
