@@ -47,7 +47,8 @@ def parse():
                     help="force the grid oversampling factor (cost-model calibration, not the bench line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy 4 times (counter calibration for rocprof --pmc)")
-    ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting", "gains"],
+    ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting", "gains",
+                                                        "vis_passes"],
                     help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="also time the FP64 plain mode for this many steps")
@@ -703,6 +704,119 @@ def gains_study(reps=5, seed=13, cpu_times=(1, 2)):
     return out
 
 
+def vis_passes_study(reps=5, seed=17):
+    """The fused visibility passes of invert_ng / predict_ng at the production shape:
+    16,744,448 rows x 64 channels of linear-polarisation complex128 visibilities (4.29e9) with
+    int32 flags and f64 imaging weights; the image centre 0.5 deg from the phase centre, image
+    frame stokesIQUV; gridder precision mixed (complex64 / f32). Prologue: flag mask, phase
+    shift, linear -> IQUV, cast and layout for the gridder; epilogue: IQUV -> linear and the
+    inverse phase shift of complex64 degridder output. Inputs resident in HBM. Algorithmic
+    bytes per 4-pol item: prologue 64 (vis) + 16 (flags) + 32 (weights) read, 32 (ms) + 16
+    (weights) written; epilogue 32 read + 64 written; uvw 24 B per row. CPU baseline: the
+    oracle's numpy pieces (phasor, mask, polarisation matrix) on the visibilities of 65,536
+    rows."""
+    import ctypes
+
+    import numpy
+    import torch
+
+    from rascil_amd import _lib, simulation
+    from rascil_amd.data_models import Direction, PolarisationFrame, direction_to_lmn
+    from rascil_amd.ng import _conv
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    lib = _lib.load()
+    uvw, freq, pc = simulation.config1_geometry()
+    nrow, nchan, npol = uvw.shape[0], len(freq), 4
+    nvis = nrow * nchan * npol
+    l, m, nm1 = direction_to_lmn(Direction(pc.ra + math.radians(0.5) / math.cos(pc.dec), pc.dec), pc)
+    lmn = numpy.array([l, m, nm1])
+    conv_in, _ = _conv(PolarisationFrame("linear"), PolarisationFrame("stokesIQUV"))
+    conv_out, _ = _conv(PolarisationFrame("stokesIQUV"), PolarisationFrame("linear"))
+    fq = numpy.ascontiguousarray(freq, "f8")
+    pd = ctypes.POINTER(ctypes.c_double)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    uvw_d = torch.from_numpy(numpy.ascontiguousarray(uvw)).to(dev)
+    out = {"study": "vis_passes", "nvis": int(nvis), "rows": int(nrow), "channels": nchan, "pols": npol,
+           "frames": "linear visibilities, stokesIQUV image", "shift_deg": 0.5, "gridder_precision": "mixed",
+           "data": "synthetic"}
+
+    def timed(fn, name, nbytes):
+        fn()
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        ms = []
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        prof = _lib.profile_read()
+        _lib.profile_enable(False)
+        t = float(numpy.median(ms))
+        res = {"ms_per_call": round(t, 3), "value": round(nvis / (t * 1e-3) / 1e6, 1), "unit": "Mvis/s",
+               "kernel_ms": {k: round(tot / cnt, 3) for k, (tot, cnt) in prof.items()},
+               "roofline": {"bound": "hbm", "achieved": round(nbytes / (t * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(nbytes / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "algorithmic_bytes_per_call": int(nbytes)}}
+        out[name] = res
+        print("[study] vis_passes", name, res, file=sys.stderr, flush=True)
+
+    n = nrow * nchan
+    vis = torch.randn((nrow, nchan, npol), dtype=torch.complex128, device=dev, generator=g)
+    flags = (torch.rand((nrow, nchan, npol), device=dev, generator=g) < 0.05).to(torch.int32)
+    iwt = torch.rand((nrow, nchan, npol), dtype=torch.float64, device=dev, generator=g) + 0.5
+    ms = torch.empty((nrow, nchan, npol), dtype=torch.complex64, device=dev)
+    wgt = torch.empty((nrow, nchan, npol), dtype=torch.float32, device=dev)
+    nz = torch.empty((nchan, npol), dtype=torch.int32, device=dev)
+
+    def pro():
+        _lib.check(lib.hipng_vis_prologue(
+            ctypes.c_void_p(vis.data_ptr()), ctypes.c_void_p(flags.data_ptr()), ctypes.c_void_p(iwt.data_ptr()),
+            ctypes.c_void_p(uvw_d.data_ptr()), fq.ctypes.data_as(pd), nrow, nchan, npol, npol,
+            conv_in.ctypes.data_as(pd), lmn.ctypes.data_as(pd), _lib.PRECISIONS["mixed"],
+            ctypes.c_void_p(ms.data_ptr()), ctypes.c_void_p(wgt.data_ptr()), ctypes.c_void_p(nz.data_ptr()), st))
+
+    timed(pro, "prologue", 160.0 * n + 24.0 * nrow)
+    vis_h = vis[:65536].cpu().numpy()
+    flags_h = flags[:65536].cpu().numpy()
+    del flags, iwt, wgt, nz
+    torch.cuda.empty_cache()
+
+    def epi():
+        _lib.check(lib.hipng_vis_epilogue(
+            ctypes.c_void_p(ms.data_ptr()), ctypes.c_void_p(uvw_d.data_ptr()), fq.ctypes.data_as(pd), nrow, nchan,
+            npol, npol, conv_out.ctypes.data_as(pd), lmn.ctypes.data_as(pd), _lib.PRECISIONS["mixed"],
+            ctypes.c_void_p(vis.data_ptr()), st))
+
+    timed(epi, "epilogue", 96.0 * n + 24.0 * nrow)
+    # CPU baseline: the oracle's pieces on 8,192 rows (prologue then epilogue)
+    from oracle import explicit as ox
+
+    rows = 65536
+    ul = uvw[:rows, None, :] * (freq / 299792458.0)[None, :, None]
+    t0 = time.time()
+    ph = ox.phasor(ul, l, m, nm1)
+    x = ox.convert_pol(vis_h * (1 - flags_h) * numpy.conj(ph)[..., None], "linear", "stokesIQUV", polaxis=-1)
+    x.astype(numpy.complex64)
+    ox.convert_pol(x.astype(numpy.complex64).astype(complex), "stokesIQUV", "linear", polaxis=-1) * ph[..., None]
+    dt = time.time() - t0
+    cpu = rows * nchan * npol / dt / 1e6
+    out["cpu_baseline"] = {"value": round(cpu, 2), "unit": "Mvis/s", "cores": 1, "kind": "port",
+                           "sample": f"numpy oracle pieces (prologue + epilogue) on {rows} rows x {nchan} ch x "
+                                     f"{npol} pol ({dt:.2f} s)"}
+    gpu = nvis / ((out["prologue"]["ms_per_call"] + out["epilogue"]["ms_per_call"]) * 1e-3) / 1e6
+    out["prologue_plus_epilogue_mvis_per_s"] = round(gpu, 1)
+    out["speedup_vs_cpu"] = round(gpu / cpu, 1)
+    return out
+
+
 def copy_probe(nbytes=1 << 30, reps=4):
     """A plain device copy of known size: the rocprof FETCH_SIZE / WRITE_SIZE counters read on
     it give the scale factor between counter units and bytes moved."""
@@ -730,7 +844,7 @@ def main():
         os.environ["HIPNG_SIGMA"] = str(args.sigma)
     if args.study:
         fn = {"precision": precision_study, "autograd": autograd_study, "batched": batched_study,
-              "weighting": weighting_study, "gains": gains_study}[args.study]
+              "weighting": weighting_study, "gains": gains_study, "vis_passes": vis_passes_study}[args.study]
         print(json.dumps(fn()), flush=True)
         return
     if args.gpus != 1:

@@ -199,6 +199,32 @@ int32_t hipng_apply_gaintable(void* vis, double* weight, const double* flags, co
                               int32_t gain_nchan, int32_t nrec, const int32_t* time_to_gain,
                               int32_t inverse, void* stream);
 
+/* Device: the visibility passes of invert_ng before the gridder, fused
+ * (rascil/processing_components/imaging/ng.py:169-181 -> shift_vis_to_image
+ * imaging/base.py:67-106 with phaserotate_visibility visibility/base.py:296-357,
+ * flagged_vis / flagged_imaging_weight data_models/memory_data_models.py:1502-1524,
+ * convert_pol_frame data_models/polarisation.py:164-419). Per (row, chan):
+ * x = vis (1 - flags) e^{+2 pi i f/c (u l + v m + w nm1)} (no phase term when lmn is NULL),
+ * ms = conv x with conv [npol_image][npol_vis] complex (interleaved re, im; host),
+ * wgt[q] = imaging_weight[q] (1 - flags[q]) for q < npol_image; ms / wgt in the gridder's
+ * precision (HIPNG_FP64: complex128 / f64, else complex64 / f32), layout [nrow][nchan][npol_image];
+ * nonzero [nchan][npol_image] int32 = 1 where some ms of the channel is not 0 (the
+ * reference skips all-zero polarisations). vis complex128, flags int32 (or NULL) and
+ * imaging_weight f64 [nrow][nchan][npol_vis], uvw f64 [nrow][3]: device; freq, conv, lmn: host. */
+int32_t hipng_vis_prologue(const void* vis, const int32_t* flags, const double* imaging_weight, const double* uvw,
+                           const double* freq, int64_t nrow, int32_t nchan, int32_t npol_vis, int32_t npol_image,
+                           const double* conv, const double* lmn, int32_t precision, void* ms, void* wgt,
+                           int32_t* nonzero, void* stream);
+
+/* Device: the visibility passes of predict_ng after the degridder, fused (ng.py:122-133 ->
+ * convert_pol_frame, then shift_vis_to_image(inverse=True)): vis = (conv vist)
+ * e^{-2 pi i f/c (u l + v m + w nm1)} (no phase term when lmn is NULL), conv
+ * [npol_vis][npol_image] complex (host); vist [nrow][nchan][npol_image] in the gridder's
+ * precision, vis complex128 [nrow][nchan][npol_vis]. */
+int32_t hipng_vis_epilogue(const void* vist, const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                           int32_t npol_image, int32_t npol_vis, const double* conv, const double* lmn,
+                           int32_t precision, void* vis, void* stream);
+
 /* Diagnostics (no counterpart in the reference): per-kernel-class timing with HIP
  * events recorded on the launch stream around every kernel (and rocFFT execution).
  * hipng_profile_enable(1) resets and starts, (0) stops. hipng_profile_query returns

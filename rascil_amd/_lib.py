@@ -34,6 +34,8 @@ EXPORTS = (
     "hipng_dft",
     "hipng_weight_visibility",
     "hipng_apply_gaintable",
+    "hipng_vis_prologue",
+    "hipng_vis_epilogue",
     "hipng_profile_enable",
     "hipng_profile_query",
     "hipng_release",
@@ -141,6 +143,11 @@ def load():
         lib.hipng_apply_gaintable.argtypes = [c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp,
                                               c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_i32, c_vp]
         lib.hipng_apply_gaintable.restype = c_i32
+        lib.hipng_vis_prologue.argtypes = [c_vp, c_vp, c_vp, c_vp, pd, c_i64, c_i32, c_i32, c_i32, pd, pd, c_i32,
+                                           c_vp, c_vp, c_vp, c_vp]
+        lib.hipng_vis_prologue.restype = c_i32
+        lib.hipng_vis_epilogue.argtypes = [c_vp, c_vp, pd, c_i64, c_i32, c_i32, c_i32, pd, pd, c_i32, c_vp, c_vp]
+        lib.hipng_vis_epilogue.restype = c_i32
         lib.hipng_profile_enable.argtypes = [c_i32]
         lib.hipng_profile_enable.restype = c_i32
         lib.hipng_profile_query.argtypes = [c_i32, ctypes.c_char_p, c_i32, pd,

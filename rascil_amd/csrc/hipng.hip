@@ -2796,6 +2796,100 @@ __global__ void k_apply_gains(cplx<double>* __restrict__ vis, double* __restrict
 }
 
 // ---------------------------------------------------------------------------------
+// Visibility prologue / epilogue of invert_ng / predict_ng: flag mask, phase shift to the
+// image centre, polarisation conversion and the gridder's layout and precision, in one pass.
+// ---------------------------------------------------------------------------------
+struct PolArgs {
+  int64_t nrow;
+  int nchan, nin, nout, shift;
+  double l, m, nm1;             // direction cosines of the image centre seen from the vis centre
+  double cre[16], cim[16];      // conversion [nout][nin], row-major
+};
+
+__device__ __forceinline__ void to_c(cplx<double> v, cplx<float>* o) { *o = cplx<float>{(float)v.re, (float)v.im}; }
+__device__ __forceinline__ void to_c(cplx<double> v, cplx<double>* o) { *o = v; }
+__device__ __forceinline__ cplx<double> from_c(cplx<float> v) { return cplx<double>{v.re, v.im}; }
+__device__ __forceinline__ cplx<double> from_c(cplx<double> v) { return v; }
+
+// ph = f/c (u l + v m + w (n-1)) of item (row, channel)
+__device__ __forceinline__ double shift_phase(const double* __restrict__ uvw, int64_t row, double fc,
+                                              const PolArgs& a) {
+  return (uvw[3 * row] * fc) * a.l + (uvw[3 * row + 1] * fc) * a.m + (uvw[3 * row + 2] * fc) * a.nm1;
+}
+
+// invert: x_p = vis_p (1 - flags_p) e^{+2 pi i ph}; ms_q = sum_p C[q][p] x_p; wgt_q =
+// imaging_weight_q (1 - flags_q); nonzero[chan][q] = 1 where some ms_q of the channel is not 0
+template <typename CT, typename RT>
+__global__ void k_vis_prologue(const cplx<double>* __restrict__ vis, const int* __restrict__ flags,
+                               const double* __restrict__ iwt, const double* __restrict__ uvw,
+                               const double* __restrict__ fc, PolArgs a, CT* __restrict__ ms,
+                               RT* __restrict__ wgt, int* __restrict__ nonzero) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nrow * a.nchan) return;
+  const int64_t row = i / a.nchan;
+  const int ch = (int)(i - row * a.nchan);
+  const int ni = a.nin, no = a.nout;
+  double k[4] = {1.0, 1.0, 1.0, 1.0};
+  cplx<double> x[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    if (p >= ni) break;
+    if (flags) k[p] = 1.0 - (double)flags[i * ni + p];
+    const cplx<double> v = vis[i * ni + p];
+    x[p] = cplx<double>{v.re * k[p], v.im * k[p]};
+  }
+  if (a.shift) {
+    double sn, cs;
+    sincospi(2.0 * shift_phase(uvw, row, fc[ch], a), &sn, &cs);
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < ni) x[p] = cmul(x[p], cplx<double>{cs, sn});
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q >= no) break;
+    cplx<double> y{0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < ni) y = cadd(y, cmul(cplx<double>{a.cre[q * ni + p], a.cim[q * ni + p]}, x[p]));
+    to_c(y, &ms[i * no + q]);
+    int* nz = &nonzero[ch * no + q];
+    if ((y.re != 0.0 || y.im != 0.0) && *nz == 0) *nz = 1;
+    wgt[i * no + q] = (RT)(iwt[i * ni + q] * k[q]);
+  }
+}
+
+// predict: y_p = sum_q C[p][q] vist_q, times e^{-2 pi i ph} when shifting
+template <typename CT>
+__global__ void k_vis_epilogue(const CT* __restrict__ vist, const double* __restrict__ uvw,
+                               const double* __restrict__ fc, PolArgs a, cplx<double>* __restrict__ vis) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nrow * a.nchan) return;
+  const int64_t row = i / a.nchan;
+  const int ch = (int)(i - row * a.nchan);
+  const int ni = a.nin, no = a.nout;
+  cplx<double> x[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (q < ni) x[q] = from_c(vist[i * ni + q]);
+  cplx<double> ph{1.0, 0.0};
+  if (a.shift) {
+    double sn, cs;
+    sincospi(-2.0 * shift_phase(uvw, row, fc[ch], a), &sn, &cs);
+    ph = cplx<double>{cs, sn};
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    if (p >= no) break;
+    cplx<double> y{0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < ni) y = cadd(y, cmul(cplx<double>{a.cre[p * ni + q], a.cim[p * ni + q]}, x[q]));
+    vis[i * no + p] = a.shift ? cmul(y, ph) : y;
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // Optional per-kernel timing with HIP events on the launch stream
 // ---------------------------------------------------------------------------------
 struct ProfRec {
@@ -3841,6 +3935,38 @@ int32_t guarded(F&& f) {
 // =================================================================================
 // C-ABI
 // =================================================================================
+namespace {
+PolArgs pol_args(int64_t nrow, int32_t nchan, int32_t nin, int32_t nout, const double* conv, const double* lmn) {
+  if (nrow < 0 || nchan <= 0 || nin < 1 || nin > 4 || nout < 1 || nout > 4) HIPNG_THROW(HIPNG_EINVAL, "bad sizes");
+  if (!conv) HIPNG_THROW(HIPNG_EINVAL, "NULL conversion matrix");
+  PolArgs a{};
+  a.nrow = nrow;
+  a.nchan = nchan;
+  a.nin = nin;
+  a.nout = nout;
+  for (int k = 0; k < nin * nout; ++k) {
+    a.cre[k] = conv[2 * k];
+    a.cim[k] = conv[2 * k + 1];
+  }
+  a.shift = lmn ? 1 : 0;
+  if (lmn) {
+    a.l = lmn[0];
+    a.m = lmn[1];
+    a.nm1 = lmn[2];
+  }
+  return a;
+}
+
+double* upload_fc(DevCtx& c, const double* freq, int32_t nchan, hipStream_t st) {
+  if (!freq) HIPNG_THROW(HIPNG_EINVAL, "NULL freq");
+  std::vector<double> h(nchan);
+  for (int i = 0; i < nchan; ++i) h[i] = freq[i] / kSpeedOfLight;
+  auto* d = (double*)ws(c, S_SMALL, nchan * sizeof(double) + 64);
+  HIP_TRY(hipMemcpyAsync(d, h.data(), nchan * sizeof(double), hipMemcpyHostToDevice, st));
+  return d;
+}
+}  // namespace
+
 extern "C" {
 
 int32_t hipng_version(void) { return 100; }
@@ -4040,6 +4166,54 @@ int32_t hipng_apply_gaintable(void* vis, double* weight, const double* flags, co
     PROF("apply_gains");
     k_apply_gains<<<grid1(n), kBlock, 0, st>>>((cplx<double>*)vis, weight, flags, time_flagged, ant1, ant2, m,
                                                valid, gain_weight, time_to_gain, a);
+    LAUNCH_CHECK();
+  });
+}
+
+int32_t hipng_vis_prologue(const void* vis, const int32_t* flags, const double* imaging_weight, const double* uvw,
+                           const double* freq, int64_t nrow, int32_t nchan, int32_t npol_vis, int32_t npol_image,
+                           const double* conv, const double* lmn, int32_t precision, void* ms, void* wgt,
+                           int32_t* nonzero, void* stream) {
+  return guarded([&] {
+    const PolArgs a = pol_args(nrow, nchan, npol_vis, npol_image, conv, lmn);
+    if (npol_image > npol_vis) HIPNG_THROW(HIPNG_EINVAL, "more image than visibility polarisations");
+    if (!vis || !imaging_weight || !ms || !wgt || !nonzero || (lmn && !uvw)) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    DevCtx& c = ctx();
+    const double* fc = upload_fc(c, freq, nchan, st);
+    k_zero<int><<<1, kBlock, 0, st>>>(nonzero, (int64_t)nchan * npol_image);
+    LAUNCH_CHECK();
+    const int64_t n = nrow * (int64_t)nchan;
+    if (n == 0) return;
+    PROF("vis_prologue");
+    if (precision == HIPNG_FP64)
+      k_vis_prologue<cplx<double>, double><<<grid1(n), kBlock, 0, st>>>(
+          (const cplx<double>*)vis, flags, imaging_weight, uvw, fc, a, (cplx<double>*)ms, (double*)wgt, nonzero);
+    else
+      k_vis_prologue<cplx<float>, float><<<grid1(n), kBlock, 0, st>>>(
+          (const cplx<double>*)vis, flags, imaging_weight, uvw, fc, a, (cplx<float>*)ms, (float*)wgt, nonzero);
+    LAUNCH_CHECK();
+  });
+}
+
+int32_t hipng_vis_epilogue(const void* vist, const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
+                           int32_t npol_image, int32_t npol_vis, const double* conv, const double* lmn,
+                           int32_t precision, void* vis, void* stream) {
+  return guarded([&] {
+    const PolArgs a = pol_args(nrow, nchan, npol_image, npol_vis, conv, lmn);
+    if (!vist || !vis || (lmn && !uvw)) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    DevCtx& c = ctx();
+    const double* fc = upload_fc(c, freq, nchan, st);
+    const int64_t n = nrow * (int64_t)nchan;
+    if (n == 0) return;
+    PROF("vis_epilogue");
+    if (precision == HIPNG_FP64)
+      k_vis_epilogue<cplx<double>><<<grid1(n), kBlock, 0, st>>>((const cplx<double>*)vist, uvw, fc, a,
+                                                                 (cplx<double>*)vis);
+    else
+      k_vis_epilogue<cplx<float>><<<grid1(n), kBlock, 0, st>>>((const cplx<float>*)vist, uvw, fc, a,
+                                                                (cplx<double>*)vis);
     LAUNCH_CHECK();
   });
 }
