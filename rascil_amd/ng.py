@@ -6,7 +6,9 @@ do_wstacking, verbosity), conventions (u and w negated before the gridder, image
 transposed to [x, y], MFS when the image has one channel, per-channel otherwise via the
 WCS frequency axis) and results. The ducc0.wgridder calls are replaced by
 rascil_amd.wgridder, which runs on the GPU and processes all polarisations of a call in
-one batched pass instead of one call per polarisation.
+one batched pass instead of one call per polarisation. The visibility passes around the
+gridder (phase shift to the image centre, flag mask, polarisation conversion, precision
+and layout) run as one fused GPU pass each way (hipng_vis_prologue / hipng_vis_epilogue).
 
 Extra keyword: precision = "fp64" (default; the reference computes in FP64 with FP64
 accumulation), "mixed" or "fp32".
