@@ -154,3 +154,26 @@ def test_package_does_not_import_oracle():
             if f.endswith(".py"):
                 src = open(os.path.join(dirpath, f)).read()
                 assert not re.search(r"^\s*(from|import)\s+oracle", src, re.M), f
+
+
+def test_copy_visibility_is_deep_and_zeroes_on_request():
+    """visibility/base.py:105-118: a deep copy; zero=True clears vis only."""
+    import math
+
+    from rascil_amd import Direction, PolarisationFrame, copy_visibility, simulation
+
+    rng = numpy.random.default_rng(2)
+    ants = simulation.gaussian_disc_stations(6, 100.0, 300.0, rng)
+    bvis = simulation.create_blockvisibility(ants, numpy.array([0.0, 0.1]), [1e8, 1.1e8],
+                                             Direction(0.0, math.radians(-30)), PolarisationFrame("linear"))
+    bvis.vis = rng.normal(size=bvis.vis.shape) + 1j * rng.normal(size=bvis.vis.shape)
+    c = copy_visibility(bvis)
+    numpy.testing.assert_array_equal(c.vis, bvis.vis)
+    c.vis[0, 0, 0, 0] = 99.0
+    c.weight[...] = 7.0
+    assert bvis.vis[0, 0, 0, 0] != 99.0 and not numpy.any(bvis.weight == 7.0)
+    z = copy_visibility(bvis, zero=True)
+    assert not z.vis.any()
+    numpy.testing.assert_array_equal(z.weight, bvis.weight)
+    numpy.testing.assert_array_equal(z.uvw, bvis.uvw)
+    assert bvis.vis.any()
