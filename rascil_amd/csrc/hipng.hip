@@ -3446,7 +3446,9 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     }
   }
   g.tbuf = g.buf.get(S_TBUF, (size_t)g.pc * nb * P.ny * P.gx * csz);
-  g.tbuf2 = g.buf.get(S_TBUF2, (size_t)g.pc * nb * P.ny * P.gx * csz);
+  // FP32 planes: room for two passes, so that invert screens both into the FP64 image
+  // accumulator with one read-modify-write
+  g.tbuf2 = g.buf.get(S_TBUF2, (size_t)(csz == 8 ? 2 : 1) * g.pc * nb * P.ny * P.gx * csz);
   g.fft_b_inv = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 0);
   g.fft_b_fwd = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1);
   g.fft_t_inv = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 0, false);
@@ -3730,6 +3732,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   // FP64 (whose vector accumulators would not leave room for a second sub-tile)
   constexpr int RG = grid8_rows<R>();
   if (smw > kLdsBudget) HIPNG_THROW(HIPNG_ERUNTIME, "gridder LDS %zu exceeds %zu", smw, kLdsBudget);
+  const bool pair_screen = sizeof(R) == 4;  // T2 holds two passes of FP32 planes
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
     {
@@ -3784,13 +3787,19 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
           k, g.xlo, g.xhi, xb, plane, tbuf);
       LAUNCH_CHECK();
     }
+    // FP32 planes of an even pass wait in the first half of T2 for the next pass's, and the
+    // screen adds both to the accumulator at once (the last pass flushes what is held)
+    const int half = pair_screen ? (p0 / PCn) & 1 : 0;
     {
       PROF("fft_c2c_inverse");
-      geom_fft(g, g.fft_t_inv, tbuf, st, tbuf2);
+      geom_fft(g, g.fft_t_inv, tbuf, st, tbuf2 + (int64_t)half * PCn * g.nb * P.ny * P.gx);
     }
-    PROF("screen_acc");
-    k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, p0, npl, tbuf2, acc);
-    LAUNCH_CHECK();
+    if (!pair_screen || half == 1 || p0 + npl >= P.nplanes) {
+      PROF("screen_acc");
+      const int s0 = half == 1 ? p0 - PCn : p0;
+      k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, s0, p0 + npl - s0, tbuf2, acc);
+      LAUNCH_CHECK();
+    }
   }
   PROF("finalize");
   k_finalize_t<A, O><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
