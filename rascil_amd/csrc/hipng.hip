@@ -3827,29 +3827,36 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   auto* plane = (cplx<R>*)g.plane;
   auto* tbuf = (cplx<R>*)g.tbuf;
   auto* tbuf2 = (cplx<R>*)g.tbuf2;
+  // The screens fill the image part of S (cleared once per call) and the transforms along
+  // the grid rows go out of place into D. FP32 planes: S = T2, which holds two passes, so one
+  // read of the image fills the planes of an even pass and the next; FP64: S = T.
+  const bool pair_fill = sizeof(R) == 4;
+  cplx<R>* sbuf = pair_fill ? tbuf2 : tbuf;
+  cplx<R>* dbuf = pair_fill ? tbuf : tbuf2;
   {
-    // the per-pass kernels write only the band (invert) or the image (predict) part of T
     PROF("transpose");
-    const int64_t nt = (int64_t)g.pc * g.nb * P.ny * P.gx;
-    k_zero<cplx<R>><<<gridcap(nt), kBlock, 0, st>>>(tbuf, nt);
+    const int64_t nt = (int64_t)(pair_fill ? 2 : 1) * g.pc * g.nb * P.ny * P.gx;
+    k_zero<cplx<R>><<<gridcap(nt), kBlock, 0, st>>>(sbuf, nt);
     LAUNCH_CHECK();
   }
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
     const int npl = std::min(g.pc, P.nplanes - p0);
-    {
+    const int half = pair_fill ? (p0 / g.pc) & 1 : 0;
+    if (half == 0) {
       PROF("screen_fill");
-      k_screen_fill_t<R, A, F32C><<<dim3(grid1(P.gx), P.ny), kBlock, 0, st>>>(k, p0, npl, dc, tbuf);
+      const int nfill = std::min((pair_fill ? 2 : 1) * g.pc, P.nplanes - p0);
+      k_screen_fill_t<R, A, F32C><<<dim3(grid1(P.gx), P.ny), kBlock, 0, st>>>(k, p0, nfill, dc, sbuf);
       LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_forward");
-      geom_fft(g, g.fft_t_fwd, tbuf, st, tbuf2);
+      geom_fft(g, g.fft_t_fwd, sbuf + (int64_t)half * g.pc * g.nb * P.ny * P.gx, st, dbuf);
     }
     {
       PROF("transpose");
       k_tr_out<R><<<dim3((g.xhi - g.xlo + 31) / 32, (P.gy + 31) / 32, npl * g.nb), kBlock, 0, st>>>(
-          k, g.xlo, g.xhi, tbuf2, plane);
+          k, g.xlo, g.xhi, dbuf, plane);
       LAUNCH_CHECK();
     }
     {
