@@ -2003,8 +2003,9 @@ __global__ __launch_bounds__(kBlock) void k_tr_in(KP k, int xlo, int xhi, int xb
   }
 }
 
-// band rows: plane[i][x][c] = T[i][img(c)][x] for image columns c, 0 for the others
-template <typename R>
+// band rows: plane[i][x][c] = T[i][img(c)][x] for image columns c, and 0 for the others when
+// ZERO (without it those columns are left as they are)
+template <typename R, bool ZERO>
 __global__ __launch_bounds__(kBlock) void k_tr_out(KP k, int xlo, int xhi, const cplx<R>* __restrict__ tb,
                                                    cplx<R>* __restrict__ plane) {
   __shared__ cplx<R> t[32][33];
@@ -2021,9 +2022,10 @@ __global__ __launch_bounds__(kBlock) void k_tr_out(KP k, int xlo, int xhi, const
     t[r][lx] = v;
   }
   __syncthreads();
+  const bool img = ZERO || grid_col_to_img(c0 + lx, k.ny, k.gy) >= 0;
   for (int r = ly; r < 32; r += 8) {
     const int x = x0 + r, c = c0 + lx;
-    if (x < xhi && c < k.gy) plane[((int64_t)i * k.gx + x) * k.gy + c] = t[lx][r];
+    if (x < xhi && c < k.gy && img) plane[((int64_t)i * k.gx + x) * k.gy + c] = t[lx][r];
   }
 }
 
@@ -2957,7 +2959,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_TASKTAB, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_PLANE2, S_NSLOTS
 };
 
 struct FftKey {
@@ -3272,6 +3274,10 @@ struct Geom {
   size_t fftwork_size = 0;
   // pruned plane transforms: grid rows [xlo, xhi) (band, along gy), image side T (along gx)
   rocfft_plan fft_b_inv = nullptr, fft_b_fwd = nullptr, fft_t_inv = nullptr, fft_t_fwd = nullptr;
+  // FP32 planes: predict's band transforms go out of place into plane2, so the non-image
+  // columns of the band rows in plane stay zero through a call
+  void* plane2 = nullptr;
+  rocfft_plan fft_b_fwd_oop = nullptr;
   int xlo = 0, xhi = 0;
   void* tbuf = nullptr;   // transposed band / image rows, zero where no kernel writes (cleared per call)
   void* tbuf2 = nullptr;  // its transforms along the grid rows
@@ -3453,8 +3459,13 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   g.fft_b_fwd = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1);
   g.fft_t_inv = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 0, false);
   g.fft_t_fwd = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 1, false);
+  if (csz == 8) {
+    g.plane2 = g.buf.get(S_PLANE2, g.pc * plane_bytes);
+    g.fft_b_fwd_oop = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1, false);
+  }
   g.fftwork_size = 0;
-  for (rocfft_plan pl : {g.fft_b_inv, g.fft_b_fwd, g.fft_t_inv, g.fft_t_fwd}) {
+  for (rocfft_plan pl : {g.fft_b_inv, g.fft_b_fwd, g.fft_t_inv, g.fft_t_fwd, g.fft_b_fwd_oop}) {
+    if (!pl) continue;
     size_t w = 0;
     FFT_TRY(rocfft_plan_get_work_buffer_size(pl, &w));
     g.fftwork_size = std::max(g.fftwork_size, w);
@@ -3840,6 +3851,17 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     LAUNCH_CHECK();
   }
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
+  // band transforms out of place (FP32): the band rows of plane are cleared once and only
+  // their image columns are rewritten per pass
+  const bool oop = g.plane2 != nullptr;
+  cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
+  if (oop) {
+    PROF("transpose");
+    const int64_t nrows = (int64_t)(g.xhi - g.xlo) * P.gy;
+    for (int i = 0; i < g.pc * g.nb; ++i)
+      k_zero<cplx<R>><<<gridcap(nrows), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nrows);
+    LAUNCH_CHECK();
+  }
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
     const int npl = std::min(g.pc, P.nplanes - p0);
     const int half = pair_fill ? (p0 / g.pc) & 1 : 0;
@@ -3855,14 +3877,18 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     }
     {
       PROF("transpose");
-      k_tr_out<R><<<dim3((g.xhi - g.xlo + 31) / 32, (P.gy + 31) / 32, npl * g.nb), kBlock, 0, st>>>(
-          k, g.xlo, g.xhi, dbuf, plane);
+      const dim3 trg((g.xhi - g.xlo + 31) / 32, (P.gy + 31) / 32, npl * g.nb);
+      if (oop) k_tr_out<R, false><<<trg, kBlock, 0, st>>>(k, g.xlo, g.xhi, dbuf, plane);
+      else k_tr_out<R, true><<<trg, kBlock, 0, st>>>(k, g.xlo, g.xhi, dbuf, plane);
       LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_forward");
-      for (int i = 0; i < npl * g.nb; ++i)
-        geom_fft(g, g.fft_b_fwd, plane + ((int64_t)i * P.gx + g.xlo) * P.gy, st);
+      for (int i = 0; i < npl * g.nb; ++i) {
+        const int64_t off = ((int64_t)i * P.gx + g.xlo) * P.gy;
+        if (oop) geom_fft(g, g.fft_b_fwd_oop, plane + off, st, bplane + off);
+        else geom_fft(g, g.fft_b_fwd, plane + off, st);
+      }
     }
     const int dq = degrid_planes(g, npl);
     for (int q = 0; q < npl; q += dq) {
@@ -3876,11 +3902,11 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
       }
       {
         PROF("degrid");
-        launch_degrid<R>(g, p0 + q, nq, plane + q * np1, accs, st);
+        launch_degrid<R>(g, p0 + q, nq, bplane + q * np1, accs, st);
       }
       if (degrid_split<R>(g)) {
         PROF("degrid_dense");
-        launch_degrid_dense(g, p0 + q, nq, plane + q * np1, accs, st);
+        launch_degrid_dense(g, p0 + q, nq, bplane + q * np1, accs, st);
       }
     }
   }
