@@ -794,6 +794,36 @@ __global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
     a[i] = v;
 }
 
+// Invert with the band transforms out of place: the gridded planes then hold only what the
+// gridders wrote, and lastw[tile] records the last pass that wrote each tile. At pass pj only
+// the tiles written at pass pj - 1 can be nonzero (every earlier one was cleared when it went
+// stale); those are cleared unless the sub-tile gatherer stores them in full this pass.
+template <typename R>
+__global__ void k_zero_stale(KP k, int xlo, int xhi, int nplb, const uint32_t* __restrict__ task_off,
+                             const uint32_t* __restrict__ counters, uint32_t solo,
+                             const uint32_t* __restrict__ lastw, uint32_t prev, cplx<R>* __restrict__ plane) {
+  const int64_t nb = (int64_t)(xhi - xlo) * k.gy;
+  const uint32_t total = counters[1];
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nb; e += (int64_t)gridDim.x * kBlock) {
+    const int x = xlo + (int)(e / k.gy), y = (int)(e - (e / k.gy) * k.gy);
+    const uint32_t tile = (uint32_t)(x / k.T) * (uint32_t)k.nty + (uint32_t)(y / k.T);
+    if (lastw[tile] != prev) continue;
+    const uint32_t nt = (tile + 1 < k.ntiles ? task_off[tile + 1] : total) - task_off[tile];
+    if (nt == solo) continue;
+    for (int i = 0; i < nplb; ++i) plane[((int64_t)i * k.gx + x) * k.gy + y] = cplx<R>{(R)0, (R)0};
+  }
+}
+
+// lastw[tile] = pass for the tiles with sub-tile gatherer tasks this pass
+__global__ void k_mark_tiles(KP k, const uint32_t* __restrict__ task_off, const uint32_t* __restrict__ counters,
+                             uint32_t* __restrict__ lastw, uint32_t pass) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= k.ntiles) return;
+  const uint32_t total = counters[1];
+  const uint32_t nt = (t + 1 < k.ntiles ? task_off[t + 1] : total) - task_off[t];
+  if (nt) lastw[t] = pass;
+}
+
 // clears the band rows [xlo, xhi) of the nplb planes except the cells of tiles with exactly
 // `solo` tasks (those are stored in full by the sub-tile gatherer)
 template <typename R>
@@ -1777,6 +1807,29 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense(KP k, int p0, int npl, co
     cur = nxt;
     b0 = 0;
     tn = pop();
+  }
+}
+
+// lastw[tile] = pass for the tiles the footprints of this pass's dense-run tasks touch (a W x W
+// footprint from the packed first cell ax | ay << 16 meets at most 2 x 2 tiles, W <= tile)
+__global__ void k_mark_dense(KP k, int p0, int npl, const uint4* __restrict__ dtask,
+                             const uint32_t* __restrict__ dplane, const uint32_t* __restrict__ counters,
+                             uint32_t* __restrict__ lastw, uint32_t pass) {
+  int slo, shi;
+  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
+  const uint32_t total = counters[kDenseTotal];
+  const uint32_t lo = lower_bound_u32(dplane, total, (uint32_t)slo);
+  const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
+  for (uint32_t i = lo + blockIdx.x * kBlock + threadIdx.x; i < hi; i += gridDim.x * kBlock) {
+    const uint32_t A = dtask[i].z;
+    const int ax = (int)(A & 0xffffu), ay = (int)(A >> 16);
+#pragma unroll
+    for (int cx = 0; cx < 2; ++cx)
+#pragma unroll
+      for (int cy = 0; cy < 2; ++cy) {
+        const int x = (ax + cx * (k.W - 1)) % k.gx, y = (ay + cy * (k.W - 1)) % k.gy;
+        lastw[(uint32_t)(x / k.T) * (uint32_t)k.nty + (uint32_t)(y / k.T)] = pass;
+      }
   }
 }
 
@@ -2959,7 +3012,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_TASKTAB, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_PLANE2, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_PLANE2, S_LASTW, S_NSLOTS
 };
 
 struct FftKey {
@@ -3278,6 +3331,9 @@ struct Geom {
   // columns of the band rows in plane stay zero through a call
   void* plane2 = nullptr;
   rocfft_plan fft_b_fwd_oop = nullptr;
+  // ... and invert's, with lastw[tile] = the last pass that gridded into the tile
+  rocfft_plan fft_b_inv_oop = nullptr;
+  uint32_t* lastw = nullptr;
   int xlo = 0, xhi = 0;
   void* tbuf = nullptr;   // transposed band / image rows, zero where no kernel writes (cleared per call)
   void* tbuf2 = nullptr;  // its transforms along the grid rows
@@ -3462,9 +3518,11 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   if (csz == 8) {
     g.plane2 = g.buf.get(S_PLANE2, g.pc * plane_bytes);
     g.fft_b_fwd_oop = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1, false);
+    g.fft_b_inv_oop = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 0, false);
+    g.lastw = (uint32_t*)g.buf.get(S_LASTW, (size_t)P.ntx * P.nty * 4);
   }
   g.fftwork_size = 0;
-  for (rocfft_plan pl : {g.fft_b_inv, g.fft_b_fwd, g.fft_t_inv, g.fft_t_fwd, g.fft_b_fwd_oop}) {
+  for (rocfft_plan pl : {g.fft_b_inv, g.fft_b_fwd, g.fft_t_inv, g.fft_t_fwd, g.fft_b_fwd_oop, g.fft_b_inv_oop}) {
     if (!pl) continue;
     size_t w = 0;
     FFT_TRY(rocfft_plan_get_work_buffer_size(pl, &w));
@@ -3744,8 +3802,21 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   constexpr int RG = grid8_rows<R>();
   if (smw > kLdsBudget) HIPNG_THROW(HIPNG_ERUNTIME, "gridder LDS %zu exceeds %zu", smw, kLdsBudget);
   const bool pair_screen = sizeof(R) == 4;  // T2 holds two passes of FP32 planes
+  // FP32 planes: band transforms out of place into plane2, so a pass clears only the tiles the
+  // previous pass gridded into (lastw); the band is cleared once per call
+  const bool oop = g.plane2 != nullptr;
+  cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
+  const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
+  if (oop) {
+    PROF("grid_setup");
+    for (int i = 0; i < PCn * g.nb; ++i)
+      k_zero<cplx<R>><<<gridcap(nband), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nband);
+    k_fill_u32<<<gridcap(k.ntiles), kBlock, 0, st>>>(g.lastw, k.ntiles, 0xfffffff0u);
+    LAUNCH_CHECK();
+  }
   for (int p0 = 0; p0 < P.nplanes; p0 += PCn) {
     const int npl = std::min(PCn, P.nplanes - p0);
+    const uint32_t pass = (uint32_t)(p0 / PCn);
     {
       PROF("grid_setup");
       int slo, shi;
@@ -3754,9 +3825,16 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       build_tasks(g, slo, shi, 4, g.chunk_grid, st, P.tile / 8, gb0, gb1);
       // footprints stay inside the band, so only its rows need clearing; tiles the gatherer
       // owns alone are stored in full by it and are skipped here
-      const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
-      k_zero_shared<R><<<gridcap(nband), kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
-                                                         (uint32_t)(P.tile / 8), plane);
+      if (oop) {
+        k_zero_stale<R><<<gridcap(nband), kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
+                                                          (uint32_t)(P.tile / 8), g.lastw, pass - 1u, plane);
+        k_mark_tiles<<<grid1(k.ntiles), kBlock, 0, st>>>(k, g.task, g.counters, g.lastw, pass);
+        if (g.dense)
+          k_mark_dense<<<1024, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, g.counters, g.lastw, pass);
+      } else {
+        k_zero_shared<R><<<gridcap(nband), kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
+                                                           (uint32_t)(P.tile / 8), plane);
+      }
       LAUNCH_CHECK();
     }
     {
@@ -3788,14 +3866,17 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     }
     {
       PROF("fft_c2c_inverse");
-      for (int i = 0; i < npl * g.nb; ++i)
-        geom_fft(g, g.fft_b_inv, plane + ((int64_t)i * P.gx + g.xlo) * P.gy, st);
+      for (int i = 0; i < npl * g.nb; ++i) {
+        const int64_t off = ((int64_t)i * P.gx + g.xlo) * P.gy;
+        if (oop) geom_fft(g, g.fft_b_inv_oop, plane + off, st, bplane + off);
+        else geom_fft(g, g.fft_b_inv, plane + off, st);
+      }
     }
     {
       PROF("transpose");
       const int xb = g.xlo & ~31;
       k_tr_in<R><<<dim3((g.xhi - xb + 31) / 32, (P.ny + 31) / 32, npl * g.nb), kBlock, 0, st>>>(
-          k, g.xlo, g.xhi, xb, plane, tbuf);
+          k, g.xlo, g.xhi, xb, bplane, tbuf);
       LAUNCH_CHECK();
     }
     // FP32 planes of an even pass wait in the first half of T2 for the next pass's, and the
