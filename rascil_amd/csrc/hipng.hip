@@ -1035,6 +1035,12 @@ __global__ void k_task_tiles(const uint32_t* __restrict__ task_off, int64_t ntil
 // Gridding constants and the plane window of a pass
 // ---------------------------------------------------------------------------------
 constexpr int kMaxPC = 2;                    // w planes gridded per pass
+// FP32 modes: passes whose image-side transforms T2 holds at once, so that one update of the
+// FP64 image accumulator (invert, kScreenGroup) or one read of the image (predict,
+// kFillGroup) serves all of them. Measured per production-shape step: invert screens 46.3 /
+// 35.7 / 30.5 ms for groups of 1 / 2 / 4; predict fills 33.0 / 27.9 / 30.1 ms.
+constexpr int kScreenGroup = 4;
+constexpr int kFillGroup = 2;
 constexpr int kXcd = 8;                      // XCDs per device: work queues of the sub-tile gridder
 constexpr int kQueueBase = 8;                // counters[kQueueBase + x]: pops from queue x
 constexpr int kMaxSeg = 4 * (16 + kMaxPC - 1);  // 4 neighbour buckets x start planes
@@ -3510,7 +3516,8 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   g.tbuf = g.buf.get(S_TBUF, (size_t)g.pc * nb * P.ny * P.gx * csz);
   // FP32 planes: room for two passes, so that invert screens both into the FP64 image
   // accumulator with one read-modify-write
-  g.tbuf2 = g.buf.get(S_TBUF2, (size_t)(csz == 8 ? 2 : 1) * g.pc * nb * P.ny * P.gx * csz);
+  g.tbuf2 = g.buf.get(S_TBUF2, (size_t)(csz == 8 ? std::max(kScreenGroup, kFillGroup) : 1) * g.pc * nb * P.ny *
+                                   P.gx * csz);
   g.fft_b_inv = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 0);
   g.fft_b_fwd = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1);
   g.fft_t_inv = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 0, false);
@@ -3801,7 +3808,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   // FP64 (whose vector accumulators would not leave room for a second sub-tile)
   constexpr int RG = grid8_rows<R>();
   if (smw > kLdsBudget) HIPNG_THROW(HIPNG_ERUNTIME, "gridder LDS %zu exceeds %zu", smw, kLdsBudget);
-  const bool pair_screen = sizeof(R) == 4;  // T2 holds two passes of FP32 planes
+  const int ngrp = sizeof(R) == 4 ? kScreenGroup : 1;  // passes of FP32 planes T2 holds
   // FP32 planes: band transforms out of place into plane2, so a pass clears only the tiles the
   // previous pass gridded into (lastw); the band is cleared once per call
   const bool oop = g.plane2 != nullptr;
@@ -3879,16 +3886,16 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
           k, g.xlo, g.xhi, xb, bplane, tbuf);
       LAUNCH_CHECK();
     }
-    // FP32 planes of an even pass wait in the first half of T2 for the next pass's, and the
-    // screen adds both to the accumulator at once (the last pass flushes what is held)
-    const int half = pair_screen ? (p0 / PCn) & 1 : 0;
+    // FP32 planes of a group of kScreenGroup passes wait in their slots of T2, and the screen
+    // adds them to the accumulator at once (the last pass flushes what is held)
+    const int slot = (p0 / PCn) % ngrp;
     {
       PROF("fft_c2c_inverse");
-      geom_fft(g, g.fft_t_inv, tbuf, st, tbuf2 + (int64_t)half * PCn * g.nb * P.ny * P.gx);
+      geom_fft(g, g.fft_t_inv, tbuf, st, tbuf2 + (int64_t)slot * PCn * g.nb * P.ny * P.gx);
     }
-    if (!pair_screen || half == 1 || p0 + npl >= P.nplanes) {
+    if (slot == ngrp - 1 || p0 + npl >= P.nplanes) {
       PROF("screen_acc");
-      const int s0 = half == 1 ? p0 - PCn : p0;
+      const int s0 = p0 - slot * PCn;
       k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, s0, p0 + npl - s0, tbuf2, acc);
       LAUNCH_CHECK();
     }
@@ -3920,14 +3927,14 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   auto* tbuf = (cplx<R>*)g.tbuf;
   auto* tbuf2 = (cplx<R>*)g.tbuf2;
   // The screens fill the image part of S (cleared once per call) and the transforms along
-  // the grid rows go out of place into D. FP32 planes: S = T2, which holds two passes, so one
-  // read of the image fills the planes of an even pass and the next; FP64: S = T.
-  const bool pair_fill = sizeof(R) == 4;
-  cplx<R>* sbuf = pair_fill ? tbuf2 : tbuf;
-  cplx<R>* dbuf = pair_fill ? tbuf : tbuf2;
+  // the grid rows go out of place into D. FP32 planes: S = T2 (kFillGroup passes of it), so
+  // one read of the image fills the planes of a group of passes; FP64: S = T.
+  const int ngrp = sizeof(R) == 4 ? kFillGroup : 1;
+  cplx<R>* sbuf = ngrp > 1 ? tbuf2 : tbuf;
+  cplx<R>* dbuf = ngrp > 1 ? tbuf : tbuf2;
   {
     PROF("transpose");
-    const int64_t nt = (int64_t)(pair_fill ? 2 : 1) * g.pc * g.nb * P.ny * P.gx;
+    const int64_t nt = (int64_t)ngrp * g.pc * g.nb * P.ny * P.gx;
     k_zero<cplx<R>><<<gridcap(nt), kBlock, 0, st>>>(sbuf, nt);
     LAUNCH_CHECK();
   }
@@ -3945,16 +3952,16 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   }
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
     const int npl = std::min(g.pc, P.nplanes - p0);
-    const int half = pair_fill ? (p0 / g.pc) & 1 : 0;
-    if (half == 0) {
+    const int slot = (p0 / g.pc) % ngrp;
+    if (slot == 0) {
       PROF("screen_fill");
-      const int nfill = std::min((pair_fill ? 2 : 1) * g.pc, P.nplanes - p0);
+      const int nfill = std::min(ngrp * g.pc, P.nplanes - p0);
       k_screen_fill_t<R, A, F32C><<<dim3(grid1(P.gx), P.ny), kBlock, 0, st>>>(k, p0, nfill, dc, sbuf);
       LAUNCH_CHECK();
     }
     {
       PROF("fft_c2c_forward");
-      geom_fft(g, g.fft_t_fwd, sbuf + (int64_t)half * g.pc * g.nb * P.ny * P.gx, st, dbuf);
+      geom_fft(g, g.fft_t_fwd, sbuf + (int64_t)slot * g.pc * g.nb * P.ny * P.gx, st, dbuf);
     }
     {
       PROF("transpose");
