@@ -94,6 +94,25 @@ def test_ms2dirty_vs_explicit(small_obs, prec, eps, tol, wst):
     assert err <= tol
 
 
+@pytest.mark.parametrize("prec,eps,tol", [("mixed", 1e-5, 1e-5), ("fp32", 1e-3, 1e-3), ("fp64", 1e-7, 1e-7)])
+def test_odd_half_height_image(small_obs, prec, eps, tol):
+    """An image whose half-height is odd (ny = 90): the image columns around the grid wrap
+    (the fftshift of an odd half) in every precision, both directions."""
+    uvw, _ = small_obs
+    nx, ny, px, py = 64, 90, 6e-4, 7e-4
+    ms, wgt = _case(uvw, FREQ2)
+    if prec != "fp64":
+        ms, wgt = ms.astype("c8"), wgt.astype("f4")
+    d = wgridder.ms2dirty(uvw, FREQ2, ms, wgt, nx, ny, px, py, 0, 0, eps, True, precision=prec)
+    ref = ox.explicit_ms2dirty(uvw, FREQ2, ms.astype("c16"), wgt.astype("f8"), nx, ny, px, py, True)
+    assert rel_l2(d, ref) <= tol
+    rng = numpy.random.default_rng(11)
+    img = rng.normal(size=(nx, ny)).astype("f8" if prec == "fp64" else "f4")
+    v = wgridder.dirty2ms(uvw, FREQ2, img, None, px, py, 0, 0, eps, True, precision=prec)
+    vref = ox.explicit_dirty2ms(uvw, FREQ2, img.astype("f8"), None, px, py, True)
+    assert rel_l2(v, vref) <= tol
+
+
 @pytest.mark.parametrize("prec,eps,tol", CASES)
 @pytest.mark.parametrize("wst", [False, True])
 def test_dirty2ms_vs_explicit(small_obs, prec, eps, tol, wst):
