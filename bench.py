@@ -849,7 +849,8 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus %d is not supported "
-                                   "(the sharding design is described in DESIGN.md)" % args.gpus}))
+                                   "(invert shards by channel or row block with one all-reduce of the image; predict by rows "
+                                   "after one broadcast of the model)" % args.gpus}))
         sys.exit(2)
     import numpy
     import torch
