@@ -846,6 +846,18 @@ __global__ void k_zero(T* __restrict__ a, int64_t n) {
     a[i] = T{};
 }
 
+// zero columns [c0, c1) of nrows consecutive rows of length rowlen
+template <typename T>
+__global__ void k_zero_cols(T* __restrict__ a, int64_t nrows, int rowlen, int c0, int c1) {
+  const int w = c1 - c0;
+  const int64_t n = nrows * w;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t row = i / w;
+    a[row * rowlen + c0 + (int)(i - row * w)] = T{};
+  }
+}
+
+
 __global__ void k_bounds(const uint32_t* __restrict__ keys, int64_t n, uint32_t* __restrict__ bstart,
                          uint32_t* __restrict__ bend) {
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -3083,6 +3095,12 @@ rocfft_plan fft_plan1(DevCtx& c, int len, int batch, int prec, int fwd, bool inp
 
 unsigned grid1(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 unsigned gridcap(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 20); }
+// zero columns [c0, c1) of nrows rows of length rowlen
+template <typename T>
+void zero_cols(T* a, int64_t nrows, int rowlen, int c0, int c1, hipStream_t st) {
+  if (c1 <= c0 || nrows <= 0) return;
+  k_zero_cols<T><<<gridcap(nrows * (c1 - c0)), kBlock, 0, st>>>(a, nrows, rowlen, c0, c1);
+}
 
 void extent(DevCtx& c, const double* uvw, int64_t nrow, const double* freq, int nchan, double out[4],
             hipStream_t st) {
@@ -3794,8 +3812,10 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   {
     // the per-pass kernels write only the band (invert) or the image (predict) part of T
     PROF("transpose");
-    const int64_t nt = (int64_t)g.pc * g.nb * P.ny * P.gx;
-    k_zero<cplx<R>><<<gridcap(nt), kBlock, 0, st>>>(tbuf, nt);
+    // k_tr_in rewrites the band columns [xlo, xhi) of T every pass; the others stay zero
+    const int64_t rows = (int64_t)g.pc * g.nb * P.ny;
+    zero_cols(tbuf, rows, P.gx, 0, g.xlo, st);
+    zero_cols(tbuf, rows, P.gx, g.xhi, P.gx, st);
     LAUNCH_CHECK();
   }
   const int PCn = g.pc;
@@ -3934,8 +3954,8 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   cplx<R>* dbuf = ngrp > 1 ? tbuf : tbuf2;
   {
     PROF("transpose");
-    const int64_t nt = (int64_t)ngrp * g.pc * g.nb * P.ny * P.gx;
-    k_zero<cplx<R>><<<gridcap(nt), kBlock, 0, st>>>(sbuf, nt);
+    // the fills rewrite the image columns of S; the others stay zero
+    zero_cols(sbuf, (int64_t)ngrp * g.pc * g.nb * P.ny, P.gx, P.nx / 2, P.gx - P.nx / 2, st);
     LAUNCH_CHECK();
   }
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
@@ -3945,9 +3965,10 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
   if (oop) {
     PROF("transpose");
-    const int64_t nrows = (int64_t)(g.xhi - g.xlo) * P.gy;
+    // k_tr_out rewrites the image columns of the band rows; the others stay zero
     for (int i = 0; i < g.pc * g.nb; ++i)
-      k_zero<cplx<R>><<<gridcap(nrows), kBlock, 0, st>>>(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, nrows);
+      zero_cols(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, (int64_t)(g.xhi - g.xlo), P.gy, P.ny / 2,
+                P.gy - P.ny / 2, st);
     LAUNCH_CHECK();
   }
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
