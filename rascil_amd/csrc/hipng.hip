@@ -2389,9 +2389,11 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
           }
         }
         if (qlo < qhi) {
+          // the launch holding the start plane is the first to reach the record: it stores
+          const bool first = s >= p0;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
-            cplx<R> o = accs[(int64_t)j * NB + b];
+            cplx<R> o = first ? cplx<R>{(R)0, (R)0} : accs[(int64_t)j * NB + b];
             o.re += sr[b];
             o.im += si[b];
             accs[(int64_t)j * NB + b] = o;
@@ -2510,9 +2512,10 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
       wave_sync();
       if (lane < cnt) {
         cplx<float>* o = accs + (int64_t)(cur.x + b0 + (uint32_t)lane) * NB;
+        const bool first = (int)cur.w >= p0;  // the launch holding the run's start plane stores
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          cplx<float> a = o[b];
+          cplx<float> a = first ? cplx<float>{0.f, 0.f} : o[b];
           a.re += s_out[wv][lane][b][0];
           a.im += s_out[wv][lane][b][1];
           o[b] = a;
@@ -3940,9 +3943,9 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
         k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
     LAUNCH_CHECK();
   }
+  // the sorted accumulators need no clearing: the degridders store at each record's first
+  // pass (the one holding its start plane) and add at the later ones
   auto* accs = (cplx<R>*)g.recv;
-  k_zero<cplx<R>><<<gridcap(g.n * g.nb), kBlock, 0, st>>>(accs, g.n * g.nb);
-  LAUNCH_CHECK();
   auto* plane = (cplx<R>*)g.plane;
   auto* tbuf = (cplx<R>*)g.tbuf;
   auto* tbuf2 = (cplx<R>*)g.tbuf2;
