@@ -2102,7 +2102,8 @@ __global__ __launch_bounds__(kBlock) void k_tr_out(KP k, int xlo, int xhi, const
 
 // invert: accT[b][iy][ix] += (-1)^(tx+ty) sum_q Re(T_q[b][iy][gxi] exp(-2 pi i w_q nm1s))
 template <typename R, typename A, bool F32C>
-__global__ void k_screen_acc_t(KP k, int p0, int npl, const cplx<R>* __restrict__ tb, A* __restrict__ acc) {
+__global__ void k_screen_acc_t(KP k, int p0, int npl, const cplx<R>* __restrict__ tb, A* __restrict__ acc,
+                               int first) {
   const int ix = blockIdx.x * kBlock + threadIdx.x;
   const int iy = blockIdx.y;
   if (ix >= k.nx) return;
@@ -2122,7 +2123,11 @@ __global__ void k_screen_acc_t(KP k, int p0, int npl, const cplx<R>* __restrict_
       }
     }
   }
-  for (int b = 0; b < k.nb; ++b) acc[((int64_t)b * k.ny + iy) * k.nx + ix] += (A)sign * sum[b];
+  // the first update stores (the accumulator is not cleared beforehand)
+  for (int b = 0; b < k.nb; ++b) {
+    A* a = &acc[((int64_t)b * k.ny + iy) * k.nx + ix];
+    *a = first ? (A)sign * sum[b] : *a + (A)sign * sum[b];
+  }
 }
 
 // predict: T_q[b][iy][gxi] = (-1)^(tx+ty) dcT[b][iy][ix] exp(+2 pi i w_q nm1s) on image rows (T stays
@@ -3806,9 +3811,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     launch_values<R>(g, (const cplx<R>*)ms, (const R*)wgt, recv, st);
     LAUNCH_CHECK();
   }
-  auto* acc = (A*)g.accimg;
-  k_zero<A><<<gridcap(nimg), kBlock, 0, st>>>(acc, nimg);
-  LAUNCH_CHECK();
+  auto* acc = (A*)g.accimg;  // the first screen update stores into it
   auto* plane = (cplx<R>*)g.plane;
   auto* tbuf = (cplx<R>*)g.tbuf;
   auto* tbuf2 = (cplx<R>*)g.tbuf2;
@@ -3919,7 +3922,8 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     if (slot == ngrp - 1 || p0 + npl >= P.nplanes) {
       PROF("screen_acc");
       const int s0 = p0 - slot * PCn;
-      k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, s0, p0 + npl - s0, tbuf2, acc);
+      k_screen_acc_t<R, A, F32C><<<dim3(grid1(P.nx), P.ny), kBlock, 0, st>>>(k, s0, p0 + npl - s0, tbuf2, acc,
+                                                                               s0 == 0 ? 1 : 0);
       LAUNCH_CHECK();
     }
   }
