@@ -802,14 +802,17 @@ template <typename R>
 __global__ void k_zero_stale(KP k, int xlo, int xhi, int nplb, const uint32_t* __restrict__ task_off,
                              const uint32_t* __restrict__ counters, uint32_t solo,
                              const uint32_t* __restrict__ lastw, uint32_t prev, cplx<R>* __restrict__ plane) {
-  const int64_t nb = (int64_t)(xhi - xlo) * k.gy;
+  // one block per tile of the band (tile rows from xlo / T); the whole block leaves when the
+  // tile is not stale or the gatherer stores it in full this pass
+  const int tx = xlo / k.T + (int)blockIdx.y, ty = (int)blockIdx.x;
+  const uint32_t tile = (uint32_t)tx * (uint32_t)k.nty + (uint32_t)ty;
+  if (lastw[tile] != prev) return;
   const uint32_t total = counters[1];
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < nb; e += (int64_t)gridDim.x * kBlock) {
-    const int x = xlo + (int)(e / k.gy), y = (int)(e - (e / k.gy) * k.gy);
-    const uint32_t tile = (uint32_t)(x / k.T) * (uint32_t)k.nty + (uint32_t)(y / k.T);
-    if (lastw[tile] != prev) continue;
-    const uint32_t nt = (tile + 1 < k.ntiles ? task_off[tile + 1] : total) - task_off[tile];
-    if (nt == solo) continue;
+  const uint32_t nt = (tile + 1 < k.ntiles ? task_off[tile + 1] : total) - task_off[tile];
+  if (nt == solo) return;
+  for (int c = threadIdx.x; c < k.T * k.T; c += kBlock) {
+    const int x = tx * k.T + c / k.T, y = ty * k.T + c % k.T;
+    if (x < xlo || x >= xhi) continue;
     for (int i = 0; i < nplb; ++i) plane[((int64_t)i * k.gx + x) * k.gy + y] = cplx<R>{(R)0, (R)0};
   }
 }
@@ -3859,8 +3862,9 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       // footprints stay inside the band, so only its rows need clearing; tiles the gatherer
       // owns alone are stored in full by it and are skipped here
       if (oop) {
-        k_zero_stale<R><<<gridcap(nband), kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
-                                                          (uint32_t)(P.tile / 8), g.lastw, pass - 1u, plane);
+        const dim3 zg((unsigned)P.nty, (unsigned)((g.xhi - 1) / P.tile - g.xlo / P.tile + 1));
+        k_zero_stale<R><<<zg, kBlock, 0, st>>>(k, g.xlo, g.xhi, npl * g.nb, g.task, g.counters,
+                                               (uint32_t)(P.tile / 8), g.lastw, pass - 1u, plane);
         k_mark_tiles<<<grid1(k.ntiles), kBlock, 0, st>>>(k, g.task, g.counters, g.lastw, pass);
         if (g.dense)
           k_mark_dense<<<1024, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, g.counters, g.lastw, pass);
