@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy 4 times (counter calibration for rocprof --pmc)")
     ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting", "gains",
-                                                        "vis_passes"],
+                                                        "vis_passes", "dft"],
                     help="run a study instead of the bench line (prints one JSON line)")
     ap.add_argument("--fp64-steps", type=int, default=0,
                     help="also time the FP64 plain mode for this many steps")
@@ -817,6 +817,81 @@ def vis_passes_study(reps=5, seed=17):
     return out
 
 
+def dft_study(reps=3, seed=19, ncomp=100, nrows=2 ** 21):
+    """FP64 point-source DFT (hipng_dft): 2,097,152 rows of the production geometry (its first
+    rows) x 64 channels x 4 polarisations, 100 components within 2 deg, complex128 fluxes;
+    inputs resident in HBM. Per (visibility, component): the phase (3 FMA), one FP64 sincospi
+    and 4 complex multiply-adds; the roofline counts 38 FLOP per pair (the sincospi not
+    included) against the FP64 vector peak. CPU baseline: the oracle's numpy DFT on 8,192 rows
+    with all components."""
+    import ctypes
+
+    import numpy
+    import torch
+
+    from rascil_amd import _lib, simulation
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    lib = _lib.load()
+    uvw, freq, _ = simulation.config1_geometry()
+    uvw = uvw[:nrows]
+    nchan, npol = len(freq), 4
+    rng = numpy.random.default_rng(seed)
+    r = numpy.radians(2.0) * numpy.sqrt(rng.uniform(0, 1, ncomp))
+    phi = rng.uniform(0, 2 * numpy.pi, ncomp)
+    l, m = r * numpy.cos(phi), r * numpy.sin(phi)
+    dc = numpy.stack([l, m, numpy.sqrt(1 - l * l - m * m) - 1], axis=1)
+    flux = (rng.uniform(0.1, 1.0, (ncomp, nchan, npol)) + 0j)
+    uvl = uvw[:, None, :] * (freq / 299792458.0)[None, :, None]
+    dc_d = torch.from_numpy(dc).to(dev)
+    fl_d = torch.from_numpy(flux).to(dev)
+    uvl_d = torch.from_numpy(numpy.ascontiguousarray(uvl)).to(dev)
+    vis = torch.empty((nrows, nchan, npol), dtype=torch.complex128, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    nvis = nrows * nchan * npol
+
+    def call():
+        _lib.check(lib.hipng_dft(ctypes.c_void_p(dc_d.data_ptr()), ctypes.c_void_p(fl_d.data_ptr()),
+                                 ctypes.c_void_p(uvl_d.data_ptr()), ncomp, nrows, nchan, npol,
+                                 ctypes.c_void_p(vis.data_ptr()), st))
+
+    call()
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call()
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    t = float(numpy.median(ms)) * 1e-3
+    flop = 38.0 * nrows * nchan * ncomp
+    out = {"study": "dft", "nvis": int(nvis), "rows": nrows, "channels": nchan, "pols": npol,
+           "components": ncomp, "data": "synthetic", "ms_per_call": round(t * 1e3, 2),
+           "value": round(nvis / t / 1e6, 1), "unit": "Mvis/s",
+           "vis_components_per_s": round(nrows * nchan * ncomp / t / 1e9, 2),
+           "roofline": {"bound": "fp64", "achieved": round(flop / t / 1e12, 2), "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(flop / t / 1e12 / FP64_PEAK_TFLOPS, 4),
+                        "flop_per_call": int(flop), "note": "38 FLOP per (visibility row-channel, component); "
+                                                            "sincospi not counted"}}
+    print("[study] dft", out, file=sys.stderr, flush=True)
+    from oracle import explicit as ox
+
+    rows = 8192
+    t0 = time.time()
+    ox.dft_cpu_looped(dc, uvl[:rows][None], flux)
+    dt = time.time() - t0
+    cpu = rows * nchan * npol / dt / 1e6
+    out["cpu_baseline"] = {"value": round(cpu, 3), "unit": "Mvis/s", "cores": 1, "kind": "port",
+                           "sample": f"numpy oracle DFT on {rows} rows x {nchan} ch x {npol} pol x {ncomp} "
+                                     f"components ({dt:.2f} s)"}
+    out["speedup_vs_cpu"] = round(out["value"] / cpu, 1)
+    return out
+
+
 def copy_probe(nbytes=1 << 30, reps=4):
     """A plain device copy of known size: the rocprof FETCH_SIZE / WRITE_SIZE counters read on
     it give the scale factor between counter units and bytes moved."""
@@ -844,7 +919,8 @@ def main():
         os.environ["HIPNG_SIGMA"] = str(args.sigma)
     if args.study:
         fn = {"precision": precision_study, "autograd": autograd_study, "batched": batched_study,
-              "weighting": weighting_study, "gains": gains_study, "vis_passes": vis_passes_study}[args.study]
+              "weighting": weighting_study, "gains": gains_study, "vis_passes": vis_passes_study,
+              "dft": dft_study}[args.study]
         print(json.dumps(fn()), flush=True)
         return
     if args.gpus != 1:
