@@ -44,14 +44,15 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy probe")
     ap.add_argument("--sigma", type=float, default=None,
-                    help="force the grid oversampling factor (cost-model calibration, not the bench line)")
+                    help="pin the grid oversampling factor through the plan options (cost-model "
+                         "calibration, not the bench line)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy 4 times (counter calibration for rocprof --pmc)")
     ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting", "gains",
                                                         "vis_passes", "dft"],
                     help="run a study instead of the bench line (prints one JSON line)")
-    ap.add_argument("--fp64-steps", type=int, default=0,
-                    help="also time the FP64 plain mode for this many steps")
+    ap.add_argument("--fp64-steps", type=int, default=2,
+                    help="also time the FP64 plain mode for this many steps (0: skip)")
     return ap.parse_args()
 
 
@@ -76,8 +77,9 @@ def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc, dense_degrid=False):
     FLOP follow the per-visibility counts of the gridding work model: 4 W^2 W_w (one complex x
     real FMA pair per cell and w plane) + 3 W c_ker (kernel taps, c_ker = 10) per visibility and
     direction; FFT 5 G^2 log2(G^2) per plane and polarisation. Bytes: each record is read once
-    per pass of pc planes its w window meets, plus the planes written (gridder) or read
-    (degridder, with the sorted accumulators read and written). The gridding of an invert is
+    per pass of pc planes its w window meets, plus the occupied grid cells written (gridder:
+    (plane, tile) pairs some footprint touches, from the plan) or the planes read (degridder,
+    with the sorted accumulators read and written). The gridding of an invert is
     split between the sub-tile gatherer ("grid": planes written, records outside dense runs)
     and the dense-run gridder ("grid_dense": the plan's dense_items records); with
     `dense_degrid` the degridding of a predict is split the same way ("degrid": planes read,
@@ -92,9 +94,12 @@ def algorithmic_work(name, plan, nvis, nitems, nb, csz, pc, dense_degrid=False):
     dense = int(plan.get("dense_items", 0))
     per_vis = 4.0 * W * W * ww + 30.0 * W
     if name == "grid":
+        # records and values read once per pass; the grid cells footprints touch written once
+        # (at the tiles' granularity: occupied (plane, tile) pairs x tile^2, both gridding paths)
         items = nitems - dense
+        occ = int(plan.get("occupied_tiles", 0)) * plan["tile"] ** 2
         return (items * nb * per_vis,
-                items * passes * (4 + 3 * rsz + nb * csz) + P * nb * G2 * csz)
+                items * passes * (4 + 3 * rsz + nb * csz) + occ * nb * csz)
     if name == "grid_dense":
         return (dense * nb * per_vis, dense * passes * (3 * rsz + nb * csz))
     if name == "degrid":
@@ -139,77 +144,147 @@ def cpu_plans(plan_i, npol, nitems):
     from rascil_amd import wgridder
 
     out = []
-    old = os.environ.get("HIPNG_SIGMA")
-    try:
-        for sg in (1.25, 2.0):
-            if abs(sg - plan_i["sigma"]) < 1e-9:
-                continue
-            os.environ["HIPNG_SIGMA"] = str(sg)
-            p = wgridder.plan_params(plan_i["umax"], plan_i["vmax"], plan_i["wmin"], plan_i["wmax"],
-                                     plan_i["nx"], plan_i["ny"], plan_i["pixsize_x"], plan_i["pixsize_y"],
-                                     plan_i["epsilon"], True, "fp64", npol, nitems).as_dict()
-            out.append((p, p))
-    finally:
-        if old is None:
-            os.environ.pop("HIPNG_SIGMA", None)
-        else:
-            os.environ["HIPNG_SIGMA"] = old
+    for sg in (1.25, 2.0):
+        if abs(sg - plan_i["sigma"]) < 1e-9:
+            continue
+        p = wgridder.plan_params(plan_i["umax"], plan_i["vmax"], plan_i["wmin"], plan_i["wmax"],
+                                 plan_i["nx"], plan_i["ny"], plan_i["pixsize_x"], plan_i["pixsize_y"],
+                                 plan_i["epsilon"], True, "fp64", npol, nitems,
+                                 plan_opts={"sigma": sg}).as_dict()
+        out.append((p, p))
     return out
 
 
-def accuracy_probe(wl, prec, nsrc=20, frac=0.01, seed=7):
-    """Max / RMS relative error of predict and invert at the workload's plan, on a row
-    subsample, against FP64 direct sums: predict vs the FP64 HIP DFT of pixel-centred
-    sources (flux / n, the w-stacking convention); invert at the source pixels vs the FP64
-    adjoint sum evaluated with torch on the GPU."""
+def accuracy_at_timed_plan(wl, prec, uvw_d, ms_buf, wgt_d, timed_plans, nsrc=20, seed=7, chunk_rows=1 << 20):
+    """Max / RMS relative error of predict and invert against FP64 direct sums over the WHOLE
+    workload, through the same one-shot calls as the timed step (same uvw, batch, weights,
+    epsilon and precision, hence the same plan, which is compared with the timed one):
+
+    * predict of `nsrc` pixel-centred sources, polarisation b scaled by (1, 0.5, -0.25,
+      0.125)[b], against the FP64 HIP DFT of flux / n (the w-stacking convention) at every
+      visibility;
+    * invert of those exact visibilities (rounded to the input type, written into `ms_buf`)
+      with the timed step's weights, at the source pixels, against the FP64 adjoint sum
+      Re sum_i w_i V_i exp(+2 pi i phi_i) / n over every visibility (torch, FP64, on the GPU).
+    """
     import numpy
     import torch
 
     from rascil_amd import wgridder
-    from rascil_amd.dft import dft_kernel
+    from rascil_amd.dft import dft_device
 
+    t_start = time.time()
+    dev = uvw_d.device
     rng = numpy.random.default_rng(seed)
-    uvw = wl["uvw"]
-    rows = numpy.sort(rng.choice(uvw.shape[0], max(1, int(uvw.shape[0] * frac)), replace=False))
-    su = numpy.ascontiguousarray(uvw[rows])
-    freq, N, cell, eps = wl["freq"], wl["npix"], wl["cell"], wl["eps"]
-    rdt = numpy.float64 if prec == "fp64" else numpy.float32
-    model = numpy.zeros((N, N), rdt)
+    freq, N, cell, eps = numpy.asarray(wl["freq"], "f8"), wl["npix"], wl["cell"], wl["eps"]
+    nrow, nchan, nb = ms_buf.shape
+    rdt = torch.float64 if prec == "fp64" else torch.float32
     ix = rng.integers(N // 8, N - N // 8, nsrc)
     iy = rng.integers(N // 8, N - N // 8, nsrc)
     flux = rng.uniform(0.1, 1.0, nsrc)
-    model[ix, iy] = flux
-    # ducc0 convention (the caller negates u and w): x = (ix - N/2) cell -> l = -x, m = y
+    scale = numpy.array([1.0, 0.5, -0.25, 0.125])[:nb]
+    model = torch.zeros((nb, N, N), dtype=rdt, device=dev)
+    ixt, iyt = torch.tensor(ix, device=dev), torch.tensor(iy, device=dev)
+    for b in range(nb):
+        model[b].index_put_((ixt, iyt), torch.tensor(flux * scale[b], dtype=rdt, device=dev), accumulate=True)
     x = (ix - N // 2) * cell
     y = (iy - N // 2) * cell
     n = numpy.sqrt(1 - x * x - y * y)
-    ruvw = su * numpy.array([-1.0, 1.0, -1.0])
-    uvl = (ruvw[:, None, :] * (numpy.asarray(freq) / 299792458.0)[None, :, None])[None]
-    dc = numpy.stack([-x, y, n - 1], axis=1)
-    ref = dft_kernel(dc, (flux / n)[:, None, None] * numpy.ones((1, len(freq), 1)), uvl)[0, :, :, 0]
-    t = time.time()
-    vis = wgridder.dirty2ms(su, freq, model, None, cell, cell, 0, 0, eps, True, precision=prec)
-    err = numpy.abs(vis - ref)
-    out = {"rows": len(rows), "nvis": int(ref.size), "sources": nsrc,
-           "predict": {"max_rel": float(err.max() / numpy.abs(ref).max()),
-                       "rms_rel": float(numpy.sqrt((err ** 2).mean() / (numpy.abs(ref) ** 2).mean()))}}
-    # invert the exact visibilities; compare the source pixels with the FP64 adjoint sum
-    dirty = wgridder.ms2dirty(su, freq, ref.astype(vis.dtype), None, N, N, cell, cell, 0, 0, eps, True,
-                              precision=prec)
-    dev = torch.device("cuda", 0)
-    U = torch.tensor(uvl[0].reshape(-1, 3), device=dev)
-    V = torch.tensor(ref.reshape(-1), device=dev)
-    exact = []
-    for k in range(nsrc):
-        ph = U[:, 0] * (-x[k]) + U[:, 1] * y[k] + U[:, 2] * (n[k] - 1)  # +phase of the DFT above
-        exact.append(float((V * torch.exp(2j * numpy.pi * ph.to(torch.complex128))).real.sum()) / n[k])
-    exact = numpy.array(exact)
-    got = dirty[ix, iy].astype(numpy.float64)
-    e2 = numpy.abs(got - exact)
-    out["invert"] = {"max_rel": float(e2.max() / numpy.abs(exact).max()),
-                     "rms_rel": float(numpy.sqrt((e2 ** 2).mean() / (exact ** 2).mean()))}
-    out["seconds"] = round(time.time() - t, 1)
+    plan_p = {}
+    vis = wgridder.dirty2ms(uvw_d, freq, model, None, cell, cell, 0, 0, eps, True, precision=prec, plan_out=plan_p)
+    dc = torch.tensor(numpy.stack([-x, y, n - 1], axis=1), dtype=torch.float64, device=dev)
+    fl = torch.tensor(flux / n, dtype=torch.complex128, device=dev).reshape(nsrc, 1, 1).expand(nsrc, nchan, 1).contiguous()
+    sgn = torch.tensor([-1.0, 1.0, -1.0], dtype=torch.float64, device=dev)
+    fc = torch.tensor(freq / 299792458.0, dtype=torch.float64, device=dev)
+    sc = torch.tensor(scale, dtype=torch.complex128, device=dev)
+    pe_max = torch.zeros(nb, dtype=torch.float64, device=dev)
+    pe_sq = torch.zeros(nb, dtype=torch.float64, device=dev)
+    pr_max = torch.zeros(nb, dtype=torch.float64, device=dev)
+    pr_sq = torch.zeros(nb, dtype=torch.float64, device=dev)
+    for r0 in range(0, nrow, chunk_rows):
+        r1 = min(nrow, r0 + chunk_rows)
+        uvl = ((uvw_d[r0:r1] * sgn)[:, None, :] * fc[None, :, None]).contiguous()
+        ref = dft_device(dc, fl, uvl.reshape(1, r1 - r0, nchan, 3))[0, :, :, 0]
+        refb = ref[..., None] * sc  # [rows, nchan, nb]
+        err = (vis[r0:r1].to(torch.complex128) - refb).abs()
+        pe_max = torch.maximum(pe_max, err.amax(dim=(0, 1)))
+        pe_sq += (err ** 2).sum(dim=(0, 1))
+        ra = refb.abs()
+        pr_max = torch.maximum(pr_max, ra.amax(dim=(0, 1)))
+        pr_sq += (ra ** 2).sum(dim=(0, 1))
+        ms_buf[r0:r1] = refb.to(ms_buf.dtype)
+        del uvl, ref, refb, err, ra
+    del vis
+    torch.cuda.empty_cache()
+    plan_i = {}
+    dirty = wgridder.ms2dirty(uvw_d, freq, ms_buf, wgt_d, N, N, cell, cell, 0, 0, eps, True, precision=prec,
+                              plan_out=plan_i)
+    got = dirty[:, ixt, iyt].double()  # [nb, nsrc]
+    del dirty
+    exact = torch.zeros((nb, nsrc), dtype=torch.float64, device=dev)
+    dcs = dc.cpu().numpy()
+    for r0 in range(0, nrow, chunk_rows):
+        r1 = min(nrow, r0 + chunk_rows)
+        uvl = (uvw_d[r0:r1] * sgn)[:, None, :] * fc[None, :, None]
+        V = ms_buf[r0:r1].to(torch.complex128)
+        if wgt_d is not None:
+            V = V * wgt_d[r0:r1].double()
+        for k in range(nsrc):
+            ph = uvl[..., 0] * dcs[k, 0] + uvl[..., 1] * dcs[k, 1] + uvl[..., 2] * dcs[k, 2]
+            e = torch.polar(torch.ones_like(ph), 2 * numpy.pi * ph)
+            exact[:, k] += (V * e[..., None]).real.sum(dim=(0, 1))
+        del uvl, V
+    exact = exact / torch.tensor(n, dtype=torch.float64, device=dev)
+    e2 = (got - exact).abs()
+    nv = nrow * nchan
+
+    def _res(emax, esq, rmax, rsq):
+        return {"max_rel": float(emax.max() / rmax.max()), "rms_rel": float(torch.sqrt(esq.sum() / rsq.sum())),
+                "per_pol_max_rel": [float(a) for a in (emax / rmax).cpu()]}
+
+    keys = ("W", "sigma", "gx", "gy", "nplanes", "tile", "dense_min", "dense_items")
+    same = all(plan_i.get(k) == timed_plans[0].get(k) for k in keys) and all(
+        plan_p.get(k) == timed_plans[1].get(k) for k in keys)
+    out = {"visibilities": int(nv * nb), "rows": int(nrow), "sources": nsrc, "pol_scales": scale.tolist(),
+           "plan": {k: plan_i.get(k) for k in keys}, "plan_is_timed_plan": bool(same),
+           "predict": _res(pe_max, pe_sq, pr_max, pr_sq),
+           "invert": {"max_rel": float(e2.max() / exact.abs().max()),
+                      "rms_rel": float(torch.sqrt((e2 ** 2).mean() / (exact ** 2).mean())),
+                      "per_pol_max_rel": [float(a) for a in (e2.amax(dim=1) / exact.abs().amax(dim=1)).cpu()]},
+           "seconds": round(time.time() - t_start, 1)}
+    out["meets_epsilon"] = bool(out["predict"]["max_rel"] <= eps and out["invert"]["max_rel"] <= eps and same)
     return out
+
+
+def path_roofline(plan_i, plan_p, nrow, nchan, nb, prec, with_wgt, ms_per_step):
+    """Whole-path floor of one invert + predict by the work model of the scope table:
+    T_roof = max(HBM bytes / 8 TB/s, FLOP / peak) with
+    FLOP = per direction N_vis (4 W^2 W_w + 3 W 10) + N_pol N_planes 5 G^2 log2 G^2, and bytes =
+    visibilities read (+ weights) and written, uvw read per direction, the sorted index
+    written and read per direction (one per row-channel item, shared by the polarisations),
+    N_pol N_planes G^2 b (2 p_fft + 2) per direction for the plane FFTs (p_fft = 2) and the
+    screen, plus the image written / read."""
+    bv = 16 if prec == "fp64" else 8
+    bw = 8 if prec == "fp64" else 4
+    br = 8 if prec == "fp64" else 4
+    items = nrow * nchan
+    nvis = items * nb
+    flop = 0.0
+    nbytes = nvis * (bv + (bw if with_wgt else 0)) + nvis * bv + 2 * nrow * 24 + 2 * 2 * items * 4
+    for P in (plan_i, plan_p):
+        W = P["W"]
+        ww = W if P["wstacking"] else 1
+        G2 = P["gx"] * P["gy"]
+        flop += nvis * (4.0 * W * W * ww + 30.0 * W) + nb * P["nplanes"] * 5.0 * G2 * math.log2(G2)
+        nbytes += nb * P["nplanes"] * G2 * bv * (2 * 2 + 2) + nb * P["nx"] * P["ny"] * br
+    peak = (FP64_PEAK_TFLOPS if prec == "fp64" else FP32_PEAK_TFLOPS) * 1e12
+    t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
+    t_flop = flop / peak
+    t_roof = max(t_hbm, t_flop)
+    return {"bound": "hbm" if t_hbm >= t_flop else "flop", "hbm_bytes": int(nbytes), "flop": int(flop),
+            "t_hbm_ms": round(t_hbm * 1e3, 1), "t_flop_ms": round(t_flop * 1e3, 1),
+            "t_roof_ms": round(t_roof * 1e3, 1), "ms_per_step": round(ms_per_step, 1),
+            "frac": round(t_roof * 1e3 / ms_per_step, 4)}
 
 
 def precision_study(seed=3):
@@ -915,8 +990,6 @@ def main():
     if args.copy_probe:
         print(json.dumps(copy_probe()), flush=True)
         return
-    if args.sigma:
-        os.environ["HIPNG_SIGMA"] = str(args.sigma)
     if args.study:
         fn = {"precision": precision_study, "autograd": autograd_study, "batched": batched_study,
               "weighting": weighting_study, "gains": gains_study, "vis_passes": vis_passes_study,
@@ -939,6 +1012,7 @@ def main():
     prec = args.precision if args.config == "c1" else "fp64"
     nrow, nchan, npol, N = wl["uvw"].shape[0], len(wl["freq"]), wl["npol"], wl["npix"]
     nvis = nrow * nchan * npol
+    opts = {"sigma": args.sigma} if args.sigma else None
     print(f"[bench] {wl['name']}: {nrow} rows x {nchan} ch x {npol} pol = {nvis:.3e} vis, "
           f"{N}^2, precision {prec}", file=sys.stderr, flush=True)
     cdt = torch.complex128 if prec == "fp64" else torch.complex64
@@ -947,20 +1021,23 @@ def main():
     g.manual_seed(1234)
     uvw_d = torch.from_numpy(numpy.ascontiguousarray(wl["uvw"])).to(dev)
     vis_d = torch.randn((nrow, nchan, npol), dtype=cdt, device=dev, generator=g)
+    # imaging weights for the invert, as RASCIL passes them (ng.py:232-248); predict has none
+    wgt_d = torch.empty((nrow, nchan, npol), dtype=rdt, device=dev).uniform_(0.5, 1.5, generator=g)
     model_d = torch.randn((npol, N, N), dtype=rdt, device=dev, generator=g)
     torch.cuda.synchronize()
     plan_i, plan_p = {}, {}
 
-    def step(p, bc=None):
-        # unit weights: wgt=None is the same sum without reading a weight array
-        wgridder.ms2dirty(uvw_d, wl["freq"], vis_d, None, N, N, wl["cell"], wl["cell"], 0, 0,
-                          wl["eps"], True, precision=p, plan_out=plan_i, batch_chunk=bc)
+    def step(p, pi=None, pp=None):
+        wgridder.ms2dirty(uvw_d, wl["freq"], vis_d, wgt_d, N, N, wl["cell"], wl["cell"], 0, 0,
+                          wl["eps"], True, precision=p, plan_out=pi, plan_opts=opts)
         wgridder.dirty2ms(uvw_d, wl["freq"], model_d, None, wl["cell"], wl["cell"], 0, 0,
-                          wl["eps"], True, precision=p, plan_out=plan_p, batch_chunk=bc)
+                          wl["eps"], True, precision=p, plan_out=pp, plan_opts=opts)
 
-    for i in range(args.warmup):
+    for i in range(max(1, args.warmup)):
         t = time.time()
-        step(prec)
+        last = i == max(1, args.warmup) - 1
+        # the plans are read back on the last warm-up step only (that costs a device sync)
+        step(prec, plan_i if last else None, plan_p if last else None)
         torch.cuda.synchronize()
         print(f"[bench] warmup {i} {time.time() - t:.2f} s", file=sys.stderr, flush=True)
     _lib.profile_enable(True)
@@ -997,9 +1074,12 @@ def main():
             continue
         avg_s = tot / cnt * 1e-3
         traffic, src = measured_traffic(name, plan, prec, npol)
-        hbm_view = {"achieved": round(nbytes / per_step / avg_s / 1e9, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "algorithmic_bytes_per_launch": int(nbytes / per_step)}
+        alg_bytes = nbytes / per_step
+        hbm_view = {"achieved": round(alg_bytes / avg_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "algorithmic_bytes_per_launch": int(alg_bytes)}
         hbm_view["frac"] = round(hbm_view["achieved"] / HBM_PEAK_GBS, 4)
+        if traffic:
+            hbm_view["traffic_over_algorithmic"] = round(traffic / alg_bytes, 2)
         if flop is not None:
             # FP32 modes accumulate on the matrix cores (v_mfma_f32_16x16x4_f32, dense FP32
             # matrix peak); FP64 gridding is vector FMA (dense FP64 peak)
@@ -1027,24 +1107,25 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {"fp64": "f64", "mixed": "f32", "fp32": "f32"}[prec], "data": "synthetic",
-        "config": {"workload": wl["desc"], "nvis": nvis, "precision": prec,
+        "config": {"workload": wl["desc"], "nvis": nvis, "precision": prec, "invert_weights": True,
                    "W": plan_i.get("W"), "sigma": plan_i.get("sigma"), "grid": plan_i.get("gx"),
                    "nplanes": plan_i.get("nplanes"), "tile": plan_i.get("tile"),
                    "dense_min": plan_i.get("dense_min"),
                    "dense_fraction": round(plan_i.get("dense_items", 0) / max(1, nrow * nchan), 4)},
         "roofline": roof,
+        "path_roofline": path_roofline(plan_i, plan_p, nrow, nchan, npol, prec, True, ms_per_step),
         "kernel_ms_total": breakdown,
     }
 
     if not args.no_accuracy:
         try:
-            out["accuracy"] = accuracy_probe(wl, prec)
+            out["accuracy"] = accuracy_at_timed_plan(wl, prec, uvw_d, vis_d, wgt_d, (plan_i, plan_p))
             print(f"[bench] accuracy {out['accuracy']}", file=sys.stderr, flush=True)
         except Exception as e:
             out["accuracy"] = {"error": repr(e)}
 
     if args.fp64_steps > 0 and prec != "fp64":
-        del vis_d, model_d
+        del vis_d, model_d, wgt_d
         torch.cuda.empty_cache()
         _lib.load().hipng_release()
         # FP64 buffers of all four polarisations at once do not fit next to the FP64 records in
@@ -1053,34 +1134,50 @@ def main():
         gp = npol // ngrp
         vis64 = [torch.randn((nrow, nchan, gp), dtype=torch.complex128, device=dev, generator=g)
                  for _ in range(ngrp)]
+        wgt64 = torch.empty((nrow, nchan, gp), dtype=torch.float64, device=dev).uniform_(0.5, 1.5, generator=g)
         mod64 = [torch.randn((gp, N, N), dtype=torch.float64, device=dev, generator=g)
                  for _ in range(ngrp)]
+        p64 = ({}, {})
 
-        def step64():
+        def step64(pi=None, pp=None):
             for v64, m64 in zip(vis64, mod64):
-                wgridder.ms2dirty(uvw_d, wl["freq"], v64, None, N, N, wl["cell"], wl["cell"], 0, 0,
-                                  wl["eps"], True, precision="fp64", plan_out=plan_i)
+                wgridder.ms2dirty(uvw_d, wl["freq"], v64, wgt64, N, N, wl["cell"], wl["cell"], 0, 0,
+                                  wl["eps"], True, precision="fp64", plan_out=pi)
                 wgridder.dirty2ms(uvw_d, wl["freq"], m64, None, wl["cell"], wl["cell"], 0, 0,
-                                  wl["eps"], True, precision="fp64", plan_out=plan_p)
+                                  wl["eps"], True, precision="fp64", plan_out=pp)
 
-        step64()
+        step64(*p64)
         torch.cuda.synchronize()
         t = time.time()
-        for _ in range(args.fp64_steps):
+        for i in range(args.fp64_steps):
             step64()
+            print(f"[bench] fp64 step {i} done", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         ms64 = (time.time() - t) * 1e3 / args.fp64_steps
         out["fp64_plain"] = {"ms_per_step": round(ms64, 1), "value": round(nvis / (ms64 * 1e-3) / 1e6, 2),
                              "speedup_of_fp32_mode": round(ms64 / ms_per_step, 2),
-                             "jobs": ngrp, "W": plan_i.get("W"), "grid": plan_i.get("gx"),
-                             "nplanes": plan_i.get("nplanes")}
+                             "jobs": ngrp, "pols_per_job": gp, "W": p64[0].get("W"), "sigma": p64[0].get("sigma"),
+                             "grid": p64[0].get("gx"), "nplanes": p64[0].get("nplanes")}
+        if not args.no_accuracy:
+            try:
+                del vis64[1:], mod64
+                torch.cuda.empty_cache()
+                out["fp64_plain"]["accuracy"] = accuracy_at_timed_plan(wl, "fp64", uvw_d, vis64[0], wgt64, p64)
+                print(f"[bench] fp64 accuracy {out['fp64_plain']['accuracy']}", file=sys.stderr, flush=True)
+            except Exception as e:
+                out["fp64_plain"]["accuracy"] = {"error": repr(e)}
+        del vis64, wgt64
+        torch.cuda.empty_cache()
+        _lib.load().hipng_release()
 
     if not args.no_cpu:
         try:
             from oracle import wstack_cpu
 
+            # the small configuration runs in full; the production shape on 1 % of its rows
             out["cpu_baseline"] = wstack_cpu.cpu_baseline(wl, plan_i, plan_p, args.cpu_seconds,
-                                                          plans=cpu_plans(plan_i, npol, nrow * nchan))
+                                                          plans=cpu_plans(plan_i, npol, nrow * nchan),
+                                                          frac=1.0 if nvis < 1e7 else 0.01)
         except Exception as e:  # the baseline leg must not hide the GPU result
             out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)

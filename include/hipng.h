@@ -68,7 +68,27 @@ typedef struct hipng_plan {
   int32_t reserved;
   int64_t dense_items; /* visibilities (rows x channels) on the dense-run path; filled in
                           plan_out by the compute calls (one extra device synchronisation) */
+  int64_t occupied_tiles; /* (w plane, uv tile) pairs some visibility footprint touches, i.e.
+                             the grid cells (x tile^2) gridding must write; filled like
+                             dense_items */
 } hipng_plan;
+
+/* Plan options: every field 0 means "automatic" (the library's own choice). They pin a plan
+ * for tests, studies and autotuning without touching the process environment; a pinned plan
+ * still has to reach epsilon in its precision mode (HIPNG_EINVAL otherwise). Pass NULL for
+ * all-automatic. No counterpart in ducc0 0.24 (which has no plan overrides). */
+typedef struct hipng_plan_opts {
+  double sigma;         /* oversampling factor: 1.25, 1.5, 1.75, 2 or 2.5 */
+  int32_t W;            /* kernel support 4..16 (at least what epsilon needs) */
+  int32_t tile;         /* uv tile side: 16 or 32 */
+  int32_t dense_min;    /* shortest run the dense-run path takes (>= 2); -1: path off */
+  int32_t chunk_grid;   /* candidates per gridding task (rounded down to a multiple of 256) */
+  int32_t chunk_degrid; /* visibilities per degridding task (likewise) */
+  int32_t sort_pack;    /* -1: recompute the bucket keys after the digit passes (k_rekey) */
+  int32_t use_extent;   /* 1: plan for at least the extent below (the union with the data's) */
+  int32_t reserved;
+  double umax, vmax, wmin, wmax; /* wavelengths, after the w >= 0 hermitian flip */
+} hipng_plan_opts;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int32_t hipng_version(void);
@@ -79,11 +99,16 @@ const char* hipng_last_error(void);
 /* Host-only: choose W, beta, oversampling, grid, tile and w planes for the given uvw
  * extents (wavelengths, after the w >= 0 hermitian flip), image and number of
  * visibilities nvis = nrow * nchan (the cost model weighs gridding against FFTs).
- * The compute calls make the same choice internally. No GPU use. */
+ * The compute calls make the same choice internally. No GPU use.
+ * Accuracy contract (like ducc0, which rejects an epsilon its data type cannot reach):
+ * HIPNG_EINVAL unless epsilon >= 1e-13 (HIPNG_FP64), >= 1e-5 (HIPNG_MIXED: FP32 kernel,
+ * grid and FFT) or, for HIPNG_FP32, >= max(1e-5, the phase error of FP32 grid coordinates
+ * at the image edge, 2 pi (2 |u| + ulp(u_max)/2) (npix/2) / grid, in cells), so that an
+ * accepted epsilon is met. est_error in the plan is the modelled total error. */
 int32_t hipng_plan_params(double umax, double vmax, double wmin, double wmax,
                           int32_t npix_x, int32_t npix_y, double pixsize_x, double pixsize_y,
                           double epsilon, int32_t do_wstacking, int32_t precision,
-                          int32_t nbatch, int64_t nvis, hipng_plan* out);
+                          int32_t nbatch, int64_t nvis, const hipng_plan_opts* opts, hipng_plan* out);
 
 /* Host-only: per-channel scale factors the coordinate kernel uses (bit-exact contract
  * with the CPU reference sort): sx[c] = f_c/c0 * pixsize_x * gx, sy[c] likewise,
@@ -102,18 +127,19 @@ int32_t hipng_uvw_extent(const double* uvw, int64_t nrow, const double* freq, in
  * sorted order, perm_out[j] its original index row*nchan + chan. Stable: equal
  * (key, digit) pairs keep increasing original index. */
 int32_t hipng_bucket_sort(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
-                          const hipng_plan* plan, uint32_t* keys_out, uint32_t* perm_out,
-                          void* stream);
+                          const hipng_plan* plan, const hipng_plan_opts* opts, uint32_t* keys_out,
+                          uint32_t* perm_out, void* stream);
 
 /* Device: adjoint NUFFT with w-stacking (ducc0.wgridder.ms2dirty semantics, see above).
  * uvw [nrow][3] f64 (device); freq [nchan] f64 (host); ms [nrow][nchan][nbatch] complex
  * (c16 for HIPNG_FP64, c8 otherwise); wgt same layout, real, or NULL; dirty out
- * [nbatch][npix_x][npix_y] real. plan_out (host, nullable) receives the parameters used. */
+ * [nbatch][npix_x][npix_y] real. opts (host, nullable) pins plan choices; plan_out (host,
+ * nullable) receives the parameters used. */
 int32_t hipng_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
                        const void* ms, const void* wgt, int32_t nbatch,
                        int32_t npix_x, int32_t npix_y, double pixsize_x, double pixsize_y,
                        double epsilon, int32_t do_wstacking, int32_t precision,
-                       void* dirty, void* stream, hipng_plan* plan_out);
+                       void* dirty, void* stream, const hipng_plan_opts* opts, hipng_plan* plan_out);
 
 /* Device: forward NUFFT with w-stacking (ducc0.wgridder.dirty2ms semantics).
  * dirty [nbatch][npix_x][npix_y] real in; ms out [nrow][nchan][nbatch] complex; wgt as for
@@ -122,7 +148,7 @@ int32_t hipng_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int3
                        const void* dirty, const void* wgt, int32_t nbatch,
                        int32_t npix_x, int32_t npix_y, double pixsize_x, double pixsize_y,
                        double epsilon, int32_t do_wstacking, int32_t precision,
-                       void* ms, void* stream, hipng_plan* plan_out);
+                       void* ms, void* stream, const hipng_plan_opts* opts, hipng_plan* plan_out);
 
 /* Reusable geometry ("plan once, execute many"): everything that depends only on uvw, freq,
  * the image, epsilon, precision and nbatch (plan, bucket sort, coordinate records, FFT plans,
@@ -136,8 +162,8 @@ typedef struct hipng_geometry_s* hipng_geometry;
 int32_t hipng_geometry_create(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
                               int32_t nbatch, int32_t npix_x, int32_t npix_y, double pixsize_x,
                               double pixsize_y, double epsilon, int32_t do_wstacking,
-                              int32_t precision, void* stream, hipng_geometry* out,
-                              hipng_plan* plan_out);
+                              int32_t precision, void* stream, const hipng_plan_opts* opts,
+                              hipng_geometry* out, hipng_plan* plan_out);
 int32_t hipng_geometry_ms2dirty(hipng_geometry geom, const void* ms, const void* wgt, void* dirty,
                                 void* stream);
 int32_t hipng_geometry_dirty2ms(hipng_geometry geom, const void* dirty, const void* wgt, void* ms,
@@ -146,8 +172,10 @@ int32_t hipng_geometry_destroy(hipng_geometry geom);
 
 /* Diagnostics (no counterpart in the reference): out[0] = visibilities (rows x channels),
  * out[1] = dense-run tasks, out[2] = visibilities gridded by the dense-run path, out[3] = 1
- * if that path is enabled for this geometry (W <= 8, grid <= 32768 per axis). Writes
- * min(nout, 4) entries; synchronises the device. */
+ * if that path is enabled for this geometry (W <= 8, grid <= 32768 per axis), out[4] = 1 if
+ * every out-of-place rocFFT plan of the geometry was found to leave its input untouched
+ * (checked once per plan on a pattern; 0: the buffers they read are cleared before every
+ * pass instead of once per call). Writes min(nout, 5) entries; synchronises the device. */
 int32_t hipng_geometry_stats(hipng_geometry geom, int64_t* out, int32_t nout);
 
 /* Device: FP64 point-source DFT, replacing dft_kernel / dft_cpu_looped

@@ -110,6 +110,15 @@ POL_MATRICES = {
     ("circular", "stokesI"): numpy.array([[0.5, 0, 0, 0.5]]),
     ("stokesIQUV", "stokesI"): numpy.array([[1, 0, 0, 0]]),
     ("stokesI", "stokesIQUV"): numpy.array([[1], [0], [0], [0]]),
+    # two-polarisation frames (polarisation.py:229-231, 250-256, 275-276, 290-292, 310-316,
+    # 336-337 and the dispatch :379-409): stokesIQ <-> linearnp (XX, YY) and stokesIV <->
+    # circularnp (RR, LL), and their reductions to Stokes I
+    ("stokesIQ", "linearnp"): numpy.array([[1, 1], [1, -1]]),
+    ("linearnp", "stokesIQ"): numpy.array([[0.5, 0.5], [0.5, -0.5]]),
+    ("stokesIV", "circularnp"): numpy.array([[1, 1], [1, -1]]),
+    ("circularnp", "stokesIV"): numpy.array([[0.5, 0.5], [0.5, -0.5]]),
+    ("linearnp", "stokesI"): numpy.array([[0.5, 0.5]]),
+    ("circularnp", "stokesI"): numpy.array([[0.5, 0.5]]),
 }
 
 

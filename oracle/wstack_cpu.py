@@ -235,42 +235,49 @@ def _time_planes(P, npol, plane, nthreads):
     return t_inv, t_fwd
 
 
-def cpu_baseline(wl, plan_i, plan_p, seconds=20.0, plans=None):
+def cpu_baseline(wl, plan_i, plan_p, seconds=20.0, plans=None, frac=0.01):
     """Time the CPU port on a bounded sample of the bench workload and extrapolate.
 
-    For each candidate plan (the GPU's and, when given, other oversampling factors; the
-    fastest one is reported, as a CPU gridder would pick its own parameters): the
-    row-proportional part (sort, grid, degrid through every plane) is timed at two row
-    subsamples and extrapolated linearly to all rows; the per-plane part (one full plane
-    of each direction: fill, FFT, screen) is timed once and multiplied by the plane count.
-    """
+    The candidate plans (the GPU's and, when given, other oversampling factors; a CPU
+    gridder would pick its own parameters) are compared on a small row sample; the fastest
+    one is then timed on `frac` / 2 and `frac` of the rows (sort, grid and degrid through
+    every plane), and the row-proportional time is the line through those two points
+    extended to all rows. The per-plane part (one full plane of each direction: fill, FFT,
+    screen) is timed once and multiplied by the plane count. `seconds` bounds the small
+    sample (rows x channels x polarisations of about 1.5e5 visibilities each)."""
     nthreads = _threads()
     uvw, freq, npol = wl["uvw"], wl["freq"], wl["npol"]
     nrow, nchan = uvw.shape[0], len(freq)
     rng = numpy.random.default_rng(0)
-    r2 = max(2, min(nrow, int(1.5e5 / (nchan * npol)) + 1))
-    r1 = max(1, r2 // 3)
     perm = rng.permutation(nrow)
+    rs = max(2, min(nrow, int(1.5e5 / (nchan * npol)) + 1))
     cands = [(plan_i, plan_p)] + list(plans or [])
     best = None
     for P, Pp in cands:
-        t1, _ = _time_rows(uvw, freq, npol, P, Pp, numpy.sort(perm[:r1]))
-        t2, plane = _time_rows(uvw, freq, npol, P, Pp, numpy.sort(perm[:r2]))
-        slope = max(0.0, (t2 - t1) / (r2 - r1))
-        t_rows = t2 + slope * (nrow - r2)
+        ts, plane = _time_rows(uvw, freq, npol, P, Pp, numpy.sort(perm[:rs]))
         t_inv, t_fwd = _time_planes(P, npol, plane, nthreads)
         del plane
-        total = t_rows + t_inv * P["nplanes"] + t_fwd * Pp["nplanes"]
-        desc = (f"sigma {P['sigma']} W {P['W']} grid {P['gx']} planes {P['nplanes']}: rows {r1}/{r2} "
-                f"of {nrow} x {nchan} ch x {npol} pol through all planes ({t1:.1f}/{t2:.1f} s, linear "
-                f"fit -> {t_rows:.0f} s) + one {P['gx']}^2 x {npol} plane per direction ({t_inv:.1f} + "
-                f"{t_fwd:.1f} s, x{P['nplanes']}) = {total:.0f} s")
-        if best is None or total < best[0]:
-            best = (total, desc)
+        est = ts * nrow / rs + t_inv * P["nplanes"] + t_fwd * Pp["nplanes"]
+        if best is None or est < best[0]:
+            best = (est, P, Pp, t_inv, t_fwd)
+    _, P, Pp, t_inv, t_fwd = best
+    r2 = max(2, min(nrow, int(round(frac * nrow))))
+    r1 = max(1, r2 // 2)
+    t1, _ = _time_rows(uvw, freq, npol, P, Pp, numpy.sort(perm[:r1]))
+    t2, plane = _time_rows(uvw, freq, npol, P, Pp, numpy.sort(perm[:r2]))
+    del plane
+    slope = max(0.0, (t2 - t1) / (r2 - r1))
+    t_rows = t2 + slope * (nrow - r2)
+    total = t_rows + t_inv * P["nplanes"] + t_fwd * Pp["nplanes"]
     nvis = nrow * nchan * npol
+    desc = (f"CPU port of the same algorithm (FP64, OpenMP C + scipy.fft; ducc0 is not installed), "
+            f"fastest of {len(cands)} plans on a {rs}-row sample: sigma {P['sigma']} W {P['W']} grid {P['gx']} "
+            f"planes {P['nplanes']}; rows {r1} and {r2} ({100.0 * r2 / nrow:.2f} % of {nrow}) x {nchan} ch x "
+            f"{npol} pol through all planes ({t1:.1f} / {t2:.1f} s, linear fit -> {t_rows:.0f} s) + one "
+            f"{P['gx']}^2 x {npol} plane per direction ({t_inv:.1f} + {t_fwd:.1f} s, x{P['nplanes']}) = {total:.0f} s")
     return {
-        "value": round(nvis / best[0] / 1e6, 4), "unit": "Mvis/s", "cores": nthreads, "kind": "port",
-        "sample": ("CPU port of the same algorithm (FP64, OpenMP C + scipy.fft; ducc0 is not "
-                   "installed), fastest of the plans tried: " + best[1]),
-        "extrapolated_seconds": round(best[0], 1),
+        "value": round(nvis / total / 1e6, 4), "unit": "Mvis/s", "cores": nthreads, "kind": "port",
+        "sample": desc, "extrapolated_seconds": round(total, 1),
+        "fit": {"rows": [int(r1), int(r2)], "seconds": [round(t1, 2), round(t2, 2)],
+                "fraction_of_rows": round(r2 / nrow, 4), "plane_seconds": [round(t_inv, 2), round(t_fwd, 2)]},
     }

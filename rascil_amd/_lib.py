@@ -74,10 +74,62 @@ class HipngPlan(ctypes.Structure):
         ("dense_min", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("dense_items", ctypes.c_int64),
+        ("occupied_tiles", ctypes.c_int64),
     ]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class HipngPlanOpts(ctypes.Structure):
+    """Mirror of ``hipng_plan_opts`` in include/hipng.h: plan choices pinned by the caller
+    (0 = automatic)."""
+
+    _fields_ = [
+        ("sigma", ctypes.c_double),
+        ("W", ctypes.c_int32),
+        ("tile", ctypes.c_int32),
+        ("dense_min", ctypes.c_int32),
+        ("chunk_grid", ctypes.c_int32),
+        ("chunk_degrid", ctypes.c_int32),
+        ("sort_pack", ctypes.c_int32),
+        ("use_extent", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("umax", ctypes.c_double),
+        ("vmax", ctypes.c_double),
+        ("wmin", ctypes.c_double),
+        ("wmax", ctypes.c_double),
+    ]
+
+    FIELDS = ("sigma", "W", "tile", "dense_min", "chunk_grid", "chunk_degrid", "sort_pack", "extent")
+
+
+def plan_opts(opts):
+    """ctypes pointer to a HipngPlanOpts built from a dict (or None for all-automatic).
+
+    Keys: sigma, W, tile, dense_min (-1 turns the dense-run path off), chunk_grid,
+    chunk_degrid, sort_pack (False recomputes the bucket keys), extent (umax, vmax, wmin,
+    wmax) in wavelengths that the plan covers at least."""
+    if opts is None:
+        return None
+    if isinstance(opts, HipngPlanOpts):
+        return ctypes.byref(opts)
+    unknown = set(opts) - set(HipngPlanOpts.FIELDS)
+    if unknown:
+        raise ValueError(f"unknown plan options {sorted(unknown)}; use {list(HipngPlanOpts.FIELDS)}")
+    o = HipngPlanOpts()
+    for k in ("sigma",):
+        if opts.get(k) is not None:
+            o.sigma = float(opts[k])
+    for k in ("W", "tile", "dense_min", "chunk_grid", "chunk_degrid"):
+        if opts.get(k) is not None:
+            setattr(o, k, int(opts[k]))
+    if opts.get("sort_pack") is not None and not opts["sort_pack"]:
+        o.sort_pack = -1
+    if opts.get("extent") is not None:
+        o.use_extent = 1
+        o.umax, o.vmax, o.wmin, o.wmax = (float(x) for x in opts["extent"])
+    return ctypes.byref(o)
 
 
 _lib = None
@@ -105,24 +157,25 @@ def load():
         c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
         pd = ctypes.POINTER(ctypes.c_double)
         pplan = ctypes.POINTER(HipngPlan)
+        popts = ctypes.POINTER(HipngPlanOpts)
         lib.hipng_version.restype = c_i32
         lib.hipng_last_error.restype = ctypes.c_char_p
         lib.hipng_plan_params.argtypes = [c_dbl, c_dbl, c_dbl, c_dbl, c_i32, c_i32, c_dbl, c_dbl,
-                                          c_dbl, c_i32, c_i32, c_i32, c_i64, pplan]
+                                          c_dbl, c_i32, c_i32, c_i32, c_i64, popts, pplan]
         lib.hipng_plan_params.restype = c_i32
         lib.hipng_channel_scales.argtypes = [pplan, pd, c_i32, pd, pd, pd]
         lib.hipng_channel_scales.restype = c_i32
         lib.hipng_uvw_extent.argtypes = [c_vp, c_i64, pd, c_i32, pd, c_vp]
         lib.hipng_uvw_extent.restype = c_i32
-        lib.hipng_bucket_sort.argtypes = [c_vp, pd, c_i64, c_i32, pplan, c_vp, c_vp, c_vp]
+        lib.hipng_bucket_sort.argtypes = [c_vp, pd, c_i64, c_i32, pplan, popts, c_vp, c_vp, c_vp]
         lib.hipng_bucket_sort.restype = c_i32
         for name in ("hipng_ms2dirty", "hipng_dirty2ms"):
             fn = getattr(lib, name)
             fn.argtypes = [c_vp, pd, c_i64, c_i32, c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_dbl,
-                           c_dbl, c_i32, c_i32, c_vp, c_vp, pplan]
+                           c_dbl, c_i32, c_i32, c_vp, c_vp, popts, pplan]
             fn.restype = c_i32
         lib.hipng_geometry_create.argtypes = [c_vp, pd, c_i64, c_i32, c_i32, c_i32, c_i32, c_dbl,
-                                              c_dbl, c_dbl, c_i32, c_i32, c_vp,
+                                              c_dbl, c_dbl, c_i32, c_i32, c_vp, popts,
                                               ctypes.POINTER(c_vp), pplan]
         lib.hipng_geometry_create.restype = c_i32
         lib.hipng_geometry_ms2dirty.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp]

@@ -190,18 +190,66 @@ int bits_for(uint64_t v) {
   return b;
 }
 
+// Mixed and FP32 modes (FP32 kernel, accumulation and FFTs) refuse epsilon below 1e-5, like
+// ducc0 for single precision: the measured floor at 8192^2 is 1.7e-6 to 3.2e-6.
+constexpr double kMixedEpsMin = 1e-5;
+
+// FP32 mode: a grid coordinate fmaf((float)u, (float)s, G/2) carries up to 2^-24 |u s| of
+// error from each rounded operand plus half an ulp of the result (cells); a point source at
+// the image edge, npix/2 pixels out, sees 2 pi delta (npix/2) / G radians of phase error.
+// The w coordinate (planes) works the same way with plane spacing dw = 1 / (2 sigma h),
+// h = max |(n-1) - nshift|: 2 pi delta_w dw h = pi delta_w / sigma.
+double fp32_coord_error(double cells, double top) {
+  const double ulp = top > 0.0 ? std::ldexp(1.0, std::ilogb(top) - 23) : 0.0;
+  return 2.0 * cells * std::ldexp(1.0, -24) + 0.5 * ulp;
+}
+
+// Roundoff of the plane transforms, amplified by the grid correction. The transforms' rounding
+// (relative eps_r of a plane) lands evenly on the image, which is then divided by
+// Phi_x Phi_y: at the image corner that is D = Phi(0)^2 / (Phi(nx/2/gx) Phi(ny/2/gy)) times
+// the centre's gain, large for low oversampling (D ~ 1190 at sigma 1.25, W 8; 64 at 1.5, W 8;
+// 4.7 at 1.75, W 6). Measured with FP32 planes: 1.0-1.2 eps_r D of extra relative L2 error on
+// noise-like images (60 snapshot plans, sigma 1.25-2.5) and ~3e-6 (D = 64) at the
+// 4.29e9-visibility 8192^2 plan; 1.25 eps_r D is used.
+double transform_floor(int W, double beta, int nx, int ny, int gx, int gy, double eps_r) {
+  KernelFT ft;
+  ft.init(W, beta);
+  const double p0 = ft(0.0, W);
+  const double dx = p0 / ft(0.5 * nx / gx, W), dy = p0 / ft(0.5 * ny / gy, W);
+  return 1.25 * eps_r * dx * dy;
+}
+
 void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int ny, double px,
                double py, double eps, int wstack, int prec, int nbatch, int64_t nitems,
-               hipng_plan* P) {
+               const hipng_plan_opts* opts, hipng_plan* P) {
   if (nx <= 0 || ny <= 0 || (nx & 1) || (ny & 1))
     HIPNG_THROW(HIPNG_EINVAL, "npix_x and npix_y must be positive and even (got %d, %d)", nx, ny);
   if (!(px > 0) || !(py > 0)) HIPNG_THROW(HIPNG_EINVAL, "pixel sizes must be positive");
   if (prec < 0 || prec > 2) HIPNG_THROW(HIPNG_EINVAL, "unknown precision mode %d", prec);
   if (nbatch < 1 || nbatch > kMaxBatch)
     HIPNG_THROW(HIPNG_EINVAL, "nbatch must be in [1, %d] (got %d)", kMaxBatch, nbatch);
-  const double epsmin = prec == HIPNG_FP64 ? 1e-13 : 1e-7;
+  hipng_plan_opts o;
+  std::memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  if (o.sigma != 0.0) {
+    bool known = false;
+    for (double sg : kSigmas) known = known || std::fabs(sg - o.sigma) < 1e-9;
+    if (!known) HIPNG_THROW(HIPNG_EINVAL, "opts.sigma %g is not one of 1.25, 1.5, 1.75, 2, 2.5", o.sigma);
+  }
+  if (o.W != 0 && (o.W < kWMin || o.W > kWMax)) HIPNG_THROW(HIPNG_EINVAL, "opts.W %d outside [%d, %d]", o.W, kWMin, kWMax);
+  if (o.tile != 0 && o.tile != 16 && o.tile != 32) HIPNG_THROW(HIPNG_EINVAL, "opts.tile must be 16 or 32 (got %d)", o.tile);
+  if (o.use_extent) {
+    if (!(o.umax >= 0.0) || !(o.vmax >= 0.0) || !(o.wmin >= 0.0) || !(o.wmax >= o.wmin))
+      HIPNG_THROW(HIPNG_EINVAL, "opts extent must satisfy umax, vmax >= 0 and 0 <= wmin <= wmax");
+    umax = std::max(umax, o.umax);
+    vmax = std::max(vmax, o.vmax);
+    wmin = nitems > 0 ? std::min(wmin, o.wmin) : o.wmin;
+    wmax = std::max(wmax, o.wmax);
+  }
+  const double epsmin = prec == HIPNG_FP64 ? 1e-13 : kMixedEpsMin;
   if (!(eps >= epsmin) || !(eps < 1.0))
-    HIPNG_THROW(HIPNG_EINVAL, "epsilon %g outside [%g, 1) for this precision", eps, epsmin);
+    HIPNG_THROW(HIPNG_EINVAL, "epsilon %g outside [%g, 1) for %s precision", eps, epsmin,
+                prec == HIPNG_FP64 ? "fp64" : prec == HIPNG_MIXED ? "mixed" : "fp32");
   std::memset(P, 0, sizeof(*P));
   P->precision = prec;
   P->wstacking = wstack ? 1 : 0;
@@ -227,43 +275,62 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
   const size_t csz = prec == HIPNG_FP64 ? 16 : 8;
   double best_cost = 1e300;
   bool lds_limited = false;
-  // HIPNG_SIGMA restricts the search to one oversampling factor (cost-model calibration)
-  const char* force = std::getenv("HIPNG_SIGMA");
-  const double forced = force ? std::atof(force) : 0.0;
-  // HIPNG_TILE=32 selects 32 x 32 sort tiles where they fit (tile-size calibration); 16 x 16
-  // tiles are the default: the sub-tile gridder scans a tile's candidates once per 8 x 8
-  // sub-tile, and a smaller tile has fewer candidates per sub-tile
-  const char* ftile = std::getenv("HIPNG_TILE");
-  const int forced_tile = ftile ? std::atoi(ftile) : 0;
+  double fp32_floor_best = 1e300;  // smallest FP32 coordinate floor over the candidates
   for (int is = 0; is < kNSig; ++is) {
-    if (forced > 0.0 && std::fabs(kSigmas[is] - forced) > 1e-9) continue;
-    int W = -1;
-    for (int w = kWMin; w <= kWMax; ++w)
-      if (kErr1D[is][w - kWMin] * safety <= eps) {
-        W = w;
-        break;
-      }
-    if (W < 0) continue;
+    if (o.sigma != 0.0 && std::fabs(kSigmas[is] - o.sigma) > 1e-9) continue;
     const double sigma = kSigmas[is];
-    // tile: 16, or 32 on request when the degridder's LDS tile with halo (T+W-1)^2 * batch fits
-    int T = forced_tile == 32 ? 32 : 16;
-    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) T = 16;
-    if ((size_t)(T + W - 1) * (T + W - 1) * nbatch * csz > (size_t)lds_budget) {
-      lds_limited = true;
-      continue;
+    // smallest support whose kernel error (with the safety factor) plus the roundoff floor of
+    // this mode reaches epsilon (or the pinned support, if it does)
+    int W = -1, T = 16, gx = 0, gy = 0, nplanes = 1;
+    double dw = 1.0, w0 = 0.0, floor_err = 0.0;
+    for (int w = (o.W ? o.W : kWMin); w <= (o.W ? o.W : kWMax); ++w) {
+      // tile: 16 x 16 by default (the sub-tile gridder scans a tile's candidates once per
+      // 8 x 8 sub-tile, and a smaller tile has fewer candidates per sub-tile); 32 on request
+      // when the degridder's LDS tile with halo (T+W-1)^2 * batch fits
+      int t = o.tile == 32 ? 32 : 16;
+      if ((size_t)(t + w - 1) * (t + w - 1) * nbatch * csz > (size_t)lds_budget) {
+        if (o.tile == 32) continue;
+        t = 16;
+      }
+      if ((size_t)(t + w - 1) * (t + w - 1) * nbatch * csz > (size_t)lds_budget) {
+        lds_limited = true;
+        continue;
+      }
+      if (t < w) continue;
+      const int cgx = good_grid(std::max(sigma * nx, 2.0 * t), t);
+      const int cgy = good_grid(std::max(sigma * ny, 2.0 * t), t);
+      const double beta = kBetaW[is][w - kWMin] * w;
+      double fl = transform_floor(w, beta, nx, ny, cgx, cgy,
+                                  prec == HIPNG_FP64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24));
+      int np = 1;
+      double cdw = 1.0, cw0 = 0.0;
+      if (wstack) {
+        cdw = 0.5 / (sigma * nm1half);
+        cw0 = wmin - (0.5 * w + 1.0) * cdw;
+        const double span = std::floor((wmax - wmin) / cdw);
+        if (!(span < 1e6)) continue;
+        np = (int)span + w + 3;
+      }
+      if (prec == HIPNG_FP32) {
+        const double ux = umax * px * cgx, vy = vmax * py * cgy;
+        const double fx = 2.0 * kPi * fp32_coord_error(ux, 0.5 * cgx + ux + 0.5 * w) * (0.5 * nx) / cgx;
+        const double fy = 2.0 * kPi * fp32_coord_error(vy, 0.5 * cgy + vy + 0.5 * w) * (0.5 * ny) / cgy;
+        const double fw = wstack ? kPi * fp32_coord_error(wmax / cdw, (double)np) / sigma : 0.0;
+        fl += std::max(fx, fy) + fw;
+        fp32_floor_best = std::min(fp32_floor_best, fl);
+      }
+      if (kErr1D[is][w - kWMin] * safety + fl > eps) continue;
+      W = w;
+      T = t;
+      gx = cgx;
+      gy = cgy;
+      nplanes = np;
+      dw = cdw;
+      w0 = cw0;
+      floor_err = fl;
+      break;
     }
-    if (T < W) continue;
-    const int gx = good_grid(std::max(sigma * nx, 2.0 * T), T);
-    const int gy = good_grid(std::max(sigma * ny, 2.0 * T), T);
-    int nplanes = 1;
-    double dw = 1.0, w0 = 0.0;
-    if (wstack) {
-      dw = 0.5 / (sigma * nm1half);
-      w0 = wmin - (0.5 * W + 1.0) * dw;
-      double span = std::floor((wmax - wmin) / dw);
-      if (!(span < 1e6)) continue;
-      nplanes = (int)span + W + 3;
-    }
+    if (W < 0) continue;
     const double ntiles = (double)(gx / T) * (gy / T);
     if ((double)nplanes * ntiles >= 4294967295.0) continue;
     // cost model, constants measured on MI355X at the production shape (mixed precision,
@@ -292,13 +359,19 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
       P->dw = dw;
       P->w0 = w0;
       P->nshift = wstack ? 0.5 * nm1min : 0.0;
-      P->est_error = kErr1D[is][W - kWMin] * std::sqrt((double)ndim);
+      P->est_error = kErr1D[is][W - kWMin] * std::sqrt((double)ndim) + floor_err;
     }
   }
   if (P->W == 0 && lds_limited && nbatch > 1)
     HIPNG_THROW(HIPNG_EINVAL, "nbatch %d too large for the LDS tile at epsilon %g: split the batch", nbatch, eps);
+  if (P->W == 0 && prec == HIPNG_FP32 && fp32_floor_best < 1e300)
+    HIPNG_THROW(HIPNG_EINVAL,
+                "epsilon %g is below the %.2e that FP32 grid coordinates and transforms reach for "
+                "this image and uv extent: use mixed or fp64 precision",
+                eps, fp32_floor_best);
   if (P->W == 0)
-    HIPNG_THROW(HIPNG_EINVAL, "no kernel/grid reaches epsilon %g for this problem", eps);
+    HIPNG_THROW(HIPNG_EINVAL, "no kernel/grid reaches epsilon %g for this problem%s", eps,
+                opts ? " with these plan options" : "");
   const uint64_t nkeys = (uint64_t)P->nplanes * P->ntx * P->nty;
   P->keybits = bits_for(nkeys - 1);
   if (P->keybits == 0) P->keybits = 1;
@@ -787,6 +860,29 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
     kout[g] = kk;
     vout[g] = s_v[idx];
   }
+}
+
+// FFT input check: a finite pseudo-random pattern in n reals, and the wrapping sum of the
+// 32-bit words of n reals (integer sums: independent of the order of the additions)
+template <typename R>
+__global__ void k_pattern(R* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const uint32_t h = (uint32_t)(i * 2654435761ull) ^ (uint32_t)(i >> 17);
+    a[i] = (R)((double)(h & 0xfffffu) * (1.0 / 1048576.0) - 0.5);
+  }
+}
+__global__ void k_word_sum(const uint32_t* __restrict__ a, int64_t nw, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s[kBlock];
+  unsigned long long acc = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nw; i += (int64_t)gridDim.x * kBlock)
+    acc += (unsigned long long)a[i] * (unsigned long long)((i & 1023) + 1);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, s[0]);
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ a, int64_t n, uint32_t v) {
@@ -1676,6 +1772,32 @@ __global__ void k_dense_stats(const uint4* __restrict__ dtask, const uint32_t* _
     atomicAdd(&out[0], nt);
     atomicAdd(&out[1], nv);
   }
+}
+
+// (plane, tile) pairs that some footprint touches: plane p is met by the records starting at
+// planes p - Ww + 1 .. p, and tile (tx, ty) by those whose first cell lies in tiles
+// (tx or tx - 1, ty or ty - 1) (periodic), since W <= T. One thread per (plane, tile).
+__global__ void k_occupied(KP k, const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
+                           unsigned long long* __restrict__ out) {
+  const int64_t total = (int64_t)k.nplanes * k.ntiles;
+  unsigned long long cnt = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
+    const int p = (int)(i / k.ntiles);
+    const uint32_t t = (uint32_t)(i - (int64_t)p * k.ntiles);
+    const int tx = (int)(t / (uint32_t)k.nty), ty = (int)(t % (uint32_t)k.nty);
+    const int txm = tx == 0 ? k.ntx - 1 : tx - 1, tym = ty == 0 ? k.nty - 1 : ty - 1;
+    const uint32_t nb4[4] = {(uint32_t)tx * k.nty + ty, (uint32_t)txm * k.nty + ty, (uint32_t)tx * k.nty + tym,
+                             (uint32_t)txm * k.nty + tym};
+    bool occ = false;
+    const int s0 = max(0, p - k.Ww + 1), s1 = min(p, k.nplanes - k.Ww);
+    for (int sp = s0; sp <= s1 && !occ; ++sp)
+      for (int q = 0; q < 4 && !occ; ++q) {
+        const int64_t key = (int64_t)sp * k.ntiles + nb4[q];
+        occ = bend[key] > bstart[key];
+      }
+    cnt += occ ? 1ull : 0ull;
+  }
+  if (cnt) atomicAdd(out, cnt);
 }
 
 // first index in [0, n) with a[i] >= v (a ascending)
@@ -3055,6 +3177,12 @@ struct DevCtx {
   void* buf[S_NSLOTS] = {};
   size_t cap[S_NSLOTS] = {};
   std::map<FftKey, rocfft_plan> plans;
+  // out-of-place plans checked for leaving their input untouched (value: it does)
+  std::map<rocfft_plan, bool> keeps_input;
+  // the last stream that enqueued work on the shared workspace, and an event after that work
+  hipEvent_t ws_ev = nullptr;
+  hipStream_t ws_st = nullptr;
+  bool ws_set = false;
 };
 
 std::mutex g_mu;
@@ -3103,6 +3231,30 @@ rocfft_plan fft_plan1(DevCtx& c, int len, int batch, int prec, int fwd, bool inp
   c.plans[key] = plan;
   return plan;
 }
+
+// One-shot calls borrow the per-device workspace and return before their kernels finish, so a
+// call on another stream first waits for the work the previous call enqueued on it; the
+// scope then records where its own work ends (also when it throws).
+struct WsUse {
+  DevCtx& c;
+  hipStream_t st;
+  WsUse(DevCtx& c_, hipStream_t s) : c(c_), st(s) {
+    if (c.ws_set && c.ws_st != st) HIP_TRY(hipStreamWaitEvent(st, c.ws_ev, 0));
+  }
+  ~WsUse() {
+    if (!c.ws_ev && hipEventCreateWithFlags(&c.ws_ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      c.ws_ev = nullptr;
+      return;
+    }
+    if (hipEventRecord(c.ws_ev, st) == hipSuccess) {
+      c.ws_set = true;
+      c.ws_st = st;
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+};
 
 unsigned grid1(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 unsigned gridcap(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 20); }
@@ -3253,8 +3405,8 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
 // channels of a row and stay cache resident, which makes this cheaper than keeping the keys
 // of every item and gathering them (measured at the production shape).
 void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const KP& k,
-                const hipng_plan& P, hipStream_t st, uint32_t** keys_out, uint32_t** perm_out,
-                uint32_t** free0 = nullptr, uint32_t** free1 = nullptr) {
+                const hipng_plan& P, bool allow_pack, hipStream_t st, uint32_t** keys_out,
+                uint32_t** perm_out, uint32_t** free0 = nullptr, uint32_t** free1 = nullptr) {
   // one spare entry each: the ping-pong pair left free after the sort is reused as scratch
   // of n + 1 entries (run starts of the dense-run detection)
   auto* k0 = (uint32_t*)ws(c, S_KEYS0, (n + 1) * 4);
@@ -3269,8 +3421,7 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
   // index bits fit in 64; the first key pass unpacks it (no second pass over uvw)
   const int dbits = 2 * bits_for((uint64_t)P.tile - 1);
   const int ibits = std::max(1, bits_for((uint64_t)std::max<int64_t>(n, 1) - 1));
-  const char* pe = std::getenv("HIPNG_SORT_PACK");  // 0: always recompute the keys (k_rekey)
-  const bool pack = !(pe && std::atoi(pe) == 0) && dbits < 32 && ibits < 32 && P.keybits + dbits + ibits <= 64;
+  const bool pack = allow_pack && dbits < 32 && ibits < 32 && P.keybits + dbits + ibits <= 64;
   // `bits` bits from bit 0 in 8-bit passes, or in 9-bit passes where that saves a pass (a
   // 9-bit pass while the remaining bits need more than 8 per remaining pass)
   auto passes = [&](int bits, int udb) {
@@ -3369,6 +3520,10 @@ struct Geom {
   // ... and invert's, with lastw[tile] = the last pass that gridded into the tile
   rocfft_plan fft_b_inv_oop = nullptr;
   uint32_t* lastw = nullptr;
+  // every out-of-place transform of this geometry leaves its input untouched (checked once per
+  // rocFFT plan on a pattern): the buffers cleared once per call stay clear. Otherwise the
+  // band transforms run in place and the transposed buffers are cleared before every pass.
+  bool oop_ok = true;
   int xlo = 0, xhi = 0;
   void* tbuf = nullptr;   // transposed band / image rows, zero where no kernel writes (cleared per call)
   void* tbuf2 = nullptr;  // its transforms along the grid rows
@@ -3387,27 +3542,9 @@ struct Geom {
   Buffers buf;
 };
 
-// HIPNG_DENSE=0 turns the dense-run gridder off (every record through the sub-tile gatherer)
-bool dense_enabled() {
-  static int v = -1;
-  if (v < 0) {
-    const char* e = std::getenv("HIPNG_DENSE");
-    v = (e && std::strcmp(e, "0") == 0) ? 0 : 1;
-  }
-  return v == 1;
-}
-
-
-// shortest run the dense-run gridder takes (HIPNG_DENSE_MIN overrides, for calibration)
-uint32_t dense_min_run() {
-  static uint32_t v = 0;
-  if (v == 0) {
-    const char* e = std::getenv("HIPNG_DENSE_MIN");
-    const long x = e ? std::atol(e) : 0;
-    v = x >= 2 ? (uint32_t)x : kDenseMin;
-  }
-  return v;
-}
+// shortest run the dense-run gridder takes: kDenseMin unless the plan options pin it
+// (dense_min -1 turns the path off, every record then goes through the sub-tile gatherer)
+uint32_t dense_min_run(const hipng_plan_opts& o) { return o.dense_min >= 2 ? (uint32_t)o.dense_min : kDenseMin; }
 
 int device_cus() {
   static int cus = 0;
@@ -3437,20 +3574,27 @@ void excl_scan(uint32_t* a, int64_t m, uint32_t* part, hipStream_t st) {
   LAUNCH_CHECK();
 }
 
+void check_fft_inputs(Geom& g, DevCtx& c, hipStream_t st);
+
 void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64_t nrow, int nchan,
                 int nb, int nx, int ny, double px, double py, double eps, int wstack, int prec,
-                hipStream_t st, bool own) {
+                const hipng_plan_opts* opts, hipStream_t st, bool own) {
   if (!freq) HIPNG_THROW(HIPNG_EINVAL, "freq must not be NULL");
   if (nrow < 0 || nchan <= 0) HIPNG_THROW(HIPNG_EINVAL, "bad nrow/nchan (%lld, %d)", (long long)nrow, nchan);
   if (nrow > 0 && !uvw) HIPNG_THROW(HIPNG_EINVAL, "uvw must not be NULL");
   const int64_t n = nrow * (int64_t)nchan;
   if (n >= 4294967295LL) HIPNG_THROW(HIPNG_EINVAL, "too many visibilities for 32-bit indices (%lld)", (long long)n);
+  hipng_plan_opts o;
+  std::memset(&o, 0, sizeof(o));
+  if (opts) o = *opts;
+  if (o.dense_min < -1 || o.dense_min == 1) HIPNG_THROW(HIPNG_EINVAL, "opts.dense_min must be -1, 0 or >= 2");
+  if (o.chunk_grid < 0 || o.chunk_degrid < 0) HIPNG_THROW(HIPNG_EINVAL, "opts chunk sizes must be >= 0");
   set_lds_limits();
   HIP_TRY(hipGetDevice(&g.device));
   g.buf.borrow = own ? nullptr : &c;
   double e[4];
   extent(c, uvw, nrow, freq, nchan, e, st);
-  make_plan(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wstack, prec, nb, n, &g.P);
+  make_plan(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wstack, prec, nb, n, opts, &g.P);
   g.k = make_kp(g.P, nb, nchan);
   g.tb = upload_tables(g.buf, g.P, freq, nchan, st);
   g.nrow = nrow;
@@ -3472,14 +3616,14 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   }
   uint32_t *keys, *perm, *scr0, *scr1;
   g.rec_a = (uint32_t*)g.buf.get(S_RECA, n * 4);
-  sort_items(c, g.uvw, n, g.tb, g.k, P, st, &keys, &perm, &scr0, &scr1);
+  sort_items(c, g.uvw, n, g.tb, g.k, P, o.sort_pack != -1, st, &keys, &perm, &scr0, &scr1);
   // dense-run gridding needs W <= 8 (one lane per footprint cell) and a free flag bit
   // in the packed first cell (grids up to 32768 cells per axis)
-  g.dense = dense_enabled() && P.W <= 8 && P.gx <= 32768 && P.gy <= 32768;
+  g.dense = o.dense_min != -1 && P.W <= 8 && P.gx <= 32768 && P.gy <= 32768;
   if (g.dense) {
     g.k.amask = 0x7fff7fffu;
     g.k.dflag = 0x80000000u;
-    g.P.dense_min = (int32_t)dense_min_run();
+    g.P.dense_min = (int32_t)dense_min_run(o);
   }
   if (own) {
     g.perm = (uint32_t*)g.buf.get(S_VALS0, n * 4);
@@ -3565,6 +3709,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     g.fftwork_size = std::max(g.fftwork_size, w);
   }
   if (g.fftwork_size) g.fftwork = g.buf.get(S_FFTWORK, g.fftwork_size);
+  check_fft_inputs(g, c, st);
   const int64_t ntiles = (int64_t)P.ntx * P.nty;
   g.task = (uint32_t*)g.buf.get(S_TASK, ntiles * 4);
   g.taskpart = (uint32_t*)g.buf.get(S_TASKPART, ((ntiles + kRsItems - 1) / kRsItems) * 4);
@@ -3573,10 +3718,9 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   const double ww = P.wstacking ? P.W : 1;
   g.chunk_grid = pick_chunk((double)n * ww / P.nplanes * 4.0, g.npersist);
   g.chunk_degrid = pick_chunk((double)n * ww / P.nplanes, g.npersist);
-  // HIPNG_CHUNK / HIPNG_CHUNK_DEGRID override the candidates per task (calibration)
-  if (const char* e = std::getenv("HIPNG_CHUNK")) g.chunk_grid = (uint32_t)std::max(256, std::atoi(e)) / 256 * 256;
-  if (const char* e = std::getenv("HIPNG_CHUNK_DEGRID"))
-    g.chunk_degrid = (uint32_t)std::max(256, std::atoi(e)) / 256 * 256;
+  // the plan options may pin the candidates per task (calibration)
+  if (o.chunk_grid > 0) g.chunk_grid = (uint32_t)std::max(256, o.chunk_grid) / 256 * 256;
+  if (o.chunk_degrid > 0) g.chunk_degrid = (uint32_t)std::max(256, o.chunk_degrid) / 256 * 256;
   {
     // tasks of a pass: sum over tiles of ceil(candidates / chunk) x strips, where a record is
     // a candidate of at most 4 tiles (gridder) or 1 (degridder)
@@ -3590,7 +3734,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     // (n + 1 entries); per-run task counts and dense lengths in workspace, scanned over the
     // runs only. Two synchronisations: the run count and the dense total size the lists.
     uint32_t* part = (uint32_t*)ws(c, S_HIST, (size_t)((n + kRsItems) / kRsItems + 1) * 4);
-    const uint32_t dmin = dense_min_run();
+    const uint32_t dmin = dense_min_run(o);
     {
       const int64_t np = (n + kRsItems - 1) / kRsItems;
       k_runscan_partial<<<(unsigned)np, kBlock, 0, st>>>(keys, g.rec_a, n, part);
@@ -3664,6 +3808,50 @@ void geom_fft(const Geom& g, rocfft_plan plan, void* data, hipStream_t st, void*
   if (e == rocfft_status_success) e = rocfft_execute(plan, bufs, out ? obufs : nullptr, info);
   rocfft_execution_info_destroy(info);
   if (e != rocfft_status_success) HIPNG_THROW(HIPNG_ERUNTIME, "rocfft failed: %d", (int)e);
+}
+
+// Does an out-of-place plan leave its input untouched? Checked once per rocFFT plan: a
+// pattern in `in` (n complex values of the geometry's precision), its word sum before and
+// after one execution into `out`. Every buffer involved is re-initialised at the start of
+// each invert / predict, so the check may write them.
+bool fft_keeps_input(const Geom& g, DevCtx& c, rocfft_plan plan, void* in, int64_t n, void* out, hipStream_t st) {
+  auto it = c.keeps_input.find(plan);
+  if (it != c.keeps_input.end()) return it->second;
+  const size_t rsz = g.prec == HIPNG_FP64 ? 8 : 4;
+  const int64_t nr = 2 * n, nw = nr * (int64_t)rsz / 4;
+  if (g.prec == HIPNG_FP64) k_pattern<double><<<gridcap(nr), kBlock, 0, st>>>((double*)in, nr);
+  else k_pattern<float><<<gridcap(nr), kBlock, 0, st>>>((float*)in, nr);
+  LAUNCH_CHECK();
+  auto* d = (unsigned long long*)ws(c, S_EXT, 2 * sizeof(unsigned long long));
+  k_zero<unsigned long long><<<1, kBlock, 0, st>>>(d, 2);
+  k_word_sum<<<std::min<unsigned>(gridcap(nw), 4096), kBlock, 0, st>>>((const uint32_t*)in, nw, d);
+  LAUNCH_CHECK();
+  geom_fft(g, plan, in, st, out);
+  k_word_sum<<<std::min<unsigned>(gridcap(nw), 4096), kBlock, 0, st>>>((const uint32_t*)in, nw, d + 1);
+  LAUNCH_CHECK();
+  unsigned long long h[2] = {0ull, 0ull};
+  HIP_TRY(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const bool keeps = h[0] == h[1];
+  c.keeps_input[plan] = keeps;
+  return keeps;
+}
+
+// The out-of-place transforms of a geometry: band rows plane -> plane2 (FP32 planes, both
+// directions) and the transposed rows T -> T2 (invert) / S -> D (predict).
+void check_fft_inputs(Geom& g, DevCtx& c, hipStream_t st) {
+  const hipng_plan& P = g.P;
+  const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
+  const int64_t ntr = (int64_t)g.pc * g.nb * P.ny * P.gx;
+  const size_t csz = g.prec == HIPNG_FP64 ? 16 : 8;
+  char* plane = (char*)g.plane + (size_t)g.xlo * P.gy * csz;
+  char* plane2 = g.plane2 ? (char*)g.plane2 + (size_t)g.xlo * P.gy * csz : nullptr;
+  const bool grouped = csz == 8;  // FP32 predict fills S = T2 and transforms into D = T
+  bool ok = fft_keeps_input(g, c, g.fft_t_inv, g.tbuf, ntr, g.tbuf2, st);
+  ok = fft_keeps_input(g, c, g.fft_t_fwd, grouped ? g.tbuf2 : g.tbuf, ntr, grouped ? g.tbuf : g.tbuf2, st) && ok;
+  if (g.fft_b_inv_oop) ok = fft_keeps_input(g, c, g.fft_b_inv_oop, plane, nband, plane2, st) && ok;
+  if (g.fft_b_fwd_oop) ok = fft_keeps_input(g, c, g.fft_b_fwd_oop, plane, nband, plane2, st) && ok;
+  g.oop_ok = ok;
 }
 
 // planes whose tiles the degridder holds in LDS at once: as many of the pass's planes as fit
@@ -3840,7 +4028,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   const int ngrp = sizeof(R) == 4 ? kScreenGroup : 1;  // passes of FP32 planes T2 holds
   // FP32 planes: band transforms out of place into plane2, so a pass clears only the tiles the
   // previous pass gridded into (lastw); the band is cleared once per call
-  const bool oop = g.plane2 != nullptr;
+  const bool oop = g.plane2 != nullptr && g.oop_ok;
   cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
   const int64_t nband = (int64_t)(g.xhi - g.xlo) * P.gy;
   if (oop) {
@@ -3911,6 +4099,12 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     }
     {
       PROF("transpose");
+      if (!g.oop_ok) {
+        // the transform along the grid rows may have overwritten T: clear it again
+        const int64_t rows = (int64_t)g.pc * g.nb * P.ny;
+        zero_cols(tbuf, rows, P.gx, 0, g.xlo, st);
+        zero_cols(tbuf, rows, P.gx, g.xhi, P.gx, st);
+      }
       const int xb = g.xlo & ~31;
       k_tr_in<R><<<dim3((g.xhi - xb + 31) / 32, (P.ny + 31) / 32, npl * g.nb), kBlock, 0, st>>>(
           k, g.xlo, g.xhi, xb, bplane, tbuf);
@@ -3972,7 +4166,7 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
   // band transforms out of place (FP32): the band rows of plane are cleared once and only
   // their image columns are rewritten per pass
-  const bool oop = g.plane2 != nullptr;
+  const bool oop = g.plane2 != nullptr && g.oop_ok;
   cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
   if (oop) {
     PROF("transpose");
@@ -3985,6 +4179,12 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
     const int npl = std::min(g.pc, P.nplanes - p0);
     const int slot = (p0 / g.pc) % ngrp;
+    if (slot == 0 && !g.oop_ok) {
+      PROF("transpose");
+      // the transforms along the grid rows may have overwritten S: clear it again
+      zero_cols(sbuf, (int64_t)ngrp * g.pc * g.nb * P.ny, P.gx, P.nx / 2, P.gx - P.nx / 2, st);
+      LAUNCH_CHECK();
+    }
     if (slot == 0) {
       PROF("screen_fill");
       const int nfill = std::min(ngrp * g.pc, P.nplanes - p0);
@@ -4068,11 +4268,26 @@ void dense_counts(const Geom& g, hipStream_t st, int64_t* ntask, int64_t* nvis) 
   *nvis = (int64_t)h[1];
 }
 
+// (plane, tile) pairs that footprints touch (device reduction, one synchronisation)
+int64_t occupied_tiles(const Geom& g, hipStream_t st) {
+  if (g.n == 0) return 0;
+  auto* d = (unsigned long long*)ws(ctx(), S_EXT, sizeof(unsigned long long));
+  k_zero<unsigned long long><<<1, kBlock, 0, st>>>(d, 1);
+  LAUNCH_CHECK();
+  k_occupied<<<2048, kBlock, 0, st>>>(g.k, g.bstart, g.bend, d);
+  LAUNCH_CHECK();
+  unsigned long long h = 0ull;
+  HIP_TRY(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return (int64_t)h;
+}
+
 void fill_plan_out(const Geom& g, hipStream_t st, hipng_plan* out) {
   if (!out) return;
   *out = g.P;
   int64_t nt = 0;
   dense_counts(g, st, &nt, &out->dense_items);
+  out->occupied_tiles = occupied_tiles(g, st);
 }
 
 template <typename F>
@@ -4119,6 +4334,42 @@ PolArgs pol_args(int64_t nrow, int32_t nchan, int32_t nin, int32_t nout, const d
   return a;
 }
 
+GainArgs gain_args(int64_t ntimes, int32_t nbaselines, int32_t nchan, int32_t npol, int32_t ngain_times,
+                   int32_t nant, int32_t gain_nchan, int32_t nrec) {
+  if (ntimes < 0 || nbaselines < 0 || nchan <= 0 || ngain_times < 0 || nant <= 0 || gain_nchan <= 0)
+    HIPNG_THROW(HIPNG_EINVAL, "bad sizes");
+  if (npol != 1 && npol != 2 && npol != 4) HIPNG_THROW(HIPNG_EINVAL, "npol must be 1, 2 or 4");
+  if (nrec != 1 && nrec != 2) HIPNG_THROW(HIPNG_EINVAL, "nrec must be 1 or 2");
+  if (npol > 1 && nrec != 2) HIPNG_THROW(HIPNG_EINVAL, "npol %d needs 2x2 gains", npol);
+  return GainArgs{ntimes, nbaselines, nchan, npol, gain_nchan, nant, nrec, ngain_times};
+}
+
+// prepared matrices (nrec x nrec complex128) and validity flags of every gain-table entry
+size_t gain_scratch_bytes(const GainArgs& a) {
+  const size_t nent = (size_t)a.ngain * a.nant * a.gnchan;
+  return nent * ((size_t)a.nrec * a.nrec * sizeof(cplx<double>) + sizeof(int));
+}
+
+// k_gain_prep into `buf` (gain_scratch_bytes(a) bytes), then k_apply_gains; kernels only
+void launch_gains(const GainArgs& a, void* vis, double* weight, const double* flags, const int8_t* time_flagged,
+                  const int32_t* ant1, const int32_t* ant2, const void* gain, const double* gain_weight,
+                  const int32_t* time_to_gain, int32_t inverse, void* buf, hipStream_t st) {
+  if (!vis || !weight || !ant1 || !ant2 || !gain || !gain_weight || !time_to_gain)
+    HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+  if (time_flagged && !flags) HIPNG_THROW(HIPNG_EINVAL, "time_flagged needs flags");
+  const int64_t n = a.ntimes * a.nbl * (int64_t)a.nchan;
+  const int64_t nent = (int64_t)a.ngain * a.nant * a.gnchan;
+  const int n2 = a.nrec * a.nrec;
+  auto* m = (cplx<double>*)buf;
+  auto* valid = (int*)((char*)buf + nent * n2 * sizeof(cplx<double>));
+  k_gain_prep<<<grid1(nent), kBlock, 0, st>>>((const cplx<double>*)gain, nent, a.nrec, inverse ? 1 : 0, m, valid);
+  LAUNCH_CHECK();
+  // (a version specialised on npol measured slower at the production shape: 39.9 against 35.9 ms)
+  k_apply_gains<<<grid1(n), kBlock, 0, st>>>((cplx<double>*)vis, weight, flags, time_flagged, ant1, ant2, m,
+                                             valid, gain_weight, time_to_gain, a);
+  LAUNCH_CHECK();
+}
+
 double* upload_fc(DevCtx& c, const double* freq, int32_t nchan, hipStream_t st) {
   if (!freq) HIPNG_THROW(HIPNG_EINVAL, "NULL freq");
   std::vector<double> h(nchan);
@@ -4138,11 +4389,14 @@ const char* hipng_last_error(void) { return g_err.c_str(); }
 int32_t hipng_plan_params(double umax, double vmax, double wmin, double wmax, int32_t npix_x,
                           int32_t npix_y, double pixsize_x, double pixsize_y, double epsilon,
                           int32_t do_wstacking, int32_t precision, int32_t nbatch, int64_t nvis,
-                          hipng_plan* out) {
+                          const hipng_plan_opts* opts, hipng_plan* out) {
   return guarded([&] {
     if (!out) HIPNG_THROW(HIPNG_EINVAL, "out must not be NULL");
     make_plan(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y, epsilon, do_wstacking,
-              precision, nbatch, nvis, out);
+              precision, nbatch, nvis, opts, out);
+    // the dense-run threshold the compute calls would use (0: that path is off)
+    if (!(opts && opts->dense_min == -1) && out->W <= 8 && out->gx <= 32768 && out->gy <= 32768)
+      out->dense_min = opts && opts->dense_min >= 2 ? opts->dense_min : (int32_t)kDenseMin;
   });
 }
 
@@ -4158,13 +4412,15 @@ int32_t hipng_uvw_extent(const double* uvw, int64_t nrow, const double* freq, in
                          double* out, void* stream) {
   return guarded([&] {
     if (!uvw || !freq || !out || nchan <= 0 || nrow < 0) HIPNG_THROW(HIPNG_EINVAL, "bad argument");
-    extent(ctx(), uvw, nrow, freq, nchan, out, (hipStream_t)stream);
+    DevCtx& c = ctx();
+    WsUse use(c, (hipStream_t)stream);
+    extent(c, uvw, nrow, freq, nchan, out, (hipStream_t)stream);
   });
 }
 
 int32_t hipng_bucket_sort(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
-                          const hipng_plan* plan, uint32_t* keys_out, uint32_t* perm_out,
-                          void* stream) {
+                          const hipng_plan* plan, const hipng_plan_opts* opts, uint32_t* keys_out,
+                          uint32_t* perm_out, void* stream) {
   return guarded([&] {
     if (!freq || !plan || nchan <= 0 || nrow < 0) HIPNG_THROW(HIPNG_EINVAL, "bad argument");
     const int64_t n = nrow * (int64_t)nchan;
@@ -4173,12 +4429,13 @@ int32_t hipng_bucket_sort(const double* uvw, const double* freq, int64_t nrow, i
     if (n >= 4294967295LL) HIPNG_THROW(HIPNG_EINVAL, "too many visibilities");
     hipStream_t st = (hipStream_t)stream;
     DevCtx& c = ctx();
+    WsUse use(c, st);
     Buffers B;
     B.borrow = &c;
     const KP k = make_kp(*plan, 1, nchan);
     Tables tb = upload_tables(B, *plan, freq, nchan, st);
     uint32_t *keys, *perm;
-    sort_items(c, uvw, n, tb, k, *plan, st, &keys, &perm);
+    sort_items(c, uvw, n, tb, k, *plan, !(opts && opts->sort_pack == -1), st, &keys, &perm);
     HIP_TRY(hipMemcpyAsync(keys_out, keys, n * 4, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(perm_out, perm, n * 4, hipMemcpyDeviceToDevice, st));
   });
@@ -4188,13 +4445,15 @@ int32_t hipng_ms2dirty(const double* uvw, const double* freq, int64_t nrow, int3
                        const void* ms, const void* wgt, int32_t nbatch, int32_t npix_x,
                        int32_t npix_y, double pixsize_x, double pixsize_y, double epsilon,
                        int32_t do_wstacking, int32_t precision, void* dirty, void* stream,
-                       hipng_plan* plan_out) {
+                       const hipng_plan_opts* opts, hipng_plan* plan_out) {
   return guarded([&] {
     if (!dirty) HIPNG_THROW(HIPNG_EINVAL, "dirty must not be NULL");
     hipStream_t st = (hipStream_t)stream;
     Geom g;
-    build_geom(g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
-               epsilon, do_wstacking, precision, st, false);
+    DevCtx& c = ctx();
+    WsUse use(c, st);
+    build_geom(g, c, uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
+               epsilon, do_wstacking, precision, opts, st, false);
     fill_plan_out(g, st, plan_out);
     exec_dispatch(g, true, ms, wgt, dirty, st);
   });
@@ -4204,12 +4463,14 @@ int32_t hipng_dirty2ms(const double* uvw, const double* freq, int64_t nrow, int3
                        const void* dirty, const void* wgt, int32_t nbatch, int32_t npix_x,
                        int32_t npix_y, double pixsize_x, double pixsize_y, double epsilon,
                        int32_t do_wstacking, int32_t precision, void* ms, void* stream,
-                       hipng_plan* plan_out) {
+                       const hipng_plan_opts* opts, hipng_plan* plan_out) {
   return guarded([&] {
     hipStream_t st = (hipStream_t)stream;
     Geom g;
-    build_geom(g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
-               epsilon, do_wstacking, precision, st, false);
+    DevCtx& c = ctx();
+    WsUse use(c, st);
+    build_geom(g, c, uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
+               epsilon, do_wstacking, precision, opts, st, false);
     fill_plan_out(g, st, plan_out);
     exec_dispatch(g, false, dirty, wgt, ms, st);
   });
@@ -4222,15 +4483,17 @@ struct hipng_geometry_s {
 int32_t hipng_geometry_create(const double* uvw, const double* freq, int64_t nrow, int32_t nchan,
                               int32_t nbatch, int32_t npix_x, int32_t npix_y, double pixsize_x,
                               double pixsize_y, double epsilon, int32_t do_wstacking,
-                              int32_t precision, void* stream, hipng_geometry* out,
-                              hipng_plan* plan_out) {
+                              int32_t precision, void* stream, const hipng_plan_opts* opts,
+                              hipng_geometry* out, hipng_plan* plan_out) {
   return guarded([&] {
     if (!out) HIPNG_THROW(HIPNG_EINVAL, "out must not be NULL");
     *out = nullptr;
     auto* h = new hipng_geometry_s();
+    DevCtx& c = ctx();
+    WsUse use(c, (hipStream_t)stream);
     try {
-      build_geom(h->g, ctx(), uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
-                 epsilon, do_wstacking, precision, (hipStream_t)stream, true);
+      build_geom(h->g, c, uvw, freq, nrow, nchan, nbatch, npix_x, npix_y, pixsize_x, pixsize_y,
+                 epsilon, do_wstacking, precision, opts, (hipStream_t)stream, true);
     } catch (...) {
       h->g.buf.release();
       delete h;
@@ -4264,9 +4527,10 @@ int32_t hipng_geometry_stats(hipng_geometry geom, int64_t* out, int32_t nout) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
     if (dev != g.device) HIPNG_THROW(HIPNG_EINVAL, "geometry was built on device %d, current device is %d", g.device, dev);
-    int64_t v[4] = {g.n, 0, 0, g.dense ? 1 : 0};
+    int64_t v[5] = {g.n, 0, 0, g.dense ? 1 : 0, g.oop_ok ? 1 : 0};
+    WsUse use(ctx(), nullptr);
     dense_counts(g, nullptr, &v[1], &v[2]);
-    for (int i = 0; i < nout && i < 4; ++i) out[i] = v[i];
+    for (int i = 0; i < nout && i < 5; ++i) out[i] = v[i];
   });
 }
 
@@ -4302,34 +4566,14 @@ int32_t hipng_apply_gaintable(void* vis, double* weight, const double* flags, co
                               int32_t gain_nchan, int32_t nrec, const int32_t* time_to_gain,
                               int32_t inverse, void* stream) {
   return guarded([&] {
-    if (ntimes < 0 || nbaselines < 0 || nchan <= 0 || ngain_times < 0 || nant <= 0 || gain_nchan <= 0)
-      HIPNG_THROW(HIPNG_EINVAL, "bad sizes");
-    if (npol != 1 && npol != 2 && npol != 4) HIPNG_THROW(HIPNG_EINVAL, "npol must be 1, 2 or 4");
-    if (nrec != 1 && nrec != 2) HIPNG_THROW(HIPNG_EINVAL, "nrec must be 1 or 2");
-    if (npol > 1 && nrec != 2) HIPNG_THROW(HIPNG_EINVAL, "npol %d needs 2x2 gains", npol);
-    const int64_t n = ntimes * nbaselines * (int64_t)nchan;
-    if (n == 0 || ngain_times == 0) return;
-    if (!vis || !weight || !ant1 || !ant2 || !gain || !gain_weight || !time_to_gain)
-      HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
-    if (time_flagged && !flags) HIPNG_THROW(HIPNG_EINVAL, "time_flagged needs flags");
+    const GainArgs a = gain_args(ntimes, nbaselines, nchan, npol, ngain_times, nant, gain_nchan, nrec);
+    if (a.ntimes * a.nbl * (int64_t)a.nchan == 0 || ngain_times == 0) return;
     hipStream_t st = (hipStream_t)stream;
     DevCtx& c = ctx();
-    const int64_t nent = (int64_t)ngain_times * nant * gain_nchan;
-    const int n2 = nrec * nrec;
-    auto* buf = (char*)ws(c, S_GAIN, nent * (n2 * sizeof(cplx<double>) + sizeof(int)) + 256);
-    auto* m = (cplx<double>*)buf;
-    auto* valid = (int*)(buf + nent * n2 * sizeof(cplx<double>));
-    {
-      PROF("gain_prep");
-      k_gain_prep<<<grid1(nent), kBlock, 0, st>>>((const cplx<double>*)gain, nent, nrec, inverse ? 1 : 0, m, valid);
-      LAUNCH_CHECK();
-    }
-    GainArgs a{ntimes, nbaselines, nchan, npol, gain_nchan, nant, nrec, ngain_times};
+    WsUse use(c, st);
+    void* buf = ws(c, S_GAIN, gain_scratch_bytes(a) + 256);
     PROF("apply_gains");
-    // (a version specialised on npol measured slower at the production shape: 39.9 against 35.9 ms)
-    k_apply_gains<<<grid1(n), kBlock, 0, st>>>((cplx<double>*)vis, weight, flags, time_flagged, ant1, ant2, m,
-                                               valid, gain_weight, time_to_gain, a);
-    LAUNCH_CHECK();
+    launch_gains(a, vis, weight, flags, time_flagged, ant1, ant2, gain, gain_weight, time_to_gain, inverse, buf, st);
   });
 }
 
@@ -4343,6 +4587,7 @@ int32_t hipng_vis_prologue(const void* vis, const int32_t* flags, const double* 
     if (!vis || !imaging_weight || !ms || !wgt || !nonzero || (lmn && !uvw)) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
     hipStream_t st = (hipStream_t)stream;
     DevCtx& c = ctx();
+    WsUse use(c, st);
     const double* fc = upload_fc(c, freq, nchan, st);
     k_zero<int><<<1, kBlock, 0, st>>>(nonzero, (int64_t)nchan * npol_image);
     LAUNCH_CHECK();
@@ -4381,6 +4626,7 @@ int32_t hipng_vis_epilogue(const void* vist, const double* uvw, const double* fr
     if (!vist || !vis || (lmn && !uvw)) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
     hipStream_t st = (hipStream_t)stream;
     DevCtx& c = ctx();
+    WsUse use(c, st);
     const double* fc = upload_fc(c, freq, nchan, st);
     const int64_t n = nrow * (int64_t)nchan;
     if (n == 0) return;
@@ -4446,6 +4692,7 @@ int32_t hipng_weight_visibility(const double* uvw, const double* freq, int64_t n
     if (weighting != 1 && weighting != 2) HIPNG_THROW(HIPNG_EINVAL, "weighting must be 1 (uniform) or 2 (robust)");
     hipStream_t st = (hipStream_t)stream;
     DevCtx& c = ctx();
+    WsUse use(c, st);
     const int64_t n = ntimes * nbaselines * (int64_t)nchan;
     const int64_t ng = (int64_t)nimchan * npol * npix_x * npix_y;
     if (n && !uvw) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
@@ -4507,9 +4754,11 @@ int32_t hipng_release(void) {
   return guarded([&] {
     for (auto& kv : g_ctx) {
       HIP_TRY(hipSetDevice(kv.first));
+      HIP_TRY(hipDeviceSynchronize());
       for (int s = 0; s < S_NSLOTS; ++s)
         if (kv.second.buf[s]) (void)hipFree(kv.second.buf[s]);
       for (auto& pl : kv.second.plans) rocfft_plan_destroy(pl.second);
+      if (kv.second.ws_ev) (void)hipEventDestroy(kv.second.ws_ev);
     }
     g_ctx.clear();
   });

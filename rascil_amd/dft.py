@@ -59,6 +59,33 @@ def dft_kernel(direction_cosines, vfluxes, uvw_lambda, dft_compute_kernel=None, 
     return out.cpu().numpy()
 
 
+def dft_device(direction_cosines, vfluxes, uvw_lambda, out=None):
+    """GPU DFT on device tensors (no host copies): direction_cosines [K, 3] float64,
+    vfluxes [K, nchan, npol] complex128, uvw_lambda [t, b, nchan, 3] float64 -> vis
+    [t, b, nchan, npol] complex128 (or into `out`), on the current stream."""
+    import torch
+
+    lib = _lib.load()
+    ntimes, nbl, nchan, three = uvw_lambda.shape
+    ncomp, fnchan, npol = vfluxes.shape
+    dev = uvw_lambda.device
+    if three != 3 or tuple(direction_cosines.shape) != (ncomp, 3) or fnchan != nchan:
+        raise ValueError("dft_device: shapes [K,3], [K,nchan,npol], [t,b,nchan,3] do not match")
+    for name, t, dt in (("direction_cosines", direction_cosines, torch.float64),
+                        ("vfluxes", vfluxes, torch.complex128), ("uvw_lambda", uvw_lambda, torch.float64)):
+        if t.dtype != dt or not t.is_contiguous() or t.device != dev:
+            raise ValueError(f"dft_device: {name} must be a contiguous {dt} tensor on {dev}")
+    if out is None:
+        out = torch.empty((ntimes, nbl, nchan, npol), dtype=torch.complex128, device=dev)
+    elif out.dtype != torch.complex128 or not out.is_contiguous() or tuple(out.shape) != (ntimes, nbl, nchan, npol):
+        raise ValueError("dft_device: out must be a contiguous complex128 tensor [t, b, nchan, npol]")
+    _lib.check(lib.hipng_dft(ctypes.c_void_p(direction_cosines.data_ptr()), ctypes.c_void_p(vfluxes.data_ptr()),
+                             ctypes.c_void_p(uvw_lambda.data_ptr()), ncomp, ntimes * nbl, nchan, npol,
+                             ctypes.c_void_p(out.data_ptr()),
+                             ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)))
+    return out
+
+
 def dft_skycomponent_visibility(vis: BlockVisibility, sc, **kwargs) -> BlockVisibility:
     """Set vis to the DFT of the component(s) (dft.py:44-62)."""
     if sc is None or (isinstance(sc, list) and len(sc) == 0):

@@ -2,7 +2,7 @@
 
 Mirrors rascil/processing_components/imaging/base.py:67-162 (shift_vis_to_image,
 normalise_sumwt 2-D case) and visibility/base.py:296-357, 1248-1264
-(phaserotate_visibility with tangent=True, calculate_blockvisibility_phasor).
+(phaserotate_visibility, calculate_blockvisibility_phasor).
 """
 
 import numpy
@@ -25,9 +25,12 @@ def calculate_blockvisibility_phasor(direction: Direction, vis: BlockVisibility)
 
 def phaserotate_visibility(vis: BlockVisibility, newphasecentre: Direction, tangent=True,
                            inverse=False) -> BlockVisibility:
-    """Phase rotate to a new phase centre staying on the tangent plane (visibility/base.py:296-357)."""
-    if not tangent:
-        raise NotImplementedError("only tangent-plane phase rotation is on the imaging path")
+    """Phase rotate to a new phase centre (visibility/base.py:296-357).
+
+    tangent=True adjusts only the phases (the imaging path's case). tangent=False also
+    re-projects uvw (metres) onto the new phase centre, through the earth frame (uvw -> xyz at
+    hour angle -ra of the old centre, xyz -> uvw at -ra of the new one), and moves the
+    phase centre (:330-356)."""
     l, m, nm1 = direction_to_lmn(newphasecentre, vis.phasecentre)
     if numpy.abs(nm1) < 1e-15:
         return vis
@@ -37,6 +40,15 @@ def phaserotate_visibility(vis: BlockVisibility, newphasecentre: Direction, tang
         newvis.vis = newvis.vis * phasor
     else:
         newvis.vis = newvis.vis * numpy.conj(phasor)
+    if not tangent:
+        from .simulation import uvw_to_xyz, xyz_to_uvw
+
+        nrows, nbl, _ = vis.uvw.shape
+        xyz = uvw_to_xyz(vis.uvw.reshape(nrows * nbl, 3), ha=-vis.phasecentre.ra, dec=vis.phasecentre.dec)
+        uvw = xyz_to_uvw(xyz, ha=-newphasecentre.ra, dec=newphasecentre.dec)
+        newvis.uvw = uvw.reshape(nrows, nbl, 3)
+        newvis.phasecentre = newphasecentre
+        newvis.attrs["phasecentre"] = newphasecentre
     return newvis
 
 

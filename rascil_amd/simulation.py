@@ -34,6 +34,16 @@ def xyz_to_uvw(xyz, ha, dec):
     return numpy.hstack([u, v, w])
 
 
+def uvw_to_xyz(uvw, ha, dec):
+    """Inverse of xyz_to_uvw (coordinate_support.py:382-407): the transpose of its rotation."""
+    u, v, w = numpy.hsplit(uvw, 3)
+    v0 = v * numpy.sin(dec) - w * numpy.cos(dec)
+    z = v * numpy.cos(dec) + w * numpy.sin(dec)
+    x = u * numpy.cos(ha) + v0 * numpy.sin(ha)
+    y = -u * numpy.sin(ha) + v0 * numpy.cos(ha)
+    return numpy.hstack([x, y, z])
+
+
 def baseline_indices(nant, autocorrelations=True):
     a1, a2 = numpy.triu_indices(nant, 0 if autocorrelations else 1)
     return a1, a2

@@ -12,7 +12,9 @@ same device, computed on the current stream).
 Extensions: ms of shape [nrow, nchan, nbatch] (and dirty [nbatch, npix_x, npix_y])
 processes up to nbatch independent images sharing uvw/freq in one pass; `precision`
 selects "fp64" (default for complex128 input), "mixed" (default for complex64 input) or
-"fp32"; `plan_out` (a dict) receives the kernel/grid/plane parameters that were used.
+"fp32"; `plan_out` (a dict) receives the kernel/grid/plane parameters that were used;
+`plan_opts` (a dict, see _lib.plan_opts) pins plan choices (oversampling, support, tile,
+dense-run threshold, task sizes, uv extent) without touching the process environment.
 """
 
 import ctypes
@@ -78,7 +80,8 @@ def _split_batch(fn, kwargs):
 
 def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=0,
              epsilon=1e-7, do_wstacking=False, nthreads=1, verbosity=0, mask=None,
-             double_precision_accumulation=False, precision=None, plan_out=None, batch_chunk=None):
+             double_precision_accumulation=False, precision=None, plan_out=None, batch_chunk=None,
+             plan_opts=None):
     """Weighted visibilities -> dirty image(s); see module docstring.
 
     Returns [npix_x, npix_y] (or [nbatch, npix_x, npix_y] for batched ms); float64 for
@@ -87,22 +90,23 @@ def ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=
     return _split_batch(_ms2dirty, dict(
         uvw=uvw, freq=freq, ms=ms, wgt=wgt, npix_x=npix_x, npix_y=npix_y, pixsize_x=pixsize_x,
         pixsize_y=pixsize_y, epsilon=epsilon, do_wstacking=do_wstacking, verbosity=verbosity,
-        mask=mask, precision=precision, plan_out=plan_out, batch_chunk=batch_chunk))
+        mask=mask, precision=precision, plan_out=plan_out, batch_chunk=batch_chunk, plan_opts=plan_opts))
 
 
 def dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e-7,
              do_wstacking=False, nthreads=1, verbosity=0, mask=None, precision=None,
-             plan_out=None, batch_chunk=None):
+             plan_out=None, batch_chunk=None, plan_opts=None):
     """Dirty image(s) -> visibilities [nrow, nchan] (or [nrow, nchan, nbatch])."""
     return _split_batch(_dirty2ms, dict(
         uvw=uvw, freq=freq, dirty=dirty, wgt=wgt, pixsize_x=pixsize_x, pixsize_y=pixsize_y,
         epsilon=epsilon, do_wstacking=do_wstacking, verbosity=verbosity, mask=mask,
-        precision=precision, plan_out=plan_out, batch_chunk=batch_chunk))
+        precision=precision, plan_out=plan_out, batch_chunk=batch_chunk, plan_opts=plan_opts))
 
 
 def _ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv=0,
               epsilon=1e-7, do_wstacking=False, nthreads=1, verbosity=0, mask=None,
-              double_precision_accumulation=False, precision=None, plan_out=None, batch_chunk=None):
+              double_precision_accumulation=False, precision=None, plan_out=None, batch_chunk=None,
+              plan_opts=None):
     torch = _torch()
     lib = _lib.load()
     as_numpy = isinstance(ms, numpy.ndarray)
@@ -146,7 +150,8 @@ def _ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv
                 ctypes.c_void_p(wb.data_ptr()) if wb is not None else None,
                 b1 - b0, int(npix_x), int(npix_y), float(pixsize_x), float(pixsize_y),
                 float(epsilon), int(bool(do_wstacking)), _lib.PRECISIONS[precision],
-                ctypes.c_void_p(out[b0:b1].data_ptr()), _stream(torch, device), ctypes.byref(plan))
+                ctypes.c_void_p(out[b0:b1].data_ptr()), _stream(torch, device), _lib.plan_opts(plan_opts),
+                ctypes.byref(plan))
         _lib.check(code)
     if plan_out is not None:
         plan_out.update(plan.as_dict())
@@ -161,7 +166,7 @@ def _ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, nu=0, nv
 
 def _dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1e-7,
               do_wstacking=False, nthreads=1, verbosity=0, mask=None, precision=None,
-              plan_out=None, batch_chunk=None):
+              plan_out=None, batch_chunk=None, plan_opts=None):
     torch = _torch()
     lib = _lib.load()
     as_numpy = isinstance(dirty, numpy.ndarray)
@@ -206,7 +211,8 @@ def _dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1
                 ctypes.c_void_p(wb.data_ptr()) if wb is not None else None,
                 b1 - b0, int(npix_x), int(npix_y), float(pixsize_x), float(pixsize_y),
                 float(epsilon), int(bool(do_wstacking)), _lib.PRECISIONS[precision],
-                ctypes.c_void_p(ob.data_ptr()), _stream(torch, device), ctypes.byref(plan))
+                ctypes.c_void_p(ob.data_ptr()), _stream(torch, device), _lib.plan_opts(plan_opts),
+                ctypes.byref(plan))
         _lib.check(code)
         if chunked:
             out[:, :, b0:b1] = ob
@@ -222,13 +228,13 @@ def _dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, nu=0, nv=0, epsilon=1
 
 
 def plan_params(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y, epsilon,
-                do_wstacking, precision="fp64", nbatch=1, nvis=0):
-    """Host-only plan choice (no GPU needed); returns a dict of hipng_plan fields."""
+                do_wstacking, precision="fp64", nbatch=1, nvis=0, plan_opts=None):
+    """Host-only plan choice (no GPU needed); returns the hipng_plan structure."""
     lib = _lib.load()
     plan = _lib.HipngPlan()
     _lib.check(lib.hipng_plan_params(umax, vmax, wmin, wmax, npix_x, npix_y, pixsize_x, pixsize_y,
                                      epsilon, int(bool(do_wstacking)), _lib.PRECISIONS[precision],
-                                     nbatch, int(nvis), ctypes.byref(plan)))
+                                     nbatch, int(nvis), _lib.plan_opts(plan_opts), ctypes.byref(plan)))
     return plan
 
 
@@ -242,8 +248,9 @@ def channel_scales(plan, freq):
     return sx, sy, sw
 
 
-def bucket_sort(uvw, freq, plan):
-    """GPU bucket sort by (w plane, uv tile): returns (keys, perm) as numpy uint32."""
+def bucket_sort(uvw, freq, plan, plan_opts=None):
+    """GPU bucket sort by (w plane, uv tile): returns (keys, perm) as numpy uint32.
+    plan_opts {"sort_pack": False} recomputes the bucket keys after the digit passes."""
     torch = _torch()
     lib = _lib.load()
     f = _freq_host(freq)
@@ -253,7 +260,8 @@ def bucket_sort(uvw, freq, plan):
     keys = torch.empty(n, dtype=torch.int32, device=device)
     perm = torch.empty(n, dtype=torch.int32, device=device)
     _lib.check(lib.hipng_bucket_sort(ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(f), uvw_d.shape[0],
-                                     f.shape[0], ctypes.byref(plan), ctypes.c_void_p(keys.data_ptr()),
+                                     f.shape[0], ctypes.byref(plan), _lib.plan_opts(plan_opts),
+                                     ctypes.c_void_p(keys.data_ptr()),
                                      ctypes.c_void_p(perm.data_ptr()), _stream(torch, device)))
     torch.cuda.synchronize(device)
     return keys.cpu().numpy().view(numpy.uint32), perm.cpu().numpy().view(numpy.uint32)
@@ -282,7 +290,7 @@ class Geometry:
     """
 
     def __init__(self, uvw, freq, npix_x, npix_y, pixsize_x, pixsize_y, epsilon,
-                 do_wstacking=True, precision="mixed", nbatch=1, device=None):
+                 do_wstacking=True, precision="mixed", nbatch=1, device=None, plan_opts=None):
         torch = _torch()
         self._lib = _lib.load()
         if precision not in _lib.PRECISIONS:
@@ -303,24 +311,41 @@ class Geometry:
                 ctypes.c_void_p(uvw_d.data_ptr()), _lib.dptr(self.freq), self.nrow, self.nchan, nbatch,
                 self.npix_x, self.npix_y, float(pixsize_x), float(pixsize_y), float(epsilon),
                 int(bool(do_wstacking)), _lib.PRECISIONS[precision], _stream(torch, self.device),
-                ctypes.byref(self._handle), ctypes.byref(plan)))
+                _lib.plan_opts(plan_opts), ctypes.byref(self._handle), ctypes.byref(plan)))
         self.plan = plan.as_dict()
 
     def _shape_ms(self):
         return (self.nrow, self.nchan, self.nbatch)
 
+    def _check(self, name, t, dtype, shape):
+        """t must be a contiguous `dtype` tensor of `shape` on this geometry's device (its raw
+        pointer goes to the kernels)."""
+        if (t.dtype != dtype or not t.is_contiguous() or t.device != self.device
+                or tuple(t.shape) != tuple(shape)):
+            raise ValueError("%s must be a contiguous %s tensor of shape %s on %s (got %s %s on %s)"
+                             % (name, dtype, tuple(shape), self.device, t.dtype, tuple(t.shape), t.device))
+
+    def _wgt(self, wgt):
+        if wgt is None:
+            return None
+        if wgt.numel() != self.nrow * self.nchan * self.nbatch:
+            raise ValueError("wgt must have %d elements" % (self.nrow * self.nchan * self.nbatch))
+        wgt = wgt.reshape(self._shape_ms())
+        self._check("wgt", wgt, self.rdtype, self._shape_ms())
+        return wgt
+
     def ms2dirty(self, ms, wgt=None, out=None):
         """ms [nrow, nchan, nbatch] (or [nrow, nchan] when nbatch == 1) GPU tensor -> dirty."""
         torch = _torch()
+        if ms.numel() != self.nrow * self.nchan * self.nbatch:
+            raise ValueError("ms must have %d elements" % (self.nrow * self.nchan * self.nbatch))
         ms = ms.reshape(self._shape_ms())
-        if ms.dtype != self.cdtype or not ms.is_contiguous() or ms.device != self.device:
-            raise ValueError("ms must be a contiguous %s tensor on %s" % (self.cdtype, self.device))
-        if wgt is not None:
-            wgt = wgt.reshape(self._shape_ms())
-            if wgt.dtype != self.rdtype or not wgt.is_contiguous():
-                raise ValueError("wgt must be a contiguous %s tensor" % self.rdtype)
+        self._check("ms", ms, self.cdtype, self._shape_ms())
+        wgt = self._wgt(wgt)
         if out is None:
             out = torch.empty((self.nbatch, self.npix_x, self.npix_y), dtype=self.rdtype, device=self.device)
+        else:
+            self._check("out", out, self.rdtype, (self.nbatch, self.npix_x, self.npix_y))
         _lib.check(self._lib.hipng_geometry_ms2dirty(
             self._handle, ctypes.c_void_p(ms.data_ptr()),
             ctypes.c_void_p(wgt.data_ptr()) if wgt is not None else None,
@@ -330,13 +355,18 @@ class Geometry:
     def dirty2ms(self, dirty, wgt=None, out=None):
         """dirty [nbatch, npix_x, npix_y] (or [npix_x, npix_y]) GPU tensor -> ms [nrow, nchan, nbatch]."""
         torch = _torch()
+        if dirty.numel() != self.nbatch * self.npix_x * self.npix_y:
+            raise ValueError("dirty must have %d elements" % (self.nbatch * self.npix_x * self.npix_y))
         dirty = dirty.reshape(self.nbatch, self.npix_x, self.npix_y)
-        if dirty.dtype != self.rdtype or not dirty.is_contiguous() or dirty.device != self.device:
-            raise ValueError("dirty must be a contiguous %s tensor on %s" % (self.rdtype, self.device))
-        if wgt is not None:
-            wgt = wgt.reshape(self._shape_ms())
+        self._check("dirty", dirty, self.rdtype, (self.nbatch, self.npix_x, self.npix_y))
+        wgt = self._wgt(wgt)
         if out is None:
             out = torch.empty(self._shape_ms(), dtype=self.cdtype, device=self.device)
+        else:
+            if out.numel() != self.nrow * self.nchan * self.nbatch:
+                raise ValueError("out must have %d elements" % (self.nrow * self.nchan * self.nbatch))
+            self._check("out", out.reshape(self._shape_ms()) if out.is_contiguous() else out, self.cdtype,
+                        self._shape_ms())
         _lib.check(self._lib.hipng_geometry_dirty2ms(
             self._handle, ctypes.c_void_p(dirty.data_ptr()),
             ctypes.c_void_p(wgt.data_ptr()) if wgt is not None else None,
@@ -344,11 +374,13 @@ class Geometry:
         return out
 
     def stats(self):
-        """Diagnostics: visibilities, dense-run tasks, visibilities on the dense-run path and
-        whether that path is enabled for this geometry."""
-        out = (ctypes.c_int64 * 4)()
-        _lib.check(self._lib.hipng_geometry_stats(self._handle, out, 4))
-        return {"nvis": out[0], "dense_tasks": out[1], "dense_vis": out[2], "dense_enabled": bool(out[3])}
+        """Diagnostics: visibilities, dense-run tasks, visibilities on the dense-run path,
+        whether that path is enabled for this geometry, and whether its out-of-place FFT
+        plans were found to leave their inputs untouched."""
+        out = (ctypes.c_int64 * 5)()
+        _lib.check(self._lib.hipng_geometry_stats(self._handle, out, 5))
+        return {"nvis": out[0], "dense_tasks": out[1], "dense_vis": out[2], "dense_enabled": bool(out[3]),
+                "fft_keeps_input": bool(out[4])}
 
     def close(self):
         if getattr(self, "_handle", None) and self._handle.value:

@@ -32,9 +32,10 @@ def _sort():
 
 
 def _plan_struct(d):
+    """hipng_plan from the fixture's fields (diagnostic fields added later stay 0)."""
     p = _lib.HipngPlan()
     for name, _ in p._fields_:
-        setattr(p, name, d[name])
+        setattr(p, name, d.get(name, 0))
     return p
 
 
@@ -83,7 +84,12 @@ def test_plan_and_scales_reproduce_sort_fixture():
     ul = numpy.abs(a["uvw"]).max(axis=0) * a["freq"].max() / C
     wmin = numpy.abs(a["uvw"][:, 2]).min() * a["freq"].min() / C
     p = wgridder.plan_params(ul[0], ul[1], wmin, ul[2], 128, 128, 6e-4, 6e-4, 1e-5, True, "fp64")
-    assert p.as_dict() == plan
+    got = p.as_dict()
+    # every field the fixture holds (fields added to the plan since are diagnostics); the
+    # modelled error is no part of the sort contract and now also carries the transforms'
+    # roundoff floor (1e-12 here)
+    assert {k: got[k] for k in plan if k != "est_error"} == {k: v for k, v in plan.items() if k != "est_error"}
+    assert got["est_error"] == pytest.approx(plan["est_error"], rel=1e-5) and got["est_error"] <= 1e-5
     sx, sy, sw = wgridder.channel_scales(p, a["freq"])
     assert numpy.array_equal(sx, a["sx"]) and numpy.array_equal(sy, a["sy"])
     assert numpy.array_equal(sw, a["sw"])

@@ -25,9 +25,9 @@ def _case(uvw, nchan_freq, seed=0, nb=None):
     return ms, wgt
 
 
-def _extent_plan(uvw, freq, nx, ny, px, py, eps, wst, prec):
+def _extent_plan(uvw, freq, nx, ny, px, py, eps, wst, prec, plan_opts=None):
     e = wgridder.uvw_extent(uvw, freq)
-    return wgridder.plan_params(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wst, prec)
+    return wgridder.plan_params(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wst, prec, plan_opts=plan_opts)
 
 
 @pytest.mark.parametrize("wst", [False, True])
@@ -45,21 +45,20 @@ def test_bucket_sort_bit_exact(small_obs, wst, prec, nchan):
     assert numpy.array_equal(perm, rp)
 
 
-@pytest.mark.parametrize("pack,tile", [("1", "16"), ("0", "16"), ("1", "32")])
-def test_bucket_sort_bit_exact_production_subsample(pack, tile, monkeypatch):
+@pytest.mark.parametrize("pack,tile", [(True, 16), (False, 16), (True, 32)])
+def test_bucket_sort_bit_exact_production_subsample(pack, tile):
     """1 % of the 512-station / 64-channel production geometry, 8192^2 image plan; with the
     bucket key carried through the digit passes in the sort words (pack 1, the default when
     the bits fit) or recomputed from uvw after them (pack 0), and with 32 x 32 tiles (two
     digit passes, the second also sees low key bits)."""
-    monkeypatch.setenv("HIPNG_SORT_PACK", pack)
-    monkeypatch.setenv("HIPNG_TILE", tile)
+    opts = {"sort_pack": pack, "tile": tile}
     uvw, freq, _ = simulation.config1_geometry(ntimes=128, nchan=64)
     rng = numpy.random.default_rng(0)
     rows = numpy.sort(rng.choice(uvw.shape[0], uvw.shape[0] // 100, replace=False))
     uvw = uvw[rows]
-    plan = _extent_plan(uvw, freq, 8192, 8192, 1.1e-5, 1.1e-5, 1e-5, True, "mixed")
-    assert plan.tile == int(tile)
-    keys, perm = wgridder.bucket_sort(uvw, freq, plan)
+    plan = _extent_plan(uvw, freq, 8192, 8192, 1.1e-5, 1.1e-5, 1e-5, True, "mixed", opts)
+    assert plan.tile == tile
+    keys, perm = wgridder.bucket_sort(uvw, freq, plan, plan_opts=opts)
     sx, sy, sw = wgridder.channel_scales(plan, freq)
     rk, rp = sortref.bucket_sort(uvw, sx, sy, sw, plan.as_dict())
     assert numpy.array_equal(keys, rk)
@@ -268,14 +267,13 @@ def _dense_core(seed=31):
 
 
 @pytest.mark.parametrize("prec,eps", [("fp64", 1e-5), ("mixed", 1e-5), ("fp32", 1e-4)])
-def test_dense_runs_match_explicit(prec, eps, monkeypatch):
+def test_dense_runs_match_explicit(prec, eps):
     """Runs of visibilities with a common first cell go through the dense-run gridder; with
     the scattered rest through the sub-tile gatherer, invert and predict still equal the
     brute-force sums to epsilon. Oversampling 2 gives support 7 (lanes outside the
     footprint idle) and more than half of the visibilities in dense runs."""
     import torch
 
-    monkeypatch.setenv("HIPNG_SIGMA", "2.0")
     uvw, freq = _dense_core()
     rng = numpy.random.default_rng(32)
     nx, ny, px = 48, 48, 2e-3
@@ -283,7 +281,7 @@ def test_dense_runs_match_explicit(prec, eps, monkeypatch):
     img = rng.normal(size=(nx, ny))
     dev = torch.device("cuda", 0)
     cd, rd = (torch.complex128, torch.float64) if prec == "fp64" else (torch.complex64, torch.float32)
-    geom = wgridder.Geometry(uvw, freq, nx, ny, px, px, eps, True, prec)
+    geom = wgridder.Geometry(uvw, freq, nx, ny, px, px, eps, True, prec, plan_opts={"sigma": 2.0})
     st = geom.stats()
     assert st["dense_enabled"] and st["dense_tasks"] > 0
     assert 0.5 * st["nvis"] < st["dense_vis"] < st["nvis"]  # both paths in use
@@ -297,20 +295,19 @@ def test_dense_runs_match_explicit(prec, eps, monkeypatch):
 
 
 @pytest.mark.parametrize("nb", [2, 3, 4])
-def test_dense_runs_polarisations_match_explicit(nb, monkeypatch):
+def test_dense_runs_polarisations_match_explicit(nb):
     """The batched instances of both gridding paths and of both degridders (4 polarisations
     is the production shape's batch) against the brute-force sums, polarisation by
     polarisation."""
     import torch
 
-    monkeypatch.setenv("HIPNG_SIGMA", "2.0")
     uvw, freq = _dense_core(seed=33)
     rng = numpy.random.default_rng(34)
     nx, ny, px, eps = 48, 40, 2e-3, 1e-5
     ms = rng.normal(size=(uvw.shape[0], freq.size, nb)) + 1j * rng.normal(size=(uvw.shape[0], freq.size, nb))
     img = rng.normal(size=(nb, nx, ny))
     dev = torch.device("cuda", 0)
-    geom = wgridder.Geometry(uvw, freq, nx, ny, px, px, eps, True, "mixed", nbatch=nb)
+    geom = wgridder.Geometry(uvw, freq, nx, ny, px, px, eps, True, "mixed", nbatch=nb, plan_opts={"sigma": 2.0})
     st = geom.stats()
     assert st["dense_enabled"] and 0.5 * st["nvis"] < st["dense_vis"] < st["nvis"]
     d = geom.ms2dirty(torch.tensor(ms, dtype=torch.complex64, device=dev)).double().cpu().numpy()
@@ -355,22 +352,22 @@ def test_unaligned_values_match_aligned(small_obs):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_tile32_matches_explicit(prec, monkeypatch):
+def test_tile32_matches_explicit(prec):
     """32 x 32 sort tiles (two origin-digit passes, the T = 32 gatherer and degridder) on the
     dense-core geometry, against the brute-force sums."""
     import torch
 
-    monkeypatch.setenv("HIPNG_TILE", "32")
-    monkeypatch.setenv("HIPNG_SIGMA", "2.0")
+    opts = {"tile": 32, "sigma": 2.0}
     uvw, freq = _dense_core(seed=35)
     rng = numpy.random.default_rng(36)
     nx, ny, px, eps = 64, 48, 2e-3, 1e-5
     ms = rng.normal(size=(uvw.shape[0], freq.size)) + 1j * rng.normal(size=(uvw.shape[0], freq.size))
     img = rng.normal(size=(nx, ny))
     plan = {}
-    d = wgridder.ms2dirty(uvw, freq, ms, None, nx, ny, px, px, 0, 0, eps, True, precision=prec, plan_out=plan)
+    d = wgridder.ms2dirty(uvw, freq, ms, None, nx, ny, px, px, 0, 0, eps, True, precision=prec, plan_out=plan,
+                          plan_opts=opts)
     assert plan["tile"] == 32 and plan["dense_items"] > 0
-    v = wgridder.dirty2ms(uvw, freq, img, None, px, px, 0, 0, eps, True, precision=prec)
+    v = wgridder.dirty2ms(uvw, freq, img, None, px, px, 0, 0, eps, True, precision=prec, plan_opts=opts)
     ref_d = ox.explicit_ms2dirty(uvw, freq, ms, None, nx, ny, px, px, True)
     ref_v = ox.explicit_dirty2ms(uvw, freq, img, None, px, px, True)
     assert rel_l2(numpy.asarray(d, numpy.float64), ref_d) < eps

@@ -33,7 +33,9 @@ def _uvw_lambda(bvis):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 2e-7)])
-@pytest.mark.parametrize("frames", [("linear", "stokesIQUV"), ("circular", "stokesI"), ("stokesI", "stokesI")])
+@pytest.mark.parametrize("frames", [("linear", "stokesIQUV"), ("circular", "stokesI"), ("stokesI", "stokesI"),
+                                    ("linearnp", "stokesIQ"), ("circularnp", "stokesIV"), ("linearnp", "stokesI"),
+                                    ("circularnp", "stokesI")])
 def test_prologue_matches_oracle(precision, tol, frames):
     import torch
 
@@ -77,7 +79,8 @@ def test_prologue_matches_oracle(precision, tol, frames):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-15)])
-@pytest.mark.parametrize("frames", [("stokesIQUV", "linear"), ("stokesIQUV", "circular"), ("stokesI", "stokesI")])
+@pytest.mark.parametrize("frames", [("stokesIQUV", "linear"), ("stokesIQUV", "circular"), ("stokesI", "stokesI"),
+                                    ("stokesIQ", "linearnp"), ("stokesIV", "circularnp")])
 def test_epilogue_matches_oracle(precision, tol, frames):
     import torch
 

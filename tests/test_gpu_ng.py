@@ -51,6 +51,52 @@ def test_predict_unit_point_iquv():
     assert numpy.max(numpy.abs(vis.vis[..., 3] - 1.0)) < 1e-10
 
 
+@pytest.mark.parametrize("image_pol,vis_pol,pix,expect", [
+    ("stokesIQ", "linearnp", (1.0, 0.5), (1.5, 0.5)),    # XX = I + Q, YY = I - Q
+    ("stokesIV", "circularnp", (1.0, -0.25), (0.75, 1.25)),  # RR = I + V, LL = I - V
+])
+def test_predict_unit_point_two_pol(image_pol, vis_pol, pix, expect):
+    """test_imaging.py:295-318 (IQ, IV): a point at the image centre predicts constant
+    parallel-hand visibilities through the 2x2 conversion (polarisation.py:229-231, 379-393)."""
+    bvis = _obs(pol=vis_pol, zerow=True)
+    model = create_image(128, 5e-4, bvis.phasecentre, [1e8], PolarisationFrame(image_pol))
+    model.pixels[0, :, 64, 64] = pix
+    vis = predict_blockvisibility(bvis, model, context="2d", epsilon=1e-12)
+    assert vis.vis.shape[-1] == 2
+    for p in range(2):
+        assert numpy.max(numpy.abs(vis.vis[..., p] - expect[p])) < 1e-10
+
+
+@pytest.mark.parametrize("image_pol,vis_pol,flux", [("stokesIQ", "linearnp", [10.0, 2.0]),
+                                                    ("stokesIV", "circularnp", [10.0, -1.5])])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-6), ("mixed", 1e-4)])
+def test_invert_recovers_point_two_pol(image_pol, vis_pol, flux, precision, tol):
+    """test_imaging_ng.py:290-300 (invert_ng_IQ / _IV): a point source in a 2-pol frame
+    appears at its pixel with its flux ratio after the 2x2 conversion into the image frame."""
+    from rascil_amd.data_models import Skycomponent, lmn_to_direction
+
+    bvis = _obs(pol=vis_pol, ntimes=12, nchan=2)
+    N, cell = 128, 5e-4
+    iy, ix = 70, 40
+    l, m = -cell * (ix - N // 2), cell * (iy - N // 2)
+    sc = Skycomponent(lmn_to_direction(l, m, bvis.phasecentre), bvis.frequency,
+                      numpy.array([flux] * 2), PolarisationFrame(image_pol))
+    dft_skycomponent_visibility(bvis, sc)
+    model = create_image(N, cell, bvis.phasecentre, [1.05e8], PolarisationFrame(image_pol))
+    im, sumwt = invert_ng(bvis, model, epsilon=1e-10 if precision == "fp64" else 1e-5, precision=precision)
+    p = im.pixels[0]
+    assert numpy.unravel_index(numpy.argmax(p[0]), p[0].shape) == (iy, ix)
+    n = math.sqrt(1 - l * l - m * m)
+    numpy.testing.assert_allclose(p[:, iy, ix] * n, flux, rtol=tol, atol=tol * 10)
+    assert sumwt.shape == (1, 2)
+    # and the predict of that image returns the parallel-hand visibilities of the source
+    bvis2 = _obs(pol=vis_pol, ntimes=12, nchan=2)
+    mod = create_image(N, cell, bvis.phasecentre, [1.05e8], PolarisationFrame(image_pol))
+    mod.pixels[0, :, iy, ix] = numpy.array(flux) * n  # a pixel P predicts P / n times the phasor
+    pred = predict_ng(bvis2, mod, epsilon=1e-10 if precision == "fp64" else 1e-5, precision=precision)
+    assert rel_l2(pred.vis, bvis.vis) < (1e-8 if precision == "fp64" else 1e-4)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 1e-5)])
 def test_psf_peak_is_one(precision, tol):
     """test_imaging.py:601-612: PSF peak == 1 after normalisation."""

@@ -2,6 +2,7 @@
 logic, data-model accessors, pol conversion vs the oracle matrices, dispatch errors."""
 
 import ctypes
+import math
 import os
 import re
 
@@ -64,11 +65,73 @@ def test_plan_errors():
         wgridder.plan_params(1, 1, 0, 1, 2048, 2048, 1e-3, 1e-3, 1e-5, True)  # beyond the sphere
 
 
+C2_EXTENT = (34955.4, 38229.1, 0.0, 15237.9)  # 512 stations, 8 channels 100-156 MHz
+
+
+@pytest.mark.parametrize("prec,eps", [("mixed", 1e-6), ("mixed", 5e-6), ("fp32", 1e-4), ("fp32", 1e-3)])
+def test_accuracy_contract_refuses_unreachable_epsilon(prec, eps):
+    """Like ducc0 for single precision, an epsilon the mode cannot deliver raises instead of
+    being silently missed: mixed below 1e-5 (its FP32 grid/FFT floor is measured at
+    1.7e-6 - 3.2e-6 at 8192^2), FP32 below the phase error of FP32 grid coordinates at the
+    image edge (3.9e-3 for the 8192^2, 2.2e-5 rad image of the 134M-visibility study)."""
+    with pytest.raises(ValueError, match="epsilon"):
+        wgridder.plan_params(*C2_EXTENT, 8192, 8192, 2.2e-5, 2.2e-5, eps, True, prec, 1, 134_000_000)
+
+
+def test_accuracy_contract_floor_in_est_error():
+    """Accepted plans report the modelled total error, and it stays below epsilon."""
+    for prec, eps in [("fp64", 1e-6), ("mixed", 1e-5), ("fp32", 1e-2)]:
+        p = wgridder.plan_params(*C2_EXTENT, 8192, 8192, 2.2e-5, 2.2e-5, eps, True, prec, 1, 134_000_000)
+        assert p.est_error <= eps
+    pm = wgridder.plan_params(*C2_EXTENT, 8192, 8192, 2.2e-5, 2.2e-5, 1e-5, True, "mixed", 1, 134_000_000)
+    pd = wgridder.plan_params(*C2_EXTENT, 8192, 8192, 2.2e-5, 2.2e-5, 1e-5, True, "fp64", 1, 134_000_000)
+    assert pm.est_error > pd.est_error  # the FP32 roundoff floor is part of the model
+    # the FP32 coordinate floor grows with the image: a 64^2 image accepts 1e-4
+    p = wgridder.plan_params(600.0, 550.0, 1.0, 400.0, 64, 64, 2e-3, 2e-3, 1e-4, True, "fp32")
+    assert p.est_error <= 1e-4
+
+
+def test_plan_options_pin_the_plan():
+    base = dict(umax=600.0, vmax=550.0, wmin=1.0, wmax=400.0, npix_x=512, npix_y=512, pixsize_x=5e-4,
+                pixsize_y=5e-4, epsilon=1e-5, do_wstacking=True, precision="mixed")
+    auto = wgridder.plan_params(**base)
+    for sg in (1.5, 2.0, 2.5):
+        p = wgridder.plan_params(**base, plan_opts={"sigma": sg})
+        assert p.sigma == sg and p.est_error <= 1e-5
+    # at oversampling 1.25 the grid correction amplifies the FP32 transforms' roundoff ~1000x
+    # at the image corner: that sigma cannot deliver 1e-5 in mixed precision, FP64 can
+    with pytest.raises(ValueError, match="epsilon"):
+        wgridder.plan_params(**base, plan_opts={"sigma": 1.25})
+    assert wgridder.plan_params(**dict(base, precision="fp64"), plan_opts={"sigma": 1.25}).sigma == 1.25
+    p = wgridder.plan_params(**base, plan_opts={"sigma": 2.0, "W": 9})
+    assert p.W == 9
+    p = wgridder.plan_params(**base, plan_opts={"tile": 32})
+    assert p.tile == 32
+    assert wgridder.plan_params(**base, plan_opts={"dense_min": -1}).dense_min == 0
+    assert wgridder.plan_params(**base, plan_opts={"sigma": 2.0, "dense_min": 96}).dense_min == 96
+    assert auto.dense_min in (0, 256)
+    # an extent beyond the data's widens the plane range, never narrows it
+    p = wgridder.plan_params(**base, plan_opts={"sigma": auto.sigma, "extent": (0.0, 0.0, 0.0, 800.0)})
+    assert p.nplanes > auto.nplanes and p.umax == 600.0 and p.wmin == 0.0 and p.wmax == 800.0
+    with pytest.raises(ValueError):
+        wgridder.plan_params(**base, plan_opts={"sigma": 1.6})  # not a tabulated oversampling
+    with pytest.raises(ValueError):
+        wgridder.plan_params(**base, plan_opts={"sigma": 1.25, "W": 4})  # W too small for 1e-5
+    with pytest.raises(ValueError):
+        wgridder.plan_params(**base, plan_opts={"tile": 64})
+    with pytest.raises(ValueError):
+        wgridder.plan_params(**base, plan_opts={"oversampling": 2.0})  # unknown key
+
+
 def test_pol_conversion_matches_oracle():
     rng = numpy.random.default_rng(0)
     v = rng.normal(size=(7, 3, 4)) + 1j * rng.normal(size=(7, 3, 4))
     for a, b in [("linear", "stokesIQUV"), ("stokesIQUV", "linear"), ("circular", "stokesIQUV"),
-                 ("stokesIQUV", "circular"), ("linear", "stokesI"), ("stokesIQUV", "stokesI")]:
+                 ("stokesIQUV", "circular"), ("linear", "stokesI"), ("stokesIQUV", "stokesI"),
+                 ("linearnp", "stokesIQ"), ("stokesIQ", "linearnp"), ("circularnp", "stokesIV"),
+                 ("stokesIV", "circularnp"), ("linearnp", "stokesI"), ("circularnp", "stokesI")]:
+        nv = PolarisationFrame(a).npol
+        v = rng.normal(size=(7, 3, nv)) + 1j * rng.normal(size=(7, 3, nv))
         got = convert_pol_frame(v, PolarisationFrame(a), PolarisationFrame(b), polaxis=2)
         exp = ox.convert_pol(v, a, b, polaxis=2)
         numpy.testing.assert_allclose(got, exp, atol=1e-15)
@@ -82,6 +145,68 @@ def test_direction_roundtrip_and_lmn():
     l, m, nm1 = direction_to_lmn(d, pc)
     assert (l, m) == pytest.approx((0.01, -0.02), abs=1e-14)
     assert nm1 == pytest.approx(numpy.sqrt(1 - 0.01 ** 2 - 0.02 ** 2) - 1, abs=1e-14)
+
+
+def test_direction_to_lmn_closed_forms():
+    """Spherical-trigonometry special cases (coordinate_support.py:446-465 semantics): an RA
+    offset delta at declination d gives l = cos d sin delta > 0 (l increases east-wards, with
+    RA), m = sin d cos d (1 - cos delta); a declination offset gives l = 0, m = sin delta;
+    on the equator n - 1 = cos delta - 1; at the north pole l = cos dec sin ra,
+    m = -cos dec cos ra."""
+    d0, delta = math.radians(-35.0), math.radians(1.0)
+    l, m, nm1 = direction_to_lmn(Direction(math.pi + delta, d0), Direction(math.pi, d0))
+    assert l > 0
+    assert l == pytest.approx(math.cos(d0) * math.sin(delta), abs=1e-15)
+    assert m == pytest.approx(math.sin(d0) * math.cos(d0) * (1 - math.cos(delta)), abs=1e-15)
+    assert nm1 == pytest.approx(math.sin(d0) ** 2 + math.cos(d0) ** 2 * math.cos(delta) - 1, abs=1e-15)
+    l, m, nm1 = direction_to_lmn(Direction(0.4, d0 + delta), Direction(0.4, d0))
+    assert (l, m) == pytest.approx((0.0, math.sin(delta)), abs=1e-15)
+    assert nm1 == pytest.approx(math.cos(delta) - 1, abs=1e-15)
+    l, m, nm1 = direction_to_lmn(Direction(delta, 0.0), Direction(0.0, 0.0))
+    assert (l, m, nm1) == pytest.approx((math.sin(delta), 0.0, math.cos(delta) - 1), abs=1e-15)
+    ra, dec = 0.7, math.radians(80.0)
+    l, m, _ = direction_to_lmn(Direction(ra, dec), Direction(0.0, math.pi / 2))
+    assert (l, m) == pytest.approx((math.cos(dec) * math.sin(ra), -math.cos(dec) * math.cos(ra)), abs=1e-15)
+
+
+@pytest.mark.parametrize("comp_offset", [(1.0, 0.0), (1.3, 0.4)])
+def test_phase_rotation_tangent_false_matches_direct_prediction(comp_offset):
+    """test_dft_skycomponent_visibility.py:51-132 restated on synthetic stations: the DFT of a
+    component with phase centre (180, -35) deg, phase rotated with uvw re-projection to
+    (181, -35) deg, equals the DFT computed directly with phase centre (181, -35) at hour
+    angles shifted by -1 deg, vis and uvw (rtol 3e-6 in the reference). With the component at
+    the new centre (the reference's case) the rotated vis are constant; a component elsewhere
+    also ties the phasor's sign to the l axis of the uvw synthesis (east = +u)."""
+    from rascil_amd import simulation
+    from rascil_amd.imaging_base import phaserotate_visibility
+
+    rng = numpy.random.default_rng(3)
+    ants = simulation.gaussian_disc_stations(25, 100.0, 300.0, rng)
+    ha = (numpy.pi / 43200.0) * numpy.arange(0.0, 300.0, 100.0)
+    freq = numpy.linspace(1.0e8, 1.1e8, 3)
+    pc = Direction(math.pi, math.radians(-35.0))
+    newpc = Direction(math.radians(181.0), math.radians(-35.0))
+    comp = Direction(math.radians(180.0 + comp_offset[0]), math.radians(-35.0 + comp_offset[1]))
+    flux = numpy.array([[100.0], [80.0], [60.0]])
+
+    def dft(bvis):
+        l, m, nm1 = direction_to_lmn(comp, bvis.phasecentre)
+        return ox.dft_cpu_looped(numpy.array([[l, m, nm1]]), bvis.uvw_lambda, flux[None])
+
+    vis = simulation.create_blockvisibility(ants, ha, freq, pc)
+    vis.vis = dft(vis)
+    direct = simulation.create_blockvisibility(ants, ha - (newpc.ra - pc.ra), freq, newpc)
+    direct.vis = dft(direct)
+    rot = phaserotate_visibility(vis, newpc, tangent=False)
+    assert rot.phasecentre == newpc
+    numpy.testing.assert_allclose(rot.uvw, direct.uvw, rtol=3e-6, atol=1e-9)
+    numpy.testing.assert_allclose(rot.vis, direct.vis, rtol=3e-6, atol=1e-9)
+    if comp_offset == (1.0, 0.0):
+        numpy.testing.assert_allclose(rot.vis[..., 0], numpy.broadcast_to(flux[:, 0], rot.vis.shape[:3]),
+                                      rtol=1e-9)
+    # the tangent-plane rotation keeps uvw and phase centre, as on the imaging path
+    tan = phaserotate_visibility(vis, newpc, tangent=True)
+    assert numpy.array_equal(tan.uvw, vis.uvw) and tan.phasecentre == pc
 
 
 def test_blockvisibility_accessors_and_image():

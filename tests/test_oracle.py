@@ -125,3 +125,38 @@ def test_point_source_predict_matches_dft_with_1_over_n(small_obs):
     uvl = (ruvw * 1e8 / C)[None, :, None, :]
     vis = ox.dft_cpu_looped(numpy.array([[l, m, n - 1]]), uvl, numpy.array([[[2.5 / n]]]))
     numpy.testing.assert_allclose(ms[:, 0], vis[0, :, 0, 0], rtol=1e-11, atol=1e-11)
+
+
+def test_two_pol_matrices_known_answers():
+    """Hand-computed: I = 3, Q = 1 -> XX = I + Q = 4, YY = I - Q = 2 (TMS 4.58 restricted to
+    the parallel hands); I = 3, V = -1 -> RR = I + V = 2, LL = I - V = 4 (TMS 4.59); and
+    back, and to Stokes I = (XX + YY) / 2."""
+    iq = numpy.array([[3.0, 1.0]])
+    xy = ox.convert_pol(iq, "stokesIQ", "linearnp")
+    numpy.testing.assert_array_equal(xy, [[4.0, 2.0]])
+    numpy.testing.assert_array_equal(ox.convert_pol(xy, "linearnp", "stokesIQ"), iq)
+    numpy.testing.assert_array_equal(ox.convert_pol(xy, "linearnp", "stokesI"), [[3.0]])
+    iv = numpy.array([[3.0, -1.0]])
+    rl = ox.convert_pol(iv, "stokesIV", "circularnp")
+    numpy.testing.assert_array_equal(rl, [[2.0, 4.0]])
+    numpy.testing.assert_array_equal(ox.convert_pol(rl, "circularnp", "stokesIV"), iv)
+    numpy.testing.assert_array_equal(ox.convert_pol(rl, "circularnp", "stokesI"), [[3.0]])
+
+
+def test_two_pol_matrices_are_the_parallel_hands_of_the_four_pol_ones():
+    """The 2x2 conversions equal the 4x4 ones restricted to the parallel-hand correlations
+    (XX, YY: rows / columns 0, 3) and the Stokes pair the frame carries (I, Q: 0, 1 for
+    linear; I, V: 0, 3 for circular). The 4x4 linear matrix is itself pinned by the DFT
+    known answers above, so a transposed or sign-flipped 2x2 fails here."""
+    M = ox.POL_MATRICES
+    numpy.testing.assert_array_equal(M[("stokesIQ", "linearnp")],
+                                     M[("stokesIQUV", "linear")][numpy.ix_([0, 3], [0, 1])])
+    numpy.testing.assert_array_equal(M[("linearnp", "stokesIQ")],
+                                     M[("linear", "stokesIQUV")][numpy.ix_([0, 1], [0, 3])])
+    numpy.testing.assert_array_equal(M[("stokesIV", "circularnp")],
+                                     M[("stokesIQUV", "circular")][numpy.ix_([0, 3], [0, 3])])
+    numpy.testing.assert_array_equal(M[("circularnp", "stokesIV")],
+                                     M[("circular", "stokesIQUV")][numpy.ix_([0, 3], [0, 3])])
+    for a, b in [("stokesIQ", "linearnp"), ("stokesIV", "circularnp")]:
+        numpy.testing.assert_allclose(M[(b, a)] @ M[(a, b)], numpy.eye(2), atol=0)
+
