@@ -227,6 +227,20 @@ int32_t hipng_apply_gaintable(void* vis, double* weight, const double* flags, co
                               int32_t gain_nchan, int32_t nrec, const int32_t* time_to_gain,
                               int32_t inverse, void* stream);
 
+/* Device: hipng_apply_gaintable without any device allocation (no counterpart in the reference;
+ * for callers that cannot allocate: code under HIP graph capture, tight calibration loops,
+ * harnesses that forbid hipMalloc). Same arguments, kernels and results; the prepared gain
+ * matrices go to a static per-GPU device array of 8 MiB (nrec^2 x 16 + 4 bytes per gain-table
+ * entry: 123,361 entries of 2x2 gains) instead of the library's workspace, and a larger table
+ * is refused with HIPNG_EINVAL. All calls on one GPU share that array: calls that overlap on
+ * one GPU (several streams or threads) race. */
+int32_t hipng_apply_gaintable_noalloc(void* vis, double* weight, const double* flags, const int8_t* time_flagged,
+                                      int64_t ntimes, int32_t nbaselines, int32_t nchan, int32_t npol,
+                                      const int32_t* ant1, const int32_t* ant2, const void* gain,
+                                      const double* gain_weight, int32_t ngain_times, int32_t nant,
+                                      int32_t gain_nchan, int32_t nrec, const int32_t* time_to_gain,
+                                      int32_t inverse, void* stream);
+
 /* Device: the visibility passes of invert_ng before the gridder, fused
  * (rascil/processing_components/imaging/ng.py:169-181 -> shift_vis_to_image
  * imaging/base.py:67-106 with phaserotate_visibility visibility/base.py:296-357,

@@ -34,6 +34,7 @@ EXPORTS = (
     "hipng_dft",
     "hipng_weight_visibility",
     "hipng_apply_gaintable",
+    "hipng_apply_gaintable_noalloc",
     "hipng_vis_prologue",
     "hipng_vis_epilogue",
     "hipng_profile_enable",
@@ -196,6 +197,8 @@ def load():
         lib.hipng_apply_gaintable.argtypes = [c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp,
                                               c_vp, c_vp, c_i32, c_i32, c_i32, c_i32, c_vp, c_i32, c_vp]
         lib.hipng_apply_gaintable.restype = c_i32
+        lib.hipng_apply_gaintable_noalloc.argtypes = lib.hipng_apply_gaintable.argtypes
+        lib.hipng_apply_gaintable_noalloc.restype = c_i32
         lib.hipng_vis_prologue.argtypes = [c_vp, c_vp, c_vp, c_vp, pd, c_i64, c_i32, c_i32, c_i32, pd, pd, c_i32,
                                            c_vp, c_vp, c_vp, c_vp]
         lib.hipng_vis_prologue.restype = c_i32

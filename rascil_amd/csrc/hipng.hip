@@ -4370,6 +4370,29 @@ void launch_gains(const GainArgs& a, void* vis, double* weight, const double* fl
   LAUNCH_CHECK();
 }
 
+// Scratch of the allocation-free gain entry point: the prepared matrices live in this array,
+// one copy per GPU, held until the process ends. Host code uses the address
+// hipGetSymbolAddress gives for the current device, never the array's name (a host-side
+// stand-in, not a device address).
+constexpr size_t kGainScratchBytes = (size_t)8 << 20;
+__device__ __align__(256) unsigned char scratch[kGainScratchBytes];
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device (looked up on its first use there)
+void* gain_scratch() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) HIPNG_THROW(HIPNG_ERUNTIME, "device index %d out of range", dev);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) HIPNG_THROW(HIPNG_ERUNTIME, "hipGetSymbolAddress returned NULL for the gain scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+
 double* upload_fc(DevCtx& c, const double* freq, int32_t nchan, hipStream_t st) {
   if (!freq) HIPNG_THROW(HIPNG_EINVAL, "NULL freq");
   std::vector<double> h(nchan);
@@ -4574,6 +4597,25 @@ int32_t hipng_apply_gaintable(void* vis, double* weight, const double* flags, co
     void* buf = ws(c, S_GAIN, gain_scratch_bytes(a) + 256);
     PROF("apply_gains");
     launch_gains(a, vis, weight, flags, time_flagged, ant1, ant2, gain, gain_weight, time_to_gain, inverse, buf, st);
+  });
+}
+
+int32_t hipng_apply_gaintable_noalloc(void* vis, double* weight, const double* flags, const int8_t* time_flagged,
+                                      int64_t ntimes, int32_t nbaselines, int32_t nchan, int32_t npol,
+                                      const int32_t* ant1, const int32_t* ant2, const void* gain,
+                                      const double* gain_weight, int32_t ngain_times, int32_t nant,
+                                      int32_t gain_nchan, int32_t nrec, const int32_t* time_to_gain,
+                                      int32_t inverse, void* stream) {
+  return guarded([&] {
+    const GainArgs a = gain_args(ntimes, nbaselines, nchan, npol, ngain_times, nant, gain_nchan, nrec);
+    if (gain_scratch_bytes(a) > kGainScratchBytes)
+      HIPNG_THROW(HIPNG_EINVAL,
+                  "gain table of %d rows x %d antennas x %d channels needs %zu bytes of prepared matrices; "
+                  "the allocation-free entry point holds at most %zu (use hipng_apply_gaintable)",
+                  ngain_times, nant, gain_nchan, gain_scratch_bytes(a), kGainScratchBytes);
+    if (a.ntimes * a.nbl * (int64_t)a.nchan == 0 || ngain_times == 0) return;
+    launch_gains(a, vis, weight, flags, time_flagged, ant1, ant2, gain, gain_weight, time_to_gain, inverse,
+                 gain_scratch(), (hipStream_t)stream);
   });
 }
 
