@@ -170,6 +170,75 @@ int32_t hipng_geometry_dirty2ms(hipng_geometry geom, const void* dirty, const vo
                                 void* stream);
 int32_t hipng_geometry_destroy(hipng_geometry geom);
 
+/* Visibility columns in the BlockVisibility layout, read or written by the gridder itself:
+ * vis / weight / flags [nrow][nchan][npol] (row = time x baseline), the geometry covering
+ * column channels chan0 .. chan0 + (its nchan) - 1. The passes RASCIL's invert_ng / predict_ng
+ * run on the host around ducc0 (ng.py:56, 122-133, 169-190: copy, phase shift to the image
+ * centre, flag masking, polarisation conversion, casts) and the gain product of
+ * apply_gaintable (calibration/operations.py:135-137, V = g1 conj(g2) M, per-antenna scalar
+ * gains) are fused into the gridder's value gather (invert) and its unsort (predict):
+ *   invert:  x_p = vis_p (1 - f_p) [e^{+2 pi i f/c (u L + v M + w N)}] [conj(g1 conj(g2))],
+ *            ms_q = sum_p conv[q][p] x_p, wgt_q = weight_q (1 - f_q) (weight NULL: 1), q < image pols;
+ *   predict: y_p = sum_q conv[p][q] vis_q [e^{-2 pi i f/c (u L + v M + w N)}],
+ *            model_vis_p = y_p, vis_p = y_p [g1 conj(g2)],
+ * with (u, v, w) the geometry's own uvw (so a caller that negates u and w for the gridder folds
+ * that into L, M, N), g [nant][nchan] complex128 and ant1 / ant2 [nrow] (device). */
+enum { HIPNG_COMPLEX128 = 0, HIPNG_COMPLEX64 = 1 };
+enum { HIPNG_FLOAT64 = 0, HIPNG_FLOAT32 = 1 };
+enum { HIPNG_INT32 = 0, HIPNG_INT8 = 1 };
+typedef struct hipng_columns {
+  int32_t nchan;        /* channels of the columns */
+  int32_t chan0;        /* column channel of the geometry's first channel */
+  int32_t npol;         /* polarisations of the columns (>= the geometry's nbatch) */
+  int32_t vis_type;     /* HIPNG_COMPLEX128 / HIPNG_COMPLEX64 */
+  int32_t weight_type;  /* HIPNG_FLOAT64 / HIPNG_FLOAT32 */
+  int32_t flag_type;    /* HIPNG_INT32 / HIPNG_INT8 */
+  int32_t shift;        /* 1: apply the phase with the coefficients lmn */
+  int32_t nant;         /* antennas of gains (0: no gains) */
+  double lmn[3];        /* L, M, N */
+  double conv[32];      /* [npol_out][npol_in] complex as (re, im) pairs: columns -> image
+                           (invert, out = the geometry's nbatch), image -> columns (predict) */
+  int32_t psf;          /* invert: image polarisation 0 gets ms = 1, the others 0 (the weights
+                           still apply): the point spread function of invert_ng (ng.py:221-223) */
+  int32_t image_order;  /* 0: image arrays [nbatch][npix_x][npix_y] (ducc0), 1: [nbatch][npix_y][npix_x]
+                           (RASCIL's pixel order, no transpose) */
+  int32_t image_type;   /* HIPNG_FLOAT64 / HIPNG_FLOAT32 (order 0: the geometry's precision) */
+  int32_t reserved;
+  double scale[4];      /* invert, order 1: factor per image polarisation (1: none; 1 / sumwt
+                           normalises, 0 zeroes); order 0 and predict ignore it */
+  const void* gains;    /* device [nant][nchan] complex128, or NULL */
+  const int32_t* ant1;  /* device [nrow] */
+  const int32_t* ant2;  /* device [nrow] */
+} hipng_columns;
+
+/* Geometry invert from columns (no counterpart in ducc0: it replaces ng.py:169-190 + ms2dirty
+ * ng.py:232-248 on the device): dirty [nbatch][npix_x][npix_y] as hipng_geometry_ms2dirty of
+ * the ms / wgt above. Kernels only (graph capturable). */
+int32_t hipng_geometry_invert_columns(hipng_geometry geom, const hipng_columns* cols, const void* vis,
+                                      const void* weight, const void* flags, void* dirty, void* stream);
+
+/* Geometry predict into columns (dirty2ms ng.py:91-103 + ng.py:122-133 on the device): writes
+ * vis (and model_vis, nullable: the visibilities before the gains) for the geometry's
+ * channels. Kernels only (graph capturable). */
+int32_t hipng_geometry_predict_columns(hipng_geometry geom, const hipng_columns* cols, const void* dirty, void* vis,
+                                       void* model_vis, void* stream);
+
+/* Device: out[c][p] = sum over rows of weight (1 - flags) for columns [nrow][nchan][npol]
+ * (weight NULL: 1, flags NULL: 0), FP64, fixed summation order: the sumwt of invert_ng
+ * (ng.py:178-190 sums flagged imaging weights per channel and polarisation). */
+int32_t hipng_weight_sums(const void* weight, const void* flags, int64_t nrow, int32_t nchan, int32_t npol,
+                          int32_t weight_type, int32_t flag_type, double* out, void* stream);
+
+/* Device: gradient of a real loss with respect to per-antenna scalar gains through
+ * V = g[ant1] conj(g[ant2]) M (calibration/operations.py:135-137): grad_vis = dL/dV in torch's
+ * complex convention, model_vis = M, both [nrow][nchan][npol] of vis_type; grad_gains
+ * [nant][nchan] complex128 = sum over the rows of each antenna of G conj(M) g[other] (as ant1)
+ * and conj(G conj(M)) g[other] (as ant2), summed over polarisations. No counterpart in the
+ * reference (it has no gradients); FP64 atomics, so the summation order varies. */
+int32_t hipng_gain_gradient(const void* model_vis, const void* grad_vis, int32_t vis_type, const void* gains,
+                            const int32_t* ant1, const int32_t* ant2, int64_t nrow, int32_t nchan, int32_t npol,
+                            int32_t nant, void* grad_gains, void* stream);
+
 /* Diagnostics (no counterpart in the reference): out[0] = visibilities (rows x channels),
  * out[1] = dense-run tasks, out[2] = visibilities gridded by the dense-run path, out[3] = 1
  * if that path is enabled for this geometry (W <= 8, grid <= 32768 per axis), out[4] = 1 if
@@ -186,6 +255,17 @@ int32_t hipng_geometry_stats(hipng_geometry geom, int64_t* out, int32_t nout);
 int32_t hipng_dft(const double* direction_cosines, const void* flux, const double* uvw_lambda,
                   int32_t ncomp, int64_t ntimes_baselines, int32_t nchan, int32_t npol,
                   void* vis, void* stream);
+
+/* Device: FP64 inverse DFT of point components, replacing idft_visibility_skycomponent
+ * (rascil/processing_components/imaging/dft.py:335-380): per component k, channel c and
+ * polarisation p, flux[k][c][p] = sum_{t,b} w (1 - f) vis exp(+2 pi i (u l_k + v m_k + w (n_k-1)))
+ * / sum_{t,b} w (1 - f), 0 where that weight sum is not positive; weight_sum[c][p] receives the
+ * sum. vis complex128, weight f64 and flags int32 (or NULL) [t][b][c][p], uvw_lambda
+ * [t][b][c][3], direction_cosines [k][3] = (l, m, n-1); all device pointers. Deterministic
+ * (fixed-order reductions). */
+int32_t hipng_idft(const double* direction_cosines, const void* vis, const double* weight, const int32_t* flags,
+                   const double* uvw_lambda, int32_t ncomp, int64_t ntimes_baselines, int32_t nchan, int32_t npol,
+                   void* flux, double* weight_sum, void* stream);
 
 /* Device: imaging weights, replacing weight_visibility
  * (rascil/processing_components/imaging/weighting.py:28-52) ->

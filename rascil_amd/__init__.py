@@ -4,21 +4,22 @@ Public surface (mirrors the reference's names):
   invert_ng, predict_ng                      (rascil/processing_components/imaging/ng.py)
   invert_blockvisibility, predict_blockvisibility   (imaging/imaging.py)
   wgridder.ms2dirty, wgridder.dirty2ms       (the ducc0.wgridder surface ng.py calls)
-  dft_skycomponent_visibility                (imaging/dft.py)
+  dft_skycomponent_visibility, idft_visibility_skycomponent   (imaging/dft.py)
   weight_visibility                          (imaging/weighting.py)
   apply_gaintable, create_gaintable_from_blockvisibility, GainTable
                                              (calibration/operations.py)
   wgridder.Geometry                          plan once / execute many (hipGraph capturable)
   autograd.predict                           differentiable predict (gridder = adjoint)
-  BlockVisibility, Image, PolarisationFrame, Skycomponent, Direction
+  BlockVisibility, DeviceBlockVisibility (columns resident in GPU memory), Image,
+  PolarisationFrame, Skycomponent, Direction
 The compute runs in the hipng HIP library (rascil_amd/_hipng.so); nothing falls back to
 the CPU.
 """
 
 from .calibration import GainTable, apply_gaintable, create_gaintable_from_blockvisibility
-from .data_models import (BlockVisibility, Direction, Image, PolarisationFrame, Skycomponent,
-                          copy_visibility, create_image, get_parameter, image_is_canonical)
-from .dft import dft_skycomponent_visibility
+from .data_models import (BlockVisibility, DeviceBlockVisibility, Direction, Image, PolarisationFrame,
+                          Skycomponent, copy_visibility, create_image, get_parameter, image_is_canonical)
+from .dft import dft_skycomponent_visibility, idft_visibility_skycomponent
 from .imaging import invert_blockvisibility, predict_blockvisibility
 from .imaging_base import normalise_sumwt, shift_vis_to_image
 from .ng import invert_ng, predict_ng

@@ -31,7 +31,12 @@ EXPORTS = (
     "hipng_geometry_dirty2ms",
     "hipng_geometry_destroy",
     "hipng_geometry_stats",
+    "hipng_geometry_invert_columns",
+    "hipng_geometry_predict_columns",
+    "hipng_weight_sums",
+    "hipng_gain_gradient",
     "hipng_dft",
+    "hipng_idft",
     "hipng_weight_visibility",
     "hipng_apply_gaintable",
     "hipng_apply_gaintable_noalloc",
@@ -103,6 +108,36 @@ class HipngPlanOpts(ctypes.Structure):
     ]
 
     FIELDS = ("sigma", "W", "tile", "dense_min", "chunk_grid", "chunk_degrid", "sort_pack", "extent")
+
+
+HIPNG_COMPLEX128, HIPNG_COMPLEX64 = 0, 1
+HIPNG_FLOAT64, HIPNG_FLOAT32 = 0, 1
+HIPNG_INT32, HIPNG_INT8 = 0, 1
+
+
+class HipngColumns(ctypes.Structure):
+    """Mirror of ``hipng_columns`` in include/hipng.h."""
+
+    _fields_ = [
+        ("nchan", ctypes.c_int32),
+        ("chan0", ctypes.c_int32),
+        ("npol", ctypes.c_int32),
+        ("vis_type", ctypes.c_int32),
+        ("weight_type", ctypes.c_int32),
+        ("flag_type", ctypes.c_int32),
+        ("shift", ctypes.c_int32),
+        ("nant", ctypes.c_int32),
+        ("lmn", ctypes.c_double * 3),
+        ("conv", ctypes.c_double * 32),
+        ("psf", ctypes.c_int32),
+        ("image_order", ctypes.c_int32),
+        ("image_type", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("scale", ctypes.c_double * 4),
+        ("gains", ctypes.c_void_p),
+        ("ant1", ctypes.c_void_p),
+        ("ant2", ctypes.c_void_p),
+    ]
 
 
 def plan_opts(opts):
@@ -187,8 +222,20 @@ def load():
         lib.hipng_geometry_destroy.restype = c_i32
         lib.hipng_geometry_stats.argtypes = [c_vp, ctypes.POINTER(c_i64), c_i32]
         lib.hipng_geometry_stats.restype = c_i32
+        pcols = ctypes.POINTER(HipngColumns)
+        lib.hipng_geometry_invert_columns.argtypes = [c_vp, pcols, c_vp, c_vp, c_vp, c_vp, c_vp]
+        lib.hipng_geometry_invert_columns.restype = c_i32
+        lib.hipng_geometry_predict_columns.argtypes = [c_vp, pcols, c_vp, c_vp, c_vp, c_vp]
+        lib.hipng_geometry_predict_columns.restype = c_i32
+        lib.hipng_weight_sums.argtypes = [c_vp, c_vp, c_i64, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp]
+        lib.hipng_weight_sums.restype = c_i32
+        lib.hipng_gain_gradient.argtypes = [c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_i32, c_vp,
+                                            c_vp]
+        lib.hipng_gain_gradient.restype = c_i32
         lib.hipng_dft.argtypes = [c_vp, c_vp, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp, c_vp]
         lib.hipng_dft.restype = c_i32
+        lib.hipng_idft.argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_i32, c_i32, c_vp, c_vp, c_vp]
+        lib.hipng_idft.restype = c_i32
         lib.hipng_release.restype = c_i32
         lib.hipng_weight_visibility.argtypes = [c_vp, pd, c_i64, c_i64, c_i32, c_i32,
                                                 ctypes.POINTER(ctypes.c_int32), c_i32, c_i32, c_i32,

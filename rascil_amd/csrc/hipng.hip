@@ -2316,6 +2316,40 @@ __global__ __launch_bounds__(kBlock) void k_finalize_t(KP k, const A* __restrict
   }
 }
 
+// The same two passes for images in [b][iy][ix] order (the accumulator's own): no transpose,
+// and invert scales polarisation b by scale[b] (the normalisation by the sum of weights).
+struct Scale4 {
+  double s[4];
+};
+template <typename A, typename O>
+__global__ __launch_bounds__(kBlock) void k_finalize_yx(KP k, const A* __restrict__ acc, O* __restrict__ dirty,
+                                                        const double* __restrict__ phix,
+                                                        const double* __restrict__ phiy,
+                                                        const double* __restrict__ qz,
+                                                        const double* __restrict__ qc, int nq, Scale4 sc) {
+  const int ix = blockIdx.x * kBlock + threadIdx.x, iy = blockIdx.y;
+  if (ix >= k.nx) return;
+  const A f = corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq);
+  for (int b = 0; b < k.nb; ++b) {
+    const int64_t i = ((int64_t)b * k.ny + iy) * k.nx + ix;
+    dirty[i] = (O)((double)(acc[i] * f) * sc.s[b]);
+  }
+}
+template <typename I, typename A>
+__global__ __launch_bounds__(kBlock) void k_prescale_yx(KP k, const I* __restrict__ dirty, A* __restrict__ dc,
+                                                        const double* __restrict__ phix,
+                                                        const double* __restrict__ phiy,
+                                                        const double* __restrict__ qz,
+                                                        const double* __restrict__ qc, int nq) {
+  const int ix = blockIdx.x * kBlock + threadIdx.x, iy = blockIdx.y;
+  if (ix >= k.nx) return;
+  const A f = corr_at<A>(ix, iy, k, phix, phiy, qz, qc, nq);
+  for (int b = 0; b < k.nb; ++b) {
+    const int64_t i = ((int64_t)b * k.ny + iy) * k.nx + ix;
+    dc[i] = (A)dirty[i] * f;
+  }
+}
+
 // dcT[b][iy][ix] = dirty[b][ix][iy] * correction(ix, iy)
 template <typename I, typename A>
 __global__ __launch_bounds__(kBlock) void k_prescale_t(KP k, const I* __restrict__ dirty, A* __restrict__ dc,
@@ -2847,6 +2881,76 @@ __global__ void k_dft(const double* __restrict__ dc, const cplx<double>* __restr
     if (p < npol) vis[t * npol + p] = cplx<double>{ar[p], ai[p]};
 }
 
+// Inverse DFT of components (dft.py:335-380): per component k, channel c and polarisation p,
+// partial sums over a block's share of the (time, baseline) rows of
+//   w (1 - f) v exp(+2 pi i (u l_k + v m_k + w (n_k - 1)))   and   w (1 - f),
+// reduced in LDS in a fixed order; k_idft_final adds the blocks' partials in block order and
+// divides (0 where the weight sum is not positive). Deterministic: no atomics.
+constexpr int kIdftBlocks = 256;  // blocks per (component, channel) over the rows
+__global__ __launch_bounds__(kBlock) void k_idft_partial(const double* __restrict__ dc,
+                                                         const cplx<double>* __restrict__ vis,
+                                                         const double* __restrict__ wt,
+                                                         const int* __restrict__ flags,
+                                                         const double* __restrict__ uvwl, int64_t ntb, int nchan,
+                                                         int npol, cplx<double>* __restrict__ part,
+                                                         double* __restrict__ wpart) {
+  const int ch = (int)blockIdx.y, k = (int)blockIdx.z;
+  const double l = dc[3 * k], m = dc[3 * k + 1], nm1 = dc[3 * k + 2];
+  double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0}, ws[4] = {0, 0, 0, 0};
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < ntb; r += (int64_t)gridDim.x * kBlock) {
+    const int64_t t = r * nchan + ch;
+    const double ph = uvwl[3 * t] * l + uvwl[3 * t + 1] * m + uvwl[3 * t + 2] * nm1;
+    double sn, co;
+    sincospi(2.0 * ph, &sn, &co);
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      if (p < npol) {
+        const int64_t i = t * npol + p;
+        const double w = flags ? wt[i] * (double)(1 - flags[i]) : wt[i];
+        const cplx<double> x = vis[i];
+        ar[p] += w * (x.re * co - x.im * sn);
+        ai[p] += w * (x.re * sn + x.im * co);
+        ws[p] += w;
+      }
+    }
+  }
+  __shared__ double red[3 * 4][kBlock];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    red[3 * p][threadIdx.x] = ar[p];
+    red[3 * p + 1][threadIdx.x] = ai[p];
+    red[3 * p + 2][threadIdx.x] = ws[p];
+  }
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int q = 0; q < 12; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < npol) {
+    const int p = threadIdx.x;
+    const int64_t o = (((int64_t)k * nchan + ch) * npol + p) * gridDim.x + blockIdx.x;
+    part[o] = cplx<double>{red[3 * p][0], red[3 * p + 1][0]};
+    if (k == 0) wpart[((int64_t)ch * npol + p) * gridDim.x + blockIdx.x] = red[3 * p + 2][0];
+  }
+}
+
+__global__ void k_idft_final(const cplx<double>* __restrict__ part, const double* __restrict__ wpart, int ncomp,
+                             int nchan, int npol, int nblk, cplx<double>* __restrict__ flux,
+                             double* __restrict__ weight) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (k, ch, p)
+  if (e >= (int64_t)ncomp * nchan * npol) return;
+  const int64_t cp = e % ((int64_t)nchan * npol);
+  double w = 0.0, re = 0.0, im = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    w += wpart[cp * nblk + b];
+    re += part[e * nblk + b].re;
+    im += part[e * nblk + b].im;
+  }
+  flux[e] = w > 0.0 ? cplx<double>{re / w, im / w} : cplx<double>{0.0, 0.0};
+  if (e < (int64_t)nchan * npol) weight[e] = w;
+}
+
 // ---------------------------------------------------------------------------------
 // Gain application: V' = M1 V conj(M2) per (time, baseline, channel), M the gain of the
 // baseline's antennas (or its inverse) for the gain-table row of the visibility's time.
@@ -3005,6 +3109,207 @@ __global__ void k_apply_gains(cplx<double>* __restrict__ vis, double* __restrict
 // Visibility prologue / epilogue of invert_ng / predict_ng: flag mask, phase shift to the
 // image centre, polarisation conversion and the gridder's layout and precision, in one pass.
 // ---------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------
+// Visibility columns in the BlockVisibility layout [row][column channel][npv], read and written
+// by the value gather (invert) and the unsort (predict) directly, with the passes around the
+// gridder fused in: flag mask, imaging weights, phase shift, polarisation conversion,
+// per-antenna scalar gains and the cast to / from the gridder's precision.
+// ---------------------------------------------------------------------------------
+struct ColArgs {
+  int nchan_g, nchan_t, chan0;  // geometry channels = column channels chan0 .. chan0 + nchan_g - 1
+  int vis_f32, wgt_f32, flag_i8;
+  int shift;                    // phase exp(+-2 pi i f/c (u L + v M + w N)) on the geometry's uvw
+  double L, M, N;
+  int nant;                     // gains [nant][nchan_t] complex128 with ant1 / ant2 [row], or none
+  int psf;                      // invert: image polarisation 0 gets 1, the others 0 (weights kept)
+  double cre[16], cim[16];      // conversion [nout][nin]
+};
+
+__device__ __forceinline__ cplx<double> col_load(const void* p, int64_t i, int f32) {
+  if (f32) {
+    const float2 v = reinterpret_cast<const float2*>(p)[i];
+    return cplx<double>{v.x, v.y};
+  }
+  const double2 v = reinterpret_cast<const double2*>(p)[i];
+  return cplx<double>{v.x, v.y};
+}
+__device__ __forceinline__ void col_store(void* p, int64_t i, int f32, cplx<double> v) {
+  if (f32) reinterpret_cast<float2*>(p)[i] = make_float2((float)v.re, (float)v.im);
+  else reinterpret_cast<double2*>(p)[i] = make_double2(v.re, v.im);
+}
+__device__ __forceinline__ double col_real(const void* p, int64_t i, int f32) {
+  return f32 ? (double)reinterpret_cast<const float*>(p)[i] : reinterpret_cast<const double*>(p)[i];
+}
+__device__ __forceinline__ double col_keep(const void* f, int64_t i, int i8) {
+  if (!f) return 1.0;
+  return i8 ? 1.0 - (double)reinterpret_cast<const int8_t*>(f)[i] : 1.0 - (double)reinterpret_cast<const int*>(f)[i];
+}
+// the product of the row's antenna gains g1 conj(g2) (or its conjugate)
+__device__ __forceinline__ cplx<double> row_gain(const cplx<double>* __restrict__ gains, const int* __restrict__ ant1,
+                                                 const int* __restrict__ ant2, int64_t row, int ch, int nchan_t,
+                                                 bool adjoint) {
+  const cplx<double> g1 = gains[(int64_t)ant1[row] * nchan_t + ch], g2 = gains[(int64_t)ant2[row] * nchan_t + ch];
+  const cplx<double> c{g1.re * g2.re + g1.im * g2.im, g1.im * g2.re - g1.re * g2.im};  // g1 conj(g2)
+  return adjoint ? cplx<double>{c.re, -c.im} : c;
+}
+
+// invert: per sorted item, x_p = vis_p (1 - f_p) [* exp(+2 pi i ph)] [* conj(g1 conj(g2))],
+// y_q = sum_p C[q][p] x_p, w_q = weight_q (1 - f_q); then the value gather of k_values:
+// rec_v[j][q] = conj?(w_q y_q) exp(-2 pi i |w| nshift f/c). FP64 until the final store.
+template <typename R, int NI, int NO>
+__global__ void k_values_cols(const uint32_t* __restrict__ perm, int64_t n, const cplx<R>* __restrict__ rec_p,
+                              const uint32_t* __restrict__ flipb, const double* __restrict__ uvw,
+                              const double* __restrict__ fc, ColArgs a, const void* __restrict__ vis,
+                              const void* __restrict__ wgt, const void* __restrict__ flags,
+                              const cplx<double>* __restrict__ gains, const int* __restrict__ ant1,
+                              const int* __restrict__ ant2, cplx<R>* __restrict__ rec_v) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  const int64_t row = i / a.nchan_g;
+  const int chl = (int)(i - row * a.nchan_g), ch = a.chan0 + chl;
+  const int64_t ci = row * a.nchan_t + ch;
+  double keep[4];
+  cplx<double> x[4];
+#pragma unroll
+  for (int p = 0; p < NI; ++p) {
+    keep[p] = col_keep(flags, ci * NI + p, a.flag_i8);
+    const cplx<double> v = col_load(vis, ci * NI + p, a.vis_f32);
+    x[p] = cplx<double>{v.re * keep[p], v.im * keep[p]};
+  }
+  cplx<double> m{1.0, 0.0};
+  if (a.shift) {
+    double sn, cs;
+    const double ph = fc[chl] * (uvw[3 * row] * a.L + uvw[3 * row + 1] * a.M + uvw[3 * row + 2] * a.N);
+    sincospi(2.0 * ph, &sn, &cs);
+    m = cplx<double>{cs, sn};
+  }
+  if (gains) m = cmul(m, row_gain(gains, ant1, ant2, row, ch, a.nchan_t, true));
+  const bool flip = (flipb[j >> 5] >> (j & 31)) & 1u;
+  const cplx<R> pr = rec_p[j];
+  const double c = (double)pr.re, s = -(double)pr.im;
+  cplx<R> o[NO];
+#pragma unroll
+  for (int q = 0; q < NO; ++q) {
+    cplx<double> y{0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < NI; ++p) y = cadd(y, cmul(cplx<double>{a.cre[q * NI + p], a.cim[q * NI + p]}, x[p]));
+    y = cmul(y, m);
+    if (a.psf) y = cplx<double>{q == 0 ? 1.0 : 0.0, 0.0};
+    if (flip) y.im = -y.im;
+    const double w = (wgt ? col_real(wgt, ci * NI + q, a.wgt_f32) : 1.0) * keep[q];
+    y.re *= w;
+    y.im *= w;
+    o[q] = cplx<R>{(R)(y.re * c - y.im * s), (R)(y.re * s + y.im * c)};
+  }
+  store_vals<R, NO, false>(rec_v + j * NO, o);
+}
+
+// predict: the unsort of k_unsort into the columns: o_q = conj?(acc_q exp(2 pi i |w| nshift f/c)),
+// y_p = sum_q C[p][q] o_q [* exp(-2 pi i ph)]; model (nullable) = y; vis = y [* g1 conj(g2)]
+template <typename R, int NI, int NO>
+__global__ void k_unsort_cols(const uint32_t* __restrict__ perm, int64_t n, const cplx<R>* __restrict__ rec_p,
+                              const uint32_t* __restrict__ flipb, const cplx<R>* __restrict__ accs,
+                              const double* __restrict__ uvw, const double* __restrict__ fc, ColArgs a,
+                              const cplx<double>* __restrict__ gains, const int* __restrict__ ant1,
+                              const int* __restrict__ ant2, void* __restrict__ vis, void* __restrict__ model) {
+  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = perm[j];
+  const int64_t row = i / a.nchan_g;
+  const int chl = (int)(i - row * a.nchan_g), ch = a.chan0 + chl;
+  const int64_t ci = row * a.nchan_t + ch;
+  const bool flip = (flipb[j >> 5] >> (j & 31)) & 1u;
+  const cplx<R> pr = rec_p[j];
+  const double c = (double)pr.re, s = (double)pr.im;
+  cplx<R> av[NI];
+  load_vals<R, NI, false>(accs + j * NI, av);
+  cplx<double> o[4];
+#pragma unroll
+  for (int q = 0; q < NI; ++q) {
+    const double re = (double)av[q].re, im = (double)av[q].im;
+    o[q] = cplx<double>{re * c - im * s, flip ? -(re * s + im * c) : re * s + im * c};
+  }
+  cplx<double> m{1.0, 0.0};
+  if (a.shift) {
+    double sn, cs;
+    const double ph = fc[chl] * (uvw[3 * row] * a.L + uvw[3 * row + 1] * a.M + uvw[3 * row + 2] * a.N);
+    sincospi(-2.0 * ph, &sn, &cs);
+    m = cplx<double>{cs, sn};
+  }
+  const cplx<double> g = gains ? row_gain(gains, ant1, ant2, row, ch, a.nchan_t, false) : cplx<double>{1.0, 0.0};
+#pragma unroll
+  for (int p = 0; p < NO; ++p) {
+    cplx<double> y{0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < NI; ++q) y = cadd(y, cmul(cplx<double>{a.cre[p * NI + q], a.cim[p * NI + q]}, o[q]));
+    y = cmul(y, m);
+    if (model) col_store(model, ci * NO + p, a.vis_f32, y);
+    col_store(vis, ci * NO + p, a.vis_f32, gains ? cmul(y, g) : y);
+  }
+}
+
+// Sums over rows of weight (1 - flags) per (channel, polarisation) of columns [nrow][nchan][npol]:
+// per-block partials in a fixed order, then the blocks' partials added in block order.
+constexpr int kSumBlocks = 512;
+__global__ __launch_bounds__(kBlock) void k_wsum_partial(const void* __restrict__ wgt, const void* __restrict__ flags,
+                                                         int64_t nrow, int nchan, int npol, int wgt_f32, int flag_i8,
+                                                         double* __restrict__ part) {
+  const int ch = (int)blockIdx.y;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x; r < nrow; r += (int64_t)gridDim.x * kBlock) {
+    const int64_t b = (r * nchan + ch) * npol;
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      if (p < npol) acc[p] += (wgt ? col_real(wgt, b + p, wgt_f32) : 1.0) * col_keep(flags, b + p, flag_i8);
+  }
+  __shared__ double red[4][kBlock];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) red[p][threadIdx.x] = acc[p];
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int p = 0; p < 4; ++p) red[p][threadIdx.x] += red[p][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < npol) part[((int64_t)ch * npol + threadIdx.x) * gridDim.x + blockIdx.x] = red[threadIdx.x][0];
+}
+__global__ void k_wsum_final(const double* __restrict__ part, int64_t ncp, int nblk, double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= ncp) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[e * nblk + b];
+  out[e] = s;
+}
+
+// Gradient of a real loss through V = g1 conj(g2) M (per-antenna scalar gains, all
+// polarisations): with G the loss gradient with respect to V (torch's convention),
+// dL/dg[a1] += sum_p G_p conj(M_p) g[a2] and dL/dg[a2] += conj(sum_p G_p conj(M_p)) g[a1].
+// One thread per (row, channel); FP64 atomics into grad [nant][nchan].
+__global__ void k_gain_grad(const void* __restrict__ model, const void* __restrict__ grad, int vis_f32,
+                            const cplx<double>* __restrict__ gains, const int* __restrict__ ant1,
+                            const int* __restrict__ ant2, int64_t nrow, int nchan, int npol,
+                            double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nrow * nchan) return;
+  const int64_t row = i / nchan;
+  const int ch = (int)(i - row * nchan);
+  cplx<double> s{0.0, 0.0};
+  for (int p = 0; p < npol; ++p) {
+    const cplx<double> gv = col_load(grad, i * npol + p, vis_f32), mv = col_load(model, i * npol + p, vis_f32);
+    s = cadd(s, cmulc(gv, mv));
+  }
+  const int a1 = ant1[row], a2 = ant2[row];
+  const cplx<double> g1 = gains[(int64_t)a1 * nchan + ch], g2 = gains[(int64_t)a2 * nchan + ch];
+  const cplx<double> d1 = cmul(s, g2), d2 = cmul(cplx<double>{s.re, -s.im}, g1);
+  double* o1 = out + 2 * ((int64_t)a1 * nchan + ch);
+  double* o2 = out + 2 * ((int64_t)a2 * nchan + ch);
+  atomicAdd(o1, d1.re);
+  atomicAdd(o1 + 1, d1.im);
+  atomicAdd(o2, d2.re);
+  atomicAdd(o2 + 1, d2.im);
+}
+
 struct PolArgs {
   int64_t nrow;
   int nchan, nin, nout, shift;
@@ -3163,7 +3468,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_TASKTAB, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_PLANE2, S_LASTW, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_PLANE2, S_LASTW, S_IDFT, S_NSLOTS
 };
 
 struct FftKey {
@@ -3362,7 +3667,7 @@ struct Buffers {
 
 // Small device tables: sx, sy, sw, pn [nchan]; phix [nx], phiy [ny]; quadrature z, c.
 struct Tables {
-  double *sx, *sy, *sw, *pn, *phix, *phiy, *qz, *qc;
+  double *sx, *sy, *sw, *pn, *phix, *phiy, *qz, *qc, *fc;  // fc[c] = f_c / c0
   int nq;
 };
 
@@ -3370,7 +3675,7 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
   KernelFT ft;
   ft.init(P.W, P.beta);
   const int nq = (int)ft.z.size();
-  const size_t n = 4 * (size_t)nchan + P.nx + P.ny + 2 * (size_t)nq;
+  const size_t n = 5 * (size_t)nchan + P.nx + P.ny + 2 * (size_t)nq;
   std::vector<double> host(n, 0.0);
   double* h = host.data();
   channel_scales(P, freq, nchan, h, h + nchan, h + 2 * nchan, h + 3 * nchan);
@@ -3384,6 +3689,8 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
     hz[q] = ft.z[q];
     hc[q] = ft.c[q];
   }
+  double* hf = hc + nq;
+  for (int c = 0; c < nchan; ++c) hf[c] = freq[c] / kSpeedOfLight;
   auto* d = (double*)B.get(S_SMALL, n * sizeof(double));
   HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));  // host buffer is released on return
@@ -3396,6 +3703,7 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
   t.phiy = t.phix + P.nx;
   t.qz = t.phiy + P.ny;
   t.qc = t.qz + nq;
+  t.fc = t.qc + nq;
   t.nq = nq;
   return t;
 }
@@ -3982,10 +4290,63 @@ void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<d
     k_grid_dense<double, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
 }
 
+// Columns of one call (hipng_geometry_invert_columns / _predict_columns)
+struct ColCall {
+  ColArgs a;
+  int npv = 1;
+  bool image_yx = false, image_f32 = false;  // image arrays [nb][ny][nx] (else [nb][nx][ny]), element type
+  Scale4 scale{{1.0, 1.0, 1.0, 1.0}};
+  const void *vis = nullptr, *wgt = nullptr, *flags = nullptr;
+  const cplx<double>* gains = nullptr;
+  const int *ant1 = nullptr, *ant2 = nullptr;
+  void* model = nullptr;
+};
+
+// calls f(integral_constant<NI>, integral_constant<NO>) for 1 <= NI, NO <= 4
+template <typename F>
+void pol_dispatch(int ni, int no, F&& f) {
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  using I4 = std::integral_constant<int, 4>;
+  auto on_no = [&](auto nic) {
+    if (no == 1) f(nic, I1{});
+    else if (no == 2) f(nic, I2{});
+    else if (no == 3) f(nic, I3{});
+    else f(nic, I4{});
+  };
+  if (ni == 1) on_no(I1{});
+  else if (ni == 2) on_no(I2{});
+  else if (ni == 3) on_no(I3{});
+  else on_no(I4{});
+}
+
+template <typename R>
+void launch_values_cols(const Geom& g, const ColCall& cc, cplx<R>* recv, hipStream_t st) {
+  const auto* rp = (const cplx<R>*)g.rec_p;
+  pol_dispatch(cc.npv, g.nb, [&](auto ni, auto no) {
+    k_values_cols<R, decltype(ni)::value, decltype(no)::value><<<grid1(g.n), kBlock, 0, st>>>(
+        g.perm, g.n, rp, g.flipb, g.uvw, g.tb.fc, cc.a, cc.vis, cc.wgt, cc.flags, cc.gains, cc.ant1, cc.ant2, recv);
+  });
+  LAUNCH_CHECK();
+}
+
+template <typename R>
+void launch_unsort_cols(const Geom& g, const ColCall& cc, const cplx<R>* accs, void* vis, hipStream_t st) {
+  const auto* rp = (const cplx<R>*)g.rec_p;
+  pol_dispatch(g.nb, cc.npv, [&](auto ni, auto no) {
+    k_unsort_cols<R, decltype(ni)::value, decltype(no)::value><<<grid1(g.n), kBlock, 0, st>>>(
+        g.perm, g.n, rp, g.flipb, accs, g.uvw, g.tb.fc, cc.a, cc.gains, cc.ant1, cc.ant2, vis, cc.model);
+  });
+  LAUNCH_CHECK();
+}
+
 // R: kernel/grid real type; A: image accumulator type; F32C: coordinates/phase in FP32.
-// Kernel launches and rocFFT only: no allocation, no host synchronisation.
+// Kernel launches and rocFFT only: no allocation, no host synchronisation. `cc` (nullable): the
+// visibilities come from BlockVisibility-layout columns through the fused value gather.
 template <typename R, typename A, bool F32C>
-void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, hipStream_t st) {
+void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, hipStream_t st,
+                   const ColCall* cc = nullptr) {
   using O = typename std::conditional<std::is_same<R, double>::value, double, float>::type;
   const KP& k = g.k;
   const hipng_plan& P = g.P;
@@ -3995,11 +4356,12 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     LAUNCH_CHECK();
     return;
   }
-  if (!ms) HIPNG_THROW(HIPNG_EINVAL, "ms must not be NULL");
+  if (!ms && !(cc && cc->vis)) HIPNG_THROW(HIPNG_EINVAL, "ms must not be NULL");
   auto* recv = (cplx<R>*)g.recv;
   {
     PROF("values");
-    launch_values<R>(g, (const cplx<R>*)ms, (const R*)wgt, recv, st);
+    if (cc) launch_values_cols<R>(g, *cc, recv, st);
+    else launch_values<R>(g, (const cplx<R>*)ms, (const R*)wgt, recv, st);
     LAUNCH_CHECK();
   }
   auto* acc = (A*)g.accimg;  // the first screen update stores into it
@@ -4126,13 +4488,24 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     }
   }
   PROF("finalize");
-  k_finalize_t<A, O><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
-      k, acc, (O*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
+  if (cc && cc->image_yx) {
+    const dim3 fg((P.nx + kBlock - 1) / kBlock, P.ny);
+    if (cc->image_f32)
+      k_finalize_yx<A, float><<<fg, kBlock, 0, st>>>(k, acc, (float*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc,
+                                                      g.tb.nq, cc->scale);
+    else
+      k_finalize_yx<A, double><<<fg, kBlock, 0, st>>>(k, acc, (double*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz,
+                                                       g.tb.qc, g.tb.nq, cc->scale);
+  } else {
+    k_finalize_t<A, O><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
+        k, acc, (O*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
+  }
   LAUNCH_CHECK();
 }
 
 template <typename R, typename A, bool F32C>
-void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, hipStream_t st) {
+void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, hipStream_t st,
+                   const ColCall* cc = nullptr) {
   using I = typename std::conditional<std::is_same<R, double>::value, double, float>::type;
   if (g.n == 0) return;
   if (!dirty || !ms) HIPNG_THROW(HIPNG_EINVAL, "dirty and ms must not be NULL");
@@ -4141,8 +4514,18 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   auto* dc = (A*)g.accimg;
   {
     PROF("prescale");
-    k_prescale_t<I, A><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
-        k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
+    if (cc && cc->image_yx) {
+      const dim3 fg((P.nx + kBlock - 1) / kBlock, P.ny);
+      if (cc->image_f32)
+        k_prescale_yx<float, A><<<fg, kBlock, 0, st>>>(k, (const float*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz,
+                                                        g.tb.qc, g.tb.nq);
+      else
+        k_prescale_yx<double, A><<<fg, kBlock, 0, st>>>(k, (const double*)dirty, dc, g.tb.phix, g.tb.phiy,
+                                                         g.tb.qz, g.tb.qc, g.tb.nq);
+    } else {
+      k_prescale_t<I, A><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
+          k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
+    }
     LAUNCH_CHECK();
   }
   // the sorted accumulators need no clearing: the degridders store at each record's first
@@ -4231,24 +4614,75 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     }
   }
   PROF("unsort");
-  launch_unsort<R>(g, accs, (const R*)wgt, (cplx<R>*)ms, st);
+  if (cc) launch_unsort_cols<R>(g, *cc, accs, ms, st);
+  else launch_unsort<R>(g, accs, (const R*)wgt, (cplx<R>*)ms, st);
   LAUNCH_CHECK();
 }
 
-void exec_dispatch(const Geom& g, bool invert, const void* in, const void* wgt, void* out, hipStream_t st) {
+void exec_dispatch(const Geom& g, bool invert, const void* in, const void* wgt, void* out, hipStream_t st,
+                   const ColCall* cc = nullptr) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   if (dev != g.device) HIPNG_THROW(HIPNG_EINVAL, "geometry was built on device %d, current device is %d", g.device, dev);
   if (g.prec == HIPNG_FP64) {
-    if (invert) exec_ms2dirty<double, double, false>(g, in, wgt, out, st);
-    else exec_dirty2ms<double, double, false>(g, in, wgt, out, st);
+    if (invert) exec_ms2dirty<double, double, false>(g, in, wgt, out, st, cc);
+    else exec_dirty2ms<double, double, false>(g, in, wgt, out, st, cc);
   } else if (g.prec == HIPNG_MIXED) {
-    if (invert) exec_ms2dirty<float, double, false>(g, in, wgt, out, st);
-    else exec_dirty2ms<float, double, false>(g, in, wgt, out, st);
+    if (invert) exec_ms2dirty<float, double, false>(g, in, wgt, out, st, cc);
+    else exec_dirty2ms<float, double, false>(g, in, wgt, out, st, cc);
   } else {
-    if (invert) exec_ms2dirty<float, float, true>(g, in, wgt, out, st);
-    else exec_dirty2ms<float, float, true>(g, in, wgt, out, st);
+    if (invert) exec_ms2dirty<float, float, true>(g, in, wgt, out, st, cc);
+    else exec_dirty2ms<float, float, true>(g, in, wgt, out, st, cc);
   }
+}
+
+// a validated ColCall for geometry g from the caller's column description
+ColCall col_call(const Geom& g, const hipng_columns* c, bool invert) {
+  if (!c) HIPNG_THROW(HIPNG_EINVAL, "columns must not be NULL");
+  if (c->npol < 1 || c->npol > 4) HIPNG_THROW(HIPNG_EINVAL, "columns npol must be 1..4 (got %d)", c->npol);
+  if (g.nb > c->npol)
+    HIPNG_THROW(HIPNG_EINVAL, "the geometry's %d polarisations do not fit %d column polarisations", g.nb, c->npol);
+  if (c->nchan < 1 || c->chan0 < 0 || c->chan0 + g.nchan > c->nchan)
+    HIPNG_THROW(HIPNG_EINVAL, "geometry channels %d.. +%d outside the columns' %d", c->chan0, g.nchan, c->nchan);
+  if ((c->vis_type != HIPNG_COMPLEX128 && c->vis_type != HIPNG_COMPLEX64) ||
+      (c->weight_type != HIPNG_FLOAT64 && c->weight_type != HIPNG_FLOAT32) ||
+      (c->flag_type != HIPNG_INT32 && c->flag_type != HIPNG_INT8))
+    HIPNG_THROW(HIPNG_EINVAL, "unknown column element type");
+  if (c->nant < 0 || (c->nant > 0 && (!c->gains || !c->ant1 || !c->ant2)))
+    HIPNG_THROW(HIPNG_EINVAL, "gains need the gain array and both antenna index arrays");
+  ColCall cc;
+  std::memset(&cc.a, 0, sizeof(cc.a));
+  cc.a.nchan_g = g.nchan;
+  cc.a.nchan_t = c->nchan;
+  cc.a.chan0 = c->chan0;
+  cc.a.vis_f32 = c->vis_type == HIPNG_COMPLEX64;
+  cc.a.wgt_f32 = c->weight_type == HIPNG_FLOAT32;
+  cc.a.flag_i8 = c->flag_type == HIPNG_INT8;
+  cc.a.shift = c->shift ? 1 : 0;
+  cc.a.L = c->lmn[0];
+  cc.a.M = c->lmn[1];
+  cc.a.N = c->lmn[2];
+  cc.a.nant = c->nant;
+  const int nin = invert ? c->npol : g.nb, nout = invert ? g.nb : c->npol;
+  for (int k = 0; k < nin * nout; ++k) {
+    cc.a.cre[k] = c->conv[2 * k];
+    cc.a.cim[k] = c->conv[2 * k + 1];
+  }
+  cc.npv = c->npol;
+  cc.a.psf = c->psf ? 1 : 0;
+  if (c->image_order != 0 && c->image_order != 1) HIPNG_THROW(HIPNG_EINVAL, "image_order must be 0 or 1");
+  if (c->image_type != HIPNG_FLOAT64 && c->image_type != HIPNG_FLOAT32) HIPNG_THROW(HIPNG_EINVAL, "unknown image type");
+  cc.image_yx = c->image_order == 1;
+  cc.image_f32 = c->image_type == HIPNG_FLOAT32;
+  if (!cc.image_yx && cc.image_f32 != (g.prec != HIPNG_FP64))
+    HIPNG_THROW(HIPNG_EINVAL, "images in [npix_x][npix_y] order have the geometry's precision");
+  for (int b = 0; b < 4; ++b) cc.scale.s[b] = (invert && cc.image_yx && b < g.nb) ? c->scale[b] : 1.0;
+  if (c->nant > 0) {
+    cc.gains = (const cplx<double>*)c->gains;
+    cc.ant1 = c->ant1;
+    cc.ant2 = c->ant2;
+  }
+  return cc;
 }
 
 // dense-run tasks and the visibilities they hold (device reduction, one synchronisation)
@@ -4543,6 +4977,69 @@ int32_t hipng_geometry_dirty2ms(hipng_geometry geom, const void* dirty, const vo
   });
 }
 
+int32_t hipng_geometry_invert_columns(hipng_geometry geom, const hipng_columns* cols, const void* vis,
+                                      const void* weight, const void* flags, void* dirty, void* stream) {
+  return guarded([&] {
+    if (!geom || !dirty) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    ColCall cc = col_call(geom->g, cols, true);
+    if (geom->g.n > 0 && !vis) HIPNG_THROW(HIPNG_EINVAL, "vis must not be NULL");
+    cc.vis = vis;
+    cc.wgt = weight;
+    cc.flags = flags;
+    exec_dispatch(geom->g, true, vis, nullptr, dirty, (hipStream_t)stream, &cc);
+  });
+}
+
+int32_t hipng_geometry_predict_columns(hipng_geometry geom, const hipng_columns* cols, const void* dirty, void* vis,
+                                       void* model_vis, void* stream) {
+  return guarded([&] {
+    if (!geom || !dirty || !vis) HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    ColCall cc = col_call(geom->g, cols, false);
+    cc.model = model_vis;
+    exec_dispatch(geom->g, false, dirty, nullptr, vis, (hipStream_t)stream, &cc);
+  });
+}
+
+int32_t hipng_weight_sums(const void* weight, const void* flags, int64_t nrow, int32_t nchan, int32_t npol,
+                          int32_t weight_type, int32_t flag_type, double* out, void* stream) {
+  return guarded([&] {
+    if (!out || nrow < 0 || nchan < 1 || nchan > 65535 || npol < 1 || npol > 4)
+      HIPNG_THROW(HIPNG_EINVAL, "bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    DevCtx& c = ctx();
+    WsUse use(c, st);
+    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kSumBlocks, (nrow + kBlock - 1) / kBlock));
+    const int64_t ncp = (int64_t)nchan * npol;
+    auto* part = (double*)ws(c, S_IDFT, (size_t)ncp * nblk * sizeof(double) + 256);
+    PROF("weight_sums");
+    k_wsum_partial<<<dim3(nblk, nchan), kBlock, 0, st>>>(weight, flags, nrow, nchan, npol,
+                                                         weight_type == HIPNG_FLOAT32, flag_type == HIPNG_INT8, part);
+    LAUNCH_CHECK();
+    k_wsum_final<<<grid1(ncp), kBlock, 0, st>>>(part, ncp, nblk, out);
+    LAUNCH_CHECK();
+  });
+}
+
+int32_t hipng_gain_gradient(const void* model_vis, const void* grad_vis, int32_t vis_type, const void* gains,
+                            const int32_t* ant1, const int32_t* ant2, int64_t nrow, int32_t nchan, int32_t npol,
+                            int32_t nant, void* grad_gains, void* stream) {
+  return guarded([&] {
+    if (nrow < 0 || nchan < 1 || npol < 1 || npol > 4 || nant < 1) HIPNG_THROW(HIPNG_EINVAL, "bad sizes");
+    if (vis_type != HIPNG_COMPLEX128 && vis_type != HIPNG_COMPLEX64) HIPNG_THROW(HIPNG_EINVAL, "unknown vis type");
+    if (!grad_gains || !gains || !ant1 || !ant2 || (nrow > 0 && (!model_vis || !grad_vis)))
+      HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    PROF("gain_gradient");
+    k_zero<double><<<gridcap(2 * (int64_t)nant * nchan), kBlock, 0, st>>>((double*)grad_gains, 2 * (int64_t)nant * nchan);
+    LAUNCH_CHECK();
+    if (nrow == 0) return;
+    k_gain_grad<<<grid1(nrow * nchan), kBlock, 0, st>>>(model_vis, grad_vis, vis_type == HIPNG_COMPLEX64,
+                                                        (const cplx<double>*)gains, ant1, ant2, nrow, nchan, npol,
+                                                        (double*)grad_gains);
+    LAUNCH_CHECK();
+  });
+}
+
 int32_t hipng_geometry_stats(hipng_geometry geom, int64_t* out, int32_t nout) {
   return guarded([&] {
     if (!geom || !out || nout < 1) HIPNG_THROW(HIPNG_EINVAL, "geometry and output array required");
@@ -4578,6 +5075,36 @@ int32_t hipng_dft(const double* direction_cosines, const void* flux, const doubl
     hipStream_t st = (hipStream_t)stream;
     k_dft<<<grid1(nt), kBlock, 0, st>>>(direction_cosines, (const cplx<double>*)flux, uvw_lambda,
                                         ncomp, ntimes_baselines, nchan, npol, (cplx<double>*)vis);
+    LAUNCH_CHECK();
+  });
+}
+
+int32_t hipng_idft(const double* direction_cosines, const void* vis, const double* weight, const int32_t* flags,
+                   const double* uvw_lambda, int32_t ncomp, int64_t ntimes_baselines, int32_t nchan, int32_t npol,
+                   void* flux, double* weight_sum, void* stream) {
+  return guarded([&] {
+    if (npol < 1 || npol > 4 || nchan < 1 || ncomp < 0 || ntimes_baselines < 0)
+      HIPNG_THROW(HIPNG_EINVAL, "bad idft sizes");
+    if (ncomp == 0) return;
+    if (!direction_cosines || !vis || !weight || !uvw_lambda || !flux || !weight_sum)
+      HIPNG_THROW(HIPNG_EINVAL, "NULL argument");
+    if ((int64_t)ncomp * nchan > 65535LL * 65535LL || ncomp > 65535 || nchan > 65535)
+      HIPNG_THROW(HIPNG_EINVAL, "too many components or channels for one call");
+    hipStream_t st = (hipStream_t)stream;
+    DevCtx& c = ctx();
+    WsUse use(c, st);
+    const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(kIdftBlocks, (ntimes_baselines + kBlock - 1) / kBlock));
+    const int64_t np = (int64_t)ncomp * nchan * npol * nblk;
+    auto* buf = (char*)ws(c, S_IDFT, np * sizeof(cplx<double>) + (size_t)nchan * npol * nblk * sizeof(double) + 256);
+    auto* part = (cplx<double>*)buf;
+    auto* wpart = (double*)(buf + np * sizeof(cplx<double>));
+    PROF("idft");
+    k_idft_partial<<<dim3(nblk, nchan, ncomp), kBlock, 0, st>>>(direction_cosines, (const cplx<double>*)vis, weight,
+                                                                flags, uvw_lambda, ntimes_baselines, nchan, npol, part,
+                                                                wpart);
+    LAUNCH_CHECK();
+    k_idft_final<<<grid1((int64_t)ncomp * nchan * npol), kBlock, 0, st>>>(part, wpart, ncomp, nchan, npol, nblk,
+                                                                          (cplx<double>*)flux, weight_sum);
     LAUNCH_CHECK();
   });
 }

@@ -220,8 +220,139 @@ class BlockVisibility:
         return new
 
 
-def copy_visibility(vis: BlockVisibility, zero=False) -> BlockVisibility:
-    """Deep copy, optionally zeroing vis (rascil/processing_components/visibility/base.py:105-118)."""
+class DeviceBlockVisibility:
+    """A BlockVisibility whose columns live in GPU memory as torch tensors.
+
+    Same fields, shapes and accessors as BlockVisibility (memory_data_models.py:1214-1529):
+    vis [time, baseline, chan, pol] (complex128, or complex64 to halve the memory),
+    uvw [time, baseline, 3] float64 metres, weight / imaging_weight [time, baseline, chan, pol]
+    (float64 or float32), flags int8 (or None when nothing is flagged), frequency and the
+    other small metadata on the host. invert_ng / predict_ng read and write these columns in
+    place on the device: no host copies, and the geometry (sort, records, plans) of a
+    (uvw, frequency, image, accuracy) combination is kept between calls. copy() shares
+    every column but vis (columns are treated as immutable; replace them, do not modify
+    them in place)."""
+
+    def __init__(self, vis, uvw, frequency, polarisation_frame, phasecentre, weight=None, imaging_weight=None,
+                 flags=None, channel_bandwidth=None, time=None, baselines=None):
+        import torch
+
+        if not (hasattr(vis, "is_cuda") and vis.is_cuda):
+            raise ValueError("DeviceBlockVisibility needs GPU tensors (use from_host for numpy data)")
+        nt, nb, nc, npol = vis.shape
+        if vis.dtype not in (torch.complex128, torch.complex64) or not vis.is_contiguous():
+            raise ValueError("vis must be a contiguous complex128 or complex64 tensor")
+        assert polarisation_frame.npol == npol, "pol frame does not match vis shape"
+        self.vis = vis
+        self.uvw = uvw.reshape(nt, nb, 3).to(device=vis.device, dtype=torch.float64).contiguous()
+        self.frequency = numpy.asarray(frequency, dtype="f8").reshape(nc)
+        self.polarisation_frame = polarisation_frame
+        self.phasecentre = phasecentre
+        rdt = torch.float64 if vis.dtype == torch.complex128 else torch.float32
+        self.weight = (torch.ones(vis.shape, dtype=rdt, device=vis.device) if weight is None
+                       else weight.to(vis.device).contiguous())
+        self.imaging_weight = self.weight if imaging_weight is None else imaging_weight.to(vis.device).contiguous()
+        for name in ("weight", "imaging_weight"):
+            t = getattr(self, name)
+            if tuple(t.shape) != tuple(vis.shape) or t.dtype not in (torch.float64, torch.float32):
+                raise ValueError(f"{name} must be float64 or float32 of the vis shape")
+        self.flags = None if flags is None else flags.to(device=vis.device, dtype=torch.int8).contiguous()
+        if self.flags is not None and tuple(self.flags.shape) != tuple(vis.shape):
+            raise ValueError("flags must have the vis shape")
+        self.channel_bandwidth = (numpy.ones(nc) * 1e6 if channel_bandwidth is None
+                                  else numpy.asarray(channel_bandwidth))
+        self.time = numpy.arange(nt, dtype="f8") if time is None else numpy.asarray(time)
+        self.baselines = baselines
+        self.attrs = {"phasecentre": phasecentre}
+
+    @classmethod
+    def from_host(cls, bvis: BlockVisibility, device=None, low_precision=False):
+        """Upload a BlockVisibility; low_precision stores vis complex64 and weights float32."""
+        import torch
+
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        cdt = torch.complex64 if low_precision else torch.complex128
+        rdt = torch.float32 if low_precision else torch.float64
+        up = lambda a, dt: torch.from_numpy(numpy.ascontiguousarray(a)).to(device=device, dtype=dt)  # noqa: E731
+        flags = numpy.asarray(bvis.flags)
+        return cls(up(bvis.vis, cdt), up(bvis.uvw, torch.float64), bvis.frequency, bvis.polarisation_frame,
+                   bvis.phasecentre, weight=up(bvis.weight, rdt), imaging_weight=up(bvis.imaging_weight, rdt),
+                   flags=up(flags.astype(numpy.int8), torch.int8) if flags.any() else None,
+                   channel_bandwidth=bvis.channel_bandwidth, time=bvis.time, baselines=bvis.baselines)
+
+    def to_host(self) -> BlockVisibility:
+        """The columns copied back into a (numpy) BlockVisibility."""
+        dn = lambda t: t.detach().cpu().numpy()  # noqa: E731
+        b = BlockVisibility(dn(self.vis).astype("c16"), dn(self.uvw), self.frequency, self.polarisation_frame,
+                            self.phasecentre, weight=dn(self.weight).astype("f8"),
+                            imaging_weight=dn(self.imaging_weight).astype("f8"),
+                            flags=None if self.flags is None else dn(self.flags).astype("int"),
+                            channel_bandwidth=self.channel_bandwidth, time=self.time, baselines=self.baselines)
+        b.attrs = dict(self.attrs)
+        return b
+
+    def __getitem__(self, name):
+        return _Field(self, name)
+
+    @property
+    def blockvisibility_acc(self):
+        return _DeviceBVisAccessor(self)
+
+    @property
+    def nvis(self):
+        return self.vis.numel()
+
+    @property
+    def uvw_lambda(self):
+        """uvw * f / c, [time, baseline, chan, 3] float64 on the device."""
+        import torch
+
+        k = torch.tensor(self.frequency / C_M_S, dtype=torch.float64, device=self.vis.device)
+        return self.uvw[..., None, :] * k[None, None, :, None]
+
+    def with_vis(self, vis):
+        """A new visibility set with the given vis tensor, sharing every other column."""
+        if tuple(vis.shape) != tuple(self.vis.shape) or vis.device != self.vis.device:
+            raise ValueError("vis must have the shape and device of the visibility set")
+        new = copy.copy(self)
+        new.attrs = dict(self.attrs)
+        new.vis = vis.contiguous()
+        return new
+
+    def copy(self, zero=False):
+        """A new visibility set sharing every column but vis (a copy, or zeros)."""
+        import torch
+
+        new = copy.copy(self)
+        new.attrs = dict(self.attrs)
+        new.vis = torch.zeros_like(self.vis) if zero else self.vis.clone()
+        return new
+
+
+class _DeviceBVisAccessor(_BVisAccessor):
+    """blockvisibility_acc of a DeviceBlockVisibility (torch tensors)."""
+
+    def _keep(self):
+        f = self._b.flags
+        return 1 if f is None else (1 - f.to(self._b.weight.dtype))
+
+    @property
+    def flagged_vis(self):
+        return self._b.vis * self._keep()
+
+    @property
+    def flagged_weight(self):
+        return self._b.weight * self._keep()
+
+    @property
+    def flagged_imaging_weight(self):
+        return self._b.imaging_weight * self._keep()
+
+
+def copy_visibility(vis, zero=False):
+    """Deep copy, optionally zeroing vis (rascil/processing_components/visibility/base.py:105-118);
+    a DeviceBlockVisibility shares its other columns (see DeviceBlockVisibility.copy)."""
     return vis.copy(zero=zero)
 
 
@@ -261,11 +392,13 @@ class _ImageAccessor:
 
 
 class Image:
-    """Image [chan, pol, ny, nx] float64 with RASCIL's canonical WCS conventions."""
+    """Image [chan, pol, ny, nx] float64 with RASCIL's canonical WCS conventions. The pixels
+    may also be a GPU torch tensor (float64 or float32): invert_ng / predict_ng then read
+    and write them on the device."""
 
     def __init__(self, pixels, cellsize, phasecentre, frequency, polarisation_frame,
                  channel_bandwidth=None):
-        self.pixels = numpy.asarray(pixels)
+        self.pixels = pixels if (hasattr(pixels, "is_cuda") and pixels.is_cuda) else numpy.asarray(pixels)
         nchan, npol, ny, nx = self.pixels.shape
         assert polarisation_frame.npol == npol
         self.cellsize = float(cellsize)
@@ -297,12 +430,13 @@ class Image:
 
 
 def image_is_canonical(im: Image) -> bool:
-    """4-D, RA---SIN / DEC--SIN / STOKES / FREQ (image/operations.py:75-102)."""
+    """4-D, RA---SIN / DEC--SIN / STOKES / FREQ or MOMENT (image/operations.py:75-102)."""
     if im is None:
         return True
     if len(im.pixels.shape) != 4:
         return False
-    return im.wcs.ctype == ["RA---SIN", "DEC--SIN", "STOKES", "FREQ"]
+    ct = im.wcs.ctype
+    return ct[:3] == ["RA---SIN", "DEC--SIN", "STOKES"] and ct[3] in ("FREQ", "MOMENT")
 
 
 def create_image(npixel, cellsize, phasecentre, frequency, polarisation_frame=None,
@@ -318,11 +452,11 @@ def create_image(npixel, cellsize, phasecentre, frequency, polarisation_frame=No
 
 
 def get_parameter(kwargs, key, default=None):
-    """kwargs lookup with default (rascil/data_models/parameters.py:108-125)."""
+    """kwargs lookup with default (rascil/data_models/parameters.py:108-125): a key that is
+    present is returned as given, None included."""
     if kwargs is None:
         return default
-    value = kwargs.get(key, default)
-    return default if value is None else value
+    return kwargs[key] if key in kwargs else default
 
 
 @dataclass

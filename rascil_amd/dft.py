@@ -1,12 +1,14 @@
-"""Point-source DFT on the GPU (hipng_dft kernel).
+"""Point-source DFT and inverse DFT on the GPU (hipng_dft / hipng_idft kernels).
 
 Replaces dft_skycomponent_visibility / extract_direction_and_flux / dft_kernel of
 rascil/processing_components/imaging/dft.py:44-178 (CPU loop :261-281, CUDA kernel
 :181-258): vis[t, b, c, p] = sum_k flux[k, c, p] exp(-2 pi i (u l_k + v m_k + w (n_k - 1))),
-FP64, no 1/n factor. Every dft_compute_kernel choice runs the same HIP kernel.
+FP64, no 1/n factor; every dft_compute_kernel choice runs the same HIP kernel. And
+idft_visibility_skycomponent (:335-380): the weighted mean of vis times the conjugate phasor.
 """
 
 import collections.abc
+import copy
 import ctypes
 
 import numpy
@@ -29,7 +31,17 @@ def extract_direction_and_flux(sc, vis: BlockVisibility):
         if not (len(comp.frequency) == len(vis.frequency)
                 and numpy.allclose(comp.frequency, vis.frequency, rtol=1e-15)):
             if flux.shape[0] > 1:
-                raise ValueError("frequency interpolation of component fluxes is not supported")
+                # cubic interpolation per polarisation onto the visibility channels (:94-107);
+                # like the reference this interpolates the component's own flux (before the
+                # polarisation conversion above) into a real array
+                from scipy import interpolate
+
+                cflux = numpy.asarray(comp.flux)
+                vflux = numpy.zeros([len(vis.frequency), cflux.shape[1]])
+                for pol in range(cflux.shape[1]):
+                    fint = interpolate.interp1d(comp.frequency, cflux[:, pol], kind="cubic")
+                    vflux[:, pol] = fint(vis.frequency)
+                flux = vflux
         vfluxes.append(numpy.broadcast_to(flux, (len(vis.frequency), flux.shape[-1])))
         l, m, _ = direction_to_lmn(comp.direction, vis.phasecentre)
         dcs.append([l, m, numpy.sqrt(1 - l ** 2 - m ** 2) - 1.0])
@@ -93,3 +105,50 @@ def dft_skycomponent_visibility(vis: BlockVisibility, sc, **kwargs) -> BlockVisi
     direction_cosines, vfluxes = extract_direction_and_flux(sc, vis)
     vis.vis = dft_kernel(direction_cosines, vfluxes, vis.uvw_lambda, **kwargs)
     return vis
+
+
+def idft_visibility_skycomponent(vis: BlockVisibility, sc):
+    """Inverse DFT components from the visibilities (dft.py:335-380): for each component the
+    flux [nchan, npol] = sum over times and baselines of w (1 - f) vis conj(phasor) divided by
+    the sum of w (1 - f) (0 where that is not positive), converted back to the component's
+    polarisation frame; returns (components, weights) like the reference. Runs hipng_idft."""
+    import torch
+
+    if sc is None:
+        return sc
+    if not isinstance(sc, collections.abc.Iterable):
+        sc = [sc]
+    if not torch.cuda.is_available():
+        raise RuntimeError("hipng inverse DFT needs a ROCm GPU")
+    lib = _lib.load()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    ntimes, nbl, nchan, npol = vis.vis.shape
+    dcs = []
+    for comp in sc:
+        l, m, _ = direction_to_lmn(comp.direction, vis.phasecentre)
+        dcs.append([l, m, numpy.sqrt(1 - l ** 2 - m ** 2) - 1.0])
+    ncomp = len(dcs)
+    dc = torch.tensor(numpy.array(dcs, "f8").reshape(ncomp, 3), device=dev)
+    v = torch.from_numpy(numpy.ascontiguousarray(vis.vis, "c16")).to(dev)
+    w = torch.from_numpy(numpy.ascontiguousarray(vis.weight, "f8")).to(dev)
+    f = torch.from_numpy(numpy.ascontiguousarray(vis.flags, numpy.int32)).to(dev)
+    uv = torch.from_numpy(numpy.ascontiguousarray(vis.uvw_lambda, "f8")).to(dev)
+    flux = torch.empty((ncomp, nchan, npol), dtype=torch.complex128, device=dev)
+    wsum = torch.empty((nchan, npol), dtype=torch.float64, device=dev)
+    _lib.check(lib.hipng_idft(ctypes.c_void_p(dc.data_ptr()), ctypes.c_void_p(v.data_ptr()),
+                              ctypes.c_void_p(w.data_ptr()), ctypes.c_void_p(f.data_ptr()),
+                              ctypes.c_void_p(uv.data_ptr()), ncomp, ntimes * nbl, nchan, npol,
+                              ctypes.c_void_p(flux.data_ptr()), ctypes.c_void_p(wsum.data_ptr()),
+                              ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)))
+    flux = flux.cpu().numpy()
+    weight = wsum.cpu().numpy()
+    newsc, weights = [], []
+    for k, comp in enumerate(sc):
+        newcomp = copy.deepcopy(comp)
+        fk = flux[k]
+        if comp.polarisation_frame != vis.blockvisibility_acc.polarisation_frame:
+            fk = convert_pol_frame(fk, vis.blockvisibility_acc.polarisation_frame, comp.polarisation_frame)
+        newcomp.flux = fk
+        newsc.append(newcomp)
+        weights.append(weight)
+    return newsc, weights

@@ -2,32 +2,59 @@
 
 Drop-in for rascil/processing_components/imaging/ng.py (predict_ng :33-133,
 invert_ng :136-282): same signatures, keyword parameters (threads, epsilon,
-do_wstacking, verbosity), conventions (u and w negated before the gridder, images
-transposed to [x, y], MFS when the image has one channel, per-channel otherwise via the
-WCS frequency axis) and results. The ducc0.wgridder calls are replaced by
-rascil_amd.wgridder, which runs on the GPU and processes all polarisations of a call in
-one batched pass instead of one call per polarisation. The visibility passes around the
-gridder (phase shift to the image centre, flag mask, polarisation conversion, precision
-and layout) run as one fused GPU pass each way (hipng_vis_prologue / hipng_vis_epilogue).
+do_wstacking, verbosity), conventions (u and w negated before the gridder, images in
+RASCIL's [pol, y, x] pixel order, MFS when the image has one channel, per-channel otherwise
+via the WCS frequency axis) and results. The ducc0.wgridder calls and the host passes
+around them (copies, shift_vis_to_image, flag masking, convert_pol_frame, casts,
+transposes, normalise_sumwt) become one gridder call per image channel that reads and
+writes the visibility columns in place (hipng_geometry_invert_columns /
+_predict_columns): all polarisations of a call in one batched pass.
+
+Inputs: a BlockVisibility (numpy; uploaded per call) or a DeviceBlockVisibility (columns
+resident in GPU memory; nothing crosses PCIe but the small metadata, and the gridder's
+geometry for the same uvw, frequencies, image and accuracy is kept between calls, so a
+predict after an invert of the same visibilities does not sort again). Image pixels may be
+numpy or a GPU tensor.
 
 Extra keyword: precision = "fp64" (default; the reference computes in FP64 with FP64
 accumulation), "mixed" or "fp32".
 """
 
-import ctypes
+import collections
 import logging
 
 import numpy
 
 from . import _lib, wgridder
-from .data_models import (BlockVisibility, Image, copy_visibility, direction_to_lmn, get_parameter,
-                          image_is_canonical)
-from .imaging_base import normalise_sumwt
+from .data_models import DeviceBlockVisibility, direction_to_lmn, get_parameter, image_is_canonical
 from .polarisation import pol_matrix
 
 log = logging.getLogger("rascil-logger")
 
-__all__ = ["predict_ng", "invert_ng"]
+__all__ = ["predict_ng", "invert_ng", "clear_geometry_cache", "set_geometry_cache_size"]
+
+_CACHE = collections.OrderedDict()
+_CACHE_SIZE = [1]
+
+
+def set_geometry_cache_size(n):
+    """Number of gridder geometries kept for device-resident visibilities (0 disables)."""
+    _CACHE_SIZE[0] = max(0, int(n))
+    while len(_CACHE) > _CACHE_SIZE[0]:
+        _CACHE.popitem(last=False)[1][1].close()
+
+
+def clear_geometry_cache():
+    """Release the cached geometries (and their device memory)."""
+    while _CACHE:
+        _CACHE.popitem(last=False)[1][1].close()
+
+
+def _conv(ipf, opf):
+    """Conversion matrix [opf.npol][ipf.npol] as interleaved (re, im) doubles: the layout the
+    standalone passes hipng_vis_prologue / hipng_vis_epilogue take."""
+    cm = numpy.ascontiguousarray(pol_matrix(ipf, opf), dtype=complex)
+    return numpy.ascontiguousarray(cm.view(numpy.float64).ravel()), cm.shape
 
 
 def _pixsize(im):
@@ -38,107 +65,132 @@ def _vis_to_im(model, freq):
     return numpy.round(model.image_acc.wcs.freq_to_pix(freq)).astype("int")
 
 
-def _dtypes(precision):
-    return ("c16", "f8") if precision == "fp64" else ("c8", "f4")
-
-
 def _shift_lmn(vis_pc, im_pc):
-    """(l, m, n-1) of the image centre for shift_vis_to_image (imaging/base.py:67-106 with
-    phaserotate_visibility visibility/base.py:296-357): None when the centres coincide to
-    1e-15 rad or the rotation is the identity (|n-1| < 1e-15); `moved` says whether the
-    visibility phase centre becomes the image's."""
+    """Phase coefficients on the gridder's uvw for shift_vis_to_image (imaging/base.py:67-106
+    with phaserotate_visibility visibility/base.py:296-357): the phase of RASCIL's uvw,
+    u l + v m + w (n - 1), with u and w negated for the gridder, i.e. (-l, m, -(n - 1)).
+    None when the centres coincide to 1e-15 rad or |n - 1| < 1e-15 (no rotation); `moved`
+    says whether the visibility phase centre becomes the image's."""
     moved = vis_pc.separation(im_pc) > 1e-15
     if not moved:
         return None, False
     l, m, nm1 = direction_to_lmn(im_pc, vis_pc)
     if numpy.abs(nm1) < 1e-15:
         return None, True
-    return numpy.array([l, m, nm1], dtype="f8"), True
+    return (-l, m, -nm1), True
 
 
-def _conv(ipf, opf):
-    """Conversion matrix [opf.npol][ipf.npol] interleaved (re, im) for the fused passes."""
-    cm = numpy.ascontiguousarray(pol_matrix(ipf, opf), dtype=complex)
-    return numpy.ascontiguousarray(cm.view(numpy.float64).ravel()), cm.shape
+def _device_columns(torch, bvis, dev):
+    """(vis, uvw [nrow, 3], imaging_weight, flags or None, cacheable) on the device."""
+    if isinstance(bvis, DeviceBlockVisibility):
+        return bvis.vis, bvis.uvw.reshape(-1, 3), bvis.imaging_weight, bvis.flags, True
+    up = lambda a, dt: torch.from_numpy(numpy.ascontiguousarray(a)).to(device=dev, dtype=dt)  # noqa: E731
+    flags = numpy.asarray(bvis.flags)
+    return (up(bvis.vis, torch.complex128), up(bvis.uvw, torch.float64).reshape(-1, 3),
+            up(bvis.imaging_weight, torch.float64), up(flags.astype(numpy.int8), torch.int8) if flags.any() else None,
+            False)
 
 
-def _ptr(a):
-    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+def _geometry(torch, uvw, cacheable, freq, npix, pixsize, epsilon, do_wstacking, precision, nb):
+    """The gridder geometry of the RASCIL call (u and w negated); cached for device columns,
+    keyed on the uvw tensor (held by the cache, so its memory cannot be reused while cached)
+    and its in-place version counter."""
+    key = None
+    if cacheable and _CACHE_SIZE[0] > 0:
+        key = (uvw.data_ptr(), uvw._version, tuple(uvw.shape), str(uvw.device), freq.tobytes(), int(npix),
+               float(pixsize), float(epsilon), bool(do_wstacking), precision, int(nb))
+        hit = _CACHE.get(key)
+        if hit is not None:
+            _CACHE.move_to_end(key)
+            return hit[1], False
+    fuvw = uvw * torch.tensor([-1.0, 1.0, -1.0], dtype=torch.float64, device=uvw.device)
+    g = wgridder.Geometry(fuvw, freq, npix, npix, pixsize, pixsize, epsilon, do_wstacking, precision, nbatch=nb,
+                          device=uvw.device)
+    del fuvw
+    if key is None:
+        return g, True
+    _CACHE[key] = (uvw, g)
+    while len(_CACHE) > _CACHE_SIZE[0]:
+        _CACHE.popitem(last=False)[1][1].close()
+    return g, False
 
 
-def predict_ng(bvis: BlockVisibility, model: Image, **kwargs) -> BlockVisibility:
+def _image_dtype(torch, pix):
+    if hasattr(pix, "dtype") and pix.dtype in (torch.float32, torch.float64):
+        return pix.dtype
+    return torch.float64
+
+
+def predict_ng(bvis, model, **kwargs):
     """Predict visibilities from a model image (reference ng.py:33-133).
 
-    The degridder's output goes through one fused GPU pass (hipng_vis_epilogue): the
-    polarisation conversion to the visibility frame and the phase shift of
+    One fused call per image channel: the degridder's unsort writes the visibility columns
+    after the polarisation conversion to the visibility frame and the phase shift of
     shift_vis_to_image(inverse=True).
 
-    :param bvis: BlockVisibility to be predicted (not modified)
+    :param bvis: BlockVisibility or DeviceBlockVisibility to be predicted (not modified)
     :param model: model image
-    :return: new BlockVisibility with the predicted vis
+    :return: new visibility set (same kind as bvis) with the predicted vis
     """
     assert image_is_canonical(model)
     if model is None:
         return bvis
     torch = wgridder._torch()
-    lib = _lib.load()
     epsilon = get_parameter(kwargs, "epsilon", 1e-12)
     do_wstacking = get_parameter(kwargs, "do_wstacking", True)
     verbosity = get_parameter(kwargs, "verbosity", 0)
-    precision = get_parameter(kwargs, "precision", "fp64")
-    cdt, rdt = _dtypes(precision)
+    precision = get_parameter(kwargs, "precision", "fp64") or "fp64"
 
-    newbvis = copy_visibility(bvis, zero=True)
+    on_device = isinstance(bvis, DeviceBlockVisibility)
     freq = numpy.ascontiguousarray(bvis.frequency, dtype="f8")
     nrows, nbaselines, vnchan, vnpol = bvis.vis.shape
     m_nchan, m_npol, ny, nx = model.pixels.shape
     assert m_npol == vnpol
-    conv, _ = _conv(model.image_acc.polarisation_frame, bvis.blockvisibility_acc.polarisation_frame)
-    lmn, moved = _shift_lmn(newbvis.phasecentre, model.phasecentre)
-
-    dev = torch.device("cuda", torch.cuda.current_device())
-    uvw = newbvis.uvw.reshape([nrows * nbaselines, 3])
-    fuvw = numpy.nan_to_num(uvw) * numpy.array([-1.0, 1.0, -1.0])
-    fuvw_d = torch.from_numpy(numpy.ascontiguousarray(fuvw)).to(dev)
+    conv = pol_matrix(model.image_acc.polarisation_frame, bvis.blockvisibility_acc.polarisation_frame)
+    lmn, moved = _shift_lmn(bvis.phasecentre, model.phasecentre)
+    dev = bvis.vis.device if on_device else torch.device("cuda", torch.cuda.current_device())
+    _, uvw, _, _, cacheable = _device_columns(torch, bvis, dev) if on_device else (
+        None, torch.from_numpy(numpy.ascontiguousarray(bvis.uvw.reshape(-1, 3), "f8")).to(dev), None, None, False)
+    vdt = bvis.vis.dtype if on_device else torch.complex128
+    out = torch.empty((nrows * nbaselines, vnchan, vnpol), dtype=vdt, device=dev)
     pixsize = _pixsize(model)
-    vis_to_im = _vis_to_im(model, freq)
-    tdt = torch.complex128 if precision == "fp64" else torch.complex64
-    if m_nchan == 1:
-        dirty = torch.from_numpy(numpy.ascontiguousarray(model.pixels[0].transpose(0, 2, 1), dtype=rdt)).to(dev)
-        vist = wgridder.dirty2ms(fuvw_d, freq, dirty, None, pixsize, pixsize, 0, 0, epsilon, do_wstacking,
-                                 precision=precision, verbosity=verbosity).reshape(nrows * nbaselines, vnchan, vnpol)
+    idt = _image_dtype(torch, model.pixels)
+    pix = model.pixels if hasattr(model.pixels, "is_cuda") else torch.from_numpy(numpy.ascontiguousarray(model.pixels))
+    groups = [(0, vnchan, 0)] if m_nchan == 1 else [(c, 1, int(i)) for c, i in enumerate(_vis_to_im(model, freq))]
+    for chan0, nch, imchan in groups:
+        g, transient = _geometry(torch, uvw, cacheable, freq[chan0:chan0 + nch], nx, pixsize, epsilon, do_wstacking,
+                                 precision, m_npol)
+        cols = wgridder.make_columns(vnchan, vnpol, conv, chan0=chan0, vis_dtype=vdt, lmn=lmn, image_order=1,
+                                     image_dtype=idt)
+        dirty = pix[imchan].to(device=dev, dtype=idt).contiguous()
+        g.predict_columns(cols, dirty, out)
+        if verbosity:
+            print("hipng predict_ng plan:", g.plan)
+        if transient:
+            g.close()
+    vis = out.reshape(nrows, nbaselines, vnchan, vnpol)
+    if on_device:
+        newbvis = bvis.with_vis(vis)
     else:
-        vist = torch.empty((nrows * nbaselines, vnchan, vnpol), dtype=tdt, device=dev)
-        for vchan in range(vnchan):
-            imchan = vis_to_im[vchan]
-            dirty = torch.from_numpy(
-                numpy.ascontiguousarray(model.pixels[imchan].transpose(0, 2, 1), dtype=rdt)).to(dev)
-            vist[:, vchan : vchan + 1, :] = wgridder.dirty2ms(
-                fuvw_d, freq[vchan : vchan + 1], dirty, None, pixsize, pixsize, 0, 0, epsilon,
-                do_wstacking, precision=precision, verbosity=verbosity).reshape(nrows * nbaselines, 1, vnpol)
-    vist = vist.contiguous()
-    uvw_d = torch.from_numpy(numpy.ascontiguousarray(uvw, dtype="f8")).to(dev) if lmn is not None else None
-    out = torch.empty((nrows * nbaselines, vnchan, vnpol), dtype=torch.complex128, device=dev)
-    _lib.check(lib.hipng_vis_epilogue(
-        ctypes.c_void_p(vist.data_ptr()), ctypes.c_void_p(uvw_d.data_ptr() if uvw_d is not None else None),
-        _ptr(freq), nrows * nbaselines, vnchan, m_npol, vnpol, _ptr(conv), _ptr(lmn), _lib.PRECISIONS[precision],
-        ctypes.c_void_p(out.data_ptr()), wgridder._stream(torch, dev)))
-    newbvis.vis = out.cpu().numpy().reshape([nrows, nbaselines, vnchan, vnpol])
+        newbvis = bvis.copy(zero=False)  # copy_visibility (visibility/base.py:105-118), vis replaced below
+        newbvis.vis = vis.cpu().numpy()
     if moved:
         newbvis.phasecentre = model.phasecentre
         newbvis.attrs["phasecentre"] = model.phasecentre
     return newbvis
 
 
-def invert_ng(bvis: BlockVisibility, model: Image, dopsf: bool = False, normalise: bool = True,
-              **kwargs):
+def invert_ng(bvis, model, dopsf: bool = False, normalise: bool = True, **kwargs):
     """Dirty image (or PSF) and sum of weights (reference ng.py:136-282).
 
-    The visibility passes before the gridder (phase shift to the image centre, flag mask,
-    polarisation conversion, the gridder's precision and layout) run as one fused GPU pass
-    (hipng_vis_prologue); the input is not modified.
+    One fused call per image channel: the gridder's value gather reads the visibility
+    columns with the flag mask, the phase shift to the image centre, the polarisation
+    conversion and the imaging weights applied on the way, and the grid correction writes
+    the image in RASCIL's pixel order already divided by sumwt (normalise); the input is not
+    modified. Image polarisations whose visibilities are all zero (the reference skips them)
+    come out zero as well.
 
-    :param bvis: BlockVisibility to be inverted
+    :param bvis: BlockVisibility or DeviceBlockVisibility to be inverted
     :param model: image template (not changed)
     :param dopsf: make the PSF instead of the dirty image
     :param normalise: normalise by the sum of weights
@@ -147,97 +199,69 @@ def invert_ng(bvis: BlockVisibility, model: Image, dopsf: bool = False, normalis
     assert image_is_canonical(model)
     torch = wgridder._torch()
     lib = _lib.load()
-    im = model.copy(deep=True)
     epsilon = get_parameter(kwargs, "epsilon", 1e-12)
     do_wstacking = get_parameter(kwargs, "do_wstacking", True)
     verbosity = get_parameter(kwargs, "verbosity", 0)
-    precision = get_parameter(kwargs, "precision", "fp64")
-    cdt, rdt = _dtypes(precision)
+    precision = get_parameter(kwargs, "precision", "fp64") or "fp64"
 
     freq = numpy.ascontiguousarray(bvis.frequency, dtype="f8")
     nrows, nbaselines, vnchan, vnpol = bvis.vis.shape
-    nchan, npol, ny, nx = im.pixels.shape
-    conv, cshape = _conv(bvis.blockvisibility_acc.polarisation_frame, im.image_acc.polarisation_frame)
-    if cshape[0] != npol:
+    nchan, npol, ny, nx = model.pixels.shape
+    conv = pol_matrix(bvis.blockvisibility_acc.polarisation_frame, model.image_acc.polarisation_frame)
+    if conv.shape[0] != npol:
         raise ValueError("image polarisation frame does not match the image")
     if npol > vnpol:
         # the reference indexes the weights of the visibility frame by image polarisation
         raise IndexError(f"index {vnpol} is out of bounds for axis 2 with size {vnpol}")
-    lmn, _ = _shift_lmn(bvis.phasecentre, im.phasecentre)
-
+    lmn, _ = _shift_lmn(bvis.phasecentre, model.phasecentre)
+    on_device = isinstance(bvis, DeviceBlockVisibility)
+    dev = bvis.vis.device if on_device else torch.device("cuda", torch.cuda.current_device())
+    vis, uvw, iwt, flags, cacheable = _device_columns(torch, bvis, dev)
     n = nrows * nbaselines
-    dev = torch.device("cuda", torch.cuda.current_device())
-    st = wgridder._stream(torch, dev)
-    uvw = numpy.ascontiguousarray(bvis.uvw.reshape([n, 3]), dtype="f8")
-    uvw_d = torch.from_numpy(uvw).to(dev)
-    vis_d = torch.from_numpy(numpy.ascontiguousarray(bvis.vis, dtype="c16")).to(dev)
-    flags = numpy.asarray(bvis.flags)
-    flags_d = torch.from_numpy(numpy.ascontiguousarray(flags, dtype=numpy.int32)).to(dev) if flags.any() else None
-    iwt_d = torch.from_numpy(numpy.ascontiguousarray(bvis.imaging_weight, dtype="f8")).to(dev)
-    tdt = torch.complex128 if precision == "fp64" else torch.complex64
-    wdt = torch.float64 if precision == "fp64" else torch.float32
-    ms = torch.empty((n, vnchan, npol), dtype=tdt, device=dev)
-    wgt = torch.empty((n, vnchan, npol), dtype=wdt, device=dev)
-    nz = torch.empty((vnchan, npol), dtype=torch.int32, device=dev)
-    _lib.check(lib.hipng_vis_prologue(
-        ctypes.c_void_p(vis_d.data_ptr()), ctypes.c_void_p(flags_d.data_ptr() if flags_d is not None else None),
-        ctypes.c_void_p(iwt_d.data_ptr()), ctypes.c_void_p(uvw_d.data_ptr()), _ptr(freq), n, vnchan, vnpol, npol,
-        _ptr(conv), _ptr(lmn), _lib.PRECISIONS[precision], ctypes.c_void_p(ms.data_ptr()),
-        ctypes.c_void_p(wgt.data_ptr()), ctypes.c_void_p(nz.data_ptr()), st))
-    del vis_d, iwt_d, flags_d
-    nonzero = nz.cpu().numpy() != 0
-    # sum of the flagged imaging weights per (channel, pol) of the visibility frame, in FP64
-    wsum = numpy.sum(bvis.imaging_weight * (1 - flags), axis=(0, 1))
 
-    npixdirty = im.pixels.shape[-1]
-    pixsize = _pixsize(im)
-    fuvw_d = uvw_d * torch.tensor([-1.0, 1.0, -1.0], dtype=torch.float64, device=dev)
-    im.pixels[...] = 0.0
-    sumwt = numpy.zeros([nchan, npol])
+    # sum of the flagged imaging weights per (channel, pol) of the visibility frame, FP64
+    wsum_d = torch.empty((vnchan, vnpol), dtype=torch.float64, device=dev)
+    _lib.check(lib.hipng_weight_sums(
+        wgridder.ctypes.c_void_p(iwt.data_ptr()), wgridder.ctypes.c_void_p(flags.data_ptr()) if flags is not None
+        else None, n, vnchan, vnpol, 0 if iwt.dtype == torch.float64 else 1, 1, wgridder.ctypes.c_void_p(
+            wsum_d.data_ptr()), wgridder._stream(torch, dev)))
+    wsum = wsum_d.cpu().numpy()
     vis_to_im = _vis_to_im(model, freq)
     mfs = nchan == 1 and vnchan > 1
-    if dopsf:
-        ms.zero_()
-        ms[..., 0] = 1.0
-        nonzero[...] = False
-        nonzero[:, 0] = True
-
-    def grid(sel_ms, sel_wgt, sel_freq, active_mask):
-        # polarisations with data are gridded together; all-zero ones are skipped as in the
-        # reference (their weights still enter sumwt)
-        active = [p for p in range(npol) if active_mask[p]]
-        out = {}
-        if active:
-            if len(active) == npol:
-                lms, lwt = sel_ms.contiguous(), sel_wgt.contiguous()
-            else:
-                idx = torch.tensor(active, device=dev)
-                lms = sel_ms.index_select(2, idx).contiguous()
-                lwt = sel_wgt.index_select(2, idx).contiguous()
-            dirty = wgridder.ms2dirty(fuvw_d, sel_freq, lms, lwt, npixdirty, npixdirty, pixsize, pixsize, 0, 0,
-                                      epsilon, do_wstacking, double_precision_accumulation=True,
-                                      verbosity=verbosity, precision=precision)
-            dirty = dirty.reshape(len(active), npixdirty, npixdirty).cpu().numpy()
-            for k, p in enumerate(active):
-                out[p] = dirty[k].T
-        return out
-
+    sumwt = numpy.zeros([nchan, npol])
     if mfs:
-        dirty = grid(ms, wgt, freq, nonzero.any(axis=0))
-        for pol in range(npol):
-            if pol in dirty:
-                im.pixels[0, pol] += dirty[pol]
-            sumwt[0, pol] += numpy.sum(wsum[:, pol])
+        sumwt[0, :] = wsum[:, :npol].sum(axis=0)
+        groups = [(0, vnchan, 0)]
     else:
         for vchan in range(vnchan):
-            ichan = vis_to_im[vchan]
-            dirty = grid(ms[:, vchan : vchan + 1, :], wgt[:, vchan : vchan + 1, :], freq[vchan : vchan + 1],
-                         nonzero[vchan])
-            for pol in range(npol):
-                if pol in dirty:
-                    im.pixels[ichan, pol] += dirty[pol]
-                sumwt[ichan, pol] += wsum[vchan, pol]
+            sumwt[vis_to_im[vchan], :] += wsum[vchan, :npol]
+        groups = [(c, 1, int(vis_to_im[c])) for c in range(vnchan)]
 
-    if normalise:
-        im = normalise_sumwt(im, sumwt)
+    im = model.copy(deep=True)
+    pixsize = _pixsize(im)
+    idt = _image_dtype(torch, im.pixels)
+    on_device_image = hasattr(im.pixels, "is_cuda")
+    if on_device_image:
+        im.pixels.zero_()
+    else:
+        im.pixels[...] = 0.0
+    for chan0, nch, ichan in groups:
+        scale = [1.0 / sumwt[ichan, p] if sumwt[ichan, p] > 0.0 else 0.0 for p in range(npol)] if normalise else None
+        g, transient = _geometry(torch, uvw, cacheable, freq[chan0:chan0 + nch], nx, pixsize, epsilon, do_wstacking,
+                                 precision, npol)
+        cols = wgridder.make_columns(vnchan, vnpol, conv, chan0=chan0, vis_dtype=vis.dtype, weight_dtype=iwt.dtype,
+                                     flag_dtype=torch.int8, lmn=lmn, psf=dopsf, image_order=1, image_dtype=idt,
+                                     scale=scale)
+        if mfs and on_device_image:
+            g.invert_columns(cols, vis, iwt, flags, out=im.pixels[0])
+        else:
+            dirty = g.invert_columns(cols, vis, iwt, flags)
+            if on_device_image:
+                im.pixels[ichan] += dirty
+            else:
+                im.pixels[ichan] += dirty.cpu().numpy()
+        if verbosity:
+            print("hipng invert_ng plan:", g.plan)
+        if transient:
+            g.close()
     return im, sumwt

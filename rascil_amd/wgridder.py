@@ -373,6 +373,56 @@ class Geometry:
             ctypes.c_void_p(out.data_ptr()), _stream(torch, self.device)))
         return out
 
+    def invert_columns(self, cols, vis, weight=None, flags=None, out=None):
+        """Invert straight from BlockVisibility-layout columns (hipng_geometry_invert_columns):
+        vis [nrow, nchan_cols, npol] (complex128 / complex64), weight (float64 / float32) and
+        flags (int32 / int8) of the same shape or None; `cols` from make_columns. Returns the
+        dirty images [nbatch, ...] in the order and type `cols` names."""
+        torch = _torch()
+        shape = (self.nrow, cols.nchan, cols.npol)
+        self._check("vis", vis.reshape(shape) if vis.is_contiguous() else vis,
+                    {0: torch.complex128, 1: torch.complex64}[cols.vis_type], shape)
+        if weight is not None:
+            self._check("weight", weight.reshape(shape) if weight.is_contiguous() else weight,
+                        {0: torch.float64, 1: torch.float32}[cols.weight_type], shape)
+        if flags is not None:
+            self._check("flags", flags.reshape(shape) if flags.is_contiguous() else flags,
+                        {0: torch.int32, 1: torch.int8}[cols.flag_type], shape)
+        oshape = self._image_shape(cols)
+        odt = {0: torch.float64, 1: torch.float32}[cols.image_type]
+        if out is None:
+            out = torch.empty(oshape, dtype=odt, device=self.device)
+        else:
+            self._check("out", out, odt, oshape)
+        _lib.check(self._lib.hipng_geometry_invert_columns(
+            self._handle, ctypes.byref(cols), ctypes.c_void_p(vis.data_ptr()),
+            ctypes.c_void_p(weight.data_ptr()) if weight is not None else None,
+            ctypes.c_void_p(flags.data_ptr()) if flags is not None else None,
+            ctypes.c_void_p(out.data_ptr()), _stream(torch, self.device)))
+        return out
+
+    def predict_columns(self, cols, dirty, vis_out, model_out=None):
+        """Predict into BlockVisibility-layout columns (hipng_geometry_predict_columns): writes
+        the geometry's channels of vis_out [nrow, nchan_cols, npol] (and of model_out, the
+        visibilities before the gains); dirty [nbatch, ...] in the order and type `cols` names."""
+        torch = _torch()
+        shape = (self.nrow, cols.nchan, cols.npol)
+        vdt = {0: torch.complex128, 1: torch.complex64}[cols.vis_type]
+        self._check("vis_out", vis_out.reshape(shape) if vis_out.is_contiguous() else vis_out, vdt, shape)
+        if model_out is not None:
+            self._check("model_out", model_out.reshape(shape) if model_out.is_contiguous() else model_out, vdt,
+                        shape)
+        self._check("dirty", dirty, {0: torch.float64, 1: torch.float32}[cols.image_type], self._image_shape(cols))
+        _lib.check(self._lib.hipng_geometry_predict_columns(
+            self._handle, ctypes.byref(cols), ctypes.c_void_p(dirty.data_ptr()), ctypes.c_void_p(vis_out.data_ptr()),
+            ctypes.c_void_p(model_out.data_ptr()) if model_out is not None else None, _stream(torch, self.device)))
+        return vis_out
+
+    def _image_shape(self, cols):
+        if cols.image_order == 1:
+            return (self.nbatch, self.npix_y, self.npix_x)
+        return (self.nbatch, self.npix_x, self.npix_y)
+
     def stats(self):
         """Diagnostics: visibilities, dense-run tasks, visibilities on the dense-run path,
         whether that path is enabled for this geometry, and whether its out-of-place FFT
@@ -392,3 +442,50 @@ class Geometry:
             self.close()
         except Exception:
             pass
+
+
+def make_columns(nchan, npol, conv, chan0=0, vis_dtype=None, weight_dtype=None, flag_dtype=None, lmn=None,
+                 gains=None, ant1=None, ant2=None, psf=False, image_order=0, image_dtype=None, scale=None):
+    """hipng_columns for Geometry.invert_columns / predict_columns.
+
+    nchan, npol: the columns' channels and polarisations; conv: complex matrix [npol_out,
+    npol_in] (columns -> image for invert, image -> columns for predict); chan0: the columns'
+    channel of the geometry's first channel; dtypes: torch dtypes of the column arrays
+    (complex128 / complex64, float64 / float32, int32 / int8) and of the images
+    (float64 / float32); lmn: coefficients (L, M, N) of the phase on the geometry's uvw, or
+    None; gains: device complex128 [nant, nchan] with ant1 / ant2 int32 [nrow] (the tensors
+    must outlive the call); psf: invert the point spread function; image_order 1: images in
+    [pol, y, x] order; scale: per image polarisation factor of the invert (order 1)."""
+    import torch
+
+    c = _lib.HipngColumns()
+    c.nchan, c.chan0, c.npol = int(nchan), int(chan0), int(npol)
+    c.vis_type = {None: 0, torch.complex128: 0, torch.complex64: 1}[vis_dtype]
+    c.weight_type = {None: 0, torch.float64: 0, torch.float32: 1}[weight_dtype]
+    c.flag_type = {None: 0, torch.int32: 0, torch.int8: 1}[flag_dtype]
+    cm = numpy.ascontiguousarray(numpy.asarray(conv, dtype=complex))
+    if cm.ndim != 2 or cm.size > 16:
+        raise ValueError("conv must be a complex matrix with at most 16 entries")
+    flat = cm.view(numpy.float64).ravel()
+    for k in range(flat.size):
+        c.conv[k] = float(flat[k])
+    if lmn is not None:
+        c.shift = 1
+        for k in range(3):
+            c.lmn[k] = float(lmn[k])
+    if gains is not None:
+        if gains.dtype != torch.complex128 or not gains.is_contiguous() or gains.dim() != 2 or gains.shape[1] != nchan:
+            raise ValueError("gains must be a contiguous complex128 tensor [nant, nchan]")
+        for name, a in (("ant1", ant1), ("ant2", ant2)):
+            if a is None or a.dtype != torch.int32 or not a.is_contiguous():
+                raise ValueError(f"{name} must be a contiguous int32 tensor [nrow]")
+        c.nant = int(gains.shape[0])
+        c.gains, c.ant1, c.ant2 = gains.data_ptr(), ant1.data_ptr(), ant2.data_ptr()
+    c.psf = 1 if psf else 0
+    c.image_order = int(image_order)
+    c.image_type = {None: 0, torch.float64: 0, torch.float32: 1}[image_dtype]
+    sc = [1.0, 1.0, 1.0, 1.0] if scale is None else list(scale) + [1.0] * (4 - len(scale))
+    for k in range(4):
+        c.scale[k] = float(sc[k])
+    return c
+

@@ -138,3 +138,77 @@ def test_gain_descent_reduces_chi_squared():
         opt.step()
         first = first if first is not None else L.item()
     assert L.item() < 0.2 * first
+
+
+@pytest.mark.parametrize("prec,eps,tol", [("fp64", 1e-12, 1e-12), ("mixed", 1e-5, 1e-5)])
+def test_fused_gains_match_torch_gains_forward_and_gradients(prec, eps, tol):
+    """predict(..., gains=) (gains fused into the degridder's unsort, the conjugate product
+    into the gridder's value gather in backward, the gain gradient from hipng_gain_gradient)
+    gives the same visibilities and the same image and gain gradients as predict followed by
+    torch's elementwise apply_gains (calibration/operations.py:135-137 convention)."""
+    import torch
+
+    from rascil_amd import wgridder
+
+    dtype = torch.float64 if prec == "fp64" else torch.float32
+    s = _setup(dtype)
+    geom = wgridder.Geometry(s["uvw_d"], s["freq"], 32, 32, s["cell"], s["cell"], eps, True, prec)
+    i1, i2 = s["i1"].to(torch.int32), s["i2"].to(torch.int32)
+
+    def loss_fused(img, g):
+        v = predict(img, geometry=geom, gains=g, ant1=i1, ant2=i2).to(torch.complex128)
+        return ((s["vobs"].to(torch.complex128) - v).abs() ** 2).sum(), v
+
+    def loss_torch(img, g):
+        v = apply_gains(predict(img, geometry=geom).to(torch.complex128), g.to(torch.complex128), s["i1"], s["i2"])
+        return ((s["vobs"].to(torch.complex128) - v).abs() ** 2).sum(), v
+
+    img_a, g_a = s["img"].detach().clone().requires_grad_(True), s["gains"].detach().clone().requires_grad_(True)
+    img_b, g_b = s["img"].detach().clone().requires_grad_(True), s["gains"].detach().clone().requires_grad_(True)
+    La, va = loss_fused(img_a, g_a)
+    Lb, vb = loss_torch(img_b, g_b)
+    assert (torch.linalg.norm(va - vb) / torch.linalg.norm(vb)).item() < tol
+    La.backward()
+    Lb.backward()
+    ei = (torch.linalg.norm(img_a.grad - img_b.grad) / torch.linalg.norm(img_b.grad)).item()
+    eg = (torch.linalg.norm(g_a.grad.to(torch.complex128) - g_b.grad.to(torch.complex128))
+          / torch.linalg.norm(g_b.grad.to(torch.complex128))).item()
+    print(f"{prec}: fused vs torch gains: image grad {ei:.2e}, gain grad {eg:.2e}")
+    assert ei < 10 * tol and eg < 10 * tol
+    geom.close()
+
+
+def test_fused_gains_gradients_finite_differences():
+    """Central differences of the fused-gain chi-squared in FP64: image (exact, the loss is
+    quadratic) and the real and imaginary parts of two antennas' gains, within 1e-6."""
+    import torch
+
+    from rascil_amd import wgridder
+
+    s = _setup(torch.float64)
+    geom = wgridder.Geometry(s["uvw_d"], s["freq"], 32, 32, s["cell"], s["cell"], 1e-12, True, "fp64")
+    i1, i2 = s["i1"].to(torch.int32), s["i2"].to(torch.int32)
+
+    def loss(img, g):
+        v = predict(img, geometry=geom, gains=g, ant1=i1, ant2=i2)
+        return ((s["vobs"] - v).abs() ** 2).sum()
+
+    img0, g0 = s["img"].detach().clone(), s["gains"].detach().clone()
+    img, g = img0.clone().requires_grad_(True), g0.clone().requires_grad_(True)
+    loss(img, g).backward()
+    with torch.no_grad():
+        for px, py in ((3, 7), (16, 16)):
+            ip, im_ = img0.clone(), img0.clone()
+            ip[px, py] += 1e-3
+            im_[px, py] -= 1e-3
+            fd = (loss(ip, g0) - loss(im_, g0)).item() / 2e-3
+            assert abs(fd - img.grad[px, py].item()) <= 1e-6 * max(1.0, abs(fd))
+        for k in (0, 5):
+            for dirn in (1.0, 1j):
+                gp, gm = g0.clone(), g0.clone()
+                gp[k] += 1e-5 * dirn
+                gm[k] -= 1e-5 * dirn
+                fd = (loss(img0, gp) - loss(img0, gm)).item() / 2e-5
+                an = g.grad[k].real.item() if dirn == 1.0 else g.grad[k].imag.item()
+                assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd)), (k, dirn, fd, an)
+    geom.close()

@@ -177,6 +177,66 @@ def test_spectral_invert_and_predict_channels():
         assert im.pixels[c, 0, 32, 32] == pytest.approx(c + 1, rel=1e-8)
 
 
+@pytest.mark.parametrize("vpol", ["linear", "circular", "linearnp", "stokesI"])
+def test_dft_then_idft_recovers_component_flux(vpol):
+    """test_dft_skycomponent_visibility.py:134-147: the inverse DFT of the DFT of a component
+    returns its flux (rtol 1e-10), for IQUV components in linear and circular frames; with
+    flags the flagged visibilities and their weights drop out of both sums."""
+    from rascil_amd import idft_visibility_skycomponent
+    from rascil_amd.data_models import Skycomponent, lmn_to_direction
+
+    bvis = _obs(pol=vpol, nchan=3, ntimes=5)
+    f = {"linear": [100.0, 20.0, -10.0, 1.0], "circular": [100.0, 20.0, -10.0, 1.0],
+         "linearnp": [100.0, 20.0], "stokesI": [100.0]}[vpol]
+    cpol = {"linear": "stokesIQUV", "circular": "stokesIQUV", "linearnp": "stokesIQ", "stokesI": "stokesI"}[vpol]
+    flux = numpy.array([f, [0.8 * x for x in f], [0.6 * x for x in f]])
+    comps = [Skycomponent(lmn_to_direction(0.01, -0.02, bvis.phasecentre), bvis.frequency, flux,
+                          PolarisationFrame(cpol)),
+             Skycomponent(lmn_to_direction(-0.03, 0.005, bvis.phasecentre), bvis.frequency, 0.5 * flux,
+                          PolarisationFrame(cpol))]
+    dft_skycomponent_visibility(bvis, comps[0])
+    rcomp, weights = idft_visibility_skycomponent(bvis, comps[0])
+    numpy.testing.assert_allclose(numpy.real(rcomp[0].flux), flux, rtol=1e-10)
+    assert weights[0].shape == (3, len(f))
+    numpy.testing.assert_allclose(weights[0], bvis.weight.sum(axis=(0, 1)), rtol=1e-13)
+    # two components: each returned flux equals the oracle's weighted mean of vis x conj phasor
+    rng = numpy.random.default_rng(2)
+    bvis.flags = (rng.uniform(size=bvis.vis.shape) < 0.3).astype(int)
+    bvis.weight = rng.uniform(0.5, 2.0, bvis.vis.shape)
+    rc, ws = idft_visibility_skycomponent(bvis, comps)
+    fw = bvis.weight * (1 - bvis.flags)
+    for k, comp in enumerate(comps):
+        from rascil_amd.data_models import direction_to_lmn
+
+        l, m, nm1 = direction_to_lmn(comp.direction, bvis.phasecentre)
+        ph = numpy.conj(ox.phasor(bvis.uvw_lambda, l, m, nm1))[..., None]
+        ref = (fw * bvis.vis * ph).sum(axis=(0, 1)) / fw.sum(axis=(0, 1))
+        if cpol != vpol:
+            ref = ox.convert_pol(ref, vpol, cpol, polaxis=-1)
+        numpy.testing.assert_allclose(rc[k].flux, ref, rtol=1e-11, atol=1e-11)
+        numpy.testing.assert_allclose(ws[k], fw.sum(axis=(0, 1)), rtol=1e-13)
+
+
+def test_dft_interpolates_component_flux_across_channels():
+    """dft.py:94-107: a component tabulated on other frequencies is interpolated per
+    polarisation with a cubic spline onto the visibility channels."""
+    from scipy import interpolate
+
+    from rascil_amd.data_models import Skycomponent, lmn_to_direction
+
+    bvis = _obs(pol="stokesI", nchan=5, ntimes=4)
+    cfreq = numpy.linspace(0.95e8, 1.15e8, 6)
+    cflux = (2.0 + numpy.sin(cfreq / 1e7))[:, None]
+    comp = Skycomponent(lmn_to_direction(0.02, 0.01, bvis.phasecentre), cfreq, cflux, PolarisationFrame("stokesI"))
+    dft_skycomponent_visibility(bvis, comp)
+    vflux = interpolate.interp1d(cfreq, cflux[:, 0], kind="cubic")(bvis.frequency)
+    from rascil_amd.data_models import direction_to_lmn
+
+    l, m, nm1 = direction_to_lmn(comp.direction, bvis.phasecentre)
+    ref = ox.dft_cpu_looped(numpy.array([[l, m, nm1]]), bvis.uvw_lambda, vflux[None, :, None].astype(complex))
+    numpy.testing.assert_allclose(bvis.vis, ref, rtol=1e-12, atol=1e-12)
+
+
 def test_gpu_dft_matches_oracle_and_known_answer():
     rng = numpy.random.default_rng(0)
     uvl = rng.normal(0, 300, (2, 50, 10, 3))

@@ -263,6 +263,18 @@ def test_get_parameter():
     assert get_parameter({"epsilon": 1e-5}, "epsilon", 1e-12) == 1e-5
     assert get_parameter({}, "epsilon", 1e-12) == 1e-12
     assert get_parameter(None, "x", 3) == 3
+    # a key that is present is returned as given, None included (parameters.py:119-125)
+    assert get_parameter({"epsilon": None}, "epsilon", 1e-12) is None
+
+
+def test_image_with_moment_axis_is_canonical():
+    """image/operations.py:96-98 accepts FREQ or MOMENT as the fourth axis."""
+    im = create_image(32, 1e-3, Direction(0.0, -0.5), [1e8])
+    assert image_is_canonical(im)
+    im.wcs.ctype[3] = "MOMENT"
+    assert image_is_canonical(im)
+    im.wcs.ctype[3] = "VELO"
+    assert not image_is_canonical(im)
 
 
 def test_product_has_no_cpu_fallback(monkeypatch):
