@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting", "gains",
                                                         "vis_passes", "dft"],
                     help="run a study instead of the bench line (prints one JSON line)")
+    ap.add_argument("--rascil-steps", type=int, default=3,
+                    help="also time invert_ng + predict_ng on device-resident columns (0: skip)")
     ap.add_argument("--fp64-steps", type=int, default=2,
                     help="also time the FP64 plain mode for this many steps (0: skip)")
     return ap.parse_args()
@@ -254,6 +256,55 @@ def accuracy_at_timed_plan(wl, prec, uvw_d, ms_buf, wgt_d, timed_plans, nsrc=20,
            "seconds": round(time.time() - t_start, 1)}
     out["meets_epsilon"] = bool(out["predict"]["max_rel"] <= eps and out["invert"]["max_rel"] <= eps and same)
     return out
+
+
+def rascil_level_leg(wl, prec, uvw_d, steps, wgridder_ms):
+    """invert_ng + predict_ng (the RASCIL surface) on a DeviceBlockVisibility of the workload:
+    complex64 visibilities and float32 imaging weights resident in GPU memory, a linear
+    polarisation frame imaged as Stokes IQUV (the fused gather / unsort do the conversion and
+    the weighting), the image on the device. The first call builds and keeps the geometry
+    (sort, records, plans); the timed steps then reuse it, as a major cycle does."""
+    import numpy
+    import torch
+
+    from rascil_amd import DeviceBlockVisibility, PolarisationFrame, create_image, invert_ng, ng, predict_ng
+    from rascil_amd.data_models import Direction
+
+    dev = uvw_d.device
+    nrow, nchan, N, cell = uvw_d.shape[0], len(wl["freq"]), wl["npix"], wl["cell"]
+    g = torch.Generator(device=dev)
+    g.manual_seed(99)
+    vis = torch.randn((1, nrow, nchan, 4), dtype=torch.complex64, device=dev, generator=g)
+    wt = torch.empty((1, nrow, nchan, 4), dtype=torch.float32, device=dev).uniform_(0.5, 1.5, generator=g)
+    pc = Direction(0.0, numpy.radians(-30.0))
+    dbv = DeviceBlockVisibility(vis, uvw_d.reshape(1, nrow, 3), wl["freq"], PolarisationFrame("linear"), pc,
+                                weight=wt)
+    model = create_image(N, cell, pc, [float(numpy.mean(wl["freq"]))], PolarisationFrame("stokesIQUV"))
+    model.pixels = torch.randn((1, 4, N, N), dtype=torch.float32, device=dev, generator=g)
+    kw = dict(epsilon=wl["eps"], precision=prec)
+    ng.clear_geometry_cache()
+    torch.cuda.synchronize()
+    t = time.time()
+    invert_ng(dbv, model, **kw)
+    pred = predict_ng(dbv, model, **kw)
+    torch.cuda.synchronize()
+    first = (time.time() - t) * 1e3
+    del pred
+    torch.cuda.synchronize()
+    t = time.time()
+    for i in range(steps):
+        im, sumwt = invert_ng(dbv, model, **kw)
+        pred = predict_ng(dbv, model, **kw)
+        del im, pred
+        print(f"[bench] rascil-level step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ms = (time.time() - t) * 1e3 / steps
+    ng.clear_geometry_cache()
+    nvis = nrow * nchan * 4
+    return {"api": "invert_ng + predict_ng on a DeviceBlockVisibility (complex64 / float32 columns, linear -> "
+                   "stokesIQUV, image on the device, geometry kept between calls)",
+            "ms_per_step": round(ms, 1), "value": round(nvis / (ms * 1e-3) / 1e6, 2), "steps": steps,
+            "first_call_ms": round(first, 1), "ratio_to_wgridder_level": round(ms / wgridder_ms, 3)}
 
 
 def path_roofline(plan_i, plan_p, nrow, nchan, nb, prec, with_wgt, ms_per_step):
@@ -1124,8 +1175,21 @@ def main():
         except Exception as e:
             out["accuracy"] = {"error": repr(e)}
 
+    if args.config == "c1" and args.rascil_steps > 0:
+        # the same workload through the RASCIL-level API on device-resident columns
+        del vis_d, wgt_d
+        torch.cuda.empty_cache()
+        _lib.load().hipng_release()
+        try:
+            out["rascil_level"] = rascil_level_leg(wl, prec, uvw_d, args.rascil_steps, ms_per_step)
+            print(f"[bench] rascil level {out['rascil_level']}", file=sys.stderr, flush=True)
+        except Exception as e:
+            out["rascil_level"] = {"error": repr(e)}
+        torch.cuda.empty_cache()
+        vis_d = wgt_d = None
+
     if args.fp64_steps > 0 and prec != "fp64":
-        del vis_d, model_d, wgt_d
+        vis_d = model_d = wgt_d = None
         torch.cuda.empty_cache()
         _lib.load().hipng_release()
         # FP64 buffers of all four polarisations at once do not fit next to the FP64 records in

@@ -2403,7 +2403,10 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
                                                    const uint32_t* __restrict__ task_tab, uint32_t chunk,
                                                    uint32_t* __restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);  // [npl][L][L][NB]
+  // [npl][L][L][NB], cell-major: a lane reads a cell's polarisations with two 16-byte reads
+  // (a polarisation-planar tile, four 8-byte reads per cell and fewer bank conflicts, measured
+  // 16 % slower at the production shape: the kernel is bound by vector issue, not LDS)
+  cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);
   __shared__ uint32_t s_task;
   __shared__ uint32_t s_segj[64], s_segv[65];  // per start plane: first record, exclusive prefix
   const int T = k.T, L = k.T + W - 1, LL = L * L;

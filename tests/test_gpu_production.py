@@ -367,12 +367,14 @@ def c4():
                 gtrue=gtrue, nant=nant, rng=rng)
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("prec,eps,tol,hg", [("fp64", 1e-10, 1e-6, 1e-5), ("mixed", 1e-5, 1e-3, 1e-2)])
-def test_gain_calibration_gradients_at_production_size(c4, prec, eps, tol, hg):
+def test_gain_calibration_gradients_at_production_size(c4, prec, eps, tol, hg, fused):
     """Loss = sum |V_obs - g_i conj(g_j) predict(I)|^2 over 1M rows x 4 channels: the image
     gradient (exact central differences: the loss is quadratic in the image) and the gain
     gradients (central differences) within 1e-6 (FP64) / 1e-3 (mixed), and the image gradient
-    equal to the FP64 adjoint of the residual at source pixels."""
+    equal to the FP64 adjoint of the residual at source pixels; with the gains as torch
+    operations around the degridder or fused into it (and into the gridder of backward)."""
     import torch
 
     from rascil_amd.autograd import apply_gains, predict
@@ -384,9 +386,14 @@ def test_gain_calibration_gradients_at_production_size(c4, prec, eps, tol, hg):
     geom = wgridder.Geometry(c4["uvw_d"], c4["freq"], N, N, cell, cell, eps, True, prec)
     vo, i1, i2 = c4["vobs"], c4["i1"], c4["i2"]
 
+    j1, j2 = i1.to(torch.int32), i2.to(torch.int32)
+
     def loss(img, g):
-        v = predict(img, geometry=geom).to(torch.complex128)
-        r = vo - apply_gains(v, g.to(torch.complex128), i1, i2)
+        if fused:
+            r = vo - predict(img, geometry=geom, gains=g, ant1=j1, ant2=j2).to(torch.complex128)
+        else:
+            v = predict(img, geometry=geom).to(torch.complex128)
+            r = vo - apply_gains(v, g.to(torch.complex128), i1, i2)
         return (r.abs() ** 2).sum()
 
     rng = numpy.random.default_rng(12)
