@@ -640,6 +640,34 @@ def batched_study(njobs=300, seed=5):
                            "p99_ms": round(1e3 * float(numpy.percentile(lat, 99)), 2),
                            "geometry_build_seconds": round(t_build, 3)}
     print("[study] graph", out["graph_replay"], file=sys.stderr, flush=True)
+    # plan options tuned per job class, jobs' graphs replayed concurrently over 4 streams
+    from rascil_amd.batch import AutoTuner, Batch, class_table
+
+    tuner = AutoTuner()
+    t = time.time()
+    for j in jobs:
+        tuner.tune(j["uvw"], j["freq"], j["N"], j["cell"], eps, "mixed")
+    t_tune = time.time() - t
+    lat, t_run, t_build = [], 0.0, 0.0
+    for g0 in range(0, len(jobs), group):
+        grp = [dict(uvw=j["uvw"], freq=j["freq"], npix=j["N"], cell=j["cell"], nvis=j["nvis"])
+               for j in jobs[g0:g0 + group]]
+        t = time.time()
+        b = Batch(grp, eps, "mixed", tuner=tuner, nstreams=4)
+        t_build += time.time() - t
+        r = b.run()
+        t_run += r["seconds"]
+        lat += r["latency_ms"]
+        b.close()
+        del b
+        torch.cuda.empty_cache()
+    out["batch_tuned_4_streams"] = {"seconds": round(t_run, 3), "mvis_per_s": round(total / t_run / 1e6, 2),
+                                    "p50_ms": round(float(numpy.percentile(lat, 50)), 2),
+                                    "p99_ms": round(float(numpy.percentile(lat, 99)), 2),
+                                    "geometry_build_seconds": round(t_build, 3), "tuning_seconds": round(t_tune, 1),
+                                    "classes": len(tuner.table), "tuned": class_table(tuner)}
+    print("[study] batch", {k: v for k, v in out["batch_tuned_4_streams"].items() if k != "tuned"},
+          file=sys.stderr, flush=True)
     return out
 
 

@@ -431,3 +431,50 @@ def test_gain_calibration_gradients_at_production_size(c4, prec, eps, tol, hg, f
         print(f"{prec}: image gradient vs FP64 adjoint max rel {err:.2e}")
         assert err <= (1e-9 if prec == "fp64" else 1e-4)
     geom.close()
+
+
+def test_tuned_concurrent_batch_matches_explicit():
+    """40 uneven snapshot jobs through rascil_amd.batch: plan options tuned per job class on
+    the GPU (every chosen plan meets epsilon), the jobs' invert + predict graphs replayed
+    concurrently over 4 streams, twice; every job equals the brute-force sums to epsilon."""
+    import torch
+
+    from rascil_amd.batch import AutoTuner, Batch
+
+    dev = torch.device("cuda", 0)
+    rng = numpy.random.default_rng(41)
+    ants = simulation.core_plus_arms_stations(48, 0.7, 300.0, 500.0, 3000.0, rng)
+    eps = 1e-4
+    jobs, refs = [], []
+    for job in range(40):
+        nrow = int(numpy.exp(rng.uniform(numpy.log(200), numpy.log(1500))))
+        N = int(rng.choice([32, 48, 64]))
+        nchan = int(rng.integers(1, 4))
+        dec = math.radians(rng.uniform(-80, 10))
+        a1, _ = simulation.baseline_indices(len(ants), autocorrelations=False)
+        ntimes = max(1, nrow // len(a1) + 1)
+        ha0 = rng.uniform(-2, 2) * math.pi / 12
+        uvw = simulation.baseline_uvw(ants, ha0 + numpy.linspace(0, rng.uniform(2, 20) / 720 * math.pi, ntimes),
+                                      dec, autocorrelations=False).reshape(-1, 3)[:nrow]
+        freq = 1e8 + 2e6 * numpy.arange(nchan)
+        cell = 0.2 / (numpy.abs(uvw[:, :2]).max() * freq.max() / C)
+        ms = rng.normal(size=(len(uvw), nchan)) + 1j * rng.normal(size=(len(uvw), nchan))
+        img = rng.normal(size=(N, N))
+        jobs.append(dict(uvw=uvw, freq=freq, npix=N, cell=cell, nvis=len(uvw) * nchan,
+                         ms=torch.tensor(ms, dtype=torch.complex64, device=dev),
+                         model=torch.tensor(img, dtype=torch.float32, device=dev)))
+        refs.append((ox.explicit_ms2dirty(uvw, freq, ms, None, N, N, cell, cell, True),
+                     ox.explicit_dirty2ms(uvw, freq, img, None, cell, cell, True)))
+    tuner = AutoTuner()
+    batch = Batch(jobs, eps, "mixed", tuner=tuner, nstreams=4)
+    assert len(tuner.table) >= 2
+    for _ in range(2):
+        for d in batch.dirty:
+            d.fill_(7.0)
+        r = batch.run()
+        assert len(r["latency_ms"]) == len(jobs)
+        for k, (ref_d, ref_v) in enumerate(refs):
+            ed = float(numpy.linalg.norm(batch.dirty[k][0].double().cpu().numpy() - ref_d) / numpy.linalg.norm(ref_d))
+            ev = float(numpy.linalg.norm(batch.vis[k][..., 0].cpu().numpy() - ref_v) / numpy.linalg.norm(ref_v))
+            assert ed <= eps and ev <= eps, (k, batch.opts[k], ed, ev)
+    batch.close()
