@@ -1114,6 +1114,135 @@ __device__ __forceinline__ uint32_t seg_key(const KP& k, int s, int tx, int ty, 
   return (uint32_t)s * k.ntiles + (uint32_t)stx * k.nty + (uint32_t)sty;
 }
 
+// Strip ranges of the sub-tile gatherer. A bucket's records are sorted by in-tile origin with
+// the x origin ax % T first, so the records whose footprints can reach the 8 columns of strip
+// s (x cells 8 s .. 8 s + 7 of the tile) form one contiguous range per bucket: own-tile and
+// y-neighbour buckets need ax % T in [8 s - W + 1, 8 s + 7], x-neighbour and diagonal buckets
+// (the tile to the left, offsets ax % T - T) need ax % T >= T + 8 s - W + 1. split[v][key] is
+// the first record of bucket `key` with ax % T >= vals[v] (the bucket end if none); a bound
+// code is a split index, or kBStart / kBEnd for the bucket's start / end, or kEmpty.
+constexpr int kBStart = -1, kBEnd = -2, kEmpty = -3;
+constexpr int kMaxSplit = 8;
+struct Strips {
+  int nv;
+  int vals[kMaxSplit];
+  int lo[4][2], hi[4][2];  // [strip][x-neighbour] bound codes
+};
+
+Strips make_strips(int T, int W) {
+  Strips sr;
+  std::memset(&sr, 0, sizeof(sr));
+  std::vector<int> vals;
+  auto code = [&](int v) -> int {
+    if (v <= 0) return kBStart;
+    if (v >= T) return kBEnd;
+    for (size_t i = 0; i < vals.size(); ++i)
+      if (vals[i] == v) return (int)i;
+    vals.push_back(v);
+    return (int)vals.size() - 1;
+  };
+  for (int st = 0; st < T / 8; ++st) {
+    sr.lo[st][0] = code(8 * st - W + 1);
+    sr.hi[st][0] = code(8 * st + 8);
+    const int nlo = T + 8 * st - W + 1;
+    if (nlo >= T) {
+      sr.lo[st][1] = kEmpty;
+      sr.hi[st][1] = kEmpty;
+    } else {
+      sr.lo[st][1] = code(nlo);
+      sr.hi[st][1] = kBEnd;
+    }
+  }
+  if ((int)vals.size() > kMaxSplit) HIPNG_THROW(HIPNG_ERUNTIME, "strip split table too large");
+  sr.nv = (int)vals.size();
+  for (int i = 0; i < sr.nv; ++i) sr.vals[i] = vals[i];
+  return sr;
+}
+
+__device__ __forceinline__ uint32_t strip_bound(int code, uint32_t key, const uint32_t* __restrict__ bstart,
+                                                const uint32_t* __restrict__ bend, const uint32_t* __restrict__ split,
+                                                uint32_t nkeys) {
+  if (code == kBStart) return bstart[key];
+  if (code == kBEnd) return bend[key];
+  return split[(size_t)code * nkeys + key];
+}
+
+// [start, end) of the records of bucket `key` that strip `st` (x-neighbour bucket: xn) needs
+__device__ __forceinline__ void strip_range(const Strips& sr, int st, int xn, uint32_t key,
+                                            const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
+                                            const uint32_t* __restrict__ split, uint32_t nkeys, uint32_t* a,
+                                            uint32_t* b) {
+  const int lo = sr.lo[st][xn], hi = sr.hi[st][xn];
+  if (lo == kEmpty) {
+    *a = *b = 0u;
+    return;
+  }
+  *a = strip_bound(lo, key, bstart, bend, split, nkeys);
+  *b = strip_bound(hi, key, bstart, bend, split, nkeys);
+  if (*b < *a) *b = *a;
+}
+
+__global__ void k_split_init(const uint32_t* __restrict__ bend, uint32_t nkeys, int nv, uint32_t* __restrict__ split) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < (int64_t)nkeys * nv;
+       i += (int64_t)gridDim.x * kBlock)
+    split[i] = bend[i % nkeys];
+}
+
+// record i of the gatherer's list (srec entries, or all sorted records): its bucket from the
+// sorted keys and its x origin; where the origin first reaches vals[v] inside the bucket,
+// split[v][bucket] = i
+__global__ void k_split_points(const uint32_t* __restrict__ keys, const uint2* __restrict__ srec,
+                               const uint32_t* __restrict__ rec_a, int64_t nrec, int T, uint32_t amask, Strips sr,
+                               uint32_t nkeys, uint32_t* __restrict__ split) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nrec) return;
+  auto get = [&](int64_t r, uint32_t* key, int* ax) {
+    uint32_t j = (uint32_t)r, A;
+    if (srec) {
+      const uint2 e = srec[r];
+      j = e.x;
+      A = e.y;
+    } else {
+      A = rec_a[r];
+    }
+    *key = keys[j];
+    *ax = (int)((A & amask) & 0xffffu) % T;
+  };
+  uint32_t key, kp = 0xffffffffu;
+  int a, ap = -1;
+  get(i, &key, &a);
+  if (i > 0) get(i - 1, &kp, &ap);
+  if (kp != key) ap = -1;
+  for (int v = 0; v < sr.nv; ++v)
+    if (ap < sr.vals[v] && a >= sr.vals[v]) split[(size_t)v * nkeys + key] = (uint32_t)i;
+}
+
+// gatherer tasks: per tile the longest strip candidate list over the start planes [slo, shi]
+// and the four buckets, in chunks, times the strips; counters[1] += ntask[t]
+__global__ void k_tasks_grid(KP k, int slo, int shi, const uint32_t* __restrict__ bstart,
+                             const uint32_t* __restrict__ bend, const uint32_t* __restrict__ split, Strips sr,
+                             uint32_t chunk, uint32_t* __restrict__ ntask, uint32_t* __restrict__ counters) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)k.ntiles) return;
+  const int tx = (int)(t / k.nty), ty = (int)(t - (int64_t)tx * k.nty);
+  const uint32_t nkeys = (uint32_t)k.nplanes * k.ntiles;
+  const int ns = k.T / 8;
+  uint32_t best = 0;
+  for (int st = 0; st < ns; ++st) {
+    uint32_t cnt = 0;
+    for (int s = slo; s <= shi; ++s)
+      for (int q = 0; q < 4; ++q) {
+        uint32_t a, b;
+        strip_range(sr, st, q >> 1, seg_key(k, s, tx, ty, q), bstart, bend, split, nkeys, &a, &b);
+        cnt += b - a;
+      }
+    best = max(best, cnt);
+  }
+  const uint32_t nt = (best + chunk - 1) / chunk * (uint32_t)ns;
+  ntask[t] = nt;
+  if (nt) atomicAdd(&counters[1], nt);
+}
+
 // ntask[t] = ceil(candidates / chunk) for start planes [slo, shi]; counters[1] += ntask[t]
 __global__ void k_tasks(KP k, int slo, int shi, int nq, const uint32_t* __restrict__ bstart,
                         const uint32_t* __restrict__ bend, uint32_t chunk, uint32_t* __restrict__ ntask,
@@ -1375,7 +1504,8 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
                                                   const uint32_t* __restrict__ task_off,
                                                   const uint32_t* __restrict__ task_tab, uint32_t chunk,
                                                   uint32_t* __restrict__ counters,
-                                                  cplx<R>* __restrict__ plane) {
+                                                  cplx<R>* __restrict__ plane,
+                                                  const uint32_t* __restrict__ split, Strips sr) {
   constexpr int NS = T / 8;               // strips per tile
   static_assert((RG == 8 || RG == 16) && T % RG == 0, "row groups of 8 or 16 tile the strip");
   constexpr int NT = grid8_taps(PC);      // per staged visibility: 8 x taps, 8 y taps, PC w taps
@@ -1427,15 +1557,17 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
     const uint32_t tnext = tile + 1 < (int)k.ntiles ? task_off[tile + 1] : total;
     const bool solo = tnext - task_off[tile] == (uint32_t)NS;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
-    // segment table: one bucket per lane, inclusive scan by shuffles
+    // segment table: one bucket per lane (the strip's range of it), inclusive scan by shuffles
+    const uint32_t nkeys = (uint32_t)k.nplanes * k.ntiles;
     for (int sg0 = 0; sg0 < nseg; sg0 += 64) {
       const int sg = sg0 + lane;
       uint32_t len = 0;
       if (sg < nseg) {
-        const uint32_t key = seg_key(k, slo + sg / 4, tx, ty, sg & 3);
-        const uint32_t st = bstart[key];
-        len = bend[key] - st;
-        s_segj[sg] = st;
+        uint32_t a, b;
+        strip_range(sr, sx0 / 8, (sg & 3) >> 1, seg_key(k, slo + sg / 4, tx, ty, sg & 3), bstart, bend, split, nkeys,
+                    &a, &b);
+        len = b - a;
+        s_segj[sg] = a;
       }
       uint32_t inc = len;
 #pragma unroll
@@ -3471,7 +3603,7 @@ void prof_flush() {
 enum Slot {
   S_KEYS0, S_VALS0, S_KEYS1, S_VALS1, S_HIST, S_PART, S_BSTART, S_BEND, S_RECA, S_RECF, S_RECV,
   S_PLANE, S_ACCIMG, S_ACCS, S_SMALL, S_EXT, S_FFTWORK, S_TASK, S_TASKPART, S_TASKTAB, S_CNT, S_TBUF, S_TBUF2, S_DTASK,
-  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_PLANE2, S_LASTW, S_IDFT, S_NSLOTS
+  S_DPLANE, S_RECP, S_FLIPB, S_SREC, S_SBSTART, S_SBEND, S_RUNT, S_RUND, S_GAIN, S_PLANE2, S_LASTW, S_IDFT, S_SPLIT, S_NSLOTS
 };
 
 struct FftKey {
@@ -3850,6 +3982,9 @@ struct Geom {
   // records outside dense runs: sorted record indices and their bucket table
   uint2* srec = nullptr;  // non-dense records: (record index, origin word)
   uint32_t *sbstart = nullptr, *sbend = nullptr;
+  // strip ranges of the sub-tile gatherer over its buckets (gb0 / gb1 below)
+  Strips strips;
+  uint32_t* split = nullptr;
   Buffers buf;
 };
 
@@ -4039,6 +4174,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     const int64_t td = n / g.chunk_degrid + ntiles;
     g.tasktab = (uint32_t*)g.buf.get(S_TASKTAB, (size_t)(std::max(tg, td) + 1) * 4);
   }
+  int64_t nsparse = n;  // records on the gatherer's list
   if (g.dense) {
     PROF("runs");
     // scr0 = exclusive scan of the run heads (run of each record); scr1 = run starts
@@ -4075,6 +4211,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
                                                                        g.dtask, g.dplane, g.counters);
     LAUNCH_CHECK();
     const int64_t ns = n - (int64_t)ndense;
+    nsparse = ns;
     g.srec = (uint2*)g.buf.get(S_SREC, (size_t)std::max<int64_t>(ns, 1) * sizeof(uint2));
     k_run_mark<<<grid1(n), kBlock, 0, st>>>(g.rec_a, n, scr0, scr1, dpre, g.counters, dmin, g.k.dflag, g.srec);
     LAUNCH_CHECK();
@@ -4084,17 +4221,40 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
                                                      g.sbstart, g.sbend);
     LAUNCH_CHECK();
   }
+  {
+    // strip split points over the gatherer's list (the compacted non-dense records, or all)
+    PROF("runs");
+    g.strips = make_strips(P.tile, P.W);
+    const uint32_t* gb1 = g.srec ? g.sbend : g.bend;
+    const int64_t nrec = g.srec ? nsparse : n;
+    g.split = (uint32_t*)g.buf.get(S_SPLIT, (size_t)std::max(1, g.strips.nv) * nkeys * 4);
+    if (g.strips.nv > 0) {
+      k_split_init<<<gridcap((int64_t)nkeys * g.strips.nv), kBlock, 0, st>>>(gb1, (uint32_t)nkeys, g.strips.nv,
+                                                                               g.split);
+      LAUNCH_CHECK();
+      if (nrec > 0) {
+        k_split_points<<<grid1(nrec), kBlock, 0, st>>>(keys, g.srec, g.rec_a, nrec, P.tile, g.k.amask, g.strips,
+                                                       (uint32_t)nkeys, g.split);
+        LAUNCH_CHECK();
+      }
+    }
+  }
   if (own) HIP_TRY(hipStreamSynchronize(st));
 }
 
 // per-plane task list: counters reset, per-tile task counts, exclusive scan -> task offsets
+// strips: the gatherer's tasks (the longest strip range list of a tile, times its strips)
 void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStream_t st, uint32_t mult = 1,
-                 const uint32_t* bs = nullptr, const uint32_t* be = nullptr) {
+                 const uint32_t* bs = nullptr, const uint32_t* be = nullptr, bool strips = false) {
   const int64_t ntiles = g.k.ntiles;
   k_fill_u32<<<1, kBlock, 0, st>>>(g.counters, kDenseCnt + 1, 0u);
   LAUNCH_CHECK();
-  k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, bs ? bs : g.bstart, be ? be : g.bend, chunk, g.task,
-                                            g.counters, mult);
+  if (strips)
+    k_tasks_grid<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, bs ? bs : g.bstart, be ? be : g.bend, g.split,
+                                                   g.strips, chunk, g.task, g.counters);
+  else
+    k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, bs ? bs : g.bstart, be ? be : g.bend, chunk, g.task,
+                                              g.counters, mult);
   LAUNCH_CHECK();
   const int64_t npart = (ntiles + kRsItems - 1) / kRsItems;
   k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
@@ -4411,7 +4571,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
       int slo, shi;
       chunk_range(k.Ww, P.nplanes, p0, npl, &slo, &shi);
       // the sub-tile gatherer walks the compacted table of records outside dense runs
-      build_tasks(g, slo, shi, 4, g.chunk_grid, st, P.tile / 8, gb0, gb1);
+      build_tasks(g, slo, shi, 4, g.chunk_grid, st, P.tile / 8, gb0, gb1, true);
       // footprints stay inside the band, so only its rows need clearing; tiles the gatherer
       // owns alone are stored in full by it and are skipped here
       if (oop) {
@@ -4433,7 +4593,8 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
 #define HIPNG_GW(TT, QQ, BB)                                                                      \
   k_grid8<R, TT, QQ, BB, RG><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.srec, g.rec_a,  \
                                                               (const rec3<R>*)g.rec_f, recv, g.task, \
-                                                              g.tasktab, g.chunk_grid, g.counters, plane)
+                                                              g.tasktab, g.chunk_grid, g.counters, plane, \
+                                                              g.split, g.strips)
 #define HIPNG_GWB(TT, QQ)                 \
   if (g.nb == 1) HIPNG_GW(TT, QQ, 1);     \
   else if (g.nb == 2) HIPNG_GW(TT, QQ, 2); \
