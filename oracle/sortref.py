@@ -16,9 +16,11 @@ class OraclePlan(ctypes.Structure):
 
 
 def lib():
+    """The oracle's C library: oracle/liboracle.so (built on first use), or the file named
+    by ORACLE_LIB (the sanitizer build of the tests)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             from . import build
 

@@ -273,6 +273,8 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
   const double safety = 2.0 * std::sqrt((double)ndim);
   const size_t lds_budget = kLdsBudget;
   const size_t csz = prec == HIPNG_FP64 ? 16 : 8;
+  // bytes per cell of the degridder's LDS tile: FP32 polarisations in pairs (16-byte units)
+  const size_t cellb = csz * (size_t)(prec != HIPNG_FP64 && nbatch == 3 ? 4 : nbatch);
   double best_cost = 1e300;
   bool lds_limited = false;
   double fp32_floor_best = 1e300;  // smallest FP32 coordinate floor over the candidates
@@ -288,11 +290,11 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
       // 8 x 8 sub-tile, and a smaller tile has fewer candidates per sub-tile); 32 on request
       // when the degridder's LDS tile with halo (T+W-1)^2 * batch fits
       int t = o.tile == 32 ? 32 : 16;
-      if ((size_t)(t + w - 1) * (t + w - 1) * nbatch * csz > (size_t)lds_budget) {
+      if ((size_t)(t + w - 1) * (t + w - 1) * cellb > (size_t)lds_budget) {
         if (o.tile == 32) continue;
         t = 16;
       }
-      if ((size_t)(t + w - 1) * (t + w - 1) * nbatch * csz > (size_t)lds_budget) {
+      if ((size_t)(t + w - 1) * (t + w - 1) * cellb > (size_t)lds_budget) {
         lds_limited = true;
         continue;
       }
@@ -1529,22 +1531,24 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const int nseg = (shi - slo + 1) * 4;
-  // XCD-local work queues: the NS strip tasks of a (tile, chunk) read the same records, so
-  // groups are dealt round-robin to kXcd queues and a wave pops from the queue of its own
-  // XCD (workgroups are placed on XCDs round-robin by index) before helping the others
+  // XCD-local work queues: the NS strip tasks of a (tile, chunk) read the same records, and
+  // so do the tasks of the next tile in y (its neighbour buckets), so queue x holds the
+  // contiguous x-th eighth of the (tile, chunk) groups and a wave pops from the queue of its
+  // own XCD (workgroups are placed on XCDs round-robin by index) before helping the others
   const uint32_t ngroups = total / NS;
+  const uint32_t gper = (ngroups + kXcd - 1) / kXcd;
   const int xcd0 = (int)(blockIdx.x % kXcd);
   int qi = 0;
   for (;;) {
     uint32_t task = 0xffffffffu;
     while (qi < kXcd) {
       const int x = (xcd0 + qi) % kXcd;
-      const uint32_t qsize = ngroups > (uint32_t)x ? (ngroups - x + kXcd - 1) / kXcd : 0u;
+      const uint32_t g0 = min(ngroups, (uint32_t)x * gper), qsize = min(ngroups - g0, gper);
       uint32_t l = 0;
       if (lane == 0) l = atomicAdd(&counters[kQueueBase + x], 1u);
       l = __shfl(l, 0, 64);
       if (l < qsize * NS) {
-        task = ((uint32_t)x + kXcd * (l / NS)) * NS + l % NS;
+        task = (g0 + l / NS) * NS + l % NS;
         break;
       }
       ++qi;
@@ -2523,6 +2527,16 @@ __global__ __launch_bounds__(kBlock) void k_prescale_t(KP k, const I* __restrict
 // read-modify-write of the sorted accumulator per pass; no atomics. The candidates of a
 // start-plane segment share their plane range, so the plane loop does not diverge.
 // ---------------------------------------------------------------------------------
+// polarisations per 16-byte unit of the degridder's LDS tile
+template <typename R, int NB>
+__host__ __device__ constexpr int dg_per() {
+  return NB < (int)(16 / sizeof(cplx<R>)) ? NB : (int)(16 / sizeof(cplx<R>));
+}
+template <typename R, int NB>
+__host__ __device__ constexpr int dg_slots() {
+  return (NB + dg_per<R, NB>() - 1) / dg_per<R, NB>() * dg_per<R, NB>();
+}
+
 template <typename R, int W, int NB>
 __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                    const uint32_t* __restrict__ bend,
@@ -2535,13 +2549,16 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
                                                    const uint32_t* __restrict__ task_tab, uint32_t chunk,
                                                    uint32_t* __restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  // [npl][L][L][NB], cell-major: a lane reads a cell's polarisations with two 16-byte reads
-  // (a polarisation-planar tile, four 8-byte reads per cell and fewer bank conflicts, measured
-  // 16 % slower at the production shape: the kernel is bound by vector issue, not LDS)
+  // [U][npl][L][L][P]: the NB polarisations of a cell in U units of P (16 bytes: two FP32
+  // or one FP64 polarisation), each unit a plane of its own, so that lanes reading nearby
+  // cells of a unit touch consecutive 16-byte slots (a cell-major [L][L][NB] tile puts the
+  // same half of consecutive cells 32 bytes apart, so two of every eight lanes share banks)
+  constexpr int P = dg_per<R, NB>(), U = (NB + P - 1) / P;
   cplx<R>* g = reinterpret_cast<cplx<R>*>(smem_raw);
   __shared__ uint32_t s_task;
   __shared__ uint32_t s_segj[64], s_segv[65];  // per start plane: first record, exclusive prefix
   const int T = k.T, L = k.T + W - 1, LL = L * L;
+  const int ustride = npl * LL * P;  // elements between units
   const R scale = (R)2 / (R)W;
   // the FP32 tap evaluation works in base 2
   const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
@@ -2550,11 +2567,30 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const int nseg = shi - slo + 1;  // <= W_w + npl - 1 <= 64
   int loaded = -1;
+  // XCD-local work queues: tasks of neighbouring tiles in y share halo rows of the planes, so
+  // queue x holds the contiguous x-th eighth of the tasks and a workgroup pops from the queue
+  // of its own XCD (workgroups are placed on XCDs round-robin by index) before the others
+  const uint32_t tper = (total + kXcd - 1) / kXcd;
+  const int xcd0 = (int)(blockIdx.x % kXcd);
+  int qi = 0;  // used by thread 0 only
   for (;;) {
-    if (threadIdx.x == 0) s_task = atomicAdd(&counters[0], 1u);
+    if (threadIdx.x == 0) {
+      uint32_t t = 0xffffffffu;
+      while (qi < kXcd) {
+        const int x = (xcd0 + qi) % kXcd;
+        const uint32_t t0 = min(total, (uint32_t)x * tper), qsize = min(total - t0, tper);
+        const uint32_t l = atomicAdd(&counters[kQueueBase + x], 1u);
+        if (l < qsize) {
+          t = t0 + l;
+          break;
+        }
+        ++qi;
+      }
+      s_task = t;
+    }
     __syncthreads();
     const uint32_t task = s_task;
-    if (task >= total) break;
+    if (task == 0xffffffffu) break;
     const int tile = (int)task_tab[task];
     const uint32_t vlo = (task - task_off[tile]) * chunk, vhi = vlo + chunk;
     const int tx = tile / k.nty, ty = tile - tx * k.nty;
@@ -2581,7 +2617,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
             int gxi = tx * T + r1;
             if (gxi >= k.gx) gxi -= k.gx;
             v[u] = plane[q * plane_stride + ((int64_t)b * k.gx + gxi) * k.gy + gyi];
-            dst[u] = ok ? ((q * L + r1) * L + cc) * NB + b : -1;
+            dst[u] = ok ? ((b / P) * npl * LL + (q * L + r1) * L + cc) * P + b % P : -1;
             row += RS;
             rr += RS;
             if (rr >= L) {
@@ -2657,7 +2693,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
         for (int b = 0; b < NB; ++b) sr[b] = si[b] = 0;
         for (int q = qlo; q < qhi; ++q) {
           const R kw = k.wstack ? esk_fast((f.w + (R)(p0 + q - s)) * scale, beta) : (R)1;
-          const cplx<R>* gq = g + ((q * L + rx0) * L + ry0) * NB;
+          const cplx<R>* gq = g + ((q * L + rx0) * L + ry0) * P;
           R tr[NB], ti[NB];
 #pragma unroll
           for (int b = 0; b < NB; ++b) tr[b] = ti[b] = 0;
@@ -2670,7 +2706,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
             for (int iy = 0; iy < W; ++iy) {
 #pragma unroll
               for (int b = 0; b < NB; ++b) {
-                const cplx<R> v = gq[(ix * L + iy) * NB + b];
+                const cplx<R> v = gq[(b / P) * ustride + (ix * L + iy) * P + b % P];
                 rr[b] += ky[iy] * v.re;
                 ri[b] += ky[iy] * v.im;
               }
@@ -2724,10 +2760,13 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
                                                             uint32_t* __restrict__ counters) {
   static_assert(kMaxPC == 2, "the 16 columns of A are 8 cells x 2 planes");
   constexpr int NW = kBlock / 64;
-  __shared__ float s_tx[NW][64][9];
-  __shared__ float s_ty[NW][64][9];
+  // taps tap-major, row strides chosen so that the reads of a half wave (16 visibilities x
+  // 2 rows) fall in distinct banks: x rows 4 s + {0, 1} are 16 banks apart (80 = 16 mod 32),
+  // y rows r and 4 + r likewise (4 x 68 = 16 mod 32); results with an odd stride per visibility
+  __shared__ float s_tx[NW][8][80];
+  __shared__ float s_ty[NW][8][68];
   __shared__ float s_tw[NW][64][2];
-  __shared__ float s_out[NW][64][NB][2];
+  __shared__ float s_out[NW][64][2 * NB + 1];
   const int W = k.W;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int gk = lane >> 4, gm = lane & 15;
@@ -2775,8 +2814,8 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
         if (live) f = rec_f[cur.x + b0 + (uint32_t)lane];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-          s_tx[wv][lane][i] = live && i < W ? esk_fast((f.x + (float)i) * scale, beta) : 0.f;
-          s_ty[wv][lane][i] = live && i < W ? esk_fast((f.y + (float)i) * scale, beta) : 0.f;
+          s_tx[wv][i][lane] = live && i < W ? esk_fast((f.x + (float)i) * scale, beta) : 0.f;
+          s_ty[wv][i][lane] = live && i < W ? esk_fast((f.y + (float)i) * scale, beta) : 0.f;
         }
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -2793,11 +2832,11 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
           const int kk = 4 * st + gk;
-          av[st] = s_tx[wv][m][kk & 7] * s_tw[wv][m][kk >> 3];
+          av[st] = s_tx[wv][kk & 7][m] * s_tw[wv][m][kk >> 3];
         }
         // D rows 4 gk + r = (c, h): c = 4 (gk & 1) + r, h = gk >> 1
 #pragma unroll
-        for (int r = 0; r < 4; ++r) ky[r] = s_ty[wv][m][4 * (gk & 1) + r];
+        for (int r = 0; r < 4; ++r) ky[r] = s_ty[wv][4 * (gk & 1) + r][m];
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           f32x4 u = {0.f, 0.f, 0.f, 0.f};
@@ -2805,7 +2844,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
           for (int st = 0; st < 4; ++st) u = __builtin_amdgcn_mfma_f32_16x16x4f32(bw[b][st], av[st], u, 0, 0, 0);
           float x = u[0] * ky[0] + u[1] * ky[1] + u[2] * ky[2] + u[3] * ky[3];
           x += __shfl_xor(x, 16, 64);
-          if ((gk & 1) == 0) s_out[wv][m][b][gk >> 1] = x;
+          if ((gk & 1) == 0) s_out[wv][m][2 * b + (gk >> 1)] = x;
         }
       }
       wave_sync();
@@ -2815,8 +2854,8 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           cplx<float> a = first ? cplx<float>{0.f, 0.f} : o[b];
-          a.re += s_out[wv][lane][b][0];
-          a.im += s_out[wv][lane][b][1];
+          a.re += s_out[wv][lane][2 * b];
+          a.im += s_out[wv][lane][2 * b + 1];
           o[b] = a;
         }
       }
@@ -4325,17 +4364,24 @@ void check_fft_inputs(Geom& g, DevCtx& c, hipStream_t st) {
   g.oop_ok = ok;
 }
 
+// polarisation slots per cell of the degridder's LDS tile (whole 16-byte units)
+int degrid_slots(const Geom& g) {
+  if (g.prec == HIPNG_FP64) return g.nb;
+  return g.nb == 1 ? dg_slots<float, 1>() : g.nb == 2 ? dg_slots<float, 2>() : g.nb == 3 ? dg_slots<float, 3>()
+                                                                                          : dg_slots<float, 4>();
+}
+
 // planes whose tiles the degridder holds in LDS at once: as many of the pass's planes as fit
 // in kDegridLds (four workgroups per CU), at least one
 int degrid_planes(const Geom& g, int npl) {
   const size_t L = (size_t)g.k.T + g.k.W - 1;
-  const size_t per = L * L * g.nb * (g.prec == HIPNG_FP64 ? 16 : 8);
+  const size_t per = L * L * degrid_slots(g) * (g.prec == HIPNG_FP64 ? 16 : 8);
   int q = npl;
   while (q > 1 && q * per > kDegridLds) --q;
   return q;
 }
 
-// dense-run kernels: 5 workgroups per CU fit their LDS (29 KiB each, FP32)
+// dense-run kernels: 5 workgroups per CU fit their LDS (at most 30 KiB each, FP32)
 constexpr int kDenseBlocksPerCu = 5;
 
 // FP32 with dense runs: the dense runs on the matrix cores, the other records in k_degrid;
@@ -4369,7 +4415,7 @@ template <typename R>
 void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>* accs, hipStream_t st) {
   const KP& k = g.k;
   const int L = k.T + k.W - 1;
-  const size_t sm = (size_t)npl * g.nb * L * L * sizeof(cplx<R>);
+  const size_t sm = (size_t)npl * degrid_slots(g) * L * L * sizeof(cplx<R>);
   const int64_t pstride = (int64_t)g.nb * k.gx * k.gy;
   const auto* rf = (const rec3<R>*)g.rec_f;
   const bool split = degrid_split<R>(g);
