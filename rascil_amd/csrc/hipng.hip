@@ -1340,6 +1340,60 @@ __device__ __forceinline__ float esk_fast(float z, float beta2) {
 }
 __device__ __forceinline__ double esk_fast(double z, double beta) { return esk(z, beta); }
 
+// Piecewise-polynomial kernel taps (FP32 kernel modes). Tap i of an axis is psi((f + i) 2/W)
+// for the record's offset f in (-W/2, -W/2 + 1]. For the interior taps 0 < i < W - 1 that is
+// a smooth function of s = 2 f + W - 1 in (-1, 1], replaced by a degree-kTapDeg polynomial
+// fitted at geometry build by Chebyshev interpolation (the piecewise Horner evaluation of
+// Barnett, Magland & af Klinteberg, SIAM J. Sci. Comput. 41 (2019) C479, for their ES
+// kernel): kTapDeg FMAs instead of a square root, a reciprocal and an exponential. The two
+// edge taps hold the kernel's square-root end point and keep the direct formula. `on` is 0
+// where the fit does not reach the FP32 evaluation level (small supports): direct formula.
+constexpr int kTapDeg = 7;
+struct TapPoly {
+  int on;
+  float c[kWMax][kTapDeg + 1];  // c[i][d]: coefficient of s^d of tap i
+};
+__device__ __forceinline__ float tap_poly(const TapPoly& tp, int i, float s) {
+  float p = tp.c[i][kTapDeg];
+#pragma unroll
+  for (int d = kTapDeg - 1; d >= 0; --d) p = fmaf(p, s, tp.c[i][d]);
+  return p;
+}
+// all W taps of one axis (W a compile-time support)
+template <typename R, int W>
+__device__ __forceinline__ void taps_axis(const TapPoly& tp, R f, R scale, R beta, R (&kt)[W]) {
+  if constexpr (std::is_same<R, float>::value) {
+    if (tp.on) {
+      const float s = fmaf(2.0f, f, (float)(W - 1));
+      kt[0] = esk_fast(f * scale, beta);
+#pragma unroll
+      for (int i = 1; i < W - 1; ++i) kt[i] = tap_poly(tp, i, s);
+      kt[W - 1] = esk_fast((f + (float)(W - 1)) * scale, beta);
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i) kt[i] = esk_fast((f + (R)i) * scale, beta);
+}
+// taps i < 8 of one axis for a run-time support W <= 8 (zero for i >= W or a dead lane)
+__device__ __forceinline__ void taps8(const TapPoly& tp, int W, float f, float scale, float beta, bool live,
+                                      float (&kt)[8]) {
+  if (tp.on) {
+    const float s = fmaf(2.0f, f, (float)(W - 1));
+    const float e0 = esk_fast(f * scale, beta), e1 = esk_fast((f + (float)(W - 1)) * scale, beta);
+    kt[0] = e0;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) kt[i] = i == W - 1 ? e1 : (i < W - 1 ? tap_poly(tp, i, s) : 0.f);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) kt[i] = i < W ? esk_fast((f + (float)i) * scale, beta) : 0.f;
+  }
+  if (!live) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) kt[i] = 0.f;
+  }
+}
+
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Staged entry e: taps s_t[e * NT + i] (i < 8: x taps of the sub-tile's 8 columns; 8 + i:
@@ -2128,7 +2182,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
                                                           const rec3<float>* __restrict__ rec_f,
                                                           const cplx<float>* __restrict__ rec_v,
                                                           uint32_t* __restrict__ counters,
-                                                          cplx<float>* __restrict__ plane) {
+                                                          cplx<float>* __restrict__ plane, TapPoly tp) {
   static_assert(kMaxPC == 2, "the 16 rows of a block are 8 cells x 2 planes");
   constexpr int PC = kMaxPC;
   constexpr int NW = kBlock / 64;
@@ -2181,10 +2235,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
     const int sp = (int)cur.w;
     {
       const bool live = lane < cnt;
+      float tx8[8], ty8[8];
+      taps8(tp, W, f.x, scale, beta, live, tx8);
+      taps8(tp, W, f.y, scale, beta, live, ty8);
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        s_tx[wv][lane][i] = live && i < W ? esk_fast((f.x + (float)i) * scale, beta) : 0.f;
-        s_ty[wv][lane][i] = live && i < W ? esk_fast((f.y + (float)i) * scale, beta) : 0.f;
+        s_tx[wv][lane][i] = tx8[i];
+        s_ty[wv][lane][i] = ty8[i];
       }
 #pragma unroll
       for (int q = 0; q < PC; ++q) {
@@ -2547,7 +2604,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
                                                    cplx<R>* __restrict__ accs,
                                                    const uint32_t* __restrict__ task_off,
                                                    const uint32_t* __restrict__ task_tab, uint32_t chunk,
-                                                   uint32_t* __restrict__ counters) {
+                                                   uint32_t* __restrict__ counters, TapPoly tp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // [U][npl][L][L][P]: the NB polarisations of a cell in U units of P (16 bytes: two FP32
   // or one FP64 polarisation), each unit a plane of its own, so that lanes reading nearby
@@ -2683,11 +2740,8 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
         const int rx0 = (int)(A & 0xffffu) - tx * T, ry0 = (int)(A >> 16) - ty * T;
         const rec3<R> f = rec_f[j];
         R kx[W], ky[W];
-#pragma unroll
-        for (int i = 0; i < W; ++i) {
-          kx[i] = esk_fast((f.x + (R)i) * scale, beta);
-          ky[i] = esk_fast((f.y + (R)i) * scale, beta);
-        }
+        taps_axis<R, W>(tp, f.x, scale, beta, kx);
+        taps_axis<R, W>(tp, f.y, scale, beta, ky);
         R sr[NB], si[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) sr[b] = si[b] = 0;
@@ -2757,7 +2811,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
                                                             const rec3<float>* __restrict__ rec_f,
                                                             const cplx<float>* __restrict__ plane,
                                                             int64_t plane_stride, cplx<float>* __restrict__ accs,
-                                                            uint32_t* __restrict__ counters) {
+                                                            uint32_t* __restrict__ counters, TapPoly tp) {
   static_assert(kMaxPC == 2, "the 16 columns of A are 8 cells x 2 planes");
   constexpr int NW = kBlock / 64;
   // taps tap-major, row strides chosen so that the reads of a half wave (16 visibilities x
@@ -2812,10 +2866,13 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
         const bool live = lane < cnt;
         rec3<float> f{};
         if (live) f = rec_f[cur.x + b0 + (uint32_t)lane];
+        float tx8[8], ty8[8];
+        taps8(tp, W, f.x, scale, beta, live, tx8);
+        taps8(tp, W, f.y, scale, beta, live, ty8);
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-          s_tx[wv][i][lane] = live && i < W ? esk_fast((f.x + (float)i) * scale, beta) : 0.f;
-          s_ty[wv][i][lane] = live && i < W ? esk_fast((f.y + (float)i) * scale, beta) : 0.f;
+          s_tx[wv][i][lane] = tx8[i];
+          s_ty[wv][i][lane] = ty8[i];
         }
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -3784,6 +3841,57 @@ void channel_scales(const hipng_plan& P, const double* freq, int nchan, double* 
   }
 }
 
+// Degree-kTapDeg polynomials of the interior kernel taps (TapPoly): Chebyshev interpolation
+// of tap i at kTapDeg + 1 Chebyshev points of s in [-1, 1] in FP64, converted to monomials in
+// s and rounded to FP32. on = 1 when the fit's largest deviation from the kernel, sampled
+// over every interior tap's interval, is at the FP32 evaluation level (1.2e-7) or below 5 %
+// of the kernel's own 1-D error at this oversampling and support.
+TapPoly make_tap_poly(int W, double beta, double sigma) {
+  TapPoly tp{};
+  if (W < 4 || W > kWMax) return tp;
+  double err1d = 0.0;
+  for (int is = 0; is < kNSig; ++is)
+    if (std::fabs(kSigmas[is] - sigma) < 1e-9) err1d = kErr1D[is][W - kWMin];
+  auto psi = [&](double z) {
+    const double t = 1.0 - z * z;
+    return t > 0.0 ? std::exp(beta * (std::sqrt(t) - 1.0)) : 0.0;
+  };
+  auto zof = [&](int i, double sv) { return ((sv + 1.0) * 0.5 - 0.5 * W + i) * 2.0 / W; };
+  constexpr int N = kTapDeg + 1;
+  double worst = 0.0;
+  for (int i = 1; i < W - 1; ++i) {
+    double y[N], a[N], mono[N] = {0}, tm[N] = {0}, t0[N] = {0}, t1[N] = {0};
+    for (int q = 0; q < N; ++q) y[q] = psi(zof(i, std::cos(kPi * (q + 0.5) / N)));
+    for (int j = 0; j < N; ++j) {
+      double acc = 0.0;
+      for (int q = 0; q < N; ++q) acc += y[q] * std::cos(kPi * j * (q + 0.5) / N);
+      a[j] = (j ? 2.0 : 1.0) * acc / N;
+    }
+    // sum_j a_j T_j(s) in monomials: T_0 = 1, T_1 = s, T_(j+1) = 2 s T_j - T_(j-1)
+    t0[0] = 1.0;
+    t1[1] = 1.0;
+    mono[0] = a[0];
+    for (int d = 0; d < N; ++d) mono[d] += a[1] * t1[d];
+    for (int j = 2; j < N; ++j) {
+      for (int d = 0; d < N; ++d) tm[d] = (d ? 2.0 * t1[d - 1] : 0.0) - t0[d];
+      for (int d = 0; d < N; ++d) {
+        mono[d] += a[j] * tm[d];
+        t0[d] = t1[d];
+        t1[d] = tm[d];
+      }
+    }
+    for (int d = 0; d < N; ++d) tp.c[i][d] = (float)mono[d];
+    for (int m = 0; m <= 512; ++m) {
+      const double sv = -1.0 + m / 256.0;
+      double pv = tp.c[i][kTapDeg];
+      for (int d = kTapDeg - 1; d >= 0; --d) pv = pv * sv + tp.c[i][d];
+      worst = std::max(worst, std::fabs(pv - psi(zof(i, sv))));
+    }
+  }
+  tp.on = worst <= std::max(1.2e-7, 0.05 * err1d) ? 1 : 0;
+  return tp;
+}
+
 KP make_kp(const hipng_plan& P, int nb, int nchan) {
   KP k;
   k.gx = P.gx;
@@ -4024,6 +4132,7 @@ struct Geom {
   // strip ranges of the sub-tile gatherer over its buckets (gb0 / gb1 below)
   Strips strips;
   uint32_t* split = nullptr;
+  TapPoly taps{};  // interior kernel taps as polynomials (FP32 kernel modes)
   Buffers buf;
 };
 
@@ -4081,6 +4190,7 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   extent(c, uvw, nrow, freq, nchan, e, st);
   make_plan(e[0], e[1], e[2], e[3], nx, ny, px, py, eps, wstack, prec, nb, n, opts, &g.P);
   g.k = make_kp(g.P, nb, nchan);
+  if (prec != HIPNG_FP64) g.taps = make_tap_poly(g.P.W, g.P.beta, g.P.sigma);
   g.tb = upload_tables(g.buf, g.P, freq, nchan, st);
   g.nrow = nrow;
   g.n = n;
@@ -4398,13 +4508,13 @@ void launch_degrid_dense(const Geom& g, int p0, int npl, const cplx<float>* plan
   const auto* rf = (const rec3<float>*)g.rec_f;
   const int nblk = device_cus() * kDenseBlocksPerCu;
   if (g.nb == 1)
-    k_degrid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+    k_degrid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   else if (g.nb == 2)
-    k_degrid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+    k_degrid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   else if (g.nb == 3)
-    k_degrid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+    k_degrid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   else
-    k_degrid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters);
+    k_degrid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   LAUNCH_CHECK();
 }
 void launch_degrid_dense(const Geom&, int, int, const cplx<double>*, cplx<double>*, hipStream_t) {
@@ -4425,7 +4535,7 @@ void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>
 #define HIPNG_DG1(WW, BB)                                                                          \
   k_degrid<R, WW, BB><<<g.npersist, kBlock, sm, st>>>(k, p0, npl, b0, b1, sr, g.rec_a, rf, plane, \
                                                       pstride, accs, g.task, g.tasktab, g.chunk_degrid,  \
-                                                      g.counters)
+                                                      g.counters, g.taps)
 #define HIPNG_DG(WW)                        \
   case WW:                                  \
     if (g.nb == 1) HIPNG_DG1(WW, 1);        \
@@ -4478,13 +4588,13 @@ void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<f
                        const cplx<float>* recv, cplx<float>* plane, hipStream_t st) {
   const int nblk = device_cus() * kDenseBlocksPerCu;
   if (g.nb == 1)
-    k_grid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else if (g.nb == 2)
-    k_grid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else if (g.nb == 3)
-    k_grid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else
-    k_grid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
 }
 void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<double>* rf,
                        const cplx<double>* recv, cplx<double>* plane, hipStream_t st) {
