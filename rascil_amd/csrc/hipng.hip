@@ -1346,13 +1346,27 @@ __device__ __forceinline__ double esk_fast(double z, double beta) { return esk(z
 // fitted at geometry build by Chebyshev interpolation (the piecewise Horner evaluation of
 // Barnett, Magland & af Klinteberg, SIAM J. Sci. Comput. 41 (2019) C479, for their ES
 // kernel): kTapDeg FMAs instead of a square root, a reciprocal and an exponential. The two
-// edge taps hold the kernel's square-root end point and keep the direct formula. `on` is 0
+// edge taps hold the kernel's square-root end point, where a polynomial converges slowly; they
+// keep the direct formula unless `on_all` (their fit error is also within the bound). `on` is 0
 // where the fit does not reach the FP32 evaluation level (small supports): direct formula.
 constexpr int kTapDeg = 7;
 struct TapPoly {
-  int on;
+  int on;       // interior taps from the polynomials
+  int on_all;   // edge taps too (their fit also meets the bound below)
   float c[kWMax][kTapDeg + 1];  // c[i][d]: coefficient of s^d of tap i
 };
+// Horner's rule over one coefficient row held as two float4 (c0..c3, c4..c7)
+static_assert(kTapDeg == 7, "poly_row holds 8 coefficients");
+__device__ __forceinline__ float poly_row(const float4* r, float s) {
+  const float4 a = r[0], b = r[1];
+  float p = fmaf(b.w, s, b.z);
+  p = fmaf(p, s, b.y);
+  p = fmaf(p, s, b.x);
+  p = fmaf(p, s, a.w);
+  p = fmaf(p, s, a.z);
+  p = fmaf(p, s, a.y);
+  return fmaf(p, s, a.x);
+}
 __device__ __forceinline__ float tap_poly(const TapPoly& tp, int i, float s) {
   float p = tp.c[i][kTapDeg];
 #pragma unroll
@@ -1363,6 +1377,12 @@ __device__ __forceinline__ float tap_poly(const TapPoly& tp, int i, float s) {
 template <typename R, int W>
 __device__ __forceinline__ void taps_axis(const TapPoly& tp, R f, R scale, R beta, R (&kt)[W]) {
   if constexpr (std::is_same<R, float>::value) {
+    if (tp.on_all) {
+      const float s = fmaf(2.0f, f, (float)(W - 1));
+#pragma unroll
+      for (int i = 0; i < W; ++i) kt[i] = tap_poly(tp, i, s);
+      return;
+    }
     if (tp.on) {
       const float s = fmaf(2.0f, f, (float)(W - 1));
       kt[0] = esk_fast(f * scale, beta);
@@ -1378,7 +1398,11 @@ __device__ __forceinline__ void taps_axis(const TapPoly& tp, R f, R scale, R bet
 // taps i < 8 of one axis for a run-time support W <= 8 (zero for i >= W or a dead lane)
 __device__ __forceinline__ void taps8(const TapPoly& tp, int W, float f, float scale, float beta, bool live,
                                       float (&kt)[8]) {
-  if (tp.on) {
+  if (tp.on_all) {
+    const float s = fmaf(2.0f, f, (float)(W - 1));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) kt[i] = i < W ? tap_poly(tp, i, s) : 0.f;
+  } else if (tp.on) {
     const float s = fmaf(2.0f, f, (float)(W - 1));
     const float e0 = esk_fast(f * scale, beta), e1 = esk_fast((f + (float)(W - 1)) * scale, beta);
     kt[0] = e0;
@@ -1561,7 +1585,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
                                                   const uint32_t* __restrict__ task_tab, uint32_t chunk,
                                                   uint32_t* __restrict__ counters,
                                                   cplx<R>* __restrict__ plane,
-                                                  const uint32_t* __restrict__ split, Strips sr) {
+                                                  const uint32_t* __restrict__ split, Strips sr, TapPoly tp) {
   constexpr int NS = T / 8;               // strips per tile
   static_assert((RG == 8 || RG == 16) && T % RG == 0, "row groups of 8 or 16 tile the strip");
   constexpr int NT = grid8_taps(PC);      // per staged visibility: 8 x taps, 8 y taps, PC w taps
@@ -1581,6 +1605,22 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
   const R scale = (R)2 / (R)W;
   // the FP32 tap evaluation works in base 2
   const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
+  // FP32 with every tap fitted (on_all): x / y taps from the polynomials, whose coefficient
+  // rows -8 .. W + 7 sit in LDS as two float4 each, zero outside the support, so a staging lane
+  // reads the row of its own tap index (always in -7 .. W + 6 for an accepted candidate)
+  constexpr bool kF32 = std::is_same<R, float>::value;
+  constexpr int kTapRows = kWMax + 16;
+  __shared__ float4 s_tap[kF32 ? 2 * kTapRows : 1];
+  const bool polytaps = kF32 && tp.on_all;
+  if (polytaps) {
+    for (int r = threadIdx.x; r < 2 * kTapRows; r += kBlock) {
+      const int row = r / 2 - 8, c0 = 4 * (r % 2);
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (row >= 0 && row < W) v = make_float4(tp.c[row][c0], tp.c[row][c0 + 1], tp.c[row][c0 + 2], tp.c[row][c0 + 3]);
+      s_tap[kF32 ? r : 0] = v;
+    }
+    __syncthreads();
+  }
   const uint32_t total = counters[1];
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
@@ -1671,11 +1711,21 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
                             ? (k.wstack ? esk_fast((f.w + (R)iw) * scale, beta) : (R)1)
                             : (R)0;
           }
+          if (polytaps) {
+            const float sxv = fmaf(2.0f, (float)f.x, (float)(W - 1)), syv = fmaf(2.0f, (float)f.y, (float)(W - 1));
 #pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            const int rx = sx0 + i - dx, ry = ry0 + i - dy;
-            t[i] = (unsigned)rx < (unsigned)W ? esk_fast((f.x + (R)rx) * scale, beta) : (R)0;
-            t[8 + i] = (unsigned)ry < (unsigned)W ? esk_fast((f.y + (R)ry) * scale, beta) : (R)0;
+            for (int i = 0; i < 8; ++i) {
+              const int rx = sx0 + i - dx, ry = ry0 + i - dy;
+              t[i] = (R)poly_row(s_tap + 2 * (rx + 8), sxv);
+              t[8 + i] = (R)poly_row(s_tap + 2 * (ry + 8), syv);
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+              const int rx = sx0 + i - dx, ry = ry0 + i - dy;
+              t[i] = (unsigned)rx < (unsigned)W ? esk_fast((f.x + (R)rx) * scale, beta) : (R)0;
+              t[8 + i] = (unsigned)ry < (unsigned)W ? esk_fast((f.y + (R)ry) * scale, beta) : (R)0;
+            }
           }
         }
         wave_sync();
@@ -3843,9 +3893,10 @@ void channel_scales(const hipng_plan& P, const double* freq, int nchan, double* 
 
 // Degree-kTapDeg polynomials of the interior kernel taps (TapPoly): Chebyshev interpolation
 // of tap i at kTapDeg + 1 Chebyshev points of s in [-1, 1] in FP64, converted to monomials in
-// s and rounded to FP32. on = 1 when the fit's largest deviation from the kernel, sampled
-// over every interior tap's interval, is at the FP32 evaluation level (1.2e-7) or below 5 %
-// of the kernel's own 1-D error at this oversampling and support.
+// s and rounded to FP32, for every tap. on = 1 when the fit's largest deviation from the
+// kernel, sampled over every interior tap's interval, is at the FP32 evaluation level
+// (1.2e-7) or below 5 % of the kernel's own 1-D error at this oversampling and support; on_all
+// when the two edge taps' deviation is also below 2e-7 or 10 % of that error.
 TapPoly make_tap_poly(int W, double beta, double sigma) {
   TapPoly tp{};
   if (W < 4 || W > kWMax) return tp;
@@ -3858,8 +3909,9 @@ TapPoly make_tap_poly(int W, double beta, double sigma) {
   };
   auto zof = [&](int i, double sv) { return ((sv + 1.0) * 0.5 - 0.5 * W + i) * 2.0 / W; };
   constexpr int N = kTapDeg + 1;
-  double worst = 0.0;
-  for (int i = 1; i < W - 1; ++i) {
+  double worst = 0.0, worst_edge = 0.0;
+  for (int i = 0; i < W; ++i) {
+    const bool edge = i == 0 || i == W - 1;
     double y[N], a[N], mono[N] = {0}, tm[N] = {0}, t0[N] = {0}, t1[N] = {0};
     for (int q = 0; q < N; ++q) y[q] = psi(zof(i, std::cos(kPi * (q + 0.5) / N)));
     for (int j = 0; j < N; ++j) {
@@ -3885,10 +3937,15 @@ TapPoly make_tap_poly(int W, double beta, double sigma) {
       const double sv = -1.0 + m / 256.0;
       double pv = tp.c[i][kTapDeg];
       for (int d = kTapDeg - 1; d >= 0; --d) pv = pv * sv + tp.c[i][d];
-      worst = std::max(worst, std::fabs(pv - psi(zof(i, sv))));
+      const double e = std::fabs(pv - psi(zof(i, sv)));
+      if (edge) worst_edge = std::max(worst_edge, e);
+      else worst = std::max(worst, e);
     }
   }
   tp.on = worst <= std::max(1.2e-7, 0.05 * err1d) ? 1 : 0;
+  // the edge taps: an absolute error of 2e-7 (the FP32 level of the kernel's peak) or 10 % of
+  // the kernel's own 1-D error
+  tp.on_all = tp.on && worst_edge <= std::max(2e-7, 0.1 * err1d) ? 1 : 0;
   return tp;
 }
 
@@ -4750,7 +4807,7 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   k_grid8<R, TT, QQ, BB, RG><<<g.npersist, kBlock, smw, st>>>(k, p0, npl, gb0, gb1, g.srec, g.rec_a,  \
                                                               (const rec3<R>*)g.rec_f, recv, g.task, \
                                                               g.tasktab, g.chunk_grid, g.counters, plane, \
-                                                              g.split, g.strips)
+                                                              g.split, g.strips, g.taps)
 #define HIPNG_GWB(TT, QQ)                 \
   if (g.nb == 1) HIPNG_GW(TT, QQ, 1);     \
   else if (g.nb == 2) HIPNG_GW(TT, QQ, 2); \
