@@ -145,8 +145,15 @@ constexpr int kQuadHalf = 64;  // 128-node Gauss-Legendre rule for the kernel's 
 struct KernelFT {
   std::vector<double> z, c;
   void init(int W, double beta) {
-    std::vector<double> x, w;
-    gauss_legendre_half(2 * kQuadHalf, x, w);
+    // the rule depends only on its size: computed once (the Newton iterations cost ~0.2 ms,
+    // and planning evaluates the transform for every candidate support and oversampling)
+    static const std::pair<std::vector<double>, std::vector<double>> rule = [] {
+      std::pair<std::vector<double>, std::vector<double>> r;
+      gauss_legendre_half(2 * kQuadHalf, r.first, r.second);
+      return r;
+    }();
+    const std::vector<double>& x = rule.first;
+    const std::vector<double>& w = rule.second;
     z = x;
     c.resize(x.size());
     for (size_t q = 0; q < x.size(); ++q)
