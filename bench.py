@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--sigma", type=float, default=None,
                     help="pin the grid oversampling factor through the plan options (cost-model "
                          "calibration, not the bench line)")
+    ap.add_argument("--tile", type=int, default=None, choices=[16, 32],
+                    help="pin the uv tile (study option; the bench line uses the automatic plan)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy 4 times (counter calibration for rocprof --pmc)")
     ap.add_argument("--study", default=None, choices=["precision", "autograd", "batched", "weighting", "gains",
@@ -1091,7 +1093,7 @@ def main():
     prec = args.precision if args.config == "c1" else "fp64"
     nrow, nchan, npol, N = wl["uvw"].shape[0], len(wl["freq"]), wl["npol"], wl["npix"]
     nvis = nrow * nchan * npol
-    opts = {"sigma": args.sigma} if args.sigma else None
+    opts = {k: v for k, v in (("sigma", args.sigma), ("tile", args.tile)) if v} or None
     print(f"[bench] {wl['name']}: {nrow} rows x {nchan} ch x {npol} pol = {nvis:.3e} vis, "
           f"{N}^2, precision {prec}", file=sys.stderr, flush=True)
     cdt = torch.complex128 if prec == "fp64" else torch.complex64
