@@ -218,12 +218,47 @@ double fp32_coord_error(double cells, double top) {
 // 4.7 at 1.75, W 6). Measured with FP32 planes: 1.0-1.2 eps_r D of extra relative L2 error on
 // noise-like images (60 snapshot plans, sigma 1.25-2.5) and ~3e-6 (D = 64) at the
 // 4.29e9-visibility 8192^2 plan; 1.25 eps_r D is used.
-double transform_floor(int W, double beta, int nx, int ny, int gx, int gy, double eps_r) {
+// With w-stacking the correction also divides by Phi_w(dw ((n-1) - nshift)), whose argument
+// reaches 1/(2 sigma) at the image centre and corners whatever the grid rounding (the x / y
+// arguments shrink when the grid is rounded up). The L2 error then follows the RMS over the
+// image of A = Phi(0)^3 / (Phi_x Phi_y Phi_w): measured 0.3-1.3 eps_r rms(A) on 100 snapshot
+// jobs (32^2-128^2, every oversampling), 1.3 where the w amplification is largest (sigma 1.25,
+// W 9: Phi_w(0) / Phi_w(edge) = 53); 1.5 eps_r rms(A) is used when it exceeds the term above.
+// rms(A) over a 24 x 24 midpoint grid of one image quadrant (A is even in x and y), Phi_w
+// interpolated linearly in l^2 + m^2 from 65 nodes.
+double transform_floor(int W, double beta, int nx, int ny, int gx, int gy, double eps_r, bool wstack = false,
+                       double px = 0.0, double py = 0.0, double dw = 0.0, double nshift = 0.0) {
   KernelFT ft;
   ft.init(W, beta);
   const double p0 = ft(0.0, W);
   const double dx = p0 / ft(0.5 * nx / gx, W), dy = p0 / ft(0.5 * ny / gy, W);
-  return 1.25 * eps_r * dx * dy;
+  double fl = 1.25 * eps_r * dx * dy;
+  if (wstack) {
+    constexpr int kS = 24, kR = 64;
+    const double xm = 0.5 * nx * px, ym = 0.5 * ny * py, r2max = xm * xm + ym * ym;
+    double aw[kR + 1];
+    for (int i = 0; i <= kR; ++i) {
+      const double r2 = r2max * i / kR;
+      const double nm1 = -r2 / (std::sqrt(1.0 - r2) + 1.0);
+      aw[i] = p0 / ft(dw * (nm1 - nshift), W);
+    }
+    double ax[kS], ay[kS];
+    for (int i = 0; i < kS; ++i) {
+      ax[i] = p0 / ft((i + 0.5) / kS * 0.5 * nx / gx, W);
+      ay[i] = p0 / ft((i + 0.5) / kS * 0.5 * ny / gy, W);
+    }
+    double s2 = 0.0;
+    for (int i = 0; i < kS; ++i)
+      for (int j = 0; j < kS; ++j) {
+        const double x = (i + 0.5) / kS * xm, y = (j + 0.5) / kS * ym;
+        const double t = (x * x + y * y) / r2max * kR;
+        const int k0 = std::min(kR - 1, (int)t);
+        const double a = ax[i] * ay[j] * (aw[k0] + (t - k0) * (aw[k0 + 1] - aw[k0]));
+        s2 += a * a;
+      }
+    fl = std::max(fl, 1.5 * eps_r * std::sqrt(s2 / (kS * kS)));
+  }
+  return fl;
 }
 
 void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int ny, double px,
@@ -309,8 +344,7 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
       const int cgx = good_grid(std::max(sigma * nx, 2.0 * t), t);
       const int cgy = good_grid(std::max(sigma * ny, 2.0 * t), t);
       const double beta = kBetaW[is][w - kWMin] * w;
-      double fl = transform_floor(w, beta, nx, ny, cgx, cgy,
-                                  prec == HIPNG_FP64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24));
+      const double eps_r = prec == HIPNG_FP64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
       int np = 1;
       double cdw = 1.0, cw0 = 0.0;
       if (wstack) {
@@ -320,6 +354,10 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
         if (!(span < 1e6)) continue;
         np = (int)span + w + 3;
       }
+      double fl = transform_floor(w, beta, nx, ny, cgx, cgy, eps_r);
+      // the w-stacking term costs ~100 kernel transforms: only for a support that passes without it
+      if (wstack && kErr1D[is][w - kWMin] * safety + fl <= eps)
+        fl = transform_floor(w, beta, nx, ny, cgx, cgy, eps_r, true, px, py, cdw, 0.5 * nm1min);
       if (prec == HIPNG_FP32) {
         const double ux = umax * px * cgx, vy = vmax * py * cgy;
         const double fx = 2.0 * kPi * fp32_coord_error(ux, 0.5 * cgx + ux + 0.5 * w) * (0.5 * nx) / cgx;
