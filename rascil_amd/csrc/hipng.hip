@@ -348,11 +348,15 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
       int np = 1;
       double cdw = 1.0, cw0 = 0.0;
       if (wstack) {
+        // planes: a visibility at wh = (w - w0) / dw starts at s = floor(wh - W/2) + 1. With
+        // w0 = wmin - (W/2 - 1/2) dw, s runs from floor(-1/2) + 1 = 0 at wmin to at most
+        // floor(span) + 1 at wmax (half a plane of margin for rounding at both ends), so
+        // span + W + 1 planes hold every footprint and none stays empty
         cdw = 0.5 / (sigma * nm1half);
-        cw0 = wmin - (0.5 * w + 1.0) * cdw;
+        cw0 = wmin - (0.5 * w - 0.5) * cdw;
         const double span = std::floor((wmax - wmin) / cdw);
         if (!(span < 1e6)) continue;
-        np = (int)span + w + 3;
+        np = (int)span + w + 1;
       }
       double fl = transform_floor(w, beta, nx, ny, cgx, cgy, eps_r);
       // the w-stacking term costs ~100 kernel transforms: only for a support that passes without it
