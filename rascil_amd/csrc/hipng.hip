@@ -461,12 +461,41 @@ struct KP {
   uint32_t ntiles;
   int W, Ww, nplanes, wstack;
   int nx, ny, nb, nchan;
+  int tshift;                  // log2 T (T is 16 or 32)
+  double rnchan;               // 1 / nchan, rounded (udiv32)
   double hgxd, hgyd, hw, mw0;  // G/2, W/2, -w0/dw
   double beta, px, py, nshift, dw, w0;
   // packed first cell ax | ay << 16 of a record: amask clears the dense-run flag dflag
   // (bit 31, set on records the dense-run gridder owns; both 0 / all ones when unused)
   uint32_t amask, dflag;
 };
+
+// q = i / d and *r = i - q d for i < 2^32 and 1 <= d < 2^31 without an integer divide:
+// i * rd (rd = 1/d rounded) lies within 2^-20 of i / d, so its truncation is off by at most
+// one and a single correction step either way gives the exact quotient
+__device__ __forceinline__ uint32_t udiv32(uint32_t i, uint32_t d, double rd, uint32_t* r) {
+  uint32_t q = (uint32_t)((double)i * rd);
+  int rem = (int)(i - q * d);
+  if (rem < 0) {
+    --q;
+    rem += (int)d;
+  } else if (rem >= (int)d) {
+    ++q;
+    rem -= (int)d;
+  }
+  *r = (uint32_t)rem;
+  return q;
+}
+
+// a mod g in [0, g): one add or subtract when |a| < 2 g (footprints wrap at most once unless
+// the image is undersampled several times over), the remainder otherwise
+__device__ __forceinline__ long long wrap_cell(long long a, int g) {
+  if (a >= 0 && a < g) return a;
+  if (a < 0 && a >= -(long long)g) return a + g;
+  if (a >= g && a < 2LL * g) return a - g;
+  const long long m = a % g;
+  return m < 0 ? m + g : m;
+}
 
 struct ItemPos {
   long long ax, ay;
@@ -525,18 +554,16 @@ __device__ __forceinline__ ItemPos item_pos(double u, double v, double w, double
       r.s = 0;
     }
   }
-  r.ax %= k.gx;
-  if (r.ax < 0) r.ax += k.gx;
-  r.ay %= k.gy;
-  if (r.ay < 0) r.ay += k.gy;
+  r.ax = wrap_cell(r.ax, k.gx);
+  r.ay = wrap_cell(r.ay, k.gy);
   if (r.s < 0) r.s = 0;
   if (r.s > k.nplanes - k.Ww) r.s = k.nplanes - k.Ww;
   return r;
 }
 
 __device__ __forceinline__ uint32_t item_key(const ItemPos& p, const KP& k) {
-  return (uint32_t)p.s * k.ntiles + (uint32_t)(p.ax / k.T) * (uint32_t)k.nty +
-         (uint32_t)(p.ay / k.T);
+  return (uint32_t)p.s * k.ntiles + ((uint32_t)p.ax >> k.tshift) * (uint32_t)k.nty +
+         ((uint32_t)p.ay >> k.tshift);
 }
 
 // ---------------------------------------------------------------------------------
@@ -579,7 +606,7 @@ __global__ void k_extent(const double* __restrict__ uvw, int64_t nrow,
 
 // in-tile origin digit (first cell mod tile along u, then v): the secondary sort key
 __device__ __forceinline__ uint32_t item_digit(const ItemPos& p, const KP& k) {
-  return (uint32_t)(p.ax % k.T) * (uint32_t)k.T + (uint32_t)(p.ay % k.T);
+  return (((uint32_t)p.ax & (uint32_t)(k.T - 1)) << k.tshift) + ((uint32_t)p.ay & (uint32_t)(k.T - 1));
 }
 
 // Sort words of item i. Unpacked (db = 0): digit and index. Packed (db > 0, the digit's bit
@@ -592,8 +619,8 @@ __global__ void k_digits(const double* __restrict__ uvw, int64_t nitems, const d
                          uint32_t* __restrict__ digits, uint32_t* __restrict__ vals, int db, int vb) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= nitems) return;
-  const int64_t row = i / k.nchan;
-  const int ch = (int)(i - row * k.nchan);
+  uint32_t ch;
+  const int64_t row = udiv32((uint32_t)i, (uint32_t)k.nchan, k.rnchan, &ch);
   const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch],
                                    sy[ch], sw[ch], k);
   const uint32_t dg = item_digit(p, k);
@@ -623,8 +650,8 @@ __global__ void k_rekey(const double* __restrict__ uvw, const uint32_t* __restri
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= nitems) return;
   const uint32_t i = perm[j];
-  const int64_t row = i / k.nchan;
-  const int ch = (int)(i - row * k.nchan);
+  uint32_t ch;
+  const int64_t row = udiv32(i, (uint32_t)k.nchan, k.rnchan, &ch);
   const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], uvw[3 * row + 2], sx[ch],
                                    sy[ch], sw[ch], k);
   keys[j] = item_key(p, k);
@@ -1069,8 +1096,8 @@ __global__ void k_records(const double* __restrict__ uvw, const uint32_t* __rest
   bool flip = false;
   if (j < n) {
     const uint32_t i = perm[j];
-    const int64_t row = i / k.nchan;
-    const int ch = (int)(i - row * k.nchan);
+    uint32_t ch;
+    const int64_t row = udiv32(i, (uint32_t)k.nchan, k.rnchan, &ch);
     const double w = uvw[3 * row + 2];
     const ItemPos p = item_pos<F32C>(uvw[3 * row], uvw[3 * row + 1], w, sx[ch], sy[ch], sw[ch], k);
     rec_a[j] = (uint32_t)p.ax | ((uint32_t)p.ay << 16);
@@ -1257,7 +1284,7 @@ __global__ void k_split_points(const uint32_t* __restrict__ keys, const uint2* _
       A = rec_a[r];
     }
     *key = keys[j];
-    *ax = (int)((A & amask) & 0xffffu) % T;
+    *ax = (int)((A & amask) & 0xffffu) & (T - 1);  // T is a power of two
   };
   uint32_t key, kp = 0xffffffffu;
   int a, ap = -1;
@@ -3447,6 +3474,7 @@ __global__ void k_apply_gains(cplx<double>* __restrict__ vis, double* __restrict
 // ---------------------------------------------------------------------------------
 struct ColArgs {
   int nchan_g, nchan_t, chan0;  // geometry channels = column channels chan0 .. chan0 + nchan_g - 1
+  double rnchan_g;              // 1 / nchan_g, rounded (udiv32)
   int vis_f32, wgt_f32, flag_i8;
   int shift;                    // phase exp(+-2 pi i f/c (u L + v M + w N)) on the geometry's uvw
   double L, M, N;
@@ -3496,8 +3524,9 @@ __global__ void k_values_cols(const uint32_t* __restrict__ perm, int64_t n, cons
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
-  const int64_t row = i / a.nchan_g;
-  const int chl = (int)(i - row * a.nchan_g), ch = a.chan0 + chl;
+  uint32_t chu;
+  const int64_t row = udiv32(i, (uint32_t)a.nchan_g, a.rnchan_g, &chu);
+  const int chl = (int)chu, ch = a.chan0 + chl;
   const int64_t ci = row * a.nchan_t + ch;
   double keep[4];
   cplx<double> x[4];
@@ -3546,8 +3575,9 @@ __global__ void k_unsort_cols(const uint32_t* __restrict__ perm, int64_t n, cons
   const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (j >= n) return;
   const uint32_t i = perm[j];
-  const int64_t row = i / a.nchan_g;
-  const int chl = (int)(i - row * a.nchan_g), ch = a.chan0 + chl;
+  uint32_t chu;
+  const int64_t row = udiv32(i, (uint32_t)a.nchan_g, a.rnchan_g, &chu);
+  const int chl = (int)chu, ch = a.chan0 + chl;
   const int64_t ci = row * a.nchan_t + ch;
   const bool flip = (flipb[j >> 5] >> (j & 31)) & 1u;
   const cplx<R> pr = rec_p[j];
@@ -4016,6 +4046,9 @@ KP make_kp(const hipng_plan& P, int nb, int nchan) {
   k.ny = P.ny;
   k.nb = nb;
   k.nchan = nchan;
+  k.rnchan = 1.0 / (double)std::max(nchan, 1);
+  k.tshift = 0;
+  while ((1 << k.tshift) < P.tile) ++k.tshift;
   k.hgxd = 0.5 * P.gx;
   k.hgyd = 0.5 * P.gy;
   k.hw = 0.5 * P.W;
@@ -5079,6 +5112,7 @@ ColCall col_call(const Geom& g, const hipng_columns* c, bool invert) {
   ColCall cc;
   std::memset(&cc.a, 0, sizeof(cc.a));
   cc.a.nchan_g = g.nchan;
+  cc.a.rnchan_g = 1.0 / (double)std::max(g.nchan, 1);
   cc.a.nchan_t = c->nchan;
   cc.a.chan0 = c->chan0;
   cc.a.vis_f32 = c->vis_type == HIPNG_COMPLEX64;
