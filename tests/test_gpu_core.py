@@ -45,6 +45,39 @@ def test_bucket_sort_bit_exact(small_obs, wst, prec, nchan):
     assert numpy.array_equal(perm, rp)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("px", [2e-3, 8e-3])
+def test_bucket_sort_bit_exact_wrapping_footprints(small_obs, prec, px):
+    """An undersampled 64^2 image (cell 2e-3 / 8e-3 rad against a longest baseline of ~900
+    wavelengths): grid coordinates leave the grid and wrap once (2e-3) or several times
+    (8e-3), the path where the cell index takes a remainder instead of one add or subtract."""
+    uvw, _ = small_obs
+    freq = numpy.array([1.0e8, 1.15e8, 1.3e8])
+    plan = _extent_plan(uvw, freq, 64, 64, px, px, 1e-5 if prec == "fp64" else 1e-3, True, prec)
+    assert plan.umax * px > (0.6 if px < 5e-3 else 3.0)  # how far footprints reach past the grid
+    keys, perm = wgridder.bucket_sort(uvw, freq, plan)
+    sx, sy, sw = wgridder.channel_scales(plan, freq)
+    rk, rp = sortref.bucket_sort(uvw, sx, sy, sw, plan.as_dict())
+    assert numpy.array_equal(keys, rk)
+    assert numpy.array_equal(perm, rp)
+
+
+def test_wrapping_footprints_match_explicit_sums(small_obs):
+    """The same several-times-wrapping 64^2 image through invert and predict: a periodic
+    grid makes the wrapped coordinates exact, so both meet epsilon against brute force."""
+    uvw, _ = small_obs
+    freq = numpy.array([1.0e8, 1.3e8])
+    ms, wgt = _case(uvw, freq, seed=3)
+    px, eps = 8e-3, 1e-9
+    d = wgridder.ms2dirty(uvw, freq, ms, wgt, 64, 64, px, px, 0, 0, eps, True)
+    ref = ox.explicit_ms2dirty(uvw, freq, ms, wgt, 64, 64, px, px, True)
+    assert rel_l2(d, ref) < 10 * eps
+    model = numpy.random.default_rng(4).normal(size=(64, 64))
+    v = wgridder.dirty2ms(uvw, freq, model, wgt, px, px, 0, 0, eps, True)
+    vref = ox.explicit_dirty2ms(uvw, freq, model, wgt, px, px, True)
+    assert rel_l2(v, vref) < 10 * eps
+
+
 @pytest.mark.parametrize("pack,tile", [(True, 16), (False, 16), (True, 32)])
 def test_bucket_sort_bit_exact_production_subsample(pack, tile):
     """1 % of the 512-station / 64-channel production geometry, 8192^2 image plan; with the
