@@ -52,6 +52,7 @@ namespace {
 constexpr double kSpeedOfLight = 299792458.0;  // rascil/phyconst.py:6
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kBlock = 256;
+constexpr int kXcd = 8;                      // XCDs per device: work queues of the sub-tile gridder
 constexpr int kMaxBatch = 4;
 constexpr size_t kLdsBudget = 65536;  // dynamic LDS per workgroup the tile kernels may use
 constexpr size_t kDegridLds = 40960;  // LDS of a multi-plane degridder workgroup (4 per CU)
@@ -676,6 +677,16 @@ __device__ __forceinline__ void wave_sync() {
 // adds the group size to the wave's own LDS histogram (no atomics).
 // BITS-bit digits (8 or 9) from bit `shift`; db > 0: the input is packed (k_digits) and
 // the pass sorts by the unpacked bucket key
+// Logical block of a radix-sort pass: consecutive logical blocks run on one XCD (workgroups
+// are placed on XCDs round-robin by index), so that the adjacent digit runs neighbouring
+// blocks write go through one L2 and leave it as whole lines. The grid has kXcd * per
+// workgroups; the few past nblocks return at once.
+__device__ __forceinline__ int64_t rs_block(int64_t nblocks) {
+  const int64_t per = (nblocks + kXcd - 1) / kXcd;
+  return (int64_t)(blockIdx.x % kXcd) * per + (int64_t)(blockIdx.x / kXcd);
+}
+inline unsigned rs_grid(int64_t nblocks) { return (unsigned)(kXcd * ((nblocks + kXcd - 1) / kXcd)); }
+
 template <int BITS>
 __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ vals, int db, int vb, int64_t n,
@@ -685,12 +696,14 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__
   constexpr int ND = 1 << BITS, DPT = ND / kBlock;  // digits, digits per thread
   static_assert(DPT >= 1 && ND % kBlock == 0, "digits per thread");
   __shared__ uint32_t h[NW][ND];
+  const int64_t lb = rs_block(nblocks);
+  if (lb >= nblocks) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int w = 0; w < NW; ++w)
 #pragma unroll
     for (int e = 0; e < DPT; ++e) h[w][threadIdx.x + e * kBlock] = 0;
   __syncthreads();
-  const int64_t wbase = (int64_t)blockIdx.x * kRsItems + (int64_t)wv * (kRsItems / NW);
+  const int64_t wbase = lb * kRsItems + (int64_t)wv * (kRsItems / NW);
   uint32_t key[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -724,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__
     uint32_t tot = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) tot += h[w][dd];
-    hist[(int64_t)dd * nblocks + blockIdx.x] = tot;
+    hist[(int64_t)dd * nblocks + lb] = tot;
   }
 }
 
@@ -852,9 +865,11 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
   __shared__ uint32_t wcnt[NW][ND];
   __shared__ uint32_t lbase[ND];
   __shared__ uint32_t gbase[ND];
+  const int64_t lb = rs_block(nblocks);
+  if (lb >= nblocks) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t base = (int64_t)blockIdx.x * kRsItems;
+  const int64_t base = lb * kRsItems;
   const int64_t wbase = base + (int64_t)wv * (kRsItems / NW);
   for (int w = 0; w < NW; ++w)
 #pragma unroll
@@ -910,7 +925,7 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
     for (int e = 0; e < DPT; ++e) {
       const int d = DPT * threadIdx.x + e;
       lbase[d] = ex;
-      gbase[d] = hist[(int64_t)d * nblocks + blockIdx.x];
+      gbase[d] = hist[(int64_t)d * nblocks + lb];
       uint32_t run = ex;
       for (int w = 0; w < NW; ++w) {
         const uint32_t c = wcnt[w][d];
@@ -1359,7 +1374,6 @@ constexpr int kMaxPC = 2;                    // w planes gridded per pass
 // 35.7 / 30.5 ms for groups of 1 / 2 / 4; predict fills 33.0 / 27.9 / 30.1 ms.
 constexpr int kScreenGroup = 4;
 constexpr int kFillGroup = 2;
-constexpr int kXcd = 8;                      // XCDs per device: work queues of the sub-tile gridder
 constexpr int kQueueBase = 8;                // counters[kQueueBase + x]: pops from queue x
 constexpr int kMaxSeg = 4 * (16 + kMaxPC - 1);  // 4 neighbour buckets x start planes
 constexpr int kDenseCnt = kQueueBase + kXcd;     // counters[kDenseCnt]: dense-run chunk pops
@@ -4163,9 +4177,9 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
       const int64_t m = ((int64_t)1 << wbits) * nblocks;
       const int64_t npart = (m + kRsItems - 1) / kRsItems;
       if (wbits == 9)
-        k_rs_hist<9><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
+        k_rs_hist<9><<<rs_grid(nblocks), kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
       else
-        k_rs_hist<8><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
+        k_rs_hist<8><<<rs_grid(nblocks), kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
       LAUNCH_CHECK();
       k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
       LAUNCH_CHECK();
@@ -4174,9 +4188,9 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
       k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
       LAUNCH_CHECK();
       if (wbits == 9)
-        k_rs_scatter<9><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
+        k_rs_scatter<9><<<rs_grid(nblocks), kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
       else
-        k_rs_scatter<8><<<(unsigned)nblocks, kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
+        k_rs_scatter<8><<<rs_grid(nblocks), kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
       LAUNCH_CHECK();
       std::swap(k0, k1);
       std::swap(v0, v1);
