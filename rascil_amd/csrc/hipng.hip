@@ -1374,12 +1374,24 @@ constexpr int kMaxPC = 2;                    // w planes gridded per pass
 // 35.7 / 30.5 ms for groups of 1 / 2 / 4; predict fills 33.0 / 27.9 / 30.1 ms.
 constexpr int kScreenGroup = 4;
 constexpr int kFillGroup = 2;
-constexpr int kQueueBase = 8;                // counters[kQueueBase + x]: pops from queue x
+// Work-queue counters, each on a 128-byte line of its own (see queue_pop):
+// counters[kQueueBase + x kQueueStride]: pops from XCD queue x
+constexpr int kQueueStride = 32;
+constexpr int kQueueBase = 32;
 constexpr int kMaxSeg = 4 * (16 + kMaxPC - 1);  // 4 neighbour buckets x start planes
-constexpr int kDenseCnt = kQueueBase + kXcd;     // counters[kDenseCnt]: dense-run chunk pops
-constexpr int kDenseTotal = kDenseCnt + 1;       // counters[kDenseTotal]: dense-run chunks
-constexpr int kRuns = kDenseCnt + 2;             // counters[kRuns]: runs (geometry build)
-constexpr int kNCounters = kDenseCnt + 3;
+constexpr int kDenseCnt = kQueueBase + kXcd * kQueueStride;  // counters[kDenseCnt]: dense-run chunk pops
+constexpr int kDenseTotal = kDenseCnt + kQueueStride;        // counters[kDenseTotal]: dense-run chunks
+constexpr int kRuns = kDenseTotal + 1;                       // counters[kRuns]: runs (geometry build)
+constexpr int kNCounters = kRuns + 1;
+
+// Pop from a work-queue counter: the next index, or `lim` once it is exhausted. Atomics on
+// one 128-byte line are served one after another, so each counter has a line of its own: with
+// the eight XCD queues and the dense-run counter on one line, the pops of a launch (and the
+// eight failed pops every wave makes when the queues run dry) cost ~0.4 ms per launch
+// whatever the work, and the production-shape gatherer 16 % of its time. A relaxed load
+// before the atomic to skip exhausted queues was slower still (loads of a line under atomic
+// traffic wait for it).
+__device__ __forceinline__ uint32_t queue_pop(uint32_t* c, uint32_t lim) { return min(atomicAdd(c, 1u), lim); }
 // Shortest run the dense path takes: a dense task pays a flush of 64 x PC x NB x 2 global
 // atomic adds (gridder) or a window load (degridder) per pass, which for short runs costs
 // more than the sub-tile gatherer and the tile degridder spend on them (production shape,
@@ -1729,7 +1741,7 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) vo
       const int x = (xcd0 + qi) % kXcd;
       const uint32_t g0 = min(ngroups, (uint32_t)x * gper), qsize = min(ngroups - g0, gper);
       uint32_t l = 0;
-      if (lane == 0) l = atomicAdd(&counters[kQueueBase + x], 1u);
+      if (lane == 0) l = queue_pop(&counters[kQueueBase + x * kQueueStride], qsize * NS);
       l = __shfl(l, 0, 64);
       if (l < qsize * NS) {
         task = (g0 + l / NS) * NS + l % NS;
@@ -2173,7 +2185,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense(KP k, int p0, int npl, co
   const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
   auto pop = [&]() {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
+    if (lane == 0) t = queue_pop(&counters[kDenseCnt], hi - lo);
     return lo + __shfl(t, 0, 64);
   };
   uint32_t t = pop();
@@ -2343,7 +2355,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
   const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
   auto pop = [&]() {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
+    if (lane == 0) t = queue_pop(&counters[kDenseCnt], hi - lo);
     return lo + __shfl(t, 0, 64);
   };
   auto task_at = [&](uint32_t i) {  // wave-uniform task fields
@@ -2776,7 +2788,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
       while (qi < kXcd) {
         const int x = (xcd0 + qi) % kXcd;
         const uint32_t t0 = min(total, (uint32_t)x * tper), qsize = min(total - t0, tper);
-        const uint32_t l = atomicAdd(&counters[kQueueBase + x], 1u);
+        const uint32_t l = queue_pop(&counters[kQueueBase + x * kQueueStride], qsize);
         if (l < qsize) {
           t = t0 + l;
           break;
@@ -2974,7 +2986,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
   const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
   for (;;) {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(&counters[kDenseCnt], 1u);
+    if (lane == 0) t = queue_pop(&counters[kDenseCnt], hi - lo);
     t = lo + __shfl(t, 0, 64);
     if (t >= hi) break;
     const uint4 d = dtask[t];  // first record, count, packed first cell, start plane
