@@ -2487,6 +2487,15 @@ __device__ __forceinline__ double nm1s_at(int tx, int ty, const KP& k, double* n
   }
 }
 
+// Fourier transform of the W-point kernel at xi (cycles per W cells): sum_q c_q cos(pi W z_q xi)
+// over the quadrature nodes z_q in [0, 1) with weights c_q that fold in the kernel's values
+__device__ __forceinline__ double kernel_ft(double xi, int W, const double* __restrict__ qz,
+                                            const double* __restrict__ qc, int nq) {
+  double s = 0.0;
+  for (int q = 0; q < nq; ++q) s += qc[q] * cospi((double)W * qz[q] * xi);
+  return s;
+}
+
 // invert: acc[b][ix][iy] += Re(exp(-2 pi i w_p nm1s) (-1)^(tx+ty) plane[b][tx mod G][ty mod G])
 // 1 / (Phi_x Phi_y [Phi_w(dw nm1s) n]) at pixel (ix, iy)
 template <typename A>
@@ -2497,11 +2506,20 @@ __device__ __forceinline__ A corr_at(int ix, int iy, const KP& k, const double* 
   if (k.wstack) {
     double n;
     const double xi = k.dw * nm1s_at<false>(ix - k.nx / 2, iy - k.ny / 2, k, &n);
-    double ph = 0.0;
-    for (int q = 0; q < nq; ++q) ph += qc[q] * cospi((double)k.W * qz[q] * xi);
-    den *= ph * n;
+    den *= kernel_ft(xi, k.W, qz, qc, nq) * n;
   }
   return (A)(1.0 / den);
+}
+
+// phix[i] = Phi((i - nx/2) / gx) for i < nx, then phiy likewise: one thread per pixel column
+// or row (evaluating the nx + ny quadrature sums on the host cost ~8 ms per call at 8192^2)
+__global__ void k_phi_tables(double* __restrict__ phix, int nx, int gx, double* __restrict__ phiy, int ny, int gy,
+                             int W, const double* __restrict__ qz, const double* __restrict__ qc, int nq) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nx)
+    phix[i] = kernel_ft((double)(i - nx / 2) / gx, W, qz, qc, nq);
+  else if (i < nx + ny)
+    phiy[i - nx] = kernel_ft((double)(i - nx - ny / 2) / gy, W, qz, qc, nq);
 }
 
 // ---------------------------------------------------------------------------------
@@ -4126,11 +4144,8 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
   std::vector<double> host(n, 0.0);
   double* h = host.data();
   channel_scales(P, freq, nchan, h, h + nchan, h + 2 * nchan, h + 3 * nchan);
-  double* hx = h + 4 * nchan;
-  double* hy = hx + P.nx;
-  for (int i = 0; i < P.nx; ++i) hx[i] = ft((double)(i - P.nx / 2) / P.gx, P.W);
-  for (int i = 0; i < P.ny; ++i) hy[i] = ft((double)(i - P.ny / 2) / P.gy, P.W);
-  double* hz = hy + P.ny;
+  // phix, phiy are filled on the device below
+  double* hz = h + 4 * nchan + P.nx + P.ny;
   double* hc = hz + nq;
   for (int q = 0; q < nq; ++q) {
     hz[q] = ft.z[q];
@@ -4152,6 +4167,10 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
   t.qc = t.qz + nq;
   t.fc = t.qc + nq;
   t.nq = nq;
+  const int nphi = P.nx + P.ny;
+  k_phi_tables<<<(nphi + kBlock - 1) / kBlock, kBlock, 0, st>>>(t.phix, P.nx, P.gx, t.phiy, P.ny, P.gy, P.W, t.qz,
+                                                               t.qc, nq);
+  LAUNCH_CHECK();
   return t;
 }
 
