@@ -262,9 +262,9 @@ double transform_floor(int W, double beta, int nx, int ny, int gx, int gy, doubl
   return fl;
 }
 
-void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int ny, double px,
-               double py, double eps, int wstack, int prec, int nbatch, int64_t nitems,
-               const hipng_plan_opts* opts, hipng_plan* P) {
+void make_plan_uncached(double umax, double vmax, double wmin, double wmax, int nx, int ny, double px,
+                        double py, double eps, int wstack, int prec, int nbatch, int64_t nitems,
+                        const hipng_plan_opts* opts, hipng_plan* P) {
   if (nx <= 0 || ny <= 0 || (nx & 1) || (ny & 1))
     HIPNG_THROW(HIPNG_EINVAL, "npix_x and npix_y must be positive and even (got %d, %d)", nx, ny);
   if (!(px > 0) || !(py > 0)) HIPNG_THROW(HIPNG_EINVAL, "pixel sizes must be positive");
@@ -427,6 +427,41 @@ void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int n
   const uint64_t nkeys = (uint64_t)P->nplanes * P->ntx * P->nty;
   P->keybits = bits_for(nkeys - 1);
   if (P->keybits == 0) P->keybits = 1;
+}
+
+// The plan is a pure function of its arguments, and choosing it evaluates the kernel's Fourier
+// transform for every candidate support and oversampling (0.5-2 ms on the host while the GPU
+// waits, per one-shot call). The last few choices are kept; rejected arguments are not.
+struct PlanArgs {
+  double d[9];  // umax, vmax, wmin, wmax, px, py, eps, nitems, has_opts
+  int32_t i[6];  // nx, ny, wstack, prec, nbatch, 0
+  hipng_plan_opts o;
+};
+void make_plan(double umax, double vmax, double wmin, double wmax, int nx, int ny, double px,
+               double py, double eps, int wstack, int prec, int nbatch, int64_t nitems,
+               const hipng_plan_opts* opts, hipng_plan* P) {
+  constexpr size_t kKeep = 16;
+  static std::mutex mu;
+  static std::vector<std::pair<PlanArgs, hipng_plan>> kept;
+  PlanArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const double dv[9] = {umax, vmax, wmin, wmax, px, py, eps, (double)nitems, opts ? 1.0 : 0.0};
+  std::memcpy(a.d, dv, sizeof(dv));
+  const int32_t iv[6] = {nx, ny, wstack, prec, nbatch, 0};
+  std::memcpy(a.i, iv, sizeof(iv));
+  if (opts) std::memcpy(&a.o, opts, sizeof(a.o));
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (const auto& e : kept)
+      if (std::memcmp(&e.first, &a, sizeof(a)) == 0) {
+        *P = e.second;
+        return;
+      }
+  }
+  make_plan_uncached(umax, vmax, wmin, wmax, nx, ny, px, py, eps, wstack, prec, nbatch, nitems, opts, P);
+  std::lock_guard<std::mutex> lk(mu);
+  if (kept.size() >= kKeep) kept.erase(kept.begin());
+  kept.emplace_back(a, *P);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1161,6 +1196,27 @@ __device__ __forceinline__ void store_vals(cplx<R>* __restrict__ p, const cplx<R
   }
 }
 
+// NB weights of one item: one 16-byte (NB 4) or 8-byte (NB 2) access when VEC (the caller
+// checked the alignment). The item's weights sit at a random place, and one access per item
+// instead of NB is what the gathers are bound by (k_values at C1: 42.5 -> 32.0 ms).
+template <typename R, int NB, bool VEC>
+__device__ __forceinline__ void load_wgts(const R* __restrict__ p, R (&g)[NB]) {
+  if constexpr (VEC && std::is_same<R, float>::value && NB == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    g[0] = t.x;
+    g[1] = t.y;
+    g[2] = t.z;
+    g[3] = t.w;
+  } else if constexpr (VEC && std::is_same<R, float>::value && NB == 2) {
+    const float2 t = *reinterpret_cast<const float2*>(p);
+    g[0] = t.x;
+    g[1] = t.y;
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) g[b] = p[b];
+  }
+}
+
 // invert prologue: rec_v[j] = conj?(wgt ms[perm[j]]) exp(-2 pi i |w| nshift f/c)
 template <typename R, int NB, bool VEC>
 __global__ void k_values(const uint32_t* __restrict__ perm, int64_t n, const cplx<R>* __restrict__ rec_p,
@@ -1174,13 +1230,14 @@ __global__ void k_values(const uint32_t* __restrict__ perm, int64_t n, const cpl
   const R c = ph.re, s = -ph.im;
   cplx<R> v[NB], o[NB];
   load_vals<R, NB, VEC>(ms + (int64_t)i * NB, v);
+  R gw[NB];
+  if (wgt) load_wgts<R, NB, VEC>(wgt + (int64_t)i * NB, gw);
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     if (flip) v[b].im = -v[b].im;
     if (wgt) {
-      const R g = wgt[(int64_t)i * NB + b];
-      v[b].re *= g;
-      v[b].im *= g;
+      v[b].re *= gw[b];
+      v[b].im *= gw[b];
     }
     o[b].re = v[b].re * c - v[b].im * s;
     o[b].im = v[b].re * s + v[b].im * c;
@@ -3104,15 +3161,16 @@ __global__ void k_unsort(const uint32_t* __restrict__ perm, int64_t n, const cpl
   const R c = ph.re, s = ph.im;
   cplx<R> a[NB], o[NB];
   load_vals<R, NB, true>(accs + j * NB, a);
+  R gw[NB];
+  if (wgt) load_wgts<R, NB, VEC>(wgt + (int64_t)i * NB, gw);
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     o[b].re = a[b].re * c - a[b].im * s;
     o[b].im = a[b].re * s + a[b].im * c;
     if (flip) o[b].im = -o[b].im;
     if (wgt) {
-      const R g = wgt[(int64_t)i * NB + b];
-      o[b].re *= g;
-      o[b].im *= g;
+      o[b].re *= gw[b];
+      o[b].im *= gw[b];
     }
   }
   store_vals<R, NB, VEC>(ms + (int64_t)i * NB, o);
@@ -3524,6 +3582,7 @@ struct ColArgs {
   double L, M, N;
   int nant;                     // gains [nant][nchan_t] complex128 with ant1 / ant2 [row], or none
   int psf;                      // invert: image polarisation 0 gets 1, the others 0 (weights kept)
+  int vec;                      // bits: vis (1), weights (2), flags (4) aligned for one access per item
   double cre[16], cim[16];      // conversion [nout][nin]
 };
 
@@ -3546,6 +3605,82 @@ __device__ __forceinline__ double col_keep(const void* f, int64_t i, int i8) {
   if (!f) return 1.0;
   return i8 ? 1.0 - (double)reinterpret_cast<const int8_t*>(f)[i] : 1.0 - (double)reinterpret_cast<const int*>(f)[i];
 }
+// The NP polarisations of one item of a column (vis, weights, flags): one or two 16-byte
+// accesses (or one 4-byte access for four int8 flags) when `vec` says the column is aligned for
+// it. Items sit at random places in the columns, and the gathers are bound by accesses.
+template <int NP>
+__device__ __forceinline__ void col_load_item(const void* p, int64_t i, int f32, bool vec, cplx<double> (&v)[NP]) {
+  if (vec && f32 && NP % 2 == 0) {
+    const float4* q = reinterpret_cast<const float4*>(p) + i * (NP / 2);
+#pragma unroll
+    for (int h = 0; h < NP / 2; ++h) {
+      const float4 t = q[h];
+      v[2 * h] = cplx<double>{t.x, t.y};
+      v[2 * h + 1] = cplx<double>{t.z, t.w};
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) v[k] = col_load(p, i * NP + k, f32);
+  }
+}
+template <int NP>
+__device__ __forceinline__ void col_store_item(void* p, int64_t i, int f32, bool vec, const cplx<double> (&v)[NP]) {
+  if (vec && f32 && NP % 2 == 0) {
+    float4* q = reinterpret_cast<float4*>(p) + i * (NP / 2);
+#pragma unroll
+    for (int h = 0; h < NP / 2; ++h)
+      q[h] = make_float4((float)v[2 * h].re, (float)v[2 * h].im, (float)v[2 * h + 1].re, (float)v[2 * h + 1].im);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) col_store(p, i * NP + k, f32, v[k]);
+  }
+}
+template <int NP>
+__device__ __forceinline__ void col_real_item(const void* p, int64_t i, int f32, bool vec, double (&w)[NP]) {
+  if (vec && f32 && NP == 4) {
+    const float4 t = reinterpret_cast<const float4*>(p)[i];
+    w[0] = t.x;
+    w[1] = t.y;
+    w[2] = t.z;
+    w[3] = t.w;
+  } else if (vec && f32 && NP == 2) {
+    const float2 t = reinterpret_cast<const float2*>(p)[i];
+    w[0] = t.x;
+    w[1] = t.y;
+  } else if (vec && !f32 && NP % 2 == 0) {
+    const double2* q = reinterpret_cast<const double2*>(p) + i * (NP / 2);
+#pragma unroll
+    for (int h = 0; h < NP / 2; ++h) {
+      const double2 t = q[h];
+      w[2 * h] = t.x;
+      w[2 * h + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NP; ++k) w[k] = col_real(p, i * NP + k, f32);
+  }
+}
+template <int NP>
+__device__ __forceinline__ void col_keep_item(const void* f, int64_t i, int i8, bool vec, double (&k)[NP]) {
+  if (!f) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) k[p] = 1.0;
+  } else if (vec && i8 && NP == 4) {
+    const uint32_t t = reinterpret_cast<const uint32_t*>(f)[i];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) k[p] = 1.0 - (double)(int8_t)(t >> (8 * p));
+  } else if (vec && !i8 && NP == 4) {
+    const int4 t = reinterpret_cast<const int4*>(f)[i];
+    k[0] = 1.0 - (double)t.x;
+    k[1] = 1.0 - (double)t.y;
+    k[2] = 1.0 - (double)t.z;
+    k[3] = 1.0 - (double)t.w;
+  } else {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) k[p] = col_keep(f, i * NP + p, i8);
+  }
+}
+
 // the product of the row's antenna gains g1 conj(g2) (or its conjugate)
 __device__ __forceinline__ cplx<double> row_gain(const cplx<double>* __restrict__ gains, const int* __restrict__ ant1,
                                                  const int* __restrict__ ant2, int64_t row, int ch, int nchan_t,
@@ -3572,14 +3707,13 @@ __global__ void k_values_cols(const uint32_t* __restrict__ perm, int64_t n, cons
   const int64_t row = udiv32(i, (uint32_t)a.nchan_g, a.rnchan_g, &chu);
   const int chl = (int)chu, ch = a.chan0 + chl;
   const int64_t ci = row * a.nchan_t + ch;
-  double keep[4];
-  cplx<double> x[4];
+  double keep[NI], wv[NI];
+  cplx<double> x[NI];
+  col_keep_item<NI>(flags, ci, a.flag_i8, (a.vec & 4) != 0, keep);
+  col_load_item<NI>(vis, ci, a.vis_f32, (a.vec & 1) != 0, x);
+  if (wgt) col_real_item<NI>(wgt, ci, a.wgt_f32, (a.vec & 2) != 0, wv);
 #pragma unroll
-  for (int p = 0; p < NI; ++p) {
-    keep[p] = col_keep(flags, ci * NI + p, a.flag_i8);
-    const cplx<double> v = col_load(vis, ci * NI + p, a.vis_f32);
-    x[p] = cplx<double>{v.re * keep[p], v.im * keep[p]};
-  }
+  for (int p = 0; p < NI; ++p) x[p] = cplx<double>{x[p].re * keep[p], x[p].im * keep[p]};
   cplx<double> m{1.0, 0.0};
   if (a.shift) {
     double sn, cs;
@@ -3600,7 +3734,7 @@ __global__ void k_values_cols(const uint32_t* __restrict__ perm, int64_t n, cons
     y = cmul(y, m);
     if (a.psf) y = cplx<double>{q == 0 ? 1.0 : 0.0, 0.0};
     if (flip) y.im = -y.im;
-    const double w = (wgt ? col_real(wgt, ci * NI + q, a.wgt_f32) : 1.0) * keep[q];
+    const double w = (wgt ? wv[q] : 1.0) * keep[q];
     y.re *= w;
     y.im *= w;
     o[q] = cplx<R>{(R)(y.re * c - y.im * s), (R)(y.re * s + y.im * c)};
@@ -3642,15 +3776,17 @@ __global__ void k_unsort_cols(const uint32_t* __restrict__ perm, int64_t n, cons
     m = cplx<double>{cs, sn};
   }
   const cplx<double> g = gains ? row_gain(gains, ant1, ant2, row, ch, a.nchan_t, false) : cplx<double>{1.0, 0.0};
+  cplx<double> y[NO], yg[NO];
 #pragma unroll
   for (int p = 0; p < NO; ++p) {
-    cplx<double> y{0.0, 0.0};
+    y[p] = cplx<double>{0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < NI; ++q) y = cadd(y, cmul(cplx<double>{a.cre[p * NI + q], a.cim[p * NI + q]}, o[q]));
-    y = cmul(y, m);
-    if (model) col_store(model, ci * NO + p, a.vis_f32, y);
-    col_store(vis, ci * NO + p, a.vis_f32, gains ? cmul(y, g) : y);
+    for (int q = 0; q < NI; ++q) y[p] = cadd(y[p], cmul(cplx<double>{a.cre[p * NI + q], a.cim[p * NI + q]}, o[q]));
+    y[p] = cmul(y[p], m);
+    yg[p] = gains ? cmul(y[p], g) : y[p];
   }
+  if (model) col_store_item<NO>(model, ci, a.vis_f32, (a.vec & 8) != 0, y);
+  col_store_item<NO>(vis, ci, a.vis_f32, (a.vec & 1) != 0, yg);
 }
 
 // Sums over rows of weight (1 - flags) per (channel, polarisation) of columns [nrow][nchan][npol]:
@@ -4737,16 +4873,18 @@ void launch_degrid(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>
   LAUNCH_CHECK();
 }
 
-// 16-byte value accesses need the caller's arrays 16-byte aligned (FP32 with an even batch)
+// 16-byte value accesses need the caller's arrays 16-byte aligned (FP32 with an even batch),
+// and the weights (if any) aligned to one item's NB weights
 template <typename R>
-bool vec_ok(const Geom& g, const void* p) {
-  return std::is_same<R, float>::value && g.nb % 2 == 0 && ((uintptr_t)p & 15u) == 0;
+bool vec_ok(const Geom& g, const void* p, const void* wgt) {
+  return std::is_same<R, float>::value && g.nb % 2 == 0 && ((uintptr_t)p & 15u) == 0 &&
+         (!wgt || ((uintptr_t)wgt & (uintptr_t)(g.nb * sizeof(R) - 1)) == 0);
 }
 
 template <typename R>
 void launch_values(const Geom& g, const cplx<R>* ms, const R* wgt, cplx<R>* recv, hipStream_t st) {
   const auto* rp = (const cplx<R>*)g.rec_p;
-  const bool vec = vec_ok<R>(g, ms);
+  const bool vec = vec_ok<R>(g, ms, wgt);
 #define HIPNG_KV(BB)                                                                                    \
   if (vec) k_values<R, BB, true><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, ms, wgt, recv); \
   else k_values<R, BB, false><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, ms, wgt, recv)
@@ -4758,7 +4896,7 @@ void launch_values(const Geom& g, const cplx<R>* ms, const R* wgt, cplx<R>* recv
 template <typename R>
 void launch_unsort(const Geom& g, const cplx<R>* accs, const R* wgt, cplx<R>* ms, hipStream_t st) {
   const auto* rp = (const cplx<R>*)g.rec_p;
-  const bool vec = vec_ok<R>(g, ms);
+  const bool vec = vec_ok<R>(g, ms, wgt);
 #define HIPNG_KU(BB)                                                                                    \
   if (vec) k_unsort<R, BB, true><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, accs, wgt, ms); \
   else k_unsort<R, BB, false><<<grid1(g.n), kBlock, 0, st>>>(g.perm, g.n, rp, g.flipb, accs, wgt, ms)
@@ -4805,6 +4943,21 @@ struct ColCall {
   void* model = nullptr;
 };
 
+// ColArgs.vec: which columns allow one access per item (npol items of the element type,
+// aligned to 16 bytes for complex64 pairs, to npol elements for weights and flags)
+inline bool aligned_to(const void* p, size_t a) { return p && ((uintptr_t)p % a) == 0; }
+int col_vec_bits(const ColCall& cc, int npol, const void* vis, const void* model) {
+  int v = 0;
+  const bool even = npol % 2 == 0;
+  if (cc.a.vis_f32 && even && aligned_to(vis, 16)) v |= 1;
+  const size_t ws = cc.a.wgt_f32 ? 4 : 8;
+  if ((npol == 4 || npol == 2) && aligned_to(cc.wgt, cc.a.wgt_f32 ? ws * npol : 16)) v |= 2;
+  const size_t fs = cc.a.flag_i8 ? 1 : 4;
+  if (npol == 4 && aligned_to(cc.flags, fs * 4)) v |= 4;
+  if (cc.a.vis_f32 && even && aligned_to(model, 16)) v |= 8;
+  return v;
+}
+
 // calls f(integral_constant<NI>, integral_constant<NO>) for 1 <= NI, NO <= 4
 template <typename F>
 void pol_dispatch(int ni, int no, F&& f) {
@@ -4827,9 +4980,11 @@ void pol_dispatch(int ni, int no, F&& f) {
 template <typename R>
 void launch_values_cols(const Geom& g, const ColCall& cc, cplx<R>* recv, hipStream_t st) {
   const auto* rp = (const cplx<R>*)g.rec_p;
+  ColArgs a = cc.a;
+  a.vec = col_vec_bits(cc, cc.npv, cc.vis, nullptr);
   pol_dispatch(cc.npv, g.nb, [&](auto ni, auto no) {
     k_values_cols<R, decltype(ni)::value, decltype(no)::value><<<grid1(g.n), kBlock, 0, st>>>(
-        g.perm, g.n, rp, g.flipb, g.uvw, g.tb.fc, cc.a, cc.vis, cc.wgt, cc.flags, cc.gains, cc.ant1, cc.ant2, recv);
+        g.perm, g.n, rp, g.flipb, g.uvw, g.tb.fc, a, cc.vis, cc.wgt, cc.flags, cc.gains, cc.ant1, cc.ant2, recv);
   });
   LAUNCH_CHECK();
 }
@@ -4837,9 +4992,11 @@ void launch_values_cols(const Geom& g, const ColCall& cc, cplx<R>* recv, hipStre
 template <typename R>
 void launch_unsort_cols(const Geom& g, const ColCall& cc, const cplx<R>* accs, void* vis, hipStream_t st) {
   const auto* rp = (const cplx<R>*)g.rec_p;
+  ColArgs a = cc.a;
+  a.vec = col_vec_bits(cc, cc.npv, vis, cc.model);
   pol_dispatch(g.nb, cc.npv, [&](auto ni, auto no) {
     k_unsort_cols<R, decltype(ni)::value, decltype(no)::value><<<grid1(g.n), kBlock, 0, st>>>(
-        g.perm, g.n, rp, g.flipb, accs, g.uvw, g.tb.fc, cc.a, cc.gains, cc.ant1, cc.ant2, vis, cc.model);
+        g.perm, g.n, rp, g.flipb, accs, g.uvw, g.tb.fc, a, cc.gains, cc.ant1, cc.ant2, vis, cc.model);
   });
   LAUNCH_CHECK();
 }
