@@ -123,6 +123,29 @@ def test_plan_options_pin_the_plan():
         wgridder.plan_params(**base, plan_opts={"oversampling": 2.0})  # unknown key
 
 
+def test_plan_choice_is_kept_per_argument_set():
+    # the library keeps its last plan choices: the same arguments give the same plan, evicted
+    # ones are recomputed identically, options and epsilon are part of the key, and rejected
+    # arguments are rejected on every call
+    base = dict(umax=600.0, vmax=550.0, wmin=1.0, wmax=400.0, npix_x=512, npix_y=512, pixsize_x=5e-4,
+                pixsize_y=5e-4, epsilon=1e-5, do_wstacking=True, precision="mixed", nvis=10_000)
+    names = [f for f, _ in _lib.HipngPlan._fields_]
+
+    def same(a, b):
+        return all(getattr(a, f) == getattr(b, f) for f in names)
+
+    first = wgridder.plan_params(**base)
+    assert same(first, wgridder.plan_params(**base))
+    for k in range(20):
+        assert wgridder.plan_params(**dict(base, umax=600.0 + k)).umax == 600.0 + k
+    assert same(first, wgridder.plan_params(**base))
+    assert first.tile == 16 and wgridder.plan_params(**base, plan_opts={"tile": 32}).tile == 32
+    assert wgridder.plan_params(**dict(base, epsilon=1e-3)).W < first.W
+    for _ in range(2):
+        with pytest.raises(ValueError, match="epsilon"):
+            wgridder.plan_params(**base, plan_opts={"sigma": 1.25})
+
+
 def test_pol_conversion_matches_oracle():
     rng = numpy.random.default_rng(0)
     v = rng.normal(size=(7, 3, 4)) + 1j * rng.normal(size=(7, 3, 4))
