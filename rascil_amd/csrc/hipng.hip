@@ -827,14 +827,49 @@ __global__ void k_scan_partial(const uint32_t* __restrict__ a, int64_t m, uint32
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+// one block, kRsItems partials per step (kRsIpt consecutive ones per thread): a scan over the
+// 2.6e5 partials of a 1.07e9-entry array takes 64 steps instead of 1024
 __global__ void k_scan_top(uint32_t* __restrict__ part, int64_t np) {
+  const bool al = ((uintptr_t)part & 15u) == 0;
   uint32_t carry = 0;
-  for (int64_t c = 0; c < np; c += kBlock) {
-    const int64_t i = c + threadIdx.x;
-    const uint32_t v = i < np ? part[i] : 0u;
+  for (int64_t c = 0; c < np; c += kRsItems) {
+    const int64_t base = c + (int64_t)threadIdx.x * kRsIpt;
+    const bool whole = al && base + kRsIpt <= np;
+    uint32_t loc[kRsIpt];
+    if (whole) {
+      const uint4* q = reinterpret_cast<const uint4*>(part + base);
+#pragma unroll
+      for (int k = 0; k < kRsIpt / 4; ++k) {
+        const uint4 v = q[k];
+        loc[4 * k] = v.x;
+        loc[4 * k + 1] = v.y;
+        loc[4 * k + 2] = v.z;
+        loc[4 * k + 3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRsIpt; ++k) loc[k] = base + k < np ? part[base + k] : 0u;
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kRsIpt; ++k) s += loc[k];
     uint32_t tot;
-    const uint32_t ex = block_excl_scan(v, &tot);
-    if (i < np) part[i] = carry + ex;
+    uint32_t run = carry + block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < kRsIpt; ++k) {
+      const uint32_t v = loc[k];
+      loc[k] = run;
+      run += v;
+    }
+    if (whole) {
+      uint4* q = reinterpret_cast<uint4*>(part + base);
+#pragma unroll
+      for (int k = 0; k < kRsIpt / 4; ++k) q[k] = make_uint4(loc[4 * k], loc[4 * k + 1], loc[4 * k + 2], loc[4 * k + 3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < kRsIpt; ++k)
+        if (base + k < np) part[base + k] = loc[k];
+    }
     carry += tot;
   }
 }
