@@ -4345,6 +4345,25 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
   return t;
 }
 
+// in-place exclusive scan of a[0, m); part holds ceil(m / kRsItems) entries. Arrays of up to
+// kScanOneBlock entries are scanned by one block in one launch (k_scan_top scans any array in
+// place): small jobs run dozens of these scans per call, and three launches cost more there.
+constexpr int64_t kScanOneBlock = 4 * (int64_t)kRsItems;
+void excl_scan(uint32_t* a, int64_t m, uint32_t* part, hipStream_t st) {
+  if (m <= kScanOneBlock) {
+    k_scan_top<<<1, kBlock, 0, st>>>(a, m);
+    LAUNCH_CHECK();
+    return;
+  }
+  const int64_t np = (m + kRsItems - 1) / kRsItems;
+  k_scan_partial<<<(unsigned)np, kBlock, 0, st>>>(a, m, part);
+  LAUNCH_CHECK();
+  k_scan_top<<<1, kBlock, 0, st>>>(part, np);
+  LAUNCH_CHECK();
+  k_scan_apply<<<(unsigned)np, kBlock, 0, st>>>(a, m, part);
+  LAUNCH_CHECK();
+}
+
 // keys + stable radix sort in the shared workspace; returns sorted keys and permutation
 // The bucket keys are recomputed from uvw in digit order (k_rekey): uvw rows are shared by all
 // channels of a row and stay cache resident, which makes this cheaper than keeping the keys
@@ -4377,18 +4396,12 @@ void sort_items(DevCtx& c, const double* uvw, int64_t n, const Tables& tb, const
       const int wbits = bits - shift > 8 * (np - ps) ? 9 : 8;
       const int db = ps == 0 ? udb : 0;
       const int64_t m = ((int64_t)1 << wbits) * nblocks;
-      const int64_t npart = (m + kRsItems - 1) / kRsItems;
       if (wbits == 9)
         k_rs_hist<9><<<rs_grid(nblocks), kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
       else
         k_rs_hist<8><<<rs_grid(nblocks), kBlock, 0, st>>>(k0, v0, db, ibits, n, shift, hist, nblocks);
       LAUNCH_CHECK();
-      k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
-      LAUNCH_CHECK();
-      k_scan_top<<<1, kBlock, 0, st>>>(part, npart);
-      LAUNCH_CHECK();
-      k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(hist, m, part);
-      LAUNCH_CHECK();
+      excl_scan(hist, m, part, st);
       if (wbits == 9)
         k_rs_scatter<9><<<rs_grid(nblocks), kBlock, 0, st>>>(k0, v0, k1, v1, n, shift, hist, nblocks, db, ibits);
       else
@@ -4512,16 +4525,6 @@ uint32_t pick_chunk(double visits_per_plane, int npersist) {
   return (uint32_t)((uint64_t)(c + 255) / 256 * 256);
 }
 
-// in-place exclusive scan of a[0, m); part holds ceil(m / kRsItems) entries
-void excl_scan(uint32_t* a, int64_t m, uint32_t* part, hipStream_t st) {
-  const int64_t np = (m + kRsItems - 1) / kRsItems;
-  k_scan_partial<<<(unsigned)np, kBlock, 0, st>>>(a, m, part);
-  LAUNCH_CHECK();
-  k_scan_top<<<1, kBlock, 0, st>>>(part, np);
-  LAUNCH_CHECK();
-  k_scan_apply<<<(unsigned)np, kBlock, 0, st>>>(a, m, part);
-  LAUNCH_CHECK();
-}
 
 void check_fft_inputs(Geom& g, DevCtx& c, hipStream_t st);
 
@@ -4760,13 +4763,7 @@ void build_tasks(const Geom& g, int slo, int shi, int nq, uint32_t chunk, hipStr
     k_tasks<<<grid1(ntiles), kBlock, 0, st>>>(g.k, slo, shi, nq, bs ? bs : g.bstart, be ? be : g.bend, chunk, g.task,
                                               g.counters, mult);
   LAUNCH_CHECK();
-  const int64_t npart = (ntiles + kRsItems - 1) / kRsItems;
-  k_scan_partial<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
-  LAUNCH_CHECK();
-  k_scan_top<<<1, kBlock, 0, st>>>(g.taskpart, npart);
-  LAUNCH_CHECK();
-  k_scan_apply<<<(unsigned)npart, kBlock, 0, st>>>(g.task, ntiles, g.taskpart);
-  LAUNCH_CHECK();
+  excl_scan(g.task, ntiles, g.taskpart, st);
   k_task_tiles<<<grid1(ntiles), kBlock, 0, st>>>(g.task, ntiles, g.counters, g.tasktab);
   LAUNCH_CHECK();
 }
