@@ -15,9 +15,9 @@
 //                  kernel offsets and (for invert) the weighted, phase-shifted values
 //   k_tasks        per-tile task counts (chunks of a tile's candidate visibilities)
 //   k_grid8        sub-tile gridder: persistent waves pop (tile, chunk, 8-column strip)
-//                  tasks; each 8 x 8 sub-tile is one cell per lane, all planes of the
-//                  pass and all polarisations in registers; candidates queued in LDS and
-//                  staged 64 at a time; one store (or atomic, for split tiles) per cell
+//                  tasks; candidates queued in LDS per 8 x 8 sub-tile, staged 64 at a time
+//                  and accumulated on the matrix cores (FP32 or FP64), the pass's planes and
+//                  all polarisations in registers; one store (or atomic, for split tiles) per cell
 //   rocFFT         pruned plane transform: batched 1-D c2c along gy over the band of grid
 //                  rows footprints can touch, k_tr_in transpose of the image's ny columns,
 //                  batched 1-D c2c along gx (forward: the same in reverse order)
@@ -1619,135 +1619,82 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // Per-wave accumulator of one row group of a strip (sub-tiles h < NH of 8 x 8 cells): PC
 // planes x NB polarisations of every cell; accumulate(h, ...) adds a staged batch of
 // sub-tile h.
-// Vector form: lane (cx, cy) = cells (sx0 + cx, sy0 + 8 h + cy), one entry at a time.
-template <typename R, int PC, int NB, int NT, int NH>
-struct Grid8Vec {
-  static constexpr bool kZeroPad = false;  // entries past the count are never read
-  R ar[NH][PC][NB], ai[NH][PC][NB];
-  __device__ __forceinline__ void zero() {
-#pragma unroll
-    for (int h = 0; h < NH; ++h)
-#pragma unroll
-      for (int q = 0; q < PC; ++q)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) ar[h][q][b] = ai[h][q][b] = 0;
-  }
-  // every operand of an entry is read before any is used, two entries per round, no
-  // branches, so the LDS latency of one entry overlaps the arithmetic of the other
-  template <int H>
-  __device__ __forceinline__ void accumulate(const R* s_t, const cplx<R>* s_v, int cnt, int lane) {
-    const int cx = lane >> 3, cy = lane & 7;
-    auto load = [&](int e, R (&wq)[PC], cplx<R> (&vv)[NB]) {
-#pragma unroll
-      for (int b = 0; b < NB; ++b) vv[b] = s_v[b * 64 + e];
-      const R* t = s_t + e * NT;
-      const R kxy = t[cx] * t[8 + cy];
-#pragma unroll
-      for (int q = 0; q < PC; ++q) wq[q] = kxy * t[16 + q];
-    };
-    auto accum = [&](const R (&wq)[PC], const cplx<R> (&vv)[NB]) {
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int q = 0; q < PC; ++q) {
-          ar[H][q][b] += wq[q] * vv[b].re;
-          ai[H][q][b] += wq[q] * vv[b].im;
-        }
-    };
-    int e = 0;
-    for (; e + 1 < cnt; e += 2) {
-      R w0[PC], w1[PC];
-      cplx<R> v0[NB], v1[NB];
-      load(e, w0, v0);
-      load(e + 1, w1, v1);
-      accum(w0, v0);
-      accum(w1, v1);
-    }
-    if (e < cnt) {
-      R w0[PC];
-      cplx<R> v0[NB];
-      load(e, w0, v0);
-      accum(w0, v0);
-    }
-  }
-  // solo: the wave owns the tile, every cell is stored; otherwise non-zero cells are added
-  __device__ __forceinline__ void store(cplx<R>* plane, const KP& k, int npl, int x0, int y0, bool solo,
-                                        int lane) const {
-    const int cx = lane >> 3, cy = lane & 7;
-#pragma unroll
-    for (int h = 0; h < NH; ++h)
-#pragma unroll
-      for (int q = 0; q < PC; ++q) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          if (q < npl && b < k.nb && (solo || ar[h][q][b] != (R)0 || ai[h][q][b] != (R)0)) {
-            cplx<R>* o = plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + cx) * k.gy + y0 + 8 * h + cy;
-            if (solo) {
-              *o = cplx<R>{ar[h][q][b], ai[h][q][b]};
-            } else {
-              atomicAdd(&o->re, ar[h][q][b]);
-              atomicAdd(&o->im, ai[h][q][b]);
-            }
-          }
-        }
-      }
-  }
-};
-
-// Matrix-core form (FP32, PC <= 2): per sub-tile h and polarisation b the 16 x 16 block
+// Matrix-core form (PC <= 2): per sub-tile h and polarisation b the 16 x 16 block
 //   G_hb[(a, q)][(c, p)] = sum_e kx_e[a] kw_e[q] * ky_e[c] (p ? Im : Re)(val_e[b])
 // for cells (sx0 + a, sy0 + 8 h + c), taken four entries at a time by
-// v_mfma_f32_16x16x4_f32: lane l supplies row l % 16 of A and column l % 16 of B for entry
-// l / 16 of the group, and register r of lane l receives row 4 (l / 16) + r, column l % 16.
-// Entries past the count up to the end of the batch of 64 must be staged as zeros.
-template <int PC, int NB, int NT, int NH>
+// v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f64_16x16x4_f64 (FP64): lane l supplies row l % 16
+// of A and column l % 16 of B for entry l / 16 of the group. Register r of lane l receives
+// column l % 16 and row 4 (l / 16) + r (FP32) or row 4 r + l / 16 (FP64; both checked on the
+// device against a host product). Entries past the count up to the end of the batch of 64 must
+// be staged as zeros.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <typename R>
+struct MfmaT;
+template <>
+struct MfmaT<float> {
+  using V = f32x4;
+  static __device__ __forceinline__ V mac(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int gk, int r) { return 4 * gk + r; }
+};
+template <>
+struct MfmaT<double> {
+  using V = f64x4;
+  static __device__ __forceinline__ V mac(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int gk, int r) { return 4 * r + gk; }
+};
+template <typename R, int PC, int NB, int NT, int NH>
 struct Grid8Mx {
   static_assert(PC <= 2, "the 16 rows of a block are 8 cells x 2 planes");
   static constexpr bool kZeroPad = true;
-  f32x4 acc[NH][NB];
+  using M = MfmaT<R>;
+  typename M::V acc[NH][NB];
   __device__ __forceinline__ void zero() {
 #pragma unroll
     for (int h = 0; h < NH; ++h)
 #pragma unroll
-      for (int b = 0; b < NB; ++b) acc[h][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int b = 0; b < NB; ++b) acc[h][b] = typename M::V{0, 0, 0, 0};
   }
   // rounds of 16 entries (4 MFMA groups, unrolled); the staging pads the batch of 64 with
   // zeros, so a round may run past the count
   template <int H>
-  __device__ __forceinline__ void accumulate(const float* s_t, const cplx<float>* s_v, int cnt, int lane) {
+  __device__ __forceinline__ void accumulate(const R* s_t, const cplx<R>* s_v, int cnt, int lane) {
     const int gk = lane >> 4, c8 = lane & 7, h2 = (lane >> 3) & 1;
     const int n16 = __builtin_amdgcn_readfirstlane(cnt);
     for (int e0 = 0; e0 < n16; e0 += 16) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int e = e0 + 4 * u + gk;
-        const float* t = s_t + e * NT;
-        const float av = t[c8] * (h2 < PC ? t[16 + h2] : 0.f);
-        const float ty = t[8 + c8];
+        const R* t = s_t + e * NT;
+        const R av = t[c8] * (h2 < PC ? t[16 + h2] : (R)0);
+        const R ty = t[8 + c8];
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          const float vb = reinterpret_cast<const float*>(s_v + b * 64 + e)[h2];
-          acc[H][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * vb, acc[H][b], 0, 0, 0);
+          const R vb = reinterpret_cast<const R*>(s_v + b * 64 + e)[h2];
+          acc[H][b] = M::mac(av, ty * vb, acc[H][b]);
         }
       }
     }
   }
-  __device__ __forceinline__ void store(cplx<float>* plane, const KP& k, int npl, int x0, int y0, bool solo,
+  __device__ __forceinline__ void store(cplx<R>* plane, const KP& k, int npl, int x0, int y0, bool solo,
                                         int lane) const {
     const int gk = lane >> 4, c8 = lane & 7, h2 = (lane >> 3) & 1;
 #pragma unroll
     for (int h = 0; h < NH; ++h)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 4 * gk + r;
+        const int row = M::row(gk, r);
         const int a = row & 7, q = row >> 3;
         if (q < PC && q < npl) {
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
-            const float val = acc[h][b][r];
-            if (b < k.nb && (solo || val != 0.f)) {
-              float* o = reinterpret_cast<float*>(plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + a) * k.gy + y0 +
-                                                  8 * h + c8) + h2;
+            const R val = acc[h][b][r];
+            if (b < k.nb && (solo || val != (R)0)) {
+              R* o = reinterpret_cast<R*>(plane + ((int64_t)(q * k.nb + b) * k.gx + x0 + a) * k.gy + y0 + 8 * h + c8) +
+                     h2;
               if (solo) *o = val;
               else atomicAdd(o, val);
             }
@@ -1758,8 +1705,7 @@ struct Grid8Mx {
 };
 
 template <typename R, int PC, int NB, int NT, int NH>
-using Grid8Acc = typename std::conditional<std::is_same<R, float>::value, Grid8Mx<PC, NB, NT, NH>,
-                                           Grid8Vec<R, PC, NB, NT, NH>>::type;
+using Grid8Acc = Grid8Mx<R, PC, NB, NT, NH>;
 
 // rows per candidate scan of the sub-tile gatherer (RG of k_grid8)
 template <typename R>
@@ -1767,9 +1713,10 @@ __host__ __device__ constexpr int grid8_rows() {
   return std::is_same<R, float>::value ? 16 : 8;
 }
 
-// FP32: at most 128 VGPRs, so that 4 waves per SIMD stay resident (as many as the LDS allows)
+// FP32: at most 128 VGPRs, so that 4 waves per SIMD stay resident (as many as the LDS allows);
+// FP64: at most 168, 3 waves per SIMD (C1 FP64 gatherer 353 -> 305 ms per 2-polarisation invert)
 template <typename R, int T, int PC, int NB, int RG>
-__global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 1)) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
+__global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 3)) void k_grid8(KP k, int p0, int npl, const uint32_t* __restrict__ bstart,
                                                   const uint32_t* __restrict__ bend,
                                                   const uint2* __restrict__ srec,
                                                   const uint32_t* __restrict__ rec_a,
