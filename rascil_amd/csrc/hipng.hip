@@ -2192,148 +2192,6 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* __restrict__
   return lo;
 }
 
-// Dense-run gridder (W <= 8): a wave takes a task (up to kDenseChunk visibilities of one
-// run: same start plane, same first cell); lane (ci, cj) = footprint cell (ax + ci, ay + cj)
-// and accumulates all PC planes and NB polarisations of the run in registers. Visibilities
-// are staged 64 at a time, one per lane (8 u taps, 8 v taps, the pass's w taps, values);
-// the next 64 records (of this task or the next) are loaded while the current ones are
-// accumulated, and the next task is popped one task ahead. Each task ends with one global
-// atomic add per cell, plane and polarisation. Runs after the sub-tile gatherer of the
-// same pass (whose solo tiles are written with plain stores).
-template <typename R, int NB>
-__global__ __launch_bounds__(kBlock) void k_grid_dense(KP k, int p0, int npl, const uint4* __restrict__ dtask,
-                                                       const uint32_t* __restrict__ dplane,
-                                                       const rec3<R>* __restrict__ rec_f,
-                                                       const cplx<R>* __restrict__ rec_v,
-                                                       uint32_t* __restrict__ counters, cplx<R>* __restrict__ plane) {
-  constexpr int PC = kMaxPC;
-  constexpr int NW = kBlock / 64;
-  __shared__ R s_tx[NW][64][9];
-  __shared__ R s_ty[NW][64][9];
-  __shared__ R s_tw[NW][64][PC];
-  __shared__ cplx<R> s_v[NW][NB][64];
-  const int W = k.W;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int ci = lane >> 3, cj = lane & 7;
-  const R scale = (R)2 / (R)W;
-  const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
-  int slo, shi;
-  chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
-  const uint32_t total = counters[kDenseTotal];
-  const uint32_t lo = lower_bound_u32(dplane, total, (uint32_t)slo);
-  const uint32_t hi = lower_bound_u32(dplane, total, (uint32_t)shi + 1u);
-  auto pop = [&]() {
-    uint32_t t = 0;
-    if (lane == 0) t = queue_pop(&counters[kDenseCnt], hi - lo);
-    return lo + __shfl(t, 0, 64);
-  };
-  uint32_t t = pop();
-  if (t >= hi) return;
-  uint4 cur = dtask[t];  // first record, count, packed first cell, start plane
-  uint32_t tn = pop();
-  // registers of the record this lane stages next
-  rec3<R> f{};
-  cplx<R> v[NB];
-  auto load = [&](const uint4& tk, uint32_t b0) {
-    if (b0 + (uint32_t)lane < tk.y) {
-      const uint32_t j = tk.x + b0 + (uint32_t)lane;
-      f = rec_f[j];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) v[b] = rec_v[(int64_t)j * NB + b];
-    }
-  };
-  load(cur, 0);
-  R ar[PC][NB], ai[PC][NB];
-#pragma unroll
-  for (int q = 0; q < PC; ++q)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
-  uint32_t b0 = 0;
-  for (;;) {
-    const int cnt = (int)min(64u, cur.y - b0);
-    const int sp = (int)cur.w;
-    if (lane < cnt) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        s_tx[wv][lane][i] = i < W ? esk_fast((f.x + (R)i) * scale, beta) : (R)0;
-        s_ty[wv][lane][i] = i < W ? esk_fast((f.y + (R)i) * scale, beta) : (R)0;
-      }
-#pragma unroll
-      for (int q = 0; q < PC; ++q) {
-        const int iw = p0 + q - sp;
-        s_tw[wv][lane][q] = (q < npl && iw >= 0 && iw < k.Ww)
-                                ? (k.wstack ? esk_fast((f.w + (R)iw) * scale, beta) : (R)1)
-                                : (R)0;
-      }
-#pragma unroll
-      for (int b = 0; b < NB; ++b) s_v[wv][b][lane] = v[b];
-    }
-    wave_sync();
-    // next batch: the rest of this task, else the first records of the next task
-    const bool last = b0 + 64u >= cur.y;
-    uint4 nxt = cur;
-    uint32_t nb0 = b0 + 64u;
-    if (last) {
-      nb0 = 0;
-      nxt = tn < hi ? dtask[tn] : make_uint4(0u, 0u, 0u, 0u);
-    }
-    load(nxt, nb0);
-    auto accum = [&](int e) {
-      const R kxy = s_tx[wv][e][ci] * s_ty[wv][e][cj];
-      R wq[PC];
-#pragma unroll
-      for (int q = 0; q < PC; ++q) wq[q] = kxy * s_tw[wv][e][q];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const cplx<R> vv = s_v[wv][b][e];
-#pragma unroll
-        for (int q = 0; q < PC; ++q) {
-          ar[q][b] += wq[q] * vv.re;
-          ai[q][b] += wq[q] * vv.im;
-        }
-      }
-    };
-    int e = 0;
-    for (; e + 1 < cnt; e += 2) {
-      accum(e);
-      accum(e + 1);
-    }
-    if (e < cnt) accum(e);
-    wave_sync();
-    if (!last) {
-      b0 = nb0;
-      continue;
-    }
-    if (ci < W && cj < W) {
-      const uint32_t A = cur.z;
-      int gxi = (int)(A & 0xffffu) + ci, gyi = (int)(A >> 16) + cj;
-      if (gxi >= k.gx) gxi -= k.gx;
-      if (gyi >= k.gy) gyi -= k.gy;
-#pragma unroll
-      for (int q = 0; q < PC; ++q) {
-        if (q < npl) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            if (ar[q][b] != (R)0 || ai[q][b] != (R)0) {
-              cplx<R>* o = plane + ((int64_t)(q * NB + b) * k.gx + gxi) * k.gy + gyi;
-              atomicAdd(&o->re, ar[q][b]);
-              atomicAdd(&o->im, ai[q][b]);
-            }
-          }
-        }
-      }
-    }
-    if (tn >= hi) break;
-#pragma unroll
-    for (int q = 0; q < PC; ++q)
-#pragma unroll
-      for (int b = 0; b < NB; ++b) ar[q][b] = ai[q][b] = 0;
-    cur = nxt;
-    b0 = 0;
-    tn = pop();
-  }
-}
-
 // lastw[tile] = pass for the tiles the footprints of this pass's dense-run tasks touch (a W x W
 // footprint from the packed first cell ax | ay << 16 meets at most 2 x 2 tiles, W <= tile)
 __global__ void k_mark_dense(KP k, int p0, int npl, const uint4* __restrict__ dtask,
@@ -2357,36 +2215,48 @@ __global__ void k_mark_dense(KP k, int p0, int npl, const uint4* __restrict__ dt
   }
 }
 
-// Dense-run gridder on the matrix cores (FP32 kernel modes, W <= 8, 2 planes per pass).
+// Dense-run gridder on the matrix cores (W <= 8, 2 planes per pass; FP32 or FP64).
 // For polarisation b the wave accumulates the 16 x 16 block
 //   G_b[(a, q)][(c, h)] = sum_v kx_v[a] kw_v[q] * ky_v[c] (h ? Im : Re)(val_v[b])
 // over the task's visibilities v, for footprint cells (ax + a, ay + c), a, c < 8, and the
-// pass's planes q < 2: a sum of rank-1 terms that v_mfma_f32_16x16x4_f32 takes four
-// visibilities at a time. Lane l supplies row l % 16 of A and column l % 16 of B for
-// visibility l / 16 of the group; register r of lane l receives row 4 (l / 16) + r, column
-// l % 16 of the block. Tasks, staging and the atomic flush follow k_grid_dense; staging
-// lanes past the end of a batch write zeros, so a partial group of four adds nothing.
+// pass's planes q < 2: a sum of rank-1 terms that v_mfma_f32_16x16x4_f32 (FP64:
+// v_mfma_f64_16x16x4_f64) takes four visibilities at a time, with the lane layouts of
+// Grid8Mx. Tasks (up to kDenseChunk visibilities of one run: same start plane, same first
+// cell) are popped one ahead and staged 64 at a time, the next records loaded while the
+// current ones are accumulated; each task ends with one global atomic add per cell, plane
+// and polarisation. Staging lanes past the end of a batch write zeros, so a partial group of
+// four adds nothing. Runs after the sub-tile gatherer of the same pass (whose solo tiles are
+// written with plain stores).
 
-template <int NB>
+// taps i < 8 of one axis for a run-time support W <= 8, FP64 (direct formula)
+__device__ __forceinline__ void taps8(const TapPoly&, int W, double f, double scale, double beta, bool live,
+                                      double (&kt)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) kt[i] = (live && i < W) ? esk((f + (double)i) * scale, beta) : 0.0;
+}
+
+template <typename R, int NB>
 __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl, const uint4* __restrict__ dtask,
                                                           const uint32_t* __restrict__ dplane,
-                                                          const rec3<float>* __restrict__ rec_f,
-                                                          const cplx<float>* __restrict__ rec_v,
+                                                          const rec3<R>* __restrict__ rec_f,
+                                                          const cplx<R>* __restrict__ rec_v,
                                                           uint32_t* __restrict__ counters,
-                                                          cplx<float>* __restrict__ plane, TapPoly tp) {
+                                                          cplx<R>* __restrict__ plane, TapPoly tp) {
   static_assert(kMaxPC == 2, "the 16 rows of a block are 8 cells x 2 planes");
   constexpr int PC = kMaxPC;
   constexpr int NW = kBlock / 64;
-  __shared__ float s_tx[NW][64][9];
-  __shared__ float s_ty[NW][64][9];
-  __shared__ float s_tw[NW][64][PC];
-  __shared__ float s_v[NW][NB][64][2];
+  using M = MfmaT<R>;
+  __shared__ R s_tx[NW][64][9];
+  __shared__ R s_ty[NW][64][9];
+  __shared__ R s_tw[NW][64][PC];
+  __shared__ R s_v[NW][NB][64][2];
   const int W = k.W;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int gk = lane >> 4, gr = lane & 15;  // visibility of the group, row / column of the block
   const int c8 = gr & 7, h2 = gr >> 3;       // cell, and plane (A side) or Re / Im (B side)
-  const float scale = 2.0f / (float)W;
-  const float beta = (float)(k.beta * 1.4426950408889634);
+  const R scale = (R)2 / (R)W;
+  // the FP32 tap evaluation works in base 2
+  const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const uint32_t total = counters[kDenseTotal];
@@ -2406,8 +2276,8 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
   if (t >= hi) return;
   uint4 cur = task_at(t);  // first record, count, packed first cell, start plane
   uint32_t tn = pop();
-  rec3<float> f{};
-  cplx<float> v[NB];
+  rec3<R> f{};
+  cplx<R> v[NB];
   auto load = [&](const uint4& tk, uint32_t b0) {
     if (b0 + (uint32_t)lane < tk.y) {
       const uint32_t j = tk.x + b0 + (uint32_t)lane;
@@ -2417,16 +2287,16 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
     }
   };
   load(cur, 0);
-  f32x4 acc[NB];
+  typename M::V acc[NB];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int b = 0; b < NB; ++b) acc[b] = typename M::V{0, 0, 0, 0};
   uint32_t b0 = 0;
   for (;;) {
     const int cnt = (int)min(64u, cur.y - b0);
     const int sp = (int)cur.w;
     {
       const bool live = lane < cnt;
-      float tx8[8], ty8[8];
+      R tx8[8], ty8[8];
       taps8(tp, W, f.x, scale, beta, live, tx8);
       taps8(tp, W, f.y, scale, beta, live, ty8);
 #pragma unroll
@@ -2438,13 +2308,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
       for (int q = 0; q < PC; ++q) {
         const int iw = p0 + q - sp;
         s_tw[wv][lane][q] = (live && q < npl && iw >= 0 && iw < k.Ww)
-                                ? (k.wstack ? esk_fast((f.w + (float)iw) * scale, beta) : 1.f)
-                                : 0.f;
+                                ? (k.wstack ? esk_fast((f.w + (R)iw) * scale, beta) : (R)1)
+                                : (R)0;
       }
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        s_v[wv][b][lane][0] = live ? v[b].re : 0.f;
-        s_v[wv][b][lane][1] = live ? v[b].im : 0.f;
+        s_v[wv][b][lane][0] = live ? v[b].re : (R)0;
+        s_v[wv][b][lane][1] = live ? v[b].im : (R)0;
       }
     }
     wave_sync();
@@ -2461,11 +2331,10 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int e = e0 + 4 * u + gk;
-        const float av = s_tx[wv][e][c8] * s_tw[wv][e][h2];
-        const float ty = s_ty[wv][e][c8];
+        const R av = s_tx[wv][e][c8] * s_tw[wv][e][h2];
+        const R ty = s_ty[wv][e][c8];
 #pragma unroll
-        for (int b = 0; b < NB; ++b)
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, ty * s_v[wv][b][e][h2], acc[b], 0, 0, 0);
+        for (int b = 0; b < NB; ++b) acc[b] = M::mac(av, ty * s_v[wv][b][e][h2], acc[b]);
       }
     }
     wave_sync();
@@ -2479,24 +2348,23 @@ __global__ __launch_bounds__(kBlock) void k_grid_dense_mx(KP k, int p0, int npl,
       if (gyi >= k.gy) gyi -= k.gy;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 4 * gk + r;
+        const int row = M::row(gk, r);
         const int ca = row & 7, q = row >> 3;
         if (q < npl && ca < W && c8 < W) {
           int gxi = (int)(A & 0xffffu) + ca;
           if (gxi >= k.gx) gxi -= k.gx;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
-            const float val = acc[b][r];
-            if (val != 0.f)
-              atomicAdd(reinterpret_cast<float*>(plane + ((int64_t)(q * NB + b) * k.gx + gxi) * k.gy + gyi) + h2,
-                        val);
+            const R val = acc[b][r];
+            if (val != (R)0)
+              atomicAdd(reinterpret_cast<R*>(plane + ((int64_t)(q * NB + b) * k.gx + gxi) * k.gy + gyi) + h2, val);
           }
         }
       }
     }
     if (tn >= hi) break;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NB; ++b) acc[b] = typename M::V{0, 0, 0, 0};
     cur = nxt;
     b0 = 0;
     tn = pop();
@@ -3003,39 +2871,43 @@ __global__ __launch_bounds__(kBlock) void k_degrid(KP k, int p0, int npl, const 
   }
 }
 
-// Dense-run degridder on the matrix cores (FP32 kernel modes, W <= 8, up to 2 planes per
-// launch). The visibilities of a run share their footprint window, so per polarisation b
+// Dense-run degridder on the matrix cores (W <= 8, up to 2 planes per launch; FP32 or FP64).
+// The visibilities of a run share their footprint window, so per polarisation b
 //   U_b[(c, h)][v] = sum_(a, q) (h ? Im : Re) G_q[ax + a][ay + c][b] * kx_v[a] kw_v[q]
 // is the product of the window's 16 x 16 matrix and a 16-row column per visibility, taken
-// 16 visibilities x 4 terms at a time by v_mfma_f32_16x16x4_f32 (lane l supplies A row
-// l % 16, column 4 s + l / 16 and B row 4 s + l / 16, column l % 16 of step s; register r of
-// lane l receives row 4 (l / 16) + r, column l % 16). A lane so holds 4 of the 8 y cells of
-// one visibility and part h: the y contraction sum_c ky_v[c] U_b[(c, h)][v] is 4 products in
-// registers plus one exchange with lane l ^ 16. The results are staged in LDS and added to
-// the visibilities' accumulators one visibility per lane. Tasks are the dense-run gridder's;
-// the other records go through k_degrid.
-template <int NB>
+// 16 visibilities x 4 terms at a time by v_mfma_f32_16x16x4_f32 / v_mfma_f64_16x16x4_f64
+// (lane l supplies A row l % 16, column 4 s + l / 16 and B row 4 s + l / 16, column l % 16 of
+// step s). In FP32 register r of lane l receives row 4 (l / 16) + r, so a lane holds 4 of the
+// 8 y cells of one visibility and part h: the y contraction sum_c ky_v[c] U_b[(c, h)][v] is 4
+// products in registers plus one exchange with lane l ^ 16. In FP64 it receives row
+// 4 r + l / 16 (y cells l / 16 and 4 + l / 16 of both parts): 2 products per part and a sum
+// over the four lanes of the visibility. The results are staged in LDS and added to the
+// visibilities' accumulators one visibility per lane. Tasks are the dense-run gridder's; the
+// other records go through k_degrid.
+template <typename R, int NB>
 __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int npl, const uint4* __restrict__ dtask,
                                                             const uint32_t* __restrict__ dplane,
-                                                            const rec3<float>* __restrict__ rec_f,
-                                                            const cplx<float>* __restrict__ plane,
-                                                            int64_t plane_stride, cplx<float>* __restrict__ accs,
+                                                            const rec3<R>* __restrict__ rec_f,
+                                                            const cplx<R>* __restrict__ plane,
+                                                            int64_t plane_stride, cplx<R>* __restrict__ accs,
                                                             uint32_t* __restrict__ counters, TapPoly tp) {
   static_assert(kMaxPC == 2, "the 16 columns of A are 8 cells x 2 planes");
   constexpr int NW = kBlock / 64;
   // taps tap-major, row strides chosen so that the reads of a half wave (16 visibilities x
   // 2 rows) fall in distinct banks: x rows 4 s + {0, 1} are 16 banks apart (80 = 16 mod 32),
   // y rows r and 4 + r likewise (4 x 68 = 16 mod 32); results with an odd stride per visibility
-  __shared__ float s_tx[NW][8][80];
-  __shared__ float s_ty[NW][8][68];
-  __shared__ float s_tw[NW][64][2];
-  __shared__ float s_out[NW][64][2 * NB + 1];
+  using M = MfmaT<R>;
+  __shared__ R s_tx[NW][8][80];
+  __shared__ R s_ty[NW][8][68];
+  __shared__ R s_tw[NW][64][2];
+  __shared__ R s_out[NW][64][2 * NB + 1];
   const int W = k.W;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int gk = lane >> 4, gm = lane & 15;
   const int c8 = gm & 7, h2 = gm >> 3;
-  const float scale = 2.0f / (float)W;
-  const float beta = (float)(k.beta * 1.4426950408889634);
+  const R scale = (R)2 / (R)W;
+  // the FP32 tap evaluation works in base 2
+  const R beta = std::is_same<R, float>::value ? (R)(k.beta * 1.4426950408889634) : (R)k.beta;
   int slo, shi;
   chunk_range(k.Ww, k.nplanes, p0, npl, &slo, &shi);
   const uint32_t total = counters[kDenseTotal];
@@ -3051,7 +2923,7 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
                                  __builtin_amdgcn_readfirstlane(d.z), __builtin_amdgcn_readfirstlane(d.w));
     const int sp = (int)cur.w;
     // A: the run's window, row (c, h) = l % 16, column (a, q) = 4 s + l / 16 of step s
-    float bw[NB][4];
+    R bw[NB][4];
     {
       int gyi = (int)(cur.z >> 16) + c8;
       if (gyi >= k.gy) gyi -= k.gy;
@@ -3064,18 +2936,18 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
         const bool in = q < npl && a < W && c8 < W;
 #pragma unroll
         for (int b = 0; b < NB; ++b)
-          bw[b][st] = in ? reinterpret_cast<const float*>(
+          bw[b][st] = in ? reinterpret_cast<const R*>(
                                plane + q * plane_stride + ((int64_t)b * k.gx + gxi) * k.gy + gyi)[h2]
-                         : 0.f;
+                         : (R)0;
       }
     }
     for (uint32_t b0 = 0; b0 < cur.y; b0 += 64) {
       const int cnt = (int)min(64u, cur.y - b0);
       {
         const bool live = lane < cnt;
-        rec3<float> f{};
+        rec3<R> f{};
         if (live) f = rec_f[cur.x + b0 + (uint32_t)lane];
-        float tx8[8], ty8[8];
+        R tx8[8], ty8[8];
         taps8(tp, W, f.x, scale, beta, live, tx8);
         taps8(tp, W, f.y, scale, beta, live, ty8);
 #pragma unroll
@@ -3087,39 +2959,58 @@ __global__ __launch_bounds__(kBlock) void k_degrid_dense_mx(KP k, int p0, int np
         for (int q = 0; q < 2; ++q) {
           const int iw = p0 + q - sp;
           s_tw[wv][lane][q] = (live && q < npl && iw >= 0 && iw < k.Ww)
-                                  ? (k.wstack ? esk_fast((f.w + (float)iw) * scale, beta) : 1.f)
-                                  : 0.f;
+                                  ? (k.wstack ? esk_fast((f.w + (R)iw) * scale, beta) : (R)1)
+                                  : (R)0;
         }
       }
       wave_sync();
       for (int g0 = 0; g0 < cnt; g0 += 16) {
         const int m = g0 + gm;  // this lane's visibility (B column, D column)
-        float av[4], ky[4];
+        R av[4], ky[4];
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
           const int kk = 4 * st + gk;
           av[st] = s_tx[wv][kk & 7][m] * s_tw[wv][m][kk >> 3];
         }
-        // D rows 4 gk + r = (c, h): c = 4 (gk & 1) + r, h = gk >> 1
+        if constexpr (std::is_same<R, float>::value) {
+          // D rows 4 gk + r = (c, h): c = 4 (gk & 1) + r, h = gk >> 1
 #pragma unroll
-        for (int r = 0; r < 4; ++r) ky[r] = s_ty[wv][4 * (gk & 1) + r][m];
+          for (int r = 0; r < 4; ++r) ky[r] = s_ty[wv][4 * (gk & 1) + r][m];
+        } else {
+          // D rows 4 r + gk = (c, h): c = gk (r even) or 4 + gk (r odd), h = r >> 1
+          ky[0] = ky[2] = s_ty[wv][gk][m];
+          ky[1] = ky[3] = s_ty[wv][4 + gk][m];
+        }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          f32x4 u = {0.f, 0.f, 0.f, 0.f};
+          typename M::V u = {0, 0, 0, 0};
 #pragma unroll
-          for (int st = 0; st < 4; ++st) u = __builtin_amdgcn_mfma_f32_16x16x4f32(bw[b][st], av[st], u, 0, 0, 0);
-          float x = u[0] * ky[0] + u[1] * ky[1] + u[2] * ky[2] + u[3] * ky[3];
-          x += __shfl_xor(x, 16, 64);
-          if ((gk & 1) == 0) s_out[wv][m][2 * b + (gk >> 1)] = x;
+          for (int st = 0; st < 4; ++st) u = M::mac(bw[b][st], av[st], u);
+          if constexpr (std::is_same<R, float>::value) {
+            R x = u[0] * ky[0] + u[1] * ky[1] + u[2] * ky[2] + u[3] * ky[3];
+            x += __shfl_xor(x, 16, 64);
+            if ((gk & 1) == 0) s_out[wv][m][2 * b + (gk >> 1)] = x;
+          } else {
+            // this lane holds c in {gk, 4 + gk} of both h; the four lanes of visibility m add up
+            R x0 = u[0] * ky[0] + u[1] * ky[1], x1 = u[2] * ky[2] + u[3] * ky[3];
+            x0 += __shfl_xor(x0, 16, 64);
+            x1 += __shfl_xor(x1, 16, 64);
+            x0 += __shfl_xor(x0, 32, 64);
+            x1 += __shfl_xor(x1, 32, 64);
+            if (gk == 0) {
+              s_out[wv][m][2 * b] = x0;
+              s_out[wv][m][2 * b + 1] = x1;
+            }
+          }
         }
       }
       wave_sync();
       if (lane < cnt) {
-        cplx<float>* o = accs + (int64_t)(cur.x + b0 + (uint32_t)lane) * NB;
+        cplx<R>* o = accs + (int64_t)(cur.x + b0 + (uint32_t)lane) * NB;
         const bool first = (int)cur.w >= p0;  // the launch holding the run's start plane stores
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          cplx<float> a = first ? cplx<float>{0.f, 0.f} : o[b];
+          cplx<R> a = first ? cplx<R>{0, 0} : o[b];
           a.re += s_out[wv][lane][2 * b];
           a.im += s_out[wv][lane][2 * b + 1];
           o[b] = a;
@@ -4793,31 +4684,28 @@ int degrid_planes(const Geom& g, int npl) {
 // dense-run kernels: 5 workgroups per CU fit their LDS (at most 30 KiB each, FP32)
 constexpr int kDenseBlocksPerCu = 5;
 
-// FP32 with dense runs: the dense runs on the matrix cores, the other records in k_degrid;
+// with dense runs: the dense runs on the matrix cores, the other records in k_degrid;
 // otherwise every record in k_degrid
 template <typename R>
 bool degrid_split(const Geom& g) {
-  return std::is_same<R, float>::value && g.srec != nullptr;
+  return g.srec != nullptr;
 }
 
-void launch_degrid_dense(const Geom& g, int p0, int npl, const cplx<float>* plane, cplx<float>* accs,
-                         hipStream_t st) {
+template <typename R>
+void launch_degrid_dense(const Geom& g, int p0, int npl, const cplx<R>* plane, cplx<R>* accs, hipStream_t st) {
   const KP& k = g.k;
   const int64_t pstride = (int64_t)g.nb * k.gx * k.gy;
-  const auto* rf = (const rec3<float>*)g.rec_f;
+  const auto* rf = (const rec3<R>*)g.rec_f;
   const int nblk = device_cus() * kDenseBlocksPerCu;
   if (g.nb == 1)
-    k_degrid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
+    k_degrid_dense_mx<R, 1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   else if (g.nb == 2)
-    k_degrid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
+    k_degrid_dense_mx<R, 2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   else if (g.nb == 3)
-    k_degrid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
+    k_degrid_dense_mx<R, 3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   else
-    k_degrid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
+    k_degrid_dense_mx<R, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, plane, pstride, accs, g.counters, g.taps);
   LAUNCH_CHECK();
-}
-void launch_degrid_dense(const Geom&, int, int, const cplx<double>*, cplx<double>*, hipStream_t) {
-  HIPNG_THROW(HIPNG_ERUNTIME, "dense-run degridder is FP32 only");
 }
 
 template <typename R>
@@ -4884,30 +4772,30 @@ void launch_unsort(const Geom& g, const cplx<R>* accs, const R* wgt, cplx<R>* ms
   LAUNCH_CHECK();
 }
 
-// dense-run gridder launch: the matrix-core kernel for FP32 records, the vector kernel for FP64
+// dense-run gridder launches (matrix cores, FP32 or FP64 records)
 void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<float>* rf,
                        const cplx<float>* recv, cplx<float>* plane, hipStream_t st) {
   const int nblk = device_cus() * kDenseBlocksPerCu;
   if (g.nb == 1)
-    k_grid_dense_mx<1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
+    k_grid_dense_mx<float, 1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else if (g.nb == 2)
-    k_grid_dense_mx<2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
+    k_grid_dense_mx<float, 2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else if (g.nb == 3)
-    k_grid_dense_mx<3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
+    k_grid_dense_mx<float, 3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else
-    k_grid_dense_mx<4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
+    k_grid_dense_mx<float, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
 }
 void launch_grid_dense(const Geom& g, const KP& k, int p0, int npl, const rec3<double>* rf,
                        const cplx<double>* recv, cplx<double>* plane, hipStream_t st) {
   const int nblk = device_cus() * kDenseBlocksPerCu;
   if (g.nb == 1)
-    k_grid_dense<double, 1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<double, 1><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else if (g.nb == 2)
-    k_grid_dense<double, 2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<double, 2><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else if (g.nb == 3)
-    k_grid_dense<double, 3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<double, 3><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
   else
-    k_grid_dense<double, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane);
+    k_grid_dense_mx<double, 4><<<nblk, kBlock, 0, st>>>(k, p0, npl, g.dtask, g.dplane, rf, recv, g.counters, plane, g.taps);
 }
 
 // Columns of one call (hipng_geometry_invert_columns / _predict_columns)
@@ -5249,7 +5137,7 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
       }
       if (degrid_split<R>(g)) {
         PROF("degrid_dense");
-        launch_degrid_dense(g, p0 + q, nq, bplane + q * np1, accs, st);
+        launch_degrid_dense<R>(g, p0 + q, nq, bplane + q * np1, accs, st);
       }
     }
   }
