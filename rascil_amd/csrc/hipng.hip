@@ -1980,8 +1980,9 @@ __global__ __launch_bounds__(kBlock, (std::is_same<R, float>::value ? 4 : 3)) vo
 // Dense runs. The sort orders a bucket by in-tile origin, so visibilities with the same
 // start plane and the same first cell are consecutive: a run. Runs of at least kDenseMin
 // visibilities (the dense uv core: short baselines at many channels and times) are
-// gridded by k_grid_dense with one lane per footprint cell, which accumulates the whole
-// run in registers; the sub-tile gatherer skips their records (flag bit in rec_a).
+// gridded and degridded on the matrix cores with the run's footprint as the block
+// (k_grid_dense_mx, k_degrid_dense_mx); the sub-tile gatherer and k_degrid skip their
+// records (flag bit in rec_a).
 // ---------------------------------------------------------------------------------
 // head[j] = 1 where a run starts (bucket key or first cell differ from the previous record)
 // Run heads of the kRsIpt consecutive records from `base`: a run starts where the bucket key
