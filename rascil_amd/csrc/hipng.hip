@@ -4487,7 +4487,16 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
   g.fft_b_fwd = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1);
   g.fft_t_inv = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 0, false);
   g.fft_t_fwd = fft_plan1(c, P.gx, g.pc * nb * P.ny, prec, 1, false);
-  if (csz == 8) {
+  // the band transforms go out of place into plane2, so that a pass clears only the tiles the
+  // previous pass gridded into: always for FP32 planes, for FP64 ones when the second set of
+  // planes takes at most 1/16 of the device memory (C1 FP64 with 2 polarisations: 9.7 GB; the
+  // per-pass clear 31 -> 9 ms per invert, 29 ms less per invert + predict job)
+  size_t mem_free = 0, mem_total = 0;
+  if (csz != 8 && hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) {
+    (void)hipGetLastError();
+    mem_total = 0;
+  }
+  if (csz == 8 || (size_t)g.pc * plane_bytes <= mem_total / 16) {
     g.plane2 = g.buf.get(S_PLANE2, g.pc * plane_bytes);
     g.fft_b_fwd_oop = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 1, false);
     g.fft_b_inv_oop = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 0, false);
