@@ -65,6 +65,26 @@ def explicit_ms2dirty(uvw, freq, ms, wgt, npix_x, npix_y, pixsize_x, pixsize_y, 
     return res / n
 
 
+def explicit_ms2dirty_at(uvw, freq, ms, wgt, ix, iy, npix_x, npix_y, pixsize_x, pixsize_y, do_wstacking):
+    """explicit_ms2dirty at the pixels (ix[k], iy[k]) of an npix_x x npix_y image only (for
+    images too large to sum whole)."""
+    ix = numpy.asarray(ix, "i8")
+    iy = numpy.asarray(iy, "i8")
+    x = (ix - npix_x // 2) * pixsize_x
+    y = (iy - npix_y // 2) * pixsize_y
+    if do_wstacking:
+        r2 = x ** 2 + y ** 2
+        nm1 = -r2 / (numpy.sqrt(1.0 - r2) + 1.0)
+    else:
+        nm1 = numpy.zeros_like(x)
+    u, v, w = _flat_lambda(numpy.asarray(uvw, "f8"), freq)
+    a = numpy.asarray(ms, "c16").ravel()
+    if wgt is not None:
+        a = a * numpy.asarray(wgt, "f8").ravel()
+    ph = u[:, None] * x[None] + v[:, None] * y[None] - w[:, None] * nm1[None]
+    return (a @ numpy.exp(2j * numpy.pi * ph)).real / (nm1 + 1.0)
+
+
 def explicit_dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, do_wstacking):
     """Exact forward sum; dirty [nx, ny] -> ms [nrow, nchan] complex128."""
     dirty = numpy.asarray(dirty, "f8")

@@ -160,3 +160,18 @@ def test_two_pol_matrices_are_the_parallel_hands_of_the_four_pol_ones():
     for a, b in [("stokesIQ", "linearnp"), ("stokesIV", "circularnp")]:
         numpy.testing.assert_allclose(M[(b, a)] @ M[(a, b)], numpy.eye(2), atol=0)
 
+
+
+@pytest.mark.parametrize("wst", [False, True])
+def test_probe_pixel_sum_equals_the_whole_image_sum(wst):
+    # explicit_ms2dirty_at is the whole-image sum read at chosen pixels
+    rng = numpy.random.default_rng(11)
+    uvw = rng.normal(0.0, 200.0, (7, 3))
+    freq = numpy.array([1.0e8, 1.3e8])
+    ms = rng.normal(size=(7, 2)) + 1j * rng.normal(size=(7, 2))
+    wgt = rng.uniform(0.5, 1.5, (7, 2))
+    nx, ny, px, py = 24, 20, 2e-2, 3e-2
+    full = ox.explicit_ms2dirty(uvw, freq, ms, wgt, nx, ny, px, py, wst)
+    ix, iy = numpy.meshgrid(numpy.arange(nx), numpy.arange(ny), indexing="ij")
+    at = ox.explicit_ms2dirty_at(uvw, freq, ms, wgt, ix.ravel(), iy.ravel(), nx, ny, px, py, wst)
+    assert numpy.allclose(at.reshape(nx, ny), full, rtol=0, atol=1e-12 * numpy.abs(full).max())
