@@ -105,6 +105,28 @@ def explicit_dirty2ms(uvw, freq, dirty, wgt, pixsize_x, pixsize_y, do_wstacking)
     return out
 
 
+def explicit_dirty2ms_sparse(uvw, freq, ix, iy, values, npix_x, npix_y, pixsize_x, pixsize_y, wgt,
+                            do_wstacking):
+    """explicit_dirty2ms of an npix_x x npix_y image that is zero except dirty[ix[k], iy[k]] =
+    values[k] (for images too large to sum whole)."""
+    ix = numpy.asarray(ix, "i8")
+    iy = numpy.asarray(iy, "i8")
+    x = (ix - npix_x // 2) * pixsize_x
+    y = (iy - npix_y // 2) * pixsize_y
+    if do_wstacking:
+        r2 = x ** 2 + y ** 2
+        nm1 = -r2 / (numpy.sqrt(1.0 - r2) + 1.0)
+    else:
+        nm1 = numpy.zeros_like(x)
+    u, v, w = _flat_lambda(numpy.asarray(uvw, "f8"), freq)
+    ph = u[:, None] * x[None] + v[:, None] * y[None] - w[:, None] * nm1[None]
+    out = numpy.exp(-2j * numpy.pi * ph) @ (numpy.asarray(values, "f8") / (nm1 + 1.0))
+    out = out.reshape(uvw.shape[0], len(numpy.atleast_1d(freq)))
+    if wgt is not None:
+        out = out * numpy.asarray(wgt, "f8").reshape(out.shape)
+    return out
+
+
 def dft_cpu_looped(direction_cosines, uvw_lambda, vfluxes):
     """vis[t,b,c,p] = sum_k F[k,c,p] exp(-2 pi i uvw_lambda . (l, m, n-1)_k) (dft.py:261-281)."""
     ncomp = direction_cosines.shape[0]

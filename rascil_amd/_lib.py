@@ -189,6 +189,13 @@ def load():
                 f"hipng library not built: {LIBPATH} is missing "
                 "(run rascil_amd.build.build() or __graft_entry__.build())"
             )
+        try:
+            # torch ships its own HIP, HSA and rocFFT runtimes; loaded first, they are the ones
+            # the library binds to. Loaded after the library, they would be a second copy in
+            # the process, whose device initialisation fails ("no ROCm-capable device").
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIBPATH)
         c_i32, c_i64, c_dbl, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
         pd = ctypes.POINTER(ctypes.c_double)

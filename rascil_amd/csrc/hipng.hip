@@ -2822,6 +2822,130 @@ __global__ __launch_bounds__(256) void k_fs_inv2(KP k, double w, const cplx<R>* 
   }
 }
 
+// predict, the adjoint split (w_N^-1 = conj(w_N)): input g = k1 + kFsN1 k2 (image side),
+// output x = kFsN2 n1 + n2 (grid side):
+//   X[kFsN2 n1 + n2] = sum_k1 w96^-(k1 n1) [ wG^-(k1 n2) sum_k2 w128^-(k2 n2) T[k1 + kFsN1 k2] ]
+template <typename R>
+__device__ __forceinline__ cplx<R> fs_conj(cplx<R> a) {
+  return cplx<R>{a.re, -a.im};
+}
+
+// stage 2' (predict): block = (16 values of k1, image row iy, polarisation b); 256 threads; one
+// launch per plane. Forms T = (-1)^(tx+ty) dc exp(+2 pi i w nm1s) on the image pixels (zero
+// on the others) as it loads, transforms along k2 (128 = 8 x 16: k2 = j + 16 e,
+// n2 = a + 8 b'), applies wG^-(k1 n2) and writes V[iy][n2][k1]
+template <typename R, typename A, bool F32C>
+__global__ __launch_bounds__(256) void k_fs_fwd2(KP k, double w, const A* __restrict__ dc, cplx<R>* __restrict__ V,
+                                                 const cplx<R>* __restrict__ tw) {
+  __shared__ cplx<R> s[16][kFsN2 + 1];
+  const int kb = blockIdx.x * 16, iy = blockIdx.y, b = blockIdx.z;
+  const int t = threadIdx.x;
+  const int c2 = t & 15, j2 = t >> 4;  // lanes run along k1: 128-byte pieces of the image rows
+  const int ty = iy - k.ny / 2, hx = k.nx / 2;
+  const A* drow = dc + ((int64_t)b * k.ny + iy) * k.nx + hx;
+  cplx<R> x[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int gxi = kb + c2 + kFsN1 * (j2 + 16 * e);
+    x[e] = cplx<R>{0, 0};
+    if (gxi < hx || gxi >= k.gx - hx) {
+      const int tx = gxi < hx ? gxi : gxi - k.gx;
+      R d = (R)drow[tx];
+      if ((tx + ty) & 1) d = -d;
+      R c = 1, sn = 0;
+      if (k.wstack) phase_cs<R, F32C>(w * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &sn);
+      x[e] = cplx<R>{d * c, d * sn};
+    }
+  }
+  // radix 8 over e, then the twiddle w128^-(j a); Y[j][a] goes to 16 a + j
+  FsDft<R, 8, -1>::run(x);
+#pragma unroll
+  for (int a2 = 1; a2 < 8; ++a2) x[a2] = fs_mul(x[a2], fs_conj(tw[kFsG + kFsN1 + (j2 * a2) % kFsN2]));
+#pragma unroll
+  for (int a2 = 0; a2 < 8; ++a2) s[c2][16 * a2 + j2] = x[a2];
+  __syncthreads();
+  {
+    // radix 16 over j for each a (threads t < 128): n2 = a + 8 b'; then wG^-(k1 n2)
+    const int c = (t >> 3) & 15, a2 = t & 7;
+    cplx<R> y[16];
+    if (t < 128) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) y[j] = s[c][16 * a2 + j];
+      FsDft<R, 16, -1>::run(y);
+    }
+    __syncthreads();
+    if (t < 128) {
+#pragma unroll
+      for (int bb = 0; bb < 16; ++bb) {
+        const int n2 = a2 + 8 * bb;
+        s[c][n2] = fs_mul(y[bb], fs_conj(tw[((kb + c) * n2) % kFsG]));
+      }
+    }
+  }
+  __syncthreads();
+  cplx<R>* v = V + ((int64_t)b * k.ny + iy) * kFsG;
+  for (int e = t; e < 16 * kFsN2; e += 256) {
+    const int n2 = e >> 4, c = e & 15;
+    v[n2 * kFsN1 + kb + c] = s[c][n2];
+  }
+}
+
+// stage 1' (predict): block = (n2, 16 image columns, plane i); 192 threads. Transforms
+// V[i][iy][n2][.] along k1 (96 = 8 x 12: k1 = d + 8 e, n1 = q + 12 a) and writes the band rows
+// x = kFsN2 n1 + n2 in [xlo, xhi) of the image columns of plane i (the other columns of the
+// band rows are left as they are: zero)
+template <typename R>
+__global__ __launch_bounds__(192) void k_fs_fwd1(KP k, int xlo, int xhi, const cplx<R>* __restrict__ V,
+                                                 cplx<R>* __restrict__ plane, const cplx<R>* __restrict__ tw) {
+  __shared__ cplx<R> s[16][kFsN1 + 1];
+  const int n2 = blockIdx.x, iy0 = blockIdx.y * 16, i = blockIdx.z;
+  const int t = threadIdx.x;
+  for (int e = t; e < 16 * kFsN1; e += 192) {
+    const int c = e / kFsN1, k1 = e - c * kFsN1;
+    s[c][k1] = iy0 + c < k.ny ? V[(((int64_t)i * k.ny + iy0 + c) * kFsN2 + n2) * kFsN1 + k1] : cplx<R>{0, 0};
+  }
+  __syncthreads();
+  {
+    // radix 12 over e for each d (threads t < 128), then the twiddle w96^-(d q); Y[d][q] goes
+    // to 12 d + q
+    const int c = (t >> 3) & 15, d = t & 7;
+    cplx<R> x[12];
+    if (t < 128) {
+#pragma unroll
+      for (int e = 0; e < 12; ++e) x[e] = s[c][d + 8 * e];
+      FsDft<R, 12, -1>::run(x);
+#pragma unroll
+      for (int q = 1; q < 12; ++q) x[q] = fs_mul(x[q], fs_conj(tw[kFsG + (d * q) % kFsN1]));
+    }
+    __syncthreads();
+    if (t < 128) {
+#pragma unroll
+      for (int q = 0; q < 12; ++q) s[c][12 * d + q] = x[q];
+    }
+  }
+  __syncthreads();
+  {
+    // radix 8 over d for each q: n1 = q + 12 a
+    const int c = t / 12, q = t % 12;
+    cplx<R> x[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) x[d] = s[c][12 * d + q];
+    FsDft<R, 8, -1>::run(x);
+    __syncthreads();
+#pragma unroll
+    for (int a2 = 0; a2 < 8; ++a2) s[c][q + 12 * a2] = x[a2];
+  }
+  __syncthreads();
+  const int c16 = t & 15, iy = iy0 + c16;
+  if (iy >= k.ny) return;
+  cplx<R>* pc = plane + (int64_t)i * k.gx * k.gy + img_col_to_grid(iy, k.ny, k.gy);
+#pragma unroll
+  for (int a2 = 0; a2 < 8; ++a2) {
+    const int n1 = (t >> 4) + 12 * a2, x = kFsN2 * n1 + n2;
+    if (x >= xlo && x < xhi) pc[(int64_t)x * k.gy] = s[c16][n1];
+  }
+}
+
 // dirty[b][ix][iy] = accT[b][iy][ix] * correction(ix, iy); 32 x 32 tiles through LDS
 template <typename A, typename O>
 __global__ __launch_bounds__(kBlock) void k_finalize_t(KP k, const A* __restrict__ acc, O* __restrict__ dirty,
@@ -5374,17 +5498,20 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   const int ngrp = sizeof(R) == 4 ? kFillGroup : 1;
   cplx<R>* sbuf = ngrp > 1 ? tbuf2 : tbuf;
   cplx<R>* dbuf = ngrp > 1 ? tbuf : tbuf2;
-  {
+  const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
+  // band transforms out of place: the band rows of plane are cleared once and only their
+  // image columns are rewritten per pass
+  const bool oop = g.plane2 != nullptr && g.oop_ok;
+  cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
+  // FP64 planes of 12288 points along x with out-of-place band transforms: the four-step
+  // image-side transform writes the image columns of the band rows itself (no S)
+  const bool fs = sizeof(R) == 8 && g.fstw != nullptr && oop;
+  if (!fs) {
     PROF("transpose");
     // the fills rewrite the image columns of S; the others stay zero
     zero_cols(sbuf, (int64_t)ngrp * g.pc * g.nb * P.ny, P.gx, P.nx / 2, P.gx - P.nx / 2, st);
     LAUNCH_CHECK();
   }
-  const int64_t np1 = (int64_t)g.nb * P.gx * P.gy;
-  // band transforms out of place (FP32): the band rows of plane are cleared once and only
-  // their image columns are rewritten per pass
-  const bool oop = g.plane2 != nullptr && g.oop_ok;
-  cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
   if (oop) {
     PROF("transpose");
     // k_tr_out rewrites the image columns of the band rows; the others stay zero
@@ -5396,23 +5523,44 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   for (int p0 = 0; p0 < P.nplanes; p0 += g.pc) {
     const int npl = std::min(g.pc, P.nplanes - p0);
     const int slot = (p0 / g.pc) % ngrp;
-    if (slot == 0 && !g.oop_ok) {
+    bool fs_done = false;
+    if constexpr (sizeof(R) == 8) {
+      if (fs) {
+        const auto* tw = reinterpret_cast<const cplx<R>*>(g.fstw);
+        {
+          PROF("fourstep_2");
+          for (int q = 0; q < npl; ++q) {
+            k_fs_fwd2<R, A, F32C><<<dim3(kFsN1 / 16, P.ny, g.nb), 256, 0, st>>>(
+                k, k.w0 + (p0 + q) * k.dw, dc, tbuf + (int64_t)q * g.nb * P.ny * kFsG, tw);
+            LAUNCH_CHECK();
+          }
+        }
+        {
+          PROF("fourstep_1");
+          k_fs_fwd1<R><<<dim3(kFsN2, (P.ny + 15) / 16, npl * g.nb), 192, 0, st>>>(k, g.xlo, g.xhi, tbuf, plane,
+                                                                                 tw);
+          LAUNCH_CHECK();
+        }
+        fs_done = true;
+      }
+    }
+    if (!fs_done && slot == 0 && !g.oop_ok) {
       PROF("transpose");
       // the transforms along the grid rows may have overwritten S: clear it again
       zero_cols(sbuf, (int64_t)ngrp * g.pc * g.nb * P.ny, P.gx, P.nx / 2, P.gx - P.nx / 2, st);
       LAUNCH_CHECK();
     }
-    if (slot == 0) {
+    if (!fs_done && slot == 0) {
       PROF("screen_fill");
       const int nfill = std::min(ngrp * g.pc, P.nplanes - p0);
       k_screen_fill_t<R, A, F32C><<<dim3(grid1(P.gx), P.ny), kBlock, 0, st>>>(k, p0, nfill, dc, sbuf);
       LAUNCH_CHECK();
     }
-    {
+    if (!fs_done) {
       PROF("fft_c2c_forward");
       geom_fft(g, g.fft_t_fwd, sbuf + (int64_t)slot * g.pc * g.nb * P.ny * P.gx, st, dbuf);
     }
-    {
+    if (!fs_done) {
       PROF("transpose");
       const dim3 trg((g.xhi - g.xlo + 31) / 32, (P.gy + 31) / 32, npl * g.nb);
       if (oop) k_tr_out<R, false><<<trg, kBlock, 0, st>>>(k, g.xlo, g.xhi, dbuf, plane);

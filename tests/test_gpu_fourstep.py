@@ -1,8 +1,9 @@
 """The FP64 image-side transform of 12288-point grids (an 8192^2 image at oversampling 1.5):
-invert runs it as a four-step transform (96 x 128) fused with the w-screen instead of
-transpose + rocFFT + screen. Checked through the C-ABI against the FP64 brute-force sums
-(oracle) at probe pixels: the image edges, the centre, the band limits of the grid and
-random pixels, with w-stacking off and on, for one and two polarisations."""
+invert and predict run it as a four-step transform (96 x 128) fused with the w-screen instead
+of transpose + rocFFT + screen. Checked through the C-ABI against the FP64 brute-force sums
+(oracle): invert at probe pixels, predict of images that are zero off a set of pixels. The
+pixels are the image edges, the centre, the band limits of the grid and random ones; w-stacking
+off and on; one and two polarisations."""
 
 import numpy
 import pytest
@@ -26,11 +27,11 @@ def _obs(seed, nrow=60):
 
 
 def _probes(rng):
+    """distinct pixels: the 8 x 8 combinations of edge and centre rows / columns, 400 random"""
     edge = numpy.array([0, 1, 2, N // 2 - 1, N // 2, N // 2 + 1, N - 2, N - 1])
     ex, ey = numpy.meshgrid(edge, edge, indexing="ij")
-    ix = numpy.concatenate([ex.ravel(), rng.integers(0, N, 400)])
-    iy = numpy.concatenate([ey.ravel(), rng.integers(0, N, 400)])
-    return ix, iy
+    lin = numpy.unique(numpy.concatenate([(ex * N + ey).ravel(), rng.integers(0, N * N, 400)]))
+    return lin // N, lin % N
 
 
 @pytest.mark.parametrize("wst", [False, True])
@@ -47,8 +48,9 @@ def test_fourstep_invert_one_polarisation(wst):
     finally:
         _lib.profile_enable(False)
     assert plan["gx"] == 12288 and plan["gy"] == 12288
-    # the four-step kernels ran, one stage-2 launch per plane, and the rocFFT image pass did not
-    assert prof["fourstep_1"][1] >= 1 and prof["fourstep_2"][1] == plan["nplanes"]
+    # the four-step kernels ran in every pass (one profile entry per pass), and the transposes
+    # and screen of the rocFFT image pass did not
+    assert prof["fourstep_1"][1] >= 1 and prof["fourstep_2"][1] == prof["fourstep_1"][1]
     assert "transpose" not in prof and "screen_acc" not in prof
     ix, iy = _probes(rng)
     ref = ox.explicit_ms2dirty_at(uvw, freq, ms, wgt, ix, iy, N, N, PIX, PIX, wst)
@@ -78,5 +80,56 @@ def test_fourstep_invert_two_polarisations():
         # a second call on the kept geometry stores again (no stale accumulator)
         out2 = geom.ms2dirty(torch.from_numpy(ms).to(dev), torch.from_numpy(wgt).to(dev))
         assert torch.equal(out, out2)
+    finally:
+        geom.close()
+
+
+@pytest.mark.parametrize("wst", [False, True])
+def test_fourstep_predict_one_polarisation(wst):
+    rng, uvw, freq = _obs(23)
+    ix, iy = _probes(rng)
+    val = rng.normal(size=ix.size)
+    dirty = numpy.zeros((N, N))
+    dirty[ix, iy] = val
+    wgt = rng.uniform(0.5, 1.5, (uvw.shape[0], 2))
+    plan = {}
+    _lib.profile_enable(True)
+    try:
+        v = wgridder.dirty2ms(uvw, freq, dirty, wgt, PIX, PIX, 0, 0, EPS, wst, precision="fp64",
+                              plan_out=plan, plan_opts={"sigma": 1.5})
+        prof = _lib.profile_read()
+    finally:
+        _lib.profile_enable(False)
+    assert plan["gx"] == 12288 and plan["gy"] == 12288
+    assert prof["fourstep_1"][1] >= 1 and prof["fourstep_2"][1] == prof["fourstep_1"][1]
+    assert "screen_fill" not in prof
+    ref = ox.explicit_dirty2ms_sparse(uvw, freq, ix, iy, val, N, N, PIX, PIX, wgt, wst)
+    err = rel_l2(v, ref)
+    print(f"wst={wst} planes={plan['nplanes']}: predict {err:.2e} from {ix.size} pixels")
+    assert err <= EPS
+
+
+def test_fourstep_predict_two_polarisations():
+    torch = pytest.importorskip("torch")
+    rng, uvw, freq = _obs(24, nrow=40)
+    ix, iy = _probes(rng)
+    val = rng.normal(size=(2, ix.size))
+    model = numpy.zeros((2, N, N))
+    model[0, ix, iy] = val[0]
+    model[1, ix, iy] = val[1]
+    dev = torch.device("cuda", 0)
+    geom = wgridder.Geometry(torch.from_numpy(uvw).to(dev), freq, N, N, PIX, PIX, EPS, True, "fp64",
+                             nbatch=2, plan_opts={"sigma": 1.5})
+    try:
+        assert geom.plan["gx"] == 12288
+        md = torch.from_numpy(model).to(dev)
+        v = geom.dirty2ms(md).cpu().numpy()
+        for b in range(2):
+            ref = ox.explicit_dirty2ms_sparse(uvw, freq, ix, iy, val[b], N, N, PIX, PIX, None, True)
+            err = rel_l2(v[..., b], ref)
+            print(f"polarisation {b}: predict {err:.2e}")
+            assert err <= EPS
+        # repeated calls on the kept geometry give the same visibilities
+        assert numpy.array_equal(geom.dirty2ms(md).cpu().numpy(), v)
     finally:
         geom.close()

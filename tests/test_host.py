@@ -31,6 +31,23 @@ def test_library_exports_every_header_symbol():
     assert _lib.load().hipng_version() >= 100
 
 
+def test_library_and_torch_share_one_hip_runtime():
+    # load() in a fresh process, before anything imports torch: torch's HIP / HSA runtimes must
+    # be the only ones mapped (a second copy makes device initialisation fail)
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from rascil_amd import _lib\n_lib.load()\nimport torch\n"
+            "print('\\n'.join(sorted(set(l.split()[-1] for l in open('/proc/self/maps')"
+            " if 'libamdhip64' in l or 'libhsa-runtime64' in l))))\n"
+            % ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, out.stderr
+    mapped = out.stdout.split()
+    assert sum("libamdhip64" in m for m in mapped) == 1, mapped
+    assert sum("libhsa-runtime64" in m for m in mapped) == 1, mapped
+
 @pytest.mark.parametrize("eps,prec", [(1e-4, "mixed"), (1e-5, "mixed"), (1e-7, "fp64"),
                                       (1e-12, "fp64"), (1e-3, "fp32")])
 @pytest.mark.parametrize("wst", [False, True])

@@ -175,3 +175,22 @@ def test_probe_pixel_sum_equals_the_whole_image_sum(wst):
     ix, iy = numpy.meshgrid(numpy.arange(nx), numpy.arange(ny), indexing="ij")
     at = ox.explicit_ms2dirty_at(uvw, freq, ms, wgt, ix.ravel(), iy.ravel(), nx, ny, px, py, wst)
     assert numpy.allclose(at.reshape(nx, ny), full, rtol=0, atol=1e-12 * numpy.abs(full).max())
+
+
+@pytest.mark.parametrize("wst", [False, True])
+def test_sparse_image_sum_equals_the_whole_image_sum(wst):
+    # explicit_dirty2ms_sparse is the whole-image forward sum of an image that is zero off the
+    # given pixels
+    rng = numpy.random.default_rng(12)
+    uvw = rng.normal(0.0, 200.0, (7, 3))
+    freq = numpy.array([1.0e8, 1.3e8])
+    wgt = rng.uniform(0.5, 1.5, (7, 2))
+    nx, ny, px, py = 24, 20, 2e-2, 3e-2
+    ix = numpy.array([0, 23, 12, 5, 0])
+    iy = numpy.array([0, 19, 10, 17, 19])
+    val = rng.normal(size=ix.size)
+    dirty = numpy.zeros((nx, ny))
+    dirty[ix, iy] = val
+    full = ox.explicit_dirty2ms(uvw, freq, dirty, wgt, px, py, wst)
+    sp = ox.explicit_dirty2ms_sparse(uvw, freq, ix, iy, val, nx, ny, px, py, wgt, wst)
+    assert numpy.allclose(sp, full, rtol=0, atol=1e-12 * numpy.abs(full).max())
