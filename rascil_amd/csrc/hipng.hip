@@ -2556,18 +2556,27 @@ __global__ void k_screen_fill_t(KP k, int p0, int npl, const A* __restrict__ dc,
 }
 
 // ---------------------------------------------------------------------------------
-// Four-step image-side transform (FP64 planes, grids of kFsN1 x kFsN2 = 12288 points along
-// x). rocFFT runs the FP64 length-12288 transform as two kernels (length 192 over strided
-// columns, then length 64 over rows), so transpose + transform + screen move the image-side
-// data five times. Split x = kFsN2 n1 + n2 and the output index g = k1 + kFsN1 k2:
-//   X[k1 + kFsN1 k2] = sum_n2 w128^(n2 k2) [ wG^(n2 k1) sum_n1 w96^(n1 k1) T[kFsN2 n1 + n2] ]
+// Four-step image-side transform (FP64 planes of G = N1 x kFsN2 points along x, N1 = 8 M,
+// M in {10, 12, 16, 20, 24}: G = 10240 ... 24576). rocFFT runs FP64 transforms of these
+// lengths as two kernels (a strided pass and a row pass), so transpose + transform + screen
+// move the image-side data five times. Split x = kFsN2 n1 + n2 and the output index
+// g = k1 + N1 k2:
+//   X[k1 + N1 k2] = sum_n2 w128^(n2 k2) [ wG^(n2 k1) sum_n1 wN1^(n1 k1) T[kFsN2 n1 + n2] ]
 // (w_N = exp(+2 pi i / N), the inverse transform). Stage 1 reads the band plane in 256-byte
-// strips of 16 image columns (no transpose), transforms along n1, applies wG^(n2 k1) and
-// writes U[i][iy][n2][k1]; stage 2 transforms along n2 and adds the screened real values of
-// one plane to the image accumulator. 8192^2 image, FP64: 0.61 + 0.73 ms per plane and
-// polarisation against 2.48 ms for transpose, rocFFT and screen.
+// strips of 16 image columns (no transpose), transforms along n1 (radix 8, then radix M),
+// applies wG^(n2 k1) and writes U[i][iy][n2][k1]; stage 2 transforms along n2 and adds the
+// screened real values of one plane to the image accumulator. 8192^2 image, G = 12288: 0.61 +
+// 0.73 ms per plane and polarisation against 2.48 ms for transpose, rocFFT and screen.
 // ---------------------------------------------------------------------------------
-constexpr int kFsN1 = 96, kFsN2 = 128, kFsG = kFsN1 * kFsN2;
+constexpr int kFsN2 = 128;
+// M of a grid length, or 0 when the four-step does not cover it
+inline int fs_m_for(int g) {
+  if (g % (8 * kFsN2) != 0) return 0;
+  const int m = g / (8 * kFsN2);
+  return (m == 10 || m == 12 || m == 16 || m == 20 || m == 24) ? m : 0;
+}
+// twiddle table: wG^j (j < G), then wN1^j, then w128^j
+inline int fs_tw_len(int m) { return 8 * m * kFsN2 + 8 * m + kFsN2; }
 // compile-time sine / cosine (Taylor series, |x| <= pi after reduction) for the constant
 // twiddles of the in-register transforms
 constexpr double kFsPi = 3.14159265358979323846;
@@ -2644,6 +2653,26 @@ struct FsDft<R, 4, S> {
     x[3] = fs_sub(a1, b3);
   }
 };
+template <typename R, int S>
+struct FsDft<R, 5, S> {
+  static __device__ __forceinline__ void run(cplx<R> (&x)[5]) {
+    // X_k = sum_n x_n exp(S 2 pi i n k / 5): cos / sin of 2 pi / 5 and 4 pi / 5
+    const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+    const R s1 = (R)(S * 0.95105651629515357212), s2 = (R)(S * 0.58778525229247312917);
+    const cplx<R> t1 = fs_add(x[1], x[4]), t2 = fs_add(x[2], x[3]);
+    const cplx<R> t3 = fs_sub(x[1], x[4]), t4 = fs_sub(x[2], x[3]);
+    const cplx<R> a1{x[0].re + c1 * t1.re + c2 * t2.re, x[0].im + c1 * t1.im + c2 * t2.im};
+    const cplx<R> a2{x[0].re + c2 * t1.re + c1 * t2.re, x[0].im + c2 * t1.im + c1 * t2.im};
+    const cplx<R> b1{s1 * t3.re + s2 * t4.re, s1 * t3.im + s2 * t4.im};
+    const cplx<R> b2{s2 * t3.re - s1 * t4.re, s2 * t3.im - s1 * t4.im};
+    x[0] = fs_add(x[0], fs_add(t1, t2));
+    // a +- i b
+    x[1] = cplx<R>{a1.re - b1.im, a1.im + b1.re};
+    x[4] = cplx<R>{a1.re + b1.im, a1.im - b1.re};
+    x[2] = cplx<R>{a2.re - b2.im, a2.im + b2.re};
+    x[3] = cplx<R>{a2.re + b2.im, a2.im - b2.re};
+  }
+};
 template <typename R, int P, int Q, int S>
 __device__ __forceinline__ void fs_ct(cplx<R> (&x)[P * Q]) {
   constexpr FsTw<P * Q, S> tw{};
@@ -2681,85 +2710,98 @@ struct FsDft<R, 12, S> {
   static __device__ __forceinline__ void run(cplx<R> (&x)[12]) { fs_ct<R, 3, 4, S>(x); }
 };
 template <typename R, int S>
+struct FsDft<R, 10, S> {
+  static __device__ __forceinline__ void run(cplx<R> (&x)[10]) { fs_ct<R, 2, 5, S>(x); }
+};
+template <typename R, int S>
+struct FsDft<R, 20, S> {
+  static __device__ __forceinline__ void run(cplx<R> (&x)[20]) { fs_ct<R, 4, 5, S>(x); }
+};
+template <typename R, int S>
+struct FsDft<R, 24, S> {
+  static __device__ __forceinline__ void run(cplx<R> (&x)[24]) { fs_ct<R, 8, 3, S>(x); }
+};
+template <typename R, int S>
 struct FsDft<R, 16, S> {
   static __device__ __forceinline__ void run(cplx<R> (&x)[16]) { fs_ct<R, 4, 4, S>(x); }
 };
 
-// twiddle tables of the inverse transform: tw[j] = wG^j (j < kFsG), then w96^j, then w128^j
-constexpr int kFsTw = kFsG + kFsN1 + kFsN2;
-__global__ void k_fs_twiddles(cplx<double>* __restrict__ tw) {
+// twiddle table of the inverse transform (fs_tw_len(m) entries)
+__global__ void k_fs_twiddles(cplx<double>* __restrict__ tw, int m) {
   const int j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= kFsTw) return;
-  int n, m;
-  if (j < kFsG) { n = kFsG; m = j; }
-  else if (j < kFsG + kFsN1) { n = kFsN1; m = j - kFsG; }
-  else { n = kFsN2; m = j - kFsG - kFsN1; }
+  const int n1 = 8 * m, G = n1 * kFsN2;
+  if (j >= G + n1 + kFsN2) return;
+  int n, e;
+  if (j < G) { n = G; e = j; }
+  else if (j < G + n1) { n = n1; e = j - G; }
+  else { n = kFsN2; e = j - G - n1; }
   double sn, cs;
-  sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+  sincospi(2.0 * (double)e / (double)n, &sn, &cs);
   tw[j] = cplx<double>{cs, sn};
 }
 
-// stage 1 (invert): block = (n2, 16 image columns, plane i); 192 threads
-template <typename R>
-__global__ __launch_bounds__(192) void k_fs_inv1(KP k, int xlo, int xhi, const cplx<R>* __restrict__ plane,
-                                                 cplx<R>* __restrict__ U, const cplx<R>* __restrict__ tw) {
-  __shared__ cplx<R> s[16][kFsN1 + 1];
+// stage 1 (invert): block = (n2, 16 image columns, plane i); 16 M threads
+template <typename R, int M>
+__global__ __launch_bounds__(16 * M) void k_fs_inv1(KP k, int xlo, int xhi, const cplx<R>* __restrict__ plane,
+                                                    cplx<R>* __restrict__ U, const cplx<R>* __restrict__ tw) {
+  constexpr int N1 = 8 * M, NT = 16 * M, G = N1 * kFsN2;
+  __shared__ cplx<R> s[16][N1 + 1];
   const int n2 = blockIdx.x, iy0 = blockIdx.y * 16, i = blockIdx.z;
   const int t = threadIdx.x;
   {
     const int c16 = t & 15, iy = iy0 + c16;
     const bool ok = iy < k.ny;
     const int col = ok ? img_col_to_grid(iy, k.ny, k.gy) : 0;
-    // 8 loads in flight per thread (rows n1 = t / 16 + 12 a), then the LDS writes
+    // 8 loads in flight per thread (rows n1 = t / 16 + M a), then the LDS writes
     const cplx<R>* pc = plane + (int64_t)i * k.gx * k.gy + col;
     cplx<R> v[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-      const int x = kFsN2 * ((t >> 4) + 12 * a) + n2;
+      const int x = kFsN2 * ((t >> 4) + M * a) + n2;
       const bool in = ok && x >= xlo && x < xhi;
       v[a] = pc[in ? (int64_t)x * k.gy : 0];
       if (!in) v[a] = cplx<R>{0, 0};
     }
 #pragma unroll
-    for (int a = 0; a < 8; ++a) s[c16][(t >> 4) + 12 * a] = v[a];
+    for (int a = 0; a < 8; ++a) s[c16][(t >> 4) + M * a] = v[a];
   }
   __syncthreads();
   {
-    // radix 8 over a of n1 = 12 a + q, then the twiddle w96^(q d)
-    const int c = t / 12, q = t % 12;
+    // radix 8 over a of n1 = M a + q, then the twiddle wN1^(q d)
+    const int c = t / M, q = t % M;
     cplx<R> x[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) x[a] = s[c][12 * a + q];
+    for (int a = 0; a < 8; ++a) x[a] = s[c][M * a + q];
     FsDft<R, 8, 1>::run(x);
 #pragma unroll
-    for (int d = 1; d < 8; ++d) x[d] = fs_mul(x[d], tw[kFsG + (q * d) % kFsN1]);
+    for (int d = 1; d < 8; ++d) x[d] = fs_mul(x[d], tw[G + (q * d) % N1]);
     __syncthreads();
 #pragma unroll
-    for (int d = 0; d < 8; ++d) s[c][12 * d + q] = x[d];
+    for (int d = 0; d < 8; ++d) s[c][M * d + q] = x[d];
   }
   __syncthreads();
   {
-    // radix 12 over q for each d: k1 = d + 8 e; then wG^(n2 k1)
+    // radix M over q for each d: k1 = d + 8 e; then wG^(n2 k1)
     const int c = (t >> 3) & 15, d = t & 7;
-    cplx<R> x[12];
+    cplx<R> x[M];
     if (t < 128) {
 #pragma unroll
-      for (int q = 0; q < 12; ++q) x[q] = s[c][12 * d + q];
-      FsDft<R, 12, 1>::run(x);
+      for (int q = 0; q < M; ++q) x[q] = s[c][M * d + q];
+      FsDft<R, M, 1>::run(x);
     }
     __syncthreads();
     if (t < 128) {
 #pragma unroll
-      for (int e = 0; e < 12; ++e) {
+      for (int e = 0; e < M; ++e) {
         const int k1 = d + 8 * e;
-        s[c][k1] = fs_mul(x[e], tw[(n2 * k1) % kFsG]);
+        s[c][k1] = fs_mul(x[e], tw[(n2 * k1) % G]);
       }
     }
   }
   __syncthreads();
-  for (int e = t; e < 16 * kFsN1; e += 192) {
-    const int c = e / kFsN1, k1 = e - c * kFsN1;
-    if (iy0 + c < k.ny) U[(((int64_t)i * k.ny + iy0 + c) * kFsN2 + n2) * kFsN1 + k1] = s[c][k1];
+  for (int e = t; e < 16 * N1; e += NT) {
+    const int c = e / N1, k1 = e - c * N1;
+    if (iy0 + c < k.ny) U[(((int64_t)i * k.ny + iy0 + c) * kFsN2 + n2) * N1 + k1] = s[c][k1];
   }
 }
 
@@ -2770,15 +2812,16 @@ __global__ __launch_bounds__(192) void k_fs_inv1(KP k, int xlo, int xhi, const c
 // (-1)^(tx+ty) Re(X exp(-2 pi i w nm1s)) of the image pixels to the accumulator (first != 0
 // stores instead)
 template <typename R, typename A, bool F32C>
-__global__ __launch_bounds__(256) void k_fs_inv2(KP k, double w, const cplx<R>* __restrict__ U,
+__global__ __launch_bounds__(256) void k_fs_inv2(KP k, int N1, double w, const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ tw, A* __restrict__ acc, int first) {
   __shared__ cplx<R> s[16][kFsN2 + 1];
   const int kb = blockIdx.x * 16, iy = blockIdx.y, b = blockIdx.z;
   const int t = threadIdx.x;
-  const cplx<R>* u = U + ((int64_t)b * k.ny + iy) * kFsG;
+  const int G = N1 * kFsN2;
+  const cplx<R>* u = U + ((int64_t)b * k.ny + iy) * G;
   for (int e = t; e < 16 * kFsN2; e += 256) {
     const int n2 = e >> 4, c = e & 15;
-    s[c][n2] = u[n2 * kFsN1 + kb + c];
+    s[c][n2] = u[n2 * N1 + kb + c];
   }
   __syncthreads();
   {
@@ -2790,7 +2833,7 @@ __global__ __launch_bounds__(256) void k_fs_inv2(KP k, double w, const cplx<R>* 
       for (int pp = 0; pp < 16; ++pp) x[pp] = s[c][8 * pp + r];
       FsDft<R, 16, 1>::run(x);
 #pragma unroll
-      for (int j = 1; j < 16; ++j) x[j] = fs_mul(x[j], tw[kFsG + kFsN1 + (r * j) % kFsN2]);
+      for (int j = 1; j < 16; ++j) x[j] = fs_mul(x[j], tw[G + N1 + (r * j) % kFsN2]);
     }
     __syncthreads();
     if (t < 128) {
@@ -2810,7 +2853,7 @@ __global__ __launch_bounds__(256) void k_fs_inv2(KP k, double w, const cplx<R>* 
   A* arow = acc + ((int64_t)b * k.ny + iy) * k.nx + hx;
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const int gxi = kb + c2 + kFsN1 * (j2 + 16 * e);
+    const int gxi = kb + c2 + N1 * (j2 + 16 * e);
     if (gxi >= hx && gxi < k.gx - hx) continue;
     const int tx = gxi < hx ? gxi : gxi - k.gx;
     R c = 1, sn = 0;
@@ -2822,9 +2865,9 @@ __global__ __launch_bounds__(256) void k_fs_inv2(KP k, double w, const cplx<R>* 
   }
 }
 
-// predict, the adjoint split (w_N^-1 = conj(w_N)): input g = k1 + kFsN1 k2 (image side),
+// predict, the adjoint split (w_N^-1 = conj(w_N)): input g = k1 + N1 k2 (image side),
 // output x = kFsN2 n1 + n2 (grid side):
-//   X[kFsN2 n1 + n2] = sum_k1 w96^-(k1 n1) [ wG^-(k1 n2) sum_k2 w128^-(k2 n2) T[k1 + kFsN1 k2] ]
+//   X[kFsN2 n1 + n2] = sum_k1 wN1^-(k1 n1) [ wG^-(k1 n2) sum_k2 w128^-(k2 n2) T[k1 + N1 k2] ]
 template <typename R>
 __device__ __forceinline__ cplx<R> fs_conj(cplx<R> a) {
   return cplx<R>{a.re, -a.im};
@@ -2835,10 +2878,11 @@ __device__ __forceinline__ cplx<R> fs_conj(cplx<R> a) {
 // on the others) as it loads, transforms along k2 (128 = 8 x 16: k2 = j + 16 e,
 // n2 = a + 8 b'), applies wG^-(k1 n2) and writes V[iy][n2][k1]
 template <typename R, typename A, bool F32C>
-__global__ __launch_bounds__(256) void k_fs_fwd2(KP k, double w, const A* __restrict__ dc, cplx<R>* __restrict__ V,
+__global__ __launch_bounds__(256) void k_fs_fwd2(KP k, int N1, double w, const A* __restrict__ dc, cplx<R>* __restrict__ V,
                                                  const cplx<R>* __restrict__ tw) {
   __shared__ cplx<R> s[16][kFsN2 + 1];
   const int kb = blockIdx.x * 16, iy = blockIdx.y, b = blockIdx.z;
+  const int G = N1 * kFsN2;
   const int t = threadIdx.x;
   const int c2 = t & 15, j2 = t >> 4;  // lanes run along k1: 128-byte pieces of the image rows
   const int ty = iy - k.ny / 2, hx = k.nx / 2;
@@ -2846,7 +2890,7 @@ __global__ __launch_bounds__(256) void k_fs_fwd2(KP k, double w, const A* __rest
   cplx<R> x[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const int gxi = kb + c2 + kFsN1 * (j2 + 16 * e);
+    const int gxi = kb + c2 + N1 * (j2 + 16 * e);
     x[e] = cplx<R>{0, 0};
     if (gxi < hx || gxi >= k.gx - hx) {
       const int tx = gxi < hx ? gxi : gxi - k.gx;
@@ -2860,7 +2904,7 @@ __global__ __launch_bounds__(256) void k_fs_fwd2(KP k, double w, const A* __rest
   // radix 8 over e, then the twiddle w128^-(j a); Y[j][a] goes to 16 a + j
   FsDft<R, 8, -1>::run(x);
 #pragma unroll
-  for (int a2 = 1; a2 < 8; ++a2) x[a2] = fs_mul(x[a2], fs_conj(tw[kFsG + kFsN1 + (j2 * a2) % kFsN2]));
+  for (int a2 = 1; a2 < 8; ++a2) x[a2] = fs_mul(x[a2], fs_conj(tw[G + N1 + (j2 * a2) % kFsN2]));
 #pragma unroll
   for (int a2 = 0; a2 < 8; ++a2) s[c2][16 * a2 + j2] = x[a2];
   __syncthreads();
@@ -2878,62 +2922,63 @@ __global__ __launch_bounds__(256) void k_fs_fwd2(KP k, double w, const A* __rest
 #pragma unroll
       for (int bb = 0; bb < 16; ++bb) {
         const int n2 = a2 + 8 * bb;
-        s[c][n2] = fs_mul(y[bb], fs_conj(tw[((kb + c) * n2) % kFsG]));
+        s[c][n2] = fs_mul(y[bb], fs_conj(tw[((kb + c) * n2) % G]));
       }
     }
   }
   __syncthreads();
-  cplx<R>* v = V + ((int64_t)b * k.ny + iy) * kFsG;
+  cplx<R>* v = V + ((int64_t)b * k.ny + iy) * G;
   for (int e = t; e < 16 * kFsN2; e += 256) {
     const int n2 = e >> 4, c = e & 15;
-    v[n2 * kFsN1 + kb + c] = s[c][n2];
+    v[n2 * N1 + kb + c] = s[c][n2];
   }
 }
 
-// stage 1' (predict): block = (n2, 16 image columns, plane i); 192 threads. Transforms
-// V[i][iy][n2][.] along k1 (96 = 8 x 12: k1 = d + 8 e, n1 = q + 12 a) and writes the band rows
+// stage 1' (predict): block = (n2, 16 image columns, plane i); 16 M threads. Transforms
+// V[i][iy][n2][.] along k1 (N1 = 8 M: k1 = d + 8 e, n1 = q + M a) and writes the band rows
 // x = kFsN2 n1 + n2 in [xlo, xhi) of the image columns of plane i (the other columns of the
 // band rows are left as they are: zero)
-template <typename R>
-__global__ __launch_bounds__(192) void k_fs_fwd1(KP k, int xlo, int xhi, const cplx<R>* __restrict__ V,
-                                                 cplx<R>* __restrict__ plane, const cplx<R>* __restrict__ tw) {
-  __shared__ cplx<R> s[16][kFsN1 + 1];
+template <typename R, int M>
+__global__ __launch_bounds__(16 * M) void k_fs_fwd1(KP k, int xlo, int xhi, const cplx<R>* __restrict__ V,
+                                                    cplx<R>* __restrict__ plane, const cplx<R>* __restrict__ tw) {
+  constexpr int N1 = 8 * M, NT = 16 * M, G = N1 * kFsN2;
+  __shared__ cplx<R> s[16][N1 + 1];
   const int n2 = blockIdx.x, iy0 = blockIdx.y * 16, i = blockIdx.z;
   const int t = threadIdx.x;
-  for (int e = t; e < 16 * kFsN1; e += 192) {
-    const int c = e / kFsN1, k1 = e - c * kFsN1;
-    s[c][k1] = iy0 + c < k.ny ? V[(((int64_t)i * k.ny + iy0 + c) * kFsN2 + n2) * kFsN1 + k1] : cplx<R>{0, 0};
+  for (int e = t; e < 16 * N1; e += NT) {
+    const int c = e / N1, k1 = e - c * N1;
+    s[c][k1] = iy0 + c < k.ny ? V[(((int64_t)i * k.ny + iy0 + c) * kFsN2 + n2) * N1 + k1] : cplx<R>{0, 0};
   }
   __syncthreads();
   {
-    // radix 12 over e for each d (threads t < 128), then the twiddle w96^-(d q); Y[d][q] goes
-    // to 12 d + q
+    // radix M over e for each d (threads t < 128), then the twiddle wN1^-(d q); Y[d][q] goes
+    // to M d + q
     const int c = (t >> 3) & 15, d = t & 7;
-    cplx<R> x[12];
+    cplx<R> x[M];
     if (t < 128) {
 #pragma unroll
-      for (int e = 0; e < 12; ++e) x[e] = s[c][d + 8 * e];
-      FsDft<R, 12, -1>::run(x);
+      for (int e = 0; e < M; ++e) x[e] = s[c][d + 8 * e];
+      FsDft<R, M, -1>::run(x);
 #pragma unroll
-      for (int q = 1; q < 12; ++q) x[q] = fs_mul(x[q], fs_conj(tw[kFsG + (d * q) % kFsN1]));
+      for (int q = 1; q < M; ++q) x[q] = fs_mul(x[q], fs_conj(tw[G + (d * q) % N1]));
     }
     __syncthreads();
     if (t < 128) {
 #pragma unroll
-      for (int q = 0; q < 12; ++q) s[c][12 * d + q] = x[q];
+      for (int q = 0; q < M; ++q) s[c][M * d + q] = x[q];
     }
   }
   __syncthreads();
   {
-    // radix 8 over d for each q: n1 = q + 12 a
-    const int c = t / 12, q = t % 12;
+    // radix 8 over d for each q: n1 = q + M a
+    const int c = t / M, q = t % M;
     cplx<R> x[8];
 #pragma unroll
-    for (int d = 0; d < 8; ++d) x[d] = s[c][12 * d + q];
+    for (int d = 0; d < 8; ++d) x[d] = s[c][M * d + q];
     FsDft<R, 8, -1>::run(x);
     __syncthreads();
 #pragma unroll
-    for (int a2 = 0; a2 < 8; ++a2) s[c][q + 12 * a2] = x[a2];
+    for (int a2 = 0; a2 < 8; ++a2) s[c][q + M * a2] = x[a2];
   }
   __syncthreads();
   const int c16 = t & 15, iy = iy0 + c16;
@@ -2941,8 +2986,30 @@ __global__ __launch_bounds__(192) void k_fs_fwd1(KP k, int xlo, int xhi, const c
   cplx<R>* pc = plane + (int64_t)i * k.gx * k.gy + img_col_to_grid(iy, k.ny, k.gy);
 #pragma unroll
   for (int a2 = 0; a2 < 8; ++a2) {
-    const int n1 = (t >> 4) + 12 * a2, x = kFsN2 * n1 + n2;
+    const int n1 = (t >> 4) + M * a2, x = kFsN2 * n1 + n2;
     if (x >= xlo && x < xhi) pc[(int64_t)x * k.gy] = s[c16][n1];
+  }
+}
+
+// launches of the stages templated on M
+template <typename R, int M>
+void fs_stage1_m(bool fwd, const KP& k, int xlo, int xhi, int nplanes, const cplx<R>* in, cplx<R>* out,
+                 const cplx<R>* tw, hipStream_t st) {
+  const dim3 grid(kFsN2, (k.ny + 15) / 16, nplanes);
+  if (fwd) k_fs_fwd1<R, M><<<grid, 16 * M, 0, st>>>(k, xlo, xhi, in, out, tw);
+  else k_fs_inv1<R, M><<<grid, 16 * M, 0, st>>>(k, xlo, xhi, in, out, tw);
+}
+// stage 1 (fwd false: band plane -> U) or stage 1' (fwd true: V -> band plane) over nplanes
+template <typename R>
+void fs_stage1(int m, bool fwd, const KP& k, int xlo, int xhi, int nplanes, const cplx<R>* in, cplx<R>* out,
+               const cplx<R>* tw, hipStream_t st) {
+  switch (m) {
+    case 10: fs_stage1_m<R, 10>(fwd, k, xlo, xhi, nplanes, in, out, tw, st); break;
+    case 12: fs_stage1_m<R, 12>(fwd, k, xlo, xhi, nplanes, in, out, tw, st); break;
+    case 16: fs_stage1_m<R, 16>(fwd, k, xlo, xhi, nplanes, in, out, tw, st); break;
+    case 20: fs_stage1_m<R, 20>(fwd, k, xlo, xhi, nplanes, in, out, tw, st); break;
+    case 24: fs_stage1_m<R, 24>(fwd, k, xlo, xhi, nplanes, in, out, tw, st); break;
+    default: HIPNG_THROW(HIPNG_ERUNTIME, "four-step transform: no stage for M = %d", m);
   }
 }
 
@@ -4704,7 +4771,8 @@ struct Geom {
   // FP32 planes: predict's band transforms go out of place into plane2, so the non-image
   // columns of the band rows in plane stay zero through a call
   void* plane2 = nullptr;
-  cplx<double>* fstw = nullptr;  // four-step twiddles (FP64 planes of 12288 points along x)
+  cplx<double>* fstw = nullptr;  // four-step twiddles (FP64 planes of 8 fsm x 128 points along x)
+  int fsm = 0;
   rocfft_plan fft_b_fwd_oop = nullptr;
   // ... and invert's, with lastw[tile] = the last pass that gridded into the tile
   rocfft_plan fft_b_inv_oop = nullptr;
@@ -4894,9 +4962,12 @@ void build_geom(Geom& g, DevCtx& c, const double* uvw, const double* freq, int64
     g.fft_b_inv_oop = fft_plan1(c, P.gy, g.xhi - g.xlo, prec, 0, false);
     g.lastw = (uint32_t*)g.buf.get(S_LASTW, (size_t)P.ntx * P.nty * 4);
   }
-  if (csz == 16 && P.gx == kFsG) {
-    g.fstw = (cplx<double>*)g.buf.get(S_FSTW, (size_t)kFsTw * sizeof(cplx<double>));
-    k_fs_twiddles<<<(kFsTw + kBlock - 1) / kBlock, kBlock, 0, st>>>(g.fstw);
+  g.fsm = csz == 16 ? fs_m_for(P.gx) : 0;
+  g.fstw = nullptr;
+  if (g.fsm) {
+    const int ntw = fs_tw_len(g.fsm);
+    g.fstw = (cplx<double>*)g.buf.get(S_FSTW, (size_t)ntw * sizeof(cplx<double>));
+    k_fs_twiddles<<<(ntw + kBlock - 1) / kBlock, kBlock, 0, st>>>(g.fstw, g.fsm);
     LAUNCH_CHECK();
   }
   g.fftwork_size = 0;
@@ -5302,8 +5373,8 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   auto* plane = (cplx<R>*)g.plane;
   auto* tbuf = (cplx<R>*)g.tbuf;
   auto* tbuf2 = (cplx<R>*)g.tbuf2;
-  // FP64 planes of 12288 points along x: the four-step image-side transform (stage 1 writes
-  // all of its buffer, so T needs no clearing)
+  // FP64 planes of a length the four-step image-side transform covers (stage 1 writes all of
+  // its buffer, so T needs no clearing)
   const bool fs = sizeof(R) == 8 && g.fstw != nullptr;
   if (!fs) {
     // the per-pass kernels write only the band (invert) or the image (predict) part of T
@@ -5400,17 +5471,17 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
     if constexpr (sizeof(R) == 8) {
       if (fs) {
         const auto* tw = reinterpret_cast<const cplx<R>*>(g.fstw);
+        const int n1 = 8 * g.fsm;
         {
           PROF("fourstep_1");
-          k_fs_inv1<R><<<dim3(kFsN2, (P.ny + 15) / 16, npl * g.nb), 192, 0, st>>>(k, g.xlo, g.xhi, bplane, tbuf,
-                                                                                 tw);
+          fs_stage1<R>(g.fsm, false, k, g.xlo, g.xhi, npl * g.nb, bplane, tbuf, tw, st);
           LAUNCH_CHECK();
         }
         {
           PROF("fourstep_2");
           for (int q = 0; q < npl; ++q) {
-            k_fs_inv2<R, A, F32C><<<dim3(kFsN1 / 16, P.ny, g.nb), 256, 0, st>>>(
-                k, k.w0 + (p0 + q) * k.dw, tbuf + (int64_t)q * g.nb * P.ny * kFsG, tw, acc, p0 + q == 0 ? 1 : 0);
+            k_fs_inv2<R, A, F32C><<<dim3(n1 / 16, P.ny, g.nb), 256, 0, st>>>(
+                k, n1, k.w0 + (p0 + q) * k.dw, tbuf + (int64_t)q * g.nb * P.ny * P.gx, tw, acc, p0 + q == 0 ? 1 : 0);
             LAUNCH_CHECK();
           }
         }
@@ -5503,9 +5574,9 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
   // image columns are rewritten per pass
   const bool oop = g.plane2 != nullptr && g.oop_ok;
   cplx<R>* bplane = oop ? (cplx<R>*)g.plane2 : plane;
-  // FP64 planes of 12288 points along x with out-of-place band transforms: the four-step
-  // image-side transform writes the image columns of the band rows itself (no S)
-  const bool fs = sizeof(R) == 8 && g.fstw != nullptr && oop;
+  // FP64 planes of a length the four-step image-side transform covers: it writes the image
+  // columns of the band rows itself (no S)
+  const bool fs = sizeof(R) == 8 && g.fstw != nullptr;
   if (!fs) {
     PROF("transpose");
     // the fills rewrite the image columns of S; the others stay zero
@@ -5527,18 +5598,25 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     if constexpr (sizeof(R) == 8) {
       if (fs) {
         const auto* tw = reinterpret_cast<const cplx<R>*>(g.fstw);
+        const int n1 = 8 * g.fsm;
         {
           PROF("fourstep_2");
           for (int q = 0; q < npl; ++q) {
-            k_fs_fwd2<R, A, F32C><<<dim3(kFsN1 / 16, P.ny, g.nb), 256, 0, st>>>(
-                k, k.w0 + (p0 + q) * k.dw, dc, tbuf + (int64_t)q * g.nb * P.ny * kFsG, tw);
+            k_fs_fwd2<R, A, F32C><<<dim3(n1 / 16, P.ny, g.nb), 256, 0, st>>>(
+                k, n1, k.w0 + (p0 + q) * k.dw, dc, tbuf + (int64_t)q * g.nb * P.ny * P.gx, tw);
             LAUNCH_CHECK();
           }
         }
         {
           PROF("fourstep_1");
-          k_fs_fwd1<R><<<dim3(kFsN2, (P.ny + 15) / 16, npl * g.nb), 192, 0, st>>>(k, g.xlo, g.xhi, tbuf, plane,
-                                                                                 tw);
+          if (!oop) {
+            // the in-place band transforms of the previous pass wrote every column of the band
+            // rows: clear the others again
+            for (int i = 0; i < npl * g.nb; ++i)
+              zero_cols(plane + ((int64_t)i * P.gx + g.xlo) * P.gy, (int64_t)(g.xhi - g.xlo), P.gy, P.ny / 2,
+                        P.gy - P.ny / 2, st);
+          }
+          fs_stage1<R>(g.fsm, true, k, g.xlo, g.xhi, npl * g.nb, tbuf, plane, tw, st);
           LAUNCH_CHECK();
         }
         fs_done = true;

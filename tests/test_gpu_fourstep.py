@@ -1,6 +1,7 @@
-"""The FP64 image-side transform of 12288-point grids (an 8192^2 image at oversampling 1.5):
-invert and predict run it as a four-step transform (96 x 128) fused with the w-screen instead
-of transpose + rocFFT + screen. Checked through the C-ABI against the FP64 brute-force sums
+"""The FP64 image-side transform of grids of G = 8 M x 128 points, M in {10, 12, 16, 20, 24}
+(8192^2 images at oversampling 1.25, 1.5, 2 and 2.5; a 16384^2 image at 1.5 for 24576):
+invert and predict run it as a four-step transform (8 M x 128) fused with the w-screen
+instead of transpose + rocFFT + screen. Checked through the C-ABI against the FP64 brute-force sums
 (oracle): invert at probe pixels, predict of images that are zero off a set of pixels. The
 pixels are the image edges, the centre, the band limits of the grid and random ones; w-stacking
 off and on; one and two polarisations."""
@@ -15,6 +16,10 @@ from rascil_amd import _lib, wgridder
 pytestmark = pytest.mark.gpu
 
 N, PIX, EPS = 8192, 1.1e-5, 1e-7
+# (image side, oversampling, grid length): every M of the four-step, and both w-stacking modes
+# at the production length 12288
+GRIDS = [(8192, 1.25, 10240, True), (8192, 1.5, 12288, False), (8192, 1.5, 12288, True),
+         (8192, 2.0, 16384, True), (8192, 2.5, 20480, True), (16384, 1.5, 24576, True)]
 
 
 def _obs(seed, nrow=60):
@@ -26,36 +31,37 @@ def _obs(seed, nrow=60):
     return rng, uvw, freq
 
 
-def _probes(rng):
+def _probes(rng, n=N):
     """distinct pixels: the 8 x 8 combinations of edge and centre rows / columns, 400 random"""
-    edge = numpy.array([0, 1, 2, N // 2 - 1, N // 2, N // 2 + 1, N - 2, N - 1])
+    edge = numpy.array([0, 1, 2, n // 2 - 1, n // 2, n // 2 + 1, n - 2, n - 1])
     ex, ey = numpy.meshgrid(edge, edge, indexing="ij")
-    lin = numpy.unique(numpy.concatenate([(ex * N + ey).ravel(), rng.integers(0, N * N, 400)]))
-    return lin // N, lin % N
+    lin = numpy.unique(numpy.concatenate([(ex * n + ey).ravel(), rng.integers(0, n * n, 400)]))
+    return lin // n, lin % n
 
 
-@pytest.mark.parametrize("wst", [False, True])
-def test_fourstep_invert_one_polarisation(wst):
+@pytest.mark.parametrize("n,sigma,g,wst", GRIDS)
+def test_fourstep_invert_one_polarisation(n, sigma, g, wst):
     rng, uvw, freq = _obs(21)
+    pix = PIX * N / n
     ms = rng.normal(size=(uvw.shape[0], 2)) + 1j * rng.normal(size=(uvw.shape[0], 2))
     wgt = rng.uniform(0.5, 1.5, ms.shape)
     plan = {}
     _lib.profile_enable(True)
     try:
-        d = wgridder.ms2dirty(uvw, freq, ms, wgt, N, N, PIX, PIX, 0, 0, EPS, wst, precision="fp64",
-                              plan_out=plan, plan_opts={"sigma": 1.5})
+        d = wgridder.ms2dirty(uvw, freq, ms, wgt, n, n, pix, pix, 0, 0, EPS, wst, precision="fp64",
+                              plan_out=plan, plan_opts={"sigma": sigma})
         prof = _lib.profile_read()
     finally:
         _lib.profile_enable(False)
-    assert plan["gx"] == 12288 and plan["gy"] == 12288
+    assert plan["gx"] == g and plan["gy"] == g
     # the four-step kernels ran in every pass (one profile entry per pass), and the transposes
     # and screen of the rocFFT image pass did not
     assert prof["fourstep_1"][1] >= 1 and prof["fourstep_2"][1] == prof["fourstep_1"][1]
     assert "transpose" not in prof and "screen_acc" not in prof
-    ix, iy = _probes(rng)
-    ref = ox.explicit_ms2dirty_at(uvw, freq, ms, wgt, ix, iy, N, N, PIX, PIX, wst)
+    ix, iy = _probes(rng, n)
+    ref = ox.explicit_ms2dirty_at(uvw, freq, ms, wgt, ix, iy, n, n, pix, pix, wst)
     err = rel_l2(d[ix, iy], ref)
-    print(f"wst={wst} planes={plan['nplanes']}: invert {err:.2e} at {ix.size} probe pixels")
+    print(f"G={g} wst={wst} planes={plan['nplanes']}: invert {err:.2e} at {ix.size} probe pixels")
     assert err <= EPS
 
 
@@ -84,28 +90,29 @@ def test_fourstep_invert_two_polarisations():
         geom.close()
 
 
-@pytest.mark.parametrize("wst", [False, True])
-def test_fourstep_predict_one_polarisation(wst):
+@pytest.mark.parametrize("n,sigma,g,wst", GRIDS)
+def test_fourstep_predict_one_polarisation(n, sigma, g, wst):
     rng, uvw, freq = _obs(23)
-    ix, iy = _probes(rng)
+    pix = PIX * N / n
+    ix, iy = _probes(rng, n)
     val = rng.normal(size=ix.size)
-    dirty = numpy.zeros((N, N))
+    dirty = numpy.zeros((n, n))
     dirty[ix, iy] = val
     wgt = rng.uniform(0.5, 1.5, (uvw.shape[0], 2))
     plan = {}
     _lib.profile_enable(True)
     try:
-        v = wgridder.dirty2ms(uvw, freq, dirty, wgt, PIX, PIX, 0, 0, EPS, wst, precision="fp64",
-                              plan_out=plan, plan_opts={"sigma": 1.5})
+        v = wgridder.dirty2ms(uvw, freq, dirty, wgt, pix, pix, 0, 0, EPS, wst, precision="fp64",
+                              plan_out=plan, plan_opts={"sigma": sigma})
         prof = _lib.profile_read()
     finally:
         _lib.profile_enable(False)
-    assert plan["gx"] == 12288 and plan["gy"] == 12288
+    assert plan["gx"] == g and plan["gy"] == g
     assert prof["fourstep_1"][1] >= 1 and prof["fourstep_2"][1] == prof["fourstep_1"][1]
     assert "screen_fill" not in prof
-    ref = ox.explicit_dirty2ms_sparse(uvw, freq, ix, iy, val, N, N, PIX, PIX, wgt, wst)
+    ref = ox.explicit_dirty2ms_sparse(uvw, freq, ix, iy, val, n, n, pix, pix, wgt, wst)
     err = rel_l2(v, ref)
-    print(f"wst={wst} planes={plan['nplanes']}: predict {err:.2e} from {ix.size} pixels")
+    print(f"G={g} wst={wst} planes={plan['nplanes']}: predict {err:.2e} from {ix.size} pixels")
     assert err <= EPS
 
 
