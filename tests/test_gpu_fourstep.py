@@ -20,6 +20,8 @@ N, PIX, EPS = 8192, 1.1e-5, 1e-7
 # at the production length 12288
 GRIDS = [(8192, 1.25, 10240, True), (8192, 1.5, 12288, False), (8192, 1.5, 12288, True),
          (8192, 2.0, 16384, True), (8192, 2.5, 20480, True), (16384, 1.5, 24576, True)]
+# a rectangular image: the x grid takes the four-step (12288), y is 3072 long
+RECT = (8192, 2048, 1.5, 12288, 3072)
 
 
 def _obs(seed, nrow=60):
@@ -31,12 +33,14 @@ def _obs(seed, nrow=60):
     return rng, uvw, freq
 
 
-def _probes(rng, n=N):
-    """distinct pixels: the 8 x 8 combinations of edge and centre rows / columns, 400 random"""
-    edge = numpy.array([0, 1, 2, n // 2 - 1, n // 2, n // 2 + 1, n - 2, n - 1])
-    ex, ey = numpy.meshgrid(edge, edge, indexing="ij")
-    lin = numpy.unique(numpy.concatenate([(ex * n + ey).ravel(), rng.integers(0, n * n, 400)]))
-    return lin // n, lin % n
+def _probes(rng, n=N, m=None):
+    """distinct pixels of an n x m image: the 8 x 8 combinations of edge and centre rows /
+    columns, 400 random"""
+    m = n if m is None else m
+    ex, ey = numpy.meshgrid(numpy.array([0, 1, 2, n // 2 - 1, n // 2, n // 2 + 1, n - 2, n - 1]),
+                            numpy.array([0, 1, 2, m // 2 - 1, m // 2, m // 2 + 1, m - 2, m - 1]), indexing="ij")
+    lin = numpy.unique(numpy.concatenate([(ex * m + ey).ravel(), rng.integers(0, n * m, 400)]))
+    return lin // m, lin % m
 
 
 @pytest.mark.parametrize("n,sigma,g,wst", GRIDS)
@@ -140,3 +144,30 @@ def test_fourstep_predict_two_polarisations():
         assert numpy.array_equal(geom.dirty2ms(md).cpu().numpy(), v)
     finally:
         geom.close()
+
+
+def test_fourstep_rectangular_image():
+    nx, ny, sigma, gx, gy = RECT
+    rng, uvw, freq = _obs(25)
+    ms = rng.normal(size=(uvw.shape[0], 2)) + 1j * rng.normal(size=(uvw.shape[0], 2))
+    wgt = rng.uniform(0.5, 1.5, ms.shape)
+    ix, iy = _probes(rng, nx, ny)
+    plan = {}
+    _lib.profile_enable(True)
+    try:
+        d = wgridder.ms2dirty(uvw, freq, ms, wgt, nx, ny, PIX, PIX, 0, 0, EPS, True, precision="fp64",
+                              plan_out=plan, plan_opts={"sigma": sigma})
+        val = rng.normal(size=ix.size)
+        dirty = numpy.zeros((nx, ny))
+        dirty[ix, iy] = val
+        v = wgridder.dirty2ms(uvw, freq, dirty, wgt, PIX, PIX, 0, 0, EPS, True, precision="fp64",
+                              plan_opts={"sigma": sigma})
+        prof = _lib.profile_read()
+    finally:
+        _lib.profile_enable(False)
+    assert plan["gx"] == gx and plan["gy"] == gy
+    assert prof["fourstep_1"][1] >= 2 and "screen_fill" not in prof and "screen_acc" not in prof
+    ei = rel_l2(d[ix, iy], ox.explicit_ms2dirty_at(uvw, freq, ms, wgt, ix, iy, nx, ny, PIX, PIX, True))
+    ep = rel_l2(v, ox.explicit_dirty2ms_sparse(uvw, freq, ix, iy, val, nx, ny, PIX, PIX, wgt, True))
+    print(f"{nx} x {ny}: invert {ei:.2e}, predict {ep:.2e}")
+    assert ei <= EPS and ep <= EPS
