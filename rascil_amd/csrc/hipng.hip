@@ -2819,9 +2819,13 @@ __global__ __launch_bounds__(256) void k_fs_inv2(KP k, int N1, double w, const c
   const int t = threadIdx.x;
   const int G = N1 * kFsN2;
   const cplx<R>* u = U + ((int64_t)b * k.ny + iy) * G;
-  for (int e = t; e < 16 * kFsN2; e += 256) {
-    const int n2 = e >> 4, c = e & 15;
-    s[c][n2] = u[n2 * N1 + kb + c];
+  {
+    // 8 loads in flight per thread, then the LDS writes
+    cplx<R> v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = u[((t >> 4) + 16 * i) * N1 + kb + (t & 15)];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s[t & 15][(t >> 4) + 16 * i] = v[i];
   }
   __syncthreads();
   {
@@ -2832,8 +2836,14 @@ __global__ __launch_bounds__(256) void k_fs_inv2(KP k, int N1, double w, const c
 #pragma unroll
       for (int pp = 0; pp < 16; ++pp) x[pp] = s[c][8 * pp + r];
       FsDft<R, 16, 1>::run(x);
+      // w128^(r j) = (w128^r)^j: one table read, then products
+      const cplx<R> step = tw[G + N1 + r];
+      cplx<R> cur = step;
 #pragma unroll
-      for (int j = 1; j < 16; ++j) x[j] = fs_mul(x[j], tw[G + N1 + (r * j) % kFsN2]);
+      for (int j = 1; j < 16; ++j) {
+        x[j] = fs_mul(x[j], cur);
+        cur = fs_mul(cur, step);
+      }
     }
     __syncthreads();
     if (t < 128) {
@@ -2851,17 +2861,29 @@ __global__ __launch_bounds__(256) void k_fs_inv2(KP k, int N1, double w, const c
   for (int r = 0; r < 8; ++r) x[r] = s[c2][8 * j2 + r];
   FsDft<R, 8, 1>::run(x);
   A* arow = acc + ((int64_t)b * k.ny + iy) * k.nx + hx;
+  // the accumulator reads go out together (pixels off the image read pixel 0 and are dropped)
+  A av[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int gxi = kb + c2 + N1 * (j2 + 16 * e);
-    if (gxi >= hx && gxi < k.gx - hx) continue;
+    const bool in = gxi < hx || gxi >= k.gx - hx;
+    av[e] = first ? (A)0 : arow[in ? (gxi < hx ? gxi : gxi - k.gx) : 0];
+  }
+  // all sums first, then the stores (a store ahead of a use would hold the use's wait, as
+  // the memory counter covers both)
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int gxi = kb + c2 + N1 * (j2 + 16 * e);
     const int tx = gxi < hx ? gxi : gxi - k.gx;
     R c = 1, sn = 0;
-    if (k.wstack) phase_cs<R, F32C>(-w * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &sn);
+    if (k.wstack && (gxi < hx || gxi >= k.gx - hx)) phase_cs<R, F32C>(-w * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &sn);
     const R re = x[e].re * c - x[e].im * sn;
-    const A v = ((tx + ty) & 1) ? -(A)re : (A)re;
-    A* a = arow + tx;
-    *a = first ? v : *a + v;
+    av[e] += ((tx + ty) & 1) ? -(A)re : (A)re;
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int gxi = kb + c2 + N1 * (j2 + 16 * e);
+    if (gxi < hx || gxi >= k.gx - hx) arow[gxi < hx ? gxi : gxi - k.gx] = av[e];
   }
 }
 
@@ -2887,6 +2909,14 @@ __global__ __launch_bounds__(256) void k_fs_fwd2(KP k, int N1, double w, const A
   const int c2 = t & 15, j2 = t >> 4;  // lanes run along k1: 128-byte pieces of the image rows
   const int ty = iy - k.ny / 2, hx = k.nx / 2;
   const A* drow = dc + ((int64_t)b * k.ny + iy) * k.nx + hx;
+  // the 8 image loads go out together (pixels off the image read pixel 0 and are dropped)
+  A dv[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int gxi = kb + c2 + N1 * (j2 + 16 * e);
+    const bool in = gxi < hx || gxi >= k.gx - hx;
+    dv[e] = drow[in ? (gxi < hx ? gxi : gxi - k.gx) : 0];
+  }
   cplx<R> x[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
@@ -2894,7 +2924,7 @@ __global__ __launch_bounds__(256) void k_fs_fwd2(KP k, int N1, double w, const A
     x[e] = cplx<R>{0, 0};
     if (gxi < hx || gxi >= k.gx - hx) {
       const int tx = gxi < hx ? gxi : gxi - k.gx;
-      R d = (R)drow[tx];
+      R d = (R)dv[e];
       if ((tx + ty) & 1) d = -d;
       R c = 1, sn = 0;
       if (k.wstack) phase_cs<R, F32C>(w * nm1s_at<F32C>(tx, ty, k, nullptr), &c, &sn);
@@ -2919,10 +2949,13 @@ __global__ __launch_bounds__(256) void k_fs_fwd2(KP k, int N1, double w, const A
     }
     __syncthreads();
     if (t < 128) {
+      // wG^-(k1 (a + 8 b')) = wG^-(k1 a) (wG^-(8 k1))^b': two table reads, then products
+      const cplx<R> step = fs_conj(tw[(kb + c) * 8]);
+      cplx<R> cur = fs_conj(tw[(kb + c) * a2]);
 #pragma unroll
       for (int bb = 0; bb < 16; ++bb) {
-        const int n2 = a2 + 8 * bb;
-        s[c][n2] = fs_mul(y[bb], fs_conj(tw[((kb + c) * n2) % G]));
+        s[c][a2 + 8 * bb] = fs_mul(y[bb], cur);
+        cur = fs_mul(cur, step);
       }
     }
   }
