@@ -2565,8 +2565,8 @@ __global__ void k_screen_fill_t(KP k, int p0, int npl, const A* __restrict__ dc,
 // (w_N = exp(+2 pi i / N), the inverse transform). Stage 1 reads the band plane in 256-byte
 // strips of 16 image columns (no transpose), transforms along n1 (radix 8, then radix M),
 // applies wG^(n2 k1) and writes U[i][iy][n2][k1]; stage 2 transforms along n2 and adds the
-// screened real values of one plane to the image accumulator. 8192^2 image, G = 12288: 0.61 +
-// 0.50 ms per plane and polarisation against 2.48 ms for transpose, rocFFT and screen.
+// screened real values of one plane to the image accumulator. 8192^2 image, G = 12288: 0.57 +
+// 0.52 ms per plane and polarisation against 2.48 ms for transpose, rocFFT and screen.
 // ---------------------------------------------------------------------------------
 constexpr int kFsN2 = 128;
 // M of a grid length, or 0 when the four-step does not cover it
