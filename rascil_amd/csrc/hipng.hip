@@ -2404,8 +2404,21 @@ __device__ __forceinline__ double kernel_ft(double xi, int W, const double* __re
   return s;
 }
 
+// sum_m c[m] T_m(t) for t in [-1, 1] (Clenshaw)
+__host__ __device__ inline double cheb_eval(const double* c, int n, double t) {
+  double b1 = 0.0, b2 = 0.0;
+  for (int m = n - 1; m >= 1; --m) {
+    const double b0 = 2.0 * t * b1 - b2 + c[m];
+    b2 = b1;
+    b1 = b0;
+  }
+  return t * b1 - b2 + c[0];
+}
+
 // invert: acc[b][ix][iy] += Re(exp(-2 pi i w_p nm1s) (-1)^(tx+ty) plane[b][tx mod G][ty mod G])
-// 1 / (Phi_x Phi_y [Phi_w(dw nm1s) n]) at pixel (ix, iy)
+// 1 / (Phi_x Phi_y [Phi_w(dw nm1s) n]) at pixel (ix, iy). Phi_w comes from the quadrature sum
+// (nq > 0: nodes qz, weights qc) or from a Chebyshev series of log Phi_w over the image's
+// range of dw nm1s (nq < 0: -nq coefficients in qc, qz = {centre, 1 / half-width})
 template <typename A>
 __device__ __forceinline__ A corr_at(int ix, int iy, const KP& k, const double* __restrict__ phix,
                                      const double* __restrict__ phiy, const double* __restrict__ qz,
@@ -2414,7 +2427,7 @@ __device__ __forceinline__ A corr_at(int ix, int iy, const KP& k, const double* 
   if (k.wstack) {
     double n;
     const double xi = k.dw * nm1s_at<false>(ix - k.nx / 2, iy - k.ny / 2, k, &n);
-    den *= kernel_ft(xi, k.W, qz, qc, nq) * n;
+    den *= (nq > 0 ? kernel_ft(xi, k.W, qz, qc, nq) : exp(cheb_eval(qc, -nq, (xi - qz[0]) * qz[1]))) * n;
   }
   return (A)(1.0 / den);
 }
@@ -4635,13 +4648,60 @@ struct Buffers {
 struct Tables {
   double *sx, *sy, *sw, *pn, *phix, *phiy, *qz, *qc, *fc;  // fc[c] = f_c / c0
   int nq;
+  // Phi_w of the grid correction as corr_at takes it: a Chebyshev series (cn < 0) or the
+  // quadrature (cz = qz, cc = qc, cn = nq)
+  double *cz, *cc;
+  int cn;
 };
+
+// Chebyshev coefficients of log Phi_w over the range of dw nm1s an image covers: degree 32 at
+// the first-kind nodes, kept when exp of the series matches the quadrature sum to 1e-13 of its
+// value at each of 97 points of the range (the correction divides by Phi_w, so the relative
+// error is what counts; one evaluation then costs 32 FMAs and an exp instead of nq cosines)
+constexpr int kWCheb = 33;
+bool w_cheb(const hipng_plan& P, const KernelFT& ft, double coef[kWCheb], double* centre, double* inv_half) {
+  if (!P.wstacking || !(P.dw > 0.0)) return false;
+  const double x = -(double)(P.nx / 2) * P.pixsize_x, y = -(double)(P.ny / 2) * P.pixsize_y;
+  const double r2 = x * x + y * y;
+  if (!(r2 < 1.0)) return false;
+  const double nm1c = -r2 / (std::sqrt(1.0 - r2) + 1.0);
+  double a = P.dw * (nm1c - P.nshift), b = P.dw * (-P.nshift);
+  if (a > b) std::swap(a, b);
+  const double pad = 1e-6 * (b - a);
+  a -= pad;
+  b += pad;
+  if (!(b > a)) return false;
+  auto f = [&](double xi) {
+    double sum = 0.0;
+    for (size_t q = 0; q < ft.z.size(); ++q) sum += ft.c[q] * std::cos(M_PI * P.W * ft.z[q] * xi);
+    return sum;
+  };
+  double fv[kWCheb];
+  for (int j = 0; j < kWCheb; ++j) {
+    const double v = f(0.5 * (a + b) + 0.5 * (b - a) * std::cos(M_PI * (j + 0.5) / kWCheb));
+    if (!(v > 0.0)) return false;
+    fv[j] = std::log(v);
+  }
+  for (int m = 0; m < kWCheb; ++m) {
+    double sum = 0.0;
+    for (int j = 0; j < kWCheb; ++j) sum += fv[j] * std::cos(M_PI * m * (j + 0.5) / kWCheb);
+    coef[m] = (m == 0 ? 1.0 : 2.0) * sum / kWCheb;
+  }
+  *centre = 0.5 * (a + b);
+  *inv_half = 2.0 / (b - a);
+  for (int i = 0; i < 97; ++i) {
+    const double xi = a + (b - a) * (i + 0.37) / 97.0, ex = f(xi);
+    if (!(ex > 0.0) || !(std::fabs(std::exp(cheb_eval(coef, kWCheb, (xi - *centre) * *inv_half)) - ex) <= 1e-13 * ex))
+      return false;
+  }
+  return true;
+}
 
 Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nchan, hipStream_t st) {
   KernelFT ft;
   ft.init(P.W, P.beta);
   const int nq = (int)ft.z.size();
-  const size_t n = 5 * (size_t)nchan + P.nx + P.ny + 2 * (size_t)nq;
+  const size_t n = 5 * (size_t)nchan + P.nx + P.ny + 2 * (size_t)nq + kWCheb + 2;
   std::vector<double> host(n, 0.0);
   double* h = host.data();
   channel_scales(P, freq, nchan, h, h + nchan, h + 2 * nchan, h + 3 * nchan);
@@ -4654,6 +4714,8 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
   }
   double* hf = hc + nq;
   for (int c = 0; c < nchan; ++c) hf[c] = freq[c] / kSpeedOfLight;
+  double* hw = hf + nchan;  // Chebyshev coefficients of Phi_w, then centre and 1 / half-width
+  const bool cheb = w_cheb(P, ft, hw, hw + kWCheb, hw + kWCheb + 1);
   auto* d = (double*)B.get(S_SMALL, n * sizeof(double));
   HIP_TRY(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));  // host buffer is released on return
@@ -4668,6 +4730,9 @@ Tables upload_tables(Buffers& B, const hipng_plan& P, const double* freq, int nc
   t.qc = t.qz + nq;
   t.fc = t.qc + nq;
   t.nq = nq;
+  t.cz = cheb ? t.fc + nchan + kWCheb : t.qz;
+  t.cc = cheb ? t.fc + nchan : t.qc;
+  t.cn = cheb ? -kWCheb : nq;
   const int nphi = P.nx + P.ny;
   k_phi_tables<<<(nphi + kBlock - 1) / kBlock, kBlock, 0, st>>>(t.phix, P.nx, P.gx, t.phiy, P.ny, P.gy, P.W, t.qz,
                                                                t.qc, nq);
@@ -5553,14 +5618,14 @@ void exec_ms2dirty(const Geom& g, const void* ms, const void* wgt, void* dirty, 
   if (cc && cc->image_yx) {
     const dim3 fg((P.nx + kBlock - 1) / kBlock, P.ny);
     if (cc->image_f32)
-      k_finalize_yx<A, float><<<fg, kBlock, 0, st>>>(k, acc, (float*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc,
-                                                      g.tb.nq, cc->scale);
+      k_finalize_yx<A, float><<<fg, kBlock, 0, st>>>(k, acc, (float*)dirty, g.tb.phix, g.tb.phiy, g.tb.cz, g.tb.cc,
+                                                      g.tb.cn, cc->scale);
     else
-      k_finalize_yx<A, double><<<fg, kBlock, 0, st>>>(k, acc, (double*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz,
-                                                       g.tb.qc, g.tb.nq, cc->scale);
+      k_finalize_yx<A, double><<<fg, kBlock, 0, st>>>(k, acc, (double*)dirty, g.tb.phix, g.tb.phiy, g.tb.cz,
+                                                       g.tb.cc, g.tb.cn, cc->scale);
   } else {
     k_finalize_t<A, O><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
-        k, acc, (O*)dirty, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
+        k, acc, (O*)dirty, g.tb.phix, g.tb.phiy, g.tb.cz, g.tb.cc, g.tb.cn);
   }
   LAUNCH_CHECK();
 }
@@ -5579,14 +5644,14 @@ void exec_dirty2ms(const Geom& g, const void* dirty, const void* wgt, void* ms, 
     if (cc && cc->image_yx) {
       const dim3 fg((P.nx + kBlock - 1) / kBlock, P.ny);
       if (cc->image_f32)
-        k_prescale_yx<float, A><<<fg, kBlock, 0, st>>>(k, (const float*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz,
-                                                        g.tb.qc, g.tb.nq);
+        k_prescale_yx<float, A><<<fg, kBlock, 0, st>>>(k, (const float*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.cz,
+                                                        g.tb.cc, g.tb.cn);
       else
         k_prescale_yx<double, A><<<fg, kBlock, 0, st>>>(k, (const double*)dirty, dc, g.tb.phix, g.tb.phiy,
-                                                         g.tb.qz, g.tb.qc, g.tb.nq);
+                                                         g.tb.cz, g.tb.cc, g.tb.cn);
     } else {
       k_prescale_t<I, A><<<dim3((P.nx + 31) / 32, (P.ny + 31) / 32), kBlock, 0, st>>>(
-          k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.qz, g.tb.qc, g.tb.nq);
+          k, (const I*)dirty, dc, g.tb.phix, g.tb.phiy, g.tb.cz, g.tb.cc, g.tb.cn);
     }
     LAUNCH_CHECK();
   }
