@@ -2991,9 +2991,22 @@ __global__ __launch_bounds__(16 * M) void k_fs_fwd1(KP k, int xlo, int xhi, cons
   __shared__ cplx<R> s[16][N1 + 1];
   const int n2 = blockIdx.x, iy0 = blockIdx.y * 16, i = blockIdx.z;
   const int t = threadIdx.x;
-  for (int e = t; e < 16 * N1; e += NT) {
-    const int c = e / N1, k1 = e - c * N1;
-    s[c][k1] = iy0 + c < k.ny ? V[(((int64_t)i * k.ny + iy0 + c) * kFsN2 + n2) * N1 + k1] : cplx<R>{0, 0};
+  {
+    // 8 loads in flight per thread (16 N1 = 8 NT elements), then the LDS writes; columns past
+    // the image read row iy0 and are dropped
+    cplx<R> v[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int e = t + NT * a, c = e / N1, k1 = e - c * N1;
+      const bool ok = iy0 + c < k.ny;
+      v[a] = V[(((int64_t)i * k.ny + iy0 + (ok ? c : 0)) * kFsN2 + n2) * N1 + k1];
+      if (!ok) v[a] = cplx<R>{0, 0};
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int e = t + NT * a, c = e / N1;
+      s[c][e - c * N1] = v[a];
+    }
   }
   __syncthreads();
   {
